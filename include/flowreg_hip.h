@@ -1,0 +1,174 @@
+/*
+ * flowreg_hip.h -- C ABI of libflowreg_hip.so, the MI355X (gfx950) backend
+ * for pyflowreg's variational optical-flow path.
+ *
+ * All array arguments are raw pointers.  Unless stated otherwise they point
+ * to device memory (HBM) and are C-contiguous; `stream` is a hipStream_t
+ * (NULL = default stream).  Every entry point returns FRH_OK (0) or a
+ * negative error code; frh_last_error() then returns a message that names
+ * the failing check or kernel.  No entry point falls back to the CPU.
+ *
+ * Layout conventions
+ *   image planes : (N, H, W) float32, one plane per (frame, channel)
+ *   flows        : (H, W) float32 planes, u = x (columns), v = y (rows)
+ *   level solver : (m, n, K) float64 interleaved, like the reference's
+ *                  numpy arrays (m = rows + 2, n = cols + 2, K channels)
+ *
+ * Reference citations are pyflowreg source paths (file:line).
+ */
+#ifndef FLOWREG_HIP_H
+#define FLOWREG_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FRH_OK 0
+#define FRH_EINVAL (-1)   /* bad argument (shape, size, pointer) */
+#define FRH_EHIP (-2)     /* a HIP runtime call or kernel launch failed */
+#define FRH_ENOMEM (-3)   /* device allocation failed */
+#define FRH_EUNSUP (-4)   /* valid request this build does not implement */
+
+const char *frh_last_error(void);
+int frh_abi_version(void);
+
+/* ------------------------------------------------------------ host only */
+
+/* util/resize_util.py:194-213 (_fill_tables_fused_gauss_cubic_reflect).
+ * Host memory.  g: the 2R+1 float32 Gaussian taps of
+ * util/resize_util.py:216-225 (R = 0: g = {1}).  idx, wt: (out_len, 2R+4). */
+int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g,
+                           int32_t *idx, float *wt);
+
+/* core/warping.py:237-277 (warpingDepth). */
+int frh_warping_depth(double eta, int levels, int m, int n);
+
+/* ------------------------------------------------------------ stages */
+
+/* util/image_processing.py:12-98 (normalize) followed by :101-165
+ * (apply_gaussian_filter, mode "reflect"), then the executors' float32 cast
+ * (motion_correction/parallelization/base.py:63-73).
+ * raw: (T, H, W, C) float32.  lo[c], den[c]: normalisation offset and
+ * denominator (max - min + eps) per channel, float64, host memory.
+ * wy (2*ry+1), wx (2*rx+1): Gaussian weights, float64, host memory, the
+ * values scipy.ndimage uses.  out: (T, C, H, W) float32 planes.
+ * Arithmetic is float64 in scipy's summation order (vertical pass, then
+ * horizontal). */
+int frh_prefilter(const float *raw, int T, int H, int W, int C, const double *lo,
+                  const double *den, const double *wy, int ry, const double *wx, int rx,
+                  float *out, void *stream);
+
+/* Same as frh_prefilter for float64 raw input (the reference frame is the
+ * float64 mean of raw frames, motion_correction/compensate_recording.py:
+ * 491-519). */
+int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const double *lo,
+                      const double *den, const double *wy, int ry, const double *wx, int rx,
+                      float *out, void *stream);
+
+/* util/resize_util.py:116-141,232-254 (imresize_fused_gauss_cubic).
+ * src: (N, h, w) planes; dst: (N, oh, ow); tmp: (N, h, ow) scratch.
+ * ix, wx: (ow, P) and iy, wy: (oh, P) device tables from
+ * frh_fill_resize_tables.  float32 products, float64 sums. */
+int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow,
+               const int32_t *ix, const float *wx, const int32_t *iy, const float *wy, int P,
+               float *tmp, void *stream);
+
+/* core/warping.py:160-234 (imregister_wrapper, cubic).  f2: (T, C, H, W)
+ * planes to warp; f1: (C, H, W) fill for out-of-bounds pixels, or
+ * (T, C, H, W) when f1_per_frame != 0.  u, v: (T, H, W) float32
+ * displacements, divided by h_col / h_row (float64) before use.
+ * out: (T, C, H, W).  OpenCV INTER_CUBIC + BORDER_REPLICATE semantics with
+ * 1/32-px map quantization. */
+int frh_warp_cubic(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                   int W, const float *u, const float *v, double h_col, double h_row,
+                   float *out, void *stream);
+
+/* core/optical_flow.py:73-160 (get_motion_tensor_gc), one channel.
+ * f1, f2: (h, w) float32.  J11..J23: (h+2, w+2) float64, zero border. */
+int frh_motion_tensor_gc(const float *f1, const float *f2, int h, int w, double hy, double hx,
+                         double *J11, double *J22, double *J33, double *J12, double *J13,
+                         double *J23, void *stream);
+
+/* core/optical_flow.py:348-411 (level_solver) -> core/level_solver.py:158-369
+ * (compute_flow), gc data term, no GNC.  Same positional meaning as the
+ * reference: J*, weight (m, n, K) float64; u, v (m, n) float64;
+ * a_data (K) float64 host memory; hx scales column differences, hy rows.
+ * du, dv: (m, n) float64 outputs (interior updated, ring = the last
+ * boundary copy).  Sweeps run in the reference's column-major
+ * lexicographic order (a diagonal wavefront on the GPU). */
+int frh_level_solver(const double *J11, const double *J22, const double *J33,
+                     const double *J12, const double *J13, const double *J23,
+                     const double *weight, const double *u, const double *v, int m, int n,
+                     int K, double alpha_x, double alpha_y, int iterations, int update_lag,
+                     const double *a_data, double a_smooth, double hx, double hy, double *du,
+                     double *dv, void *stream);
+
+/* scipy.ndimage.median_filter(size=(5, 5), mode="mirror") as called at
+ * core/optical_flow.py:607-609.  f: (N, h, w) float64 planes, in place. */
+int frh_median5_mirror(double *f, int N, int h, int w, void *stream);
+
+/* ----------------------------------------------------- whole pyramid */
+
+/* One pyramid level as planned by the host (core/optical_flow.py:514-577).
+ * Tables are host pointers; they are copied to the device at create. */
+typedef struct {
+    int h, w;                 /* level size (rows, cols) */
+    double h_row, h_col;      /* m / h, n / w */
+    double alpha_x, alpha_y;  /* alpha scaled by eta^(-i/2) (1 at min_level) */
+    int P_img;                /* taps of the full-res -> level resize */
+    const int32_t *img_ix;    /* (w, P_img) */
+    const float *img_wx;
+    const int32_t *img_iy;    /* (h, P_img) */
+    const float *img_wy;
+    int P_up;                 /* taps of the coarser -> this level resize (0 at top) */
+    const int32_t *up_ix;     /* (w, P_up) */
+    const float *up_wx;
+    const int32_t *up_iy;     /* (h, P_up) */
+    const float *up_wy;
+} frh_level_desc;
+
+typedef struct {
+    int H, W, C;              /* full-resolution frame geometry */
+    int n_levels;             /* levels[0] = coarsest (top) ... last = finest */
+    const frh_level_desc *levels;
+    int iterations, update_lag;
+    double a_smooth;
+    const double *a_data;     /* (C) host */
+    int max_batch;            /* largest T passed to frh_solver_run */
+} frh_solver_desc;
+
+typedef struct frh_solver frh_solver;
+
+/* Allocates every device workspace for up to max_batch frames. */
+int frh_solver_create(frh_solver **out, const frh_solver_desc *desc);
+void frh_solver_destroy(frh_solver *s);
+
+/* Per-recording inputs shared by every frame of a batch
+ * (core/optical_flow.py:519,531,565): fixed (C, H, W) float32 planes,
+ * weight (C, H, W) float32 planes (float32-cast weight map), uv (2, H, W)
+ * float32 initial flow or NULL (zero). */
+int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *weight,
+                             const float *uv, void *stream);
+
+/* core/optical_flow.py:773-906 (get_displacement) for T moving frames
+ * (T, C, H, W) float32 against the reference: w_out (T, H, W, 2) float64
+ * or float32 (out_f64 = 0), [..., 0] = u, [..., 1] = v. */
+int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int out_f64,
+                   void *stream);
+
+/* Kernel-time accounting of the last frh_solver_run, measured with HIP
+ * events on the run's stream: ms[k] is the summed duration and counts[k]
+ * the number of launch groups of category k: 0 = whole run, 1 = SOR
+ * sweeps, 2 = coefficient assembly, 3 = motion tensor, 4 = resize (pyramid
+ * + flow), 5 = warp, 6 = median/add.  Requires frh_solver_profile(s, 1)
+ * before the run (it synchronizes the stream); returns the count written. */
+int frh_solver_profile(frh_solver *s, int enable);
+int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,593 @@
+/*
+ * flowreg_oracle.c -- CPU restatement of pyflowreg's variational optical-flow
+ * path (get_displacement with the gradient-constancy data term).
+ *
+ * TEST INFRASTRUCTURE ONLY.  This file is the checker: it may be built and
+ * called only by tests/, __graft_entry__.smoke() and the cpu_baseline leg of
+ * bench.py.  The product (pyflowreg_amd) never links or calls it.
+ *
+ * Compiled twice by oracle/Makefile:
+ *   liboracle64.so  -- OR_REAL=double : the FP64 restatement (parity oracle)
+ *   liboracle32.so  -- OR_REAL=float  : the same arithmetic in FP32, used only
+ *                      for the FP32-vs-FP64 precision study.
+ *
+ * Float32 islands follow the reference exactly in both builds: level images,
+ * warped images, resized flows and resized weights are float32-valued; the
+ * resize accumulates float32 products in float64.
+ *
+ * Every function cites the reference file:line it restates (paths relative to
+ * the pyflowreg repository root).  Arrays are C-contiguous, interleaved
+ * (rows, cols, channels) like the reference's numpy arrays.
+ *
+ * Pinning (what checks each function, see tests/test_oracle.py):
+ *   or_warping_depth        closed form + level table (19 levels at 512^2)
+ *   or_fill_resize_tables   linear-ramp reproduction, partition of unity,
+ *                           identity at scale 1
+ *   or_resize_tab           scale-1 identity, ramp reproduction, constants
+ *   or_warp_cubic           reference warp tests (zero / integer shifts,
+ *                           out-of-bounds <- fixed), ramp reproduction with
+ *                           1/32-px quantization
+ *   or_motion_tensor_gc     reference known answer J11=0.8, identical frames
+ *                           give J33=J13=J23=0, quadratic closed forms
+ *   or_compute_flow         f1=f2 gives zero flow; a_data=1 converges to the
+ *                           dense direct solve; one sweep equals the matrix
+ *                           form of SOR in column-major order
+ *   (the level loop, the median and the Gaussian prefilter live in
+ *   oracle/oracle.py; the median and Gaussian call scipy, the reference's
+ *   own dependency, directly)
+ *   or_cubic_resize_cv      partially pinned (ramp reproduction, identity);
+ *                           OpenCV resize tables are parity unpinned
+ */
+#include <tgmath.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "../include/flowreg_detmath.h"
+
+#ifndef OR_REAL
+#define OR_REAL double
+#endif
+typedef OR_REAL real;
+
+#define KEYS_A (-0.75)
+
+/* Python's round(): half to even (default FP rounding mode). */
+static double py_round(double x) { return nearbyint(x); }
+
+/* core/warping.py:237-277 (warpingDepth) */
+int or_warping_depth(double eta, int levels, int m, int n)
+{
+    double min_dim = (double)(m < n ? m : n);
+    int depth = 0;
+    for (int k = 0; k < levels; ++k) {
+        depth += 1;
+        min_dim *= eta;
+        if (py_round(min_dim) < 10.0) break;
+    }
+    return depth;
+}
+
+/* util/resize_util.py:68-76 (_cubic, A=-0.75) */
+static double keys_cubic(double x)
+{
+    double ax = fabs(x);
+    if (ax < 1.0) return (KEYS_A + 2.0) * ax * ax * ax - (KEYS_A + 3.0) * ax * ax + 1.0;
+    if (ax < 2.0) return KEYS_A * ax * ax * ax - 5.0 * KEYS_A * ax * ax + 8.0 * KEYS_A * ax - 4.0 * KEYS_A;
+    return 0.0;
+}
+
+/* util/resize_util.py:182-191 (_reflect_idx: ...b a | a b...) */
+static int reflect_idx(int j, int n)
+{
+    if (n <= 1) return 0;
+    while (j < 0 || j >= n) {
+        if (j < 0) j = -j - 1;
+        else j = 2 * n - 1 - j;
+    }
+    return j;
+}
+
+/* util/resize_util.py:216-229 (_precompute_fused_gauss_cubic): tap radius. */
+int or_resize_radius(double sigma)
+{
+    if (sigma <= 0.0) return 0;
+    return (int)ceil(2.0 * sigma);
+}
+
+/* util/resize_util.py:194-213 (_fill_tables_fused_gauss_cubic_reflect):
+ * fused Gaussian+cubic tap tables, idx/wt are (out_len, 2R+4).  g is the
+ * float32 Gaussian of util/resize_util.py:216-225, computed by the caller
+ * with numpy exactly as the reference does. */
+void or_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
+{
+    int P = 2 * R + 4;
+    double scale = (double)out_len / (double)in_len;
+    for (int i = 0; i < out_len; ++i) {
+        double x = (i + 0.5) / scale - 0.5;
+        int left = (int)floor(x - 2.0) - R;
+        double ssum = 0.0;
+        for (int p = 0; p < P; ++p) {
+            int j = left + p;
+            idx[(size_t)i * P + p] = reflect_idx(j, in_len);
+            double d = x - j, acc = 0.0;
+            for (int u = -R; u <= R; ++u) acc += (double)g[u + R] * keys_cubic(d - u);
+            wt[(size_t)i * P + p] = (float)acc;
+            ssum += acc;
+        }
+        double inv = 1.0 / ssum;
+        for (int p = 0; p < P; ++p) wt[(size_t)i * P + p] = (float)((double)wt[(size_t)i * P + p] * inv);
+    }
+}
+
+/* util/resize_util.py:116-141 (_resize_h/_resize_v) + :232-254
+ * (imresize_fused_gauss_cubic), one channel of an interleaved (h,w,C)
+ * float32 image.  Products are float32, sums float64, stores float32. */
+static void resize_channel(const float *src, int h, int w, int C, int c,
+                           float *dst, int oh, int ow,
+                           const int32_t *ix, const float *wx, int Px,
+                           const int32_t *iy, const float *wy, int Py, float *tmp)
+{
+    for (int y = 0; y < h; ++y)
+        for (int x = 0; x < ow; ++x) {
+            double s = 0.0;
+            for (int p = 0; p < Px; ++p) {
+                float prod = src[((size_t)y * w + ix[(size_t)x * Px + p]) * C + c] * wx[(size_t)x * Px + p];
+                s += (double)prod;
+            }
+            tmp[(size_t)y * ow + x] = (float)s;
+        }
+    for (int x = 0; x < ow; ++x)
+        for (int y = 0; y < oh; ++y) {
+            double s = 0.0;
+            for (int p = 0; p < Py; ++p) {
+                float prod = tmp[(size_t)iy[(size_t)y * Py + p] * ow + x] * wy[(size_t)y * Py + p];
+                s += (double)prod;
+            }
+            dst[((size_t)y * ow + x) * C + c] = (float)s;
+        }
+}
+
+/* util/resize_util.py:243-254: separable resize of an interleaved (h,w,C)
+ * float32 image with precomputed tables (P taps per output sample). */
+void or_resize_tab(const float *src, int h, int w, int C, float *dst, int oh, int ow,
+                   const int32_t *ix, const float *wx, const int32_t *iy, const float *wy, int P)
+{
+    float *tmp = (float *)malloc(sizeof(float) * (size_t)h * ow);
+    for (int c = 0; c < C; ++c) resize_channel(src, h, w, C, c, dst, oh, ow, ix, wx, P, iy, wy, P, tmp);
+    free(tmp);
+}
+
+/* OpenCV's float-table bicubic coefficients (Keys, A=-0.75) at t = k/32,
+ * evaluated in float as OpenCV's interpolateCubic does. */
+static void cv_cubic_coeffs(float x, float *c)
+{
+    const float A = -0.75f;
+    c[0] = ((A * (x + 1) - 5 * A) * (x + 1) + 8 * A) * (x + 1) - 4 * A;
+    c[1] = ((A + 2) * x - (A + 3)) * x * x + 1;
+    c[2] = ((A + 2) * (1 - x) - (A + 3)) * (1 - x) * (1 - x) + 1;
+    c[3] = 1.f - c[0] - c[1] - c[2];
+}
+
+/* core/warping.py:160-234 (imregister_wrapper, cubic):
+ *   map = float32(grid + u); OOB on the unclipped map; clip; cv2.remap with
+ *   INTER_CUBIC + BORDER_REPLICATE into float32; OOB pixels <- f1.
+ * cv2.remap (opencv-contrib-python>=4.10, third party, restated from its
+ * public behaviour; parity with OpenCV itself is unpinned): float maps are
+ * rounded to 1/32 px (cvRound, half-to-even), the 4x4 weights are products
+ * of two float 1-D tables, sums are float.  Interior pixels sum row by row,
+ * border pixels (BORDER_REPLICATE) sum the 16 taps sequentially.
+ * u, v are the per-level displacements already divided by the grid spacing
+ * (u / h_col, v / h_row in the reference), as float64. */
+static int g_warp_quantize = 1;
+/* Study switch: 0 evaluates the bicubic weights at the exact fractional
+ * position instead of OpenCV's 1/32-px grid (never used for parity). */
+void or_set_warp_quantize(int q) { g_warp_quantize = q; }
+
+void or_warp_cubic(const float *f2, const float *f1, int H, int W, int C,
+                   const double *u, const double *v, float *out)
+{
+    float tab[32][4];
+    for (int k = 0; k < 32; ++k) cv_cubic_coeffs(k * (1.f / 32), tab[k]);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t p = (size_t)y * W + x;
+            float mx = (float)((double)x + u[p]);
+            float my = (float)((double)y + v[p]);
+            int oob = (mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H);
+            if (oob) {
+                for (int c = 0; c < C; ++c) out[p * C + c] = f1[p * C + c];
+                continue;
+            }
+            float cx = mx < 0.f ? 0.f : (mx > (float)(W - 1) ? (float)(W - 1) : mx);
+            float cy = my < 0.f ? 0.f : (my > (float)(H - 1) ? (float)(H - 1) : my);
+            int X = (int)lrintf(cx * 32.f), Y = (int)lrintf(cy * 32.f);
+            int sx = (X >> 5) - 1, sy = (Y >> 5) - 1;
+            const float *wxv = tab[X & 31], *wyv = tab[Y & 31];
+            float cxq[4], cyq[4];
+            if (!g_warp_quantize) {
+                sx = (int)floorf(cx) - 1; sy = (int)floorf(cy) - 1;
+                cv_cubic_coeffs(cx - floorf(cx), cxq); cv_cubic_coeffs(cy - floorf(cy), cyq);
+                wxv = cxq; wyv = cyq;
+            }
+            float w[16];
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) w[i * 4 + j] = wyv[i] * wxv[j];
+            int width1 = W - 3 > 0 ? W - 3 : 0, height1 = H - 3 > 0 ? H - 3 : 0;
+            for (int c = 0; c < C; ++c) {
+                float sum;
+                if ((unsigned)sx < (unsigned)width1 && (unsigned)sy < (unsigned)height1) {
+                    const float *S = f2 + ((size_t)sy * W + sx) * C + c;
+                    size_t rs = (size_t)W * C;
+                    sum = S[0] * w[0] + S[C] * w[1] + S[2 * C] * w[2] + S[3 * C] * w[3];
+                    S += rs;
+                    sum += S[0] * w[4] + S[C] * w[5] + S[2 * C] * w[6] + S[3 * C] * w[7];
+                    S += rs;
+                    sum += S[0] * w[8] + S[C] * w[9] + S[2 * C] * w[10] + S[3 * C] * w[11];
+                    S += rs;
+                    sum += S[0] * w[12] + S[C] * w[13] + S[2 * C] * w[14] + S[3 * C] * w[15];
+                } else {
+                    int xs[4], ys[4];
+                    for (int i = 0; i < 4; ++i) {
+                        int a = sx + i, b = sy + i;
+                        xs[i] = a < 0 ? 0 : (a >= W ? W - 1 : a);
+                        ys[i] = b < 0 ? 0 : (b >= H ? H - 1 : b);
+                    }
+                    sum = 0.f;
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j)
+                            sum += f2[((size_t)ys[i] * W + xs[j]) * C + c] * w[i * 4 + j];
+                }
+                out[p * C + c] = sum;
+            }
+        }
+}
+
+/* core/optical_flow.py:73-160 (get_motion_tensor_gc) for one channel.
+ * f1, f2: (h, w) float32 (level image and warped image of one channel,
+ * strided by C).  Outputs J* (h+2, w+2) with a zero border. */
+void or_motion_tensor_gc(const float *f1, const float *f2, int h, int w, int C, int c,
+                         double hy, double hx,
+                         real *J11, real *J22, real *J33, real *J12, real *J13, real *J23)
+{
+    int hp = h + 2, wp = w + 2;
+    size_t np_ = (size_t)hp * wp;
+    real *F1 = (real *)malloc(sizeof(real) * np_), *F2 = (real *)malloc(sizeof(real) * np_);
+    real *fx = (real *)malloc(sizeof(real) * np_), *ft = (real *)malloc(sizeof(real) * np_);
+    /* np.pad(..., mode="symmetric"): row/col -1 duplicates row/col 0 */
+    for (int j = 0; j < hp; ++j)
+        for (int i = 0; i < wp; ++i) {
+            int sj = j - 1 < 0 ? 0 : (j - 1 >= h ? h - 1 : j - 1);
+            int si = i - 1 < 0 ? 0 : (i - 1 >= w ? w - 1 : i - 1);
+            F1[(size_t)j * wp + i] = (real)f1[((size_t)sj * w + si) * C + c];
+            F2[(size_t)j * wp + i] = (real)f2[((size_t)sj * w + si) * C + c];
+        }
+    real rhx = (real)hx, rhy = (real)hy;
+    /* fx = 0.5 (d/dx F1 + d/dx F2), np.gradient along axis 1, interior only
+     * (the border is replaced by the symmetric re-pad below) */
+    for (int j = 1; j <= h; ++j)
+        for (int i = 1; i <= w; ++i) {
+            size_t q = (size_t)j * wp + i;
+            real g1 = (F1[q + 1] - F1[q - 1]) / (2 * rhx);
+            real g2 = (F2[q + 1] - F2[q - 1]) / (2 * rhx);
+            fx[q] = (real)0.5 * (g1 + g2);
+            ft[q] = F2[q] - F1[q];
+        }
+    /* re-pad interiors symmetrically (the padded ring copies its neighbour) */
+    for (int j = 0; j < hp; ++j)
+        for (int i = 0; i < wp; ++i) {
+            if (j >= 1 && j <= h && i >= 1 && i <= w) continue;
+            int sj = j < 1 ? 1 : (j > h ? h : j);
+            int si = i < 1 ? 1 : (i > w ? w : i);
+            fx[(size_t)j * wp + i] = fx[(size_t)sj * wp + si];
+            ft[(size_t)j * wp + i] = ft[(size_t)sj * wp + si];
+        }
+    for (int j = 0; j < hp; ++j)
+        for (int i = 0; i < wp; ++i) {
+            size_t q = (size_t)j * wp + i;
+            if (j == 0 || j == hp - 1 || i == 0 || i == wp - 1) {
+                J11[q] = J22[q] = J33[q] = J12[q] = J13[q] = J23[q] = 0;
+                continue;
+            }
+            real fxy = (fx[q + wp] - fx[q - wp]) / (2 * rhy);
+            real fxt = (ft[q + 1] - ft[q - 1]) / (2 * rhx);
+            real fyt = (ft[q + wp] - ft[q - wp]) / (2 * rhy);
+            real fxx1 = (F1[q - 1] - 2 * F1[q] + F1[q + 1]) / (rhx * rhx);
+            real fyy1 = (F1[q - wp] - 2 * F1[q] + F1[q + wp]) / (rhy * rhy);
+            real fxx2 = (F2[q - 1] - 2 * F2[q] + F2[q + 1]) / (rhx * rhx);
+            real fyy2 = (F2[q - wp] - 2 * F2[q] + F2[q + wp]) / (rhy * rhy);
+            real fxx = (real)0.5 * (fxx1 + fxx2), fyy = (real)0.5 * (fyy1 + fyy2);
+            real sx = sqrt(fxx * fxx + fxy * fxy), sy = sqrt(fxy * fxy + fyy * fyy);
+            real regx = 1 / (sx * sx + (real)1e-6), regy = 1 / (sy * sy + (real)1e-6);
+            /* numpy evaluates reg_x * fxx**2 as reg_x * (fxx * fxx) */
+            J11[q] = regx * (fxx * fxx) + regy * (fxy * fxy);
+            J22[q] = regx * (fxy * fxy) + regy * (fyy * fyy);
+            J33[q] = regx * (fxt * fxt) + regy * (fyt * fyt);
+            J12[q] = regx * fxx * fxy + regy * fxy * fyy;
+            J13[q] = regx * fxx * fxt + regy * fxy * fyt;
+            J23[q] = regx * fxy * fxt + regy * fyy * fyt;
+        }
+    free(F1); free(F2); free(fx); free(ft);
+}
+
+/* core/level_solver.py:45-66 (set_boundary_2d): rows, then columns. */
+static void set_boundary_2d(real *f, int m, int n)
+{
+    for (int i = 0; i < n; ++i) {
+        f[i] = f[(size_t)n + i];
+        f[(size_t)(m - 1) * n + i] = f[(size_t)(m - 2) * n + i];
+    }
+    for (int j = 0; j < m; ++j) {
+        f[(size_t)j * n] = f[(size_t)j * n + 1];
+        f[(size_t)j * n + n - 1] = f[(size_t)j * n + n - 2];
+    }
+}
+
+/* core/level_solver.py:69-155 (nonlinearity_smoothness_2d), eps = 1e-5,
+ * one-sided differences on the outer ring. */
+static void psi_smooth_2d(real *psi_s, const real *u, const real *du, const real *v, const real *dv,
+                          int m, int n, real a, real hx, real hy)
+{
+    size_t N = (size_t)m * n;
+    real *uf = (real *)malloc(sizeof(real) * N), *vf = (real *)malloc(sizeof(real) * N);
+    for (size_t q = 0; q < N; ++q) { uf[q] = u[q] + du[q]; vf[q] = v[q] + dv[q]; }
+    for (int i = 0; i < n; ++i)
+        for (int j = 0; j < m; ++j) {
+            size_t q = (size_t)j * n + i;
+            real ux = 0, vx = 0, uy = 0, vy = 0;
+            if (n > 1) {
+                if (i == 0) { ux = (uf[q + 1] - uf[q]) / hx; vx = (vf[q + 1] - vf[q]) / hx; }
+                else if (i == n - 1) { ux = (uf[q] - uf[q - 1]) / hx; vx = (vf[q] - vf[q - 1]) / hx; }
+                else { ux = (uf[q + 1] - uf[q - 1]) / (2 * hx); vx = (vf[q + 1] - vf[q - 1]) / (2 * hx); }
+            }
+            if (m > 1) {
+                if (j == 0) { uy = (uf[q + n] - uf[q]) / hy; vy = (vf[q + n] - vf[q]) / hy; }
+                else if (j == m - 1) { uy = (uf[q] - uf[q - n]) / hy; vy = (vf[q] - vf[q - n]) / hy; }
+                else { uy = (uf[q + n] - uf[q - n]) / (2 * hy); vy = (vf[q + n] - vf[q - n]) / (2 * hy); }
+            }
+            real t = ux * ux + uy * uy + vx * vx + vy * vy;
+            if (t < 0) t = 0;
+            psi_s[q] = a * pow(t + (real)1e-5, a - 1);
+        }
+    free(uf); free(vf);
+}
+
+/* One SOR update of pixel (j, i); core/level_solver.py:262-364. */
+static inline void sor_pixel(int j, int i, int m, int n, int K,
+                             const real *J11, const real *J22, const real *J12,
+                             const real *J13, const real *J23, const real *W,
+                             const real *u, const real *v, real *du, real *dv,
+                             const real *psi, const real *psi_s, int smooth_const,
+                             real ax, real ay, real omega)
+{
+    size_t c = (size_t)j * n + i;
+    size_t nb[4] = {c - 1, c + 1, c + n, c - n}; /* left, right, down, up */
+    real wdir[4] = {ax, ax, ay, ay};
+    real den_u = 0, den_v = 0, num_u = 0, num_v = 0;
+    for (int d = 0; d < 4; ++d) {
+        real t = smooth_const ? wdir[d] : (real)0.5 * (psi_s[c] + psi_s[nb[d]]) * wdir[d];
+        num_u += t * (u[nb[d]] + du[nb[d]] - u[c]);
+        num_v += t * (v[nb[d]] + dv[nb[d]] - v[c]);
+        den_u += t;
+        den_v += t;
+    }
+    for (int k = 0; k < K; ++k) {
+        size_t ck = c * K + k;
+        num_u -= W[ck] * psi[ck] * (J13[ck] + J12[ck] * dv[c]);
+        den_u += W[ck] * psi[ck] * J11[ck];
+        den_v += W[ck] * psi[ck] * J22[ck];
+    }
+    real du_k = den_u != 0 ? num_u / den_u : 0;
+    du[c] = (1 - omega) * du[c] + omega * du_k;
+    real num_v2 = num_v;
+    for (int k = 0; k < K; ++k) {
+        size_t ck = c * K + k;
+        num_v2 -= W[ck] * psi[ck] * (J23[ck] + J12[ck] * du[c]);
+    }
+    real dv_k = den_v != 0 ? num_v2 / den_v : 0;
+    dv[c] = (1 - omega) * dv[c] + omega * dv_k;
+}
+
+/* Per-cell linear-system coefficients for one lag block of compute_flow.
+ * Restates core/level_solver.py:229-255 (psi refresh) and the constant parts
+ * of :262-364 (smoothness weights, data terms) for one pixel c.  The
+ * reference recomputes these sums inside every sweep; they only change at a
+ * psi refresh, so they are formed once per lag block.  Summation order is
+ * fixed here and the GPU kernels use the same order:
+ *   weights  wl, wr, wd, wu (left, right, down, up)
+ *   Du  = (((wl + wr) + wd) + wu) + sum_k (W_k psi_k) J11_k
+ *   Dv  = (((wl + wr) + wd) + wu) + sum_k (W_k psi_k) J22_k
+ *   A12 = sum_k (W_k psi_k) J12_k
+ *   Bu  = cu - sum_k (W_k psi_k) J13_k,   cu = sum_nb w_nb (u_nb - u_c)
+ *   Bv  = cv - sum_k (W_k psi_k) J23_k
+ * psi_k = a_k (E_k + 1e-5)^(a_k - 1) with E_k the data term at (du, dv),
+ * evaluated with frh_det_pow. */
+static inline real cf_pow(real x, real b)
+{
+    if (sizeof(real) == sizeof(float)) return (real)frh_det_powf((float)x, (float)b);
+    return (real)frh_det_pow((double)x, (double)b);
+}
+
+static void assemble_pixel(size_t c, int n, int K, const real *J11, const real *J22, const real *J33,
+                           const real *J12, const real *J13, const real *J23, const real *W,
+                           const real *u, const real *v, const real *du, const real *dv,
+                           const real *psi_s, int smooth_const, real ax, real ay,
+                           const double *a_data, int psi_one, real *coef)
+{
+    size_t nb[4] = {c - 1, c + 1, c + n, c - n};
+    real w[4];
+    if (smooth_const) {
+        w[0] = ax; w[1] = ax; w[2] = ay; w[3] = ay;
+    } else {
+        real wd[4] = {ax, ax, ay, ay};
+        for (int d = 0; d < 4; ++d) w[d] = (real)0.5 * (psi_s[c] + psi_s[nb[d]]) * wd[d];
+    }
+    real cu = w[0] * (u[nb[0]] - u[c]);
+    real cv = w[0] * (v[nb[0]] - v[c]);
+    for (int d = 1; d < 4; ++d) {
+        cu = cu + w[d] * (u[nb[d]] - u[c]);
+        cv = cv + w[d] * (v[nb[d]] - v[c]);
+    }
+    real ws = ((w[0] + w[1]) + w[2]) + w[3];
+    real Du = ws, Dv = ws, A12 = 0, Bu = cu, Bv = cv;
+    for (int k = 0; k < K; ++k) {
+        size_t ck = c * K + k;
+        real psi = 1;
+        if (!psi_one) {
+            real a = (real)a_data[k];
+            real x = du[c], y = dv[c];
+            real val = J11[ck] * x * x + J22[ck] * y * y + J23[ck] * y + 2 * J12[ck] * x * y
+                     + 2 * J13[ck] * x + J23[ck] * y + J33[ck];
+            if (val < 0) val = 0;
+            psi = a * cf_pow(val + (real)1e-5, a - 1);
+        }
+        real wp = W[ck] * psi;
+        Du = Du + wp * J11[ck];
+        Dv = Dv + wp * J22[ck];
+        A12 = A12 + wp * J12[ck];
+        Bu = Bu - wp * J13[ck];
+        Bv = Bv - wp * J23[ck];
+    }
+    coef[0] = Du; coef[1] = Dv; coef[2] = A12; coef[3] = Bu; coef[4] = Bv;
+    coef[5] = w[0]; coef[6] = w[1]; coef[7] = w[2]; coef[8] = w[3];
+}
+
+/* One SOR update with the lag-block coefficients; core/level_solver.py:
+ * 342-364.  Neighbours outside the interior take the pixel's own value at
+ * the start of the sweep: set_boundary_2d (:257-258) copies the adjacent
+ * interior value into the ring before every sweep, and u is edge-padded, so
+ * the reference's u[nb] + du[nb] - u[c] reduces to du_c there. */
+static inline void sor_pixel_coef(int j, int i, int m, int n, real *du, real *dv,
+                                  const real *coef, real omega)
+{
+    size_t c = (size_t)j * n + i;
+    real dul = i > 1 ? du[c - 1] : du[c], dur = i < n - 2 ? du[c + 1] : du[c];
+    real dud = j < m - 2 ? du[c + n] : du[c], duu = j > 1 ? du[c - n] : du[c];
+    real dvl = i > 1 ? dv[c - 1] : dv[c], dvr = i < n - 2 ? dv[c + 1] : dv[c];
+    real dvd = j < m - 2 ? dv[c + n] : dv[c], dvu = j > 1 ? dv[c - n] : dv[c];
+    real Du = coef[0], Dv = coef[1], A12 = coef[2], Bu = coef[3], Bv = coef[4];
+    real wl = coef[5], wr = coef[6], wd = coef[7], wu = coef[8];
+    real nu = ((wl * dul + wr * dur) + wd * dud) + wu * duu;
+    real num_u = (Bu + nu) - A12 * dv[c];
+    real du_k = Du != 0 ? num_u / Du : 0;
+    real dun = (1 - omega) * du[c] + omega * du_k;
+    du[c] = dun;
+    real nv = ((wl * dvl + wr * dvr) + wd * dvd) + wu * dvu;
+    real num_v = (Bv + nv) - A12 * dun;
+    real dv_k = Dv != 0 ? num_v / Dv : 0;
+    dv[c] = (1 - omega) * dv[c] + omega * dv_k;
+}
+
+/* core/level_solver.py:158-369 (compute_flow).
+ * J*, W: (m, n, K); u, v: (m, n); du, dv: (m, n) outputs.
+ * order: 0 = column-major lexicographic sweep with the lag-block coefficient
+ *            form above (the parity oracle; the GPU matches it bit for bit);
+ *        1 = the same with red-black ordering (study only);
+ *        2 = the reference's per-sweep expression order with libm pow
+ *            (study only: shows the rounding sensitivity of the method). */
+void or_compute_flow(const real *J11, const real *J22, const real *J33, const real *J12,
+                     const real *J13, const real *J23, const real *W,
+                     const real *u, const real *v, int m, int n, int K,
+                     double alpha_x, double alpha_y, int iterations, int update_lag,
+                     const double *a_data, double a_smooth, double hx, double hy,
+                     real *du, real *dv, int order)
+{
+    size_t N = (size_t)m * n;
+    real *psi = (real *)malloc(sizeof(real) * N * K);
+    real *psi_s = (real *)malloc(sizeof(real) * N);
+    real *coef = (real *)malloc(sizeof(real) * N * 9);
+    for (size_t q = 0; q < N * K; ++q) psi[q] = 1;
+    for (size_t q = 0; q < N; ++q) { psi_s[q] = 1; du[q] = 0; dv[q] = 0; }
+    const real omega = (real)1.95;
+    real ax = (real)(alpha_x / (hx * hx)), ay = (real)(alpha_y / (hy * hy));
+    int smooth_const = (a_smooth == 1.0);
+    for (int it = 0; it < iterations; ++it) {
+        int refresh = ((it + 1) % update_lag == 0);
+        if (refresh) {
+            if (order == 2)
+                for (int k = 0; k < K; ++k)
+                    for (size_t q = 0; q < N; ++q) {
+                        size_t qk = q * K + k;
+                        real val = J11[qk] * du[q] * du[q] + J22[qk] * dv[q] * dv[q] + J23[qk] * dv[q]
+                                 + 2 * J12[qk] * du[q] * dv[q] + 2 * J13[qk] * du[q] + J23[qk] * dv[q]
+                                 + J33[qk];
+                        if (val < 0) val = 0;
+                        psi[qk] = (real)a_data[k] * pow(val + (real)1e-5, (real)a_data[k] - 1);
+                    }
+            if (!smooth_const)
+                psi_smooth_2d(psi_s, u, du, v, dv, m, n, (real)a_smooth, (real)hx, (real)hy);
+            else
+                for (size_t q = 0; q < N; ++q) psi_s[q] = 1;
+        }
+        if (order != 2 && (refresh || it == 0))
+            for (int i = 1; i < n - 1; ++i)
+                for (int j = 1; j < m - 1; ++j) {
+                    size_t c = (size_t)j * n + i;
+                    assemble_pixel(c, n, K, J11, J22, J33, J12, J13, J23, W, u, v, du, dv, psi_s,
+                                   smooth_const, ax, ay, a_data, !refresh, coef + c * 9);
+                }
+        set_boundary_2d(du, m, n);
+        set_boundary_2d(dv, m, n);
+        if (order == 2) {
+            for (int i = 1; i < n - 1; ++i)
+                for (int j = 1; j < m - 1; ++j)
+                    sor_pixel(j, i, m, n, K, J11, J22, J12, J13, J23, W, u, v, du, dv,
+                              psi, psi_s, smooth_const, ax, ay, omega);
+        } else if (order == 0) {
+            for (int i = 1; i < n - 1; ++i)
+                for (int j = 1; j < m - 1; ++j)
+                    sor_pixel_coef(j, i, m, n, du, dv, coef + ((size_t)j * n + i) * 9, omega);
+        } else {
+            for (int colour = 0; colour < 2; ++colour)
+                for (int i = 1; i < n - 1; ++i)
+                    for (int j = 1; j < m - 1; ++j)
+                        if (((i + j) & 1) == colour)
+                            sor_pixel_coef(j, i, m, n, du, dv, coef + ((size_t)j * n + i) * 9, omega);
+        }
+    }
+    free(psi); free(psi_s); free(coef);
+}
+
+/* cv2.resize(w, (n, m), interpolation=INTER_CUBIC) for float images
+ * (core/optical_flow.py:615-616, only when min_level > 0).  OpenCV's public
+ * behaviour: source coordinate (i+0.5)/scale-0.5, Keys cubic A=-0.75 with
+ * float coefficients, replicated borders, horizontal pass then vertical.
+ * Parity with OpenCV itself is unpinned. */
+void or_cubic_resize_cv(const real *src, int h, int w, int C, real *dst, int oh, int ow)
+{
+    double scale_x = 1.0 / ((double)ow / w), scale_y = 1.0 / ((double)oh / h);
+    real *tmp = (real *)malloc(sizeof(real) * (size_t)h * ow * C);
+    for (int x = 0; x < ow; ++x) {
+        float f = (float)((x + 0.5) * scale_x - 0.5);
+        int ix = (int)floorf(f);
+        float t = f - (float)ix, cf[4];
+        cv_cubic_coeffs(t, cf);
+        for (int y = 0; y < h; ++y)
+            for (int c = 0; c < C; ++c) {
+                real s = 0;
+                for (int k = 0; k < 4; ++k) {
+                    int xx = ix - 1 + k;
+                    xx = xx < 0 ? 0 : (xx >= w ? w - 1 : xx);
+                    s += src[((size_t)y * w + xx) * C + c] * (real)cf[k];
+                }
+                tmp[((size_t)y * ow + x) * C + c] = s;
+            }
+    }
+    for (int y = 0; y < oh; ++y) {
+        float f = (float)((y + 0.5) * scale_y - 0.5);
+        int iy = (int)floorf(f);
+        float t = f - (float)iy, cf[4];
+        cv_cubic_coeffs(t, cf);
+        for (int x = 0; x < ow; ++x)
+            for (int c = 0; c < C; ++c) {
+                real s = 0;
+                for (int k = 0; k < 4; ++k) {
+                    int yy = iy - 1 + k;
+                    yy = yy < 0 ? 0 : (yy >= h ? h - 1 : yy);
+                    s += tmp[((size_t)yy * ow + x) * C + c] * (real)cf[k];
+                }
+                dst[((size_t)y * ow + x) * C + c] = s;
+            }
+    }
+    free(tmp);
+}

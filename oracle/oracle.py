@@ -1,0 +1,352 @@
+"""CPU oracle for pyflowreg's variational optical-flow path.
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, ``__graft_entry__.smoke()`` and
+the ``cpu_baseline`` leg of ``bench.py`` as the checker.  The product package
+(``pyflowreg_amd``) never imports it.
+
+The level loop restates ``core/optical_flow.py:476-617``
+(``_solve_displacement_stage``) and the argument handling of
+``core/optical_flow.py:773-906`` (``get_displacement``).  Inner loops run in
+``flowreg_oracle.c`` (built by ``oracle/Makefile``); the 5x5 median and the
+Gaussian prefilter call scipy, which is the reference's own dependency for
+those two stages (``core/optical_flow.py:41,607-609``,
+``util/image_processing.py:7,142-160``).
+
+``precision="f64"`` is the parity oracle (float64 solver and tensor, float32
+islands where the reference has them).  ``precision="f32"`` runs the same
+code with float32 tensor/solver arithmetic; it exists only for the
+precision study.  ``order=1`` switches the SOR sweep to red-black ordering
+(ordering study only).
+
+Parity status per function (the checks live in tests/test_oracle.py):
+all functions are pinned except the fractional-position arithmetic of the
+OpenCV bicubic remap and of ``cv2.resize`` (third-party, restated from its
+public behaviour, cv2 absent here): "parity unpinned" for those two.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+from scipy.ndimage import gaussian_filter, median_filter
+
+_DIR = os.path.dirname(os.path.abspath(__file__))
+_LIBS: dict[str, ctypes.CDLL] = {}
+
+_f32p = np.ctypeslib.ndpointer(np.float32, flags="C_CONTIGUOUS")
+_f64p = np.ctypeslib.ndpointer(np.float64, flags="C_CONTIGUOUS")
+_i32p = np.ctypeslib.ndpointer(np.int32, flags="C_CONTIGUOUS")
+
+
+def build() -> None:
+    """Compile both oracle builds with gcc (oracle/Makefile)."""
+    subprocess.run(["make", "-s", "-C", _DIR], check=True)
+
+
+def lib(precision: str = "f64") -> ctypes.CDLL:
+    if precision in _LIBS:
+        return _LIBS[precision]
+    path = os.path.join(_DIR, "liboracle64.so" if precision == "f64" else "liboracle32.so")
+    if not os.path.exists(path):
+        build()
+    L = ctypes.CDLL(path)
+    rp = _f64p if precision == "f64" else _f32p
+    L.or_warping_depth.argtypes = [ctypes.c_double, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    L.or_warping_depth.restype = ctypes.c_int
+    L.or_fill_resize_tables.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p, _i32p, _f32p]
+    L.or_fill_resize_tables.restype = None
+    L.or_resize_tab.argtypes = [_f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int, _f32p,
+                                ctypes.c_int, ctypes.c_int, _i32p, _f32p, _i32p, _f32p, ctypes.c_int]
+    L.or_resize_tab.restype = None
+    L.or_warp_cubic.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                _f64p, _f64p, _f32p]
+    L.or_warp_cubic.restype = None
+    L.or_motion_tensor_gc.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int, ctypes.c_double, ctypes.c_double] + [rp] * 6
+    L.or_motion_tensor_gc.restype = None
+    L.or_compute_flow.argtypes = [rp] * 9 + [ctypes.c_int] * 3 + [
+        ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p,
+        ctypes.c_double, ctypes.c_double, ctypes.c_double, rp, rp, ctypes.c_int]
+    L.or_compute_flow.restype = None
+    L.or_cubic_resize_cv.argtypes = [rp, ctypes.c_int, ctypes.c_int, ctypes.c_int, rp,
+                                     ctypes.c_int, ctypes.c_int]
+    L.or_cubic_resize_cv.restype = None
+    _LIBS[precision] = L
+    return L
+
+
+def _dtype(precision: str):
+    return np.float64 if precision == "f64" else np.float32
+
+
+# ---------------------------------------------------------------- geometry
+def warping_depth(eta: float, levels: int, m: int, n: int) -> int:
+    """core/warping.py:237-277."""
+    return int(lib().or_warping_depth(float(eta), int(levels), int(m), int(n)))
+
+
+def level_plan(m: int, n: int, eta: float, levels: int, min_level: int):
+    """core/optical_flow.py:497-518: list of (i, (h, w)) from coarse to fine
+    and the effective min_level."""
+    mly = warping_depth(eta, levels, m, m)
+    mlx = warping_depth(eta, levels, n, n)
+    max_level = min(mlx, mly) * 4
+    mly = min(mly, max_level)
+    mlx = min(mlx, max_level)
+    top = max(mlx, mly)
+    if top <= min_level:
+        min_level = top - 1
+    if min_level < 0:
+        min_level = 0
+    plan = []
+    for i in range(top, min_level - 1, -1):
+        plan.append((i, (int(round(m * eta ** min(i, mly))), int(round(n * eta ** min(i, mlx))))))
+    return plan, min_level
+
+
+# ------------------------------------------------------------------ resize
+def gauss_g(sigma: float):
+    """util/resize_util.py:216-225: float32 Gaussian, numpy arithmetic."""
+    if sigma <= 0.0:
+        return 0, np.array([1.0], dtype=np.float32)
+    R = int(np.ceil(2.0 * sigma))
+    x = np.arange(-R, R + 1, dtype=np.float32)
+    g = np.exp(-0.5 * (x / sigma) ** 2).astype(np.float32)
+    g /= g.sum()
+    return R, np.ascontiguousarray(g)
+
+
+def resize_tables(in_len: int, out_len: int, sigma: float):
+    R, g = gauss_g(sigma)
+    P = 2 * R + 4
+    idx = np.empty((out_len, P), np.int32)
+    wt = np.empty((out_len, P), np.float32)
+    lib().or_fill_resize_tables(in_len, out_len, R, g, idx, wt)
+    return idx, wt
+
+
+def resize(img, size, sigma_coeff: float = 0.6):
+    """util/resize_util.py:232-254 (imresize_fused_gauss_cubic)."""
+    oh, ow = size[:2]
+    x = np.asarray(img).astype(np.float32, copy=False)
+    sy = oh / x.shape[0]
+    sx = ow / x.shape[1]
+    s = sy if sy < sx else sx
+    sig = (sigma_coeff / s) if s < 1.0 else 0.0
+    ix, wx = resize_tables(x.shape[1], ow, sig)
+    iy, wy = resize_tables(x.shape[0], oh, sig)
+    C = 1 if x.ndim == 2 else x.shape[2]
+    src = np.ascontiguousarray(x)
+    out = np.empty((oh, ow) if x.ndim == 2 else (oh, ow, C), np.float32)
+    lib().or_resize_tab(src, x.shape[0], x.shape[1], C, out, oh, ow, ix, wx, iy, wy, wx.shape[1])
+    return out.astype(np.asarray(img).dtype, copy=False)
+
+
+# -------------------------------------------------------------------- warp
+def imregister_wrapper(f2_level, u, v, f1_level):
+    """core/warping.py:160-234, cubic interpolation (cv2.remap restated)."""
+    squeeze = f2_level.ndim == 2
+    f2 = np.ascontiguousarray(f2_level if not squeeze else f2_level[:, :, None], dtype=np.float32)
+    f1 = np.ascontiguousarray(f1_level if not squeeze else f1_level[:, :, None], dtype=np.float32)
+    H, W, C = f2.shape
+    out = np.empty((H, W, C), np.float32)
+    lib().or_warp_cubic(f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64),
+                        np.ascontiguousarray(v, dtype=np.float64), out)
+    return out[:, :, 0] if squeeze else out
+
+
+# ------------------------------------------------------------------ tensor
+def get_motion_tensor_gc(f1, f2, hy, hx, precision: str = "f64"):
+    """core/optical_flow.py:73-160 for one channel (f1, f2 float32-valued)."""
+    a = np.ascontiguousarray(f1, dtype=np.float32)
+    b = np.ascontiguousarray(f2, dtype=np.float32)
+    h, w = a.shape
+    dt = _dtype(precision)
+    J = [np.empty((h + 2, w + 2), dt) for _ in range(6)]
+    lib(precision).or_motion_tensor_gc(a, b, h, w, 1, 0, float(hy), float(hx), *J)
+    return tuple(J)
+
+
+# ------------------------------------------------------------------ solver
+def compute_flow(J11, J22, J33, J12, J13, J23, weight, u, v, alpha_x, alpha_y, iterations,
+                 update_lag, a_data, a_smooth, hx, hy, precision: str = "f64", order: int = 0):
+    """core/level_solver.py:158-369; returns (m, n, 2)."""
+    dt = _dtype(precision)
+    arrs = [np.ascontiguousarray(a, dtype=dt) for a in (J11, J22, J33, J12, J13, J23, weight)]
+    m, n, K = arrs[0].shape
+    uu = np.ascontiguousarray(u, dtype=dt)
+    vv = np.ascontiguousarray(v, dtype=dt)
+    du = np.empty((m, n), dt)
+    dv = np.empty((m, n), dt)
+    ad = np.ascontiguousarray(np.broadcast_to(np.asarray(a_data, np.float64), (K,)))
+    lib(precision).or_compute_flow(*arrs, uu, vv, m, n, K, float(alpha_x), float(alpha_y),
+                                   int(iterations), int(update_lag), ad, float(a_smooth),
+                                   float(hx), float(hy), du, dv, int(order))
+    return np.stack([du, dv], axis=2)
+
+
+def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, update_lag,
+                 verbose, a_data, a_smooth, hx, hy, gnc_beta=None, precision="f64", order=0):
+    """core/optical_flow.py:348-411 (gc path; GNC is not restated)."""
+    if gnc_beta is not None:
+        raise NotImplementedError("the oracle does not restate compute_flow_gnc")
+    r = compute_flow(J11, J22, J33, J12, J13, J23, weight, u, v, alpha[0], alpha[1], iterations,
+                     update_lag, a_data, a_smooth, hx, hy, precision=precision, order=order)
+    return r[:, :, 0], r[:, :, 1]
+
+
+def cubic_resize_cv(w, m, n):
+    """cv2.resize(w, (n, m), INTER_CUBIC) restated (core/optical_flow.py:615-616)."""
+    src = np.ascontiguousarray(w, dtype=np.float64)
+    h, ww, C = src.shape
+    out = np.empty((m, n, C), np.float64)
+    lib().or_cubic_resize_cv(src, h, ww, C, out, m, n)
+    return out
+
+
+# ------------------------------------------------------- get_displacement
+def _expand_weight(weight, m, n, C):
+    """core/optical_flow.py:856-879."""
+    if weight is None:
+        return np.ones((m, n, C), dtype=np.float64) / C
+    weight = np.asarray(weight).astype(np.float64)
+    if weight.ndim < 3:
+        if weight.ndim == 1:
+            if len(weight) < C:
+                we = np.full(C, 1.0 / C, dtype=np.float64)
+                we[: len(weight)] = weight
+                weight = we
+            elif len(weight) > C:
+                weight = weight[:C]
+            weight = weight / weight.sum()
+            weight = np.ones((m, n, C), dtype=np.float64) * weight.reshape(1, 1, -1)
+        else:
+            weight = np.ones((m, n, C), dtype=np.float64) * weight[..., np.newaxis]
+    return weight
+
+
+def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
+                     levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, const_assumption="gc",
+                     uv=None, weight=None, level_solver_backend=None, gnc_schedule=None,
+                     warping_steps=None, *, precision="f64", order=0):
+    """core/optical_flow.py:773-906 + :476-617 (gc data term, no GNC)."""
+    assert fixed.ndim == moving.ndim
+    key = str(getattr(const_assumption, "value", const_assumption)).strip().lower()
+    if key not in ("gc", "gradient"):
+        raise NotImplementedError(f"oracle restates the gc data term only, got {key!r}")
+    if gnc_schedule is not None:
+        raise NotImplementedError("oracle does not restate GNC")
+    fixed = fixed.astype(np.float64)
+    moving = moving.astype(np.float64)
+    if fixed.ndim == 2:
+        fixed = fixed[:, :, None]
+        moving = moving[:, :, None]
+    m, n, C = fixed.shape
+    weight = _expand_weight(weight, m, n, C)
+    a_data_arr = np.full(C, a_data, dtype=np.float64) if not isinstance(a_data, np.ndarray) \
+        else np.ascontiguousarray(a_data, dtype=np.float64)
+    dt = _dtype(precision)
+
+    plan, min_level = level_plan(m, n, eta, levels, min_level)
+    top = plan[0][0]
+    if uv is not None:
+        u_init, v_init = uv[:, :, 0], uv[:, :, 1]
+    else:
+        u_init = np.zeros((m, n), np.float64)
+        v_init = np.zeros((m, n), np.float64)
+    u = v = None
+    for i, level_size in plan:
+        f1_level = resize(fixed, level_size)
+        f2_level = resize(moving, level_size)
+        h_row = float(m) / f1_level.shape[0]
+        h_col = float(n) / f1_level.shape[1]
+        if i == top:
+            u = np.pad(resize(u_init, level_size), 1, mode="edge").astype(dt)
+            v = np.pad(resize(v_init, level_size), 1, mode="edge").astype(dt)
+            tmp = f2_level.astype(np.float32)
+        else:
+            u = np.pad(resize(u[1:-1, 1:-1], level_size), 1, mode="edge").astype(dt)
+            v = np.pad(resize(v[1:-1, 1:-1], level_size), 1, mode="edge").astype(dt)
+            tmp = imregister_wrapper(f2_level.astype(np.float32),
+                                     u[1:-1, 1:-1].astype(np.float64) / h_col,
+                                     v[1:-1, 1:-1].astype(np.float64) / h_row,
+                                     f1_level.astype(np.float32))
+        lh, lw = level_size
+        J = np.zeros((6, lh + 2, lw + 2, C), dt)
+        for ch in range(C):
+            Jc = get_motion_tensor_gc(f1_level[:, :, ch], tmp[:, :, ch], h_row, h_col, precision)
+            for t in range(6):
+                J[t, :, :, ch] = Jc[t]
+        wl = resize(weight, (lh, lw))
+        weight_level = np.zeros((lh + 2, lw + 2, C), dt)
+        weight_level[1:-1, 1:-1] = wl
+        alpha_scaling = 1 if i == min_level else eta ** (-0.5 * i)
+        alpha_tmp = [alpha_scaling * alpha[j] for j in range(len(alpha))]
+        args = (J[0], J[1], J[2], J[3], J[4], J[5], weight_level, u, v, alpha_tmp, iterations,
+                update_lag, 0, a_data_arr, a_smooth, h_col, h_row)
+        if level_solver_backend is not None:
+            du, dv = level_solver_backend(*args, None)
+        else:
+            du, dv = level_solver(*args, None, precision=precision, order=order)
+        du = np.array(du, dtype=dt)
+        dv = np.array(dv, dtype=dt)
+        if min(level_size) > 5:
+            du[1:-1, 1:-1] = median_filter(du[1:-1, 1:-1], size=(5, 5), mode="mirror")
+            dv[1:-1, 1:-1] = median_filter(dv[1:-1, 1:-1], size=(5, 5), mode="mirror")
+        u = u + du
+        v = v + dv
+    w = np.zeros((u.shape[0] - 2, u.shape[1] - 2, 2), dtype=np.float64)
+    w[:, :, 0] = u[1:-1, 1:-1]
+    w[:, :, 1] = v[1:-1, 1:-1]
+    if min_level > 0:
+        w = cubic_resize_cv(w, m, n)
+    return w
+
+
+# ------------------------------------------------------------ preprocess
+def normalize(arr, ref=None, channel_normalization="together", eps=1e-8):
+    """util/image_processing.py:12-98."""
+    if channel_normalization == "separate" and arr.ndim in (3, 4):
+        result = np.zeros_like(arr, dtype=np.float64)
+        for c in range(arr.shape[-1]):
+            src = ref[..., c] if (ref is not None and ref.ndim >= 3) else arr[..., c]
+            lo, hi = src.min(), src.max()
+            result[..., c] = (arr[..., c] - lo) / (hi - lo + eps)
+        return result
+    src = ref if ref is not None else arr
+    lo, hi = src.min(), src.max()
+    return (arr - lo) / (hi - lo + eps)
+
+
+def apply_gaussian_filter(arr, sigma, mode="reflect", truncate=4.0):
+    """util/image_processing.py:101-165 (scipy, the reference's dependency)."""
+    sigma = np.asarray(sigma)
+    if arr.ndim == 3:
+        result = np.zeros_like(arr, dtype=np.float64)
+        for c in range(arr.shape[2]):
+            s = sigma[min(c, len(sigma) - 1), :2] if sigma.ndim == 2 else sigma[:2]
+            result[..., c] = gaussian_filter(arr[..., c], sigma=s, mode=mode, truncate=truncate)
+        return result
+    if arr.ndim == 4:
+        result = np.zeros_like(arr, dtype=np.float64)
+        for c in range(arr.shape[3]):
+            s = sigma[min(c, len(sigma) - 1)] if sigma.ndim == 2 else sigma
+            result[..., c] = gaussian_filter(arr[..., c], sigma=(s[2], s[0], s[1]), mode=mode,
+                                             truncate=truncate)
+        return result
+    return arr
+
+
+def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
+               channel_normalization="together"):
+    """motion_correction/compensate_recording.py:521-561 (normalize, then
+    Gaussian), followed by the executors' float32 cast (parallelization/
+    base.py:63-73)."""
+    normalized = normalize(frames, ref=normalization_ref,
+                           channel_normalization=channel_normalization)
+    filtered = apply_gaussian_filter(normalized, sigma=np.asarray(sigma), mode="reflect",
+                                     truncate=4.0)
+    return filtered.astype(np.float64)

@@ -1,0 +1,576 @@
+// C ABI of libflowreg_hip.so: argument checking, host-side planning, device
+// workspaces and the pyramid level loop that drives the kernels.
+#include <stdarg.h>
+#include <stdio.h>
+#include <string.h>
+
+#include <cmath>
+#include <string>
+#include <vector>
+
+#include "../../include/flowreg_hip.h"
+#include "frh_internal.h"
+
+using namespace frh;
+
+namespace {
+
+thread_local char g_err[512] = "";
+
+int fail(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
+#define FRH_HIP(call, what)                                                                   \
+    do {                                                                                      \
+        hipError_t e_ = (call);                                                               \
+        if (e_ != hipSuccess) return fail(FRH_EHIP, "%s: %s", what, hipGetErrorString(e_)); \
+    } while (0)
+
+#define FRH_REQ(cond, ...)                                   \
+    do {                                                     \
+        if (!(cond)) return fail(FRH_EINVAL, __VA_ARGS__);   \
+    } while (0)
+
+hipStream_t as_stream(void *s) { return (hipStream_t)s; }
+
+double py_round(double x) { return std::nearbyint(x); }
+
+// util/resize_util.py:68-76, A = -0.75
+double keys_cubic(double x)
+{
+    const double A = -0.75;
+    double ax = std::fabs(x);
+    if (ax < 1.0) return (A + 2.0) * ax * ax * ax - (A + 3.0) * ax * ax + 1.0;
+    if (ax < 2.0) return A * ax * ax * ax - 5.0 * A * ax * ax + 8.0 * A * ax - 4.0 * A;
+    return 0.0;
+}
+
+int reflect_index(int j, int n)
+{
+    if (n <= 1) return 0;
+    while (j < 0 || j >= n) j = j < 0 ? -j - 1 : 2 * n - 1 - j;
+    return j;
+}
+
+// Pass schedule of compute_flow (core/level_solver.py:229-258): psi is
+// refreshed before sweep `it` when (it + 1) % update_lag == 0; between
+// refreshes the coefficients are constant, so a segment of sweeps runs as one
+// or more wavefront passes of at most pmax sweeps.
+struct Pass {
+    int nsweeps;
+    bool assemble;  // form coefficients before this pass
+    bool psi_one;   // psi = 1 (no refresh has happened yet)
+};
+
+std::vector<Pass> plan_passes(int iterations, int lag, int pmax)
+{
+    std::vector<int> starts;
+    for (int it = 0; it < iterations; ++it)
+        if (it == 0 || (it + 1) % lag == 0) starts.push_back(it);
+    std::vector<Pass> out;
+    for (size_t k = 0; k < starts.size(); ++k) {
+        int b = starts[k], e = k + 1 < starts.size() ? starts[k + 1] : iterations;
+        bool refresh = ((b + 1) % lag == 0);
+        bool first = true;
+        for (int s = b; s < e; s += pmax) {
+            int n = e - s < pmax ? e - s : pmax;
+            out.push_back({n, first, !refresh});
+            first = false;
+        }
+    }
+    return out;
+}
+
+int sweep_pmax(int h) { return 64 * ((h + 63) / 64) <= 512 ? 5 : 2; }
+
+template <typename T>
+int dev_alloc(T **p, size_t n, const char *what)
+{
+    if (n == 0) n = 1;
+    hipError_t e = hipMalloc((void **)p, sizeof(T) * n);
+    if (e != hipSuccess) return fail(FRH_ENOMEM, "hipMalloc(%s, %zu bytes): %s", what, sizeof(T) * n, hipGetErrorString(e));
+    return FRH_OK;
+}
+
+template <typename T>
+int upload(T **dev, const T *host, size_t n, const char *what)
+{
+    int rc = dev_alloc(dev, n, what);
+    if (rc) return rc;
+    FRH_HIP(hipMemcpy(*dev, host, sizeof(T) * n, hipMemcpyHostToDevice), what);
+    return FRH_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *frh_last_error(void) { return g_err; }
+
+int frh_abi_version(void) { return 1; }
+
+int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
+{
+    FRH_REQ(in_len > 0 && out_len > 0 && R >= 0 && g && idx && wt, "frh_fill_resize_tables: bad arguments");
+    int P = 2 * R + 4;
+    double scale = (double)out_len / (double)in_len;
+    for (int i = 0; i < out_len; ++i) {
+        double x = (i + 0.5) / scale - 0.5;
+        int left = (int)std::floor(x - 2.0) - R;
+        double ssum = 0.0;
+        for (int p = 0; p < P; ++p) {
+            int j = left + p;
+            idx[(size_t)i * P + p] = reflect_index(j, in_len);
+            double d = x - j, acc = 0.0;
+            for (int u = -R; u <= R; ++u) acc += (double)g[u + R] * keys_cubic(d - u);
+            wt[(size_t)i * P + p] = (float)acc;
+            ssum += acc;
+        }
+        double inv = 1.0 / ssum;
+        for (int p = 0; p < P; ++p) wt[(size_t)i * P + p] = (float)((double)wt[(size_t)i * P + p] * inv);
+    }
+    return FRH_OK;
+}
+
+int frh_warping_depth(double eta, int levels, int m, int n)
+{
+    double min_dim = (double)(m < n ? m : n);
+    int depth = 0;
+    for (int k = 0; k < levels; ++k) {
+        depth += 1;
+        min_dim *= eta;
+        if (py_round(min_dim) < 10.0) break;
+    }
+    return depth;
+}
+
+// ------------------------------------------------------------------ stages
+
+static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, const double *lo,
+                          const double *den, const double *wy, int ry, const double *wx, int rx,
+                          float *out, void *stream)
+{
+    FRH_REQ(raw && out && lo && den && wy && wx, "frh_prefilter: null pointer");
+    FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0 && C <= 32, "frh_prefilter: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
+    FRH_REQ(ry >= 0 && ry <= 32 && rx >= 0 && rx <= 32, "frh_prefilter: Gaussian radius must be in [0, 32]");
+    hipStream_t st = as_stream(stream);
+    double *tmp = nullptr;
+    FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)T * C * H * W, st), "frh_prefilter scratch");
+    hipError_t e = f64 ? launch_prefilter_v64((const double *)raw, T, H, W, C, lo, den, wy, ry, tmp, st)
+                       : launch_prefilter_v((const float *)raw, T, H, W, C, lo, den, wy, ry, tmp, st);
+    if (e == hipSuccess) e = launch_prefilter_h(tmp, T, H, W, C, wx, rx, out, st);
+    hipError_t e2 = hipFreeAsync(tmp, st);
+    if (e != hipSuccess) return fail(FRH_EHIP, "prefilter kernels: %s", hipGetErrorString(e));
+    FRH_HIP(e2, "frh_prefilter free");
+    return FRH_OK;
+}
+
+int frh_prefilter(const float *raw, int T, int H, int W, int C, const double *lo, const double *den,
+                  const double *wy, int ry, const double *wx, int rx, float *out, void *stream)
+{
+    return prefilter_impl(raw, 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, stream);
+}
+
+int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const double *lo,
+                      const double *den, const double *wy, int ry, const double *wx, int rx,
+                      float *out, void *stream)
+{
+    return prefilter_impl(raw, 1, T, H, W, C, lo, den, wy, ry, wx, rx, out, stream);
+}
+
+int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow, const int32_t *ix,
+               const float *wx, const int32_t *iy, const float *wy, int P, float *tmp, void *stream)
+{
+    FRH_REQ(src && dst && ix && wx && iy && wy && tmp, "frh_resize: null pointer");
+    FRH_REQ(N > 0 && h > 0 && w > 0 && oh > 0 && ow > 0 && P >= 4, "frh_resize: bad shape");
+    FRH_REQ(oh <= 65535 && h <= 65535 && N <= 65535, "frh_resize: grid limits exceeded");
+    hipError_t e = launch_resize(src, N, h, w, dst, oh, ow, ix, wx, iy, wy, P, tmp, as_stream(stream));
+    FRH_HIP(e, "resize kernels");
+    return FRH_OK;
+}
+
+int frh_warp_cubic(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H, int W,
+                   const float *u, const float *v, double h_col, double h_row, float *out, void *stream)
+{
+    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_cubic: null pointer");
+    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_cubic: bad shape");
+    FRH_REQ(h_col > 0 && h_row > 0, "frh_warp_cubic: grid spacings must be positive");
+    hipError_t e = launch_warp(f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, h_col, h_row, out,
+                               as_stream(stream));
+    FRH_HIP(e, "warp kernel");
+    return FRH_OK;
+}
+
+int frh_motion_tensor_gc(const float *f1, const float *f2, int h, int w, double hy, double hx,
+                         double *J11, double *J22, double *J33, double *J12, double *J13, double *J23,
+                         void *stream)
+{
+    FRH_REQ(f1 && f2 && J11 && J22 && J33 && J12 && J13 && J23, "frh_motion_tensor_gc: null pointer");
+    FRH_REQ(h > 0 && w > 0 && h + 2 <= 65535, "frh_motion_tensor_gc: bad shape");
+    hipError_t e = launch_tensor_rowmajor(f1, f2, h, w, hy, hx, J11, J22, J33, J12, J13, J23, as_stream(stream));
+    FRH_HIP(e, "tensor kernel");
+    return FRH_OK;
+}
+
+int frh_median5_mirror(double *f, int N, int h, int w, void *stream)
+{
+    FRH_REQ(f && N > 0 && h > 0 && w > 0 && h <= 65535 && N <= 65535, "frh_median5_mirror: bad arguments");
+    hipStream_t st = as_stream(stream);
+    double *tmp = nullptr;
+    FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)N * h * w, st), "median scratch");
+    hipError_t e = launch_median_plane(f, N, h, w, tmp, st);
+    hipError_t e2 = hipFreeAsync(tmp, st);
+    FRH_HIP(e, "median kernel");
+    FRH_HIP(e2, "median free");
+    return FRH_OK;
+}
+
+int frh_level_solver(const double *J11, const double *J22, const double *J33, const double *J12,
+                     const double *J13, const double *J23, const double *weight, const double *u,
+                     const double *v, int m, int n, int K, double alpha_x, double alpha_y,
+                     int iterations, int update_lag, const double *a_data, double a_smooth,
+                     double hx, double hy, double *du, double *dv, void *stream)
+{
+    FRH_REQ(J11 && J22 && J33 && J12 && J13 && J23 && weight && u && v && a_data && du && dv,
+            "frh_level_solver: null pointer");
+    FRH_REQ(m >= 3 && n >= 3 && K >= 1 && K <= 8, "frh_level_solver: need m, n >= 3 and 1 <= K <= 8 (m=%d n=%d K=%d)", m, n, K);
+    FRH_REQ(m - 2 <= 1024, "frh_level_solver: at most 1024 interior rows (got %d)", m - 2);
+    FRH_REQ(iterations >= 0 && update_lag >= 1, "frh_level_solver: iterations >= 0, update_lag >= 1");
+    FRH_REQ(hx > 0 && hy > 0, "frh_level_solver: hx, hy must be positive");
+    if (a_smooth != 1.0)
+        return fail(FRH_EUNSUP, "frh_level_solver: a_smooth != 1 is not implemented on the GPU");
+    hipStream_t st = as_stream(stream);
+    Skew s = make_skew(m - 2, n - 2);
+    const double *Js[6] = {J11, J22, J33, J12, J13, J23};
+    size_t npl = (size_t)6 * K + K + 2 + 5 + 2 + 2;
+    double *work = nullptr;
+    FRH_HIP(hipMallocAsync((void **)&work, sizeof(double) * npl * s.plane, st), "level solver scratch");
+    double *J = work, *Wsk = J + 6 * K * s.plane, *cuv = Wsk + K * s.plane, *coef = cuv + 2 * s.plane;
+    double *duv = coef + 5 * s.plane, *ring = duv + 2 * s.plane;
+    double ax = alpha_x / (hx * hx), ay = alpha_y / (hy * hy);
+    std::vector<double> am1(K);
+    for (int k = 0; k < K; ++k) am1[k] = a_data[k] - 1.0;
+    hipError_t e = hipSuccess;
+    for (int c = 0; c < 6 && e == hipSuccess; ++c)
+        e = launch_padded_to_skew_f64(Js[c], m, n, K, s, J + (size_t)c * K * s.plane, st);
+    if (e == hipSuccess) e = launch_padded_to_skew_f64(weight, m, n, K, s, Wsk, st);
+    if (e == hipSuccess) e = launch_cuv_padded(u, v, m, n, ax, ay, s, cuv, st);
+    std::vector<Pass> passes = plan_passes(iterations, update_lag, sweep_pmax(s.h));
+    size_t fstride = npl * s.plane;
+    if (passes.empty() && e == hipSuccess) e = launch_fill_f64(duv, 4 * s.plane, 0.0, st);
+    for (size_t k = 0; k < passes.size() && e == hipSuccess; ++k) {
+        if (passes[k].assemble)
+            e = launch_assemble(J, Wsk, cuv, duv, coef, fstride, 1, K, s, ax, ay, am1.data(), a_data,
+                                passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0, st);
+        if (e == hipSuccess)
+            e = launch_sweep(passes[k].nsweeps, coef, duv, k + 1 == passes.size() ? ring : nullptr,
+                             fstride, 1, s, ax, ay, k == 0 ? 1 : 0, st);
+    }
+    if (e == hipSuccess) e = launch_skew_to_padded_f64(duv, ring, s, du, dv, st);
+    hipError_t e2 = hipFreeAsync(work, st);
+    FRH_HIP(e, "level solver kernels");
+    FRH_HIP(e2, "level solver free");
+    return FRH_OK;
+}
+
+}  // extern "C"
+
+// ------------------------------------------------------------- whole pyramid
+
+struct LevelDev {
+    int h, w;
+    double h_row, h_col, axw, ayw;
+    int P_img, P_up;
+    int32_t *img_ix = nullptr, *img_iy = nullptr, *up_ix = nullptr, *up_iy = nullptr;
+    float *img_wx = nullptr, *img_wy = nullptr, *up_wx = nullptr, *up_wy = nullptr;
+    Skew sk;
+    size_t f1_off, W_off;
+    std::vector<Pass> passes;
+};
+
+enum { TM_TOTAL, TM_SWEEP, TM_ASSEMBLE, TM_TENSOR, TM_RESIZE, TM_WARP, TM_MEDIAN, TM_N };
+
+struct frh_solver {
+    int H, W, C, nl, iterations, update_lag, max_batch;
+    double a_smooth;
+    std::vector<double> a_data, am1;
+    std::vector<LevelDev> lv;
+    float *f1lev = nullptr, *uvtop = nullptr, *tmp = nullptr, *f2l = nullptr, *f2w = nullptr;
+    float *uvl = nullptr, *uvn = nullptr;
+    double *Wlev = nullptr, *work = nullptr;
+    size_t npl = 0;
+    bool have_ref = false;
+    bool profile = false;
+    struct Mark {
+        int cat;
+        hipEvent_t a, b;
+    };
+    std::vector<Mark> marks;
+    std::vector<hipEvent_t> pool;
+    size_t pool_used = 0;
+    double tm[TM_N] = {0};
+    int counts[TM_N] = {0};
+};
+
+static void free_solver(frh_solver *s)
+{
+    if (!s) return;
+    for (auto &L : s->lv) {
+        (void)hipFree(L.img_ix); (void)hipFree(L.img_iy); (void)hipFree(L.img_wx); (void)hipFree(L.img_wy);
+        (void)hipFree(L.up_ix); (void)hipFree(L.up_iy); (void)hipFree(L.up_wx); (void)hipFree(L.up_wy);
+    }
+    (void)hipFree(s->f1lev); (void)hipFree(s->uvtop); (void)hipFree(s->tmp); (void)hipFree(s->f2l); (void)hipFree(s->f2w);
+    (void)hipFree(s->uvl); (void)hipFree(s->uvn); (void)hipFree(s->Wlev); (void)hipFree(s->work);
+    for (auto e : s->pool) (void)hipEventDestroy(e);
+    delete s;
+}
+
+// Event bracketing of one launch group (only when profiling is enabled).
+static hipEvent_t next_event(frh_solver *s)
+{
+    if (s->pool_used == s->pool.size()) {
+        hipEvent_t e;
+        (void)hipEventCreate(&e);
+        s->pool.push_back(e);
+    }
+    return s->pool[s->pool_used++];
+}
+
+struct Scope {
+    frh_solver *s;
+    int cat;
+    hipStream_t st;
+    hipEvent_t a{};
+    Scope(frh_solver *s_, int cat_, hipStream_t st_) : s(s_), cat(cat_), st(st_)
+    {
+        if (s->profile) {
+            a = next_event(s);
+            (void)hipEventRecord(a, st);
+        }
+    }
+    ~Scope()
+    {
+        if (s->profile) {
+            hipEvent_t b = next_event(s);
+            (void)hipEventRecord(b, st);
+            s->marks.push_back({cat, a, b});
+        }
+    }
+};
+
+extern "C" {
+
+int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
+{
+    FRH_REQ(out && d && d->levels && d->a_data, "frh_solver_create: null pointer");
+    FRH_REQ(d->H > 0 && d->W > 0 && d->C >= 1 && d->C <= 8, "frh_solver_create: bad geometry");
+    FRH_REQ(d->n_levels >= 1 && d->max_batch >= 1 && d->max_batch <= 65535, "frh_solver_create: bad level count or batch");
+    FRH_REQ(d->iterations >= 0 && d->update_lag >= 1, "frh_solver_create: iterations >= 0, update_lag >= 1");
+    if (d->a_smooth != 1.0) return fail(FRH_EUNSUP, "a_smooth != 1 is not implemented on the GPU");
+    frh_solver *s = new frh_solver();
+    s->H = d->H; s->W = d->W; s->C = d->C; s->nl = d->n_levels; s->max_batch = d->max_batch;
+    s->iterations = d->iterations; s->update_lag = d->update_lag; s->a_smooth = d->a_smooth;
+    s->a_data.assign(d->a_data, d->a_data + d->C);
+    for (double a : s->a_data) s->am1.push_back(a - 1.0);
+    size_t f1_total = 0, W_total = 0, plane_max = 0;
+    int hmax = 0, wmax = 0;
+    int rc = FRH_OK;
+    for (int i = 0; i < d->n_levels && rc == FRH_OK; ++i) {
+        const frh_level_desc &L = d->levels[i];
+        if (L.h < 1 || L.w < 1 || L.h > 1024 || L.P_img < 4 || (i > 0 && L.P_up < 4)) {
+            rc = fail(FRH_EINVAL, "frh_solver_create: level %d invalid (h=%d w=%d, at most 1024 rows)", i, L.h, L.w);
+            break;
+        }
+        if (i > 0 && (L.h < d->levels[i - 1].h || L.w < d->levels[i - 1].w)) {
+            rc = fail(FRH_EINVAL, "frh_solver_create: levels must go from coarse to fine");
+            break;
+        }
+        LevelDev D;
+        D.h = L.h; D.w = L.w; D.h_row = L.h_row; D.h_col = L.h_col;
+        D.axw = L.alpha_x / (L.h_col * L.h_col);
+        D.ayw = L.alpha_y / (L.h_row * L.h_row);
+        D.P_img = L.P_img; D.P_up = i > 0 ? L.P_up : 0;
+        D.sk = make_skew(L.h, L.w);
+        D.f1_off = f1_total;
+        D.W_off = W_total;
+        D.passes = plan_passes(d->iterations, d->update_lag, sweep_pmax(L.h));
+        f1_total += (size_t)d->C * L.h * L.w;
+        W_total += (size_t)d->C * D.sk.plane;
+        if (D.sk.plane > plane_max) plane_max = D.sk.plane;
+        if (L.h > hmax) hmax = L.h;
+        if (L.w > wmax) wmax = L.w;
+        rc = upload(&D.img_ix, L.img_ix, (size_t)L.w * L.P_img, "tables");
+        if (!rc) rc = upload(&D.img_wx, L.img_wx, (size_t)L.w * L.P_img, "tables");
+        if (!rc) rc = upload(&D.img_iy, L.img_iy, (size_t)L.h * L.P_img, "tables");
+        if (!rc) rc = upload(&D.img_wy, L.img_wy, (size_t)L.h * L.P_img, "tables");
+        if (!rc && i > 0) {
+            rc = upload(&D.up_ix, L.up_ix, (size_t)L.w * L.P_up, "tables");
+            if (!rc) rc = upload(&D.up_wx, L.up_wx, (size_t)L.w * L.P_up, "tables");
+            if (!rc) rc = upload(&D.up_iy, L.up_iy, (size_t)L.h * L.P_up, "tables");
+            if (!rc) rc = upload(&D.up_wy, L.up_wy, (size_t)L.h * L.P_up, "tables");
+        }
+        s->lv.push_back(D);
+    }
+    size_t T = (size_t)d->max_batch, C = d->C, HW = (size_t)d->H * d->W;
+    s->npl = 6 * C + 2 + 5 + 2;
+    size_t cmax = C > 2 ? C : 2;
+    if (!rc) rc = dev_alloc(&s->f1lev, f1_total, "reference levels");
+    if (!rc) rc = dev_alloc(&s->Wlev, W_total, "weight levels");
+    if (!rc) rc = dev_alloc(&s->uvtop, (size_t)2 * s->lv[0].h * s->lv[0].w, "initial flow");
+    if (!rc) rc = dev_alloc(&s->tmp, T * cmax * HW, "resize scratch");
+    if (!rc) rc = dev_alloc(&s->f2l, T * C * (size_t)hmax * wmax, "level images");
+    if (!rc) rc = dev_alloc(&s->f2w, T * C * (size_t)hmax * wmax, "warped images");
+    if (!rc) rc = dev_alloc(&s->uvl, T * 2 * (size_t)hmax * wmax, "level flow");
+    if (!rc) rc = dev_alloc(&s->uvn, T * 2 * (size_t)hmax * wmax, "next flow");
+    if (!rc) rc = dev_alloc(&s->work, T * s->npl * plane_max, "solver workspace");
+    if (rc) {
+        free_solver(s);
+        return rc;
+    }
+    *out = s;
+    return FRH_OK;
+}
+
+void frh_solver_destroy(frh_solver *s) { free_solver(s); }
+
+int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *weight, const float *uv,
+                             void *stream)
+{
+    FRH_REQ(s && fixed && weight, "frh_solver_set_reference: null pointer");
+    hipStream_t st = as_stream(stream);
+    int C = s->C;
+    for (int i = 0; i < s->nl; ++i) {
+        LevelDev &L = s->lv[i];
+        FRH_HIP(launch_resize(fixed, C, s->H, s->W, s->f1lev + L.f1_off, L.h, L.w, L.img_ix, L.img_wx,
+                              L.img_iy, L.img_wy, L.P_img, s->tmp, st), "reference pyramid");
+        float *wl = s->f2l;  // scratch for the resized weight planes
+        FRH_HIP(launch_resize(weight, C, s->H, s->W, wl, L.h, L.w, L.img_ix, L.img_wx, L.img_iy,
+                              L.img_wy, L.P_img, s->tmp, st), "weight pyramid");
+        FRH_HIP(launch_to_skew_f32(wl, C, L.sk, s->Wlev + L.W_off, st), "weight skew");
+    }
+    LevelDev &T0 = s->lv[0];
+    if (uv) {
+        FRH_HIP(launch_resize(uv, 2, s->H, s->W, s->uvtop, T0.h, T0.w, T0.img_ix, T0.img_wx, T0.img_iy,
+                              T0.img_wy, T0.P_img, s->tmp, st), "initial flow resize");
+    } else {
+        FRH_HIP(hipMemsetAsync(s->uvtop, 0, sizeof(float) * 2 * T0.h * T0.w, st), "initial flow clear");
+    }
+    s->have_ref = true;
+    return FRH_OK;
+}
+
+int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int out_f64, void *stream)
+{
+    FRH_REQ(s && moving && w_out, "frh_solver_run: null pointer");
+    FRH_REQ(s->have_ref, "frh_solver_run: frh_solver_set_reference has not been called");
+    FRH_REQ(T >= 1 && T <= s->max_batch, "frh_solver_run: T=%d outside [1, %d]", T, s->max_batch);
+    hipStream_t st = as_stream(stream);
+    const int C = s->C;
+    s->marks.clear();
+    s->pool_used = 0;
+    hipEvent_t t0{}, t1{};
+    if (s->profile) {
+        t0 = next_event(s);
+        (void)hipEventRecord(t0, st);
+    }
+    for (int li = 0; li < s->nl; ++li) {
+        LevelDev &L = s->lv[li];
+        const int h = L.h, w = L.w;
+        const size_t hw = (size_t)h * w;
+        const Skew &sk = L.sk;
+        const size_t fs = s->npl * sk.plane;
+        double *J = s->work, *cuv = J + 6 * C * sk.plane, *coef = cuv + 2 * sk.plane, *duv = coef + 5 * sk.plane;
+        {
+            Scope sc(s, TM_RESIZE, st);
+            FRH_HIP(launch_resize(moving, T * C, s->H, s->W, s->f2l, h, w, L.img_ix, L.img_wx, L.img_iy,
+                                  L.img_wy, L.P_img, s->tmp, st), "level images");
+        }
+        const float *f2src = s->f2l;
+        if (li == 0) {
+            FRH_HIP(launch_broadcast_f32(s->uvtop, 2 * hw, s->uvl, 2 * hw, T, st), "initial flow");
+        } else {
+            const LevelDev &Lp = s->lv[li - 1];
+            {
+                Scope sc(s, TM_RESIZE, st);
+                FRH_HIP(launch_resize(s->uvn, T * 2, Lp.h, Lp.w, s->uvl, h, w, L.up_ix, L.up_wx, L.up_iy,
+                                      L.up_wy, L.P_up, s->tmp, st), "flow upsample");
+            }
+            {
+                Scope sc(s, TM_WARP, st);
+                FRH_HIP(launch_warp(s->f2l, s->f1lev + L.f1_off, 0, T, C, h, w, s->uvl, s->uvl + hw, 2 * hw,
+                                    L.h_col, L.h_row, s->f2w, st), "warp");
+            }
+            f2src = s->f2w;
+        }
+        {
+            Scope sc(s, TM_TENSOR, st);
+            FRH_HIP(launch_tensor(s->f1lev + L.f1_off, f2src, T, C, h, w, L.h_row, L.h_col, s->uvl,
+                                  s->uvl + hw, 2 * hw, L.axw, L.ayw, J, cuv, fs, sk, st), "motion tensor");
+        }
+        for (size_t k = 0; k < L.passes.size(); ++k) {
+            const Pass &p = L.passes[k];
+            if (p.assemble) {
+                Scope sc(s, TM_ASSEMBLE, st);
+                FRH_HIP(launch_assemble(J, s->Wlev + L.W_off, cuv, duv, coef, fs, T, C, sk, L.axw, L.ayw,
+                                        s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
+                                        st), "assemble");
+            }
+            Scope sc(s, TM_SWEEP, st);
+            FRH_HIP(launch_sweep(p.nsweeps, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw, k == 0 ? 1 : 0, st),
+                    "sor sweep");
+        }
+        {
+            Scope sc(s, TM_MEDIAN, st);
+            const int med = (h < w ? h : w) > 5;
+            if (li + 1 == s->nl) {
+                FRH_HIP(launch_median_add(duv, fs, sk, T, med, s->uvl, s->uvl + hw, 2 * hw, (float *)w_out,
+                                          out_f64 ? (double *)w_out : nullptr, 1, st), "median/add (output)");
+            } else {
+                FRH_HIP(launch_median_add(duv, fs, sk, T, med, s->uvl, s->uvl + hw, 2 * hw, s->uvn, nullptr,
+                                          0, st), "median/add");
+            }
+        }
+    }
+    if (s->profile) {
+        t1 = next_event(s);
+        (void)hipEventRecord(t1, st);
+        FRH_HIP(hipStreamSynchronize(st), "profile sync");
+        for (int k = 0; k < TM_N; ++k) { s->tm[k] = 0; s->counts[k] = 0; }
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, t0, t1);
+        s->tm[TM_TOTAL] = ms;
+        s->counts[TM_TOTAL] = 1;
+        for (auto &m : s->marks) {
+            (void)hipEventElapsedTime(&ms, m.a, m.b);
+            s->tm[m.cat] += ms;
+            s->counts[m.cat] += 1;
+        }
+    }
+    return FRH_OK;
+}
+
+int frh_solver_profile(frh_solver *s, int enable)
+{
+    FRH_REQ(s, "frh_solver_profile: null solver");
+    s->profile = enable != 0;
+    return FRH_OK;
+}
+
+int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n)
+{
+    FRH_REQ(s && ms && counts, "frh_solver_timings: null pointer");
+    int k = 0;
+    for (; k < n && k < TM_N; ++k) {
+        ms[k] = s->tm[k];
+        counts[k] = s->counts[k];
+    }
+    return k;
+}
+
+}  // extern "C"

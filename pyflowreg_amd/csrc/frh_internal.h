@@ -1,0 +1,81 @@
+// Internal declarations shared by the HIP kernels and the C ABI layer.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace frh {
+
+// Skewed ("diagonal-major") layout of a level's interior cells used by the
+// solver kernels: cell (r, c) of an h x w interior lives at
+// (r + c) * hs + r, hs = 64 * ceil(h / 64).  At wavefront step t the SOR
+// kernel's lane r touches cell (r, t - r), so every load and store of a step
+// is one contiguous run of lanes.
+struct Skew {
+    int h, w, hs;
+    size_t plane;  // elements per plane: (h + w - 1) * hs
+};
+
+inline Skew make_skew(int h, int w)
+{
+    Skew s;
+    s.h = h;
+    s.w = w;
+    s.hs = 64 * ((h + 63) / 64);
+    s.plane = (size_t)(h + w - 1) * (size_t)s.hs;
+    return s;
+}
+
+__host__ __device__ inline size_t sk(int hs, int r, int c) { return (size_t)(r + c) * hs + r; }
+
+constexpr int kMaxSweepsPerPass = 8;
+constexpr int kCoefPlanes = 5;  // Du, Dv, A12, Bu, Bv
+
+// ---- launchers (frh_kernels.hip); all return hipError_t of the launch ----
+hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, const double *lo,
+                              const double *den, const double *wy, int ry, double *tmp,
+                              hipStream_t st);
+hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, const double *wx,
+                              int rx, float *out, hipStream_t st);
+hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow,
+                         const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
+                         int P, float *tmp, hipStream_t st);
+hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                       int W, const float *u, const float *v, size_t uv_frame_stride,
+                       double h_col, double h_row, float *out, hipStream_t st);
+hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
+                         double hx, const float *u, const float *v, size_t uv_frame_stride,
+                         double ax, double ay, double *J, double *cuv, size_t frame_stride,
+                         const Skew &s, hipStream_t st);
+hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
+                           const double *duv, double *coef, size_t frame_stride, int T, int C,
+                           const Skew &s, double ax, double ay, const double *a_data_minus1,
+                           const double *a_data, int psi_one, int zero_init, hipStream_t st);
+hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
+                        int T, const Skew &s, double ax, double ay, int zero_init,
+                        hipStream_t st);
+hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew &s, int T,
+                             int do_median, const float *u, const float *v,
+                             size_t uv_frame_stride, float *out_f32, double *out_f64,
+                             int out_interleaved, hipStream_t st);
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st);
+hipError_t launch_broadcast_f32(const float *src, size_t n, float *dst, size_t dst_stride, int T,
+                                hipStream_t st);
+hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *dst,
+                              hipStream_t st);
+hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, const double *lo,
+                                const double *den, const double *wy, int ry, double *tmp,
+                                hipStream_t st);
+hipError_t launch_padded_to_skew_f64(const double *src, int m, int n, int K, const Skew &s,
+                                     double *dst, hipStream_t st);
+hipError_t launch_skew_to_padded_f64(const double *duv, const double *ring, const Skew &s,
+                                     double *du, double *dv, hipStream_t st);
+hipError_t launch_median_plane(double *f, int N, int h, int w, double *tmp, hipStream_t st);
+hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w, double hy,
+                                  double hx, double *J11, double *J22, double *J33, double *J12,
+                                  double *J13, double *J23, hipStream_t st);
+hipError_t launch_cuv_padded(const double *u, const double *v, int m, int n, double ax,
+                             double ay, const Skew &s, double *cuv, hipStream_t st);
+
+}  // namespace frh

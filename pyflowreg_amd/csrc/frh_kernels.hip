@@ -1,0 +1,888 @@
+// HIP kernels for the variational optical-flow path on gfx950.
+//
+// Every kernel reproduces the arithmetic of one reference stage in a fixed
+// operation order (compiled with -ffp-contract=off, IEEE division and
+// square root) so that results are bit-identical to the CPU oracle given the
+// same inputs.  Reference citations are pyflowreg source paths.
+#include "frh_internal.h"
+#include "../../include/flowreg_detmath.h"
+
+namespace frh {
+
+__device__ inline int reflect_half(int j, int n)  // ...b a | a b...  (scipy "reflect")
+{
+    if (n <= 1) return 0;
+    while (j < 0 || j >= n) j = j < 0 ? -j - 1 : 2 * n - 1 - j;
+    return j;
+}
+
+__device__ inline int mirror_edge(int j, int n)  // ...c b | a b c...  (scipy "mirror")
+{
+    if (n == 1) return 0;
+    int period = 2 * n - 2;
+    j %= period;
+    if (j < 0) j += period;
+    return j < n ? j : period - j;
+}
+
+__device__ inline int clampi(int x, int lo, int hi) { return x < lo ? lo : (x > hi ? hi : x); }
+
+// -------------------------------------------------------------- prefilter
+// util/image_processing.py:12-98 + :101-165.  scipy.ndimage.correlate1d with
+// a symmetric kernel sums  x[0] w[0] + sum_{j=r..1} (x[-j] + x[j]) w[j].
+struct GaussW {
+    double w[65];
+};
+
+template <typename T>
+__global__ void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, GaussW lo_den,
+                              GaussW wy, int ry, double *__restrict__ tmp)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int tc = blockIdx.z;
+    if (x >= W) return;
+    int t = tc / C, c = tc % C;
+    double lo = lo_den.w[2 * c], den = lo_den.w[2 * c + 1];
+    const T *base = raw + (size_t)t * H * W * C + c;
+    auto nrm = [&](int yy) { return ((double)base[((size_t)yy * W + x) * C] - lo) / den; };
+    double acc = nrm(y) * wy.w[ry];
+    for (int j = ry; j >= 1; --j)
+        acc += (nrm(reflect_half(y - j, H)) + nrm(reflect_half(y + j, H))) * wy.w[ry - j];
+    tmp[((size_t)tc * H + y) * W + x] = acc;
+}
+
+__global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx, int rx,
+                              float *__restrict__ out)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int tc = blockIdx.z;
+    if (x >= W) return;
+    const double *row = tmp + ((size_t)tc * H + y) * W;
+    double acc = row[x] * wx.w[rx];
+    for (int j = rx; j >= 1; --j)
+        acc += (row[reflect_half(x - j, W)] + row[reflect_half(x + j, W)]) * wx.w[rx - j];
+    out[((size_t)tc * H + y) * W + x] = (float)acc;
+}
+
+hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, const double *lo,
+                              const double *den, const double *wy, int ry, double *tmp,
+                              hipStream_t st)
+{
+    GaussW ld{}, g{};
+    for (int c = 0; c < C; ++c) {
+        ld.w[2 * c] = lo[c];
+        ld.w[2 * c + 1] = den[c];
+    }
+    for (int k = 0; k < 2 * ry + 1; ++k) g.w[k] = wy[k];
+    dim3 grid((W + 63) / 64, H, T * C);
+    hipLaunchKernelGGL(k_prefilter_v<float>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, tmp);
+    return hipGetLastError();
+}
+
+hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, const double *lo,
+                                const double *den, const double *wy, int ry, double *tmp,
+                                hipStream_t st)
+{
+    GaussW ld{}, g{};
+    for (int c = 0; c < C; ++c) {
+        ld.w[2 * c] = lo[c];
+        ld.w[2 * c + 1] = den[c];
+    }
+    for (int k = 0; k < 2 * ry + 1; ++k) g.w[k] = wy[k];
+    dim3 grid((W + 63) / 64, H, T * C);
+    hipLaunchKernelGGL(k_prefilter_v<double>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, tmp);
+    return hipGetLastError();
+}
+
+hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, const double *wx,
+                              int rx, float *out, hipStream_t st)
+{
+    GaussW g{};
+    for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
+    dim3 grid((W + 63) / 64, H, T * C);
+    hipLaunchKernelGGL(k_prefilter_h, grid, dim3(64), 0, st, tmp, H, W, g, rx, out);
+    return hipGetLastError();
+}
+
+// ----------------------------------------------------------------- resize
+// util/resize_util.py:116-141: float32 products, float64 running sums,
+// float32 stores; horizontal pass then vertical pass.
+__global__ void k_resize_h(const float *__restrict__ src, int h, int w, float *__restrict__ tmp,
+                           int ow, const int32_t *__restrict__ ix, const float *__restrict__ wx,
+                           int P)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int n = blockIdx.z;
+    if (x >= ow) return;
+    const float *row = src + ((size_t)n * h + y) * w;
+    const int32_t *ii = ix + (size_t)x * P;
+    const float *ww = wx + (size_t)x * P;
+    double s = 0.0;
+    for (int p = 0; p < P; ++p) {
+        float prod = row[ii[p]] * ww[p];
+        s += (double)prod;
+    }
+    tmp[((size_t)n * h + y) * ow + x] = (float)s;
+}
+
+__global__ void k_resize_v(const float *__restrict__ tmp, int h, float *__restrict__ dst, int oh,
+                           int ow, const int32_t *__restrict__ iy, const float *__restrict__ wy,
+                           int P)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int n = blockIdx.z;
+    if (x >= ow) return;
+    const float *col = tmp + (size_t)n * h * ow + x;
+    const int32_t *ii = iy + (size_t)y * P;
+    const float *ww = wy + (size_t)y * P;
+    double s = 0.0;
+    for (int p = 0; p < P; ++p) {
+        float prod = col[(size_t)ii[p] * ow] * ww[p];
+        s += (double)prod;
+    }
+    dst[((size_t)n * oh + y) * ow + x] = (float)s;
+}
+
+hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow,
+                         const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
+                         int P, float *tmp, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_resize_h, dim3((ow + 63) / 64, h, N), dim3(64), 0, st, src, h, w, tmp, ow,
+                       ix, wx, P);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_resize_v, dim3((ow + 63) / 64, oh, N), dim3(64), 0, st, tmp, h, dst, oh,
+                       ow, iy, wy, P);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------------- warp
+// core/warping.py:160-234 with OpenCV remap INTER_CUBIC/BORDER_REPLICATE
+// semantics: float maps rounded to 1/32 px, float 1-D Keys coefficients
+// (A = -0.75), 4x4 weights as their products, float sums (interior: row by
+// row; near the border: the 16 taps in sequence on clamped indices).
+__device__ inline void cv_cubic_coeffs(float x, float *c)
+{
+    const float A = -0.75f;
+    c[0] = ((A * (x + 1) - 5 * A) * (x + 1) + 8 * A) * (x + 1) - 4 * A;
+    c[1] = ((A + 2) * x - (A + 3)) * x * x + 1;
+    c[2] = ((A + 2) * (1 - x) - (A + 3)) * (1 - x) * (1 - x) + 1;
+    c[3] = 1.f - c[0] - c[1] - c[2];
+}
+
+__global__ void k_warp(const float *__restrict__ f2, const float *__restrict__ f1,
+                       int f1_per_frame, int C, int H, int W, const float *__restrict__ u,
+                       const float *__restrict__ v, size_t uv_stride, double h_col, double h_row,
+                       float *__restrict__ out)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int t = blockIdx.z;
+    if (x >= W) return;
+    size_t HW = (size_t)H * W, p = (size_t)y * W + x;
+    const float *F2 = f2 + (size_t)t * C * HW;
+    const float *F1 = f1 + (f1_per_frame ? (size_t)t * C * HW : 0);
+    float *O = out + (size_t)t * C * HW;
+    float mx = (float)((double)x + (double)u[t * uv_stride + p] / h_col);
+    float my = (float)((double)y + (double)v[t * uv_stride + p] / h_row);
+    bool oob = (mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H);
+    if (oob) {
+        for (int c = 0; c < C; ++c) O[c * HW + p] = F1[c * HW + p];
+        return;
+    }
+    float cx = mx > (float)(W - 1) ? (float)(W - 1) : mx;
+    float cy = my > (float)(H - 1) ? (float)(H - 1) : my;
+    int X = (int)rintf(cx * 32.f), Y = (int)rintf(cy * 32.f);
+    int sx = (X >> 5) - 1, sy = (Y >> 5) - 1;
+    float wxv[4], wyv[4], w[16];
+    cv_cubic_coeffs((float)(X & 31) * (1.f / 32), wxv);
+    cv_cubic_coeffs((float)(Y & 31) * (1.f / 32), wyv);
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) w[i * 4 + j] = wyv[i] * wxv[j];
+    int width1 = W - 3 > 0 ? W - 3 : 0, height1 = H - 3 > 0 ? H - 3 : 0;
+    if ((unsigned)sx < (unsigned)width1 && (unsigned)sy < (unsigned)height1) {
+        for (int c = 0; c < C; ++c) {
+            const float *S = F2 + c * HW + (size_t)sy * W + sx;
+            float sum = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3];
+            S += W;
+            sum += S[0] * w[4] + S[1] * w[5] + S[2] * w[6] + S[3] * w[7];
+            S += W;
+            sum += S[0] * w[8] + S[1] * w[9] + S[2] * w[10] + S[3] * w[11];
+            S += W;
+            sum += S[0] * w[12] + S[1] * w[13] + S[2] * w[14] + S[3] * w[15];
+            O[c * HW + p] = sum;
+        }
+    } else {
+        int xs[4], ys[4];
+        for (int i = 0; i < 4; ++i) {
+            xs[i] = clampi(sx + i, 0, W - 1);
+            ys[i] = clampi(sy + i, 0, H - 1);
+        }
+        for (int c = 0; c < C; ++c) {
+            const float *S = F2 + c * HW;
+            float sum = 0.f;
+            for (int i = 0; i < 4; ++i)
+                for (int j = 0; j < 4; ++j) sum += S[(size_t)ys[i] * W + xs[j]] * w[i * 4 + j];
+            O[c * HW + p] = sum;
+        }
+    }
+}
+
+hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                       int W, const float *u, const float *v, size_t uv_frame_stride,
+                       double h_col, double h_row, float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_warp, dim3((W + 63) / 64, H, T), dim3(64), 0, st, f2, f1, f1_per_frame, C,
+                       H, W, u, v, uv_frame_stride, h_col, h_row, out);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------- motion tensor
+// core/optical_flow.py:73-160 (get_motion_tensor_gc).  Symmetric 1-px pads
+// are clamped reads; np.gradient is central in the interior; the re-padded
+// fx / ft ring copies the nearest interior value.  Products follow numpy's
+// left-to-right evaluation of the reference expressions.
+struct TensorOut {
+    double J11, J22, J33, J12, J13, J23;
+};
+
+__device__ inline double img(const float *f, int w, int r, int c) { return (double)f[(size_t)r * w + c]; }
+
+__device__ inline double tensor_fx(const float *f1, const float *f2, int h, int w, int r, int c,
+                                   double hx)
+{
+    int cl = c - 1 < 0 ? 0 : c - 1, cr = c + 1 > w - 1 ? w - 1 : c + 1;
+    double g1 = (img(f1, w, r, cr) - img(f1, w, r, cl)) / (2 * hx);
+    double g2 = (img(f2, w, r, cr) - img(f2, w, r, cl)) / (2 * hx);
+    return 0.5 * (g1 + g2);
+}
+
+__device__ inline double tensor_ft(const float *f1, const float *f2, int w, int r, int c)
+{
+    return img(f2, w, r, c) - img(f1, w, r, c);
+}
+
+__device__ inline TensorOut tensor_gc(const float *f1, const float *f2, int h, int w, int r, int c,
+                                      double hy, double hx)
+{
+    int rd = r + 1 > h - 1 ? h - 1 : r + 1, ru = r - 1 < 0 ? 0 : r - 1;
+    int cr = c + 1 > w - 1 ? w - 1 : c + 1, cl = c - 1 < 0 ? 0 : c - 1;
+    double fxy = (tensor_fx(f1, f2, h, w, rd, c, hx) - tensor_fx(f1, f2, h, w, ru, c, hx)) / (2 * hy);
+    double fxt = (tensor_ft(f1, f2, w, r, cr) - tensor_ft(f1, f2, w, r, cl)) / (2 * hx);
+    double fyt = (tensor_ft(f1, f2, w, rd, c) - tensor_ft(f1, f2, w, ru, c)) / (2 * hy);
+    double a0 = img(f1, w, r, c), al = img(f1, w, r, cl), ar = img(f1, w, r, cr);
+    double au = img(f1, w, ru, c), ad = img(f1, w, rd, c);
+    double b0 = img(f2, w, r, c), bl = img(f2, w, r, cl), br = img(f2, w, r, cr);
+    double bu = img(f2, w, ru, c), bd = img(f2, w, rd, c);
+    double fxx1 = (al - 2 * a0 + ar) / (hx * hx);
+    double fyy1 = (au - 2 * a0 + ad) / (hy * hy);
+    double fxx2 = (bl - 2 * b0 + br) / (hx * hx);
+    double fyy2 = (bu - 2 * b0 + bd) / (hy * hy);
+    double fxx = 0.5 * (fxx1 + fxx2), fyy = 0.5 * (fyy1 + fyy2);
+    double sx = sqrt(fxx * fxx + fxy * fxy), sy = sqrt(fxy * fxy + fyy * fyy);
+    double regx = 1 / (sx * sx + 1e-6), regy = 1 / (sy * sy + 1e-6);
+    TensorOut o;
+    o.J11 = regx * (fxx * fxx) + regy * (fxy * fxy);
+    o.J22 = regx * (fxy * fxy) + regy * (fyy * fyy);
+    o.J33 = regx * (fxt * fxt) + regy * (fyt * fyt);
+    o.J12 = regx * fxx * fxy + regy * fxy * fyy;
+    o.J13 = regx * fxx * fxt + regy * fxy * fyt;
+    o.J23 = regx * fxy * fxt + regy * fyy * fyt;
+    return o;
+}
+
+// cu = sum_nb w_nb (u_nb - u_c) over (left, right, down, up); u is
+// edge-padded, so neighbours outside the level contribute exactly 0.
+__device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, double ax,
+                                    double ay)
+{
+    double uc = img(u, w, r, c);
+    double ul = img(u, w, r, c > 0 ? c - 1 : c), ur = img(u, w, r, c < w - 1 ? c + 1 : c);
+    double ud = img(u, w, r < h - 1 ? r + 1 : r, c), uu = img(u, w, r > 0 ? r - 1 : r, c);
+    double s = ax * (ul - uc);
+    s = s + ax * (ur - uc);
+    s = s + ay * (ud - uc);
+    s = s + ay * (uu - uc);
+    return s;
+}
+
+// One thread per skewed slot (d, r); grid (h + w - 1, T), block hs.
+// J planes (component-major, then channel) and cu, cv are written skewed.
+__global__ void k_tensor(const float *__restrict__ f1, const float *__restrict__ f2, int C, int h,
+                         int w, int hs, size_t plane, double hy, double hx,
+                         const float *__restrict__ u, const float *__restrict__ v,
+                         size_t uv_stride, double ax, double ay, double *__restrict__ J,
+                         double *__restrict__ cuv, size_t fstride)
+{
+    int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
+    int c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    size_t hw = (size_t)h * w, q = (size_t)d * hs + r;
+    double *Jt = J + (size_t)t * fstride;
+    for (int ch = 0; ch < C; ++ch) {
+        TensorOut o = tensor_gc(f1 + ch * hw, f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx);
+        Jt[(size_t)(0 * C + ch) * plane + q] = o.J11;
+        Jt[(size_t)(1 * C + ch) * plane + q] = o.J22;
+        Jt[(size_t)(2 * C + ch) * plane + q] = o.J33;
+        Jt[(size_t)(3 * C + ch) * plane + q] = o.J12;
+        Jt[(size_t)(4 * C + ch) * plane + q] = o.J13;
+        Jt[(size_t)(5 * C + ch) * plane + q] = o.J23;
+    }
+    double *ct = cuv + (size_t)t * fstride;
+    ct[q] = smooth_rhs(u + t * uv_stride, h, w, r, c, ax, ay);
+    ct[plane + q] = smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay);
+}
+
+hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
+                         double hx, const float *u, const float *v, size_t uv_frame_stride,
+                         double ax, double ay, double *J, double *cuv, size_t frame_stride,
+                         const Skew &s, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tensor, dim3(h + w - 1, T), dim3(s.hs), 0, st, f1, f2, C, h, w, s.hs,
+                       s.plane, hy, hx, u, v, uv_frame_stride, ax, ay, J, cuv, frame_stride);
+    return hipGetLastError();
+}
+
+// Row-major tensor for the stand-alone get_motion_tensor_gc entry point:
+// (h+2, w+2) outputs with a zero ring.
+__global__ void k_tensor_rowmajor(const float *__restrict__ f1, const float *__restrict__ f2,
+                                  int h, int w, double hy, double hx, double *J11, double *J22,
+                                  double *J33, double *J12, double *J13, double *J23)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    int wp = w + 2;
+    if (i >= wp) return;
+    size_t q = (size_t)j * wp + i;
+    if (j == 0 || j == h + 1 || i == 0 || i == w + 1) {
+        J11[q] = J22[q] = J33[q] = J12[q] = J13[q] = J23[q] = 0.0;
+        return;
+    }
+    TensorOut o = tensor_gc(f1, f2, h, w, j - 1, i - 1, hy, hx);
+    J11[q] = o.J11;
+    J22[q] = o.J22;
+    J33[q] = o.J33;
+    J12[q] = o.J12;
+    J13[q] = o.J13;
+    J23[q] = o.J23;
+}
+
+hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w, double hy,
+                                  double hx, double *J11, double *J22, double *J33, double *J12,
+                                  double *J13, double *J23, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_tensor_rowmajor, dim3((w + 2 + 63) / 64, h + 2), dim3(64), 0, st, f1, f2,
+                       h, w, hy, hx, J11, J22, J33, J12, J13, J23);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------ coefficient assembly
+// Lag-block coefficients of core/level_solver.py:229-255 + :262-351 for one
+// cell (ordering documented with the oracle's assemble_pixel):
+//   Du = (((wl+wr)+wd)+wu) + sum_k Wpsi_k J11_k,  Dv likewise with J22,
+//   A12 = sum_k Wpsi_k J12_k, Bu = cu - sum_k Wpsi_k J13_k, Bv = cv - ...J23.
+struct AData {
+    double a[8];
+    double am1[8];
+};
+
+__global__ void k_assemble(const double *__restrict__ J, const double *__restrict__ Wsk,
+                           const double *__restrict__ cuv, const double *__restrict__ duv,
+                           double *__restrict__ coef, size_t fstride, int C, int h, int w,
+                           int hs, size_t plane, double ax, double ay, AData ad, int psi_one,
+                           int zero_init)
+{
+    int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
+    int c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    size_t q = (size_t)d * hs + r;
+    const double *Jt = J + (size_t)t * fstride;
+    const double *ct = cuv + (size_t)t * fstride;
+    double x = 0.0, y = 0.0;
+    if (!psi_one && !zero_init) {
+        x = duv[(size_t)t * fstride + q];
+        y = duv[(size_t)t * fstride + plane + q];
+    }
+    double ws = ((ax + ax) + ay) + ay;
+    double Du = ws, Dv = ws, A12 = 0.0, Bu = ct[q], Bv = ct[plane + q];
+    for (int k = 0; k < C; ++k) {
+        double j11 = Jt[(size_t)(0 * C + k) * plane + q], j22 = Jt[(size_t)(1 * C + k) * plane + q];
+        double j33 = Jt[(size_t)(2 * C + k) * plane + q], j12 = Jt[(size_t)(3 * C + k) * plane + q];
+        double j13 = Jt[(size_t)(4 * C + k) * plane + q], j23 = Jt[(size_t)(5 * C + k) * plane + q];
+        double psi = 1.0;
+        if (!psi_one) {
+            double val = j11 * x * x + j22 * y * y + j23 * y + 2 * j12 * x * y + 2 * j13 * x +
+                         j23 * y + j33;
+            if (val < 0) val = 0;
+            psi = ad.a[k] * frh_det_pow(val + 1e-5, ad.am1[k]);
+        }
+        double wp = Wsk[(size_t)k * plane + q] * psi;
+        Du = Du + wp * j11;
+        Dv = Dv + wp * j22;
+        A12 = A12 + wp * j12;
+        Bu = Bu - wp * j13;
+        Bv = Bv - wp * j23;
+    }
+    double *co = coef + (size_t)t * fstride;
+    co[q] = Du;
+    co[plane + q] = Dv;
+    co[2 * plane + q] = A12;
+    co[3 * plane + q] = Bu;
+    co[4 * plane + q] = Bv;
+}
+
+hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
+                           const double *duv, double *coef, size_t frame_stride, int T, int C,
+                           const Skew &s, double ax, double ay, const double *a_data_minus1,
+                           const double *a_data, int psi_one, int zero_init, hipStream_t st)
+{
+    AData ad{};
+    for (int k = 0; k < C && k < 8; ++k) {
+        ad.a[k] = a_data[k];
+        ad.am1[k] = a_data_minus1[k];
+    }
+    hipLaunchKernelGGL(k_assemble, dim3(s.h + s.w - 1, T), dim3(s.hs), 0, st, J, Wsk, cuv, duv,
+                       coef, frame_stride, C, s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one,
+                       zero_init);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- SOR sweeps
+// core/level_solver.py:257-364: P consecutive sweeps of the column-major
+// lexicographic SOR for one frame per work-group, lane r = interior row r.
+// Cell (r, c) of sweep p is updated at wavefront step t = r + c + 2p: its
+// up and left neighbours already hold sweep-p values and its down and right
+// neighbours still hold sweep-(p-1) values, exactly as in the sequential
+// column-by-column loop, so the result is identical to that loop.
+// Neighbour values travel between lanes with wave shuffles and between
+// waves through LDS with one barrier per step.  Out-of-level neighbours take
+// the cell's own value from the start of the sweep (set_boundary_2d).
+template <int P>
+__device__ inline void sor_cell(double Du, double Dv, double A12, double Bu, double Bv, double ax,
+                                double ay, double lu, double ru, double du_, double uu,
+                                double lv, double rv, double dv_, double uv, double ou, double ov,
+                                double &nu_out, double &nv_out)
+{
+    const double omega = 1.95;
+    double nu = ((ax * lu + ax * ru) + ay * du_) + ay * uu;
+    double num_u = (Bu + nu) - A12 * ov;
+    double duk = Du != 0 ? num_u / Du : 0.0;
+    double dun = (1 - omega) * ou + omega * duk;
+    double nv = ((ax * lv + ax * rv) + ay * dv_) + ay * uv;
+    double num_v = (Bv + nv) - A12 * dun;
+    double dvk = Dv != 0 ? num_v / Dv : 0.0;
+    nu_out = dun;
+    nv_out = (1 - omega) * ov + omega * dvk;
+}
+
+template <int P, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sweep(const double *__restrict__ coef,
+                                                double *__restrict__ duv,
+                                                double *__restrict__ ring, size_t fstride, int h,
+                                                int w, int hs, size_t plane, double ax, double ay,
+                                                int zero_init)
+{
+    const int f = blockIdx.x;
+    const int r = threadIdx.x;
+    const int lane = r & 63, wv = r >> 6, nw = blockDim.x >> 6;
+    const double *cb = coef + (size_t)f * fstride;
+    double *du = duv + (size_t)f * fstride;
+    double *dv = du + plane;
+    __shared__ double xb[2][MAXT / 64][2][P + 1][2];
+    for (int k = threadIdx.x; k < 2 * (MAXT / 64) * 2 * (P + 1) * 2; k += blockDim.x)
+        (&xb[0][0][0][0][0])[k] = 0.0;
+    const int D = h + w - 2;
+    const bool row_ok = r < h;
+    auto cell_ok = [&](int dd) { int cc = dd - r; return row_ok && cc >= 0 && cc < w; };
+
+    double K[2 * P - 1][5];
+    double pu[P], pv[P], qu[P], qv[P];
+#pragma unroll
+    for (int k = 0; k < 2 * P - 1; ++k)
+#pragma unroll
+        for (int e = 0; e < 5; ++e) K[k][e] = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) pu[p] = pv[p] = qu[p] = qv[p] = 0.0;
+    double nK[5] = {0, 0, 0, 0, 0};
+    if (cell_ok(0)) {
+        size_t q = (size_t)r;
+#pragma unroll
+        for (int e = 0; e < 5; ++e) nK[e] = cb[e * plane + q];
+    }
+    double lpu = 0, lpv = 0, ldu = 0, ldv = 0;
+    if (!zero_init && cell_ok(0)) { lpu = du[r]; lpv = dv[r]; }
+    if (!zero_init && cell_ok(1)) { ldu = du[(size_t)hs + r]; ldv = dv[(size_t)hs + r]; }
+    __syncthreads();
+
+    const int Tend = D + 2 * (P - 1);
+    for (int t = 0; t <= Tend; ++t) {
+#pragma unroll
+        for (int k = 2 * P - 2; k >= 1; --k)
+#pragma unroll
+            for (int e = 0; e < 5; ++e) K[k][e] = K[k - 1][e];
+#pragma unroll
+        for (int e = 0; e < 5; ++e) K[0][e] = nK[e];
+        if (cell_ok(t + 1)) {
+            size_t q = (size_t)(t + 1) * hs + r;
+#pragma unroll
+            for (int e = 0; e < 5; ++e) nK[e] = cb[e * plane + q];
+        }
+        double nlu = 0, nlv = 0;
+        if (!zero_init && cell_ok(t + 2)) {
+            size_t q = (size_t)(t + 2) * hs + r;
+            nlu = du[q];
+            nlv = dv[q];
+        }
+        double upu[P], upv[P], dnu[P], dnv[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            upu[p] = __shfl_up(pu[p], 1);
+            upv[p] = __shfl_up(pv[p], 1);
+            double su = p == 0 ? ldu : pu[p > 0 ? p - 1 : 0];
+            double sv = p == 0 ? ldv : pv[p > 0 ? p - 1 : 0];
+            dnu[p] = __shfl_down(su, 1);
+            dnv[p] = __shfl_down(sv, 1);
+        }
+        const int pb = (t + 1) & 1;
+        if (lane == 0 && wv > 0) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                upu[p] = xb[pb][wv - 1][1][p][0];
+                upv[p] = xb[pb][wv - 1][1][p][1];
+            }
+        }
+        if (lane == 63 && wv < nw - 1) {
+            dnu[0] = xb[pb][wv + 1][0][P][0];
+            dnv[0] = xb[pb][wv + 1][0][P][1];
+#pragma unroll
+            for (int p = 1; p < P; ++p) {
+                dnu[p] = xb[pb][wv + 1][0][p - 1][0];
+                dnv[p] = xb[pb][wv + 1][0][p - 1][1];
+            }
+        }
+        double cu[P], cv[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            int c = t - r - 2 * p;
+            cu[p] = 0.0;
+            cv[p] = 0.0;
+            if (row_ok && c >= 0 && c < w) {
+                double ou = p == 0 ? lpu : qu[p > 0 ? p - 1 : 0];
+                double ov = p == 0 ? lpv : qv[p > 0 ? p - 1 : 0];
+                double rsu = p == 0 ? ldu : pu[p > 0 ? p - 1 : 0];
+                double rsv = p == 0 ? ldv : pv[p > 0 ? p - 1 : 0];
+                double lu = c > 0 ? pu[p] : ou, lv = c > 0 ? pv[p] : ov;
+                double ru = c < w - 1 ? rsu : ou, rv = c < w - 1 ? rsv : ov;
+                double uu = r > 0 ? upu[p] : ou, uv = r > 0 ? upv[p] : ov;
+                double dd = r < h - 1 ? dnu[p] : ou, ddv = r < h - 1 ? dnv[p] : ov;
+                sor_cell<P>(K[2 * p][0], K[2 * p][1], K[2 * p][2], K[2 * p][3], K[2 * p][4], ax,
+                            ay, lu, ru, dd, uu, lv, rv, ddv, uv, ou, ov, cu[p], cv[p]);
+                if (p == P - 1) {
+                    size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
+                    du[q] = cu[p];
+                    dv[q] = cv[p];
+                    if (ring && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
+                        double *rb = ring + (size_t)f * fstride;
+                        rb[q] = ou;
+                        rb[plane + q] = ov;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            qu[p] = pu[p];
+            qv[p] = pv[p];
+            pu[p] = cu[p];
+            pv[p] = cv[p];
+        }
+        lpu = ldu;
+        lpv = ldv;
+        ldu = nlu;
+        ldv = nlv;
+        const int wb = t & 1;
+        if (lane == 0) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                xb[wb][wv][0][p][0] = pu[p];
+                xb[wb][wv][0][p][1] = pv[p];
+            }
+            xb[wb][wv][0][P][0] = ldu;
+            xb[wb][wv][0][P][1] = ldv;
+        }
+        if (lane == 63) {
+#pragma unroll
+            for (int p = 0; p < P; ++p) {
+                xb[wb][wv][1][p][0] = pu[p];
+                xb[wb][wv][1][p][1] = pv[p];
+            }
+        }
+        __syncthreads();
+    }
+}
+
+template <int P>
+static hipError_t launch_sweep_p(const double *coef, double *duv, double *ring, size_t fstride,
+                                 int T, const Skew &s, double ax, double ay, int zero_init,
+                                 hipStream_t st)
+{
+    int threads = s.hs;
+    if (threads <= 512)
+        hipLaunchKernelGGL((k_sweep<P, 512>), dim3(T), dim3(threads), 0, st, coef, duv, ring,
+                           fstride, s.h, s.w, s.hs, s.plane, ax, ay, zero_init);
+    else if (P <= 2)
+        hipLaunchKernelGGL((k_sweep<P <= 2 ? P : 2, 1024>), dim3(T), dim3(threads), 0, st, coef,
+                           duv, ring, fstride, s.h, s.w, s.hs, s.plane, ax, ay, zero_init);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
+                        int T, const Skew &s, double ax, double ay, int zero_init, hipStream_t st)
+{
+    switch (P) {
+    case 1: return launch_sweep_p<1>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
+    case 2: return launch_sweep_p<2>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
+    case 3: return launch_sweep_p<3>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
+    case 4: return launch_sweep_p<4>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
+    case 5: return launch_sweep_p<5>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+// ------------------------------------------------------- median + u += du
+// scipy.ndimage.median_filter(size=(5,5), mode="mirror") (core/optical_flow
+// .py:607-609), then u = u + du (:610-611).  The median is the 13th element
+// of Batcher's odd-even merge sort of the 25-sample window.
+__device__ inline void cswap(double &a, double &b)
+{
+    double lo = a < b ? a : b, hi = a < b ? b : a;
+    a = lo;
+    b = hi;
+}
+
+__device__ inline double median25(double (&a)[25])
+{
+#pragma unroll
+    for (int p = 1; p < 25; p += p)
+#pragma unroll
+        for (int k = p; k > 0; k /= 2)
+#pragma unroll
+            for (int j = k % p; j + k < 25; j += 2 * k)
+#pragma unroll
+                for (int i = 0; i < k; ++i)
+                    if (i + j + k < 25 && (i + j) / (2 * p) == (i + j + k) / (2 * p))
+                        cswap(a[i + j], a[i + j + k]);
+    return a[12];
+}
+
+__global__ void k_median_add(const double *__restrict__ duv, size_t fstride, int h, int w,
+                             int hs, size_t plane, int do_median, const float *__restrict__ u,
+                             const float *__restrict__ v, size_t uv_stride, float *out_f32,
+                             double *out_f64, int out_interleaved)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
+    if (c >= w) return;
+    const double *du = duv + (size_t)t * fstride, *dv = du + plane;
+    double mu, mv;
+    if (do_median) {
+        double a[25], b[25];
+        int k = 0;
+        for (int dy = -2; dy <= 2; ++dy) {
+            int rr = mirror_edge(r + dy, h);
+            for (int dx = -2; dx <= 2; ++dx) {
+                int cc = mirror_edge(c + dx, w);
+                size_t q = (size_t)(rr + cc) * hs + rr;
+                a[k] = du[q];
+                b[k] = dv[q];
+                ++k;
+            }
+        }
+        mu = median25(a);
+        mv = median25(b);
+    } else {
+        size_t q = (size_t)(r + c) * hs + r;
+        mu = du[q];
+        mv = dv[q];
+    }
+    size_t p = (size_t)r * w + c;
+    double un = (double)u[t * uv_stride + p] + mu;
+    double vn = (double)v[t * uv_stride + p] + mv;
+    if (out_interleaved) {
+        size_t o = ((size_t)t * h * w + p) * 2;
+        if (out_f64) {
+            out_f64[o] = un;
+            out_f64[o + 1] = vn;
+        } else {
+            out_f32[o] = (float)un;
+            out_f32[o + 1] = (float)vn;
+        }
+    } else {
+        out_f32[t * uv_stride + p] = (float)un;
+        out_f32[t * uv_stride + (size_t)h * w + p] = (float)vn;
+    }
+}
+
+hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew &s, int T,
+                             int do_median, const float *u, const float *v,
+                             size_t uv_frame_stride, float *out_f32, double *out_f64,
+                             int out_interleaved, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_median_add, dim3((s.w + 63) / 64, s.h, T), dim3(64), 0, st, duv,
+                       frame_stride, s.h, s.w, s.hs, s.plane, do_median, u, v, uv_frame_stride,
+                       out_f32, out_f64, out_interleaved);
+    return hipGetLastError();
+}
+
+// Median of row-major planes (stand-alone entry point), via a copy.
+__global__ void k_median_plane(const double *__restrict__ src, double *__restrict__ dst, int h,
+                               int w)
+{
+    int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, n = blockIdx.z;
+    if (c >= w) return;
+    const double *f = src + (size_t)n * h * w;
+    double a[25];
+    int k = 0;
+    for (int dy = -2; dy <= 2; ++dy)
+        for (int dx = -2; dx <= 2; ++dx)
+            a[k++] = f[(size_t)mirror_edge(r + dy, h) * w + mirror_edge(c + dx, w)];
+    dst[((size_t)n * h + r) * w + c] = median25(a);
+}
+
+hipError_t launch_median_plane(double *f, int N, int h, int w, double *tmp, hipStream_t st)
+{
+    hipError_t e = hipMemcpyAsync(tmp, f, sizeof(double) * (size_t)N * h * w,
+                                  hipMemcpyDeviceToDevice, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_median_plane, dim3((w + 63) / 64, h, N), dim3(64), 0, st, tmp, f, h, w);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- helpers
+__global__ void k_fill_f64(double *p, size_t n, double v)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = v;
+}
+
+hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st)
+{
+    size_t blocks = (n + 255) / 256;
+    if (blocks > 65536) blocks = 65536;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+__global__ void k_broadcast_f32(const float *src, size_t n, float *dst, size_t dst_stride)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int t = blockIdx.y;
+    if (i < n) dst[t * dst_stride + i] = src[i];
+}
+
+hipError_t launch_broadcast_f32(const float *src, size_t n, float *dst, size_t dst_stride, int T,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(k_broadcast_f32, dim3((unsigned)((n + 255) / 256), T), dim3(256), 0, st, src,
+                       n, dst, dst_stride);
+    return hipGetLastError();
+}
+
+// Row-major float32 planes (N, h, w) -> skewed float64 planes.
+__global__ void k_to_skew_f32(const float *__restrict__ src, int h, int w, int hs, size_t plane,
+                              double *__restrict__ dst)
+{
+    int d = blockIdx.x, r = threadIdx.x, n = blockIdx.y;
+    int c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    dst[(size_t)n * plane + (size_t)d * hs + r] = (double)src[((size_t)n * h + r) * w + c];
+}
+
+hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_skew_f32, dim3(s.h + s.w - 1, N), dim3(s.hs), 0, st, src, s.h, s.w,
+                       s.hs, s.plane, dst);
+    return hipGetLastError();
+}
+
+// (m, n, K) interleaved padded float64 -> K skewed planes of the interior.
+__global__ void k_padded_to_skew_f64(const double *__restrict__ src, int m, int n, int K, int hs,
+                                     size_t plane, double *__restrict__ dst)
+{
+    int d = blockIdx.x, r = threadIdx.x, k = blockIdx.y;
+    int h = m - 2, w = n - 2, c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    dst[(size_t)k * plane + (size_t)d * hs + r] = src[((size_t)(r + 1) * n + (c + 1)) * K + k];
+}
+
+hipError_t launch_padded_to_skew_f64(const double *src, int m, int n, int K, const Skew &s,
+                                     double *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_padded_to_skew_f64, dim3(s.h + s.w - 1, K), dim3(s.hs), 0, st, src, m, n,
+                       K, s.hs, s.plane, dst);
+    return hipGetLastError();
+}
+
+// Skewed du/dv -> padded (m, n) row-major; the ring is the reference's last
+// set_boundary_2d copy (rows first, then columns, so corners take the
+// diagonal interior neighbour), taken from the start-of-last-sweep values.
+__global__ void k_skew_to_padded_f64(const double *__restrict__ duv,
+                                     const double *__restrict__ ring, int h, int w, int hs,
+                                     size_t plane, double *du, double *dv)
+{
+    int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
+    int n = w + 2;
+    if (i >= n) return;
+    bool border = (j == 0 || j == h + 1 || i == 0 || i == w + 1);
+    int r = clampi(j - 1, 0, h - 1), c = clampi(i - 1, 0, w - 1);
+    size_t q = (size_t)(r + c) * hs + r;
+    const double *src = border ? ring : duv;
+    du[(size_t)j * n + i] = src[q];
+    dv[(size_t)j * n + i] = src[plane + q];
+}
+
+hipError_t launch_skew_to_padded_f64(const double *duv, const double *ring, const Skew &s,
+                                     double *du, double *dv, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_skew_to_padded_f64, dim3((s.w + 2 + 63) / 64, s.h + 2), dim3(64), 0, st,
+                       duv, ring, s.h, s.w, s.hs, s.plane, du, dv);
+    return hipGetLastError();
+}
+
+// cu, cv from padded float64 u, v (stand-alone level solver entry point):
+// same sum as smooth_rhs, reading the reference's edge-padded arrays.
+__global__ void k_cuv_padded(const double *__restrict__ u, const double *__restrict__ v, int m,
+                             int n, double ax, double ay, int hs, size_t plane,
+                             double *__restrict__ cuv)
+{
+    int d = blockIdx.x, r = threadIdx.x;
+    int h = m - 2, w = n - 2, c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    size_t cc = (size_t)(r + 1) * n + (c + 1);
+    size_t nb[4] = {cc - 1, cc + 1, cc + n, cc - n};
+    double wgt[4] = {ax, ax, ay, ay};
+    double su = wgt[0] * (u[nb[0]] - u[cc]), sv = wgt[0] * (v[nb[0]] - v[cc]);
+    for (int k = 1; k < 4; ++k) {
+        su = su + wgt[k] * (u[nb[k]] - u[cc]);
+        sv = sv + wgt[k] * (v[nb[k]] - v[cc]);
+    }
+    size_t q = (size_t)d * hs + r;
+    cuv[q] = su;
+    cuv[plane + q] = sv;
+}
+
+hipError_t launch_cuv_padded(const double *u, const double *v, int m, int n, double ax,
+                             double ay, const Skew &s, double *cuv, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_cuv_padded, dim3(s.h + s.w - 1), dim3(s.hs), 0, st, u, v, m, n, ax, ay,
+                       s.hs, s.plane, cuv);
+    return hipGetLastError();
+}
+
+}  // namespace frh
