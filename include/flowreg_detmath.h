@@ -11,7 +11,7 @@
  *
  * det_pow:  x > 0 finite.  log via the atanh series on m in [1/sqrt2, sqrt2),
  *           exp via range reduction by ln2 and a degree-17 Taylor polynomial;
- *           relative error a few 1e-16 over the solver's range
+ *           relative error below 4e-15 over the solver's range
  *           (x in [1e-5, 1e4], |b| <= 1).
  * det_powf: float version (series truncated for float precision).
  */

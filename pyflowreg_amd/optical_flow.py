@@ -1,0 +1,444 @@
+"""Drop-in ``get_displacement`` for pyflowreg on MI355X.
+
+Mirrors ``core/optical_flow.py`` of pyflowreg: the same function names,
+positional/keyword arguments, defaults and error behaviour, with every
+numeric stage executed by libflowreg_hip.so on the GPU.  The host side only
+plans the pyramid (level sizes, grid spacings, alpha scaling and the resize
+tap tables, computed exactly as the reference computes them) and moves
+arrays.
+
+Entry points
+  get_displacement(fixed, moving, **kw)         core/optical_flow.py:773-929
+  level_solver(J11, ..., hx, hy, gnc_beta)       core/optical_flow.py:348-411
+  get_motion_tensor_gc(f1, f2, hy, hx)           core/optical_flow.py:73-160
+  imregister_wrapper(f2, u, v, f1, method)       core/warping.py:160-234
+  imresize_fused_gauss_cubic(img, size)          util/resize_util.py:232-254
+  warpingDepth(eta, levels, m, n)                core/warping.py:237-277
+  DisplacementSolver                             batched form of get_displacement
+"""
+
+from __future__ import annotations
+
+import functools
+from dataclasses import dataclass
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_handle
+
+_SUPPORTED_DATA_TERMS = {
+    "gc": "gc", "gradient": "gc",
+    "gray": "gray", "brightness": "gray",
+    "cs": "cs", "census": "cs",
+}
+
+
+def _device() -> torch.device:
+    if not torch.cuda.is_available():
+        raise _lib.FlowregHipError("flowreg_hip needs a ROCm GPU; none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+# ------------------------------------------------------------------ geometry
+def warpingDepth(eta, levels, m, n):  # noqa: N802  (reference name)
+    """core/warping.py:237-277."""
+    return int(lib().frh_warping_depth(float(eta), int(levels), int(m), int(n)))
+
+
+def level_plan(m: int, n: int, eta: float, levels: int, min_level: int):
+    """core/optical_flow.py:497-518: [(i, (h, w)), ...] from coarse to fine
+    and the effective min_level."""
+    mly = warpingDepth(eta, levels, m, m)
+    mlx = warpingDepth(eta, levels, n, n)
+    max_level = min(mlx, mly) * 4
+    mly = min(mly, max_level)
+    mlx = min(mlx, max_level)
+    top = max(mlx, mly)
+    if top <= min_level:
+        min_level = top - 1
+    if min_level < 0:
+        min_level = 0
+    plan = [(i, (int(round(m * eta ** min(i, mly))), int(round(n * eta ** min(i, mlx)))))
+            for i in range(top, min_level - 1, -1)]
+    return plan, min_level
+
+
+def _gauss_taps(sigma: float):
+    """util/resize_util.py:216-225 with numpy float32 arithmetic."""
+    if sigma <= 0.0:
+        return 0, np.array([1.0], dtype=np.float32)
+    R = int(np.ceil(2.0 * sigma))
+    x = np.arange(-R, R + 1, dtype=np.float32)
+    g = np.exp(-0.5 * (x / sigma) ** 2).astype(np.float32)
+    g /= g.sum()
+    return R, np.ascontiguousarray(g)
+
+
+@functools.lru_cache(maxsize=512)
+def resize_tables(in_len: int, out_len: int, sigma: float):
+    """Tap indices (int32) and weights (float32), shape (out_len, 2R+4)."""
+    R, g = _gauss_taps(sigma)
+    P = 2 * R + 4
+    idx = np.empty((out_len, P), np.int32)
+    wt = np.empty((out_len, P), np.float32)
+    check(lib().frh_fill_resize_tables(in_len, out_len, R, ptr(g), ptr(idx), ptr(wt)),
+          "frh_fill_resize_tables")
+    idx.flags.writeable = False
+    wt.flags.writeable = False
+    return idx, wt
+
+
+def _resize_sigma(ih, iw, oh, ow, sigma_coeff=0.6):
+    """util/resize_util.py:235-242 (per_axis=False)."""
+    sy = oh / ih
+    sx = ow / iw
+    s = sy if sy < sx else sx
+    return (sigma_coeff / s) if s < 1.0 else 0.0
+
+
+@dataclass
+class PyramidPlan:
+    H: int
+    W: int
+    levels: list          # [(i, h, w, h_row, h_col, alpha_x, alpha_y)]
+    min_level: int
+    img_tables: list      # per level: (ix, wx, iy, wy)
+    up_tables: list       # per level: None at top, else (ix, wx, iy, wy)
+
+
+@functools.lru_cache(maxsize=64)
+def pyramid_plan(H, W, eta, levels, min_level, alpha):
+    plan, min_level = level_plan(H, W, eta, levels, min_level)
+    lv, img, up = [], [], []
+    prev = None
+    for i, (h, w) in plan:
+        h_row = float(H) / h
+        h_col = float(W) / w
+        alpha_scaling = 1 if i == min_level else eta ** (-0.5 * i)
+        ax, ay = [alpha_scaling * alpha[j] for j in range(len(alpha))][:2]
+        lv.append((i, h, w, h_row, h_col, ax, ay))
+        sig = _resize_sigma(H, W, h, w)
+        ix, wx = resize_tables(W, w, sig)
+        iy, wy = resize_tables(H, h, sig)
+        img.append((ix, wx, iy, wy))
+        if prev is None:
+            up.append(None)
+        else:
+            ph, pw = prev
+            sig_u = _resize_sigma(ph, pw, h, w)
+            ux, uwx = resize_tables(pw, w, sig_u)
+            uy, uwy = resize_tables(ph, h, sig_u)
+            up.append((ux, uwx, uy, uwy))
+        prev = (h, w)
+    return PyramidPlan(H, W, lv, min_level, img, up)
+
+
+# ---------------------------------------------------------- argument helpers
+def _resolve_const_assumption(const_assumption) -> str:
+    """core/optical_flow.py:446-473."""
+    if hasattr(const_assumption, "value"):
+        const_assumption = const_assumption.value
+    key = str(const_assumption).strip().lower()
+    if key not in _SUPPORTED_DATA_TERMS:
+        supported = "', '".join(sorted(_SUPPORTED_DATA_TERMS))
+        raise ValueError(f"Unknown constancy assumption: '{const_assumption}'. "
+                         f"Supported values are: '{supported}'.")
+    return _SUPPORTED_DATA_TERMS[key]
+
+
+def expand_weight(weight, m, n, C) -> np.ndarray:
+    """core/optical_flow.py:856-879: (m, n, C) float64 weight map."""
+    if weight is None:
+        return np.ones((m, n, C), dtype=np.float64) / C
+    weight = np.asarray(weight).astype(np.float64)
+    if weight.ndim < 3:
+        if weight.ndim == 1:
+            if len(weight) < C:
+                we = np.full(C, 1.0 / C, dtype=np.float64)
+                we[: len(weight)] = weight
+                weight = we
+            elif len(weight) > C:
+                weight = weight[:C]
+            weight = weight / weight.sum()
+            weight = np.ones((m, n, C), dtype=np.float64) * weight.reshape(1, 1, -1)
+        else:
+            weight = np.ones((m, n, C), dtype=np.float64) * weight[..., np.newaxis]
+    return weight
+
+
+def _as_planes(a, dtype=torch.float32) -> torch.Tensor:
+    """(H, W[, C]) or (T, H, W, C) array/tensor -> contiguous (.., C, H, W)
+    device planes."""
+    dev = _device()
+    t = a if isinstance(a, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(a))
+    t = t.to(device=dev, dtype=dtype)
+    if t.ndim == 2:
+        t = t[None]
+    elif t.ndim == 3:
+        t = t.permute(2, 0, 1)
+    else:
+        t = t.permute(0, 3, 1, 2)
+    return t.contiguous()
+
+
+# ------------------------------------------------------------ batched solver
+class DisplacementSolver:
+    """Device-resident form of ``get_displacement`` for a batch of moving
+    frames that share one reference, weight and initial flow (the situation
+    of one BatchMotionCorrector batch).  Wraps frh_solver_*."""
+
+    def __init__(self, H, W, C, *, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
+                 levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, max_batch=1):
+        if np.isscalar(alpha):
+            alpha = (float(alpha), float(alpha))
+        alpha = tuple(float(a) for a in alpha)
+        self.H, self.W, self.C = int(H), int(W), int(C)
+        self.max_batch = int(max_batch)
+        self.iterations, self.update_lag = int(iterations), int(update_lag)
+        self.plan = pyramid_plan(self.H, self.W, float(eta), int(levels), int(min_level), alpha)
+        if self.plan.min_level > 0:
+            raise ValueError("flowreg_hip: min_level > 0 (final cv2.resize) is not implemented")
+        if float(a_smooth) != 1.0:
+            raise ValueError("flowreg_hip: a_smooth != 1 is not implemented")
+        ad = np.broadcast_to(np.asarray(a_data, dtype=np.float64), (self.C,))
+        self._a_data = np.ascontiguousarray(ad)
+        keep = []
+        descs = (_lib.LevelDesc * len(self.plan.levels))()
+        for k, (i, h, w, h_row, h_col, ax, ay) in enumerate(self.plan.levels):
+            d = descs[k]
+            d.h, d.w, d.h_row, d.h_col, d.alpha_x, d.alpha_y = h, w, h_row, h_col, ax, ay
+            ix, wx, iy, wy = self.plan.img_tables[k]
+            d.P_img = ix.shape[1]
+            d.img_ix, d.img_wx, d.img_iy, d.img_wy = ptr(ix), ptr(wx), ptr(iy), ptr(wy)
+            keep += [ix, wx, iy, wy]
+            if self.plan.up_tables[k] is not None:
+                ux, uwx, uy, uwy = self.plan.up_tables[k]
+                d.P_up = ux.shape[1]
+                d.up_ix, d.up_wx, d.up_iy, d.up_wy = ptr(ux), ptr(uwx), ptr(uy), ptr(uwy)
+                keep += [ux, uwx, uy, uwy]
+        sd = _lib.SolverDesc()
+        sd.H, sd.W, sd.C = self.H, self.W, self.C
+        sd.n_levels = len(self.plan.levels)
+        sd.levels = descs
+        sd.iterations, sd.update_lag = int(iterations), int(update_lag)
+        sd.a_smooth = float(a_smooth)
+        sd.a_data = ptr(self._a_data)
+        sd.max_batch = self.max_batch
+        _device()
+        h = _lib.c_void_p()
+        check(lib().frh_solver_create(_lib.ctypes.byref(h), _lib.ctypes.byref(sd)),
+              "frh_solver_create")
+        self._h = h
+        self._ref = None
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().frh_solver_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_reference(self, fixed, weight=None, uv=None):
+        """fixed: (H, W[, C]); weight as get_displacement accepts; uv (H, W, 2)."""
+        f = _as_planes(fixed)
+        if f.shape != (self.C, self.H, self.W):
+            raise ValueError(f"fixed has shape {tuple(f.shape)}, expected C,H,W="
+                             f"{(self.C, self.H, self.W)}")
+        wmap = expand_weight(weight, self.H, self.W, self.C)
+        w = _as_planes(wmap.astype(np.float32))
+        u = None
+        if uv is not None:
+            u = uv if isinstance(uv, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(uv))
+            u = u.to(device=f.device, dtype=torch.float32).permute(2, 0, 1).contiguous()
+        self._ref = (f, w, u)
+        check(lib().frh_solver_set_reference(self._h, ptr(f), ptr(w), ptr(u), stream_handle()),
+              "frh_solver_set_reference")
+
+    def run(self, moving, out_dtype=torch.float64, profile=False) -> torch.Tensor:
+        """moving: (T, H, W, C) or (H, W[, C]) -> flow (T, H, W, 2) on the GPU."""
+        mv = moving if isinstance(moving, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(moving))
+        single = mv.ndim < 4
+        if mv.ndim == 2:
+            mv = mv[None, :, :, None]
+        elif mv.ndim == 3:
+            mv = mv[None]
+        planes = _as_planes(mv)
+        T = planes.shape[0]
+        if planes.shape[1:] != (self.C, self.H, self.W):
+            raise ValueError(f"moving frames have shape {tuple(planes.shape[1:])}, expected "
+                             f"{(self.C, self.H, self.W)}")
+        out = torch.empty((T, self.H, self.W, 2), dtype=out_dtype, device=planes.device)
+        check(lib().frh_solver_profile(self._h, 1 if profile else 0), "frh_solver_profile")
+        for s in range(0, T, self.max_batch):
+            e = min(T, s + self.max_batch)
+            check(lib().frh_solver_run(self._h, ptr(planes[s:e]), e - s, ptr(out[s:e]),
+                                       1 if out_dtype == torch.float64 else 0, stream_handle()),
+                  "frh_solver_run")
+        return out[0] if single else out
+
+    def timings(self):
+        """(ms, counts) per category of the last profiled run."""
+        ms = np.zeros(7, np.float64)
+        cnt = np.zeros(7, np.int32)
+        lib().frh_solver_timings(self._h, ptr(ms), ptr(cnt), 7)
+        names = ["total", "sweep", "assemble", "tensor", "resize", "warp", "median"]
+        return {n: (float(ms[k]), int(cnt[k])) for k, n in enumerate(names)}
+
+
+@functools.lru_cache(maxsize=8)
+def _cached_solver(H, W, C, alpha, update_lag, iterations, min_level, levels, eta, a_smooth,
+                   a_data, max_batch):
+    return DisplacementSolver(H, W, C, alpha=alpha, update_lag=update_lag, iterations=iterations,
+                              min_level=min_level, levels=levels, eta=eta, a_smooth=a_smooth,
+                              a_data=np.array(a_data), max_batch=max_batch)
+
+
+# ------------------------------------------------------------ get_displacement
+def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
+                     levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, const_assumption="gc",
+                     uv=None, weight=None, level_solver_backend=None, gnc_schedule=None,
+                     warping_steps=None):
+    """core/optical_flow.py:773-929 on the GPU.
+
+    Returns (H, W, 2) float64 numpy ([..., 0] = u, [..., 1] = v) for numpy
+    inputs, or a float64 CUDA tensor when ``fixed`` is a tensor.
+    """
+    assert fixed.ndim == moving.ndim, (
+        f"Fixed and moving must have same dimensions: fixed.shape={fixed.shape}, "
+        f"moving.shape={moving.shape}")
+    term = _resolve_const_assumption(const_assumption)
+    if gnc_schedule is not None:
+        raise ValueError("flowreg_hip does not implement GNC schedules (gnc_schedule)")
+    if term != "gc":
+        raise ValueError(f"flowreg_hip implements the 'gc' data term only, got '{term}'")
+    as_tensor = isinstance(fixed, torch.Tensor)
+    if fixed.ndim == 2:
+        m, n = fixed.shape
+        C = 1
+    else:
+        m, n, C = fixed.shape
+    if np.isscalar(alpha):
+        alpha = (alpha, alpha)
+    alpha = tuple(float(a) for a in alpha)
+    ad = tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist())
+    if level_solver_backend is not None:
+        from ._hooked import solve_with_level_solver
+        w = solve_with_level_solver(fixed, moving, alpha, update_lag, iterations, min_level,
+                                    levels, eta, a_smooth, np.array(ad), uv, weight,
+                                    level_solver_backend)
+        return torch.from_numpy(w).to(_device()) if as_tensor else w
+    solver = _cached_solver(m, n, C, alpha, int(update_lag), int(iterations), int(min_level),
+                            int(levels), float(eta), float(a_smooth), ad, 1)
+    solver.set_reference(fixed, weight, uv)
+    w = solver.run(moving, out_dtype=torch.float64)
+    if as_tensor:
+        return w
+    return w.cpu().numpy()
+
+
+# ---------------------------------------------------------------- stages
+def imresize_fused_gauss_cubic(img, size, sigma_coeff=0.6, per_axis=False):
+    """util/resize_util.py:232-254 (per_axis=False) on the GPU."""
+    if per_axis:
+        raise ValueError("flowreg_hip: per_axis resize is not implemented")
+    oh, ow = size[:2]
+    src = np.asarray(img) if not isinstance(img, torch.Tensor) else img
+    in_dtype = src.dtype
+    h, w = src.shape[:2]
+    sig = _resize_sigma(h, w, oh, ow, sigma_coeff)
+    ix, wx = resize_tables(w, ow, sig)
+    iy, wy = resize_tables(h, oh, sig)
+    planes = _as_planes(src)
+    N = planes.shape[0]
+    dev = planes.device
+    d = [torch.from_numpy(np.array(a)).to(dev) for a in (ix, wx, iy, wy)]
+    dst = torch.empty((N, oh, ow), dtype=torch.float32, device=dev)
+    tmp = torch.empty((N, h, ow), dtype=torch.float32, device=dev)
+    check(lib().frh_resize(ptr(planes), N, h, w, ptr(dst), oh, ow, ptr(d[0]), ptr(d[1]),
+                           ptr(d[2]), ptr(d[3]), ix.shape[1], ptr(tmp), stream_handle()),
+          "frh_resize")
+    out = dst.permute(1, 2, 0)
+    if src.ndim == 2:
+        out = out[:, :, 0]
+    if isinstance(img, torch.Tensor):
+        return out.contiguous()
+    res = out.cpu().numpy()
+    return res.astype(in_dtype, copy=False)
+
+
+def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
+    """core/warping.py:160-234 (cubic) on the GPU; numpy in, numpy out."""
+    if interpolation_method.lower() != "cubic":
+        if interpolation_method.lower() == "linear":
+            raise ValueError("flowreg_hip: linear warp is not implemented")
+        raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
+    squeeze = np.asarray(f2_level).ndim == 2 if not isinstance(f2_level, torch.Tensor) else f2_level.ndim == 2
+    f2 = _as_planes(f2_level)
+    f1 = _as_planes(f1_level)
+    C, H, W = f2.shape
+    dev = f2.device
+    uu = torch.as_tensor(np.asarray(u, np.float32) if not isinstance(u, torch.Tensor) else u,
+                         device=dev, dtype=torch.float32).contiguous()
+    vv = torch.as_tensor(np.asarray(v, np.float32) if not isinstance(v, torch.Tensor) else v,
+                         device=dev, dtype=torch.float32).contiguous()
+    out = torch.empty_like(f2)
+    check(lib().frh_warp_cubic(ptr(f2), ptr(f1), 0, 1, C, H, W, ptr(uu), ptr(vv), 1.0, 1.0,
+                               ptr(out), stream_handle()), "frh_warp_cubic")
+    res = out.permute(1, 2, 0)
+    res = res[:, :, 0] if squeeze else res
+    if isinstance(f2_level, torch.Tensor):
+        return res.contiguous()
+    return res.cpu().numpy()
+
+
+def get_motion_tensor_gc(f1, f2, hy, hx):
+    """core/optical_flow.py:73-160 on the GPU (one channel); float64 outputs
+    of shape (h+2, w+2)."""
+    dev = _device()
+    a = torch.as_tensor(np.asarray(f1, dtype=np.float32), device=dev).contiguous()
+    b = torch.as_tensor(np.asarray(f2, dtype=np.float32), device=dev).contiguous()
+    h, w = a.shape
+    J = [torch.empty((h + 2, w + 2), dtype=torch.float64, device=dev) for _ in range(6)]
+    check(lib().frh_motion_tensor_gc(ptr(a), ptr(b), h, w, float(hy), float(hx),
+                                     *[ptr(j) for j in J], stream_handle()),
+          "frh_motion_tensor_gc")
+    return tuple(j.cpu().numpy() for j in J)
+
+
+def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, update_lag,
+                 verbose, a_data, a_smooth, hx, hy, gnc_beta=None):
+    """core/optical_flow.py:348-411 on the GPU: returns (du, dv) (m, n) float64."""
+    if gnc_beta is not None:
+        raise ValueError("flowreg_hip does not implement the GNC level solver (gnc_beta)")
+    dev = _device()
+
+    def d(a):
+        return torch.as_tensor(np.ascontiguousarray(a, dtype=np.float64), device=dev)
+    Js = [d(j) for j in (J11, J22, J33, J12, J13, J23)]
+    m, n, K = Js[0].shape
+    W_ = d(weight).reshape(m, n, -1)
+    uu, vv = d(u), d(v)
+    ad = np.ascontiguousarray(np.broadcast_to(np.asarray(a_data, np.float64), (K,)))
+    du = torch.empty((m, n), dtype=torch.float64, device=dev)
+    dv = torch.empty((m, n), dtype=torch.float64, device=dev)
+    check(lib().frh_level_solver(*[ptr(j) for j in Js], ptr(W_), ptr(uu), ptr(vv), m, n, K,
+                                 float(alpha[0]), float(alpha[1]), int(iterations),
+                                 int(update_lag), ptr(ad), float(a_smooth), float(hx), float(hy),
+                                 ptr(du), ptr(dv), stream_handle()), "frh_level_solver")
+    return du.cpu().numpy(), dv.cpu().numpy()
+
+
+def median5_mirror(f):
+    """scipy.ndimage.median_filter(f, size=(5,5), mode='mirror') on the GPU
+    for a 2-D float64 array (core/optical_flow.py:607-609)."""
+    dev = _device()
+    t = torch.as_tensor(np.ascontiguousarray(f, dtype=np.float64), device=dev).clone()
+    h, w = t.shape
+    check(lib().frh_median5_mirror(ptr(t), 1, h, w, stream_handle()), "frh_median5_mirror")
+    return t.cpu().numpy()

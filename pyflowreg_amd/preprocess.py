@@ -1,0 +1,81 @@
+"""Frame preprocessing on the GPU: normalize, then Gaussian prefilter.
+
+Mirrors ``BatchMotionCorrector._preprocess_frames``
+(motion_correction/compensate_recording.py:521-561): ``normalize``
+(util/image_processing.py:12-98) with the raw reference's range, then
+``apply_gaussian_filter`` (util/image_processing.py:101-165, scipy
+``gaussian_filter`` with mode "reflect", truncate 4), then the executors'
+float32 cast (motion_correction/parallelization/base.py:63-73).
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from ._lib import FlowregHipError, check, lib, ptr, stream_handle
+
+
+def gaussian_weights(sigma: float, truncate: float = 4.0):
+    """The 1-D kernel scipy.ndimage.gaussian_filter1d correlates with
+    (order 0): radius int(truncate * sigma + 0.5), normalised exp."""
+    sd = float(sigma)
+    radius = int(truncate * sd + 0.5)
+    x = np.arange(-radius, radius + 1)
+    phi = np.exp(-0.5 / (sd * sd) * x ** 2)
+    phi = phi / phi.sum()
+    return radius, np.ascontiguousarray(phi[::-1], dtype=np.float64)
+
+
+def _range(src, C, channel_normalization, eps):
+    """Per-channel (lo, den) of normalize(): den = (max - min) + eps."""
+    a = src if isinstance(src, torch.Tensor) else torch.as_tensor(np.asarray(src))
+    lo = np.empty(C)
+    den = np.empty(C)
+    if channel_normalization == "separate" and a.ndim >= 3:
+        for c in range(C):
+            mn = float(a[..., c].min())
+            mx = float(a[..., c].max())
+            lo[c], den[c] = mn, (mx - mn) + eps
+    else:
+        mn = float(a.min())
+        mx = float(a.max())
+        lo[:] = mn
+        den[:] = (mx - mn) + eps
+    return lo, den
+
+
+def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
+               channel_normalization="together", eps=1e-8, truncate=4.0) -> torch.Tensor:
+    """frames: (H, W, C) or (T, H, W, C) numpy/tensor (float32 or float64).
+    Returns float32 CUDA planes (T, C, H, W) (T = 1 for a single frame)."""
+    if not torch.cuda.is_available():
+        raise FlowregHipError("flowreg_hip needs a ROCm GPU; none is visible")
+    dev = torch.device("cuda", torch.cuda.current_device())
+    x = frames if isinstance(frames, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(frames))
+    spatial_only = x.ndim == 3
+    if x.ndim == 3:
+        x = x[None]
+    if x.ndim != 4:
+        raise ValueError("preprocess expects (H, W, C) or (T, H, W, C) frames")
+    f64 = x.dtype == torch.float64
+    x = x.to(device=dev, dtype=torch.float64 if f64 else torch.float32).contiguous()
+    T, H, W, C = x.shape
+    sig = np.asarray(sigma, dtype=np.float64)
+    if sig.ndim == 2:
+        if not np.all(sig == sig[0]):
+            raise ValueError("flowreg_hip: per-channel sigmas must be equal")
+        sig = sig[0]
+    sy, sx = float(sig[0]), float(sig[1])
+    st = float(sig[2]) if (len(sig) > 2 and not spatial_only) else 0.0
+    if st > 1e-15 and int(truncate * st + 0.5) > 0:
+        raise ValueError("flowreg_hip: temporal Gaussian radius > 0 is not implemented")
+    ref = normalization_ref if normalization_ref is not None else frames
+    lo, den = _range(ref, C, channel_normalization, eps)
+    ry, wy = gaussian_weights(sy, truncate) if sy > 1e-15 else (0, np.ones(1))
+    rx, wx = gaussian_weights(sx, truncate) if sx > 1e-15 else (0, np.ones(1))
+    out = torch.empty((T, C, H, W), dtype=torch.float32, device=dev)
+    fn = lib().frh_prefilter_f64 if f64 else lib().frh_prefilter
+    check(fn(ptr(x), T, H, W, C, ptr(lo), ptr(den), ptr(wy), ry, ptr(wx), rx, ptr(out),
+             stream_handle()), "frh_prefilter")
+    return out

@@ -1,0 +1,63 @@
+"""The C ABI library loads and exports every entry point include/*.h declares
+(no compute calls: runs without a GPU)."""
+
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def _declared():
+    names = set()
+    for fn in os.listdir(os.path.join(ROOT, "include")):
+        if fn.endswith(".h") and fn != "flowreg_detmath.h":
+            txt = open(os.path.join(ROOT, "include", fn)).read()
+            txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+            names |= set(re.findall(r"\b(frh_[a-z0-9_]+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    from pyflowreg_amd import _lib
+    if not os.path.exists(_lib.LIB_PATH):
+        pytest.fail("libflowreg_hip.so is not built (run __graft_entry__.build())")
+    L = _lib.lib()
+    declared = _declared()
+    assert len(declared) >= 15
+    missing = [n for n in declared if not hasattr(L, n)]
+    assert not missing, missing
+    assert set(_lib.SIGNATURES) == declared
+
+
+def test_host_entry_points_without_gpu():
+    import numpy as np
+    from pyflowreg_amd import _lib
+    L = _lib.lib()
+    assert L.frh_abi_version() == 1
+    assert L.frh_warping_depth(0.8, 50, 512, 512) == 18
+    g = np.array([1.0], np.float32)
+    idx = np.zeros((5, 4), np.int32)
+    wt = np.zeros((5, 4), np.float32)
+    assert L.frh_fill_resize_tables(5, 5, 0, g.ctypes.data, idx.ctypes.data, wt.ctypes.data) == 0
+    np.testing.assert_array_equal(wt[:, 2], 1.0)  # scale 1 is the identity
+    rc = L.frh_fill_resize_tables(0, 5, 0, g.ctypes.data, idx.ctypes.data, wt.ctypes.data)
+    assert rc == -1 and b"bad arguments" in L.frh_last_error()
+
+
+def test_get_displacement_raises_without_gpu():
+    import numpy as np
+    import torch
+    if torch.cuda.is_available():
+        pytest.skip("a GPU is visible")
+    import pyflowreg_amd as fr
+    with pytest.raises(RuntimeError, match="GPU"):
+        fr.get_displacement(np.zeros((16, 16)), np.zeros((16, 16)))
+
+
+def test_unknown_constancy_assumption_is_value_error():
+    import numpy as np
+    import pyflowreg_amd as fr
+    with pytest.raises(ValueError, match="Unknown constancy assumption"):
+        fr.get_displacement(np.zeros((8, 8)), np.zeros((8, 8)), const_assumption="nope")

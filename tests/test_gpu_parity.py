@@ -1,0 +1,142 @@
+"""GPU kernels vs the CPU oracle on identical seeded inputs.
+
+Bar: bit-exact (max |diff| == 0) for every stage and for the end-to-end flow,
+since the kernels replay the oracle's float64/float32 operation order.
+"""
+
+import numpy as np
+import pytest
+from scipy.ndimage import median_filter
+
+pytestmark = pytest.mark.gpu
+
+
+def _rng(seed):
+    return np.random.default_rng(seed)
+
+
+def _smooth_image(rng, h, w, C=1):
+    from scipy.ndimage import gaussian_filter
+    img = rng.random((h, w, C))
+    for c in range(C):
+        img[:, :, c] = gaussian_filter(img[:, :, c], 1.5)
+    return img.astype(np.float32)
+
+
+@pytest.mark.parametrize("src,dst", [((37, 53), (20, 29)), ((11, 13), (20, 29)), ((64, 64), (9, 9)),
+                                     ((30, 40), (30, 40))])
+def test_resize_bit_exact(gpu, oracle, src, dst):
+    img = _rng(1).random((*src, 2)).astype(np.float32)
+    a = gpu.imresize_fused_gauss_cubic(img, dst)
+    b = oracle.resize(img, dst)
+    assert a.shape == b.shape
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("H,W,C,scale", [(31, 47, 2, 3.0), (20, 20, 1, 0.4), (64, 33, 1, 8.0)])
+def test_warp_bit_exact(gpu, oracle, H, W, C, scale):
+    rng = _rng(2)
+    f2 = rng.random((H, W, C)).astype(np.float32)
+    f1 = rng.random((H, W, C)).astype(np.float32)
+    u = (rng.standard_normal((H, W)) * scale).astype(np.float32)
+    v = (rng.standard_normal((H, W)) * scale).astype(np.float32)
+    a = gpu.imregister_wrapper(f2, u, v, f1)
+    b = oracle.imregister_wrapper(f2, u.astype(np.float64), v.astype(np.float64), f1)
+    np.testing.assert_array_equal(a, b)
+
+
+@pytest.mark.parametrize("h,w,hy,hx", [(19, 23, 1.0, 2.0), (9, 9, 56.9, 56.9), (40, 17, 1.25, 1.3)])
+def test_motion_tensor_bit_exact(gpu, oracle, h, w, hy, hx):
+    rng = _rng(3)
+    f1 = _smooth_image(rng, h, w)[:, :, 0]
+    f2 = _smooth_image(rng, h, w)[:, :, 0]
+    a = gpu.get_motion_tensor_gc(f1, f2, hy, hx)
+    b = oracle.get_motion_tensor_gc(f1, f2, hy, hx)
+    for x, y in zip(a, b):
+        np.testing.assert_array_equal(x, y)
+
+
+def _level_inputs(oracle, rng, h, w, K):
+    f1 = _smooth_image(rng, h, w, K)
+    f2 = _smooth_image(rng, h, w, K)
+    J = np.zeros((6, h + 2, w + 2, K))
+    for k in range(K):
+        Jk = oracle.get_motion_tensor_gc(f1[:, :, k], f2[:, :, k], 1.1, 0.9)
+        for t in range(6):
+            J[t, :, :, k] = Jk[t]
+    W = np.zeros((h + 2, w + 2, K))
+    W[1:-1, 1:-1] = rng.uniform(0.2, 1.0, (h, w, K)).astype(np.float32)
+    u = np.pad(rng.standard_normal((h, w)).astype(np.float32).astype(np.float64), 1, mode="edge")
+    v = np.pad(rng.standard_normal((h, w)).astype(np.float32).astype(np.float64), 1, mode="edge")
+    return J, W, u, v
+
+
+@pytest.mark.parametrize("h,w,K,its,lag", [(12, 15, 1, 50, 5), (40, 33, 2, 50, 5),
+                                          (70, 130, 2, 12, 5), (9, 9, 1, 7, 3), (20, 11, 2, 3, 1),
+                                          (130, 70, 1, 6, 2)])
+def test_level_solver_bit_exact(gpu, oracle, h, w, K, its, lag):
+    rng = _rng(4)
+    J, W, u, v = _level_inputs(oracle, rng, h, w, K)
+    ad = np.full(K, 0.45)
+    args = (J[0], J[1], J[2], J[3], J[4], J[5], W, u, v, [2.0, 2.5], its, lag, 0, ad, 1.0, 1.3, 0.8)
+    du, dv = gpu.level_solver(*args)
+    ou, ov = oracle.level_solver(*args)
+    np.testing.assert_array_equal(du, ou)
+    np.testing.assert_array_equal(dv, ov)
+
+
+@pytest.mark.parametrize("h,w", [(20, 17), (6, 6), (64, 9)])
+def test_median_bit_exact(gpu, h, w):
+    from pyflowreg_amd.optical_flow import median5_mirror
+    f = _rng(5).standard_normal((h, w))
+    np.testing.assert_array_equal(median5_mirror(f), median_filter(f, size=(5, 5), mode="mirror"))
+
+
+def _pair(H, W, C, seed):
+    from pyflowreg_amd import synth
+    import torch
+    frames, _ = synth.video(2, H, W, n_channels=C, max_disp=2.0, sigma_d=H / 6, seed=seed)
+    return frames[0].numpy(), frames[1].numpy()
+
+
+@pytest.mark.parametrize("H,W,C", [(64, 64, 1), (48, 80, 2), (96, 72, 2)])
+def test_get_displacement_bit_exact(gpu, oracle, H, W, C):
+    f, m = _pair(H, W, C, 10 + H)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(m, f.astype(np.float64)).astype(np.float32)
+    uv = np.zeros((H, W, 2), np.float32)
+    uv[..., 0] = 0.3
+    uv[..., 1] = -0.2
+    kw = dict(weight=[0.5, 0.5][:C] if C > 1 else None, uv=uv)
+    a = gpu.get_displacement(ref, mov, **kw)
+    b = oracle.get_displacement(ref, mov, **kw)
+    assert a.shape == (H, W, 2)
+    d = np.abs(a - b)
+    assert d.max() == 0.0, f"max |dflow| {d.max():.3e} rms {np.sqrt((d ** 2).mean()):.3e}"
+
+
+def test_prefilter_bit_exact(gpu, oracle):
+    from pyflowreg_amd.preprocess import preprocess
+    f, m = _pair(40, 56, 2, 3)
+    fr = f.astype(np.float64)
+    a = preprocess(fr).permute(0, 2, 3, 1)[0].cpu().numpy()
+    b = oracle.preprocess(fr).astype(np.float32)
+    np.testing.assert_array_equal(a, b)
+    batch = np.stack([m, f])
+    a2 = preprocess(batch, normalization_ref=fr).permute(0, 2, 3, 1).cpu().numpy()
+    b2 = oracle.preprocess(batch, fr).astype(np.float32)
+    np.testing.assert_array_equal(a2, b2)
+
+
+def test_batched_solver_equals_single(gpu):
+    import torch
+    from pyflowreg_amd import synth
+    frames, _ = synth.video(4, 64, 48, n_channels=2, max_disp=2.0, sigma_d=10, seed=77)
+    fr = frames.numpy()
+    ref = fr[0]
+    s = gpu.DisplacementSolver(64, 48, 2, max_batch=3)
+    s.set_reference(ref, [0.5, 0.5])
+    wb = s.run(fr[1:]).cpu().numpy()
+    for t in range(3):
+        w1 = gpu.get_displacement(ref, fr[1 + t], weight=[0.5, 0.5])
+        np.testing.assert_array_equal(wb[t], w1)

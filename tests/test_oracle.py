@@ -1,0 +1,447 @@
+"""Pins of the CPU oracle (oracle/) against the reference's own known-answer
+tests, closed forms and independent library routines.  CPU only.
+
+Each test names the oracle function it pins.  A check that merely re-typed
+the oracle's formula would pin nothing, so the expected values here come from
+a different route: torch's bicubic resampler (Keys, A = -0.75), scipy's
+Gaussian filter, analytic derivatives of polynomials, dense linear algebra,
+and the numbers the reference's tests hold.
+"""
+
+import math
+
+import numpy as np
+import pytest
+import torch
+from scipy.ndimage import gaussian_filter1d
+
+
+# ---------------------------------------------------------------- geometry
+@pytest.mark.parametrize("size,n_levels", [((512, 512), 19), ((256, 256), 16),
+                                           ((796, 512), 21), ((1024, 1024), 22)])
+def test_level_counts(oracle, size, n_levels):
+    plan, ml = oracle.level_plan(*size, 0.8, 50, 0)
+    assert len(plan) == n_levels and ml == 0
+    assert plan[-1][1] == size
+
+
+def test_level_sizes_512(oracle):
+    plan, _ = oracle.level_plan(512, 512, 0.8, 50, 0)
+    sides = [s[0] for _, s in plan]
+    assert sides == [9, 12, 14, 18, 23, 28, 35, 44, 55, 69, 86, 107, 134, 168, 210, 262, 328,
+                     410, 512]
+    # closed form: round(512 * 0.8^k) >= 10 for k < depth, < 10 at depth
+    depth = oracle.warping_depth(0.8, 50, 512, 512)
+    assert round(512 * 0.8 ** (depth - 1)) >= 10 > round(512 * 0.8 ** depth)
+
+
+# ------------------------------------------------------------------ resize
+def _torch_bicubic(img, oh, ow):
+    t = torch.from_numpy(img.astype(np.float64))[None, None]
+    return torch.nn.functional.interpolate(t, size=(oh, ow), mode="bicubic",
+                                           align_corners=False)[0, 0].numpy()
+
+
+def test_resize_identity_at_scale_one(oracle):
+    img = np.random.default_rng(0).random((23, 31)).astype(np.float32)
+    np.testing.assert_array_equal(oracle.resize(img, (23, 31)), img)
+
+
+def test_resize_tables_partition_of_unity(oracle):
+    for (i, o, s) in [(512, 9, 0.6 * 512 / 9), (100, 37, 0.6 * 100 / 37), (13, 29, 0.0)]:
+        idx, wt = oracle.resize_tables(i, o, s)
+        np.testing.assert_allclose(wt.astype(np.float64).sum(axis=1), 1.0, atol=2e-6)
+        assert idx.min() >= 0 and idx.max() < i
+
+
+def _torch_keys_matrix(n, o):
+    """T[i, j] = torch's bicubic (Keys, A = -0.75) weight of input j for
+    output i, from resizing unit impulses."""
+    eye = torch.eye(n, dtype=torch.float64)[:, None, None, :]
+    out = torch.nn.functional.interpolate(eye, size=(1, o), mode="bicubic", align_corners=False)
+    return out[:, 0, 0, :].numpy().T
+
+
+def test_resize_upsample_uses_reference_tap_window(oracle):
+    """sigma = 0 (flow upsampling): util/resize_util.py:199-213 takes taps
+    j = floor(x-2) ... floor(x)+1 and renormalises, i.e. the Keys tap at
+    floor(x)+2 is dropped.  Expected values use torch's Keys weights on the
+    three remaining non-zero taps."""
+    n, o = 20, 37
+    f = np.random.default_rng(1).random((3, n)).astype(np.float32)
+    out = oracle.resize(f, (3, o))
+    T = _torch_keys_matrix(n, o)
+    checked = 0
+    for i in range(o):
+        x = (i + 0.5) / (o / n) - 0.5
+        fl = int(np.floor(x))
+        if fl - 2 < 0 or fl + 2 >= n:
+            continue
+        keep = [fl - 1, fl, fl + 1]
+        w = T[i, keep]
+        np.testing.assert_allclose(out[:, i], f[:, keep].astype(np.float64) @ w / w.sum(),
+                                   atol=1e-6)
+        checked += 1
+    assert checked > 25
+    # ... which differs from plain bicubic by far more than rounding
+    full = f.astype(np.float64) @ T.T
+    assert np.abs(out[:, 5:-5] - full[:, 5:-5]).max() > 1e-3
+
+
+def test_resize_downsample_is_gaussian_then_cubic(oracle):
+    """1-D downsample: the fused kernel is the float32 Gaussian g followed by
+    Keys-cubic resampling (torch), minus the one dropped tap
+    j* = floor(x) + 2 + R whose weight is g[0] k(frac - 2), renormalised."""
+    n, o = 200, 61
+    f = np.random.default_rng(2).random((1, n)).astype(np.float32)
+    out = oracle.resize(f, (1, o))[0].astype(np.float64)
+    sig = 0.6 / (o / n)
+    R, g = oracle.gauss_g(sig)
+    g = g.astype(np.float64)
+    gf = np.convolve(np.pad(f[0].astype(np.float64), R, mode="symmetric"), g, mode="valid")
+    T = _torch_keys_matrix(n, o)
+    checked = 0
+    for i in range(o):
+        x = (i + 0.5) / (o / n) - 0.5
+        fl = int(np.floor(x))
+        if fl - 3 - 2 * R < 0 or fl + 3 + 2 * R >= n:
+            continue
+        wmiss = g[0] * T[i, fl + 2]
+        want = (T[i] @ gf - wmiss * f[0, fl + 2 + R]) / (1 - wmiss)
+        assert abs(out[i] - want) < 2e-6
+        checked += 1
+    assert checked > 20
+
+
+# -------------------------------------------------------------------- warp
+def test_warp_zero_displacement_is_identity(oracle):
+    """tests/core/test_warping.py:22-33 (zero shift preserves frames)."""
+    rng = np.random.default_rng(3)
+    f2 = rng.random((32, 40, 2)).astype(np.float32)
+    f1 = rng.random((32, 40, 2)).astype(np.float32)
+    z = np.zeros((32, 40))
+    np.testing.assert_array_equal(oracle.imregister_wrapper(f2, z, z, f1), f2)
+
+
+@pytest.mark.parametrize("sx,sy", [(10, 5), (-8, -6), (3, 0)])
+def test_warp_integer_shift_content_and_oob(oracle, sx, sy):
+    """tests/core/test_warping.py:250-360: warped[y, x] = f2[y+sy, x+sx];
+    pixels mapped outside take the fixed frame's value."""
+    H, W = 50, 64
+    yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    f2 = (yy * 1000 + xx).astype(np.float32)
+    f1 = np.full((H, W), -7.0, np.float32)
+    u = np.full((H, W), float(sx))
+    v = np.full((H, W), float(sy))
+    out = oracle.imregister_wrapper(f2, u, v, f1)
+    ys = slice(max(0, -sy), H - max(0, sy))
+    xs = slice(max(0, -sx), W - max(0, sx))
+    np.testing.assert_array_equal(out[ys, xs],
+                                  f2[ys.start + sy:ys.stop + sy, xs.start + sx:xs.stop + sx])
+    oob = (xx + sx < 0) | (xx + sx >= W) | (yy + sy < 0) | (yy + sy >= H)
+    assert oob.any()
+    np.testing.assert_array_equal(out[oob], -7.0)
+
+
+def _torch_warp(f2, mx, my):
+    H, W = f2.shape
+    grid = torch.from_numpy(np.stack([2 * mx / (W - 1) - 1, 2 * my / (H - 1) - 1], -1).astype(np.float64))[None]
+    return torch.nn.functional.grid_sample(torch.from_numpy(f2.astype(np.float64))[None, None],
+                                           grid, mode="bicubic", padding_mode="border",
+                                           align_corners=True)[0, 0].numpy()
+
+
+def test_warp_quantizes_positions_to_1_32_px(oracle):
+    """A 0.3 px shift is applied as round(0.3 * 32) / 32 = 0.3125 px (OpenCV's
+    INTER_BITS = 5): the result equals torch's bicubic at 0.3125, not 0.3."""
+    H, W = 24, 40
+    f2 = np.random.default_rng(11).random((H, W)).astype(np.float32)
+    yy, xx = np.meshgrid(np.arange(H, dtype=np.float64), np.arange(W, dtype=np.float64),
+                         indexing="ij")
+    out = oracle.imregister_wrapper(f2, np.full((H, W), 0.3), np.zeros((H, W)), f2)
+    q = _torch_warp(f2, xx + 0.3125, yy)
+    c = _torch_warp(f2, xx + 0.3, yy)
+    inner = (slice(3, -3), slice(3, -3))
+    np.testing.assert_allclose(out[inner], q[inner], atol=1e-5)
+    assert np.abs(out[inner] - c[inner]).max() > 1e-3
+
+
+def test_warp_matches_torch_bicubic_at_quantized_positions(oracle):
+    rng = np.random.default_rng(4)
+    H, W = 30, 36
+    f2 = rng.random((H, W)).astype(np.float32)
+    u = rng.uniform(-0.9, 0.9, (H, W))
+    v = rng.uniform(-0.9, 0.9, (H, W))
+    out = oracle.imregister_wrapper(f2, u, v, f2)
+    yy, xx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+    mx = np.round((xx + u).astype(np.float32) * 32) / 32
+    my = np.round((yy + v).astype(np.float32) * 32) / 32
+    ref = _torch_warp(f2, mx, my)
+    np.testing.assert_allclose(out[3:-3, 3:-3], ref[3:-3, 3:-3], atol=1e-5)
+
+
+# ------------------------------------------------------------------ tensor
+def test_motion_tensor_reference_known_answer(oracle):
+    """tests/core/test_optical_flow.py:532-551: f = 1e-3 x^2 with hx = 2 gives
+    J11 = 0.8 at interior pixels; identical frames give J33 = J13 = 0."""
+    hy, hx = 1.0, 2.0
+    H, W = 17, 23
+    x = np.arange(W, dtype=np.float64) * hx
+    f = np.tile(1e-3 * x ** 2, (H, 1)).astype(np.float32)
+    J11, J22, J33, J12, J13, J23 = oracle.get_motion_tensor_gc(f, f, hy, hx)
+    c = (H // 2, W // 2)
+    np.testing.assert_allclose(J11[c], 0.8, atol=0.01)
+    np.testing.assert_allclose(J33[c], 0.0, atol=1e-12)
+    np.testing.assert_allclose(J13[c], 0.0, atol=1e-12)
+
+
+def test_motion_tensor_quadratic_closed_form(oracle):
+    """f1 = a x^2 + b x y + c y^2 and f2 = f1 + d x on a grid with spacings
+    (hx, hy): central differences are exact, so fxx = 2a, fyy = 2c, fxy = b,
+    fxt = d, fyt = 0 and J follows in closed form at interior pixels."""
+    a, b, c, d = 0.003, -0.002, 0.0015, 0.01
+    hy, hx = 1.5, 0.75
+    H, W = 15, 19
+    yy, xx = np.meshgrid(np.arange(H) * hy, np.arange(W) * hx, indexing="ij")
+    f1 = (a * xx ** 2 + b * xx * yy + c * yy ** 2 + 0.3).astype(np.float32)
+    f2 = (f1.astype(np.float64) + d * xx).astype(np.float32)
+    J = oracle.get_motion_tensor_gc(f1, f2, hy, hx)
+    fxx, fyy, fxy, fxt, fyt = 2 * a, 2 * c, b, d, 0.0
+    rx = 1 / (fxx ** 2 + fxy ** 2 + 1e-6)
+    ry = 1 / (fxy ** 2 + fyy ** 2 + 1e-6)
+    want = [rx * fxx ** 2 + ry * fxy ** 2, rx * fxy ** 2 + ry * fyy ** 2,
+            rx * fxt ** 2 + ry * fyt ** 2, rx * fxx * fxy + ry * fxy * fyy,
+            rx * fxx * fxt + ry * fxy * fyt, rx * fxy * fxt + ry * fyy * fyt]
+    inner = (slice(4, -4), slice(4, -4))
+    for got, w in zip(J, want):
+        np.testing.assert_allclose(got[inner], w, rtol=2e-3, atol=2e-4)
+        assert np.all(got[0, :] == 0) and np.all(got[:, -1] == 0)
+
+
+# ------------------------------------------------------------------ solver
+def _random_level(oracle, rng, h, w, K, zero_data=False):
+    from scipy.ndimage import gaussian_filter
+    f1 = gaussian_filter(rng.random((h, w)), 1.2).astype(np.float32)
+    f2 = f1 if zero_data else gaussian_filter(rng.random((h, w)), 1.2).astype(np.float32)
+    J = np.zeros((6, h + 2, w + 2, K))
+    for k in range(K):
+        Jk = oracle.get_motion_tensor_gc(f1, f2, 1.0, 1.0)
+        for t in range(6):
+            J[t, :, :, k] = Jk[t] * (1 + 0.3 * k)
+    W = np.zeros((h + 2, w + 2, K))
+    W[1:-1, 1:-1] = rng.uniform(0.3, 1.0, (h, w, K))
+    u = np.pad(rng.standard_normal((h, w)) * 0.2, 1, mode="edge")
+    v = np.pad(rng.standard_normal((h, w)) * 0.2, 1, mode="edge")
+    return J, W, u, v
+
+
+def _dense_system(J, W, u, v, ax, ay, psi):
+    """Linear system of one lag block in unknown order (pixel column-major,
+    du before dv): returns A (true Neumann operator), Dp (the solver's
+    diagonal incl. out-of-level weights) and b."""
+    m, n, K = J[0].shape
+    h, w = m - 2, n - 2
+    N = h * w
+
+    def idx(r, c):
+        return c * h + r
+    A = np.zeros((2 * N, 2 * N))
+    Dp = np.zeros(2 * N)
+    b = np.zeros(2 * N)
+    for c in range(w):
+        for r in range(h):
+            p = idx(r, c)
+            jr, jc = r + 1, c + 1
+            wp = W[jr, jc] * psi[jr, jc]
+            a11 = (wp * J[0][jr, jc]).sum()
+            a22 = (wp * J[1][jr, jc]).sum()
+            a12 = (wp * J[3][jr, jc]).sum()
+            b1 = -(wp * J[4][jr, jc]).sum()
+            b2 = -(wp * J[5][jr, jc]).sum()
+            ws = 0.0
+            for (dr, dc, wt) in [(0, -1, ax), (0, 1, ax), (1, 0, ay), (-1, 0, ay)]:
+                rr, cc = r + dr, c + dc
+                ws += wt
+                b1 += wt * (u[jr + dr, jc + dc] - u[jr, jc])
+                b2 += wt * (v[jr + dr, jc + dc] - v[jr, jc])
+                if 0 <= rr < h and 0 <= cc < w:
+                    q = idx(rr, cc)
+                    A[2 * p, 2 * q] -= wt
+                    A[2 * p + 1, 2 * q + 1] -= wt
+                    A[2 * p, 2 * p] += wt
+                    A[2 * p + 1, 2 * p + 1] += wt
+            A[2 * p, 2 * p] += a11
+            A[2 * p + 1, 2 * p + 1] += a22
+            A[2 * p, 2 * p + 1] = a12
+            A[2 * p + 1, 2 * p] = a12
+            Dp[2 * p] = ws + a11
+            Dp[2 * p + 1] = ws + a22
+            b[2 * p], b[2 * p + 1] = b1, b2
+    return A, Dp, b
+
+
+def _to_field(x, h, w):
+    du = x[0::2].reshape(w, h).T
+    dv = x[1::2].reshape(w, h).T
+    return du, dv
+
+
+def test_solver_zero_flow_for_identical_frames(oracle):
+    rng = np.random.default_rng(5)
+    J, W, _, _ = _random_level(oracle, rng, 11, 13, 2, zero_data=True)
+    u = np.full((13, 15), 0.7)
+    v = np.full((13, 15), -0.2)
+    f = oracle.compute_flow(*J, W, u, v, 2, 2, 50, 5, [0.45, 0.45], 1.0, 1.0, 1.0)
+    assert np.all(f == 0)
+
+
+def test_solver_converges_to_dense_solve(oracle):
+    """a_data = 1 makes psi = 1: after many sweeps the SOR iterate equals the
+    solution of the 2N x 2N linear system."""
+    rng = np.random.default_rng(6)
+    h, w = 6, 7
+    J, W, u, v = _random_level(oracle, rng, h, w, 2)
+    ax, ay = 2.0 / 1.3 ** 2, 2.5 / 0.8 ** 2
+    f = oracle.compute_flow(*J, W, u, v, 2.0, 2.5, 3000, 10 ** 6, [1.0, 1.0], 1.0, 1.3, 0.8)
+    A, _, b = _dense_system(J, W, u, v, ax, ay, np.ones_like(W))
+    du, dv = _to_field(np.linalg.solve(A, b), h, w)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 0], du, atol=1e-9)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 1], dv, atol=1e-9)
+
+
+@pytest.mark.parametrize("sweeps", [1, 2, 3])
+def test_solver_sweeps_equal_matrix_form_sor(oracle, sweeps):
+    """x <- x + M^-1 (b - A x), M = Dp/omega + strictly-lower(A) in column-major
+    order with du before dv: this is lexicographic SOR; red-black or
+    row-major orderings fail it."""
+    rng = np.random.default_rng(7)
+    h, w = 5, 6
+    J, W, u, v = _random_level(oracle, rng, h, w, 1)
+    ax, ay = 2.0 / 1.1 ** 2, 2.0 / 0.9 ** 2
+    A, Dp, b = _dense_system(J, W, u, v, ax, ay, np.ones_like(W))
+    M = np.tril(A, -1) + np.diag(Dp / 1.95)
+    x = np.zeros(2 * h * w)
+    for _ in range(sweeps):
+        x = x + np.linalg.solve(M, b - A @ x)
+    f = oracle.compute_flow(*J, W, u, v, 2.0, 2.0, sweeps, 10 ** 6, [1.0], 1.0, 1.1, 0.9)
+    du, dv = _to_field(x, h, w)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 0], du, atol=1e-12)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 1], dv, atol=1e-12)
+    if sweeps == 2:
+        rb = oracle.compute_flow(*J, W, u, v, 2.0, 2.0, sweeps, 10 ** 6, [1.0], 1.0, 1.1, 0.9,
+                                 order=1)
+        assert np.abs(rb[1:-1, 1:-1, 0] - du).max() > 1e-6
+
+
+def test_solver_psi_refresh_matches_charbonnier(oracle):
+    """update_lag = 1 refreshes psi before sweep 0 from du = dv = 0:
+    psi_k = a (J33_k + 1e-5)^(a - 1), then one matrix-form SOR sweep."""
+    rng = np.random.default_rng(8)
+    h, w = 5, 5
+    J, W, u, v = _random_level(oracle, rng, h, w, 2)
+    a = 0.45
+    psi = a * (np.maximum(J[2], 0) + 1e-5) ** (a - 1)
+    ax = ay = 2.0
+    A, Dp, b = _dense_system(J, W, u, v, ax, ay, psi)
+    M = np.tril(A, -1) + np.diag(Dp / 1.95)
+    x = np.linalg.solve(M, b)
+    f = oracle.compute_flow(*J, W, u, v, 2.0, 2.0, 1, 1, [a, a], 1.0, 1.0, 1.0)
+    du, dv = _to_field(x, h, w)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 0], du, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 1], dv, rtol=1e-10, atol=1e-12)
+
+
+def test_solver_literal_reference_order_agrees(oracle):
+    """The lag-block coefficient form and the reference's per-sweep expression
+    order (order=2, libm pow) agree to rounding on a smooth problem."""
+    rng = np.random.default_rng(9)
+    J, W, u, v = _random_level(oracle, rng, 16, 19, 2)
+    args = (*J, W, u, v, 2.0, 2.0, 50, 5, [0.45, 0.45], 1.0, 1.2, 0.9)
+    a = oracle.compute_flow(*args)
+    b = oracle.compute_flow(*args, order=2)
+    np.testing.assert_allclose(a, b, atol=1e-12)
+
+
+def _load_detpow():
+    """Compile include/flowreg_detmath.h alone (gcc, no contraction)."""
+    import ctypes
+    import os
+    import subprocess
+    import tempfile
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = ('#include "%s/include/flowreg_detmath.h"\n'
+           "double dp(double x, double b){return frh_det_pow(x,b);}\n"
+           "float dpf(float x, float b){return frh_det_powf(x,b);}\n" % root)
+    d = tempfile.mkdtemp()
+    c, so = os.path.join(d, "dp.c"), os.path.join(d, "dp.so")
+    open(c, "w").write(src)
+    subprocess.run(["gcc", "-O2", "-fPIC", "-shared", "-ffp-contract=off", "-o", so, c, "-lm"],
+                   check=True)
+    L = ctypes.CDLL(so)
+    L.dp.restype, L.dp.argtypes = ctypes.c_double, [ctypes.c_double, ctypes.c_double]
+    L.dpf.restype, L.dpf.argtypes = ctypes.c_float, [ctypes.c_float, ctypes.c_float]
+    return L
+
+
+def test_det_pow_accuracy():
+    """frh_det_pow against numpy's pow: relative error below 4e-15."""
+    L = _load_detpow()
+    xs = np.geomspace(1e-5, 1e4, 3000)
+    for bexp in (-0.55, -0.9, 0.3, -0.1, 0.999):
+        got = np.array([L.dp(x, bexp) for x in xs])
+        assert np.max(np.abs(got / np.power(xs, bexp) - 1)) < 4e-15
+
+
+def test_det_powf_accuracy():
+    L = _load_detpow()
+    xs = np.geomspace(1e-5, 1e4, 500).astype(np.float32)
+    for bexp in (-0.55, 0.3):
+        got = np.array([L.dpf(float(x), bexp) for x in xs], dtype=np.float64)
+        assert np.max(np.abs(got / np.power(xs.astype(np.float64), bexp) - 1)) < 2e-6
+
+
+# ------------------------------------------------------- get_displacement
+def test_driver_binds_geometric_spacings(oracle):
+    """tests/core/test_optical_flow.py:471-500: the solver receives
+    hx = n / cols and hy = m / rows (32 x 96 input)."""
+    rng = np.random.default_rng(7)
+    m, n = 32, 96
+    recs = []
+
+    def solver(*args):
+        J11 = args[0]
+        recs.append(dict(rows=J11.shape[0] - 2, cols=J11.shape[1] - 2, hx=args[15], hy=args[16]))
+        return np.zeros_like(args[7]), np.zeros_like(args[8])
+    flow = oracle.get_displacement(rng.random((m, n)), rng.random((m, n)), iterations=1,
+                                   update_lag=1, level_solver_backend=solver)
+    assert flow.shape == (m, n, 2) and len(recs) > 1
+    for r in recs:
+        assert math.isclose(r["hx"], n / r["cols"], rel_tol=1e-12)
+        assert math.isclose(r["hy"], m / r["rows"], rel_tol=1e-12)
+    assert any(not math.isclose(r["hx"], r["hy"]) for r in recs)
+
+
+def test_driver_recovers_constant_shift(oracle):
+    """Loose end-to-end sanity (cf. tests/session/test_session_integration.py:
+    165-240): a constant shift of the scene is recovered on average."""
+    from pyflowreg_amd import synth
+    H = W = 96
+    scene = synth.blob_scene(H, W, 200, seed=3)
+    d = torch.zeros((1, 2, H, W), dtype=torch.float64)
+    d[:, 0] = 1.5
+    d[:, 1] = -0.75
+    mov = synth.warp_scene(scene[None], d)[0, 0].numpy().astype(np.float32)
+    fix = scene.numpy().astype(np.float32)
+    ref = oracle.preprocess(fix[:, :, None].astype(np.float64)).astype(np.float32)
+    mv = oracle.preprocess(mov[:, :, None], fix[:, :, None].astype(np.float64)).astype(np.float32)
+    w = oracle.get_displacement(ref, mv)
+    inner = w[16:-16, 16:-16]
+    assert abs(inner[..., 0].mean() - 1.5) < 0.1
+    assert abs(inner[..., 1].mean() + 0.75) < 0.1
+
+
+def test_preprocess_normalizes_with_reference_range(oracle):
+    """util/image_processing.py:90-98: (x - min(ref)) / (max(ref) - min(ref) + eps)."""
+    ref = np.array([[[2.0], [6.0]]])
+    arr = np.array([[[4.0], [8.0]]])
+    np.testing.assert_allclose(oracle.normalize(arr, ref), (arr - 2) / (4 + 1e-8))
