@@ -400,6 +400,8 @@ static inline void sor_pixel(int j, int i, int m, int n, int K,
  *   A12 = sum_k (W_k psi_k) J12_k
  *   Bu  = cu - sum_k (W_k psi_k) J13_k,   cu = sum_nb w_nb (u_nb - u_c)
  *   Bv  = cv - sum_k (W_k psi_k) J23_k
+ *   coef = {A12, Bu, Bv, omega / Du, omega / Dv}  (0 where D == 0, as the
+ *          reference's "num / den if den != 0 else 0")
  * psi_k = a_k (E_k + 1e-5)^(a_k - 1) with E_k the data term at (du, dv),
  * evaluated with frh_det_pow. */
 static inline real cf_pow(real x, real b)
@@ -407,6 +409,8 @@ static inline real cf_pow(real x, real b)
     if (sizeof(real) == sizeof(float)) return (real)frh_det_powf((float)x, (float)b);
     return (real)frh_det_pow((double)x, (double)b);
 }
+
+static const real OMEGA = (real)1.95;
 
 static void assemble_pixel(size_t c, int n, int K, const real *J11, const real *J22, const real *J33,
                            const real *J12, const real *J13, const real *J23, const real *W,
@@ -448,34 +452,51 @@ static void assemble_pixel(size_t c, int n, int K, const real *J11, const real *
         Bu = Bu - wp * J13[ck];
         Bv = Bv - wp * J23[ck];
     }
-    coef[0] = Du; coef[1] = Dv; coef[2] = A12; coef[3] = Bu; coef[4] = Bv;
+    coef[0] = A12; coef[1] = Bu; coef[2] = Bv;
+    coef[3] = Du != 0 ? OMEGA / Du : 0;
+    coef[4] = Dv != 0 ? OMEGA / Dv : 0;
     coef[5] = w[0]; coef[6] = w[1]; coef[7] = w[2]; coef[8] = w[3];
 }
 
 /* One SOR update with the lag-block coefficients; core/level_solver.py:
- * 342-364.  Neighbours outside the interior take the pixel's own value at
- * the start of the sweep: set_boundary_2d (:257-258) copies the adjacent
- * interior value into the ring before every sweep, and u is edge-padded, so
- * the reference's u[nb] + du[nb] - u[c] reduces to du_c there. */
+ * 342-364.  A neighbour outside the interior holds the pixel's own value
+ * from the start of the sweep (set_boundary_2d, :257-258, copies the
+ * adjacent interior value into the ring before every sweep, and u is
+ * edge-padded), so its weight moves to the pixel's own term:
+ *   du' = (1 - omega) du + (omega / Du) (Bu + sum_in w_nb du_nb
+ *          + b du - A12 dv),   b = sum of out-of-level weights
+ *       = fma(wu, n, fma(wu, b, 1 - omega) du)
+ *   n   = fma(-A12, dv, fma(wd_sum...)), in the order below (shared with the
+ *         GPU kernel). */
 static inline void sor_pixel_coef(int j, int i, int m, int n, real *du, real *dv,
-                                  const real *coef, real omega)
+                                  const real *coef, int smooth_const)
 {
     size_t c = (size_t)j * n + i;
-    real dul = i > 1 ? du[c - 1] : du[c], dur = i < n - 2 ? du[c + 1] : du[c];
-    real dud = j < m - 2 ? du[c + n] : du[c], duu = j > 1 ? du[c - n] : du[c];
-    real dvl = i > 1 ? dv[c - 1] : dv[c], dvr = i < n - 2 ? dv[c + 1] : dv[c];
-    real dvd = j < m - 2 ? dv[c + n] : dv[c], dvu = j > 1 ? dv[c - n] : dv[c];
-    real Du = coef[0], Dv = coef[1], A12 = coef[2], Bu = coef[3], Bv = coef[4];
-    real wl = coef[5], wr = coef[6], wd = coef[7], wu = coef[8];
-    real nu = ((wl * dul + wr * dur) + wd * dud) + wu * duu;
-    real num_u = (Bu + nu) - A12 * dv[c];
-    real du_k = Du != 0 ? num_u / Du : 0;
-    real dun = (1 - omega) * du[c] + omega * du_k;
+    int in_l = i > 1, in_r = i < n - 2, in_d = j < m - 2, in_u = j > 1;
+    real dul = in_l ? du[c - 1] : 0, dur = in_r ? du[c + 1] : 0;
+    real dud = in_d ? du[c + n] : 0, duu = in_u ? du[c - n] : 0;
+    real dvl = in_l ? dv[c - 1] : 0, dvr = in_r ? dv[c + 1] : 0;
+    real dvd = in_d ? dv[c + n] : 0, dvu = in_u ? dv[c - n] : 0;
+    real A12 = coef[0], Bu = coef[1], Bv = coef[2], wu = coef[3], wv = coef[4];
+    real wl = coef[5], wr = coef[6], wd = coef[7], wup = coef[8];
+    real b = (((in_l ? 0 : wl) + (in_r ? 0 : wr)) + (in_u ? 0 : wup)) + (in_d ? 0 : wd);
+    real ou = du[c], ov = dv[c];
+    const real om1 = 1 - OMEGA;
+    real nu, nv;
+    if (smooth_const) {
+        nu = fma(wl, dul + dur, Bu);
+        nu = fma(wd, dud + duu, nu);
+        nv = fma(wl, dvl + dvr, Bv);
+        nv = fma(wd, dvd + dvu, nv);
+    } else {
+        nu = fma(wup, duu, fma(wd, dud, fma(wr, dur, fma(wl, dul, Bu))));
+        nv = fma(wup, dvu, fma(wd, dvd, fma(wr, dvr, fma(wl, dvl, Bv))));
+    }
+    nu = fma(-A12, ov, nu);
+    real dun = fma(wu, nu, fma(wu, b, om1) * ou);
+    nv = fma(-A12, dun, nv);
     du[c] = dun;
-    real nv = ((wl * dvl + wr * dvr) + wd * dvd) + wu * dvu;
-    real num_v = (Bv + nv) - A12 * dun;
-    real dv_k = Dv != 0 ? num_v / Dv : 0;
-    dv[c] = (1 - omega) * dv[c] + omega * dv_k;
+    dv[c] = fma(wv, nv, fma(wv, b, om1) * ov);
 }
 
 /* core/level_solver.py:158-369 (compute_flow).
@@ -498,7 +519,7 @@ void or_compute_flow(const real *J11, const real *J22, const real *J33, const re
     real *coef = (real *)malloc(sizeof(real) * N * 9);
     for (size_t q = 0; q < N * K; ++q) psi[q] = 1;
     for (size_t q = 0; q < N; ++q) { psi_s[q] = 1; du[q] = 0; dv[q] = 0; }
-    const real omega = (real)1.95;
+    const real omega = OMEGA;
     real ax = (real)(alpha_x / (hx * hx)), ay = (real)(alpha_y / (hy * hy));
     int smooth_const = (a_smooth == 1.0);
     for (int it = 0; it < iterations; ++it) {
@@ -536,13 +557,13 @@ void or_compute_flow(const real *J11, const real *J22, const real *J33, const re
         } else if (order == 0) {
             for (int i = 1; i < n - 1; ++i)
                 for (int j = 1; j < m - 1; ++j)
-                    sor_pixel_coef(j, i, m, n, du, dv, coef + ((size_t)j * n + i) * 9, omega);
+                    sor_pixel_coef(j, i, m, n, du, dv, coef + ((size_t)j * n + i) * 9, smooth_const);
         } else {
             for (int colour = 0; colour < 2; ++colour)
                 for (int i = 1; i < n - 1; ++i)
                     for (int j = 1; j < m - 1; ++j)
                         if (((i + j) & 1) == colour)
-                            sor_pixel_coef(j, i, m, n, du, dv, coef + ((size_t)j * n + i) * 9, omega);
+                            sor_pixel_coef(j, i, m, n, du, dv, coef + ((size_t)j * n + i) * 9, smooth_const);
         }
     }
     free(psi); free(psi_s); free(coef);

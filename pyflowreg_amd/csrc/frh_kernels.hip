@@ -4,6 +4,8 @@
 // operation order (compiled with -ffp-contract=off, IEEE division and
 // square root) so that results are bit-identical to the CPU oracle given the
 // same inputs.  Reference citations are pyflowreg source paths.
+#include <utility>
+
 #include "frh_internal.h"
 #include "../../include/flowreg_detmath.h"
 
@@ -383,7 +385,8 @@ hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w
 // Lag-block coefficients of core/level_solver.py:229-255 + :262-351 for one
 // cell (ordering documented with the oracle's assemble_pixel):
 //   Du = (((wl+wr)+wd)+wu) + sum_k Wpsi_k J11_k,  Dv likewise with J22,
-//   A12 = sum_k Wpsi_k J12_k, Bu = cu - sum_k Wpsi_k J13_k, Bv = cv - ...J23.
+//   A12 = sum_k Wpsi_k J12_k, Bu = cu - sum_k Wpsi_k J13_k, Bv = cv - ...J23,
+//   stored as {A12, Bu, Bv, omega/Du, omega/Dv}.
 struct AData {
     double a[8];
     double am1[8];
@@ -426,12 +429,13 @@ __global__ void k_assemble(const double *__restrict__ J, const double *__restric
         Bu = Bu - wp * j13;
         Bv = Bv - wp * j23;
     }
+    const double omega = 1.95;
     double *co = coef + (size_t)t * fstride;
-    co[q] = Du;
-    co[plane + q] = Dv;
-    co[2 * plane + q] = A12;
-    co[3 * plane + q] = Bu;
-    co[4 * plane + q] = Bv;
+    co[q] = A12;
+    co[plane + q] = Bu;
+    co[2 * plane + q] = Bv;
+    co[3 * plane + q] = Du != 0 ? omega / Du : 0.0;
+    co[4 * plane + q] = Dv != 0 ? omega / Dv : 0.0;
 }
 
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
@@ -457,185 +461,265 @@ hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv
 // up and left neighbours already hold sweep-p values and its down and right
 // neighbours still hold sweep-(p-1) values, exactly as in the sequential
 // column-by-column loop, so the result is identical to that loop.
-// Neighbour values travel between lanes with wave shuffles and between
-// waves through LDS with one barrier per step.  Out-of-level neighbours take
-// the cell's own value from the start of the sweep (set_boundary_2d).
-template <int P>
-__device__ inline void sor_cell(double Du, double Dv, double A12, double Bu, double Bv, double ax,
-                                double ay, double lu, double ru, double du_, double uu,
-                                double lv, double rv, double dv_, double uv, double ou, double ov,
-                                double &nu_out, double &nv_out)
+//
+// Registers per lane: the coefficients of the last 2P-1 cells the lane
+// entered (a ring indexed at compile time: the step loop is unrolled over
+// one ring period), the last two results of every sweep, and the pass-start
+// values of the next cells (prefetched two steps ahead).  Row neighbours
+// move between lanes with DPP wave shifts (out-of-wave lanes read 0) and
+// between waves through LDS, with one barrier per step that waits for LDS
+// traffic only.  Values of cells outside the level are 0; the weight of an
+// out-of-level neighbour (which holds the cell's own start value in the
+// reference) is folded into the cell's own term, see sor_pixel_coef in the
+// oracle, which performs the same operations in the same order.
+__device__ inline double wave_from_prev_lane(double x)  // lane i <- lane i-1, lane 0 <- 0
 {
-    const double omega = 1.95;
-    double nu = ((ax * lu + ax * ru) + ay * du_) + ay * uu;
-    double num_u = (Bu + nu) - A12 * ov;
-    double duk = Du != 0 ? num_u / Du : 0.0;
-    double dun = (1 - omega) * ou + omega * duk;
-    double nv = ((ax * lv + ax * rv) + ay * dv_) + ay * uv;
-    double num_v = (Bv + nv) - A12 * dun;
-    double dvk = Dv != 0 ? num_v / Dv : 0.0;
-    nu_out = dun;
-    nv_out = (1 - omega) * ov + omega * dvk;
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x138, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x138, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
 }
 
-template <int P, int MAXT>
-__global__ __launch_bounds__(MAXT) void k_sweep(const double *__restrict__ coef,
-                                                double *__restrict__ duv,
-                                                double *__restrict__ ring, size_t fstride, int h,
-                                                int w, int hs, size_t plane, double ax, double ay,
-                                                int zero_init)
+__device__ inline double wave_from_next_lane(double x)  // lane i <- lane i+1, lane 63 <- 0
 {
-    const int f = blockIdx.x;
-    const int r = threadIdx.x;
-    const int lane = r & 63, wv = r >> 6, nw = blockDim.x >> 6;
-    const double *cb = coef + (size_t)f * fstride;
-    double *du = duv + (size_t)f * fstride;
-    double *dv = du + plane;
-    __shared__ double xb[2][MAXT / 64][2][P + 1][2];
-    for (int k = threadIdx.x; k < 2 * (MAXT / 64) * 2 * (P + 1) * 2; k += blockDim.x)
-        (&xb[0][0][0][0][0])[k] = 0.0;
-    const int D = h + w - 2;
+    int lo = __double2loint(x), hi = __double2hiint(x);
+    lo = __builtin_amdgcn_update_dpp(0, lo, 0x130, 0xf, 0xf, true);
+    hi = __builtin_amdgcn_update_dpp(0, hi, 0x130, 0xf, 0xf, true);
+    return __hiloint2double(hi, lo);
+}
+
+__device__ inline void lds_barrier()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+struct SweepArgs {
+    const double *coef;
+    double *duv;
+    double *ring;
+    size_t fstride, plane;
+    int h, w, hs;
+    double ax, ay;
+    int zero_init;
+};
+
+template <int P>
+struct SweepState {
+    static constexpr int NR = 2 * P - 1;
+    double K[NR][5];      // coefficients of diag t - k in slot (t - k) % NR
+    double Ru[2][P];      // results of step t-1 / t-2, slot by step parity
+    double Rv[2][P];
+    double nK[2][5];      // coefficient prefetch (diag t+2 loaded at step t)
+    double nLu[2], nLv[2];  // pass-start prefetch (diag t+3 loaded at step t)
+    double lpu, lpv;      // pass-start of diag t   (own start value, p = 0)
+    double ldu, ldv;      // pass-start of diag t+1 (right neighbour, p = 0)
+};
+
+template <int P, int MAXW, int S>
+__device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a, const double *cb,
+                                           double *du, double *dv, double *rdu, double *rdv,
+                                           double (*xb)[MAXW][2][P + 1][2], int t, int r,
+                                           int lane, int wv, int nw, int Tend)
+{
+    constexpr int NR = 2 * P - 1;
+    if (t > Tend) return;  // uniform
+    const int h = a.h, w = a.w, hs = a.hs;
+    const size_t plane = a.plane;
     const bool row_ok = r < h;
-    auto cell_ok = [&](int dd) { int cc = dd - r; return row_ok && cc >= 0 && cc < w; };
-
-    double K[2 * P - 1][5];
-    double pu[P], pv[P], qu[P], qv[P];
+    // prefetch: coefficients of diag t+2, pass-start values of diag t+3
+    {
+        const int d = t + 2, c = d - r;
+        double *k = st.nK[S & 1];
+        if (row_ok && c >= 0 && c < w) {
+            size_t q = (size_t)d * hs + r;
 #pragma unroll
-    for (int k = 0; k < 2 * P - 1; ++k)
+            for (int e = 0; e < 5; ++e) k[e] = cb[e * plane + q];
+        } else {
 #pragma unroll
-        for (int e = 0; e < 5; ++e) K[k][e] = 0.0;
-#pragma unroll
-    for (int p = 0; p < P; ++p) pu[p] = pv[p] = qu[p] = qv[p] = 0.0;
-    double nK[5] = {0, 0, 0, 0, 0};
-    if (cell_ok(0)) {
-        size_t q = (size_t)r;
-#pragma unroll
-        for (int e = 0; e < 5; ++e) nK[e] = cb[e * plane + q];
+            for (int e = 0; e < 5; ++e) k[e] = 0.0;
+        }
+        const int d3 = t + 3, c3 = d3 - r;
+        st.nLu[S & 1] = 0.0;
+        st.nLv[S & 1] = 0.0;
+        if (!a.zero_init && row_ok && c3 >= 0 && c3 < w) {
+            size_t q = (size_t)d3 * hs + r;
+            st.nLu[S & 1] = du[q];
+            st.nLv[S & 1] = dv[q];
+        }
     }
-    double lpu = 0, lpv = 0, ldu = 0, ldv = 0;
-    if (!zero_init && cell_ok(0)) { lpu = du[r]; lpv = dv[r]; }
-    if (!zero_init && cell_ok(1)) { ldu = du[(size_t)hs + r]; ldv = dv[(size_t)hs + r]; }
-    __syncthreads();
-
-    const int Tend = D + 2 * (P - 1);
-    for (int t = 0; t <= Tend; ++t) {
+    constexpr int PV = (S + 1) & 1;  // results of step t-1
+    constexpr int P2 = S & 1;        // results of step t-2 (overwritten below)
+    double upu[P], upv[P], dnu[P], dnv[P];
 #pragma unroll
-        for (int k = 2 * P - 2; k >= 1; --k)
-#pragma unroll
-            for (int e = 0; e < 5; ++e) K[k][e] = K[k - 1][e];
-#pragma unroll
-        for (int e = 0; e < 5; ++e) K[0][e] = nK[e];
-        if (cell_ok(t + 1)) {
-            size_t q = (size_t)(t + 1) * hs + r;
-#pragma unroll
-            for (int e = 0; e < 5; ++e) nK[e] = cb[e * plane + q];
-        }
-        double nlu = 0, nlv = 0;
-        if (!zero_init && cell_ok(t + 2)) {
-            size_t q = (size_t)(t + 2) * hs + r;
-            nlu = du[q];
-            nlv = dv[q];
-        }
-        double upu[P], upv[P], dnu[P], dnv[P];
+    for (int p = 0; p < P; ++p) {
+        upu[p] = wave_from_prev_lane(st.Ru[PV][p]);
+        upv[p] = wave_from_prev_lane(st.Rv[PV][p]);
+        dnu[p] = wave_from_next_lane(p == 0 ? st.ldu : st.Ru[PV][p > 0 ? p - 1 : 0]);
+        dnv[p] = wave_from_next_lane(p == 0 ? st.ldv : st.Rv[PV][p > 0 ? p - 1 : 0]);
+    }
+    const int pb = (t + 1) & 1;
+    if (lane == 0 && wv > 0) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            upu[p] = __shfl_up(pu[p], 1);
-            upv[p] = __shfl_up(pv[p], 1);
-            double su = p == 0 ? ldu : pu[p > 0 ? p - 1 : 0];
-            double sv = p == 0 ? ldv : pv[p > 0 ? p - 1 : 0];
-            dnu[p] = __shfl_down(su, 1);
-            dnv[p] = __shfl_down(sv, 1);
+            upu[p] = xb[pb][wv - 1][1][p][0];
+            upv[p] = xb[pb][wv - 1][1][p][1];
         }
-        const int pb = (t + 1) & 1;
-        if (lane == 0 && wv > 0) {
+    }
+    if (lane == 63 && wv < nw - 1) {
+        dnu[0] = xb[pb][wv + 1][0][P][0];
+        dnv[0] = xb[pb][wv + 1][0][P][1];
 #pragma unroll
-            for (int p = 0; p < P; ++p) {
-                upu[p] = xb[pb][wv - 1][1][p][0];
-                upv[p] = xb[pb][wv - 1][1][p][1];
-            }
+        for (int p = 1; p < P; ++p) {
+            dnu[p] = xb[pb][wv + 1][0][p - 1][0];
+            dnv[p] = xb[pb][wv + 1][0][p - 1][1];
         }
-        if (lane == 63 && wv < nw - 1) {
-            dnu[0] = xb[pb][wv + 1][0][P][0];
-            dnv[0] = xb[pb][wv + 1][0][P][1];
+    }
+    const double ax = a.ax, ay = a.ay, om1 = 1 - 1.95;
+    const int rlo = wv * 64, rhi = (rlo + 63 < h - 1 ? rlo + 63 : h - 1);
+    double cu[P], cv[P];
 #pragma unroll
-            for (int p = 1; p < P; ++p) {
-                dnu[p] = xb[pb][wv + 1][0][p - 1][0];
-                dnv[p] = xb[pb][wv + 1][0][p - 1][1];
-            }
-        }
-        double cu[P], cv[P];
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            int c = t - r - 2 * p;
-            cu[p] = 0.0;
-            cv[p] = 0.0;
-            if (row_ok && c >= 0 && c < w) {
-                double ou = p == 0 ? lpu : qu[p > 0 ? p - 1 : 0];
-                double ov = p == 0 ? lpv : qv[p > 0 ? p - 1 : 0];
-                double rsu = p == 0 ? ldu : pu[p > 0 ? p - 1 : 0];
-                double rsv = p == 0 ? ldv : pv[p > 0 ? p - 1 : 0];
-                double lu = c > 0 ? pu[p] : ou, lv = c > 0 ? pv[p] : ov;
-                double ru = c < w - 1 ? rsu : ou, rv = c < w - 1 ? rsv : ov;
-                double uu = r > 0 ? upu[p] : ou, uv = r > 0 ? upv[p] : ov;
-                double dd = r < h - 1 ? dnu[p] : ou, ddv = r < h - 1 ? dnv[p] : ov;
-                sor_cell<P>(K[2 * p][0], K[2 * p][1], K[2 * p][2], K[2 * p][3], K[2 * p][4], ax,
-                            ay, lu, ru, dd, uu, lv, rv, ddv, uv, ou, ov, cu[p], cv[p]);
-                if (p == P - 1) {
-                    size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
-                    du[q] = cu[p];
-                    dv[q] = cv[p];
-                    if (ring && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
-                        double *rb = ring + (size_t)f * fstride;
-                        rb[q] = ou;
-                        rb[plane + q] = ov;
-                    }
+    for (int p = 0; p < P; ++p) {
+        cu[p] = 0.0;
+        cv[p] = 0.0;
+        // uniform: does any lane of this wave own a cell of sweep p?
+        if (t - 2 * p - rlo >= 0 && t - 2 * p - rhi <= w - 1) {
+            const int c = t - r - 2 * p;
+            const double *Kp = st.K[((S - 2 * p) % NR + NR) % NR];
+            double ou = p == 0 ? st.lpu : st.Ru[P2][p > 0 ? p - 1 : 0];
+            double ov = p == 0 ? st.lpv : st.Rv[P2][p > 0 ? p - 1 : 0];
+            double ru = p == 0 ? st.ldu : st.Ru[PV][p > 0 ? p - 1 : 0];
+            double rv = p == 0 ? st.ldv : st.Rv[PV][p > 0 ? p - 1 : 0];
+            double lu = st.Ru[PV][p], lv = st.Rv[PV][p];
+            double b = (((c == 0 ? ax : 0.0) + (c == w - 1 ? ax : 0.0)) + (r == 0 ? ay : 0.0)) +
+                       (r == h - 1 ? ay : 0.0);
+            double A12 = Kp[0], Bu = Kp[1], Bv = Kp[2], wu = Kp[3], wvv = Kp[4];
+            double nu = fma(ax, lu + ru, Bu);
+            nu = fma(ay, dnu[p] + upu[p], nu);
+            nu = fma(-A12, ov, nu);
+            double dun = fma(wu, nu, fma(wu, b, om1) * ou);
+            double nv = fma(ax, lv + rv, Bv);
+            nv = fma(ay, dnv[p] + upv[p], nv);
+            nv = fma(-A12, dun, nv);
+            double dvn = fma(wvv, nv, fma(wvv, b, om1) * ov);
+            const bool ok = row_ok && c >= 0 && c < w;
+            cu[p] = ok ? dun : 0.0;
+            cv[p] = ok ? dvn : 0.0;
+            if (p == P - 1 && ok) {
+                size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
+                du[q] = dun;
+                dv[q] = dvn;
+                if (rdu && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
+                    rdu[q] = ou;
+                    rdv[q] = ov;
                 }
             }
         }
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        st.Ru[P2][p] = cu[p];
+        st.Rv[P2][p] = cv[p];
+    }
+    st.lpu = st.ldu;
+    st.lpv = st.ldv;
+    st.ldu = st.nLu[PV];
+    st.ldv = st.nLv[PV];
+    constexpr int NS = (S + 1) % NR;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) st.K[NS][e] = st.nK[PV][e];
+    const int wb = t & 1;
+    if (lane == 0) {
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            qu[p] = pu[p];
-            qv[p] = pv[p];
-            pu[p] = cu[p];
-            pv[p] = cv[p];
+            xb[wb][wv][0][p][0] = cu[p];
+            xb[wb][wv][0][p][1] = cv[p];
         }
-        lpu = ldu;
-        lpv = ldv;
-        ldu = nlu;
-        ldv = nlv;
-        const int wb = t & 1;
-        if (lane == 0) {
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                xb[wb][wv][0][p][0] = pu[p];
-                xb[wb][wv][0][p][1] = pv[p];
-            }
-            xb[wb][wv][0][P][0] = ldu;
-            xb[wb][wv][0][P][1] = ldv;
-        }
-        if (lane == 63) {
-#pragma unroll
-            for (int p = 0; p < P; ++p) {
-                xb[wb][wv][1][p][0] = pu[p];
-                xb[wb][wv][1][p][1] = pv[p];
-            }
-        }
-        __syncthreads();
+        xb[wb][wv][0][P][0] = st.ldu;
+        xb[wb][wv][0][P][1] = st.ldv;
     }
+    if (lane == 63) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            xb[wb][wv][1][p][0] = cu[p];
+            xb[wb][wv][1][p][1] = cv[p];
+        }
+    }
+    lds_barrier();
+    __builtin_amdgcn_sched_barrier(0);  // keep each step's loads within the step
+}
+
+template <int P, int MAXW, int... S>
+__device__ __forceinline__ void sweep_block(SweepState<P> &st, const SweepArgs &a,
+                                            const double *cb, double *du, double *dv, double *rdu,
+                                            double *rdv, double (*xb)[MAXW][2][P + 1][2], int t0,
+                                            int r, int lane, int wv, int nw, int Tend,
+                                            std::integer_sequence<int, S...>)
+{
+    (sweep_step<P, MAXW, S>(st, a, cb, du, dv, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
+}
+
+template <int P, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
+{
+    constexpr int NR = 2 * P - 1, U = 2 * NR, MAXW = MAXT / 64;
+    const int f = blockIdx.x;
+    const int r = threadIdx.x;
+    const int lane = r & 63;
+    const int wv = __builtin_amdgcn_readfirstlane(r >> 6);
+    const int nw = blockDim.x >> 6;
+    const double *cb = a.coef + (size_t)f * a.fstride;
+    double *du = a.duv + (size_t)f * a.fstride;
+    double *dv = du + a.plane;
+    double *rdu = a.ring ? a.ring + (size_t)f * a.fstride : nullptr;
+    double *rdv = a.ring ? rdu + a.plane : nullptr;
+    __shared__ double xb[2][MAXW][2][P + 1][2];
+    for (int k = threadIdx.x; k < 2 * MAXW * 2 * (P + 1) * 2; k += blockDim.x)
+        (&xb[0][0][0][0][0])[k] = 0.0;
+    const int h = a.h, w = a.w, hs = a.hs;
+    const bool row_ok = r < h;
+    SweepState<P> st;
+#pragma unroll
+    for (int k = 0; k < NR; ++k)
+#pragma unroll
+        for (int e = 0; e < 5; ++e) st.K[k][e] = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) st.Ru[0][p] = st.Ru[1][p] = st.Rv[0][p] = st.Rv[1][p] = 0.0;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) st.nK[0][e] = st.nK[1][e] = 0.0;
+    st.nLu[0] = st.nLu[1] = st.nLv[0] = st.nLv[1] = 0.0;
+    st.lpu = st.lpv = st.ldu = st.ldv = 0.0;
+    // prologue: diag 0 coefficients in slot 0, diag 1 in the prefetch slot of
+    // step -1; pass-start values of diags 0, 1 and 2.
+    auto cell_ok = [&](int d) { int c = d - r; return row_ok && c >= 0 && c < w; };
+    if (cell_ok(0)) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) st.K[0][e] = cb[e * a.plane + r];
+    }
+    if (cell_ok(1)) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) st.nK[1][e] = cb[e * a.plane + (size_t)hs + r];
+    }
+    if (!a.zero_init) {
+        if (cell_ok(0)) { st.lpu = du[r]; st.lpv = dv[r]; }
+        if (cell_ok(1)) { st.ldu = du[(size_t)hs + r]; st.ldv = dv[(size_t)hs + r]; }
+        if (cell_ok(2)) { st.nLu[1] = du[(size_t)2 * hs + r]; st.nLv[1] = dv[(size_t)2 * hs + r]; }
+    }
+    __syncthreads();
+    const int Tend = (h + w - 2) + 2 * (P - 1);
+    for (int t0 = 0; t0 <= Tend; t0 += U)
+        sweep_block<P, MAXW>(st, a, cb, du, dv, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
+                             std::make_integer_sequence<int, U>{});
 }
 
 template <int P>
-static hipError_t launch_sweep_p(const double *coef, double *duv, double *ring, size_t fstride,
-                                 int T, const Skew &s, double ax, double ay, int zero_init,
-                                 hipStream_t st)
+static hipError_t launch_sweep_p(const SweepArgs &a, int T, hipStream_t st)
 {
-    int threads = s.hs;
-    if (threads <= 512)
-        hipLaunchKernelGGL((k_sweep<P, 512>), dim3(T), dim3(threads), 0, st, coef, duv, ring,
-                           fstride, s.h, s.w, s.hs, s.plane, ax, ay, zero_init);
+    if (a.hs <= 512)
+        hipLaunchKernelGGL((k_sweep<P, 512>), dim3(T), dim3(a.hs), 0, st, a);
     else if (P <= 2)
-        hipLaunchKernelGGL((k_sweep<P <= 2 ? P : 2, 1024>), dim3(T), dim3(threads), 0, st, coef,
-                           duv, ring, fstride, s.h, s.w, s.hs, s.plane, ax, ay, zero_init);
+        hipLaunchKernelGGL((k_sweep<(P <= 2 ? P : 2), 1024>), dim3(T), dim3(a.hs), 0, st, a);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
@@ -644,12 +728,13 @@ static hipError_t launch_sweep_p(const double *coef, double *duv, double *ring, 
 hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
                         int T, const Skew &s, double ax, double ay, int zero_init, hipStream_t st)
 {
+    SweepArgs a{coef, duv, ring, frame_stride, s.plane, s.h, s.w, s.hs, ax, ay, zero_init};
     switch (P) {
-    case 1: return launch_sweep_p<1>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
-    case 2: return launch_sweep_p<2>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
-    case 3: return launch_sweep_p<3>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
-    case 4: return launch_sweep_p<4>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
-    case 5: return launch_sweep_p<5>(coef, duv, ring, frame_stride, T, s, ax, ay, zero_init, st);
+    case 1: return launch_sweep_p<1>(a, T, st);
+    case 2: return launch_sweep_p<2>(a, T, st);
+    case 3: return launch_sweep_p<3>(a, T, st);
+    case 4: return launch_sweep_p<4>(a, T, st);
+    case 5: return launch_sweep_p<5>(a, T, st);
     default: return hipErrorInvalidValue;
     }
 }
