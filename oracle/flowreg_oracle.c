@@ -479,7 +479,11 @@ static inline void sor_pixel_coef(int j, int i, int m, int n, real *du, real *dv
     real dvd = in_d ? dv[c + n] : 0, dvu = in_u ? dv[c - n] : 0;
     real A12 = coef[0], Bu = coef[1], Bv = coef[2], wu = coef[3], wv = coef[4];
     real wl = coef[5], wr = coef[6], wd = coef[7], wup = coef[8];
-    real b = (((in_l ? 0 : wl) + (in_r ? 0 : wr)) + (in_u ? 0 : wup)) + (in_d ? 0 : wd);
+    real b;
+    if (smooth_const) /* ax * (#left/right outside) + ay * (#up/down outside) */
+        b = fma(wl, (real)(!in_l + !in_r), wd * (real)(!in_u + !in_d));
+    else
+        b = (((in_l ? 0 : wl) + (in_r ? 0 : wr)) + (in_u ? 0 : wup)) + (in_d ? 0 : wd);
     real ou = du[c], ov = dv[c];
     const real om1 = 1 - OMEGA;
     real nu, nv;

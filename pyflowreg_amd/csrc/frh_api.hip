@@ -68,6 +68,15 @@ struct Pass {
     bool psi_one;   // psi = 1 (no refresh has happened yet)
 };
 
+// Passes can form their coefficients on entry (fused) when every lag block
+// fits in one pass, i.e. no pass continues a block started by another.
+bool passes_fusable(const std::vector<Pass> &ps)
+{
+    for (const Pass &p : ps)
+        if (!p.assemble) return false;
+    return true;
+}
+
 std::vector<Pass> plan_passes(int iterations, int lag, int pmax)
 {
     std::vector<int> starts;
@@ -254,6 +263,7 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
     double *J = work, *Wsk = J + 6 * K * s.plane, *cuv = Wsk + K * s.plane, *coef = cuv + 2 * s.plane;
     double *duv = coef + 5 * s.plane, *ring = duv + 2 * s.plane;
     double ax = alpha_x / (hx * hx), ay = alpha_y / (hy * hy);
+    FRH_HIP(launch_fill_f64(work, npl * s.plane, 0.0, st), "level solver clear");
     std::vector<double> am1(K);
     for (int k = 0; k < K; ++k) am1[k] = a_data[k] - 1.0;
     hipError_t e = hipSuccess;
@@ -264,13 +274,19 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
     std::vector<Pass> passes = plan_passes(iterations, update_lag, sweep_pmax(s.h));
     size_t fstride = npl * s.plane;
     if (passes.empty() && e == hipSuccess) e = launch_fill_f64(duv, 4 * s.plane, 0.0, st);
+    const bool fused = fused_sweep_supported(K) && passes_fusable(passes);
     for (size_t k = 0; k < passes.size() && e == hipSuccess; ++k) {
+        double *rg = k + 1 == passes.size() ? ring : nullptr;
+        if (fused) {
+            e = launch_sweep_fused(passes[k].nsweeps, J, Wsk, cuv, duv, rg, fstride, 1, K, s, ax, ay,
+                                   am1.data(), a_data, passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0, st);
+            continue;
+        }
         if (passes[k].assemble)
             e = launch_assemble(J, Wsk, cuv, duv, coef, fstride, 1, K, s, ax, ay, am1.data(), a_data,
                                 passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0, st);
         if (e == hipSuccess)
-            e = launch_sweep(passes[k].nsweeps, coef, duv, k + 1 == passes.size() ? ring : nullptr,
-                             fstride, 1, s, ax, ay, k == 0 ? 1 : 0, st);
+            e = launch_sweep(passes[k].nsweeps, coef, duv, rg, fstride, 1, s, ax, ay, k == 0 ? 1 : 0, st);
     }
     if (e == hipSuccess) e = launch_skew_to_padded_f64(duv, ring, s, du, dv, st);
     hipError_t e2 = hipFreeAsync(work, st);
@@ -304,6 +320,10 @@ struct frh_solver {
     float *f1lev = nullptr, *uvtop = nullptr, *tmp = nullptr, *f2l = nullptr, *f2w = nullptr;
     float *uvl = nullptr, *uvn = nullptr;
     double *Wlev = nullptr, *work = nullptr;
+    int Wsum = 0;                 // sum of level widths (multi-level horizontal pass)
+    std::vector<int> xoff;        // per level: column offset in the staged rows
+    int32_t *hoff = nullptr, *hix = nullptr;
+    float *hwx = nullptr, *pyr = nullptr;
     size_t npl = 0;
     bool have_ref = false;
     bool profile = false;
@@ -327,6 +347,7 @@ static void free_solver(frh_solver *s)
     }
     (void)hipFree(s->f1lev); (void)hipFree(s->uvtop); (void)hipFree(s->tmp); (void)hipFree(s->f2l); (void)hipFree(s->f2w);
     (void)hipFree(s->uvl); (void)hipFree(s->uvn); (void)hipFree(s->Wlev); (void)hipFree(s->work);
+    (void)hipFree(s->hoff); (void)hipFree(s->hix); (void)hipFree(s->hwx); (void)hipFree(s->pyr);
     for (auto e : s->pool) (void)hipEventDestroy(e);
     delete s;
 }
@@ -417,6 +438,25 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
         }
         s->lv.push_back(D);
     }
+    if (!rc) {
+        std::vector<int32_t> hoff(1, 0), hix;
+        std::vector<float> hwx;
+        for (int i = 0; i < d->n_levels; ++i) {
+            const frh_level_desc &L = d->levels[i];
+            s->xoff.push_back(s->Wsum);
+            s->Wsum += L.w;
+            for (int x = 0; x < L.w; ++x) {
+                for (int p = 0; p < L.P_img; ++p) {
+                    hix.push_back(L.img_ix[(size_t)x * L.P_img + p]);
+                    hwx.push_back(L.img_wx[(size_t)x * L.P_img + p]);
+                }
+                hoff.push_back((int32_t)hix.size());
+            }
+        }
+        rc = upload(&s->hoff, hoff.data(), hoff.size(), "pyramid tables");
+        if (!rc) rc = upload(&s->hix, hix.data(), hix.size(), "pyramid tables");
+        if (!rc) rc = upload(&s->hwx, hwx.data(), hwx.size(), "pyramid tables");
+    }
     size_t T = (size_t)d->max_batch, C = d->C, HW = (size_t)d->H * d->W;
     s->npl = 6 * C + 2 + 5 + 2;
     size_t cmax = C > 2 ? C : 2;
@@ -429,6 +469,7 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
     if (!rc) rc = dev_alloc(&s->uvl, T * 2 * (size_t)hmax * wmax, "level flow");
     if (!rc) rc = dev_alloc(&s->uvn, T * 2 * (size_t)hmax * wmax, "next flow");
     if (!rc) rc = dev_alloc(&s->work, T * s->npl * plane_max, "solver workspace");
+    if (!rc) rc = dev_alloc(&s->pyr, T * C * (size_t)d->H * s->Wsum, "pyramid rows");
     if (rc) {
         free_solver(s);
         return rc;
@@ -445,13 +486,20 @@ int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *wei
     FRH_REQ(s && fixed && weight, "frh_solver_set_reference: null pointer");
     hipStream_t st = as_stream(stream);
     int C = s->C;
+    FRH_HIP(launch_pyr_h(fixed, C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
+            "reference pyramid");
     for (int i = 0; i < s->nl; ++i) {
         LevelDev &L = s->lv[i];
-        FRH_HIP(launch_resize(fixed, C, s->H, s->W, s->f1lev + L.f1_off, L.h, L.w, L.img_ix, L.img_wx,
-                              L.img_iy, L.img_wy, L.P_img, s->tmp, st), "reference pyramid");
+        FRH_HIP(launch_pyr_v(s->pyr, C, s->H, s->Wsum, s->xoff[i], s->f1lev + L.f1_off, L.h, L.w,
+                             L.img_iy, L.img_wy, L.P_img, st), "reference pyramid");
+    }
+    FRH_HIP(launch_pyr_h(weight, C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
+            "weight pyramid");
+    for (int i = 0; i < s->nl; ++i) {
+        LevelDev &L = s->lv[i];
         float *wl = s->f2l;  // scratch for the resized weight planes
-        FRH_HIP(launch_resize(weight, C, s->H, s->W, wl, L.h, L.w, L.img_ix, L.img_wx, L.img_iy,
-                              L.img_wy, L.P_img, s->tmp, st), "weight pyramid");
+        FRH_HIP(launch_pyr_v(s->pyr, C, s->H, s->Wsum, s->xoff[i], wl, L.h, L.w, L.img_iy, L.img_wy,
+                             L.P_img, st), "weight pyramid");
         FRH_HIP(launch_to_skew_f32(wl, C, L.sk, s->Wlev + L.W_off, st), "weight skew");
     }
     LevelDev &T0 = s->lv[0];
@@ -479,6 +527,11 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         t0 = next_event(s);
         (void)hipEventRecord(t0, st);
     }
+    {
+        Scope sc(s, TM_RESIZE, st);
+        FRH_HIP(launch_pyr_h(moving, T * C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
+                "moving pyramid rows");
+    }
     for (int li = 0; li < s->nl; ++li) {
         LevelDev &L = s->lv[li];
         const int h = L.h, w = L.w;
@@ -488,8 +541,8 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         double *J = s->work, *cuv = J + 6 * C * sk.plane, *coef = cuv + 2 * sk.plane, *duv = coef + 5 * sk.plane;
         {
             Scope sc(s, TM_RESIZE, st);
-            FRH_HIP(launch_resize(moving, T * C, s->H, s->W, s->f2l, h, w, L.img_ix, L.img_wx, L.img_iy,
-                                  L.img_wy, L.P_img, s->tmp, st), "level images");
+            FRH_HIP(launch_pyr_v(s->pyr, T * C, s->H, s->Wsum, s->xoff[li], s->f2l, h, w, L.img_iy,
+                                 L.img_wy, L.P_img, st), "level images");
         }
         const float *f2src = s->f2l;
         if (li == 0) {
@@ -511,10 +564,22 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         {
             Scope sc(s, TM_TENSOR, st);
             FRH_HIP(launch_tensor(s->f1lev + L.f1_off, f2src, T, C, h, w, L.h_row, L.h_col, s->uvl,
-                                  s->uvl + hw, 2 * hw, L.axw, L.ayw, J, cuv, fs, sk, st), "motion tensor");
+                                  s->uvl + hw, 2 * hw, L.axw, L.ayw, J, cuv, fs, coef, sk, st),
+                    "motion tensor");
         }
+        // The separate assemble kernel is faster than forming coefficients
+        // inside the wavefront (measured on MI355X), so the pyramid solver
+        // uses it; frh_level_solver keeps the fused form.
+        const bool fused = false;
         for (size_t k = 0; k < L.passes.size(); ++k) {
             const Pass &p = L.passes[k];
+            if (fused) {
+                Scope sc(s, TM_SWEEP, st);
+                FRH_HIP(launch_sweep_fused(p.nsweeps, J, s->Wlev + L.W_off, cuv, duv, nullptr, fs, T, C, sk,
+                                           L.axw, L.ayw, s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0,
+                                           k == 0 ? 1 : 0, st), "sor sweep (fused)");
+                continue;
+            }
             if (p.assemble) {
                 Scope sc(s, TM_ASSEMBLE, st);
                 FRH_HIP(launch_assemble(J, s->Wlev + L.W_off, cuv, duv, coef, fs, T, C, sk, L.axw, L.ayw,

@@ -47,7 +47,7 @@ hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T
 hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
                          double hx, const float *u, const float *v, size_t uv_frame_stride,
                          double ax, double ay, double *J, double *cuv, size_t frame_stride,
-                         const Skew &s, hipStream_t st);
+                         double *pad, const Skew &s, hipStream_t st);
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
                            const double *duv, double *coef, size_t frame_stride, int T, int C,
                            const Skew &s, double ax, double ay, const double *a_data_minus1,
@@ -77,5 +77,16 @@ hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w
                                   double *J13, double *J23, hipStream_t st);
 hipError_t launch_cuv_padded(const double *u, const double *v, int m, int n, double ax,
                              double ay, const Skew &s, double *cuv, hipStream_t st);
+
+hipError_t launch_pyr_h(const float *src, int N, int H, int W, int Wsum, const int32_t *hoff,
+                        const int32_t *hix, const float *hwx, float *tmp, hipStream_t st);
+hipError_t launch_pyr_v(const float *tmp, int N, int H, int Wsum, int off, float *dst, int oh,
+                        int ow, const int32_t *iy, const float *wy, int P, hipStream_t st);
+
+bool fused_sweep_supported(int C);
+hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const double *cuv,
+                              double *duv, double *ring, size_t frame_stride, int T, int C,
+                              const Skew &s, double ax, double ay, const double *a_data_minus1,
+                              const double *a_data, int psi_one, int zero_init, hipStream_t st);
 
 }  // namespace frh

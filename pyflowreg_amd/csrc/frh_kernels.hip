@@ -20,6 +20,10 @@ __device__ inline int reflect_half(int j, int n)  // ...b a | a b...  (scipy "re
 
 __device__ inline int mirror_edge(int j, int n)  // ...c b | a b c...  (scipy "mirror")
 {
+    if (n >= 3 && j >= -(n - 1) && j <= 2 * n - 2) {  // one reflection suffices
+        j = j < 0 ? -j : j;
+        return j >= n ? 2 * n - 2 - j : j;
+    }
     if (n == 1) return 0;
     int period = 2 * n - 2;
     j %= period;
@@ -159,6 +163,89 @@ hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int 
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL(k_resize_v, dim3((ow + 63) / 64, oh, N), dim3(64), 0, st, tmp, h, dst, oh,
                        ow, iy, wy, P);
+    return hipGetLastError();
+}
+
+// Multi-level pyramid from full resolution (core/optical_flow.py:519-520:
+// every level is resized from the full-resolution frame).  Horizontal pass:
+// one work-group per (plane, block of R rows) stages the rows in LDS and
+// emits their output samples for every level at once (outputs of all levels
+// concatenated, Wsum = sum of level widths; per-output tap lists in hoff /
+// hix / hwx, each tap applied to the R rows so the tables are read once per
+// R rows).  Vertical pass: one launch per level over the staged rows.
+// Same float32 products and float64 sums as k_resize_h / k_resize_v.
+template <int R>
+__global__ void k_pyr_h(const float *__restrict__ src, int H, int W, int Wsum,
+                        const int32_t *__restrict__ hoff, const int32_t *__restrict__ hix,
+                        const float *__restrict__ hwx, float *__restrict__ tmp)
+{
+    extern __shared__ float rows[];  // R staged full-res rows
+    const int y0 = blockIdx.x * R, n = blockIdx.y;
+    const int nr = H - y0 < R ? H - y0 : R;
+    const float *s = src + ((size_t)n * H + y0) * W;
+    for (int i = threadIdx.x; i < nr * W; i += blockDim.x) rows[i] = s[i];
+    __syncthreads();
+    float *o = tmp + ((size_t)n * H + y0) * Wsum;
+    for (int x = threadIdx.x; x < Wsum; x += blockDim.x) {
+        const int b = hoff[x], e = hoff[x + 1];
+        double acc[R];
+#pragma unroll
+        for (int i = 0; i < R; ++i) acc[i] = 0.0;
+        for (int k = b; k < e; ++k) {
+            const int j = hix[k];
+            const float wt = hwx[k];
+#pragma unroll
+            for (int i = 0; i < R; ++i) {
+                float prod = rows[i * W + j] * wt;
+                acc[i] += (double)prod;
+            }
+        }
+#pragma unroll
+        for (int i = 0; i < R; ++i)
+            if (i < nr) o[(size_t)i * Wsum + x] = (float)acc[i];
+    }
+}
+
+__global__ void k_pyr_v(const float *__restrict__ tmp, int H, int Wsum, int off,
+                        float *__restrict__ dst, int oh, int ow, const int32_t *__restrict__ iy,
+                        const float *__restrict__ wy, int P)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int n = blockIdx.z;
+    if (x >= ow) return;
+    const float *col = tmp + (size_t)n * H * Wsum + off + x;
+    const int32_t *ii = iy + (size_t)y * P;
+    const float *ww = wy + (size_t)y * P;
+    double acc = 0.0;
+    for (int p = 0; p < P; ++p) {
+        float prod = col[(size_t)ii[p] * Wsum] * ww[p];
+        acc += (double)prod;
+    }
+    dst[((size_t)n * oh + y) * ow + x] = (float)acc;
+}
+
+hipError_t launch_pyr_h(const float *src, int N, int H, int W, int Wsum, const int32_t *hoff,
+                        const int32_t *hix, const float *hwx, float *tmp, hipStream_t st)
+{
+    if (W <= 1024) {
+        constexpr int R = 16;
+        hipLaunchKernelGGL(k_pyr_h<R>, dim3((H + R - 1) / R, N), dim3(256), sizeof(float) * R * W, st,
+                           src, H, W, Wsum, hoff, hix, hwx, tmp);
+    } else {
+        constexpr int R = 4;
+        hipLaunchKernelGGL(k_pyr_h<R>, dim3((H + R - 1) / R, N), dim3(256), sizeof(float) * R * W, st,
+                           src, H, W, Wsum, hoff, hix, hwx, tmp);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_pyr_v(const float *tmp, int N, int H, int Wsum, int off, float *dst, int oh,
+                        int ow, const int32_t *iy, const float *wy, int P, hipStream_t st)
+{
+    int bx = ow >= 64 ? 64 : (ow >= 32 ? 32 : 16);
+    hipLaunchKernelGGL(k_pyr_v, dim3((ow + bx - 1) / bx, oh, N), dim3(bx), 0, st, tmp, H, Wsum, off,
+                       dst, oh, ow, iy, wy, P);
     return hipGetLastError();
 }
 
@@ -318,12 +405,19 @@ __global__ void k_tensor(const float *__restrict__ f1, const float *__restrict__
                          int w, int hs, size_t plane, double hy, double hx,
                          const float *__restrict__ u, const float *__restrict__ v,
                          size_t uv_stride, double ax, double ay, double *__restrict__ J,
-                         double *__restrict__ cuv, size_t fstride)
+                         double *__restrict__ cuv, size_t fstride, double *__restrict__ pad)
 {
     int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
     int c = d - r;
-    if (r >= h || c < 0 || c >= w) return;
     size_t hw = (size_t)h * w, q = (size_t)d * hs + r;
+    if (r >= h || c < 0 || c >= w) {
+        // padding slot of the skewed layout: the SOR passes read it as a
+        // cell outside the level, whose coefficients and values are 0
+        double *pt = pad + (size_t)t * fstride;
+#pragma unroll
+        for (int e = 0; e < kCoefPlanes + 2; ++e) pt[(size_t)e * plane + q] = 0.0;
+        return;
+    }
     double *Jt = J + (size_t)t * fstride;
     for (int ch = 0; ch < C; ++ch) {
         TensorOut o = tensor_gc(f1 + ch * hw, f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx);
@@ -342,10 +436,10 @@ __global__ void k_tensor(const float *__restrict__ f1, const float *__restrict__
 hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
                          double hx, const float *u, const float *v, size_t uv_frame_stride,
                          double ax, double ay, double *J, double *cuv, size_t frame_stride,
-                         const Skew &s, hipStream_t st)
+                         double *pad, const Skew &s, hipStream_t st)
 {
     hipLaunchKernelGGL(k_tensor, dim3(h + w - 1, T), dim3(s.hs), 0, st, f1, f2, C, h, w, s.hs,
-                       s.plane, hy, hx, u, v, uv_frame_stride, ax, ay, J, cuv, frame_stride);
+                       s.plane, hy, hx, u, v, uv_frame_stride, ax, ay, J, cuv, frame_stride, pad);
     return hipGetLastError();
 }
 
@@ -392,10 +486,46 @@ struct AData {
     double am1[8];
 };
 
+// Coefficients of one cell from its raw data: J (6 components x C channels,
+// component-major), W (C), cu, cv, and the cell's start values (x, y).
+template <int C>
+__device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc, double cu,
+                                              double cv, double x, double y, double ax, double ay,
+                                              const AData &ad, int psi_one, double *K)
+{
+    double ws = ((ax + ax) + ay) + ay;
+    double Du = ws, Dv = ws, A12 = 0.0, Bu = cu, Bv = cv;
+#pragma unroll
+    for (int k = 0; k < C; ++k) {
+        double j11 = J[0 * C + k], j22 = J[1 * C + k], j33 = J[2 * C + k];
+        double j12 = J[3 * C + k], j13 = J[4 * C + k], j23 = J[5 * C + k];
+        double psi = 1.0;
+        if (!psi_one) {
+            double val = j11 * x * x + j22 * y * y + j23 * y + 2 * j12 * x * y + 2 * j13 * x +
+                         j23 * y + j33;
+            if (val < 0) val = 0;
+            psi = ad.a[k] * frh_det_pow(val + 1e-5, ad.am1[k]);
+        }
+        double wp = Wc[k] * psi;
+        Du = Du + wp * j11;
+        Dv = Dv + wp * j22;
+        A12 = A12 + wp * j12;
+        Bu = Bu - wp * j13;
+        Bv = Bv - wp * j23;
+    }
+    const double omega = 1.95;
+    K[0] = A12;
+    K[1] = Bu;
+    K[2] = Bv;
+    K[3] = Du != 0 ? omega / Du : 0.0;
+    K[4] = Dv != 0 ? omega / Dv : 0.0;
+}
+
+template <int C>
 __global__ void k_assemble(const double *__restrict__ J, const double *__restrict__ Wsk,
                            const double *__restrict__ cuv, const double *__restrict__ duv,
-                           double *__restrict__ coef, size_t fstride, int C, int h, int w,
-                           int hs, size_t plane, double ax, double ay, AData ad, int psi_one,
+                           double *__restrict__ coef, size_t fstride, int h, int w, int hs,
+                           size_t plane, double ax, double ay, AData ad, int psi_one,
                            int zero_init)
 {
     int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
@@ -409,33 +539,15 @@ __global__ void k_assemble(const double *__restrict__ J, const double *__restric
         x = duv[(size_t)t * fstride + q];
         y = duv[(size_t)t * fstride + plane + q];
     }
-    double ws = ((ax + ax) + ay) + ay;
-    double Du = ws, Dv = ws, A12 = 0.0, Bu = ct[q], Bv = ct[plane + q];
-    for (int k = 0; k < C; ++k) {
-        double j11 = Jt[(size_t)(0 * C + k) * plane + q], j22 = Jt[(size_t)(1 * C + k) * plane + q];
-        double j33 = Jt[(size_t)(2 * C + k) * plane + q], j12 = Jt[(size_t)(3 * C + k) * plane + q];
-        double j13 = Jt[(size_t)(4 * C + k) * plane + q], j23 = Jt[(size_t)(5 * C + k) * plane + q];
-        double psi = 1.0;
-        if (!psi_one) {
-            double val = j11 * x * x + j22 * y * y + j23 * y + 2 * j12 * x * y + 2 * j13 * x +
-                         j23 * y + j33;
-            if (val < 0) val = 0;
-            psi = ad.a[k] * frh_det_pow(val + 1e-5, ad.am1[k]);
-        }
-        double wp = Wsk[(size_t)k * plane + q] * psi;
-        Du = Du + wp * j11;
-        Dv = Dv + wp * j22;
-        A12 = A12 + wp * j12;
-        Bu = Bu - wp * j13;
-        Bv = Bv - wp * j23;
-    }
-    const double omega = 1.95;
+    double Jr[6 * C], Wr[C], K[5];
+#pragma unroll
+    for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
+#pragma unroll
+    for (int k = 0; k < C; ++k) Wr[k] = Wsk[(size_t)k * plane + q];
+    coef_from_raw<C>(Jr, Wr, ct[q], ct[plane + q], x, y, ax, ay, ad, psi_one, K);
     double *co = coef + (size_t)t * fstride;
-    co[q] = A12;
-    co[plane + q] = Bu;
-    co[2 * plane + q] = Bv;
-    co[3 * plane + q] = Du != 0 ? omega / Du : 0.0;
-    co[4 * plane + q] = Dv != 0 ? omega / Dv : 0.0;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) co[e * plane + q] = K[e];
 }
 
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
@@ -448,9 +560,22 @@ hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv
         ad.a[k] = a_data[k];
         ad.am1[k] = a_data_minus1[k];
     }
-    hipLaunchKernelGGL(k_assemble, dim3(s.h + s.w - 1, T), dim3(s.hs), 0, st, J, Wsk, cuv, duv,
-                       coef, frame_stride, C, s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one,
-                       zero_init);
+    dim3 g(s.h + s.w - 1, T), b(s.hs);
+#define FRH_ASM(CC)                                                                            \
+    hipLaunchKernelGGL(k_assemble<CC>, g, b, 0, st, J, Wsk, cuv, duv, coef, frame_stride, s.h, s.w, \
+                       s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
+    switch (C) {
+    case 1: FRH_ASM(1); break;
+    case 2: FRH_ASM(2); break;
+    case 3: FRH_ASM(3); break;
+    case 4: FRH_ASM(4); break;
+    case 5: FRH_ASM(5); break;
+    case 6: FRH_ASM(6); break;
+    case 7: FRH_ASM(7); break;
+    case 8: FRH_ASM(8); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef FRH_ASM
     return hipGetLastError();
 }
 
@@ -496,29 +621,51 @@ __device__ inline void lds_barrier()
 }
 
 struct SweepArgs {
-    const double *coef;
+    const double *coef;   // 5 coefficient planes (unfused passes)
     double *duv;
     double *ring;
     size_t fstride, plane;
     int h, w, hs;
     double ax, ay;
     int zero_init;
+    // fused passes: coefficients formed on entry from the raw planes
+    const double *J, *Wsk, *cuv;
+    AData ad;
+    int psi_one;
 };
 
-template <int P>
+// C = 0: the pass reads precomputed coefficients; C >= 1: the pass forms
+// each cell's coefficients (coef_from_raw, the same operations as
+// k_assemble) when the cell enters the wavefront, from its J / W / cu, cv
+// planes and start values.
+template <int P, int C>
 struct SweepState {
     static constexpr int NR = 2 * P - 1;
+    static constexpr int NRAW = C > 0 ? 7 * C + 2 : 1;
     double K[NR][5];      // coefficients of diag t - k in slot (t - k) % NR
     double Ru[2][P];      // results of step t-1 / t-2, slot by step parity
     double Rv[2][P];
-    double nK[2][5];      // coefficient prefetch (diag t+2 loaded at step t)
+    double nK[2][5];      // coefficient prefetch (diag t+2 loaded at step t), C = 0
+    double raw[NRAW];     // raw data of diag t+1 loaded at step t, C >= 1
     double nLu[2], nLv[2];  // pass-start prefetch (diag t+3 loaded at step t)
     double lpu, lpv;      // pass-start of diag t   (own start value, p = 0)
     double ldu, ldv;      // pass-start of diag t+1 (right neighbour, p = 0)
 };
 
-template <int P, int MAXW, int S>
-__device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a, const double *cb,
+template <int C>
+__device__ __forceinline__ void load_raw(double *raw, const SweepArgs &a, size_t fo, size_t q)
+{
+    const double *J = a.J + fo, *cuv = a.cuv + fo;
+#pragma unroll
+    for (int e = 0; e < 6 * C; ++e) raw[e] = J[(size_t)e * a.plane + q];
+#pragma unroll
+    for (int k = 0; k < C; ++k) raw[6 * C + k] = a.Wsk[(size_t)k * a.plane + q];
+    raw[7 * C] = cuv[q];
+    raw[7 * C + 1] = cuv[a.plane + q];
+}
+
+template <int P, int MAXW, int C, int S>
+__device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs &a, const double *cb,
                                            double *du, double *dv, double *rdu, double *rdv,
                                            double (*xb)[MAXW][2][P + 1][2], int t, int r,
                                            int lane, int wv, int nw, int Tend)
@@ -528,25 +675,48 @@ __device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a
     const int h = a.h, w = a.w, hs = a.hs;
     const size_t plane = a.plane;
     const bool row_ok = r < h;
-    // prefetch: coefficients of diag t+2, pass-start values of diag t+3
+    // prefetch: coefficients of diag t+2 (or raw data of diag t+1),
+    // pass-start values of diag t+3
     {
-        const int d = t + 2, c = d - r;
-        double *k = st.nK[S & 1];
-        if (row_ok && c >= 0 && c < w) {
-            size_t q = (size_t)d * hs + r;
+        if constexpr (C == 0) {
+            // padding slots hold zeros, so only the diagonal range is checked
+            const int d = t + 2;
+            double *k = st.nK[S & 1];
+            if (d <= h + w - 2) {
+                size_t q = (size_t)d * hs + r;
 #pragma unroll
-            for (int e = 0; e < 5; ++e) k[e] = cb[e * plane + q];
+                for (int e = 0; e < 5; ++e) k[e] = cb[e * plane + q];
+            } else {  // beyond the last diagonal: cells outside the level
+#pragma unroll
+                for (int e = 0; e < 5; ++e) k[e] = 0.0;
+            }
         } else {
+            const int d = t + 1, c = d - r;
+            if (row_ok && c >= 0 && c < w) {
+                load_raw<C>(st.raw, a, (size_t)blockIdx.x * a.fstride, (size_t)d * hs + r);
+            } else {
 #pragma unroll
-            for (int e = 0; e < 5; ++e) k[e] = 0.0;
+                for (int e = 0; e < 7 * C + 2; ++e) st.raw[e] = 0.0;
+            }
         }
         const int d3 = t + 3, c3 = d3 - r;
-        st.nLu[S & 1] = 0.0;
-        st.nLv[S & 1] = 0.0;
-        if (!a.zero_init && row_ok && c3 >= 0 && c3 < w) {
-            size_t q = (size_t)d3 * hs + r;
-            st.nLu[S & 1] = du[q];
-            st.nLv[S & 1] = dv[q];
+        if constexpr (C == 0) {
+            if (!a.zero_init && d3 <= h + w - 2) {
+                size_t q = (size_t)d3 * hs + r;
+                st.nLu[S & 1] = du[q];
+                st.nLv[S & 1] = dv[q];
+            } else {
+                st.nLu[S & 1] = 0.0;
+                st.nLv[S & 1] = 0.0;
+            }
+        } else {
+            st.nLu[S & 1] = 0.0;
+            st.nLv[S & 1] = 0.0;
+            if (!a.zero_init && row_ok && c3 >= 0 && c3 < w) {
+                size_t q = (size_t)d3 * hs + r;
+                st.nLu[S & 1] = du[q];
+                st.nLv[S & 1] = dv[q];
+            }
         }
     }
     constexpr int PV = (S + 1) & 1;  // results of step t-1
@@ -577,6 +747,8 @@ __device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a
         }
     }
     const double ax = a.ax, ay = a.ay, om1 = 1 - 1.95;
+    // out-of-level neighbour weight b = ax * (#left/right outside) + ay * (#up/down outside)
+    const double brow = ay * (double)((r == 0) + (r == h - 1));
     const int rlo = wv * 64, rhi = (rlo + 63 < h - 1 ? rlo + 63 : h - 1);
     double cu[P], cv[P];
 #pragma unroll
@@ -592,8 +764,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a
             double ru = p == 0 ? st.ldu : st.Ru[PV][p > 0 ? p - 1 : 0];
             double rv = p == 0 ? st.ldv : st.Rv[PV][p > 0 ? p - 1 : 0];
             double lu = st.Ru[PV][p], lv = st.Rv[PV][p];
-            double b = (((c == 0 ? ax : 0.0) + (c == w - 1 ? ax : 0.0)) + (r == 0 ? ay : 0.0)) +
-                       (r == h - 1 ? ay : 0.0);
+            double b = fma(ax, (double)((c == 0) + (c == w - 1)), brow);
             double A12 = Kp[0], Bu = Kp[1], Bv = Kp[2], wu = Kp[3], wvv = Kp[4];
             double nu = fma(ax, lu + ru, Bu);
             nu = fma(ay, dnu[p] + upu[p], nu);
@@ -604,16 +775,29 @@ __device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a
             nv = fma(-A12, dun, nv);
             double dvn = fma(wvv, nv, fma(wvv, b, om1) * ov);
             const bool ok = row_ok && c >= 0 && c < w;
-            cu[p] = ok ? dun : 0.0;
-            cv[p] = ok ? dvn : 0.0;
-            if (p == P - 1 && ok) {
-                size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
-                du[q] = dun;
-                dv[q] = dvn;
-                if (rdu && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
-                    rdu[q] = ou;
-                    rdv[q] = ov;
+            if constexpr (C == 0) {
+                // a padding cell has zero coefficients and a zero start value,
+                // so it computes 0 (up to sign) and can be stored as is
+                cu[p] = dun;
+                cv[p] = dvn;
+                if (p == P - 1 && t - 2 * (P - 1) >= 0) {
+                    size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
+                    du[q] = dun;
+                    dv[q] = dvn;
                 }
+            } else {
+                cu[p] = ok ? dun : 0.0;
+                cv[p] = ok ? dvn : 0.0;
+                if (p == P - 1 && ok) {
+                    size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
+                    du[q] = dun;
+                    dv[q] = dvn;
+                }
+            }
+            if (p == P - 1 && rdu && ok && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
+                size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
+                rdu[q] = ou;
+                rdv[q] = ov;
             }
         }
     }
@@ -622,13 +806,20 @@ __device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a
         st.Ru[P2][p] = cu[p];
         st.Rv[P2][p] = cv[p];
     }
+    constexpr int NS = (S + 1) % NR;
+    if constexpr (C == 0) {
+#pragma unroll
+        for (int e = 0; e < 5; ++e) st.K[NS][e] = st.nK[PV][e];
+    } else {
+        // coefficients of the cell entering at step t+1 (diag t+1), from its
+        // raw data and start values (ld = pass-start of diag t+1)
+        coef_from_raw<C>(st.raw, st.raw + 6 * C, st.raw[7 * C], st.raw[7 * C + 1], st.ldu, st.ldv,
+                         ax, ay, a.ad, a.psi_one, st.K[NS]);
+    }
     st.lpu = st.ldu;
     st.lpv = st.ldv;
     st.ldu = st.nLu[PV];
     st.ldv = st.nLv[PV];
-    constexpr int NS = (S + 1) % NR;
-#pragma unroll
-    for (int e = 0; e < 5; ++e) st.K[NS][e] = st.nK[PV][e];
     const int wb = t & 1;
     if (lane == 0) {
 #pragma unroll
@@ -650,17 +841,17 @@ __device__ __forceinline__ void sweep_step(SweepState<P> &st, const SweepArgs &a
     __builtin_amdgcn_sched_barrier(0);  // keep each step's loads within the step
 }
 
-template <int P, int MAXW, int... S>
-__device__ __forceinline__ void sweep_block(SweepState<P> &st, const SweepArgs &a,
+template <int P, int MAXW, int C, int... S>
+__device__ __forceinline__ void sweep_block(SweepState<P, C> &st, const SweepArgs &a,
                                             const double *cb, double *du, double *dv, double *rdu,
                                             double *rdv, double (*xb)[MAXW][2][P + 1][2], int t0,
                                             int r, int lane, int wv, int nw, int Tend,
                                             std::integer_sequence<int, S...>)
 {
-    (sweep_step<P, MAXW, S>(st, a, cb, du, dv, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
+    (sweep_step<P, MAXW, C, S>(st, a, cb, du, dv, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
 }
 
-template <int P, int MAXT>
+template <int P, int MAXT, int C>
 __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
 {
     constexpr int NR = 2 * P - 1, U = 2 * NR, MAXW = MAXT / 64;
@@ -679,7 +870,7 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
         (&xb[0][0][0][0][0])[k] = 0.0;
     const int h = a.h, w = a.w, hs = a.hs;
     const bool row_ok = r < h;
-    SweepState<P> st;
+    SweepState<P, C> st;
 #pragma unroll
     for (int k = 0; k < NR; ++k)
 #pragma unroll
@@ -693,76 +884,246 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     // prologue: diag 0 coefficients in slot 0, diag 1 in the prefetch slot of
     // step -1; pass-start values of diags 0, 1 and 2.
     auto cell_ok = [&](int d) { int c = d - r; return row_ok && c >= 0 && c < w; };
-    if (cell_ok(0)) {
 #pragma unroll
-        for (int e = 0; e < 5; ++e) st.K[0][e] = cb[e * a.plane + r];
-    }
-    if (cell_ok(1)) {
-#pragma unroll
-        for (int e = 0; e < 5; ++e) st.nK[1][e] = cb[e * a.plane + (size_t)hs + r];
-    }
+    for (int e = 0; e < SweepState<P, C>::NRAW; ++e) st.raw[e] = 0.0;
     if (!a.zero_init) {
         if (cell_ok(0)) { st.lpu = du[r]; st.lpv = dv[r]; }
         if (cell_ok(1)) { st.ldu = du[(size_t)hs + r]; st.ldv = dv[(size_t)hs + r]; }
         if (cell_ok(2)) { st.nLu[1] = du[(size_t)2 * hs + r]; st.nLv[1] = dv[(size_t)2 * hs + r]; }
     }
+    if constexpr (C == 0) {
+        if (cell_ok(0)) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) st.K[0][e] = cb[e * a.plane + r];
+        }
+        if (cell_ok(1)) {
+#pragma unroll
+            for (int e = 0; e < 5; ++e) st.nK[1][e] = cb[e * a.plane + (size_t)hs + r];
+        }
+    } else {
+        if (cell_ok(0)) {
+            double raw0[7 * C + 2];
+            load_raw<C>(raw0, a, (size_t)f * a.fstride, (size_t)r);
+            coef_from_raw<C>(raw0, raw0 + 6 * C, raw0[7 * C], raw0[7 * C + 1], st.lpu, st.lpv,
+                             a.ax, a.ay, a.ad, a.psi_one, st.K[0]);
+        }
+    }
     __syncthreads();
     const int Tend = (h + w - 2) + 2 * (P - 1);
     for (int t0 = 0; t0 <= Tend; t0 += U)
-        sweep_block<P, MAXW>(st, a, cb, du, dv, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
-                             std::make_integer_sequence<int, U>{});
+        sweep_block<P, MAXW, C>(st, a, cb, du, dv, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
+                                std::make_integer_sequence<int, U>{});
 }
 
-template <int P>
-static hipError_t launch_sweep_p(const SweepArgs &a, int T, hipStream_t st)
+template <int P, int C>
+static hipError_t launch_sweep_pc(const SweepArgs &a, int T, hipStream_t st)
 {
     if (a.hs <= 512)
-        hipLaunchKernelGGL((k_sweep<P, 512>), dim3(T), dim3(a.hs), 0, st, a);
+        hipLaunchKernelGGL((k_sweep<P, 512, C>), dim3(T), dim3(a.hs), 0, st, a);
     else if (P <= 2)
-        hipLaunchKernelGGL((k_sweep<(P <= 2 ? P : 2), 1024>), dim3(T), dim3(a.hs), 0, st, a);
+        hipLaunchKernelGGL((k_sweep<(P <= 2 ? P : 2), 1024, C>), dim3(T), dim3(a.hs), 0, st, a);
     else
         return hipErrorInvalidValue;
     return hipGetLastError();
 }
 
+template <int P>
+static hipError_t launch_sweep_p(const SweepArgs &a, int C, int T, hipStream_t st)
+{
+    switch (C) {
+    case 0: return launch_sweep_pc<P, 0>(a, T, st);
+    case 1: return launch_sweep_pc<P, 1>(a, T, st);
+    case 2: return launch_sweep_pc<P, 2>(a, T, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
 hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
                         int T, const Skew &s, double ax, double ay, int zero_init, hipStream_t st)
 {
-    SweepArgs a{coef, duv, ring, frame_stride, s.plane, s.h, s.w, s.hs, ax, ay, zero_init};
+    SweepArgs a{};
+    a.coef = coef; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
+    a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
     switch (P) {
-    case 1: return launch_sweep_p<1>(a, T, st);
-    case 2: return launch_sweep_p<2>(a, T, st);
-    case 3: return launch_sweep_p<3>(a, T, st);
-    case 4: return launch_sweep_p<4>(a, T, st);
-    case 5: return launch_sweep_p<5>(a, T, st);
+    case 1: return launch_sweep_p<1>(a, 0, T, st);
+    case 2: return launch_sweep_p<2>(a, 0, T, st);
+    case 3: return launch_sweep_p<3>(a, 0, T, st);
+    case 4: return launch_sweep_p<4>(a, 0, T, st);
+    case 5: return launch_sweep_p<5>(a, 0, T, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+bool fused_sweep_supported(int C) { return C == 1 || C == 2; }
+
+hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const double *cuv,
+                              double *duv, double *ring, size_t frame_stride, int T, int C,
+                              const Skew &s, double ax, double ay, const double *a_data_minus1,
+                              const double *a_data, int psi_one, int zero_init, hipStream_t st)
+{
+    SweepArgs a{};
+    a.coef = nullptr; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
+    a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
+    a.J = J; a.Wsk = Wsk; a.cuv = cuv; a.psi_one = psi_one;
+    for (int k = 0; k < C && k < 8; ++k) {
+        a.ad.a[k] = a_data[k];
+        a.ad.am1[k] = a_data_minus1[k];
+    }
+    switch (P) {
+    case 1: return launch_sweep_p<1>(a, C, T, st);
+    case 2: return launch_sweep_p<2>(a, C, T, st);
+    case 3: return launch_sweep_p<3>(a, C, T, st);
+    case 4: return launch_sweep_p<4>(a, C, T, st);
+    case 5: return launch_sweep_p<5>(a, C, T, st);
     default: return hipErrorInvalidValue;
     }
 }
 
 // ------------------------------------------------------- median + u += du
 // scipy.ndimage.median_filter(size=(5,5), mode="mirror") (core/optical_flow
-// .py:607-609), then u = u + du (:610-611).  The median is the 13th element
-// of Batcher's odd-even merge sort of the 25-sample window.
-__device__ inline void cswap(double &a, double &b)
+// .py:607-609), then u = u + du (:610-611).  The median is element 12 of
+// Batcher's odd-even merge sort of the 25 window samples; only the 113
+// comparators that can reach output 12 are kept, and of those only the
+// min or max that is used later (202 min/max operations).  Selection is
+// exact, so the result equals scipy's.
+struct MedCmp {
+    int a, b, lo, hi;
+};
+static constexpr MedCmp kMed25[] = {
+    {0, 1, 1, 1},
+    {2, 3, 1, 1},
+    {4, 5, 1, 1},
+    {6, 7, 1, 1},
+    {8, 9, 1, 1},
+    {10, 11, 1, 1},
+    {12, 13, 1, 1},
+    {14, 15, 1, 1},
+    {16, 17, 1, 1},
+    {18, 19, 1, 1},
+    {20, 21, 1, 1},
+    {22, 23, 1, 1},
+    {0, 2, 1, 1},
+    {1, 3, 1, 1},
+    {4, 6, 1, 1},
+    {5, 7, 1, 1},
+    {8, 10, 1, 1},
+    {9, 11, 1, 1},
+    {12, 14, 1, 1},
+    {13, 15, 1, 1},
+    {16, 18, 1, 1},
+    {17, 19, 1, 1},
+    {20, 22, 1, 1},
+    {21, 23, 1, 1},
+    {1, 2, 1, 1},
+    {5, 6, 1, 1},
+    {9, 10, 1, 1},
+    {13, 14, 1, 1},
+    {17, 18, 1, 1},
+    {21, 22, 1, 1},
+    {0, 4, 1, 1},
+    {1, 5, 1, 1},
+    {2, 6, 1, 1},
+    {3, 7, 1, 1},
+    {8, 12, 1, 1},
+    {9, 13, 1, 1},
+    {10, 14, 1, 1},
+    {11, 15, 1, 1},
+    {16, 20, 1, 1},
+    {17, 21, 1, 1},
+    {18, 22, 1, 1},
+    {19, 23, 1, 1},
+    {2, 4, 1, 1},
+    {3, 5, 1, 1},
+    {10, 12, 1, 1},
+    {11, 13, 1, 1},
+    {18, 20, 1, 1},
+    {19, 21, 1, 1},
+    {1, 2, 1, 1},
+    {3, 4, 1, 1},
+    {5, 6, 1, 1},
+    {9, 10, 1, 1},
+    {11, 12, 1, 1},
+    {13, 14, 1, 1},
+    {17, 18, 1, 1},
+    {19, 20, 1, 1},
+    {21, 22, 1, 1},
+    {0, 8, 1, 1},
+    {1, 9, 1, 1},
+    {2, 10, 1, 1},
+    {3, 11, 1, 1},
+    {4, 12, 1, 1},
+    {5, 13, 1, 1},
+    {6, 14, 1, 1},
+    {7, 15, 1, 0},
+    {16, 24, 1, 1},
+    {4, 8, 1, 1},
+    {5, 9, 1, 1},
+    {6, 10, 1, 1},
+    {7, 11, 1, 1},
+    {20, 24, 1, 1},
+    {2, 4, 1, 1},
+    {3, 5, 1, 1},
+    {6, 8, 1, 1},
+    {7, 9, 1, 1},
+    {10, 12, 1, 1},
+    {11, 13, 1, 1},
+    {18, 20, 1, 1},
+    {19, 21, 1, 1},
+    {22, 24, 1, 1},
+    {1, 2, 1, 1},
+    {3, 4, 1, 1},
+    {5, 6, 1, 1},
+    {7, 8, 1, 1},
+    {9, 10, 1, 1},
+    {11, 12, 1, 1},
+    {13, 14, 1, 0},
+    {17, 18, 1, 1},
+    {19, 20, 1, 1},
+    {21, 22, 1, 1},
+    {23, 24, 1, 1},
+    {0, 16, 0, 1},
+    {1, 17, 0, 1},
+    {2, 18, 0, 1},
+    {3, 19, 0, 1},
+    {4, 20, 0, 1},
+    {5, 21, 0, 1},
+    {6, 22, 1, 0},
+    {7, 23, 1, 0},
+    {8, 24, 1, 0},
+    {8, 16, 0, 1},
+    {9, 17, 0, 1},
+    {10, 18, 1, 0},
+    {11, 19, 1, 0},
+    {12, 20, 1, 0},
+    {13, 21, 1, 0},
+    {6, 10, 0, 1},
+    {7, 11, 0, 1},
+    {12, 16, 1, 0},
+    {13, 17, 1, 0},
+    {10, 12, 0, 1},
+    {11, 13, 1, 0},
+    {11, 12, 0, 1}
+};
+
+template <int I>
+__device__ __forceinline__ void med25_step(double (&v)[25])
 {
-    double lo = a < b ? a : b, hi = a < b ? b : a;
-    a = lo;
-    b = hi;
+    constexpr MedCmp c = kMed25[I];
+    double x = v[c.a], y = v[c.b];
+    if constexpr (c.lo != 0) v[c.a] = __builtin_fmin(x, y);
+    if constexpr (c.hi != 0) v[c.b] = __builtin_fmax(x, y);
+}
+
+template <int... I>
+__device__ __forceinline__ double median25_net(double (&v)[25], std::integer_sequence<int, I...>)
+{
+    (med25_step<I>(v), ...);
+    return v[12];
 }
 
 __device__ inline double median25(double (&a)[25])
 {
-#pragma unroll
-    for (int p = 1; p < 25; p += p)
-#pragma unroll
-        for (int k = p; k > 0; k /= 2)
-#pragma unroll
-            for (int j = k % p; j + k < 25; j += 2 * k)
-#pragma unroll
-                for (int i = 0; i < k; ++i)
-                    if (i + j + k < 25 && (i + j) / (2 * p) == (i + j + k) / (2 * p))
-                        cswap(a[i + j], a[i + j + k]);
-    return a[12];
+    return median25_net(a, std::make_integer_sequence<int, sizeof(kMed25) / sizeof(MedCmp)>{});
 }
 
 __global__ void k_median_add(const double *__restrict__ duv, size_t fstride, int h, int w,
@@ -776,17 +1137,20 @@ __global__ void k_median_add(const double *__restrict__ duv, size_t fstride, int
     double mu, mv;
     if (do_median) {
         double a[25], b[25];
-        int k = 0;
-        for (int dy = -2; dy <= 2; ++dy) {
-            int rr = mirror_edge(r + dy, h);
-            for (int dx = -2; dx <= 2; ++dx) {
-                int cc = mirror_edge(c + dx, w);
-                size_t q = (size_t)(rr + cc) * hs + rr;
-                a[k] = du[q];
-                b[k] = dv[q];
-                ++k;
-            }
+        int rr[5], cc[5];
+#pragma unroll
+        for (int k = 0; k < 5; ++k) {
+            rr[k] = mirror_edge(r + k - 2, h);
+            cc[k] = mirror_edge(c + k - 2, w);
         }
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                size_t q = (size_t)(rr[i] + cc[j]) * hs + rr[i];
+                a[i * 5 + j] = du[q];
+                b[i * 5 + j] = dv[q];
+            }
         mu = median25(a);
         mv = median25(b);
     } else {
@@ -831,10 +1195,16 @@ __global__ void k_median_plane(const double *__restrict__ src, double *__restric
     if (c >= w) return;
     const double *f = src + (size_t)n * h * w;
     double a[25];
-    int k = 0;
-    for (int dy = -2; dy <= 2; ++dy)
-        for (int dx = -2; dx <= 2; ++dx)
-            a[k++] = f[(size_t)mirror_edge(r + dy, h) * w + mirror_edge(c + dx, w)];
+    int rr[5], cc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        rr[k] = mirror_edge(r + k - 2, h);
+        cc[k] = mirror_edge(c + k - 2, w);
+    }
+#pragma unroll
+    for (int i = 0; i < 5; ++i)
+#pragma unroll
+        for (int j = 0; j < 5; ++j) a[i * 5 + j] = f[(size_t)rr[i] * w + cc[j]];
     dst[((size_t)n * h + r) * w + c] = median25(a);
 }
 

@@ -73,7 +73,11 @@ def _level_inputs(oracle, rng, h, w, K):
 
 @pytest.mark.parametrize("h,w,K,its,lag", [(12, 15, 1, 50, 5), (40, 33, 2, 50, 5),
                                           (70, 130, 2, 12, 5), (9, 9, 1, 7, 3), (20, 11, 2, 3, 1),
-                                          (130, 70, 1, 6, 2)])
+                                          (130, 70, 1, 6, 2),
+                                          # lag blocks split over passes, 3 channels and
+                                          # more than 512 rows take the unfused path
+                                          (30, 25, 2, 16, 7), (17, 21, 3, 10, 5),
+                                          (530, 12, 1, 7, 5)])
 def test_level_solver_bit_exact(gpu, oracle, h, w, K, its, lag):
     rng = _rng(4)
     J, W, u, v = _level_inputs(oracle, rng, h, w, K)
