@@ -105,6 +105,12 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33,
                      const double *a_data, double a_smooth, double hx, double hy, double *du,
                      double *dv, void *stream);
 
+/* cv2.resize(w, (ow, oh), interpolation=cv2.INTER_CUBIC) of float64 flows
+ * (core/optical_flow.py:615-616, only when min_level > 0), restated from
+ * OpenCV's public behaviour.  src: (T, h, w, C), dst: (T, oh, ow, C). */
+int frh_resize_cubic_cv(const double *src, int T, int h, int w, int C, double *dst, int oh,
+                        int ow, void *stream);
+
 /* scipy.ndimage.median_filter(size=(5, 5), mode="mirror") as called at
  * core/optical_flow.py:607-609.  f: (N, h, w) float64 planes, in place. */
 int frh_median5_mirror(double *f, int N, int h, int w, void *stream);

@@ -71,9 +71,9 @@ def solve_with_level_solver(fixed, moving, alpha, update_lag, iterations, min_le
             dv[1:-1, 1:-1] = of.median5_mirror(dv[1:-1, 1:-1])
         u = u + du
         v = v + dv
-    if min_level > 0:
-        raise ValueError("flowreg_hip: min_level > 0 (final cv2.resize) is not implemented")
     w = np.zeros((u.shape[0] - 2, u.shape[1] - 2, 2))
     w[:, :, 0] = u[1:-1, 1:-1]
     w[:, :, 1] = v[1:-1, 1:-1]
+    if min_level > 0:
+        w = of.cv_resize_cubic(w, m, n)
     return w

@@ -62,6 +62,8 @@ SIGNATURES = {
                                                   c_int, c_void_p, c_double, c_double, c_double,
                                                   c_void_p, c_void_p, c_void_p]),
     "frh_median5_mirror": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "frh_resize_cubic_cv": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
+                                    c_void_p]),
     "frh_solver_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(SolverDesc)]),
     "frh_solver_destroy": (None, [c_void_p]),
     "frh_solver_set_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

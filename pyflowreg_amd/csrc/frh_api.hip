@@ -227,6 +227,21 @@ int frh_motion_tensor_gc(const float *f1, const float *f2, int h, int w, double 
     return FRH_OK;
 }
 
+int frh_resize_cubic_cv(const double *src, int T, int h, int w, int C, double *dst, int oh, int ow,
+                        void *stream)
+{
+    FRH_REQ(src && dst && T > 0 && h > 0 && w > 0 && C > 0 && oh > 0 && ow > 0, "frh_resize_cubic_cv: bad arguments");
+    FRH_REQ(T <= 65535 && oh <= 65535 && h <= 65535, "frh_resize_cubic_cv: grid limits exceeded");
+    hipStream_t st = as_stream(stream);
+    double *tmp = nullptr;
+    FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)T * h * ow * C, st), "cubic resize scratch");
+    hipError_t e = launch_cv_resize(src, T, h, w, C, tmp, dst, oh, ow, st);
+    hipError_t e2 = hipFreeAsync(tmp, st);
+    FRH_HIP(e, "cubic resize kernels");
+    FRH_HIP(e2, "cubic resize free");
+    return FRH_OK;
+}
+
 int frh_median5_mirror(double *f, int N, int h, int w, void *stream)
 {
     FRH_REQ(f && N > 0 && h > 0 && w > 0 && h <= 65535 && N <= 65535, "frh_median5_mirror: bad arguments");
@@ -324,6 +339,7 @@ struct frh_solver {
     std::vector<int> xoff;        // per level: column offset in the staged rows
     int32_t *hoff = nullptr, *hix = nullptr;
     float *hwx = nullptr, *pyr = nullptr;
+    double *wlow = nullptr, *wtmp = nullptr;  // min_level > 0: coarse flow, resize scratch
     size_t npl = 0;
     bool have_ref = false;
     bool profile = false;
@@ -348,6 +364,7 @@ static void free_solver(frh_solver *s)
     (void)hipFree(s->f1lev); (void)hipFree(s->uvtop); (void)hipFree(s->tmp); (void)hipFree(s->f2l); (void)hipFree(s->f2w);
     (void)hipFree(s->uvl); (void)hipFree(s->uvn); (void)hipFree(s->Wlev); (void)hipFree(s->work);
     (void)hipFree(s->hoff); (void)hipFree(s->hix); (void)hipFree(s->hwx); (void)hipFree(s->pyr);
+    (void)hipFree(s->wlow); (void)hipFree(s->wtmp);
     for (auto e : s->pool) (void)hipEventDestroy(e);
     delete s;
 }
@@ -470,6 +487,10 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
     if (!rc) rc = dev_alloc(&s->uvn, T * 2 * (size_t)hmax * wmax, "next flow");
     if (!rc) rc = dev_alloc(&s->work, T * s->npl * plane_max, "solver workspace");
     if (!rc) rc = dev_alloc(&s->pyr, T * C * (size_t)d->H * s->Wsum, "pyramid rows");
+    if (!rc && (hmax != d->H || wmax != d->W)) {
+        rc = dev_alloc(&s->wlow, T * 2 * (size_t)hmax * wmax, "coarse flow");
+        if (!rc) rc = dev_alloc(&s->wtmp, T * 2 * (size_t)hmax * d->W, "final resize scratch");
+    }
     if (rc) {
         free_solver(s);
         return rc;
@@ -593,7 +614,18 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         {
             Scope sc(s, TM_MEDIAN, st);
             const int med = (h < w ? h : w) > 5;
-            if (li + 1 == s->nl) {
+            if (li + 1 == s->nl && s->wlow) {
+                // min_level > 0: float64 flow at the finest computed level, then
+                // cv2.resize INTER_CUBIC to full resolution
+                FRH_HIP(launch_median_add(duv, fs, sk, T, med, s->uvl, s->uvl + hw, 2 * hw, nullptr, s->wlow,
+                                          1, st), "median/add (coarse output)");
+                if (out_f64) {
+                    FRH_HIP(launch_cv_resize(s->wlow, T, h, w, 2, s->wtmp, (double *)w_out, s->H, s->W, st),
+                            "final cubic resize");
+                } else {
+                    return fail(FRH_EUNSUP, "min_level > 0 needs float64 output");
+                }
+            } else if (li + 1 == s->nl) {
                 FRH_HIP(launch_median_add(duv, fs, sk, T, med, s->uvl, s->uvl + hw, 2 * hw, (float *)w_out,
                                           out_f64 ? (double *)w_out : nullptr, 1, st), "median/add (output)");
             } else {

@@ -89,4 +89,7 @@ hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const d
                               const Skew &s, double ax, double ay, const double *a_data_minus1,
                               const double *a_data, int psi_one, int zero_init, hipStream_t st);
 
+hipError_t launch_cv_resize(const double *src, int T, int h, int w, int C, double *tmp,
+                            double *dst, int oh, int ow, hipStream_t st);
+
 }  // namespace frh

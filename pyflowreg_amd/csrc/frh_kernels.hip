@@ -1217,6 +1217,59 @@ hipError_t launch_median_plane(double *f, int N, int h, int w, double *tmp, hipS
     return hipGetLastError();
 }
 
+// ------------------------------------------------- final cubic upsampling
+// cv2.resize(w, (n, m), interpolation=INTER_CUBIC) on the float64 flow when
+// min_level > 0 (core/optical_flow.py:615-616), restated from OpenCV's
+// public behaviour: source coordinate computed in float, Keys cubic
+// coefficients (A = -0.75) in float, replicated borders, horizontal pass then
+// vertical pass, float64 sums.
+__global__ void k_cv_resize_h(const double *__restrict__ src, int T, int h, int w, int C,
+                              double *__restrict__ tmp, int ow, double scale_x)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, t = blockIdx.z;
+    if (x >= ow) return;
+    float f = (float)((x + 0.5) * scale_x - 0.5);
+    int ix = (int)floorf(f);
+    float cf[4];
+    cv_cubic_coeffs(f - (float)ix, cf);
+    const double *row = src + ((size_t)t * h + y) * w * C;
+    for (int c = 0; c < C; ++c) {
+        double s = 0;
+        for (int k = 0; k < 4; ++k) s += row[(size_t)clampi(ix - 1 + k, 0, w - 1) * C + c] * (double)cf[k];
+        tmp[(((size_t)t * h + y) * ow + x) * C + c] = s;
+    }
+}
+
+__global__ void k_cv_resize_v(const double *__restrict__ tmp, int T, int h, int ow, int C,
+                              double *__restrict__ dst, int oh, double scale_y)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, t = blockIdx.z;
+    if (x >= ow) return;
+    float f = (float)((y + 0.5) * scale_y - 0.5);
+    int iy = (int)floorf(f);
+    float cf[4];
+    cv_cubic_coeffs(f - (float)iy, cf);
+    for (int c = 0; c < C; ++c) {
+        double s = 0;
+        for (int k = 0; k < 4; ++k)
+            s += tmp[(((size_t)t * h + clampi(iy - 1 + k, 0, h - 1)) * ow + x) * C + c] * (double)cf[k];
+        dst[(((size_t)t * oh + y) * ow + x) * C + c] = s;
+    }
+}
+
+hipError_t launch_cv_resize(const double *src, int T, int h, int w, int C, double *tmp,
+                            double *dst, int oh, int ow, hipStream_t st)
+{
+    double sx = 1.0 / ((double)ow / w), sy = 1.0 / ((double)oh / h);
+    hipLaunchKernelGGL(k_cv_resize_h, dim3((ow + 63) / 64, h, T), dim3(64), 0, st, src, T, h, w, C, tmp,
+                       ow, sx);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(k_cv_resize_v, dim3((ow + 63) / 64, oh, T), dim3(64), 0, st, tmp, T, h, ow, C, dst,
+                       oh, sy);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- helpers
 __global__ void k_fill_f64(double *p, size_t n, double v)
 {

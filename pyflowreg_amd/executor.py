@@ -1,0 +1,173 @@
+"""Batched GPU executor with pyflowreg's executor contract.
+
+``HipBatchedExecutor.process_batch`` follows
+motion_correction/parallelization/base.py:76-110 and the sequential
+executor's semantics (parallelization/sequential.py:27-169): every frame of
+the batch is registered against ``reference_proc`` starting from ``w_init``,
+then the raw frame is warped onto ``reference_raw`` with the flow.  Instead of
+one ``get_displacement`` call per frame, all frames of the batch are solved
+together by one ``DisplacementSolver`` run (one launch per kernel and pyramid
+level for the whole batch), and the registration warp is one launch.
+"""
+
+from __future__ import annotations
+
+from typing import Callable, Optional, Tuple
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib, ptr, stream_handle
+from .optical_flow import DisplacementSolver, _resolve_const_assumption
+
+_EXECUTORS = {}
+
+
+def register_parallelization_executor(name: str, cls) -> None:
+    """Mirror of RuntimeContext.register_parallelization_executor
+    (pyflowreg/_runtime.py:151-171) for environments without pyflowreg."""
+    _EXECUTORS[name] = cls
+
+
+def get_parallelization_executor(name: str):
+    return _EXECUTORS[name]
+
+
+def _device():
+    if not torch.cuda.is_available():
+        raise _lib.FlowregHipError("flowreg_hip needs a ROCm GPU; none is visible")
+    return torch.device("cuda", torch.cuda.current_device())
+
+
+def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor) -> torch.Tensor:
+    """core/warping.py:160-234 (cubic) for a batch on the GPU.
+    frames (T, H, W, C) float32, flow (T, H, W, 2), fill (H, W, C) float32
+    -> (T, H, W, C) float32."""
+    T, H, W, C = frames.shape
+    f2 = frames.permute(0, 3, 1, 2).contiguous()
+    f1 = fill.permute(2, 0, 1).contiguous()
+    u = flow[..., 0].to(torch.float32).contiguous()
+    v = flow[..., 1].to(torch.float32).contiguous()
+    out = torch.empty_like(f2)
+    check(lib().frh_warp_cubic(ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u), ptr(v), 1.0, 1.0,
+                               ptr(out), stream_handle()), "frh_warp_cubic")
+    return out.permute(0, 2, 3, 1)
+
+
+def compute_batch_valid_masks(w) -> np.ndarray:
+    """core/warping.py:14-60,128-157: uint8 masks (0/255) of pixels whose
+    backward-warp position x + u, y + v stays inside the frame."""
+    t = torch.as_tensor(w, device=_device())
+    T, H, W = t.shape[:3]
+    gy = torch.arange(H, device=t.device, dtype=torch.float64)[None, :, None]
+    gx = torch.arange(W, device=t.device, dtype=torch.float64)[None, None, :]
+    mx = gx + t[..., 0].double()
+    my = gy + t[..., 1].double()
+    valid = (mx >= 0) & (mx < W) & (my >= 0) & (my < H)
+    return (valid.to(torch.uint8) * 255).cpu().numpy()
+
+
+class HipBatchedExecutor:
+    """Executor 'hip_batched': whole batches per GPU launch."""
+
+    name = "hip_batched"
+
+    def __init__(self, n_workers: Optional[int] = None, max_batch: int = 256):
+        self.n_workers = 1
+        self.max_batch = int(max_batch)
+        self._key = None
+        self._solver = None
+
+    @classmethod
+    def register(cls):
+        register_parallelization_executor(cls.name, cls)
+        try:  # pyflowreg present: register there too
+            from pyflowreg._runtime import RuntimeContext  # type: ignore
+            RuntimeContext.register_parallelization_executor(cls.name, cls)
+        except Exception:
+            pass
+
+    def setup(self):
+        _device()
+
+    def cleanup(self):
+        if self._solver is not None:
+            self._solver.close()
+        self._solver = None
+        self._key = None
+
+    def __enter__(self):
+        self.setup()
+        return self
+
+    def __exit__(self, exc_type, exc, tb):
+        self.cleanup()
+        return False
+
+    def get_info(self) -> dict:
+        return {"name": self.name, "n_workers": 1, "parallel": True,
+                "description": "GPU batched frame registration (flowreg_hip)"}
+
+    def _get_solver(self, H, W, C, T, fp):
+        alpha = fp.get("alpha", (2, 2))
+        if np.isscalar(alpha):
+            alpha = (alpha, alpha)
+        a_data = fp.get("a_data", 0.45)
+        key = (H, W, C, tuple(float(a) for a in alpha), int(fp.get("update_lag", 5)),
+               int(fp.get("iterations", 50)), int(fp.get("min_level", 0)),
+               int(fp.get("levels", 50)), float(fp.get("eta", 0.8)),
+               float(fp.get("a_smooth", 1.0)),
+               tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist()),
+               min(T, self.max_batch))
+        if key != self._key:
+            self.cleanup()
+            self._solver = DisplacementSolver(
+                H, W, C, alpha=key[3], update_lag=key[4], iterations=key[5], min_level=key[6],
+                levels=key[7], eta=key[8], a_smooth=key[9], a_data=np.array(key[10]),
+                max_batch=key[11])
+            self._key = key
+        return self._solver
+
+    def process_batch(self, batch: np.ndarray, batch_proc: np.ndarray, reference_raw: np.ndarray,
+                      reference_proc: np.ndarray, w_init: np.ndarray,
+                      get_displacement_func: Optional[Callable] = None,
+                      imregister_func: Optional[Callable] = None,
+                      interpolation_method: str = "cubic",
+                      progress_callback: Optional[Callable[[int], None]] = None,
+                      **kwargs) -> Tuple[np.ndarray, np.ndarray]:
+        """parallelization/base.py:76-110.  Returns (registered (T, H, W, C)
+        in batch's dtype, flow (T, H, W, 2) float32)."""
+        flow_params_all = dict(kwargs.get("flow_params", {}))
+        if bool(flow_params_all.get("cc_initialization", False)):
+            raise ValueError("hip_batched: cc_initialization is not implemented")
+        fp = {k: v for k, v in flow_params_all.items()
+              if k not in ("cc_initialization", "cc_hw", "cc_up")}
+        if fp.get("gnc_schedule") is not None:
+            raise ValueError("hip_batched: GNC schedules are not implemented")
+        if _resolve_const_assumption(fp.get("const_assumption", "gc")) != "gc":
+            raise ValueError("hip_batched implements the 'gc' data term only")
+        if str(interpolation_method).lower() != "cubic":
+            raise ValueError("hip_batched: only cubic registration warps are implemented")
+        as4 = lambda x: x[..., None] if x.ndim == 3 else x  # noqa: E731
+        as3 = lambda x: x[..., None] if x.ndim == 2 else x  # noqa: E731
+        batch = np.ascontiguousarray(as4(np.asarray(batch)))
+        bp = np.ascontiguousarray(as4(np.asarray(batch_proc)).astype(np.float32, copy=False))
+        rr = np.ascontiguousarray(as3(np.asarray(reference_raw)).astype(np.float32, copy=False))
+        rp = np.ascontiguousarray(as3(np.asarray(reference_proc)).astype(np.float32, copy=False))
+        wi = np.ascontiguousarray(np.asarray(w_init).astype(np.float32, copy=False))
+        T, H, W, C = batch.shape
+        dev = _device()
+        solver = self._get_solver(H, W, C, T, fp)
+        solver.set_reference(rp, fp.get("weight"), wi)
+        flow = solver.run(torch.from_numpy(bp).to(dev), out_dtype=torch.float32)
+        reg = warp_frames(torch.from_numpy(batch.astype(np.float32)).to(dev), flow,
+                          torch.from_numpy(rr).to(dev))
+        registered = reg.cpu().numpy().astype(batch.dtype, copy=False)
+        flows = flow.cpu().numpy()
+        if progress_callback is not None:
+            progress_callback(T)
+        return registered, flows
+
+
+HipBatchedExecutor.register()

@@ -198,8 +198,6 @@ class DisplacementSolver:
         self.max_batch = int(max_batch)
         self.iterations, self.update_lag = int(iterations), int(update_lag)
         self.plan = pyramid_plan(self.H, self.W, float(eta), int(levels), int(min_level), alpha)
-        if self.plan.min_level > 0:
-            raise ValueError("flowreg_hip: min_level > 0 (final cv2.resize) is not implemented")
         if float(a_smooth) != 1.0:
             raise ValueError("flowreg_hip: a_smooth != 1 is not implemented")
         ad = np.broadcast_to(np.asarray(a_data, dtype=np.float64), (self.C,))
@@ -273,13 +271,16 @@ class DisplacementSolver:
         if planes.shape[1:] != (self.C, self.H, self.W):
             raise ValueError(f"moving frames have shape {tuple(planes.shape[1:])}, expected "
                              f"{(self.C, self.H, self.W)}")
-        out = torch.empty((T, self.H, self.W, 2), dtype=out_dtype, device=planes.device)
+        f64 = out_dtype == torch.float64 or self.plan.min_level > 0
+        out = torch.empty((T, self.H, self.W, 2), dtype=torch.float64 if f64 else out_dtype,
+                          device=planes.device)
         check(lib().frh_solver_profile(self._h, 1 if profile else 0), "frh_solver_profile")
         for s in range(0, T, self.max_batch):
             e = min(T, s + self.max_batch)
             check(lib().frh_solver_run(self._h, ptr(planes[s:e]), e - s, ptr(out[s:e]),
-                                       1 if out_dtype == torch.float64 else 0, stream_handle()),
+                                       1 if f64 else 0, stream_handle()),
                   "frh_solver_run")
+        out = out.to(out_dtype)
         return out[0] if single else out
 
     def timings(self):
@@ -442,3 +443,15 @@ def median5_mirror(f):
     h, w = t.shape
     check(lib().frh_median5_mirror(ptr(t), 1, h, w, stream_handle()), "frh_median5_mirror")
     return t.cpu().numpy()
+
+
+def cv_resize_cubic(w, m, n):
+    """cv2.resize(w, (n, m), interpolation=cv2.INTER_CUBIC) for an (h, w, C)
+    float64 flow (core/optical_flow.py:615-616) on the GPU."""
+    dev = _device()
+    src = torch.as_tensor(np.ascontiguousarray(w, dtype=np.float64), device=dev)
+    h, ww, C = src.shape
+    dst = torch.empty((m, n, C), dtype=torch.float64, device=dev)
+    check(lib().frh_resize_cubic_cv(ptr(src), 1, h, ww, C, ptr(dst), m, n, stream_handle()),
+          "frh_resize_cubic_cv")
+    return dst.cpu().numpy()

@@ -1,0 +1,122 @@
+"""GPU pipeline pieces against the oracle: the batched executor, the
+compensate_arr batch loop (w_init from 22 frames, mean of the last 20),
+min_level > 0 (final cv2.resize), registration warps and valid masks."""
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _video(T, H, W, C, seed):
+    from pyflowreg_amd import synth
+    frames, _ = synth.video(T, H, W, n_channels=C, max_disp=2.0, sigma_d=H / 5, seed=seed)
+    return frames.numpy()
+
+
+def _oracle_batch(oracle, batch, ref_raw, ref_proc, w_init, fp):
+    """parallelization/sequential.py:97-169 with the oracle."""
+    bp = oracle.preprocess(batch, ref_raw).astype(np.float32)
+    flows, regs = [], []
+    for t in range(batch.shape[0]):
+        f = oracle.get_displacement(ref_proc, bp[t], uv=w_init.astype(np.float32).copy(),
+                                    **fp).astype(np.float32)
+        reg = oracle.imregister_wrapper(batch[t].astype(np.float32), f[..., 0].astype(np.float64),
+                                        f[..., 1].astype(np.float64), ref_raw.astype(np.float32))
+        flows.append(f)
+        regs.append(reg)
+    return np.stack(regs), np.stack(flows)
+
+
+def test_executor_matches_sequential_oracle(gpu, oracle):
+    from pyflowreg_amd.executor import HipBatchedExecutor
+    v = _video(5, 48, 40, 2, 21)
+    ref_raw = v[0].astype(np.float64)
+    ref_proc = oracle.preprocess(ref_raw).astype(np.float32)
+    batch = v[1:]
+    w_init = np.zeros((48, 40, 2), np.float32)
+    w_init[..., 0] = 0.25
+    fp = dict(alpha=(2.0, 2.0), weight=np.ones((48, 40, 2)) * 0.5, levels=50, min_level=0,
+              eta=0.8, iterations=50, update_lag=5, a_data=0.45, a_smooth=1.0)
+    ex = HipBatchedExecutor(max_batch=3)
+    bp = oracle.preprocess(batch, ref_raw).astype(np.float32)
+    reg, w = ex.process_batch(batch, bp, ref_raw, ref_proc, w_init, flow_params=fp)
+    oreg, ow = _oracle_batch(oracle, batch, ref_raw, ref_proc, w_init, fp)
+    np.testing.assert_array_equal(w, ow)
+    np.testing.assert_array_equal(reg, oreg)
+    assert reg.dtype == batch.dtype and w.dtype == np.float32
+
+
+def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle):
+    """compensate_recording.py:645-678,786-806 with buffer_size 10 over 26
+    frames: w_init from the first min(22, 10) frames, then the mean of each
+    batch's last (up to) 20 flows."""
+    from pyflowreg_amd.compensate import RegistrationOptions, compensate_arr
+    v = _video(27, 40, 32, 2, 31)
+    ref = v[0].astype(np.float64)
+    video = v[1:]
+    opt = RegistrationOptions(alpha=(2.0, 2.0), buffer_size=10, sigma=(1.0, 1.0, 0.1))
+    reg, w = compensate_arr(video, ref, opt)
+    # oracle restatement of the batch loop
+    ref_proc = oracle.preprocess(ref).astype(np.float32)
+    fp = dict(alpha=(2.0, 2.0), weight=np.ones((40, 32, 2)) * 0.5, levels=100, min_level=0,
+              eta=0.8, iterations=50, update_lag=5, a_data=0.45, a_smooth=1.0)
+    w_init = None
+    oregs, ows = [], []
+    for s in range(0, video.shape[0], 10):
+        b = video[s:s + 10]
+        if w_init is None:
+            _, w0 = _oracle_batch(oracle, b[:10], ref, ref_proc, np.zeros((40, 32, 2)), fp)
+            w_init = np.mean(w0, axis=0)
+        r, ww = _oracle_batch(oracle, b, ref, ref_proc, w_init, fp)
+        w_init = np.mean(ww[-20:], axis=0) if ww.shape[0] > 20 else np.mean(ww, axis=0)
+        oregs.append(r)
+        ows.append(ww)
+    np.testing.assert_array_equal(w, np.concatenate(ows))
+    np.testing.assert_array_equal(reg, np.concatenate(oregs))
+
+
+def test_min_level_above_zero_matches_oracle(gpu, oracle):
+    """quality 'balanced' (min_level 4): the flow stops at level 4 and is
+    brought to full resolution with cv2.resize INTER_CUBIC."""
+    import pyflowreg_amd as fr
+    v = _video(2, 80, 64, 1, 41)
+    ref = oracle.preprocess(v[0].astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(v[1], v[0].astype(np.float64)).astype(np.float32)
+    a = fr.get_displacement(ref, mov, min_level=4)
+    b = oracle.get_displacement(ref, mov, min_level=4)
+    assert a.shape == (80, 64, 2)
+    np.testing.assert_array_equal(a, b)
+
+
+def test_hooked_level_solver_path_matches(gpu, oracle):
+    """get_displacement(level_solver_backend=...) runs the reference's level
+    loop with the caller's solver; with the oracle's solver it equals the
+    oracle end to end."""
+    import pyflowreg_amd as fr
+    v = _video(2, 40, 48, 2, 51)
+    ref = oracle.preprocess(v[0].astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(v[1], v[0].astype(np.float64)).astype(np.float32)
+    calls = []
+
+    def solver(*args):
+        calls.append(args[0].shape)
+        return oracle.level_solver(*args)
+    a = fr.get_displacement(ref, mov, weight=[0.5, 0.5], level_solver_backend=solver)
+    b = oracle.get_displacement(ref, mov, weight=[0.5, 0.5])
+    assert len(calls) == len(oracle.level_plan(40, 48, 0.8, 50, 0)[0])
+    np.testing.assert_array_equal(a, b)
+
+
+def test_valid_masks():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.fail("gpu test without a GPU")
+    from pyflowreg_amd.executor import compute_batch_valid_masks
+    w = np.zeros((2, 10, 12, 2), np.float32)
+    w[0, ..., 0] = 2.5
+    w[1, ..., 1] = -3.0
+    m = compute_batch_valid_masks(w)
+    assert m.dtype == np.uint8
+    assert np.all(m[0][:, :9] == 255) and np.all(m[0][:, 10:] == 0)
+    assert np.all(m[1][:3] == 0) and np.all(m[1][3:] == 255)
