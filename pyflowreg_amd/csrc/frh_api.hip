@@ -585,7 +585,7 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         {
             Scope sc(s, TM_TENSOR, st);
             FRH_HIP(launch_tensor(s->f1lev + L.f1_off, f2src, T, C, h, w, L.h_row, L.h_col, s->uvl,
-                                  s->uvl + hw, 2 * hw, L.axw, L.ayw, J, cuv, fs, coef, sk, st),
+                                  s->uvl + hw, 2 * hw, L.axw, L.ayw, J, cuv, fs, sk, st),
                     "motion tensor");
         }
         // The separate assemble kernel is faster than forming coefficients

@@ -11,7 +11,8 @@ namespace frh {
 // solver kernels: cell (r, c) of an h x w interior lives at
 // (r + c) * hs + r, hs = 64 * ceil(h / 64).  At wavefront step t the SOR
 // kernel's lane r touches cell (r, t - r), so every load and store of a step
-// is one contiguous run of lanes.
+// is one contiguous run of lanes.  Padding slots (diagonal d, row r with
+// r >= h or d - r outside [0, w)) are never read or written.
 struct Skew {
     int h, w, hs;
     size_t plane;  // elements per plane: (h + w - 1) * hs
@@ -47,7 +48,7 @@ hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T
 hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
                          double hx, const float *u, const float *v, size_t uv_frame_stride,
                          double ax, double ay, double *J, double *cuv, size_t frame_stride,
-                         double *pad, const Skew &s, hipStream_t st);
+                         const Skew &s, hipStream_t st);
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
                            const double *duv, double *coef, size_t frame_stride, int T, int C,
                            const Skew &s, double ax, double ay, const double *a_data_minus1,

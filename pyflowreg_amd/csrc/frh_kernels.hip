@@ -18,6 +18,17 @@ __device__ inline int reflect_half(int j, int n)  // ...b a | a b...  (scipy "re
     return j;
 }
 
+// Work-groups are dispatched to the 8 XCDs of an MI355X round-robin (group b
+// runs on XCD b % 8).  Renumbering them so that each XCD receives one
+// contiguous range of the nb logical groups keeps neighbouring tiles in the
+// same L2.  A bijection of [0, nb).
+__device__ inline unsigned xcd_block(unsigned b, unsigned nb)
+{
+    const unsigned X = 8, per = nb / X, rem = nb % X;
+    const unsigned x = b % X, k = b / X;
+    return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
+}
+
 __device__ inline int mirror_edge(int j, int n)  // ...c b | a b c...  (scipy "mirror")
 {
     if (n >= 3 && j >= -(n - 1) && j <= 2 * n - 2) {  // one reflection suffices
@@ -399,25 +410,28 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
     return s;
 }
 
-// One thread per skewed slot (d, r); grid (h + w - 1, T), block hs.
+// Tiles of kTensorRows rows x 64 columns, one thread per cell (lanes run
+// along a row, so the image reads are coalesced).  A tile's 16 rows fill
+// whole 128-byte lines of each skewed plane on its inner diagonals.  Tiles
+// are numbered frame-major, then row-major, and dealt to the XCDs in
+// contiguous ranges (xcd_block) so neighbouring tiles share one L2.
 // J planes (component-major, then channel) and cu, cv are written skewed.
-__global__ void k_tensor(const float *__restrict__ f1, const float *__restrict__ f2, int C, int h,
-                         int w, int hs, size_t plane, double hy, double hx,
-                         const float *__restrict__ u, const float *__restrict__ v,
-                         size_t uv_stride, double ax, double ay, double *__restrict__ J,
-                         double *__restrict__ cuv, size_t fstride, double *__restrict__ pad)
+constexpr int kTensorRows = 16;
+
+__global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
+    const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
+    size_t plane, double hy, double hx, const float *__restrict__ u, const float *__restrict__ v,
+    size_t uv_stride, double ax, double ay, double *__restrict__ J, double *__restrict__ cuv,
+    size_t fstride, int T)
 {
-    int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
-    int c = d - r;
-    size_t hw = (size_t)h * w, q = (size_t)d * hs + r;
-    if (r >= h || c < 0 || c >= w) {
-        // padding slot of the skewed layout: the SOR passes read it as a
-        // cell outside the level, whose coefficients and values are 0
-        double *pt = pad + (size_t)t * fstride;
-#pragma unroll
-        for (int e = 0; e < kCoefPlanes + 2; ++e) pt[(size_t)e * plane + q] = 0.0;
-        return;
-    }
+    const int tiles_x = (w + 63) / 64, tiles_y = (h + kTensorRows - 1) / kTensorRows;
+    const unsigned nb = (unsigned)tiles_x * tiles_y * T;
+    const unsigned L = xcd_block(blockIdx.x, nb);
+    const int t = L / (tiles_x * tiles_y), rem = L % (tiles_x * tiles_y);
+    const int r = (rem / tiles_x) * kTensorRows + threadIdx.y;
+    const int c = (rem % tiles_x) * 64 + threadIdx.x;
+    if (r >= h || c >= w) return;
+    size_t hw = (size_t)h * w, q = (size_t)(r + c) * hs + r;
     double *Jt = J + (size_t)t * fstride;
     for (int ch = 0; ch < C; ++ch) {
         TensorOut o = tensor_gc(f1 + ch * hw, f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx);
@@ -436,10 +450,13 @@ __global__ void k_tensor(const float *__restrict__ f1, const float *__restrict__
 hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
                          double hx, const float *u, const float *v, size_t uv_frame_stride,
                          double ax, double ay, double *J, double *cuv, size_t frame_stride,
-                         double *pad, const Skew &s, hipStream_t st)
+                         const Skew &s, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tensor, dim3(h + w - 1, T), dim3(s.hs), 0, st, f1, f2, C, h, w, s.hs,
-                       s.plane, hy, hx, u, v, uv_frame_stride, ax, ay, J, cuv, frame_stride, pad);
+    const size_t nb = (size_t)((w + 63) / 64) * ((h + kTensorRows - 1) / kTensorRows) * T;
+    if (nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_tensor, dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, C, h, w,
+                       s.hs, s.plane, hy, hx, u, v, uv_frame_stride, ax, ay, J, cuv, frame_stride,
+                       T);
     return hipGetLastError();
 }
 
@@ -638,18 +655,38 @@ struct SweepArgs {
 // each cell's coefficients (coef_from_raw, the same operations as
 // k_assemble) when the cell enters the wavefront, from its J / W / cu, cv
 // planes and start values.
+//
+// Register rings.  The coefficients of diagonal x live in K[x % NK] and the
+// pass-start values of diagonal x in Lu/Lv[x % NL].  With C = 0 the
+// coefficients of diagonal t + DK and the start values of diagonal t + DL
+// are loaded at step t straight into their ring slots, so a load has DK
+// (DL - 2 for the value handed across waves) steps to arrive before it is
+// first used and no register copy forces an earlier wait.  The step loop is
+// unrolled over U = lcm(NK, NL) steps so every slot index is static.
+constexpr int sweep_gcd(int a, int b) { return b == 0 ? a : sweep_gcd(b, a % b); }
+constexpr int sweep_lcm(int a, int b) { return a / sweep_gcd(a, b) * b; }
+constexpr int kSweepNL = 6, kSweepDL = 5;
+// smallest ring size >= nmin whose unroll period lcm(n, NL) is at most 18
+constexpr int sweep_ring(int nmin)
+{
+    int n = nmin;
+    while (sweep_lcm(n, kSweepNL) > 18) ++n;
+    return n;
+}
+
 template <int P, int C>
 struct SweepState {
-    static constexpr int NR = 2 * P - 1;
+    static constexpr int NR = 2 * P - 1;  // diagonals a lane's P sweeps span
+    static constexpr int NK = sweep_ring(C == 0 ? NR + 3 : NR);
+    static constexpr int DK = NK - NR;    // coefficient prefetch distance (C = 0)
+    static constexpr int NL = kSweepNL, DL = kSweepDL;
+    static constexpr int U = sweep_lcm(NK, NL);
     static constexpr int NRAW = C > 0 ? 7 * C + 2 : 1;
-    double K[NR][5];      // coefficients of diag t - k in slot (t - k) % NR
+    double K[NK][5];
     double Ru[2][P];      // results of step t-1 / t-2, slot by step parity
     double Rv[2][P];
-    double nK[2][5];      // coefficient prefetch (diag t+2 loaded at step t), C = 0
+    double Lu[NL], Lv[NL];
     double raw[NRAW];     // raw data of diag t+1 loaded at step t, C >= 1
-    double nLu[2], nLv[2];  // pass-start prefetch (diag t+3 loaded at step t)
-    double lpu, lpv;      // pass-start of diag t   (own start value, p = 0)
-    double ldu, ldv;      // pass-start of diag t+1 (right neighbour, p = 0)
 };
 
 template <int C>
@@ -664,32 +701,65 @@ __device__ __forceinline__ void load_raw(double *raw, const SweepArgs &a, size_t
     raw[7 * C + 1] = cuv[a.plane + q];
 }
 
+// Raw buffer access for the SOR kernel (CDNA buffer resource: base, no
+// stride, size in bytes).  A lane whose byte offset is at or beyond the
+// size reads 0 and its store is dropped, without a memory access.
+constexpr unsigned kOutOfRange = 0x80000000u;
+
+__device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
+}
+
+__device__ inline double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+
+__device__ inline void buf_store_f64(double x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    using u2 = __attribute__((ext_vector_type(2))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, (int)soff, 0);
+}
+
+// byte offset of cell (r, d - r) of the skewed plane, or kOutOfRange when
+// the lane has no cell on diagonal d
+__device__ inline unsigned slot_offset(bool row_ok, int d, int r, int w, int hs)
+{
+    const bool ok = row_ok && (unsigned)(d - r) < (unsigned)w;
+    return ok ? ((unsigned)d * (unsigned)hs + (unsigned)r) * 8u : kOutOfRange;
+}
+
+struct SweepBufs {
+    __amdgpu_buffer_rsrc_t coef, duv;  // the 5 coefficient planes; du, dv
+    unsigned plane_bytes;
+};
+
 template <int P, int MAXW, int C, int S>
-__device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs &a, const double *cb,
-                                           double *du, double *dv, double *rdu, double *rdv,
+__device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs &a, SweepBufs &bf,
+                                           double *rdu, double *rdv,
                                            double (*xb)[MAXW][2][P + 1][2], int t, int r,
                                            int lane, int wv, int nw, int Tend)
 {
-    constexpr int NR = 2 * P - 1;
-    if (t > Tend) return;  // uniform
+    using St = SweepState<P, C>;
+    constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL;
+    // Every step, also the ones past the last diagonal (t > Tend, where no
+    // lane has a cell), issues the same vector memory instructions, with no
+    // branch around them, so the compiler can count the loads in flight and
+    // wait for each one only where its value is first used.  Lanes without a
+    // cell use an out-of-range buffer offset: their loads return 0 (a cell
+    // outside the level has zero coefficients and value) and their stores
+    // are dropped, without touching memory.
+    (void)Tend;
     const int h = a.h, w = a.w, hs = a.hs;
-    const size_t plane = a.plane;
     const bool row_ok = r < h;
-    // prefetch: coefficients of diag t+2 (or raw data of diag t+1),
-    // pass-start values of diag t+3
     {
         if constexpr (C == 0) {
-            // padding slots hold zeros, so only the diagonal range is checked
-            const int d = t + 2;
-            double *k = st.nK[S & 1];
-            if (d <= h + w - 2) {
-                size_t q = (size_t)d * hs + r;
+            const int d = t + DK;
+            const unsigned q = slot_offset(row_ok, d, r, w, hs);
+            double *k = st.K[(S + DK) % NK];
 #pragma unroll
-                for (int e = 0; e < 5; ++e) k[e] = cb[e * plane + q];
-            } else {  // beyond the last diagonal: cells outside the level
-#pragma unroll
-                for (int e = 0; e < 5; ++e) k[e] = 0.0;
-            }
+            for (int e = 0; e < 5; ++e) k[e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
         } else {
             const int d = t + 1, c = d - r;
             if (row_ok && c >= 0 && c < w) {
@@ -699,26 +769,13 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
                 for (int e = 0; e < 7 * C + 2; ++e) st.raw[e] = 0.0;
             }
         }
-        const int d3 = t + 3, c3 = d3 - r;
-        if constexpr (C == 0) {
-            if (!a.zero_init && d3 <= h + w - 2) {
-                size_t q = (size_t)d3 * hs + r;
-                st.nLu[S & 1] = du[q];
-                st.nLv[S & 1] = dv[q];
-            } else {
-                st.nLu[S & 1] = 0.0;
-                st.nLv[S & 1] = 0.0;
-            }
-        } else {
-            st.nLu[S & 1] = 0.0;
-            st.nLv[S & 1] = 0.0;
-            if (!a.zero_init && row_ok && c3 >= 0 && c3 < w) {
-                size_t q = (size_t)d3 * hs + r;
-                st.nLu[S & 1] = du[q];
-                st.nLv[S & 1] = dv[q];
-            }
-        }
+        const int d = t + DL;
+        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d, r, w, hs);
+        st.Lu[(S + DL) % NL] = buf_load_f64(bf.duv, q, 0);
+        st.Lv[(S + DL) % NL] = buf_load_f64(bf.duv, q, bf.plane_bytes);
     }
+    const double lpu = st.Lu[S % NL], lpv = st.Lv[S % NL];              // own start value
+    const double ldu = st.Lu[(S + 1) % NL], ldv = st.Lv[(S + 1) % NL];  // right neighbour's
     constexpr int PV = (S + 1) & 1;  // results of step t-1
     constexpr int P2 = S & 1;        // results of step t-2 (overwritten below)
     double upu[P], upv[P], dnu[P], dnv[P];
@@ -726,8 +783,8 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
     for (int p = 0; p < P; ++p) {
         upu[p] = wave_from_prev_lane(st.Ru[PV][p]);
         upv[p] = wave_from_prev_lane(st.Rv[PV][p]);
-        dnu[p] = wave_from_next_lane(p == 0 ? st.ldu : st.Ru[PV][p > 0 ? p - 1 : 0]);
-        dnv[p] = wave_from_next_lane(p == 0 ? st.ldv : st.Rv[PV][p > 0 ? p - 1 : 0]);
+        dnu[p] = wave_from_next_lane(p == 0 ? ldu : st.Ru[PV][p > 0 ? p - 1 : 0]);
+        dnv[p] = wave_from_next_lane(p == 0 ? ldv : st.Rv[PV][p > 0 ? p - 1 : 0]);
     }
     const int pb = (t + 1) & 1;
     if (lane == 0 && wv > 0) {
@@ -758,11 +815,11 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
         // uniform: does any lane of this wave own a cell of sweep p?
         if (t - 2 * p - rlo >= 0 && t - 2 * p - rhi <= w - 1) {
             const int c = t - r - 2 * p;
-            const double *Kp = st.K[((S - 2 * p) % NR + NR) % NR];
-            double ou = p == 0 ? st.lpu : st.Ru[P2][p > 0 ? p - 1 : 0];
-            double ov = p == 0 ? st.lpv : st.Rv[P2][p > 0 ? p - 1 : 0];
-            double ru = p == 0 ? st.ldu : st.Ru[PV][p > 0 ? p - 1 : 0];
-            double rv = p == 0 ? st.ldv : st.Rv[PV][p > 0 ? p - 1 : 0];
+            const double *Kp = st.K[((S - 2 * p) % NK + NK) % NK];
+            double ou = p == 0 ? lpu : st.Ru[P2][p > 0 ? p - 1 : 0];
+            double ov = p == 0 ? lpv : st.Rv[P2][p > 0 ? p - 1 : 0];
+            double ru = p == 0 ? ldu : st.Ru[PV][p > 0 ? p - 1 : 0];
+            double rv = p == 0 ? ldv : st.Rv[PV][p > 0 ? p - 1 : 0];
             double lu = st.Ru[PV][p], lv = st.Rv[PV][p];
             double b = fma(ax, (double)((c == 0) + (c == w - 1)), brow);
             double A12 = Kp[0], Bu = Kp[1], Bv = Kp[2], wu = Kp[3], wvv = Kp[4];
@@ -774,25 +831,15 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
             nv = fma(ay, dnv[p] + upv[p], nv);
             nv = fma(-A12, dun, nv);
             double dvn = fma(wvv, nv, fma(wvv, b, om1) * ov);
-            const bool ok = row_ok && c >= 0 && c < w;
+            const bool ok = row_ok && (unsigned)c < (unsigned)w;
             if constexpr (C == 0) {
-                // a padding cell has zero coefficients and a zero start value,
-                // so it computes 0 (up to sign) and can be stored as is
+                // a cell outside the level has zero coefficients and a zero
+                // start value, so it computes 0 (up to sign)
                 cu[p] = dun;
                 cv[p] = dvn;
-                if (p == P - 1 && t - 2 * (P - 1) >= 0) {
-                    size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
-                    du[q] = dun;
-                    dv[q] = dvn;
-                }
             } else {
                 cu[p] = ok ? dun : 0.0;
                 cv[p] = ok ? dvn : 0.0;
-                if (p == P - 1 && ok) {
-                    size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
-                    du[q] = dun;
-                    dv[q] = dvn;
-                }
             }
             if (p == P - 1 && rdu && ok && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
                 size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
@@ -801,25 +848,23 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
             }
         }
     }
+    {
+        // sweep P-1 leaves diag t - 2(P-1): store its results
+        const unsigned q = slot_offset(row_ok, t - 2 * (P - 1), r, w, hs);
+        buf_store_f64(cu[P - 1], bf.duv, q, 0);
+        buf_store_f64(cv[P - 1], bf.duv, q, bf.plane_bytes);
+    }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
         st.Ru[P2][p] = cu[p];
         st.Rv[P2][p] = cv[p];
     }
-    constexpr int NS = (S + 1) % NR;
-    if constexpr (C == 0) {
-#pragma unroll
-        for (int e = 0; e < 5; ++e) st.K[NS][e] = st.nK[PV][e];
-    } else {
+    if constexpr (C > 0) {
         // coefficients of the cell entering at step t+1 (diag t+1), from its
-        // raw data and start values (ld = pass-start of diag t+1)
-        coef_from_raw<C>(st.raw, st.raw + 6 * C, st.raw[7 * C], st.raw[7 * C + 1], st.ldu, st.ldv,
-                         ax, ay, a.ad, a.psi_one, st.K[NS]);
+        // raw data and its start values
+        coef_from_raw<C>(st.raw, st.raw + 6 * C, st.raw[7 * C], st.raw[7 * C + 1], ldu, ldv, ax, ay,
+                         a.ad, a.psi_one, st.K[(S + 1) % NK]);
     }
-    st.lpu = st.ldu;
-    st.lpv = st.ldv;
-    st.ldu = st.nLu[PV];
-    st.ldv = st.nLv[PV];
     const int wb = t & 1;
     if (lane == 0) {
 #pragma unroll
@@ -827,8 +872,9 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
             xb[wb][wv][0][p][0] = cu[p];
             xb[wb][wv][0][p][1] = cv[p];
         }
-        xb[wb][wv][0][P][0] = st.ldu;
-        xb[wb][wv][0][P][1] = st.ldv;
+        // start value of diag t+2: the upper neighbour wave's p = 0 down value at step t+1
+        xb[wb][wv][0][P][0] = st.Lu[(S + 2) % NL];
+        xb[wb][wv][0][P][1] = st.Lv[(S + 2) % NL];
     }
     if (lane == 63) {
 #pragma unroll
@@ -843,18 +889,20 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
 
 template <int P, int MAXW, int C, int... S>
 __device__ __forceinline__ void sweep_block(SweepState<P, C> &st, const SweepArgs &a,
-                                            const double *cb, double *du, double *dv, double *rdu,
+                                            SweepBufs &bf, double *rdu,
                                             double *rdv, double (*xb)[MAXW][2][P + 1][2], int t0,
                                             int r, int lane, int wv, int nw, int Tend,
                                             std::integer_sequence<int, S...>)
 {
-    (sweep_step<P, MAXW, C, S>(st, a, cb, du, dv, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
+    (sweep_step<P, MAXW, C, S>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
 }
 
 template <int P, int MAXT, int C>
 __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
 {
-    constexpr int NR = 2 * P - 1, U = 2 * NR, MAXW = MAXT / 64;
+    using St = SweepState<P, C>;
+    constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL, U = St::U;
+    constexpr int MAXW = MAXT / 64;
     const int f = blockIdx.x;
     const int r = threadIdx.x;
     const int lane = r & 63;
@@ -862,7 +910,6 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     const int nw = blockDim.x >> 6;
     const double *cb = a.coef + (size_t)f * a.fstride;
     double *du = a.duv + (size_t)f * a.fstride;
-    double *dv = du + a.plane;
     double *rdu = a.ring ? a.ring + (size_t)f * a.fstride : nullptr;
     double *rdv = a.ring ? rdu + a.plane : nullptr;
     __shared__ double xb[2][MAXW][2][P + 1][2];
@@ -870,54 +917,57 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
         (&xb[0][0][0][0][0])[k] = 0.0;
     const int h = a.h, w = a.w, hs = a.hs;
     const bool row_ok = r < h;
-    SweepState<P, C> st;
+    St st;
 #pragma unroll
-    for (int k = 0; k < NR; ++k)
+    for (int k = 0; k < NK; ++k)
 #pragma unroll
         for (int e = 0; e < 5; ++e) st.K[k][e] = 0.0;
 #pragma unroll
     for (int p = 0; p < P; ++p) st.Ru[0][p] = st.Ru[1][p] = st.Rv[0][p] = st.Rv[1][p] = 0.0;
 #pragma unroll
-    for (int e = 0; e < 5; ++e) st.nK[0][e] = st.nK[1][e] = 0.0;
-    st.nLu[0] = st.nLu[1] = st.nLv[0] = st.nLv[1] = 0.0;
-    st.lpu = st.lpv = st.ldu = st.ldv = 0.0;
-    // prologue: diag 0 coefficients in slot 0, diag 1 in the prefetch slot of
-    // step -1; pass-start values of diags 0, 1 and 2.
-    auto cell_ok = [&](int d) { int c = d - r; return row_ok && c >= 0 && c < w; };
+    for (int k = 0; k < NL; ++k) st.Lu[k] = st.Lv[k] = 0.0;
 #pragma unroll
-    for (int e = 0; e < SweepState<P, C>::NRAW; ++e) st.raw[e] = 0.0;
-    if (!a.zero_init) {
-        if (cell_ok(0)) { st.lpu = du[r]; st.lpv = dv[r]; }
-        if (cell_ok(1)) { st.ldu = du[(size_t)hs + r]; st.ldv = dv[(size_t)hs + r]; }
-        if (cell_ok(2)) { st.nLu[1] = du[(size_t)2 * hs + r]; st.nLv[1] = dv[(size_t)2 * hs + r]; }
+    for (int e = 0; e < St::NRAW; ++e) st.raw[e] = 0.0;
+    // prologue: start values of diags 0 .. DL-1; coefficients of diags
+    // 0 .. DK-1 (C = 0) or of diag 0 (C >= 1)
+    auto cell_ok = [&](int d) { int c = d - r; return row_ok && c >= 0 && c < w; };
+    SweepBufs bf;
+    bf.plane_bytes = (unsigned)(a.plane * sizeof(double));
+    bf.coef = make_rsrc(cb, C == 0 ? 5 * bf.plane_bytes : 0u);
+    bf.duv = make_rsrc(du, 2 * bf.plane_bytes);
+#pragma unroll
+    for (int d = 0; d < DL; ++d) {
+        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d, r, w, hs);
+        st.Lu[d] = buf_load_f64(bf.duv, q, 0);
+        st.Lv[d] = buf_load_f64(bf.duv, q, bf.plane_bytes);
     }
     if constexpr (C == 0) {
-        if (cell_ok(0)) {
 #pragma unroll
-            for (int e = 0; e < 5; ++e) st.K[0][e] = cb[e * a.plane + r];
-        }
-        if (cell_ok(1)) {
+        for (int d = 0; d < DK; ++d) {
+            const unsigned q = slot_offset(row_ok, d, r, w, hs);
 #pragma unroll
-            for (int e = 0; e < 5; ++e) st.nK[1][e] = cb[e * a.plane + (size_t)hs + r];
+            for (int e = 0; e < 5; ++e) st.K[d][e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
         }
     } else {
         if (cell_ok(0)) {
             double raw0[7 * C + 2];
             load_raw<C>(raw0, a, (size_t)f * a.fstride, (size_t)r);
-            coef_from_raw<C>(raw0, raw0 + 6 * C, raw0[7 * C], raw0[7 * C + 1], st.lpu, st.lpv,
+            coef_from_raw<C>(raw0, raw0 + 6 * C, raw0[7 * C], raw0[7 * C + 1], st.Lu[0], st.Lv[0],
                              a.ax, a.ay, a.ad, a.psi_one, st.K[0]);
         }
     }
     __syncthreads();
     const int Tend = (h + w - 2) + 2 * (P - 1);
     for (int t0 = 0; t0 <= Tend; t0 += U)
-        sweep_block<P, MAXW, C>(st, a, cb, du, dv, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
+        sweep_block<P, MAXW, C>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
                                 std::make_integer_sequence<int, U>{});
 }
 
 template <int P, int C>
 static hipError_t launch_sweep_pc(const SweepArgs &a, int T, hipStream_t st)
 {
+    // buffer offsets are 32-bit and must stay below kOutOfRange
+    if (a.plane * sizeof(double) * 5 >= kOutOfRange) return hipErrorInvalidValue;
     if (a.hs <= 512)
         hipLaunchKernelGGL((k_sweep<P, 512, C>), dim3(T), dim3(a.hs), 0, st, a);
     else if (P <= 2)
@@ -1126,13 +1176,21 @@ __device__ inline double median25(double (&a)[25])
     return median25_net(a, std::make_integer_sequence<int, sizeof(kMed25) / sizeof(MedCmp)>{});
 }
 
-__global__ void k_median_add(const double *__restrict__ duv, size_t fstride, int h, int w,
-                             int hs, size_t plane, int do_median, const float *__restrict__ u,
-                             const float *__restrict__ v, size_t uv_stride, float *out_f32,
-                             double *out_f64, int out_interleaved)
+// One wave per 64 consecutive rows of one diagonal of the skewed layout, so
+// each of the 50 window loads is one contiguous run of slots.  Waves are
+// numbered frame-major, diagonal, row chunk and dealt to the XCDs in
+// contiguous ranges (neighbouring diagonals share the lines of the
+// row-major flow planes).
+__global__ __launch_bounds__(64) void k_median_add(
+    const double *__restrict__ duv, size_t fstride, int h, int w, int hs, size_t plane,
+    int do_median, const float *__restrict__ u, const float *__restrict__ v, size_t uv_stride,
+    float *out_f32, double *out_f64, int out_interleaved, int T)
 {
-    int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-    if (c >= w) return;
+    const int nchunk = hs / 64, ndiag = h + w - 1;
+    const unsigned L = xcd_block(blockIdx.x, (unsigned)nchunk * ndiag * T);
+    const int t = L / (nchunk * ndiag), rem = L % (nchunk * ndiag);
+    const int d = rem / nchunk, r = (rem % nchunk) * 64 + threadIdx.x, c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
     const double *du = duv + (size_t)t * fstride, *dv = du + plane;
     double mu, mv;
     if (do_median) {
@@ -1181,9 +1239,11 @@ hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew 
                              size_t uv_frame_stride, float *out_f32, double *out_f64,
                              int out_interleaved, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_median_add, dim3((s.w + 63) / 64, s.h, T), dim3(64), 0, st, duv,
-                       frame_stride, s.h, s.w, s.hs, s.plane, do_median, u, v, uv_frame_stride,
-                       out_f32, out_f64, out_interleaved);
+    const size_t nb = (size_t)(s.hs / 64) * (s.h + s.w - 1) * T;
+    if (nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_median_add, dim3((unsigned)nb), dim3(64), 0, st, duv, frame_stride, s.h,
+                       s.w, s.hs, s.plane, do_median, u, v, uv_frame_stride, out_f32, out_f64,
+                       out_interleaved, T);
     return hipGetLastError();
 }
 
