@@ -5,9 +5,17 @@ in HBM: GPU preprocessing (normalize + Gaussian prefilter) of the batch,
 then the batched get_displacement solve of every frame against the shared
 reference (one BatchMotionCorrector batch, all frames sharing w_init).
 
-Prints one JSON line.  The plain path (one frame per call, the sequential
-executor's pattern) is timed beside it, and the CPU oracle is timed on the
-host cores as ``cpu_baseline`` before the GPU is initialised.
+Prints one JSON line.  Beside the batched path it reports
+* the plain path (one frame per solver call, the sequential executor's
+  pattern) and the batched/plain ratio;
+* parity: the CPU oracle's flows for the first frames of the same batch
+  against the GPU's float64 flows;
+* ``cpu_baseline``: the CPU oracle timed on the host cores (worker
+  processes forked before the GPU is initialised);
+* ``roofline`` for the dominant kernel (algorithmic bytes over HIP-event
+  time on the launch stream; HBM traffic per launch from the rocprofv3
+  FETCH_SIZE/WRITE_SIZE summary in profiles/, when recorded for this
+  workload).
 """
 
 from __future__ import annotations
@@ -23,73 +31,87 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_HBM_GBS = 8000.0
-METRIC = "frames/s registered (512x512x2ch), batched vs plain GPU at \u22641e-3 px max |\u0394flow|"
-# rocprofv3 FETCH_SIZE / WRITE_SIZE of the sweep kernel on this workload,
-# scaled by the same counters on a copy kernel of known size.
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_sweep_hbm_traffic_512x512x2_b256.md")
+METRIC = "frames/s registered (512x512x2ch), batched vs plain GPU at ≤1e-3 px max |Δflow|"
+# rocprofv3 FETCH_SIZE / WRITE_SIZE per launch of the solver kernels on this
+# workload, scaled by the same counters on a copy kernel of known size.
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hbm_traffic_512x512x2_b256.md")
 
 
 def _cpu_frame(args):
     """CPU oracle on one frame (runs in a worker process, no GPU)."""
-    import numpy as np
     from oracle import oracle as O
     ref, mov = args
     t = time.perf_counter()
-    O.get_displacement(ref, mov, weight=[0.5, 0.5])
-    return time.perf_counter() - t
+    w = O.get_displacement(ref, mov, weight=[0.5, 0.5])
+    return time.perf_counter() - t, w
 
 
-def cpu_baseline(H, W, C, n_frames, workers):
-    """FP64 CPU restatement of the reference path (the oracle), one frame per
-    worker process with 1 thread each, like pyflowreg's multiprocessing
-    executor.  Returns (frames/s, cores, seconds of CPU work)."""
+def cpu_baseline(raw, n_frames, workers):
+    """FP64 CPU restatement of the reference path (the oracle) on the first
+    n_frames moving frames, one frame per worker process with 1 thread each,
+    like pyflowreg's multiprocessing executor.  Returns (frames/s, seconds
+    of CPU work, flows)."""
     os.environ["OMP_NUM_THREADS"] = "1"
     import numpy as np
     from oracle import oracle as O
-    from pyflowreg_amd import synth
     O.build()
-    frames, _ = synth.video(n_frames + 1, H, W, n_channels=C, seed=1000)
-    f = frames.numpy()
-    ref64 = f[0].astype(np.float64)
+    ref64 = raw[0].astype(np.float64)
     ref = O.preprocess(ref64).astype(np.float32)
-    movs = O.preprocess(f[1:], ref64).astype(np.float32)
+    movs = O.preprocess(raw[1:n_frames + 1], ref64).astype(np.float32)
     jobs = [(ref, movs[k]) for k in range(n_frames)]
     ctx = mp.get_context("fork")
     t = time.perf_counter()
     with ctx.Pool(workers) as pool:
-        per = pool.map(_cpu_frame, jobs)
+        res = pool.map(_cpu_frame, jobs)
     wall = time.perf_counter() - t
-    return n_frames / wall, workers, float(sum(per))
+    return n_frames / wall, float(sum(r[0] for r in res)), np.stack([r[1] for r in res])
 
 
-def sweep_algorithmic_bytes(solver, T):
-    """Algorithmic HBM bytes of all SOR sweep launches of one run: per cell
-    and pass, 5 float64 coefficients + 2 float64 start values read (not on
-    the first pass of a level) and 2 float64 results written."""
+def _passes(solver, h):
+    """Sweep counts and assemble flags of a level's passes (frh_api.hip
+    plan_passes with the sweep kernel's pass-size limit)."""
     its, lag = solver.iterations, solver.update_lag
+    pmax = 5 if 64 * ((h + 63) // 64) <= 512 else 2
     starts = [it for it in range(its) if it == 0 or (it + 1) % lag == 0]
-    total = 0
+    out = []
+    for k, b in enumerate(starts):
+        e = starts[k + 1] if k + 1 < len(starts) else its
+        first = True
+        for s in range(b, e, pmax):
+            out.append((min(pmax, e - s), first))
+            first = False
+    return out
+
+
+def algorithmic_bytes(solver, T):
+    """Algorithmic HBM bytes of one run's k_sweep and k_assemble launches.
+    k_sweep, per cell and pass: 5 float64 coefficients and 2 float64 start
+    values read (not on a level's first pass) and 2 float64 results written.
+    k_assemble, per cell: 6C J + C W + cu, cv read (+ du, dv after the
+    level's first pass) and 5 coefficients written, all float64."""
+    C = solver.C
+    sweep = assemble = 0
     for (_, h, w, *_rest) in solver.plan.levels:
-        pmax = 5 if 64 * ((h + 63) // 64) <= 512 else 2
-        n_pass = 0
-        for k, b in enumerate(starts):
-            e = starts[k + 1] if k + 1 < len(starts) else its
-            n_pass += -(-(e - b) // pmax)
-        cells = h * w
-        total += T * cells * 8 * (9 * n_pass - 2)
-    return total
+        cells = T * h * w
+        for k, (n, asm) in enumerate(_passes(solver, h)):
+            sweep += cells * 8 * (7 + 2 if k > 0 else 5 + 2)
+            if asm:
+                assemble += cells * 8 * (7 * C + 2 + (2 if k > 0 else 0) + 5)
+    return {"sweep": sweep, "assemble": assemble}
 
 
 def measured_traffic(H, W, C, T):
-    """HBM bytes per sweep launch recorded by rocprofv3 for this workload
-    (line ``traffic_bytes_per_launch: N`` of the profile summary), or None."""
+    """HBM bytes per launch recorded by rocprofv3 for this workload (lines
+    ``<kernel> traffic_bytes_per_launch: N`` of the profile summary)."""
+    out = {}
     if (H, W, C, T) != (512, 512, 2, 256) or not os.path.exists(TRAFFIC_FILE):
-        return None
+        return out
     with open(TRAFFIC_FILE) as fh:
         for line in fh:
-            if line.startswith("traffic_bytes_per_launch:"):
-                return int(float(line.split(":", 1)[1]))
-    return None
+            parts = line.split()
+            if len(parts) == 3 and parts[1] == "traffic_bytes_per_launch:":
+                out[parts[0]] = int(float(parts[2]))
+    return out
 
 
 def copy_probe(mib):
@@ -111,11 +133,11 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256)
+    ap.add_argument("--batch", type=int, default=256, help="frames per launch (64, 128 or 256)")
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--channels", type=int, default=2)
-    ap.add_argument("--plain-frames", type=int, default=3)
-    ap.add_argument("--cpu-frames", type=int, default=8)
+    ap.add_argument("--plain-frames", type=int, default=3, help="0 skips the plain path")
+    ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
@@ -128,30 +150,36 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
-    H = W = args.size
-    C = args.channels
-    cpu = None
-    if not args.no_cpu:
-        workers = min(8, os.cpu_count() or 1, args.cpu_frames)
-        fps, cores, work = cpu_baseline(H, W, C, args.cpu_frames, workers)
-        cpu = {"value": round(fps, 4), "unit": "frames/s", "cores": cores, "kind": "port",
-               "sample": f"{args.cpu_frames} frames {H}x{W}x{C}, one per worker process, "
-                         f"{work:.1f} s of CPU work, FP64 oracle (C + scipy median)"}
-
     import numpy as np
     import torch
-    import pyflowreg_amd as fr
     from pyflowreg_amd import synth
+
+    H = W = args.size
+    C = args.channels
+    T = args.batch
+    # inputs are generated once on the host, so the oracle and the GPU see
+    # the same frames (frame 0 is the reference)
+    frames_h, truth_h = synth.video(T + 1, H, W, n_channels=C, seed=1000)
+    raw = frames_h.numpy()
+    cpu = None
+    cpu_w = None
+    if not args.no_cpu:
+        n_cpu = min(args.cpu_frames, T)
+        workers = min(16, os.cpu_count() or 1, n_cpu)  # one GPU's share of the host
+        fps_cpu, work, cpu_w = cpu_baseline(raw, n_cpu, workers)
+        cpu = {"value": round(fps_cpu, 4), "unit": "frames/s", "cores": workers, "kind": "port",
+               "sample": f"first {n_cpu} frames of the batch ({H}x{W}x{C}), one per worker "
+                         f"process, {work:.1f} s of CPU work, FP64 oracle (C + scipy median)"}
+
+    import pyflowreg_amd as fr
     from pyflowreg_amd.preprocess import preprocess
 
     dev = torch.device("cuda", 0)
     torch.cuda.set_device(dev)
-    T = args.batch
-    frames, _ = synth.video(T + 1, H, W, n_channels=C, seed=1000, device=dev)
+    frames = frames_h.to(dev)
     raw_ref = frames[0].double()
     batch = frames[1:].contiguous()
-    ref_planes = preprocess(raw_ref)
-    ref_hwc = ref_planes[0].permute(1, 2, 0).contiguous()
+    ref_hwc = preprocess(raw_ref)[0].permute(1, 2, 0).contiguous()
     solver = fr.DisplacementSolver(H, W, C, max_batch=T)
     solver.set_reference(ref_hwc, [0.5] * C)
 
@@ -169,25 +197,46 @@ def main():
     dt = time.perf_counter() - t0
     fps = args.steps * T / dt
 
-    # kernel timing of one profiled run (HIP events on the launch stream)
+    # kernel timing of one profiled run (HIP events on the launch stream),
+    # float64 output for the parity check
     p = preprocess(batch, normalization_ref=raw_ref).permute(0, 2, 3, 1)
-    solver.run(p, out_dtype=torch.float32, profile=True)
+    w64 = solver.run(p, out_dtype=torch.float64, profile=True)
     tm = solver.timings()
-    sweep_ms, sweep_n = tm["sweep"]
-    alg = sweep_algorithmic_bytes(solver, T)
-    achieved = alg / (sweep_ms * 1e-3) / 1e9
+    alg = algorithmic_bytes(solver, T)
+    traffic = measured_traffic(H, W, C, T)
+    kernels = {}
+    for key, name in (("sweep", "k_sweep"), ("assemble", "k_assemble")):
+        ms, n = tm[key]
+        gbs = alg[key] / (ms * 1e-3) / 1e9
+        kernels[name] = {"bound": "hbm", "achieved": round(gbs, 1), "peak": PEAK_HBM_GBS,
+                         "unit": "GB/s", "frac": round(gbs / PEAK_HBM_GBS, 4),
+                         "traffic": traffic.get(name),
+                         "algorithmic_bytes_per_launch": int(alg[key] / max(n, 1)),
+                         "launches": n, "avg_launch_ms": round(ms / max(n, 1), 4),
+                         "total_ms": round(ms, 3)}
+    dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
+    roofline = dict(kernels[dominant], kernel=dominant)
 
-    # plain path: one frame per call
-    plain = fr.DisplacementSolver(H, W, C, max_batch=1)
-    plain.set_reference(ref_hwc, [0.5] * C)
-    pp = preprocess(batch[: args.plain_frames], normalization_ref=raw_ref).permute(0, 2, 3, 1)
-    plain.run(pp[0], out_dtype=torch.float32)
-    torch.cuda.synchronize()
-    t1 = time.perf_counter()
-    for k in range(args.plain_frames):
-        plain.run(pp[k], out_dtype=torch.float32)
-    torch.cuda.synchronize()
-    plain_fps = args.plain_frames / (time.perf_counter() - t1)
+    parity = None
+    if cpu_w is not None:
+        g = w64[: cpu_w.shape[0]].cpu().numpy()
+        d = np.abs(g - cpu_w)
+        parity = {"frames": int(cpu_w.shape[0]), "max_abs_px": float(d.max()),
+                  "rms_px": float(np.sqrt((d ** 2).mean())), "reference": "FP64 CPU oracle"}
+    epe = float(torch.linalg.vector_norm(w64.cpu() - truth_h[1:], dim=-1).mean())
+
+    plain_fps = None
+    if args.plain_frames > 0:
+        plain = fr.DisplacementSolver(H, W, C, max_batch=1)
+        plain.set_reference(ref_hwc, [0.5] * C)
+        pp = preprocess(batch[: args.plain_frames], normalization_ref=raw_ref).permute(0, 2, 3, 1)
+        plain.run(pp[0], out_dtype=torch.float32)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for k in range(args.plain_frames):
+            plain.run(pp[k], out_dtype=torch.float32)
+        torch.cuda.synchronize()
+        plain_fps = args.plain_frames / (time.perf_counter() - t1)
 
     out = {
         "metric": METRIC,
@@ -205,15 +254,13 @@ def main():
         "config": {"workload": f"batched uniform: {T} frames {H}x{W}x{C} per launch, "
                                "alpha=2, 50 iterations, update_lag 5, eta 0.8, a_data 0.45",
                    "frames_per_step": T, "H": H, "W": W, "channels": C},
-        "plain_fps": round(plain_fps, 3),
-        "batched_over_plain": round(fps / plain_fps, 2),
+        "plain_fps": round(plain_fps, 3) if plain_fps else None,
+        "batched_over_plain": round(fps / plain_fps, 2) if plain_fps else None,
+        "parity": parity,
+        "epe_vs_truth_px": round(epe, 4),
         "kernel_ms": {k: round(v[0], 3) for k, v in tm.items()},
-        "roofline": {"bound": "hbm", "kernel": "k_sweep (SOR wavefront)",
-                     "achieved": round(achieved, 1), "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": round(achieved / PEAK_HBM_GBS, 4),
-                     "traffic": measured_traffic(H, W, C, T),
-                     "algorithmic_bytes_per_launch": int(alg / max(sweep_n, 1)),
-                     "launches": sweep_n, "avg_launch_ms": round(sweep_ms / max(sweep_n, 1), 4)},
+        "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))
