@@ -41,6 +41,7 @@
 #include <tgmath.h>
 #include <stdint.h>
 #include <stdlib.h>
+#include <stddef.h>
 #include <string.h>
 
 #include "../include/flowreg_detmath.h"
@@ -243,37 +244,80 @@ void or_warp_cubic(const float *f2, const float *f1, int H, int W, int C,
         }
 }
 
-/* core/optical_flow.py:73-160 (get_motion_tensor_gc) for one channel.
- * f1, f2: (h, w) float32 (level image and warped image of one channel,
- * strided by C).  Outputs J* (h+2, w+2) with a zero border. */
-void or_motion_tensor_gc(const float *f1, const float *f2, int h, int w, int C, int c,
-                         double hy, double hx,
-                         real *J11, real *J22, real *J33, real *J12, real *J13, real *J23)
+/* ---------------------------------------------------------------------
+ * Motion tensors (core/optical_flow.py:73-345).  numpy evaluates each
+ * expression in the dtype of its array operands: the level image f1 is
+ * float64 (the resize casts back to the input dtype), the warped image f2
+ * is float32 below the top level (imregister_wrapper returns float32) and
+ * float64 at the top level (a copy of the resized level).  p1 / p2 flag a
+ * float32 input; an operation on float32 operands only is rounded to
+ * float32, and one with a float64 operand is float64.  Rounding the double
+ * result of one +, -, *, / or sqrt of float32 values to float32 equals the
+ * float32 operation (double has more than 2 * 24 + 2 bits).  Python float
+ * scalars are cast to the array dtype first (numpy's weak scalars).
+ * --------------------------------------------------------------------- */
+static double r32(int p, double x) { return p ? (double)(float)x : x; }
+
+/* np.gradient interior value (x[+1] - x[-1]) / (2. * h) */
+static double t_grad(int p, double xp, double xm, double h)
+{
+    return r32(p, r32(p, xp - xm) / r32(p, 2.0 * h));
+}
+
+/* (x[-1] - 2 * x[0] + x[+1]) / (h ** 2) */
+static double t_second(int p, double xm, double x0, double xp, double h)
+{
+    return r32(p, r32(p, r32(p, xm - 2 * x0) + xp) / r32(p, h * h));
+}
+
+/* symmetric / edge pad of an (h, w) channel: the padded ring copies its
+ * neighbour (the same for mode="symmetric" and "edge" with width 1) */
+static double *t_pad(const double *f, int h, int w, int C, int c)
 {
     int hp = h + 2, wp = w + 2;
-    size_t np_ = (size_t)hp * wp;
-    real *F1 = (real *)malloc(sizeof(real) * np_), *F2 = (real *)malloc(sizeof(real) * np_);
-    real *fx = (real *)malloc(sizeof(real) * np_), *ft = (real *)malloc(sizeof(real) * np_);
-    /* np.pad(..., mode="symmetric"): row/col -1 duplicates row/col 0 */
+    double *F = (double *)malloc(sizeof(double) * (size_t)hp * wp);
     for (int j = 0; j < hp; ++j)
         for (int i = 0; i < wp; ++i) {
             int sj = j - 1 < 0 ? 0 : (j - 1 >= h ? h - 1 : j - 1);
             int si = i - 1 < 0 ? 0 : (i - 1 >= w ? w - 1 : i - 1);
-            F1[(size_t)j * wp + i] = (real)f1[((size_t)sj * w + si) * C + c];
-            F2[(size_t)j * wp + i] = (real)f2[((size_t)sj * w + si) * C + c];
+            F[(size_t)j * wp + i] = f[((size_t)sj * w + si) * C + c];
         }
-    real rhx = (real)hx, rhy = (real)hy;
-    /* fx = 0.5 (d/dx F1 + d/dx F2), np.gradient along axis 1, interior only
-     * (the border is replaced by the symmetric re-pad below) */
+    return F;
+}
+
+static void t_zero_ring(int hp, int wp, real *J11, real *J22, real *J33, real *J12, real *J13,
+                        real *J23)
+{
+    for (int j = 0; j < hp; ++j)
+        for (int i = 0; i < wp; ++i)
+            if (j == 0 || j == hp - 1 || i == 0 || i == wp - 1) {
+                size_t q = (size_t)j * wp + i;
+                J11[q] = J22[q] = J33[q] = J12[q] = J13[q] = J23[q] = 0;
+            }
+}
+
+/* core/optical_flow.py:73-160 (get_motion_tensor_gc) for one channel.
+ * f1, f2: (h, w) images (strided by C); p1 / p2 mark a float32 dtype.
+ * Outputs J* (h+2, w+2) with a zero border. */
+void or_motion_tensor_gc(const double *f1, const double *f2, int h, int w, int C, int c,
+                         double hy, double hx, int p1, int p2,
+                         real *J11, real *J22, real *J33, real *J12, real *J13, real *J23)
+{
+    const int pm = p1 && p2;
+    int hp = h + 2, wp = w + 2;
+    size_t np_ = (size_t)hp * wp;
+    double *F1 = t_pad(f1, h, w, C, c), *F2 = t_pad(f2, h, w, C, c);
+    double *fx = (double *)malloc(sizeof(double) * np_), *ft = (double *)malloc(sizeof(double) * np_);
+    /* fx = 0.5 (d/dx F1 + d/dx F2), ft = F2 - F1 on the interior; the ring
+     * is replaced by the symmetric re-pad */
     for (int j = 1; j <= h; ++j)
         for (int i = 1; i <= w; ++i) {
             size_t q = (size_t)j * wp + i;
-            real g1 = (F1[q + 1] - F1[q - 1]) / (2 * rhx);
-            real g2 = (F2[q + 1] - F2[q - 1]) / (2 * rhx);
-            fx[q] = (real)0.5 * (g1 + g2);
-            ft[q] = F2[q] - F1[q];
+            double g1 = t_grad(p1, F1[q + 1], F1[q - 1], hx);
+            double g2 = t_grad(p2, F2[q + 1], F2[q - 1], hx);
+            fx[q] = r32(pm, 0.5 * r32(pm, g1 + g2));
+            ft[q] = r32(pm, F2[q] - F1[q]);
         }
-    /* re-pad interiors symmetrically (the padded ring copies its neighbour) */
     for (int j = 0; j < hp; ++j)
         for (int i = 0; i < wp; ++i) {
             if (j >= 1 && j <= h && i >= 1 && i <= w) continue;
@@ -282,32 +326,126 @@ void or_motion_tensor_gc(const float *f1, const float *f2, int h, int w, int C, 
             fx[(size_t)j * wp + i] = fx[(size_t)sj * wp + si];
             ft[(size_t)j * wp + i] = ft[(size_t)sj * wp + si];
         }
-    for (int j = 0; j < hp; ++j)
-        for (int i = 0; i < wp; ++i) {
+    for (int j = 1; j <= h; ++j)
+        for (int i = 1; i <= w; ++i) {
             size_t q = (size_t)j * wp + i;
-            if (j == 0 || j == hp - 1 || i == 0 || i == wp - 1) {
-                J11[q] = J22[q] = J33[q] = J12[q] = J13[q] = J23[q] = 0;
-                continue;
-            }
-            real fxy = (fx[q + wp] - fx[q - wp]) / (2 * rhy);
-            real fxt = (ft[q + 1] - ft[q - 1]) / (2 * rhx);
-            real fyt = (ft[q + wp] - ft[q - wp]) / (2 * rhy);
-            real fxx1 = (F1[q - 1] - 2 * F1[q] + F1[q + 1]) / (rhx * rhx);
-            real fyy1 = (F1[q - wp] - 2 * F1[q] + F1[q + wp]) / (rhy * rhy);
-            real fxx2 = (F2[q - 1] - 2 * F2[q] + F2[q + 1]) / (rhx * rhx);
-            real fyy2 = (F2[q - wp] - 2 * F2[q] + F2[q + wp]) / (rhy * rhy);
-            real fxx = (real)0.5 * (fxx1 + fxx2), fyy = (real)0.5 * (fyy1 + fyy2);
-            real sx = sqrt(fxx * fxx + fxy * fxy), sy = sqrt(fxy * fxy + fyy * fyy);
-            real regx = 1 / (sx * sx + (real)1e-6), regy = 1 / (sy * sy + (real)1e-6);
-            /* numpy evaluates reg_x * fxx**2 as reg_x * (fxx * fxx) */
-            J11[q] = regx * (fxx * fxx) + regy * (fxy * fxy);
-            J22[q] = regx * (fxy * fxy) + regy * (fyy * fyy);
-            J33[q] = regx * (fxt * fxt) + regy * (fyt * fyt);
-            J12[q] = regx * fxx * fxy + regy * fxy * fyy;
-            J13[q] = regx * fxx * fxt + regy * fxy * fyt;
-            J23[q] = regx * fxy * fxt + regy * fyy * fyt;
+            double fxy = t_grad(pm, fx[q + wp], fx[q - wp], hy);
+            double fxt = t_grad(pm, ft[q + 1], ft[q - 1], hx);
+            double fyt = t_grad(pm, ft[q + wp], ft[q - wp], hy);
+            double fxx1 = t_second(p1, F1[q - 1], F1[q], F1[q + 1], hx);
+            double fyy1 = t_second(p1, F1[q - wp], F1[q], F1[q + wp], hy);
+            double fxx2 = t_second(p2, F2[q - 1], F2[q], F2[q + 1], hx);
+            double fyy2 = t_second(p2, F2[q - wp], F2[q], F2[q + wp], hy);
+            double fxx = r32(pm, 0.5 * r32(pm, fxx1 + fxx2));
+            double fyy = r32(pm, 0.5 * r32(pm, fyy1 + fyy2));
+            /* reg = 1.0 / ((np.sqrt(a**2 + b**2) ** 2) + 1e-6) */
+            double sx = r32(pm, sqrt(r32(pm, r32(pm, fxx * fxx) + r32(pm, fxy * fxy))));
+            double sy = r32(pm, sqrt(r32(pm, r32(pm, fxy * fxy) + r32(pm, fyy * fyy))));
+            double regx = r32(pm, 1.0 / r32(pm, r32(pm, sx * sx) + r32(pm, 1e-6)));
+            double regy = r32(pm, 1.0 / r32(pm, r32(pm, sy * sy) + r32(pm, 1e-6)));
+            /* reg_x * fxx**2 is reg_x * (fxx * fxx); products left to right */
+            J11[q] = (real)r32(pm, r32(pm, regx * r32(pm, fxx * fxx)) + r32(pm, regy * r32(pm, fxy * fxy)));
+            J22[q] = (real)r32(pm, r32(pm, regx * r32(pm, fxy * fxy)) + r32(pm, regy * r32(pm, fyy * fyy)));
+            J33[q] = (real)r32(pm, r32(pm, regx * r32(pm, fxt * fxt)) + r32(pm, regy * r32(pm, fyt * fyt)));
+            J12[q] = (real)r32(pm, r32(pm, r32(pm, regx * fxx) * fxy) + r32(pm, r32(pm, regy * fxy) * fyy));
+            J13[q] = (real)r32(pm, r32(pm, r32(pm, regx * fxx) * fxt) + r32(pm, r32(pm, regy * fxy) * fyt));
+            J23[q] = (real)r32(pm, r32(pm, r32(pm, regx * fxy) * fxt) + r32(pm, r32(pm, regy * fyy) * fyt));
         }
+    t_zero_ring(hp, wp, J11, J22, J33, J12, J13, J23);
     free(F1); free(F2); free(fx); free(ft);
+}
+
+/* core/optical_flow.py:163-228 (get_motion_tensor_gray) for one channel. */
+void or_motion_tensor_gray(const double *f1, const double *f2, int h, int w, int C, int c,
+                           double hy, double hx, int p1, int p2,
+                           real *J11, real *J22, real *J33, real *J12, real *J13, real *J23)
+{
+    const int pm = p1 && p2;
+    int hp = h + 2, wp = w + 2;
+    double *F1 = t_pad(f1, h, w, C, c), *F2 = t_pad(f2, h, w, C, c);
+    for (int j = 1; j <= h; ++j)
+        for (int i = 1; i <= w; ++i) {
+            size_t q = (size_t)j * wp + i;
+            double fx1 = t_grad(p1, F1[q + 1], F1[q - 1], hx), fy1 = t_grad(p1, F1[q + wp], F1[q - wp], hy);
+            double fx2 = t_grad(p2, F2[q + 1], F2[q - 1], hx), fy2 = t_grad(p2, F2[q + wp], F2[q - wp], hy);
+            double fx = r32(pm, 0.5 * r32(pm, fx1 + fx2));
+            double fy = r32(pm, 0.5 * r32(pm, fy1 + fy2));
+            double ft = r32(pm, F2[q] - F1[q]);
+            J11[q] = (real)r32(pm, fx * fx);
+            J22[q] = (real)r32(pm, fy * fy);
+            J33[q] = (real)r32(pm, ft * ft);
+            J12[q] = (real)r32(pm, fx * fy);
+            J13[q] = (real)r32(pm, fx * ft);
+            J23[q] = (real)r32(pm, fy * ft);
+        }
+    t_zero_ring(hp, wp, J11, J22, J33, J12, J13, J23);
+    free(F1); free(F2);
+}
+
+/* smoothed Heaviside of one directional difference,
+ * 0.5 * (1.0 + r / np.sqrt(r * r + eps2)), r = (neighbour - centre) / dist */
+static double t_census_s(int p, double nb, double ce, double dist, double eps2)
+{
+    double r = r32(p, r32(p, nb - ce) / r32(p, dist));
+    double d = r32(p, sqrt(r32(p, r32(p, r * r) + r32(p, eps2))));
+    return r32(p, 0.5 * r32(p, 1.0 + r32(p, r / d)));
+}
+
+/* core/optical_flow.py:231-345 (get_motion_tensor_cs) for one channel.
+ * The eight neighbour offsets in row-major order; per offset, the census
+ * images s1, s2 on the h x w core, edge-padded; their np.gradient
+ * derivatives at the core (the padded ring of J is zeroed at the end). */
+void or_motion_tensor_cs(const double *f1, const double *f2, int h, int w, int C, int c,
+                         double hy, double hx, double eps, int p1, int p2,
+                         real *J11, real *J22, real *J33, real *J12, real *J13, real *J23)
+{
+    const int pm = p1 && p2;
+    int hp = h + 2, wp = w + 2;
+    double eps2 = eps * eps;
+    double *F1 = t_pad(f1, h, w, C, c), *F2 = t_pad(f2, h, w, C, c);
+    double *S1 = (double *)malloc(sizeof(double) * (size_t)h * w);
+    double *S2 = (double *)malloc(sizeof(double) * (size_t)h * w);
+    double *A[6];
+    for (int e = 0; e < 6; ++e) A[e] = (double *)calloc((size_t)h * w, sizeof(double));
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (dy == 0 && dx == 0) continue;
+            double dist = sqrt((hy * dy) * (hy * dy) + (hx * dx) * (hx * dx));
+            for (int j = 0; j < h; ++j)
+                for (int i = 0; i < w; ++i) {
+                    size_t qc = (size_t)(j + 1) * wp + (i + 1), qn = qc + (ptrdiff_t)dy * wp + dx;
+                    S1[(size_t)j * w + i] = t_census_s(p1, F1[qn], F1[qc], dist, eps2);
+                    S2[(size_t)j * w + i] = t_census_s(p2, F2[qn], F2[qc], dist, eps2);
+                }
+            for (int j = 0; j < h; ++j)
+                for (int i = 0; i < w; ++i) {
+                    int il = i > 0 ? i - 1 : 0, ir = i < w - 1 ? i + 1 : w - 1;
+                    int ju = j > 0 ? j - 1 : 0, jd = j < h - 1 ? j + 1 : h - 1;
+                    size_t q = (size_t)j * w + i;
+                    double sx1 = t_grad(p1, S1[(size_t)j * w + ir], S1[(size_t)j * w + il], hx);
+                    double sx2 = t_grad(p2, S2[(size_t)j * w + ir], S2[(size_t)j * w + il], hx);
+                    double sy1 = t_grad(p1, S1[(size_t)jd * w + i], S1[(size_t)ju * w + i], hy);
+                    double sy2 = t_grad(p2, S2[(size_t)jd * w + i], S2[(size_t)ju * w + i], hy);
+                    double sx = r32(pm, 0.5 * r32(pm, sx1 + sx2));
+                    double sy = r32(pm, 0.5 * r32(pm, sy1 + sy2));
+                    double st = r32(pm, S2[q] - S1[q]);
+                    A[0][q] += r32(pm, sx * sx);
+                    A[1][q] += r32(pm, sy * sy);
+                    A[2][q] += r32(pm, st * st);
+                    A[3][q] += r32(pm, sx * sy);
+                    A[4][q] += r32(pm, sx * st);
+                    A[5][q] += r32(pm, sy * st);
+                }
+        }
+    real *out[6] = {J11, J22, J33, J12, J13, J23};
+    const double scale = 2.0 / 8.0;
+    for (int e = 0; e < 6; ++e)
+        for (int j = 0; j < h; ++j)
+            for (int i = 0; i < w; ++i)
+                out[e][(size_t)(j + 1) * wp + (i + 1)] = (real)(A[e][(size_t)j * w + i] * scale);
+    t_zero_ring(hp, wp, J11, J22, J33, J12, J13, J23);
+    for (int e = 0; e < 6; ++e) free(A[e]);
+    free(F1); free(F2); free(S1); free(S2);
 }
 
 /* core/level_solver.py:45-66 (set_boundary_2d): rows, then columns. */
@@ -412,11 +550,20 @@ static inline real cf_pow(real x, real b)
 
 static const real OMEGA = (real)1.95;
 
+/* core/level_solver.py:411-431 (compute_flow_gnc): a data (or smoothness)
+ * exponent strictly between 0 and 1 and a stage weight below 1 blend the
+ * robust weight with the quadratic one, (1 - beta) + beta * robust.
+ * gnc_beta < 0 selects compute_flow (no GNC). */
+static int gnc_uses_mix(double a, double gnc_beta)
+{
+    return gnc_beta >= 0 && a > 0.0 && a < 1.0 && gnc_beta < 1.0;
+}
+
 static void assemble_pixel(size_t c, int n, int K, const real *J11, const real *J22, const real *J33,
                            const real *J12, const real *J13, const real *J23, const real *W,
                            const real *u, const real *v, const real *du, const real *dv,
                            const real *psi_s, int smooth_const, real ax, real ay,
-                           const double *a_data, int psi_one, real *coef)
+                           const double *a_data, int psi_one, double gnc_beta, real *coef)
 {
     size_t nb[4] = {c - 1, c + 1, c + n, c - n};
     real w[4];
@@ -444,6 +591,8 @@ static void assemble_pixel(size_t c, int n, int K, const real *J11, const real *
                      + 2 * J13[ck] * x + J23[ck] * y + J33[ck];
             if (val < 0) val = 0;
             psi = a * cf_pow(val + (real)1e-5, a - 1);
+            if (gnc_uses_mix(a_data[k], gnc_beta))
+                psi = (real)(1.0 - gnc_beta) + (real)gnc_beta * psi;
         }
         real wp = W[ck] * psi;
         Du = Du + wp * J11[ck];
@@ -515,7 +664,7 @@ void or_compute_flow(const real *J11, const real *J22, const real *J33, const re
                      const real *u, const real *v, int m, int n, int K,
                      double alpha_x, double alpha_y, int iterations, int update_lag,
                      const double *a_data, double a_smooth, double hx, double hy,
-                     real *du, real *dv, int order)
+                     double gnc_beta, real *du, real *dv, int order)
 {
     size_t N = (size_t)m * n;
     real *psi = (real *)malloc(sizeof(real) * N * K);
@@ -538,10 +687,15 @@ void or_compute_flow(const real *J11, const real *J22, const real *J33, const re
                                  + J33[qk];
                         if (val < 0) val = 0;
                         psi[qk] = (real)a_data[k] * pow(val + (real)1e-5, (real)a_data[k] - 1);
+                        if (gnc_uses_mix(a_data[k], gnc_beta))
+                            psi[qk] = (real)(1.0 - gnc_beta) + (real)gnc_beta * psi[qk];
                     }
-            if (!smooth_const)
+            if (!smooth_const) {
                 psi_smooth_2d(psi_s, u, du, v, dv, m, n, (real)a_smooth, (real)hx, (real)hy);
-            else
+                if (gnc_uses_mix(a_smooth, gnc_beta))
+                    for (size_t q = 0; q < N; ++q)
+                        psi_s[q] = (real)(1.0 - gnc_beta) + (real)gnc_beta * psi_s[q];
+            } else
                 for (size_t q = 0; q < N; ++q) psi_s[q] = 1;
         }
         if (order != 2 && (refresh || it == 0))
@@ -549,7 +703,7 @@ void or_compute_flow(const real *J11, const real *J22, const real *J33, const re
                 for (int j = 1; j < m - 1; ++j) {
                     size_t c = (size_t)j * n + i;
                     assemble_pixel(c, n, K, J11, J22, J33, J12, J13, J23, W, u, v, du, dv, psi_s,
-                                   smooth_const, ax, ay, a_data, !refresh, coef + c * 9);
+                                   smooth_const, ax, ay, a_data, !refresh, gnc_beta, coef + c * 9);
                 }
         set_boundary_2d(du, m, n);
         set_boundary_2d(dv, m, n);

@@ -64,12 +64,17 @@ def lib(precision: str = "f64") -> ctypes.CDLL:
     L.or_warp_cubic.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _f64p, _f64p, _f32p]
     L.or_warp_cubic.restype = None
-    L.or_motion_tensor_gc.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
-                                      ctypes.c_int, ctypes.c_double, ctypes.c_double] + [rp] * 6
+    tens = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_double, ctypes.c_double]
+    L.or_motion_tensor_gc.argtypes = tens + [ctypes.c_int, ctypes.c_int] + [rp] * 6
     L.or_motion_tensor_gc.restype = None
+    L.or_motion_tensor_gray.argtypes = tens + [ctypes.c_int, ctypes.c_int] + [rp] * 6
+    L.or_motion_tensor_gray.restype = None
+    L.or_motion_tensor_cs.argtypes = tens + [ctypes.c_double, ctypes.c_int, ctypes.c_int] + [rp] * 6
+    L.or_motion_tensor_cs.restype = None
     L.or_compute_flow.argtypes = [rp] * 9 + [ctypes.c_int] * 3 + [
         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p,
-        ctypes.c_double, ctypes.c_double, ctypes.c_double, rp, rp, ctypes.c_int]
+        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, rp, rp, ctypes.c_int]
     L.or_compute_flow.restype = None
     L.or_cubic_resize_cv.argtypes = [rp, ctypes.c_int, ctypes.c_int, ctypes.c_int, rp,
                                      ctypes.c_int, ctypes.c_int]
@@ -159,21 +164,87 @@ def imregister_wrapper(f2_level, u, v, f1_level):
 
 
 # ------------------------------------------------------------------ tensor
-def get_motion_tensor_gc(f1, f2, hy, hx, precision: str = "f64"):
-    """core/optical_flow.py:73-160 for one channel (f1, f2 float32-valued)."""
-    a = np.ascontiguousarray(f1, dtype=np.float32)
-    b = np.ascontiguousarray(f2, dtype=np.float32)
+def _tensor(fn, f1, f2, hy, hx, precision, *extra):
+    """Runs one of the C tensor restatements; a float32 input array makes
+    numpy compute that input's derivatives in float32 (the warped image below
+    the top level), any other dtype in float64."""
+    p1 = int(np.asarray(f1).dtype == np.float32)
+    p2 = int(np.asarray(f2).dtype == np.float32)
+    a = np.ascontiguousarray(f1, dtype=np.float64)
+    b = np.ascontiguousarray(f2, dtype=np.float64)
     h, w = a.shape
     dt = _dtype(precision)
     J = [np.empty((h + 2, w + 2), dt) for _ in range(6)]
-    lib(precision).or_motion_tensor_gc(a, b, h, w, 1, 0, float(hy), float(hx), *J)
+    getattr(lib(precision), fn)(a, b, h, w, 1, 0, float(hy), float(hx), *extra, p1, p2, *J)
     return tuple(J)
+
+
+def get_motion_tensor_gc(f1, f2, hy, hx, precision: str = "f64"):
+    """core/optical_flow.py:73-160 for one channel."""
+    return _tensor("or_motion_tensor_gc", f1, f2, hy, hx, precision)
+
+
+def get_motion_tensor_gray(f1, f2, hy, hx, precision: str = "f64"):
+    """core/optical_flow.py:163-228 for one channel."""
+    return _tensor("or_motion_tensor_gray", f1, f2, hy, hx, precision)
+
+
+def get_motion_tensor_cs(f1, f2, hy, hx, eps=None, precision: str = "f64"):
+    """core/optical_flow.py:231-345 for one channel (eps defaults to 0.1/255)."""
+    e = 0.1 / 255.0 if eps is None else float(eps)
+    return _tensor("or_motion_tensor_cs", f1, f2, hy, hx, precision, e)
+
+
+_TENSORS = {"gc": get_motion_tensor_gc, "gradient": get_motion_tensor_gc,
+            "gray": get_motion_tensor_gray, "brightness": get_motion_tensor_gray,
+            "cs": get_motion_tensor_cs, "census": get_motion_tensor_cs}
+
+
+def resolve_motion_tensor(const_assumption):
+    """core/optical_flow.py:446-473."""
+    key = str(getattr(const_assumption, "value", const_assumption)).strip().lower()
+    if key not in _TENSORS:
+        raise ValueError(f"Unknown constancy assumption: '{const_assumption}'. Supported values "
+                         f"are: '{chr(39).join(sorted(_TENSORS))}'.")
+    return _TENSORS[key]
+
+
+def normalize_gnc_schedule(gnc_schedule):
+    """core/optical_flow.py:414-432."""
+    if gnc_schedule is None:
+        return None
+    sch = np.asarray(gnc_schedule, dtype=np.float64)
+    if sch.ndim != 1:
+        raise ValueError("gnc_schedule must be a 1D sequence of stage weights")
+    if sch.size < 2:
+        raise ValueError("gnc_schedule must contain at least two stages")
+    if np.any(sch < 0.0) or np.any(sch > 1.0):
+        raise ValueError("gnc_schedule entries must lie in [0, 1]")
+    if not np.all(np.diff(sch) >= 0.0):
+        raise ValueError("gnc_schedule must be monotone nondecreasing")
+    if not np.isclose(sch[0], 0.0):
+        raise ValueError("gnc_schedule must start at 0.0")
+    if not np.isclose(sch[-1], 1.0):
+        raise ValueError("gnc_schedule must end at 1.0")
+    return np.ascontiguousarray(sch)
+
+
+def normalize_warping_steps(warping_steps):
+    """core/optical_flow.py:435-443."""
+    if warping_steps is None:
+        return None
+    steps = int(warping_steps)
+    if steps < 1:
+        raise ValueError("warping_steps must be a positive integer")
+    return steps
 
 
 # ------------------------------------------------------------------ solver
 def compute_flow(J11, J22, J33, J12, J13, J23, weight, u, v, alpha_x, alpha_y, iterations,
-                 update_lag, a_data, a_smooth, hx, hy, precision: str = "f64", order: int = 0):
-    """core/level_solver.py:158-369; returns (m, n, 2)."""
+                 update_lag, a_data, a_smooth, hx, hy, precision: str = "f64", order: int = 0,
+                 gnc_beta=None):
+    """core/level_solver.py:158-369 (compute_flow) or, with gnc_beta,
+    :372-555 (compute_flow_gnc); returns (m, n, 2)."""
     dt = _dtype(precision)
     arrs = [np.ascontiguousarray(a, dtype=dt) for a in (J11, J22, J33, J12, J13, J23, weight)]
     m, n, K = arrs[0].shape
@@ -184,17 +255,17 @@ def compute_flow(J11, J22, J33, J12, J13, J23, weight, u, v, alpha_x, alpha_y, i
     ad = np.ascontiguousarray(np.broadcast_to(np.asarray(a_data, np.float64), (K,)))
     lib(precision).or_compute_flow(*arrs, uu, vv, m, n, K, float(alpha_x), float(alpha_y),
                                    int(iterations), int(update_lag), ad, float(a_smooth),
-                                   float(hx), float(hy), du, dv, int(order))
+                                   float(hx), float(hy), -1.0 if gnc_beta is None else float(gnc_beta),
+                                   du, dv, int(order))
     return np.stack([du, dv], axis=2)
 
 
 def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, update_lag,
                  verbose, a_data, a_smooth, hx, hy, gnc_beta=None, precision="f64", order=0):
-    """core/optical_flow.py:348-411 (gc path; GNC is not restated)."""
-    if gnc_beta is not None:
-        raise NotImplementedError("the oracle does not restate compute_flow_gnc")
+    """core/optical_flow.py:348-411."""
     r = compute_flow(J11, J22, J33, J12, J13, J23, weight, u, v, alpha[0], alpha[1], iterations,
-                     update_lag, a_data, a_smooth, hx, hy, precision=precision, order=order)
+                     update_lag, a_data, a_smooth, hx, hy, precision=precision, order=order,
+                     gnc_beta=gnc_beta)
     return r[:, :, 0], r[:, :, 1]
 
 
@@ -228,28 +299,40 @@ def _expand_weight(weight, m, n, C):
     return weight
 
 
-def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
-                     levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, const_assumption="gc",
-                     uv=None, weight=None, level_solver_backend=None, gnc_schedule=None,
-                     warping_steps=None, *, precision="f64", order=0):
-    """core/optical_flow.py:773-906 + :476-617 (gc data term, no GNC)."""
-    assert fixed.ndim == moving.ndim
-    key = str(getattr(const_assumption, "value", const_assumption)).strip().lower()
-    if key not in ("gc", "gradient"):
-        raise NotImplementedError(f"oracle restates the gc data term only, got {key!r}")
-    if gnc_schedule is not None:
-        raise NotImplementedError("oracle does not restate GNC")
-    fixed = fixed.astype(np.float64)
-    moving = moving.astype(np.float64)
-    if fixed.ndim == 2:
-        fixed = fixed[:, :, None]
-        moving = moving[:, :, None]
-    m, n, C = fixed.shape
-    weight = _expand_weight(weight, m, n, C)
-    a_data_arr = np.full(C, a_data, dtype=np.float64) if not isinstance(a_data, np.ndarray) \
-        else np.ascontiguousarray(a_data, dtype=np.float64)
-    dt = _dtype(precision)
+def _median_mirror(x):
+    return median_filter(x, size=(5, 5), mode="mirror")
 
+
+def _level_tensor(tensor_fn, f1_level, tmp, h_row, h_col, C, dt, precision):
+    lh, lw = f1_level.shape[:2]
+    J = np.zeros((6, lh + 2, lw + 2, C), dt)
+    for ch in range(C):
+        Jc = tensor_fn(f1_level[:, :, ch], tmp[:, :, ch], h_row, h_col, precision=precision)
+        for t in range(6):
+            J[t, :, :, ch] = Jc[t]
+    return J
+
+
+def _weight_level(weight, lh, lw, C, dt):
+    wl = resize(weight, (lh, lw))
+    out = np.zeros((lh + 2, lw + 2, C), dt)
+    out[1:-1, 1:-1] = wl
+    return out
+
+
+def _solve(level_solver_backend, args, gnc_beta, precision, order, dt):
+    if level_solver_backend is not None:
+        du, dv = level_solver_backend(*args, gnc_beta)
+    else:
+        du, dv = level_solver(*args, gnc_beta, precision=precision, order=order)
+    return np.array(du, dtype=dt), np.array(dv, dtype=dt)
+
+
+def _stage(fixed, moving, alpha, update_lag, iterations, min_level, levels, eta, a_smooth,
+           a_data_arr, uv, weight, level_solver_backend, tensor_fn, precision, order):
+    """core/optical_flow.py:476-617 (one pyramid pass)."""
+    m, n, C = fixed.shape
+    dt = _dtype(precision)
     plan, min_level = level_plan(m, n, eta, levels, min_level)
     top = plan[0][0]
     if uv is not None:
@@ -266,7 +349,7 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
         if i == top:
             u = np.pad(resize(u_init, level_size), 1, mode="edge").astype(dt)
             v = np.pad(resize(v_init, level_size), 1, mode="edge").astype(dt)
-            tmp = f2_level.astype(np.float32)
+            tmp = f2_level.copy()  # float64 dtype: the tensor runs in float64
         else:
             u = np.pad(resize(u[1:-1, 1:-1], level_size), 1, mode="edge").astype(dt)
             v = np.pad(resize(v[1:-1, 1:-1], level_size), 1, mode="edge").astype(dt)
@@ -275,27 +358,16 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
                                      v[1:-1, 1:-1].astype(np.float64) / h_row,
                                      f1_level.astype(np.float32))
         lh, lw = level_size
-        J = np.zeros((6, lh + 2, lw + 2, C), dt)
-        for ch in range(C):
-            Jc = get_motion_tensor_gc(f1_level[:, :, ch], tmp[:, :, ch], h_row, h_col, precision)
-            for t in range(6):
-                J[t, :, :, ch] = Jc[t]
-        wl = resize(weight, (lh, lw))
-        weight_level = np.zeros((lh + 2, lw + 2, C), dt)
-        weight_level[1:-1, 1:-1] = wl
+        J = _level_tensor(tensor_fn, f1_level, tmp, h_row, h_col, C, dt, precision)
+        weight_level = _weight_level(weight, lh, lw, C, dt)
         alpha_scaling = 1 if i == min_level else eta ** (-0.5 * i)
         alpha_tmp = [alpha_scaling * alpha[j] for j in range(len(alpha))]
         args = (J[0], J[1], J[2], J[3], J[4], J[5], weight_level, u, v, alpha_tmp, iterations,
                 update_lag, 0, a_data_arr, a_smooth, h_col, h_row)
-        if level_solver_backend is not None:
-            du, dv = level_solver_backend(*args, None)
-        else:
-            du, dv = level_solver(*args, None, precision=precision, order=order)
-        du = np.array(du, dtype=dt)
-        dv = np.array(dv, dtype=dt)
+        du, dv = _solve(level_solver_backend, args, None, precision, order, dt)
         if min(level_size) > 5:
-            du[1:-1, 1:-1] = median_filter(du[1:-1, 1:-1], size=(5, 5), mode="mirror")
-            dv[1:-1, 1:-1] = median_filter(dv[1:-1, 1:-1], size=(5, 5), mode="mirror")
+            du[1:-1, 1:-1] = _median_mirror(du[1:-1, 1:-1])
+            dv[1:-1, 1:-1] = _median_mirror(dv[1:-1, 1:-1])
         u = u + du
         v = v + dv
     w = np.zeros((u.shape[0] - 2, u.shape[1] - 2, 2), dtype=np.float64)
@@ -304,6 +376,90 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
     if min_level > 0:
         w = cubic_resize_cv(w, m, n)
     return w
+
+
+def _stage_gnc(fixed, moving, alpha, update_lag, iterations, min_level, levels, eta, a_smooth,
+               a_data_arr, uv, weight, level_solver_backend, tensor_fn, gnc_beta, warping_steps,
+               precision, order):
+    """core/optical_flow.py:620-770: every level is warped (the top level
+    too) and relinearised warping_steps times; after each solve u += du and
+    the median acts on u itself."""
+    m, n, C = fixed.shape
+    dt = _dtype(precision)
+    plan, min_level = level_plan(m, n, eta, levels, min_level)
+    top = plan[0][0]
+    if uv is not None:
+        u_init, v_init = uv[:, :, 0], uv[:, :, 1]
+    else:
+        u_init = np.zeros((m, n), np.float64)
+        v_init = np.zeros((m, n), np.float64)
+    u = v = None
+    for i, level_size in plan:
+        f1_level = resize(fixed, level_size)
+        f2_level = resize(moving, level_size)
+        h_row = float(m) / f1_level.shape[0]
+        h_col = float(n) / f1_level.shape[1]
+        # u keeps numpy's dtype (float32 for a float32 initial flow at the
+        # top level), which sets the dtype of u / h_col in the warp
+        if i == top:
+            u = np.pad(resize(u_init, level_size), 1, mode="edge")
+            v = np.pad(resize(v_init, level_size), 1, mode="edge")
+        else:
+            u = np.pad(resize(u[1:-1, 1:-1], level_size), 1, mode="edge")
+            v = np.pad(resize(v[1:-1, 1:-1], level_size), 1, mode="edge")
+        lh, lw = level_size
+        weight_level = _weight_level(weight, lh, lw, C, dt)
+        alpha_scaling = 1 if i == min_level else eta ** (-0.5 * i)
+        alpha_tmp = [alpha_scaling * alpha[j] for j in range(len(alpha))]
+        for _ in range(warping_steps):
+            tmp = imregister_wrapper(f2_level.astype(np.float32),
+                                     (u[1:-1, 1:-1] / h_col).astype(np.float64),
+                                     (v[1:-1, 1:-1] / h_row).astype(np.float64),
+                                     f1_level.astype(np.float32))
+            J = _level_tensor(tensor_fn, f1_level, tmp, h_row, h_col, C, dt, precision)
+            args = (J[0], J[1], J[2], J[3], J[4], J[5], weight_level, u.astype(dt), v.astype(dt),
+                    alpha_tmp, iterations, update_lag, 0, a_data_arr, a_smooth, h_col, h_row)
+            du, dv = _solve(level_solver_backend, args, gnc_beta, precision, order, dt)
+            u = u + du
+            v = v + dv
+            if min(level_size) > 5:
+                u = np.pad(_median_mirror(u[1:-1, 1:-1]), 1, mode="edge")
+                v = np.pad(_median_mirror(v[1:-1, 1:-1]), 1, mode="edge")
+    w = np.zeros((u.shape[0] - 2, u.shape[1] - 2, 2), dtype=np.float64)
+    w[:, :, 0] = u[1:-1, 1:-1]
+    w[:, :, 1] = v[1:-1, 1:-1]
+    if min_level > 0:
+        w = cubic_resize_cv(w, m, n)
+    return w
+
+
+def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
+                     levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, const_assumption="gc",
+                     uv=None, weight=None, level_solver_backend=None, gnc_schedule=None,
+                     warping_steps=None, *, precision="f64", order=0):
+    """core/optical_flow.py:773-929."""
+    assert fixed.ndim == moving.ndim
+    tensor_fn = resolve_motion_tensor(const_assumption)
+    fixed = fixed.astype(np.float64)
+    moving = moving.astype(np.float64)
+    if fixed.ndim == 2:
+        fixed = fixed[:, :, None]
+        moving = moving[:, :, None]
+    m, n, C = fixed.shape
+    weight = _expand_weight(weight, m, n, C)
+    a_data_arr = np.full(C, a_data, dtype=np.float64) if not isinstance(a_data, np.ndarray) \
+        else np.ascontiguousarray(a_data, dtype=np.float64)
+    sched = normalize_gnc_schedule(gnc_schedule)
+    steps = normalize_warping_steps(warping_steps)
+    common = (alpha, update_lag, iterations, min_level, levels, eta, a_smooth, a_data_arr)
+    if sched is None:
+        return _stage(fixed, moving, *common, uv, weight, level_solver_backend, tensor_fn,
+                      precision, order)
+    flow = uv
+    for beta in sched:
+        flow = _stage_gnc(fixed, moving, *common, flow, weight, level_solver_backend, tensor_fn,
+                          float(beta), 10 if steps is None else steps, precision, order)
+    return flow
 
 
 # ------------------------------------------------------------ preprocess

@@ -90,8 +90,15 @@ def main():
 
     a = gaussian_filter(rng.random((18, 21)), 1.5).astype(np.float32)
     b = gaussian_filter(rng.random((18, 21)), 1.5).astype(np.float32)
+    # float32 inputs: numpy computes the derivatives in float32
     J = O.get_motion_tensor_gc(a, b, 1.1, 0.9)
     save("tensor_gc", {"hy": 1.1, "hx": 0.9}, f1=a, f2=b, J=np.stack(J))
+    # the pipeline's case below the top level: float64 level image, float32 warped image
+    a64 = a.astype(np.float64)
+    for kind, fn in (("gc", O.get_motion_tensor_gc), ("gray", O.get_motion_tensor_gray),
+                     ("cs", O.get_motion_tensor_cs)):
+        J = fn(a64, b, 1.1, 0.9)
+        save(f"tensor_{kind}_mixed", {"hy": 1.1, "hx": 0.9, "kind": kind}, f1=a64, f2=b, J=np.stack(J))
 
     for name, (h, w, K, its, lag) in {"level_solver_a": (16, 14, 2, 12, 5),
                                       "level_solver_b": (9, 9, 1, 7, 3)}.items():
@@ -109,13 +116,24 @@ def main():
         du, dv = O.level_solver(*Js, Wt, uu, vv, meta["alpha"], its, lag, 0, np.full(K, 0.45), 1.0,
                                 1.3, 0.8)
         save(name, meta, J=Js, weight=Wt, u=uu, v=vv, du=du, dv=dv)
+        # compute_flow_gnc at a mid stage weight
+        meta_g = dict(meta, gnc_beta=0.5)
+        du, dv = O.level_solver(*Js, Wt, uu, vv, meta["alpha"], its, lag, 0, np.full(K, 0.45), 1.0,
+                                1.3, 0.8, 0.5)
+        save(name + "_gnc", meta_g, J=Js, weight=Wt, u=uu, v=vv, du=du, dv=dv)
 
-    for name, (seed, H, W, C, uv0) in {"displacement_32x32x1": (7, 32, 32, 1, None),
-                                       "displacement_40x48x2": (8, 40, 48, 2, (0.3, -0.2))}.items():
+    cases = {"displacement_32x32x1": (7, 32, 32, 1, None, {}),
+             "displacement_40x48x2": (8, 40, 48, 2, (0.3, -0.2), {}),
+             "displacement_gray_24x28x1": (9, 24, 28, 1, None, {"const_assumption": "gray"}),
+             "displacement_cs_24x28x2": (10, 24, 28, 2, None, {"const_assumption": "cs"}),
+             "displacement_gnc_20x24x1": (11, 20, 24, 1, (0.2, 0.1),
+                                          {"gnc_schedule": [0.0, 0.5, 1.0], "warping_steps": 2,
+                                           "iterations": 10})}
+    for name, (seed, H, W, C, uv0, opts) in cases.items():
         fixed, mov, d = pair(seed, H, W, C)
         ref = O.preprocess(fixed.astype(np.float64)).astype(np.float32)
         mv = O.preprocess(mov, fixed.astype(np.float64)).astype(np.float32)
-        kw = {}
+        kw = dict(opts)
         if C > 1:
             kw["weight"] = [0.5, 0.5]
         uv = None
@@ -126,8 +144,8 @@ def main():
         w = O.get_displacement(ref, mv, **kw)
         sig = level_sigmas(H, W)
         gauss = {f"gauss_{k}": O.gauss_g(s)[1] for k, s in enumerate(sig)}
-        meta = {"seed": seed, "weight": kw.get("weight"), "sigmas": sig,
-                "options": "alpha=2, iterations=50, update_lag=5, eta=0.8, a_data=0.45"}
+        meta = {"seed": seed, "weight": kw.get("weight"), "sigmas": sig, "opts": opts,
+                "options": "alpha=2, iterations=50, update_lag=5, eta=0.8, a_data=0.45 unless in opts"}
         save(name, meta, fixed_raw=fixed, moving_raw=mov, fixed=ref, moving=mv,
              uv=uv if uv is not None else np.zeros(0, np.float32), w=w, truth=d, **gauss)
 

@@ -50,18 +50,31 @@ def test_oracle_tensor(oracle):
     np.testing.assert_array_equal(np.stack(J), z["J"])
 
 
-@pytest.mark.parametrize("name", ["level_solver_a", "level_solver_b"])
+@pytest.mark.parametrize("kind", ["gc", "gray", "cs"])
+def test_oracle_tensor_mixed(oracle, kind):
+    meta, z = load(f"tensor_{kind}_mixed")
+    fn = {"gc": oracle.get_motion_tensor_gc, "gray": oracle.get_motion_tensor_gray,
+          "cs": oracle.get_motion_tensor_cs}[kind]
+    np.testing.assert_array_equal(np.stack(fn(z["f1"], z["f2"], meta["hy"], meta["hx"])), z["J"])
+
+
+LEVEL_CASES = ["level_solver_a", "level_solver_b", "level_solver_a_gnc", "level_solver_b_gnc"]
+DISPLACEMENT_CASES = ["displacement_32x32x1", "displacement_40x48x2", "displacement_gray_24x28x1",
+                      "displacement_cs_24x28x2", "displacement_gnc_20x24x1"]
+
+
+@pytest.mark.parametrize("name", LEVEL_CASES)
 def test_oracle_level_solver(oracle, name):
     m, z = load(name)
     K = z["J"].shape[-1]
     du, dv = oracle.level_solver(*z["J"], z["weight"], z["u"], z["v"], m["alpha"], m["iterations"],
                                  m["update_lag"], 0, np.full(K, m["a_data"]), m["a_smooth"], m["hx"],
-                                 m["hy"])
+                                 m["hy"], m.get("gnc_beta"))
     np.testing.assert_array_equal(du, z["du"])
     np.testing.assert_array_equal(dv, z["dv"])
 
 
-@pytest.mark.parametrize("name", ["displacement_32x32x1", "displacement_40x48x2"])
+@pytest.mark.parametrize("name", DISPLACEMENT_CASES)
 def test_oracle_displacement(oracle, name):
     meta, z = load(name)
     _check_taps(meta, z, oracle.gauss_g)
@@ -69,7 +82,7 @@ def test_oracle_displacement(oracle, name):
     mov = oracle.preprocess(z["moving_raw"], z["fixed_raw"].astype(np.float64)).astype(np.float32)
     np.testing.assert_array_equal(ref, z["fixed"])
     np.testing.assert_array_equal(mov, z["moving"])
-    kw = {}
+    kw = dict(meta["opts"])
     if meta["weight"] is not None:
         kw["weight"] = meta["weight"]
     if z["uv"].size:
