@@ -85,25 +85,41 @@ int frh_warp_cubic(const float *f2, const float *f1, int f1_per_frame, int T, in
                    int W, const float *u, const float *v, double h_col, double h_row,
                    float *out, void *stream);
 
-/* core/optical_flow.py:73-160 (get_motion_tensor_gc), one channel.
- * f1, f2: (h, w) float32.  J11..J23: (h+2, w+2) float64, zero border. */
-int frh_motion_tensor_gc(const float *f1, const float *f2, int h, int w, double hy, double hx,
-                         double *J11, double *J22, double *J33, double *J12, double *J13,
-                         double *J23, void *stream);
+/* Same warp for float64 displacements already divided by the grid spacing
+ * (the reference's imregister_wrapper(f2, u / h_col, v / h_row, f1) call
+ * with u of any dtype): the map is float32(x + u) formed in float64. */
+int frh_warp_cubic_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                       int W, const double *u, const double *v, float *out, void *stream);
+
+/* Data terms (core/optical_flow.py:446-473, _resolve_motion_tensor_func). */
+#define FRH_TENSOR_GC 0   /* "gc" / "gradient": get_motion_tensor_gc, :73-160 */
+#define FRH_TENSOR_GRAY 1 /* "gray" / "brightness": get_motion_tensor_gray, :163-228 */
+#define FRH_TENSOR_CS 2   /* "cs" / "census": get_motion_tensor_cs, :231-345 */
+
+/* get_motion_tensor_{gc,gray,cs}(f1, f2, hy, hx[, eps]) for one channel.
+ * f1, f2: (h, w) float32-valued images.  f1_f32 / f2_f32: the caller's
+ * array is float32 (numpy then evaluates that image's derivatives in
+ * float32, as for the warped image below the top level); 0 = float64.
+ * eps: census smoothing width (ignored by gc and gray; the reference's
+ * default is 0.1 / 255).  J11..J23: (h+2, w+2) float64, zero border. */
+int frh_motion_tensor(int kind, const float *f1, const float *f2, int h, int w, double hy,
+                      double hx, double eps, int f1_f32, int f2_f32, double *J11, double *J22,
+                      double *J33, double *J12, double *J13, double *J23, void *stream);
 
 /* core/optical_flow.py:348-411 (level_solver) -> core/level_solver.py:158-369
- * (compute_flow), gc data term, no GNC.  Same positional meaning as the
- * reference: J*, weight (m, n, K) float64; u, v (m, n) float64;
- * a_data (K) float64 host memory; hx scales column differences, hy rows.
- * du, dv: (m, n) float64 outputs (interior updated, ring = the last
- * boundary copy).  Sweeps run in the reference's column-major
+ * (compute_flow), or :372-555 (compute_flow_gnc) when gnc_beta >= 0 (the
+ * reference's gnc_beta=None is any negative value).  Same positional
+ * meaning as the reference: J*, weight (m, n, K) float64; u, v (m, n)
+ * float64; a_data (K) float64 host memory; hx scales column differences,
+ * hy rows.  du, dv: (m, n) float64 outputs (interior updated, ring = the
+ * last boundary copy).  Sweeps run in the reference's column-major
  * lexicographic order (a diagonal wavefront on the GPU). */
 int frh_level_solver(const double *J11, const double *J22, const double *J33,
                      const double *J12, const double *J13, const double *J23,
                      const double *weight, const double *u, const double *v, int m, int n,
                      int K, double alpha_x, double alpha_y, int iterations, int update_lag,
-                     const double *a_data, double a_smooth, double hx, double hy, double *du,
-                     double *dv, void *stream);
+                     const double *a_data, double a_smooth, double hx, double hy,
+                     double gnc_beta, double *du, double *dv, void *stream);
 
 /* cv2.resize(w, (ow, oh), interpolation=cv2.INTER_CUBIC) of float64 flows
  * (core/optical_flow.py:615-616, only when min_level > 0), restated from
@@ -143,6 +159,7 @@ typedef struct {
     double a_smooth;
     const double *a_data;     /* (C) host */
     int max_batch;            /* largest T passed to frh_solver_run */
+    int const_assumption;     /* FRH_TENSOR_GC, _GRAY or _CS */
 } frh_solver_desc;
 
 typedef struct frh_solver frh_solver;

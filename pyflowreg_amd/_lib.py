@@ -39,6 +39,7 @@ class SolverDesc(ctypes.Structure):
         ("iterations", c_int), ("update_lag", c_int),
         ("a_smooth", c_double), ("a_data", c_void_p),
         ("max_batch", c_int),
+        ("const_assumption", c_int),
     ]
 
 
@@ -56,11 +57,13 @@ SIGNATURES = {
                            c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "frh_warp_cubic": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
                                c_void_p, c_double, c_double, c_void_p, c_void_p]),
-    "frh_motion_tensor_gc": (c_int, [c_void_p, c_void_p, c_int, c_int, c_double, c_double]
-                             + [c_void_p] * 6 + [c_void_p]),
+    "frh_warp_cubic_f64": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
+                                   c_void_p, c_void_p, c_void_p]),
+    "frh_motion_tensor": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_double, c_double,
+                                  c_double, c_int, c_int] + [c_void_p] * 6 + [c_void_p]),
     "frh_level_solver": (c_int, [c_void_p] * 9 + [c_int, c_int, c_int, c_double, c_double, c_int,
                                                   c_int, c_void_p, c_double, c_double, c_double,
-                                                  c_void_p, c_void_p, c_void_p]),
+                                                  c_double, c_void_p, c_void_p, c_void_p]),
     "frh_median5_mirror": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
     "frh_resize_cubic_cv": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p]),

@@ -13,6 +13,10 @@
 
 using namespace frh;
 
+// the census smoothing width the reference's get_motion_tensor_cs uses when
+// called from the level loop (core/optical_flow.py:287-288)
+static constexpr double kCensusEps = 0.1 / 255.0;
+
 namespace {
 
 thread_local char g_err[512] = "";
@@ -122,7 +126,7 @@ extern "C" {
 
 const char *frh_last_error(void) { return g_err; }
 
-int frh_abi_version(void) { return 1; }
+int frh_abi_version(void) { return 2; }
 
 int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
 {
@@ -216,13 +220,28 @@ int frh_warp_cubic(const float *f2, const float *f1, int f1_per_frame, int T, in
     return FRH_OK;
 }
 
-int frh_motion_tensor_gc(const float *f1, const float *f2, int h, int w, double hy, double hx,
-                         double *J11, double *J22, double *J33, double *J12, double *J13, double *J23,
-                         void *stream)
+int frh_warp_cubic_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H, int W,
+                       const double *u, const double *v, float *out, void *stream)
 {
-    FRH_REQ(f1 && f2 && J11 && J22 && J33 && J12 && J13 && J23, "frh_motion_tensor_gc: null pointer");
-    FRH_REQ(h > 0 && w > 0 && h + 2 <= 65535, "frh_motion_tensor_gc: bad shape");
-    hipError_t e = launch_tensor_rowmajor(f1, f2, h, w, hy, hx, J11, J22, J33, J12, J13, J23, as_stream(stream));
+    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_cubic_f64: null pointer");
+    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_cubic_f64: bad shape");
+    hipError_t e = launch_warp_f64(f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, out,
+                                   as_stream(stream));
+    FRH_HIP(e, "warp kernel");
+    return FRH_OK;
+}
+
+int frh_motion_tensor(int kind, const float *f1, const float *f2, int h, int w, double hy, double hx,
+                      double eps, int f1_f32, int f2_f32, double *J11, double *J22, double *J33,
+                      double *J12, double *J13, double *J23, void *stream)
+{
+    FRH_REQ(f1 && f2 && J11 && J22 && J33 && J12 && J13 && J23, "frh_motion_tensor: null pointer");
+    FRH_REQ(kind >= FRH_TENSOR_GC && kind <= FRH_TENSOR_CS, "frh_motion_tensor: unknown data term %d", kind);
+    FRH_REQ(h > 0 && w > 0 && h + 2 <= 65535, "frh_motion_tensor: bad shape");
+    FRH_REQ(hy > 0 && hx > 0, "frh_motion_tensor: grid spacings must be positive");
+    FRH_REQ(kind != FRH_TENSOR_CS || eps > 0, "frh_motion_tensor: census eps must be positive");
+    hipError_t e = launch_tensor_rowmajor(kind, f1_f32, f2_f32, f1, f2, h, w, hy, hx, eps, J11, J22, J33,
+                                          J12, J13, J23, as_stream(stream));
     FRH_HIP(e, "tensor kernel");
     return FRH_OK;
 }
@@ -259,7 +278,7 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
                      const double *J13, const double *J23, const double *weight, const double *u,
                      const double *v, int m, int n, int K, double alpha_x, double alpha_y,
                      int iterations, int update_lag, const double *a_data, double a_smooth,
-                     double hx, double hy, double *du, double *dv, void *stream)
+                     double hx, double hy, double gnc_beta, double *du, double *dv, void *stream)
 {
     FRH_REQ(J11 && J22 && J33 && J12 && J13 && J23 && weight && u && v && a_data && du && dv,
             "frh_level_solver: null pointer");
@@ -294,12 +313,13 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
         double *rg = k + 1 == passes.size() ? ring : nullptr;
         if (fused) {
             e = launch_sweep_fused(passes[k].nsweeps, J, Wsk, cuv, duv, rg, fstride, 1, K, s, ax, ay,
-                                   am1.data(), a_data, passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0, st);
+                                   am1.data(), a_data, passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0,
+                                   gnc_beta, st);
             continue;
         }
         if (passes[k].assemble)
             e = launch_assemble(J, Wsk, cuv, duv, coef, fstride, 1, K, s, ax, ay, am1.data(), a_data,
-                                passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0, st);
+                                passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0, gnc_beta, st);
         if (e == hipSuccess)
             e = launch_sweep(passes[k].nsweeps, coef, duv, rg, fstride, 1, s, ax, ay, k == 0 ? 1 : 0, st);
     }
@@ -329,6 +349,7 @@ enum { TM_TOTAL, TM_SWEEP, TM_ASSEMBLE, TM_TENSOR, TM_RESIZE, TM_WARP, TM_MEDIAN
 
 struct frh_solver {
     int H, W, C, nl, iterations, update_lag, max_batch;
+    int const_assumption = FRH_TENSOR_GC;
     double a_smooth;
     std::vector<double> a_data, am1;
     std::vector<LevelDev> lv;
@@ -411,9 +432,12 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
     FRH_REQ(d->n_levels >= 1 && d->max_batch >= 1 && d->max_batch <= 65535, "frh_solver_create: bad level count or batch");
     FRH_REQ(d->iterations >= 0 && d->update_lag >= 1, "frh_solver_create: iterations >= 0, update_lag >= 1");
     if (d->a_smooth != 1.0) return fail(FRH_EUNSUP, "a_smooth != 1 is not implemented on the GPU");
+    FRH_REQ(d->const_assumption >= FRH_TENSOR_GC && d->const_assumption <= FRH_TENSOR_CS,
+            "frh_solver_create: unknown data term %d", d->const_assumption);
     frh_solver *s = new frh_solver();
     s->H = d->H; s->W = d->W; s->C = d->C; s->nl = d->n_levels; s->max_batch = d->max_batch;
     s->iterations = d->iterations; s->update_lag = d->update_lag; s->a_smooth = d->a_smooth;
+    s->const_assumption = d->const_assumption;
     s->a_data.assign(d->a_data, d->a_data + d->C);
     for (double a : s->a_data) s->am1.push_back(a - 1.0);
     size_t f1_total = 0, W_total = 0, plane_max = 0;
@@ -584,8 +608,11 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         }
         {
             Scope sc(s, TM_TENSOR, st);
-            FRH_HIP(launch_tensor(s->f1lev + L.f1_off, f2src, T, C, h, w, L.h_row, L.h_col, s->uvl,
-                                  s->uvl + hw, 2 * hw, L.axw, L.ayw, J, cuv, fs, sk, st),
+            // the top level's "warped" image is the resized level itself
+            // (float64 data in the reference); below it the warp's float32
+            FRH_HIP(launch_tensor(s->const_assumption, li == 0 ? 0 : 1, s->f1lev + L.f1_off, f2src, T, C,
+                                  h, w, L.h_row, L.h_col, kCensusEps, s->uvl, s->uvl + hw, 2 * hw, L.axw,
+                                  L.ayw, J, cuv, fs, sk, st),
                     "motion tensor");
         }
         // The separate assemble kernel is faster than forming coefficients
@@ -598,14 +625,14 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
                 Scope sc(s, TM_SWEEP, st);
                 FRH_HIP(launch_sweep_fused(p.nsweeps, J, s->Wlev + L.W_off, cuv, duv, nullptr, fs, T, C, sk,
                                            L.axw, L.ayw, s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0,
-                                           k == 0 ? 1 : 0, st), "sor sweep (fused)");
+                                           k == 0 ? 1 : 0, -1.0, st), "sor sweep (fused)");
                 continue;
             }
             if (p.assemble) {
                 Scope sc(s, TM_ASSEMBLE, st);
                 FRH_HIP(launch_assemble(J, s->Wlev + L.W_off, cuv, duv, coef, fs, T, C, sk, L.axw, L.ayw,
                                         s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
-                                        st), "assemble");
+                                        -1.0, st), "assemble");
             }
             Scope sc(s, TM_SWEEP, st);
             FRH_HIP(launch_sweep(p.nsweeps, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw, k == 0 ? 1 : 0, st),

@@ -45,14 +45,18 @@ hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int 
 hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
                        int W, const float *u, const float *v, size_t uv_frame_stride,
                        double h_col, double h_row, float *out, hipStream_t st);
-hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
-                         double hx, const float *u, const float *v, size_t uv_frame_stride,
-                         double ax, double ay, double *J, double *cuv, size_t frame_stride,
-                         const Skew &s, hipStream_t st);
+hipError_t launch_warp_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                           int W, const double *u, const double *v, size_t uv_frame_stride,
+                           float *out, hipStream_t st);
+hipError_t launch_tensor(int kind, int f2_f32, const float *f1, const float *f2, int T, int C, int h,
+                         int w, double hy, double hx, double eps, const float *u, const float *v,
+                         size_t uv_frame_stride, double ax, double ay, double *J, double *cuv,
+                         size_t frame_stride, const Skew &s, hipStream_t st);
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
                            const double *duv, double *coef, size_t frame_stride, int T, int C,
                            const Skew &s, double ax, double ay, const double *a_data_minus1,
-                           const double *a_data, int psi_one, int zero_init, hipStream_t st);
+                           const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                           hipStream_t st);
 hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
                         int T, const Skew &s, double ax, double ay, int zero_init,
                         hipStream_t st);
@@ -73,9 +77,10 @@ hipError_t launch_padded_to_skew_f64(const double *src, int m, int n, int K, con
 hipError_t launch_skew_to_padded_f64(const double *duv, const double *ring, const Skew &s,
                                      double *du, double *dv, hipStream_t st);
 hipError_t launch_median_plane(double *f, int N, int h, int w, double *tmp, hipStream_t st);
-hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w, double hy,
-                                  double hx, double *J11, double *J22, double *J33, double *J12,
-                                  double *J13, double *J23, hipStream_t st);
+hipError_t launch_tensor_rowmajor(int kind, int f1_f32, int f2_f32, const float *f1, const float *f2,
+                                  int h, int w, double hy, double hx, double eps, double *J11,
+                                  double *J22, double *J33, double *J12, double *J13, double *J23,
+                                  hipStream_t st);
 hipError_t launch_cuv_padded(const double *u, const double *v, int m, int n, double ax,
                              double ay, const Skew &s, double *cuv, hipStream_t st);
 
@@ -88,7 +93,8 @@ bool fused_sweep_supported(int C);
 hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const double *cuv,
                               double *duv, double *ring, size_t frame_stride, int T, int C,
                               const Skew &s, double ax, double ay, const double *a_data_minus1,
-                              const double *a_data, int psi_one, int zero_init, hipStream_t st);
+                              const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                              hipStream_t st);
 
 hipError_t launch_cv_resize(const double *src, int T, int h, int w, int C, double *tmp,
                             double *dst, int oh, int ow, hipStream_t st);

@@ -274,9 +274,12 @@ __device__ inline void cv_cubic_coeffs(float x, float *c)
     c[3] = 1.f - c[0] - c[1] - c[2];
 }
 
+// U: float (level flows, divided by h_col / h_row in float64 here) or
+// double (displacements already divided by the caller, h = 1).
+template <typename U>
 __global__ void k_warp(const float *__restrict__ f2, const float *__restrict__ f1,
-                       int f1_per_frame, int C, int H, int W, const float *__restrict__ u,
-                       const float *__restrict__ v, size_t uv_stride, double h_col, double h_row,
+                       int f1_per_frame, int C, int H, int W, const U *__restrict__ u,
+                       const U *__restrict__ v, size_t uv_stride, double h_col, double h_row,
                        float *__restrict__ out)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -336,63 +339,212 @@ hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T
                        int W, const float *u, const float *v, size_t uv_frame_stride,
                        double h_col, double h_row, float *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_warp, dim3((W + 63) / 64, H, T), dim3(64), 0, st, f2, f1, f1_per_frame, C,
-                       H, W, u, v, uv_frame_stride, h_col, h_row, out);
+    hipLaunchKernelGGL(k_warp<float>, dim3((W + 63) / 64, H, T), dim3(64), 0, st, f2, f1, f1_per_frame,
+                       C, H, W, u, v, uv_frame_stride, h_col, h_row, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_warp_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                           int W, const double *u, const double *v, size_t uv_frame_stride,
+                           float *out, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_warp<double>, dim3((W + 63) / 64, H, T), dim3(64), 0, st, f2, f1, f1_per_frame,
+                       C, H, W, u, v, uv_frame_stride, 1.0, 1.0, out);
     return hipGetLastError();
 }
 
 // ---------------------------------------------------------- motion tensor
-// core/optical_flow.py:73-160 (get_motion_tensor_gc).  Symmetric 1-px pads
-// are clamped reads; np.gradient is central in the interior; the re-padded
-// fx / ft ring copies the nearest interior value.  Products follow numpy's
-// left-to-right evaluation of the reference expressions.
+// core/optical_flow.py:73-345 (get_motion_tensor_gc / _gray / _cs), one
+// cell of one channel.  Symmetric 1-px pads are clamped reads; np.gradient
+// is central in the interior; the re-padded ring of the first derivatives
+// copies the nearest interior value.
+//
+// numpy evaluates each expression in the dtype of its array operands: the
+// level image f1 is float64 (the resize casts back to the input dtype);
+// the warped image f2 is float32 below the top level (imregister_wrapper
+// returns float32) and float64 at the top level.  P1 / P2 mark a float32
+// input and PM = P1 && P2 the terms that combine both.  A float32 operation
+// is evaluated as the double operation rounded to float32 (exact for
+// + - * / sqrt of float32 operands, since double has more than 2 * 24 + 2
+// bits); a Python scalar meets a float32 array as a float32.
+enum TensorKind { kTensorGC = 0, kTensorGray = 1, kTensorCS = 2 };
+
 struct TensorOut {
     double J11, J22, J33, J12, J13, J23;
 };
 
+template <bool F32>
+__device__ inline double rnd(double x)
+{
+    if constexpr (F32)
+        return (double)(float)x;
+    else
+        return x;
+}
+
+// np.gradient interior value (x[+1] - x[-1]) / (2. * h)
+template <bool F32>
+__device__ inline double dgrad(double xp, double xm, double h)
+{
+    return rnd<F32>(rnd<F32>(xp - xm) / rnd<F32>(2.0 * h));
+}
+
+// (x[-1] - 2 * x[0] + x[+1]) / (h ** 2)
+template <bool F32>
+__device__ inline double dsecond(double xm, double x0, double xp, double h)
+{
+    return rnd<F32>(rnd<F32>(rnd<F32>(xm - 2 * x0) + xp) / rnd<F32>(h * h));
+}
+
 __device__ inline double img(const float *f, int w, int r, int c) { return (double)f[(size_t)r * w + c]; }
 
-__device__ inline double tensor_fx(const float *f1, const float *f2, int h, int w, int r, int c,
-                                   double hx)
+struct Clamp3 {
+    int ru, rd, cl, cr;
+};
+
+__device__ inline Clamp3 clamp3(int h, int w, int r, int c)
 {
-    int cl = c - 1 < 0 ? 0 : c - 1, cr = c + 1 > w - 1 ? w - 1 : c + 1;
-    double g1 = (img(f1, w, r, cr) - img(f1, w, r, cl)) / (2 * hx);
-    double g2 = (img(f2, w, r, cr) - img(f2, w, r, cl)) / (2 * hx);
-    return 0.5 * (g1 + g2);
+    Clamp3 k;
+    k.ru = r > 0 ? r - 1 : 0;
+    k.rd = r < h - 1 ? r + 1 : h - 1;
+    k.cl = c > 0 ? c - 1 : 0;
+    k.cr = c < w - 1 ? c + 1 : w - 1;
+    return k;
 }
 
-__device__ inline double tensor_ft(const float *f1, const float *f2, int w, int r, int c)
-{
-    return img(f2, w, r, c) - img(f1, w, r, c);
-}
-
+template <bool P1, bool P2>
 __device__ inline TensorOut tensor_gc(const float *f1, const float *f2, int h, int w, int r, int c,
                                       double hy, double hx)
 {
-    int rd = r + 1 > h - 1 ? h - 1 : r + 1, ru = r - 1 < 0 ? 0 : r - 1;
-    int cr = c + 1 > w - 1 ? w - 1 : c + 1, cl = c - 1 < 0 ? 0 : c - 1;
-    double fxy = (tensor_fx(f1, f2, h, w, rd, c, hx) - tensor_fx(f1, f2, h, w, ru, c, hx)) / (2 * hy);
-    double fxt = (tensor_ft(f1, f2, w, r, cr) - tensor_ft(f1, f2, w, r, cl)) / (2 * hx);
-    double fyt = (tensor_ft(f1, f2, w, rd, c) - tensor_ft(f1, f2, w, ru, c)) / (2 * hy);
-    double a0 = img(f1, w, r, c), al = img(f1, w, r, cl), ar = img(f1, w, r, cr);
-    double au = img(f1, w, ru, c), ad = img(f1, w, rd, c);
-    double b0 = img(f2, w, r, c), bl = img(f2, w, r, cl), br = img(f2, w, r, cr);
-    double bu = img(f2, w, ru, c), bd = img(f2, w, rd, c);
-    double fxx1 = (al - 2 * a0 + ar) / (hx * hx);
-    double fyy1 = (au - 2 * a0 + ad) / (hy * hy);
-    double fxx2 = (bl - 2 * b0 + br) / (hx * hx);
-    double fyy2 = (bu - 2 * b0 + bd) / (hy * hy);
-    double fxx = 0.5 * (fxx1 + fxx2), fyy = 0.5 * (fyy1 + fyy2);
-    double sx = sqrt(fxx * fxx + fxy * fxy), sy = sqrt(fxy * fxy + fyy * fyy);
-    double regx = 1 / (sx * sx + 1e-6), regy = 1 / (sy * sy + 1e-6);
+    constexpr bool PM = P1 && P2;
+    const Clamp3 k = clamp3(h, w, r, c);
+    // fx = 0.5 (d/dx f1 + d/dx f2) on rows r-1, r+1 (clamped: the re-pad)
+    auto fx_at = [&](int rr) {
+        double g1 = dgrad<P1>(img(f1, w, rr, k.cr), img(f1, w, rr, k.cl), hx);
+        double g2 = dgrad<P2>(img(f2, w, rr, k.cr), img(f2, w, rr, k.cl), hx);
+        return rnd<PM>(0.5 * rnd<PM>(g1 + g2));
+    };
+    auto ft_at = [&](int rr, int cc) { return rnd<PM>(img(f2, w, rr, cc) - img(f1, w, rr, cc)); };
+    const double fxy = dgrad<PM>(fx_at(k.rd), fx_at(k.ru), hy);
+    const double fxt = dgrad<PM>(ft_at(r, k.cr), ft_at(r, k.cl), hx);
+    const double fyt = dgrad<PM>(ft_at(k.rd, c), ft_at(k.ru, c), hy);
+    const double a0 = img(f1, w, r, c), b0 = img(f2, w, r, c);
+    const double fxx1 = dsecond<P1>(img(f1, w, r, k.cl), a0, img(f1, w, r, k.cr), hx);
+    const double fyy1 = dsecond<P1>(img(f1, w, k.ru, c), a0, img(f1, w, k.rd, c), hy);
+    const double fxx2 = dsecond<P2>(img(f2, w, r, k.cl), b0, img(f2, w, r, k.cr), hx);
+    const double fyy2 = dsecond<P2>(img(f2, w, k.ru, c), b0, img(f2, w, k.rd, c), hy);
+    const double fxx = rnd<PM>(0.5 * rnd<PM>(fxx1 + fxx2));
+    const double fyy = rnd<PM>(0.5 * rnd<PM>(fyy1 + fyy2));
+    // reg = 1.0 / ((np.sqrt(a**2 + b**2) ** 2) + 1e-6)
+    const double sx = rnd<PM>(sqrt(rnd<PM>(rnd<PM>(fxx * fxx) + rnd<PM>(fxy * fxy))));
+    const double sy = rnd<PM>(sqrt(rnd<PM>(rnd<PM>(fxy * fxy) + rnd<PM>(fyy * fyy))));
+    const double regx = rnd<PM>(1.0 / rnd<PM>(rnd<PM>(sx * sx) + rnd<PM>(1e-6)));
+    const double regy = rnd<PM>(1.0 / rnd<PM>(rnd<PM>(sy * sy) + rnd<PM>(1e-6)));
+    // reg_x * fxx**2 is reg_x * (fxx * fxx); products left to right
     TensorOut o;
-    o.J11 = regx * (fxx * fxx) + regy * (fxy * fxy);
-    o.J22 = regx * (fxy * fxy) + regy * (fyy * fyy);
-    o.J33 = regx * (fxt * fxt) + regy * (fyt * fyt);
-    o.J12 = regx * fxx * fxy + regy * fxy * fyy;
-    o.J13 = regx * fxx * fxt + regy * fxy * fyt;
-    o.J23 = regx * fxy * fxt + regy * fyy * fyt;
+    o.J11 = rnd<PM>(rnd<PM>(regx * rnd<PM>(fxx * fxx)) + rnd<PM>(regy * rnd<PM>(fxy * fxy)));
+    o.J22 = rnd<PM>(rnd<PM>(regx * rnd<PM>(fxy * fxy)) + rnd<PM>(regy * rnd<PM>(fyy * fyy)));
+    o.J33 = rnd<PM>(rnd<PM>(regx * rnd<PM>(fxt * fxt)) + rnd<PM>(regy * rnd<PM>(fyt * fyt)));
+    o.J12 = rnd<PM>(rnd<PM>(rnd<PM>(regx * fxx) * fxy) + rnd<PM>(rnd<PM>(regy * fxy) * fyy));
+    o.J13 = rnd<PM>(rnd<PM>(rnd<PM>(regx * fxx) * fxt) + rnd<PM>(rnd<PM>(regy * fxy) * fyt));
+    o.J23 = rnd<PM>(rnd<PM>(rnd<PM>(regx * fxy) * fxt) + rnd<PM>(rnd<PM>(regy * fyy) * fyt));
     return o;
+}
+
+// core/optical_flow.py:163-228
+template <bool P1, bool P2>
+__device__ inline TensorOut tensor_gray(const float *f1, const float *f2, int h, int w, int r,
+                                        int c, double hy, double hx)
+{
+    constexpr bool PM = P1 && P2;
+    const Clamp3 k = clamp3(h, w, r, c);
+    const double fx1 = dgrad<P1>(img(f1, w, r, k.cr), img(f1, w, r, k.cl), hx);
+    const double fy1 = dgrad<P1>(img(f1, w, k.rd, c), img(f1, w, k.ru, c), hy);
+    const double fx2 = dgrad<P2>(img(f2, w, r, k.cr), img(f2, w, r, k.cl), hx);
+    const double fy2 = dgrad<P2>(img(f2, w, k.rd, c), img(f2, w, k.ru, c), hy);
+    const double fx = rnd<PM>(0.5 * rnd<PM>(fx1 + fx2));
+    const double fy = rnd<PM>(0.5 * rnd<PM>(fy1 + fy2));
+    const double ft = rnd<PM>(img(f2, w, r, c) - img(f1, w, r, c));
+    TensorOut o;
+    o.J11 = rnd<PM>(fx * fx);
+    o.J22 = rnd<PM>(fy * fy);
+    o.J33 = rnd<PM>(ft * ft);
+    o.J12 = rnd<PM>(fx * fy);
+    o.J13 = rnd<PM>(fx * ft);
+    o.J23 = rnd<PM>(fy * ft);
+    return o;
+}
+
+// smoothed Heaviside 0.5 * (1.0 + r / np.sqrt(r * r + eps2)) of the
+// directional difference r = (neighbour - centre) / dist
+template <bool F32>
+__device__ inline double census_s(double nb, double ce, double dist, double eps2)
+{
+    const double rr = rnd<F32>(rnd<F32>(nb - ce) / rnd<F32>(dist));
+    const double d = rnd<F32>(sqrt(rnd<F32>(rnd<F32>(rr * rr) + rnd<F32>(eps2))));
+    return rnd<F32>(0.5 * rnd<F32>(1.0 + rnd<F32>(rr / d)));
+}
+
+// core/optical_flow.py:231-345: the eight neighbour offsets in row-major
+// order; per offset, the census images of both frames on the h x w core
+// (edge-padded, i.e. clamped) and their np.gradient derivatives at the
+// cell; float64 accumulation, then the scale 2 / 8.
+template <bool P1, bool P2>
+__device__ inline TensorOut tensor_cs(const float *f1, const float *f2, int h, int w, int r, int c,
+                                      double hy, double hx, double eps)
+{
+    constexpr bool PM = P1 && P2;
+    const double eps2 = eps * eps;
+    const Clamp3 k = clamp3(h, w, r, c);
+    double A0 = 0, A1 = 0, A2 = 0, A3 = 0, A4 = 0, A5 = 0;
+    for (int dy = -1; dy <= 1; ++dy)
+        for (int dx = -1; dx <= 1; ++dx) {
+            if (dy == 0 && dx == 0) continue;
+            const double dist = sqrt((hy * dy) * (hy * dy) + (hx * dx) * (hx * dx));
+            // census value of image f at core pixel (rr, cc) for this offset;
+            // the neighbour comes from the symmetric pad (clamped)
+            auto s_at = [&](const float *f, bool f32, int rr, int cc) {
+                int nr = rr + dy, nc = cc + dx;
+                nr = nr < 0 ? 0 : (nr > h - 1 ? h - 1 : nr);
+                nc = nc < 0 ? 0 : (nc > w - 1 ? w - 1 : nc);
+                const double nb = img(f, w, nr, nc), ce = img(f, w, rr, cc);
+                return f32 ? census_s<true>(nb, ce, dist, eps2) : census_s<false>(nb, ce, dist, eps2);
+            };
+            const double sx1 = dgrad<P1>(s_at(f1, P1, r, k.cr), s_at(f1, P1, r, k.cl), hx);
+            const double sx2 = dgrad<P2>(s_at(f2, P2, r, k.cr), s_at(f2, P2, r, k.cl), hx);
+            const double sy1 = dgrad<P1>(s_at(f1, P1, k.rd, c), s_at(f1, P1, k.ru, c), hy);
+            const double sy2 = dgrad<P2>(s_at(f2, P2, k.rd, c), s_at(f2, P2, k.ru, c), hy);
+            const double sx = rnd<PM>(0.5 * rnd<PM>(sx1 + sx2));
+            const double sy = rnd<PM>(0.5 * rnd<PM>(sy1 + sy2));
+            const double st = rnd<PM>(s_at(f2, P2, r, c) - s_at(f1, P1, r, c));
+            A0 = A0 + rnd<PM>(sx * sx);
+            A1 = A1 + rnd<PM>(sy * sy);
+            A2 = A2 + rnd<PM>(st * st);
+            A3 = A3 + rnd<PM>(sx * sy);
+            A4 = A4 + rnd<PM>(sx * st);
+            A5 = A5 + rnd<PM>(sy * st);
+        }
+    const double scale = 2.0 / 8.0;
+    TensorOut o;
+    o.J11 = A0 * scale;
+    o.J22 = A1 * scale;
+    o.J33 = A2 * scale;
+    o.J12 = A3 * scale;
+    o.J13 = A4 * scale;
+    o.J23 = A5 * scale;
+    return o;
+}
+
+template <int KIND, bool P1, bool P2>
+__device__ inline TensorOut tensor_cell(const float *f1, const float *f2, int h, int w, int r, int c,
+                                        double hy, double hx, double eps)
+{
+    if constexpr (KIND == kTensorGC)
+        return tensor_gc<P1, P2>(f1, f2, h, w, r, c, hy, hx);
+    else if constexpr (KIND == kTensorGray)
+        return tensor_gray<P1, P2>(f1, f2, h, w, r, c, hy, hx);
+    else
+        return tensor_cs<P1, P2>(f1, f2, h, w, r, c, hy, hx, eps);
 }
 
 // cu = sum_nb w_nb (u_nb - u_c) over (left, right, down, up); u is
@@ -416,13 +568,16 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
 // are numbered frame-major, then row-major, and dealt to the XCDs in
 // contiguous ranges (xcd_block) so neighbouring tiles share one L2.
 // J planes (component-major, then channel) and cu, cv are written skewed.
+// The level image f1 is float64 data in the reference; P2 marks a float32
+// warped image (every level below the top).
 constexpr int kTensorRows = 16;
 
+template <int KIND, bool P2>
 __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
-    size_t plane, double hy, double hx, const float *__restrict__ u, const float *__restrict__ v,
-    size_t uv_stride, double ax, double ay, double *__restrict__ J, double *__restrict__ cuv,
-    size_t fstride, int T)
+    size_t plane, double hy, double hx, double eps, const float *__restrict__ u,
+    const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
+    double *__restrict__ cuv, size_t fstride, int T)
 {
     const int tiles_x = (w + 63) / 64, tiles_y = (h + kTensorRows - 1) / kTensorRows;
     const unsigned nb = (unsigned)tiles_x * tiles_y * T;
@@ -434,7 +589,8 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     size_t hw = (size_t)h * w, q = (size_t)(r + c) * hs + r;
     double *Jt = J + (size_t)t * fstride;
     for (int ch = 0; ch < C; ++ch) {
-        TensorOut o = tensor_gc(f1 + ch * hw, f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx);
+        TensorOut o = tensor_cell<KIND, false, P2>(f1 + ch * hw, f2 + ((size_t)t * C + ch) * hw, h, w,
+                                                  r, c, hy, hx, eps);
         Jt[(size_t)(0 * C + ch) * plane + q] = o.J11;
         Jt[(size_t)(1 * C + ch) * plane + q] = o.J22;
         Jt[(size_t)(2 * C + ch) * plane + q] = o.J33;
@@ -447,24 +603,36 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     ct[plane + q] = smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay);
 }
 
-hipError_t launch_tensor(const float *f1, const float *f2, int T, int C, int h, int w, double hy,
-                         double hx, const float *u, const float *v, size_t uv_frame_stride,
-                         double ax, double ay, double *J, double *cuv, size_t frame_stride,
-                         const Skew &s, hipStream_t st)
+hipError_t launch_tensor(int kind, int f2_f32, const float *f1, const float *f2, int T, int C, int h,
+                         int w, double hy, double hx, double eps, const float *u, const float *v,
+                         size_t uv_frame_stride, double ax, double ay, double *J, double *cuv,
+                         size_t frame_stride, const Skew &s, hipStream_t st)
 {
     const size_t nb = (size_t)((w + 63) / 64) * ((h + kTensorRows - 1) / kTensorRows) * T;
     if (nb > 0x7fffffffu) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_tensor, dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, C, h, w,
-                       s.hs, s.plane, hy, hx, u, v, uv_frame_stride, ax, ay, J, cuv, frame_stride,
-                       T);
+#define FRH_TEN(K, P)                                                                             \
+    hipLaunchKernelGGL((k_tensor<K, P>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, C, \
+                       h, w, s.hs, s.plane, hy, hx, eps, u, v, uv_frame_stride, ax, ay, J, cuv,     \
+                       frame_stride, T)
+    switch (kind * 2 + (f2_f32 ? 1 : 0)) {
+    case 0: FRH_TEN(kTensorGC, false); break;
+    case 1: FRH_TEN(kTensorGC, true); break;
+    case 2: FRH_TEN(kTensorGray, false); break;
+    case 3: FRH_TEN(kTensorGray, true); break;
+    case 4: FRH_TEN(kTensorCS, false); break;
+    case 5: FRH_TEN(kTensorCS, true); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef FRH_TEN
     return hipGetLastError();
 }
 
-// Row-major tensor for the stand-alone get_motion_tensor_gc entry point:
+// Row-major tensor for the stand-alone motion-tensor entry point:
 // (h+2, w+2) outputs with a zero ring.
+template <int KIND, bool P1, bool P2>
 __global__ void k_tensor_rowmajor(const float *__restrict__ f1, const float *__restrict__ f2,
-                                  int h, int w, double hy, double hx, double *J11, double *J22,
-                                  double *J33, double *J12, double *J13, double *J23)
+                                  int h, int w, double hy, double hx, double eps, double *J11,
+                                  double *J22, double *J33, double *J12, double *J13, double *J23)
 {
     int i = blockIdx.x * blockDim.x + threadIdx.x, j = blockIdx.y;
     int wp = w + 2;
@@ -474,7 +642,7 @@ __global__ void k_tensor_rowmajor(const float *__restrict__ f1, const float *__r
         J11[q] = J22[q] = J33[q] = J12[q] = J13[q] = J23[q] = 0.0;
         return;
     }
-    TensorOut o = tensor_gc(f1, f2, h, w, j - 1, i - 1, hy, hx);
+    TensorOut o = tensor_cell<KIND, P1, P2>(f1, f2, h, w, j - 1, i - 1, hy, hx, eps);
     J11[q] = o.J11;
     J22[q] = o.J22;
     J33[q] = o.J33;
@@ -483,12 +651,31 @@ __global__ void k_tensor_rowmajor(const float *__restrict__ f1, const float *__r
     J23[q] = o.J23;
 }
 
-hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w, double hy,
-                                  double hx, double *J11, double *J22, double *J33, double *J12,
-                                  double *J13, double *J23, hipStream_t st)
+hipError_t launch_tensor_rowmajor(int kind, int f1_f32, int f2_f32, const float *f1, const float *f2,
+                                  int h, int w, double hy, double hx, double eps, double *J11,
+                                  double *J22, double *J33, double *J12, double *J13, double *J23,
+                                  hipStream_t st)
 {
-    hipLaunchKernelGGL(k_tensor_rowmajor, dim3((w + 2 + 63) / 64, h + 2), dim3(64), 0, st, f1, f2,
-                       h, w, hy, hx, J11, J22, J33, J12, J13, J23);
+    dim3 g((w + 2 + 63) / 64, h + 2), b(64);
+#define FRH_TRM(K, A, B)                                                                          \
+    hipLaunchKernelGGL((k_tensor_rowmajor<K, A, B>), g, b, 0, st, f1, f2, h, w, hy, hx, eps, J11, J22, \
+                       J33, J12, J13, J23)
+    switch (kind * 4 + (f1_f32 ? 2 : 0) + (f2_f32 ? 1 : 0)) {
+    case 0: FRH_TRM(kTensorGC, false, false); break;
+    case 1: FRH_TRM(kTensorGC, false, true); break;
+    case 2: FRH_TRM(kTensorGC, true, false); break;
+    case 3: FRH_TRM(kTensorGC, true, true); break;
+    case 4: FRH_TRM(kTensorGray, false, false); break;
+    case 5: FRH_TRM(kTensorGray, false, true); break;
+    case 6: FRH_TRM(kTensorGray, true, false); break;
+    case 7: FRH_TRM(kTensorGray, true, true); break;
+    case 8: FRH_TRM(kTensorCS, false, false); break;
+    case 9: FRH_TRM(kTensorCS, false, true); break;
+    case 10: FRH_TRM(kTensorCS, true, false); break;
+    case 11: FRH_TRM(kTensorCS, true, true); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef FRH_TRM
     return hipGetLastError();
 }
 
@@ -498,10 +685,28 @@ hipError_t launch_tensor_rowmajor(const float *f1, const float *f2, int h, int w
 //   Du = (((wl+wr)+wd)+wu) + sum_k Wpsi_k J11_k,  Dv likewise with J22,
 //   A12 = sum_k Wpsi_k J12_k, Bu = cu - sum_k Wpsi_k J13_k, Bv = cv - ...J23,
 //   stored as {A12, Bu, Bv, omega/Du, omega/Dv}.
+// a_k, a_k - 1 per channel; GNC (core/level_solver.py:372-555): when the
+// stage weight beta is set and 0 < a_k < 1, beta < 1, psi_k becomes
+// (1 - beta) + beta * robust.
 struct AData {
     double a[8];
     double am1[8];
+    int gnc[8];
+    double mix, beta;
 };
+
+inline AData make_adata(int C, const double *a_data, const double *a_data_minus1, double gnc_beta)
+{
+    AData ad{};
+    for (int k = 0; k < C && k < 8; ++k) {
+        ad.a[k] = a_data[k];
+        ad.am1[k] = a_data_minus1[k];
+        ad.gnc[k] = gnc_beta >= 0 && a_data[k] > 0.0 && a_data[k] < 1.0 && gnc_beta < 1.0;
+    }
+    ad.mix = 1.0 - gnc_beta;
+    ad.beta = gnc_beta;
+    return ad;
+}
 
 // Coefficients of one cell from its raw data: J (6 components x C channels,
 // component-major), W (C), cu, cv, and the cell's start values (x, y).
@@ -522,6 +727,7 @@ __device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc,
                          j23 * y + j33;
             if (val < 0) val = 0;
             psi = ad.a[k] * frh_det_pow(val + 1e-5, ad.am1[k]);
+            if (ad.gnc[k]) psi = ad.mix + ad.beta * psi;
         }
         double wp = Wc[k] * psi;
         Du = Du + wp * j11;
@@ -570,13 +776,10 @@ __global__ void k_assemble(const double *__restrict__ J, const double *__restric
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
                            const double *duv, double *coef, size_t frame_stride, int T, int C,
                            const Skew &s, double ax, double ay, const double *a_data_minus1,
-                           const double *a_data, int psi_one, int zero_init, hipStream_t st)
+                           const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                           hipStream_t st)
 {
-    AData ad{};
-    for (int k = 0; k < C && k < 8; ++k) {
-        ad.a[k] = a_data[k];
-        ad.am1[k] = a_data_minus1[k];
-    }
+    const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
     dim3 g(s.h + s.w - 1, T), b(s.hs);
 #define FRH_ASM(CC)                                                                            \
     hipLaunchKernelGGL(k_assemble<CC>, g, b, 0, st, J, Wsk, cuv, duv, coef, frame_stride, s.h, s.w, \
@@ -1009,16 +1212,14 @@ bool fused_sweep_supported(int C) { return C == 1 || C == 2; }
 hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const double *cuv,
                               double *duv, double *ring, size_t frame_stride, int T, int C,
                               const Skew &s, double ax, double ay, const double *a_data_minus1,
-                              const double *a_data, int psi_one, int zero_init, hipStream_t st)
+                              const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                              hipStream_t st)
 {
     SweepArgs a{};
     a.coef = nullptr; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
     a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
     a.J = J; a.Wsk = Wsk; a.cuv = cuv; a.psi_one = psi_one;
-    for (int k = 0; k < C && k < 8; ++k) {
-        a.ad.a[k] = a_data[k];
-        a.ad.am1[k] = a_data_minus1[k];
-    }
+    a.ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
     switch (P) {
     case 1: return launch_sweep_p<1>(a, C, T, st);
     case 2: return launch_sweep_p<2>(a, C, T, st);

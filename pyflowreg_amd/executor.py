@@ -109,7 +109,7 @@ class HipBatchedExecutor:
         return {"name": self.name, "n_workers": 1, "parallel": True,
                 "description": "GPU batched frame registration (flowreg_hip)"}
 
-    def _get_solver(self, H, W, C, T, fp):
+    def _get_solver(self, H, W, C, T, fp, term):
         alpha = fp.get("alpha", (2, 2))
         if np.isscalar(alpha):
             alpha = (alpha, alpha)
@@ -119,13 +119,13 @@ class HipBatchedExecutor:
                int(fp.get("levels", 50)), float(fp.get("eta", 0.8)),
                float(fp.get("a_smooth", 1.0)),
                tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist()),
-               min(T, self.max_batch))
+               min(T, self.max_batch), term)
         if key != self._key:
             self.cleanup()
             self._solver = DisplacementSolver(
                 H, W, C, alpha=key[3], update_lag=key[4], iterations=key[5], min_level=key[6],
                 levels=key[7], eta=key[8], a_smooth=key[9], a_data=np.array(key[10]),
-                max_batch=key[11])
+                max_batch=key[11], const_assumption=term)
             self._key = key
         return self._solver
 
@@ -143,10 +143,7 @@ class HipBatchedExecutor:
             raise ValueError("hip_batched: cc_initialization is not implemented")
         fp = {k: v for k, v in flow_params_all.items()
               if k not in ("cc_initialization", "cc_hw", "cc_up")}
-        if fp.get("gnc_schedule") is not None:
-            raise ValueError("hip_batched: GNC schedules are not implemented")
-        if _resolve_const_assumption(fp.get("const_assumption", "gc")) != "gc":
-            raise ValueError("hip_batched implements the 'gc' data term only")
+        term = _resolve_const_assumption(fp.get("const_assumption", "gc"))
         if str(interpolation_method).lower() != "cubic":
             raise ValueError("hip_batched: only cubic registration warps are implemented")
         as4 = lambda x: x[..., None] if x.ndim == 3 else x  # noqa: E731
@@ -158,9 +155,18 @@ class HipBatchedExecutor:
         wi = np.ascontiguousarray(np.asarray(w_init).astype(np.float32, copy=False))
         T, H, W, C = batch.shape
         dev = _device()
-        solver = self._get_solver(H, W, C, T, fp)
-        solver.set_reference(rp, fp.get("weight"), wi)
-        flow = solver.run(torch.from_numpy(bp).to(dev), out_dtype=torch.float32)
+        if fp.get("gnc_schedule") is not None:
+            # GNC stages relinearise on the host loop (pyflowreg_amd.stage_loop),
+            # one frame at a time, as the sequential executor does
+            from .optical_flow import get_displacement
+            kw = {k: v for k, v in fp.items() if k != "weight"}
+            flows_np = np.stack([get_displacement(rp, bp[t], weight=fp.get("weight"), uv=wi.copy(), **kw)
+                                 for t in range(T)]).astype(np.float32)
+            flow = torch.from_numpy(flows_np).to(dev)
+        else:
+            solver = self._get_solver(H, W, C, T, fp, term)
+            solver.set_reference(rp, fp.get("weight"), wi)
+            flow = solver.run(torch.from_numpy(bp).to(dev), out_dtype=torch.float32)
         reg = warp_frames(torch.from_numpy(batch.astype(np.float32)).to(dev), flow,
                           torch.from_numpy(rr).to(dev))
         registered = reg.cpu().numpy().astype(batch.dtype, copy=False)

@@ -33,6 +33,9 @@ _SUPPORTED_DATA_TERMS = {
     "gray": "gray", "brightness": "gray",
     "cs": "cs", "census": "cs",
 }
+# FRH_TENSOR_* of include/flowreg_hip.h
+_TENSOR_KIND = {"gc": 0, "gray": 1, "cs": 2}
+_CENSUS_EPS = 0.1 / 255.0
 
 
 def _device() -> torch.device:
@@ -148,6 +151,36 @@ def _resolve_const_assumption(const_assumption) -> str:
     return _SUPPORTED_DATA_TERMS[key]
 
 
+def normalize_gnc_schedule(gnc_schedule):
+    """core/optical_flow.py:414-432: validated float64 stage weights or None."""
+    if gnc_schedule is None:
+        return None
+    sch = np.asarray(gnc_schedule, dtype=np.float64)
+    if sch.ndim != 1:
+        raise ValueError("gnc_schedule must be a 1D sequence of stage weights")
+    if sch.size < 2:
+        raise ValueError("gnc_schedule must contain at least two stages")
+    if np.any(sch < 0.0) or np.any(sch > 1.0):
+        raise ValueError("gnc_schedule entries must lie in [0, 1]")
+    if not np.all(np.diff(sch) >= 0.0):
+        raise ValueError("gnc_schedule must be monotone nondecreasing")
+    if not np.isclose(sch[0], 0.0):
+        raise ValueError("gnc_schedule must start at 0.0")
+    if not np.isclose(sch[-1], 1.0):
+        raise ValueError("gnc_schedule must end at 1.0")
+    return np.ascontiguousarray(sch)
+
+
+def normalize_warping_steps(warping_steps):
+    """core/optical_flow.py:435-443."""
+    if warping_steps is None:
+        return None
+    steps = int(warping_steps)
+    if steps < 1:
+        raise ValueError("warping_steps must be a positive integer")
+    return steps
+
+
 def expand_weight(weight, m, n, C) -> np.ndarray:
     """core/optical_flow.py:856-879: (m, n, C) float64 weight map."""
     if weight is None:
@@ -190,7 +223,8 @@ class DisplacementSolver:
     of one BatchMotionCorrector batch).  Wraps frh_solver_*."""
 
     def __init__(self, H, W, C, *, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
-                 levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, max_batch=1):
+                 levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, max_batch=1,
+                 const_assumption="gc"):
         if np.isscalar(alpha):
             alpha = (float(alpha), float(alpha))
         alpha = tuple(float(a) for a in alpha)
@@ -224,6 +258,7 @@ class DisplacementSolver:
         sd.a_smooth = float(a_smooth)
         sd.a_data = ptr(self._a_data)
         sd.max_batch = self.max_batch
+        sd.const_assumption = _TENSOR_KIND[_resolve_const_assumption(const_assumption)]
         _device()
         h = _lib.c_void_p()
         check(lib().frh_solver_create(_lib.ctypes.byref(h), _lib.ctypes.byref(sd)),
@@ -294,10 +329,10 @@ class DisplacementSolver:
 
 @functools.lru_cache(maxsize=8)
 def _cached_solver(H, W, C, alpha, update_lag, iterations, min_level, levels, eta, a_smooth,
-                   a_data, max_batch):
+                   a_data, max_batch, term="gc"):
     return DisplacementSolver(H, W, C, alpha=alpha, update_lag=update_lag, iterations=iterations,
                               min_level=min_level, levels=levels, eta=eta, a_smooth=a_smooth,
-                              a_data=np.array(a_data), max_batch=max_batch)
+                              a_data=np.array(a_data), max_batch=max_batch, const_assumption=term)
 
 
 # ------------------------------------------------------------ get_displacement
@@ -308,16 +343,17 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
     """core/optical_flow.py:773-929 on the GPU.
 
     Returns (H, W, 2) float64 numpy ([..., 0] = u, [..., 1] = v) for numpy
-    inputs, or a float64 CUDA tensor when ``fixed`` is a tensor.
+    inputs, or a float64 CUDA tensor when ``fixed`` is a tensor.  Without a
+    GNC schedule or a caller-supplied level solver the whole pyramid runs on
+    the device (DisplacementSolver); otherwise the reference's level loop
+    runs on the host over the GPU stages (pyflowreg_amd.stage_loop).
     """
     assert fixed.ndim == moving.ndim, (
         f"Fixed and moving must have same dimensions: fixed.shape={fixed.shape}, "
         f"moving.shape={moving.shape}")
     term = _resolve_const_assumption(const_assumption)
-    if gnc_schedule is not None:
-        raise ValueError("flowreg_hip does not implement GNC schedules (gnc_schedule)")
-    if term != "gc":
-        raise ValueError(f"flowreg_hip implements the 'gc' data term only, got '{term}'")
+    sched = normalize_gnc_schedule(gnc_schedule)
+    steps = normalize_warping_steps(warping_steps)
     as_tensor = isinstance(fixed, torch.Tensor)
     if fixed.ndim == 2:
         m, n = fixed.shape
@@ -328,14 +364,26 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
         alpha = (alpha, alpha)
     alpha = tuple(float(a) for a in alpha)
     ad = tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist())
-    if level_solver_backend is not None:
-        from ._hooked import solve_with_level_solver
-        w = solve_with_level_solver(fixed, moving, alpha, update_lag, iterations, min_level,
-                                    levels, eta, a_smooth, np.array(ad), uv, weight,
-                                    level_solver_backend)
+    if sched is not None or level_solver_backend is not None:
+        from . import stage_loop
+        tensor_fn = _TENSOR_FUNCS[term]
+        solver_func = level_solver_backend if level_solver_backend is not None else level_solver
+        f = fixed.cpu().numpy() if as_tensor else np.asarray(fixed)
+        mv = moving.cpu().numpy() if isinstance(moving, torch.Tensor) else np.asarray(moving)
+        uvn = uv.cpu().numpy() if isinstance(uv, torch.Tensor) else uv
+        if sched is None:
+            w = stage_loop.solve_stage(f, mv, alpha, update_lag, iterations, min_level, levels, eta,
+                                       a_smooth, np.array(ad), uvn, weight, solver_func, tensor_fn)
+        else:
+            w = uvn
+            for beta in sched:
+                w = stage_loop.solve_stage_gnc(f, mv, alpha, update_lag, iterations, min_level,
+                                               levels, eta, a_smooth, np.array(ad), w, weight,
+                                               solver_func, tensor_fn, float(beta),
+                                               10 if steps is None else steps)
         return torch.from_numpy(w).to(_device()) if as_tensor else w
     solver = _cached_solver(m, n, C, alpha, int(update_lag), int(iterations), int(min_level),
-                            int(levels), float(eta), float(a_smooth), ad, 1)
+                            int(levels), float(eta), float(a_smooth), ad, 1, term)
     solver.set_reference(fixed, weight, uv)
     w = solver.run(moving, out_dtype=torch.float64)
     if as_tensor:
@@ -374,7 +422,10 @@ def imresize_fused_gauss_cubic(img, size, sigma_coeff=0.6, per_axis=False):
 
 
 def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
-    """core/warping.py:160-234 (cubic) on the GPU; numpy in, numpy out."""
+    """core/warping.py:160-234 (cubic) on the GPU; numpy in, numpy (float32)
+    out.  u, v are the displacements in level pixels (the reference's
+    u / h_col, v / h_row) of any dtype: the map float32(x + u) is formed in
+    float64, as numpy forms grid + u."""
     if interpolation_method.lower() != "cubic":
         if interpolation_method.lower() == "linear":
             raise ValueError("flowreg_hip: linear warp is not implemented")
@@ -384,13 +435,15 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
     f1 = _as_planes(f1_level)
     C, H, W = f2.shape
     dev = f2.device
-    uu = torch.as_tensor(np.asarray(u, np.float32) if not isinstance(u, torch.Tensor) else u,
-                         device=dev, dtype=torch.float32).contiguous()
-    vv = torch.as_tensor(np.asarray(v, np.float32) if not isinstance(v, torch.Tensor) else v,
-                         device=dev, dtype=torch.float32).contiguous()
+
+    def disp(a):
+        if isinstance(a, torch.Tensor):
+            return a.to(device=dev, dtype=torch.float64).contiguous()
+        return torch.as_tensor(np.asarray(a).astype(np.float64), device=dev).contiguous()
+    uu, vv = disp(u), disp(v)
     out = torch.empty_like(f2)
-    check(lib().frh_warp_cubic(ptr(f2), ptr(f1), 0, 1, C, H, W, ptr(uu), ptr(vv), 1.0, 1.0,
-                               ptr(out), stream_handle()), "frh_warp_cubic")
+    check(lib().frh_warp_cubic_f64(ptr(f2), ptr(f1), 0, 1, C, H, W, ptr(uu), ptr(vv), ptr(out),
+                                   stream_handle()), "frh_warp_cubic_f64")
     res = out.permute(1, 2, 0)
     res = res[:, :, 0] if squeeze else res
     if isinstance(f2_level, torch.Tensor):
@@ -398,25 +451,54 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
     return res.cpu().numpy()
 
 
+def _motion_tensor(kind, f1, f2, hy, hx, eps):
+    """One channel on the GPU.  A float32 input makes numpy evaluate that
+    image's derivatives in float32 (the reference's warped image below the
+    top level); float64 inputs must hold float32-representable values (as
+    every image on the reference's path does)."""
+    dev = _device()
+    flags = []
+    planes = []
+    for f in (f1, f2):
+        t = f if isinstance(f, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(f))
+        flags.append(1 if t.dtype == torch.float32 else 0)
+        t32 = t.to(device=dev, dtype=torch.float32).contiguous()
+        if t.dtype != torch.float32 and not torch.equal(t32.double(), t.to(device=dev, dtype=torch.float64)):
+            raise ValueError("flowreg_hip motion tensors take float32-valued images")
+        planes.append(t32)
+    h, w = planes[0].shape
+    J = [torch.empty((h + 2, w + 2), dtype=torch.float64, device=dev) for _ in range(6)]
+    check(lib().frh_motion_tensor(kind, ptr(planes[0]), ptr(planes[1]), h, w, float(hy), float(hx),
+                                  float(eps), flags[0], flags[1], *[ptr(j) for j in J],
+                                  stream_handle()), "frh_motion_tensor")
+    return tuple(j.cpu().numpy() for j in J)
+
+
 def get_motion_tensor_gc(f1, f2, hy, hx):
     """core/optical_flow.py:73-160 on the GPU (one channel); float64 outputs
     of shape (h+2, w+2)."""
-    dev = _device()
-    a = torch.as_tensor(np.asarray(f1, dtype=np.float32), device=dev).contiguous()
-    b = torch.as_tensor(np.asarray(f2, dtype=np.float32), device=dev).contiguous()
-    h, w = a.shape
-    J = [torch.empty((h + 2, w + 2), dtype=torch.float64, device=dev) for _ in range(6)]
-    check(lib().frh_motion_tensor_gc(ptr(a), ptr(b), h, w, float(hy), float(hx),
-                                     *[ptr(j) for j in J], stream_handle()),
-          "frh_motion_tensor_gc")
-    return tuple(j.cpu().numpy() for j in J)
+    return _motion_tensor(0, f1, f2, hy, hx, _CENSUS_EPS)
+
+
+def get_motion_tensor_gray(f1, f2, hy, hx):
+    """core/optical_flow.py:163-228 on the GPU (one channel)."""
+    return _motion_tensor(1, f1, f2, hy, hx, _CENSUS_EPS)
+
+
+def get_motion_tensor_cs(f1, f2, hy, hx, eps=None):
+    """core/optical_flow.py:231-345 on the GPU (one channel); eps defaults to
+    0.1 / 255."""
+    return _motion_tensor(2, f1, f2, hy, hx, _CENSUS_EPS if eps is None else float(eps))
+
+
+_TENSOR_FUNCS = {"gc": get_motion_tensor_gc, "gray": get_motion_tensor_gray,
+                 "cs": get_motion_tensor_cs}
 
 
 def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, update_lag,
                  verbose, a_data, a_smooth, hx, hy, gnc_beta=None):
-    """core/optical_flow.py:348-411 on the GPU: returns (du, dv) (m, n) float64."""
-    if gnc_beta is not None:
-        raise ValueError("flowreg_hip does not implement the GNC level solver (gnc_beta)")
+    """core/optical_flow.py:348-411 on the GPU (compute_flow, or
+    compute_flow_gnc with gnc_beta): returns (du, dv) (m, n) float64."""
     dev = _device()
 
     def d(a):
@@ -431,6 +513,7 @@ def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, 
     check(lib().frh_level_solver(*[ptr(j) for j in Js], ptr(W_), ptr(uu), ptr(vv), m, n, K,
                                  float(alpha[0]), float(alpha[1]), int(iterations),
                                  int(update_lag), ptr(ad), float(a_smooth), float(hx), float(hy),
+                                 -1.0 if gnc_beta is None else float(gnc_beta),
                                  ptr(du), ptr(dv), stream_handle()), "frh_level_solver")
     return du.cpu().numpy(), dv.cpu().numpy()
 

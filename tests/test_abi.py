@@ -35,7 +35,7 @@ def test_host_entry_points_without_gpu():
     import numpy as np
     from pyflowreg_amd import _lib
     L = _lib.lib()
-    assert L.frh_abi_version() == 1
+    assert L.frh_abi_version() == 2
     assert L.frh_warping_depth(0.8, 50, 512, 512) == 18
     g = np.array([1.0], np.float32)
     idx = np.zeros((5, 4), np.int32)
@@ -61,3 +61,16 @@ def test_unknown_constancy_assumption_is_value_error():
     import pyflowreg_amd as fr
     with pytest.raises(ValueError, match="Unknown constancy assumption"):
         fr.get_displacement(np.zeros((8, 8)), np.zeros((8, 8)), const_assumption="nope")
+
+
+def test_gnc_options_are_validated_before_gpu_work():
+    import numpy as np
+    import pyflowreg_amd as fr
+    f = np.zeros((8, 8))
+    for bad in ([0.5, 1.0], [0.0], [0.0, 0.7, 0.5, 1.0], [[0.0, 1.0]]):
+        with pytest.raises(ValueError):
+            fr.get_displacement(f, f, gnc_schedule=bad)
+    with pytest.raises(ValueError, match="warping_steps"):
+        fr.get_displacement(f, f, gnc_schedule=[0.0, 1.0], warping_steps=0)
+    np.testing.assert_array_equal(fr.normalize_gnc_schedule((0, 0.5, 1)), [0.0, 0.5, 1.0])
+    assert fr.normalize_gnc_schedule(None) is None and fr.normalize_warping_steps(None) is None

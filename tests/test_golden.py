@@ -126,18 +126,28 @@ def test_gpu_tensor(gpu):
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["level_solver_a", "level_solver_b"])
+@pytest.mark.parametrize("kind", ["gc", "gray", "cs"])
+def test_gpu_tensor_mixed(gpu, kind):
+    meta, z = load(f"tensor_{kind}_mixed")
+    fn = {"gc": gpu.get_motion_tensor_gc, "gray": gpu.get_motion_tensor_gray,
+          "cs": gpu.get_motion_tensor_cs}[kind]
+    np.testing.assert_array_equal(np.stack(fn(z["f1"], z["f2"], meta["hy"], meta["hx"])), z["J"])
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", LEVEL_CASES)
 def test_gpu_level_solver(gpu, name):
     m, z = load(name)
     K = z["J"].shape[-1]
     du, dv = gpu.level_solver(*z["J"], z["weight"], z["u"], z["v"], m["alpha"], m["iterations"],
-                              m["update_lag"], 0, np.full(K, m["a_data"]), m["a_smooth"], m["hx"], m["hy"])
+                              m["update_lag"], 0, np.full(K, m["a_data"]), m["a_smooth"], m["hx"], m["hy"],
+                              m.get("gnc_beta"))
     np.testing.assert_array_equal(du, z["du"])
     np.testing.assert_array_equal(dv, z["dv"])
 
 
 @pytest.mark.gpu
-@pytest.mark.parametrize("name", ["displacement_32x32x1", "displacement_40x48x2"])
+@pytest.mark.parametrize("name", DISPLACEMENT_CASES)
 def test_gpu_displacement(gpu, name):
     meta, z = load(name)
     from pyflowreg_amd.optical_flow import _gauss_taps
@@ -148,7 +158,7 @@ def test_gpu_displacement(gpu, name):
     mov = preprocess(z["moving_raw"], normalization_ref=fr).permute(0, 2, 3, 1)[0].cpu().numpy()
     np.testing.assert_array_equal(ref, z["fixed"])
     np.testing.assert_array_equal(mov, z["moving"])
-    kw = {}
+    kw = dict(meta["opts"])
     if meta["weight"] is not None:
         kw["weight"] = meta["weight"]
     if z["uv"].size:
