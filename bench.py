@@ -128,6 +128,54 @@ def copy_probe(mib):
                       "launches": 5}))
 
 
+def uneven(args):
+    """Independent pairs with sizes drawn from {256^2, 512^2, 796x512,
+    1024^2} and 1-2 channels, max displacement U[2, 8] px, Poisson peak
+    U[100, 1000]: grouped by size and solved in batched launches; one step =
+    all pairs.  Prints one JSON line (pairs/s)."""
+    import numpy as np
+    import torch
+    import pyflowreg_amd as fr
+    from pyflowreg_amd import synth
+    from pyflowreg_amd.preprocess import preprocess
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    dev = torch.device("cuda", 0)
+    pairs = synth.uneven_pairs(args.pairs, seed=3000, device=dev)
+    groups = {}
+    for f, m in pairs:
+        groups.setdefault(tuple(f.shape), []).append((f, m))
+    solvers = []
+    for (H, W, C), items in sorted(groups.items()):
+        fx = torch.stack([preprocess(f)[0] for f, _ in items])
+        mv = torch.stack([preprocess(m[None], normalization_ref=f)[0] for f, m in items])
+        s = fr.DisplacementSolver(H, W, C, max_batch=len(items))
+        s.set_reference(None, None)
+        solvers.append((s, fx.permute(0, 2, 3, 1).contiguous(), mv.permute(0, 2, 3, 1).contiguous()))
+
+    def step():
+        return fr.optical_flow.run_groups(solvers, n_streams=args.streams, out_dtype=torch.float32)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "pairs/s registered, uneven batch (256^2, 512^2, 796x512, 1024^2; 1-2 channels)",
+        "value": round(args.steps * len(pairs) / dt, 3), "unit": "pairs/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic pairs (blob scenes, smooth displacement max U[2,8] px, Poisson U[100,1000])",
+        "config": {"workload": f"batched uneven: {len(pairs)} independent pairs per step, "
+                               f"{len(groups)} size/channel groups on {args.streams} streams",
+                   "groups": {f"{H}x{W}x{C}": len(v) for (H, W, C), v in sorted(groups.items())}},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -141,9 +189,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--workload", choices=("uniform", "uneven"), default="uniform",
+                    help="uneven: 512 independent pairs of mixed sizes and channel counts")
+    ap.add_argument("--pairs", type=int, default=512)
+    ap.add_argument("--streams", type=int, default=4, help="streams for the uneven workload's groups")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)
+        return
+    if args.workload == "uneven":
+        uneven(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",

@@ -169,9 +169,10 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *desc);
 void frh_solver_destroy(frh_solver *s);
 
 /* Per-recording inputs shared by every frame of a batch
- * (core/optical_flow.py:519,531,565): fixed (C, H, W) float32 planes,
- * weight (C, H, W) float32 planes (float32-cast weight map), uv (2, H, W)
- * float32 initial flow or NULL (zero). */
+ * (core/optical_flow.py:519,531,565): fixed (C, H, W) float32 planes (or
+ * NULL when only frh_solver_run_pairs follows), weight (C, H, W) float32
+ * planes (float32-cast weight map), uv (2, H, W) float32 initial flow or
+ * NULL (zero). */
 int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *weight,
                              const float *uv, void *stream);
 
@@ -180,6 +181,13 @@ int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *wei
  * or float32 (out_f64 = 0), [..., 0] = u, [..., 1] = v. */
 int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int out_f64,
                    void *stream);
+
+/* get_displacement for T independent pairs of one size: fixed[t] against
+ * moving[t], both (T, C, H, W) float32 planes, sharing the weight and the
+ * initial flow of frh_solver_set_reference.  Each pair's result equals a
+ * separate get_displacement call (uneven batches group pairs by size). */
+int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving, int T,
+                         void *w_out, int out_f64, void *stream);
 
 /* Kernel-time accounting of the last frh_solver_run, measured with HIP
  * events on the run's stream: ms[k] is the summed duration and counts[k]

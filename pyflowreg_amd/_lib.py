@@ -71,6 +71,7 @@ SIGNATURES = {
     "frh_solver_destroy": (None, [c_void_p]),
     "frh_solver_set_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_solver_run": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "frh_solver_run_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "frh_solver_profile": (c_int, [c_void_p, c_int]),
     "frh_solver_timings": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
 }

@@ -361,6 +361,8 @@ struct frh_solver {
     int32_t *hoff = nullptr, *hix = nullptr;
     float *hwx = nullptr, *pyr = nullptr;
     double *wlow = nullptr, *wtmp = nullptr;  // min_level > 0: coarse flow, resize scratch
+    float *pyr1 = nullptr, *f1l = nullptr;    // per-frame references: pyramid rows, level images
+    bool have_fixed = false;
     size_t npl = 0;
     bool have_ref = false;
     bool profile = false;
@@ -385,6 +387,7 @@ static void free_solver(frh_solver *s)
     (void)hipFree(s->f1lev); (void)hipFree(s->uvtop); (void)hipFree(s->tmp); (void)hipFree(s->f2l); (void)hipFree(s->f2w);
     (void)hipFree(s->uvl); (void)hipFree(s->uvn); (void)hipFree(s->Wlev); (void)hipFree(s->work);
     (void)hipFree(s->hoff); (void)hipFree(s->hix); (void)hipFree(s->hwx); (void)hipFree(s->pyr);
+    (void)hipFree(s->pyr1); (void)hipFree(s->f1l);
     (void)hipFree(s->wlow); (void)hipFree(s->wtmp);
     for (auto e : s->pool) (void)hipEventDestroy(e);
     delete s;
@@ -422,6 +425,11 @@ struct Scope {
         }
     }
 };
+
+// The level loop for T moving frames against the shared reference levels
+// (fixed == nullptr) or against per-frame references fixed (T, C, H, W).
+static int solver_run(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
+                      int out_f64, hipStream_t st);
 
 extern "C" {
 
@@ -528,15 +536,17 @@ void frh_solver_destroy(frh_solver *s) { free_solver(s); }
 int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *weight, const float *uv,
                              void *stream)
 {
-    FRH_REQ(s && fixed && weight, "frh_solver_set_reference: null pointer");
+    FRH_REQ(s && weight, "frh_solver_set_reference: null pointer");
     hipStream_t st = as_stream(stream);
     int C = s->C;
-    FRH_HIP(launch_pyr_h(fixed, C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
-            "reference pyramid");
-    for (int i = 0; i < s->nl; ++i) {
-        LevelDev &L = s->lv[i];
-        FRH_HIP(launch_pyr_v(s->pyr, C, s->H, s->Wsum, s->xoff[i], s->f1lev + L.f1_off, L.h, L.w,
-                             L.img_iy, L.img_wy, L.P_img, st), "reference pyramid");
+    if (fixed) {
+        FRH_HIP(launch_pyr_h(fixed, C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
+                "reference pyramid");
+        for (int i = 0; i < s->nl; ++i) {
+            LevelDev &L = s->lv[i];
+            FRH_HIP(launch_pyr_v(s->pyr, C, s->H, s->Wsum, s->xoff[i], s->f1lev + L.f1_off, L.h, L.w,
+                                 L.img_iy, L.img_wy, L.P_img, st), "reference pyramid");
+        }
     }
     FRH_HIP(launch_pyr_h(weight, C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
             "weight pyramid");
@@ -555,15 +565,38 @@ int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *wei
         FRH_HIP(hipMemsetAsync(s->uvtop, 0, sizeof(float) * 2 * T0.h * T0.w, st), "initial flow clear");
     }
     s->have_ref = true;
+    s->have_fixed = fixed != nullptr;
     return FRH_OK;
 }
 
 int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int out_f64, void *stream)
 {
     FRH_REQ(s && moving && w_out, "frh_solver_run: null pointer");
-    FRH_REQ(s->have_ref, "frh_solver_run: frh_solver_set_reference has not been called");
+    FRH_REQ(s->have_ref && s->have_fixed, "frh_solver_run: frh_solver_set_reference has not set a reference");
     FRH_REQ(T >= 1 && T <= s->max_batch, "frh_solver_run: T=%d outside [1, %d]", T, s->max_batch);
-    hipStream_t st = as_stream(stream);
+    return solver_run(s, nullptr, moving, T, w_out, out_f64, as_stream(stream));
+}
+
+int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
+                         int out_f64, void *stream)
+{
+    FRH_REQ(s && fixed && moving && w_out, "frh_solver_run_pairs: null pointer");
+    FRH_REQ(s->have_ref, "frh_solver_run_pairs: frh_solver_set_reference has not been called");
+    FRH_REQ(T >= 1 && T <= s->max_batch, "frh_solver_run_pairs: T=%d outside [1, %d]", T, s->max_batch);
+    if (!s->pyr1) {
+        size_t T_ = (size_t)s->max_batch, C = s->C;
+        int rc = dev_alloc(&s->pyr1, T_ * C * (size_t)s->H * s->Wsum, "reference pyramid rows");
+        if (!rc) rc = dev_alloc(&s->f1l, T_ * C * (size_t)s->lv.back().h * s->lv.back().w, "reference levels");
+        if (rc) return rc;
+    }
+    return solver_run(s, fixed, moving, T, w_out, out_f64, as_stream(stream));
+}
+
+}  // extern "C"
+
+static int solver_run(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
+                      int out_f64, hipStream_t st)
+{
     const int C = s->C;
     s->marks.clear();
     s->pool_used = 0;
@@ -576,6 +609,9 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
         Scope sc(s, TM_RESIZE, st);
         FRH_HIP(launch_pyr_h(moving, T * C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
                 "moving pyramid rows");
+        if (fixed)
+            FRH_HIP(launch_pyr_h(fixed, T * C, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr1, st),
+                    "reference pyramid rows");
     }
     for (int li = 0; li < s->nl; ++li) {
         LevelDev &L = s->lv[li];
@@ -588,7 +624,13 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
             Scope sc(s, TM_RESIZE, st);
             FRH_HIP(launch_pyr_v(s->pyr, T * C, s->H, s->Wsum, s->xoff[li], s->f2l, h, w, L.img_iy,
                                  L.img_wy, L.P_img, st), "level images");
+            if (fixed)
+                FRH_HIP(launch_pyr_v(s->pyr1, T * C, s->H, s->Wsum, s->xoff[li], s->f1l, h, w, L.img_iy,
+                                     L.img_wy, L.P_img, st), "reference level images");
         }
+        // reference level images: shared, or one per frame (T, C, h, w)
+        const float *f1 = fixed ? s->f1l : s->f1lev + L.f1_off;
+        const size_t f1_fs = fixed ? (size_t)C * hw : 0;
         const float *f2src = s->f2l;
         if (li == 0) {
             FRH_HIP(launch_broadcast_f32(s->uvtop, 2 * hw, s->uvl, 2 * hw, T, st), "initial flow");
@@ -601,7 +643,7 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
             }
             {
                 Scope sc(s, TM_WARP, st);
-                FRH_HIP(launch_warp(s->f2l, s->f1lev + L.f1_off, 0, T, C, h, w, s->uvl, s->uvl + hw, 2 * hw,
+                FRH_HIP(launch_warp(s->f2l, f1, fixed ? 1 : 0, T, C, h, w, s->uvl, s->uvl + hw, 2 * hw,
                                     L.h_col, L.h_row, s->f2w, st), "warp");
             }
             f2src = s->f2w;
@@ -610,9 +652,9 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
             Scope sc(s, TM_TENSOR, st);
             // the top level's "warped" image is the resized level itself
             // (float64 data in the reference); below it the warp's float32
-            FRH_HIP(launch_tensor(s->const_assumption, li == 0 ? 0 : 1, s->f1lev + L.f1_off, f2src, T, C,
-                                  h, w, L.h_row, L.h_col, kCensusEps, s->uvl, s->uvl + hw, 2 * hw, L.axw,
-                                  L.ayw, J, cuv, fs, sk, st),
+            FRH_HIP(launch_tensor(s->const_assumption, li == 0 ? 0 : 1, f1, f1_fs, f2src, T, C, h, w,
+                                  L.h_row, L.h_col, kCensusEps, s->uvl, s->uvl + hw, 2 * hw, L.axw, L.ayw,
+                                  J, cuv, fs, sk, st),
                     "motion tensor");
         }
         // The separate assemble kernel is faster than forming coefficients
@@ -678,6 +720,8 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
     }
     return FRH_OK;
 }
+
+extern "C" {
 
 int frh_solver_profile(frh_solver *s, int enable)
 {

@@ -48,10 +48,10 @@ hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T
 hipError_t launch_warp_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
                            int W, const double *u, const double *v, size_t uv_frame_stride,
                            float *out, hipStream_t st);
-hipError_t launch_tensor(int kind, int f2_f32, const float *f1, const float *f2, int T, int C, int h,
-                         int w, double hy, double hx, double eps, const float *u, const float *v,
-                         size_t uv_frame_stride, double ax, double ay, double *J, double *cuv,
-                         size_t frame_stride, const Skew &s, hipStream_t st);
+hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_stride, const float *f2,
+                         int T, int C, int h, int w, double hy, double hx, double eps, const float *u,
+                         const float *v, size_t uv_frame_stride, double ax, double ay, double *J,
+                         double *cuv, size_t frame_stride, const Skew &s, hipStream_t st);
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
                            const double *duv, double *coef, size_t frame_stride, int T, int C,
                            const Skew &s, double ax, double ay, const double *a_data_minus1,

@@ -575,7 +575,7 @@ constexpr int kTensorRows = 16;
 template <int KIND, bool P2>
 __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
-    size_t plane, double hy, double hx, double eps, const float *__restrict__ u,
+    size_t plane, double hy, double hx, double eps, size_t f1_fstride, const float *__restrict__ u,
     const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
     double *__restrict__ cuv, size_t fstride, int T)
 {
@@ -589,8 +589,8 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     size_t hw = (size_t)h * w, q = (size_t)(r + c) * hs + r;
     double *Jt = J + (size_t)t * fstride;
     for (int ch = 0; ch < C; ++ch) {
-        TensorOut o = tensor_cell<KIND, false, P2>(f1 + ch * hw, f2 + ((size_t)t * C + ch) * hw, h, w,
-                                                  r, c, hy, hx, eps);
+        TensorOut o = tensor_cell<KIND, false, P2>(f1 + (size_t)t * f1_fstride + ch * hw,
+                                                  f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx, eps);
         Jt[(size_t)(0 * C + ch) * plane + q] = o.J11;
         Jt[(size_t)(1 * C + ch) * plane + q] = o.J22;
         Jt[(size_t)(2 * C + ch) * plane + q] = o.J33;
@@ -603,17 +603,17 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     ct[plane + q] = smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay);
 }
 
-hipError_t launch_tensor(int kind, int f2_f32, const float *f1, const float *f2, int T, int C, int h,
-                         int w, double hy, double hx, double eps, const float *u, const float *v,
-                         size_t uv_frame_stride, double ax, double ay, double *J, double *cuv,
-                         size_t frame_stride, const Skew &s, hipStream_t st)
+hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_stride, const float *f2,
+                         int T, int C, int h, int w, double hy, double hx, double eps, const float *u,
+                         const float *v, size_t uv_frame_stride, double ax, double ay, double *J,
+                         double *cuv, size_t frame_stride, const Skew &s, hipStream_t st)
 {
     const size_t nb = (size_t)((w + 63) / 64) * ((h + kTensorRows - 1) / kTensorRows) * T;
     if (nb > 0x7fffffffu) return hipErrorInvalidValue;
 #define FRH_TEN(K, P)                                                                             \
     hipLaunchKernelGGL((k_tensor<K, P>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, C, \
-                       h, w, s.hs, s.plane, hy, hx, eps, u, v, uv_frame_stride, ax, ay, J, cuv,     \
-                       frame_stride, T)
+                       h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
+                       ay, J, cuv, frame_stride, T)
     switch (kind * 2 + (f2_f32 ? 1 : 0)) {
     case 0: FRH_TEN(kTensorGC, false); break;
     case 1: FRH_TEN(kTensorGC, true); break;

@@ -278,17 +278,20 @@ class DisplacementSolver:
             pass
 
     def set_reference(self, fixed, weight=None, uv=None):
-        """fixed: (H, W[, C]); weight as get_displacement accepts; uv (H, W, 2)."""
-        f = _as_planes(fixed)
-        if f.shape != (self.C, self.H, self.W):
-            raise ValueError(f"fixed has shape {tuple(f.shape)}, expected C,H,W="
-                             f"{(self.C, self.H, self.W)}")
+        """fixed: (H, W[, C]), or None before run_pairs only; weight as
+        get_displacement accepts; uv (H, W, 2)."""
+        f = None
+        if fixed is not None:
+            f = _as_planes(fixed)
+            if f.shape != (self.C, self.H, self.W):
+                raise ValueError(f"fixed has shape {tuple(f.shape)}, expected C,H,W="
+                                 f"{(self.C, self.H, self.W)}")
         wmap = expand_weight(weight, self.H, self.W, self.C)
         w = _as_planes(wmap.astype(np.float32))
         u = None
         if uv is not None:
             u = uv if isinstance(uv, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(uv))
-            u = u.to(device=f.device, dtype=torch.float32).permute(2, 0, 1).contiguous()
+            u = u.to(device=w.device, dtype=torch.float32).permute(2, 0, 1).contiguous()
         self._ref = (f, w, u)
         check(lib().frh_solver_set_reference(self._h, ptr(f), ptr(w), ptr(u), stream_handle()),
               "frh_solver_set_reference")
@@ -317,6 +320,32 @@ class DisplacementSolver:
                   "frh_solver_run")
         out = out.to(out_dtype)
         return out[0] if single else out
+
+    def run_pairs(self, fixed, moving, out_dtype=torch.float64, profile=False) -> torch.Tensor:
+        """Independent pairs: fixed, moving (T, H, W, C) -> flow (T, H, W, 2),
+        each pair solved as its own get_displacement call (shared weight and
+        initial flow from set_reference)."""
+        def planes_of(x):
+            t = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(x))
+            if t.ndim == 3:
+                t = t[..., None]
+            p = _as_planes(t)
+            if p.shape[1:] != (self.C, self.H, self.W):
+                raise ValueError(f"frames have shape {tuple(p.shape[1:])}, expected {(self.C, self.H, self.W)}")
+            return p
+        fp, mp = planes_of(fixed), planes_of(moving)
+        if fp.shape[0] != mp.shape[0]:
+            raise ValueError("fixed and moving hold different numbers of frames")
+        T = fp.shape[0]
+        f64 = out_dtype == torch.float64 or self.plan.min_level > 0
+        out = torch.empty((T, self.H, self.W, 2), dtype=torch.float64 if f64 else out_dtype,
+                          device=mp.device)
+        check(lib().frh_solver_profile(self._h, 1 if profile else 0), "frh_solver_profile")
+        for s in range(0, T, self.max_batch):
+            e = min(T, s + self.max_batch)
+            check(lib().frh_solver_run_pairs(self._h, ptr(fp[s:e]), ptr(mp[s:e]), e - s, ptr(out[s:e]),
+                                             1 if f64 else 0, stream_handle()), "frh_solver_run_pairs")
+        return out.to(out_dtype)
 
     def timings(self):
         """(ms, counts) per category of the last profiled run."""
@@ -389,6 +418,71 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
     if as_tensor:
         return w
     return w.cpu().numpy()
+
+
+def get_displacements(fixed_seq, moving_seq, alpha=(2, 2), update_lag=5, iterations=50,
+                      min_level=0, levels=50, eta=0.8, a_smooth=1.0, a_data=0.45,
+                      const_assumption="gc", weight=None, max_batch=256):
+    """get_displacement for many independent (fixed, moving) pairs of uneven
+    sizes and channel counts: pairs are grouped by (H, W, C) and each group is
+    solved in batched launches (DisplacementSolver.run_pairs).  Returns a list
+    of (H, W, 2) float64 numpy flows in input order; each equals the
+    corresponding get_displacement(fixed, moving, ...) call."""
+    term = _resolve_const_assumption(const_assumption)
+    if np.isscalar(alpha):
+        alpha = (alpha, alpha)
+    alpha = tuple(float(a) for a in alpha)
+    groups = {}
+    pairs = list(zip(fixed_seq, moving_seq))
+    for k, (f, m) in enumerate(pairs):
+        f = np.asarray(f)
+        key = f.shape if f.ndim == 3 else f.shape + (1,)
+        groups.setdefault(key, []).append(k)
+    out = [None] * len(pairs)
+    jobs = []
+    try:
+        for (H, W, C), idx in groups.items():
+            ad = tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist())
+            solver = DisplacementSolver(H, W, C, alpha=alpha, update_lag=update_lag,
+                                        iterations=iterations, min_level=min_level, levels=levels,
+                                        eta=eta, a_smooth=a_smooth, a_data=np.array(ad),
+                                        max_batch=min(max_batch, len(idx)), const_assumption=term)
+            solver.set_reference(None, weight)
+            fx = np.stack([np.asarray(pairs[k][0], np.float32).reshape(H, W, C) for k in idx])
+            mv = np.stack([np.asarray(pairs[k][1], np.float32).reshape(H, W, C) for k in idx])
+            jobs.append((solver, fx, mv, idx))
+        results = run_groups([(s, fx, mv) for s, fx, mv, _ in jobs])
+        for (_, _, _, idx), w in zip(jobs, results):
+            w = w.cpu().numpy()
+            for j, k in enumerate(idx):
+                out[k] = w[j]
+    finally:
+        for s, *_ in jobs:
+            s.close()
+    return out
+
+
+def run_groups(groups, n_streams=4, out_dtype=torch.float64):
+    """Runs DisplacementSolver.run_pairs for several (solver, fixed, moving)
+    groups on up to n_streams streams (default priority), so that the
+    latency-bound sweeps of small groups share the device; returns the flows
+    in group order after the caller's stream has waited for all of them."""
+    dev = _device()
+    cur = torch.cuda.current_stream(dev)
+    streams = [torch.cuda.Stream(dev) for _ in range(max(1, min(n_streams, len(groups))))]
+    outs = []
+    # largest groups first, dealt round-robin
+    order = sorted(range(len(groups)), key=lambda g: -groups[g][0].H * groups[g][0].W * groups[g][0].C)
+    res = [None] * len(groups)
+    for j, g in enumerate(order):
+        s, fx, mv = groups[g]
+        st = streams[j % len(streams)]
+        st.wait_stream(cur)
+        with torch.cuda.stream(st):
+            res[g] = s.run_pairs(fx, mv, out_dtype=out_dtype)
+    for st in streams:
+        cur.wait_stream(st)
+    return res
 
 
 # ---------------------------------------------------------------- stages

@@ -105,3 +105,21 @@ def test_hooked_solver_with_census(gpu, oracle):
     a = gpu.get_displacement(ref, mov, const_assumption="cs", level_solver_backend=oracle.level_solver)
     b = oracle.get_displacement(ref, mov, const_assumption="cs")
     np.testing.assert_array_equal(a, b)
+
+
+def test_uneven_pairs_bit_exact(gpu, oracle):
+    """Independent pairs of uneven sizes and channel counts (the uneven-batch
+    workload at small sizes): grouped, batched solves equal per-pair oracle
+    runs; each pair has its own reference."""
+    rng = np.random.default_rng(17)
+    sizes = [(32, 32, 1), (40, 48, 2), (32, 32, 1), (56, 40, 1), (40, 48, 2), (32, 32, 2)]
+    fixed, moving = [], []
+    for k, (H, W, C) in enumerate(sizes):
+        f, m = _pair(H, W, C, 100 + k)
+        f = f[..., :C]
+        m = m[..., :C]
+        fixed.append(oracle.preprocess(f.astype(np.float64)).astype(np.float32))
+        moving.append(oracle.preprocess(m, f.astype(np.float64)).astype(np.float32))
+    ws = gpu.get_displacements(fixed, moving)
+    for k in range(len(sizes)):
+        np.testing.assert_array_equal(ws[k], oracle.get_displacement(fixed[k], moving[k]))
