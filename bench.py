@@ -176,6 +176,69 @@ def uneven(args):
     }))
 
 
+def grad(args):
+    """Differentiable registration: 32 moving frames (512x512, 2 channels)
+    against one fixed frame, full pyramid, continuous warp; one step =
+    forward (get_displacement + masked SSE of the registered frames) and
+    backward to alpha (2), a_data (2) and the moving frames.  Prints one
+    JSON line (frames/s through forward + backward)."""
+    import torch
+    from pyflowreg_amd import autograd as AG
+    from pyflowreg_amd import synth
+    from pyflowreg_amd.preprocess import preprocess
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    dev = torch.device("cuda", 0)
+    H = W = args.size
+    C = args.channels
+    T = args.grad_frames
+    frames, _ = synth.video(T + 1, H, W, n_channels=C, seed=1000, device=dev)
+    raw_ref = frames[0].double()
+    fixed = preprocess(raw_ref)[0].permute(1, 2, 0).double().contiguous()
+    moving = preprocess(frames[1:], normalization_ref=raw_ref).permute(0, 2, 3, 1).double().contiguous()
+    mask = torch.zeros((H, W), dtype=torch.float64, device=dev)
+    mask[8:-8, 8:-8] = 1.0
+
+    def step():
+        alpha = torch.tensor([2.0, 2.0], dtype=torch.float64, device=dev, requires_grad=True)
+        a_data = torch.tensor([0.45] * C, dtype=torch.float64, device=dev, requires_grad=True)
+        mv = moving.clone().requires_grad_(True)
+        flow = AG.get_displacement(fixed, mv, alpha=alpha, a_data=a_data)
+        loss = AG.registration_loss(fixed, mv, flow, mask)
+        loss.backward()
+        return loss.detach(), alpha.grad, a_data.grad, mv.grad
+
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        loss, g_alpha, g_a, g_m = step()
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    with torch.no_grad():
+        AG.get_displacement(fixed, moving, alpha=torch.tensor([2.0, 2.0], dtype=torch.float64, device=dev))
+    torch.cuda.synchronize()
+    fwd = time.perf_counter() - t1
+    print(json.dumps({
+        "metric": "frames/s through differentiable registration (forward + backward), 512x512x2ch",
+        "value": round(args.steps * T / dt, 3), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (blob scenes, smooth random displacement + drift, Poisson noise)",
+        "config": {"workload": f"differentiable: {T} frames {H}x{W}x{C} against one fixed frame, full "
+                               "pyramid, continuous warp, alpha=2, 50 iterations, update_lag 5, a_data 0.45; "
+                               "loss = masked SSE of the registered frames; gradients to alpha, a_data, "
+                               "moving frames", "frames_per_step": T},
+        "forward_only_ms": round(fwd * 1e3, 3),
+        "loss": float(loss), "grad_alpha": [float(x) for x in g_alpha],
+        "grad_a_data": [float(x) for x in g_a], "grad_moving_rms": float(g_m.pow(2).mean().sqrt()),
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -189,8 +252,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--workload", choices=("uniform", "uneven"), default="uniform",
-                    help="uneven: 512 independent pairs of mixed sizes and channel counts")
+    ap.add_argument("--workload", choices=("uniform", "uneven", "grad"), default="uniform",
+                    help="uneven: 512 independent pairs of mixed sizes and channel counts; "
+                         "grad: differentiable registration (forward + backward)")
+    ap.add_argument("--grad-frames", type=int, default=32, help="moving frames for the grad workload")
     ap.add_argument("--pairs", type=int, default=512)
     ap.add_argument("--streams", type=int, default=4, help="streams for the uneven workload's groups")
     args = ap.parse_args()
@@ -199,6 +264,9 @@ def main():
         return
     if args.workload == "uneven":
         uneven(args)
+        return
+    if args.workload == "grad":
+        grad(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",

@@ -73,6 +73,17 @@ SIGNATURES = {
     "frh_solver_run": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "frh_solver_run_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "frh_solver_profile": (c_int, [c_void_p, c_int]),
+    "frh_grad_tensor": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
+                                c_void_p, c_void_p, c_void_p, c_void_p]),
+    "frh_grad_coef": (c_int, [c_void_p] * 4 + [c_int] * 4 + [c_double, c_double, c_void_p, c_int]
+                      + [c_void_p] * 7),
+    "frh_grad_sweep": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
+                               c_void_p]),
+    "frh_grad_sweep_adjoint": (c_int, [c_void_p] * 8 + [c_int, c_int, c_int, c_double, c_double,
+                                                         c_void_p]),
+    "frh_grad_median": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    "frh_grad_warp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,
+                              c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_solver_timings": (c_int, [c_void_p, c_void_p, c_void_p, c_int]),
 }
 

@@ -11,7 +11,7 @@ import subprocess
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
 LIB = os.path.join(HERE, "libflowreg_hip.so")
-SOURCES = ["csrc/frh_kernels.hip", "csrc/frh_api.hip"]
+SOURCES = ["csrc/frh_kernels.hip", "csrc/frh_api.hip", "csrc/frh_grad.hip"]
 
 # -ffp-contract=off keeps every multiply and add separately rounded so the
 # kernels match the float64 CPU oracle bit for bit; packed FP32 is disabled.

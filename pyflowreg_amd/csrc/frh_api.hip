@@ -122,6 +122,16 @@ int upload(T **dev, const T *host, size_t n, const char *what)
 
 }  // namespace
 
+// error reporting shared with the other translation units of the library
+int frh::frh_report_error(int code, const char *fmt, ...)
+{
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(g_err, sizeof(g_err), fmt, ap);
+    va_end(ap);
+    return code;
+}
+
 extern "C" {
 
 const char *frh_last_error(void) { return g_err; }

@@ -30,6 +30,9 @@ inline Skew make_skew(int h, int w)
 
 __host__ __device__ inline size_t sk(int hs, int r, int c) { return (size_t)(r + c) * hs + r; }
 
+// Stores a formatted message for frh_last_error() and returns code.
+int frh_report_error(int code, const char *fmt, ...);
+
 constexpr int kMaxSweepsPerPass = 8;
 constexpr int kCoefPlanes = 5;  // Du, Dv, A12, Bu, Bv
 

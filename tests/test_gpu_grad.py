@@ -1,0 +1,409 @@
+"""Differentiable registration on the GPU (pyflowreg_amd.autograd): each HIP
+forward/backward pair against the pure-torch autograd restatement
+(tests/grad_reference.py), the single-level forward against the oracle, and
+the whole-pyramid gradient against central FP64 finite differences.
+
+Tolerances: stage forwards agree with torch to 1e-11 relative (the same
+float64 formulas in another evaluation order); gradients of one level solve
+to 1e-8 and of a whole pyramid to 1e-6 relative (rounding through 50 sweeps
+per level and sums in another order); finite differences (h = 1e-4) to 1e-3
+relative, the stated target for the adjoint gradient.
+"""
+
+import numpy as np
+import pytest
+import torch
+
+import grad_reference as R
+
+pytestmark = pytest.mark.gpu
+
+F64 = torch.float64
+
+
+def _rel(a, b):
+    a = a.detach().double().cpu()
+    b = b.detach().double().cpu()
+    return float((a - b).abs().max() / b.abs().max().clamp_min(1e-300))
+
+
+def _frames(T, H, W, C, seed):
+    """Preprocessed synthetic frames, float32-valued float64: fixed (H, W, C)
+    and moving (T, H, W, C) numpy arrays."""
+    from oracle import oracle as O
+    from pyflowreg_amd import synth
+    vid, _ = synth.video(T + 1, H, W, C, seed=seed, max_disp=2.0)
+    raw = vid.double().numpy()
+    fixed = O.preprocess(raw[0]).astype(np.float32).astype(np.float64)
+    moving = O.preprocess(raw[1:], raw[0]).astype(np.float32).astype(np.float64)
+    return fixed, moving
+
+
+def _planes(fixed, moving):
+    f = torch.from_numpy(fixed).permute(2, 0, 1).contiguous()
+    m = torch.from_numpy(moving).permute(0, 3, 1, 2).contiguous()
+    return f, m
+
+
+def _cuda(*ts):
+    return [t.to("cuda").contiguous() for t in ts]
+
+
+# ------------------------------------------------------------------ stages
+def test_motion_tensor_forward_and_backward(gpu):
+    from pyflowreg_amd import _lib
+    fixed, moving = _frames(2, 20, 26, 2, seed=21)
+    f, m = _planes(fixed, moving)
+    T, C, h, w = m.shape
+    hy, hx = 1.25, 1.1
+    mt = m.clone().requires_grad_(True)
+    Jr = R.motion_tensor_gc(f, mt, hy, hx)
+    gJ = torch.randn(Jr.shape, dtype=F64, generator=torch.Generator().manual_seed(3))
+    (Jr * gJ).sum().backward()
+    fd, md, gJd = _cuda(f, m, gJ)
+    J = torch.empty((T, 6, C, h, w), dtype=F64, device="cuda")
+    g_m = torch.zeros_like(md)
+    L = _lib.lib()
+    _lib.check(L.frh_grad_tensor(fd.data_ptr(), md.data_ptr(), T, C, h, w, hy, hx, J.data_ptr(),
+                                 gJd.data_ptr(), g_m.data_ptr(), _lib.stream_handle()), "tensor")
+    torch.cuda.synchronize()
+    assert _rel(J, Jr) < 1e-12
+    assert _rel(g_m, mt.grad) < 1e-10
+
+
+@pytest.mark.parametrize("psi_one", [False, True])
+def test_coefficients_forward_and_backward(gpu, psi_one):
+    from pyflowreg_amd import _lib
+    g = torch.Generator().manual_seed(5)
+    fixed, moving = _frames(2, 18, 23, 2, seed=22)
+    f, m = _planes(fixed, moving)
+    T, C, h, w = m.shape
+    J = R.motion_tensor_gc(f, m, 1.0, 1.0).detach()
+    W = torch.rand((C, h, w), dtype=F64, generator=g) * 0.5 + 0.25
+    x = (torch.randn((T, 2, h, w), dtype=F64, generator=g) * 0.3).requires_grad_(True)
+    u0 = (torch.randn((T, 2, h, w), dtype=F64, generator=g) * 0.5).requires_grad_(True)
+    Jt = J.clone().requires_grad_(True)
+    ax = torch.tensor(1.7, dtype=F64, requires_grad=True)
+    ay = torch.tensor(2.3, dtype=F64, requires_grad=True)
+    a = torch.tensor([0.45, 0.6], dtype=F64, requires_grad=True)
+    K = R.coefficients(x, u0, Jt, W, ax, ay, a, psi_one)
+    gK = torch.randn(K.shape, dtype=F64, generator=g)
+    (K * gK).sum().backward()
+
+    xd, u0d, Jd, Wd, ad, gKd = _cuda(x.detach(), u0.detach(), J, W, a.detach(), gK)
+    coef = torch.empty((T, 6, h, w), dtype=F64, device="cuda")
+    g_x = torch.zeros_like(xd)
+    g_u0 = torch.zeros_like(u0d)
+    g_J = torch.zeros_like(Jd)
+    g_par = torch.zeros((T, 2 + C, h, w), dtype=F64, device="cuda")
+    L = _lib.lib()
+    _lib.check(L.frh_grad_coef(xd.data_ptr(), u0d.data_ptr(), Jd.data_ptr(), Wd.data_ptr(), T, C, h, w,
+                               1.7, 2.3, ad.data_ptr(), int(psi_one), coef.data_ptr(), gKd.data_ptr(),
+                               g_x.data_ptr(), g_u0.data_ptr(), g_J.data_ptr(), g_par.data_ptr(),
+                               _lib.stream_handle()), "coef")
+    torch.cuda.synchronize()
+    assert _rel(coef, K) < 1e-12
+    if psi_one:
+        assert float(g_x.abs().max()) == 0.0
+    else:
+        assert _rel(g_x, x.grad) < 1e-10
+    assert _rel(g_u0, u0.grad) < 1e-12
+    assert _rel(g_J, Jt.grad) < 1e-10
+    assert abs(float(g_par[:, 0].sum()) - float(ax.grad)) <= 1e-10 * abs(float(ax.grad))
+    assert abs(float(g_par[:, 1].sum()) - float(ay.grad)) <= 1e-10 * abs(float(ay.grad))
+    ga = g_par[:, 2:].sum(dim=(0, 2, 3)).cpu()
+    if psi_one:
+        assert float(ga.abs().max()) == 0.0
+    else:
+        assert _rel(ga, a.grad) < 1e-10
+
+
+def test_sweep_forward_and_adjoint(gpu):
+    from pyflowreg_amd import _lib
+    g = torch.Generator().manual_seed(7)
+    fixed, moving = _frames(2, 17, 21, 1, seed=23)
+    f, m = _planes(fixed, moving)
+    T, C, h, w = m.shape
+    J = R.motion_tensor_gc(f, m, 1.0, 1.0)
+    W = torch.full((C, h, w), 1.0, dtype=F64)
+    x0 = torch.randn((T, 2, h, w), dtype=F64, generator=g) * 0.3
+    u0 = torch.randn((T, 2, h, w), dtype=F64, generator=g) * 0.3
+    coef = R.coefficients(x0, u0, J, W, 2.0, 1.5, torch.tensor([0.45], dtype=F64), False).detach()
+    coef_t = coef.clone().requires_grad_(True)
+    x = x0.clone().requires_grad_(True)
+    ax = torch.tensor(2.0, dtype=F64, requires_grad=True)
+    ay = torch.tensor(1.5, dtype=F64, requires_grad=True)
+    y = R.sor_sweep(coef_t, x, ax, ay)
+    lam = torch.randn(y.shape, dtype=F64, generator=g)
+    (y * lam).sum().backward()
+
+    cd, xd, lamd = _cuda(coef, x0, lam)
+    out = torch.empty_like(xd)
+    lam_in = torch.empty_like(xd)
+    g_coef = torch.zeros_like(cd)
+    g_axy = torch.zeros((T, 2, h, w), dtype=F64, device="cuda")
+    scratch = torch.empty((4, T, h, w), dtype=F64, device="cuda")
+    L = _lib.lib()
+    st = _lib.stream_handle()
+    _lib.check(L.frh_grad_sweep(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), T, h, w, 2.0, 1.5, st), "sweep")
+    _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
+                                        lam_in.data_ptr(), g_coef.data_ptr(), g_axy.data_ptr(),
+                                        scratch.data_ptr(), T, h, w, 2.0, 1.5, st), "adjoint")
+    torch.cuda.synchronize()
+    assert _rel(out, y) < 1e-12
+    assert _rel(lam_in, x.grad) < 1e-11
+    assert _rel(g_coef, coef_t.grad) < 1e-11
+    assert abs(float(g_axy[:, 0].sum()) - float(ax.grad)) <= 1e-10 * abs(float(ax.grad))
+    assert abs(float(g_axy[:, 1].sum()) - float(ay.grad)) <= 1e-10 * abs(float(ay.grad))
+
+
+def test_sweep_one_row_and_one_column(gpu):
+    """Degenerate levels: a single row or column (every cell on the level
+    border), against the torch sweep."""
+    from pyflowreg_amd import _lib
+    g = torch.Generator().manual_seed(8)
+    L = _lib.lib()
+    for h, w in ((1, 9), (9, 1), (1, 1)):
+        coef = torch.rand((2, 6, h, w), dtype=F64, generator=g) * 0.5
+        x = torch.randn((2, 2, h, w), dtype=F64, generator=g)
+        y = R.sor_sweep(coef, x, 1.3, 0.7)
+        cd, xd = _cuda(coef, x)
+        out = torch.empty_like(xd)
+        _lib.check(L.frh_grad_sweep(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), 2, h, w, 1.3, 0.7,
+                                    _lib.stream_handle()), "sweep")
+        torch.cuda.synchronize()
+        assert _rel(out, y) < 1e-13, (h, w)
+
+
+def test_median_forward_and_backward(gpu):
+    from pyflowreg_amd import _lib
+    g = torch.Generator().manual_seed(9)
+    x = torch.randn((3, 11, 14), dtype=F64, generator=g).requires_grad_(True)
+    y = R.median5(x)
+    gy = torch.randn(y.shape, dtype=F64, generator=g)
+    (y * gy).sum().backward()
+    xd, gyd = _cuda(x.detach(), gy)
+    out = torch.empty_like(xd)
+    gx = torch.zeros_like(xd)
+    L = _lib.lib()
+    _lib.check(L.frh_grad_median(xd.data_ptr(), 3, 11, 14, out.data_ptr(), gyd.data_ptr(), gx.data_ptr(),
+                                 _lib.stream_handle()), "median")
+    torch.cuda.synchronize()
+    assert torch.equal(out.cpu(), y.detach())
+    assert _rel(gx, x.grad) < 1e-14
+
+
+def test_continuous_warp_forward_and_backward(gpu):
+    from pyflowreg_amd import autograd as AG
+    g = torch.Generator().manual_seed(10)
+    T, C, H, W = 2, 2, 13, 17
+    img = torch.rand((T, C, H, W), dtype=F64, generator=g)
+    fill = torch.rand((C, H, W), dtype=F64, generator=g)
+    # displacements up to 3 px: some samples leave the image, some clip
+    u = (torch.rand((T, H, W), dtype=F64, generator=g) - 0.5) * 6
+    v = (torch.rand((T, H, W), dtype=F64, generator=g) - 0.5) * 6
+    it, ut, vt = (t.clone().requires_grad_(True) for t in (img, u, v))
+    ft = fill.clone().requires_grad_(True)
+    out_r = R.warp(it, ut, vt, ft)
+    go = torch.randn(out_r.shape, dtype=F64, generator=g)
+    (out_r * go).sum().backward()
+    id_, fd, ud, vd, god = _cuda(img, fill, u, v, go)
+    id_.requires_grad_(True)
+    fd.requires_grad_(True)
+    ud.requires_grad_(True)
+    vd.requires_grad_(True)
+    out = AG.warp(id_, ud, vd, fd)
+    (out * god).sum().backward()
+    assert _rel(out, out_r) < 1e-13
+    assert _rel(id_.grad, it.grad) < 1e-12
+    assert _rel(ud.grad, ut.grad) < 1e-12
+    assert _rel(vd.grad, vt.grad) < 1e-12
+    assert _rel(fd.grad, ft.grad) < 1e-14
+
+
+def test_continuous_warp_integer_shift_and_zero_flow(gpu):
+    """Zero flow is the identity and an integer shift moves pixels exactly;
+    samples that leave the image take the fill (core/warping.py:160-234)."""
+    from pyflowreg_amd import autograd as AG
+    img = torch.rand((1, 1, 9, 12), dtype=F64, device="cuda")
+    fill = torch.rand((1, 9, 12), dtype=F64, device="cuda")
+    z = torch.zeros((1, 9, 12), dtype=F64, device="cuda")
+    assert torch.equal(AG.warp(img, z, z, fill), img)
+    out = AG.warp(img, z + 2.0, z, fill)
+    assert torch.equal(out[0, 0, :, :-2], img[0, 0, :, 2:])
+    assert torch.equal(out[0, 0, :, -2:], fill[0, :, -2:])
+
+
+# ------------------------------------------------------------- level solve
+def test_level_solve_matches_torch_autograd(gpu):
+    from pyflowreg_amd import autograd as AG
+    g = torch.Generator().manual_seed(12)
+    fixed, moving = _frames(2, 21, 19, 2, seed=24)
+    f, m = _planes(fixed, moving)
+    T, C, h, w = m.shape
+    u0 = torch.randn((T, 2, h, w), dtype=F64, generator=g) * 0.4
+    W = torch.full((C, h, w), 0.5, dtype=F64)
+    hx, hy = 1.3, 1.2
+    leaves = dict(m=m.clone(), u0=u0.clone(), ax=torch.tensor(2.0 / hx ** 2, dtype=F64),
+                  ay=torch.tensor(2.5 / hy ** 2, dtype=F64), a=torch.tensor([0.45, 0.55], dtype=F64))
+    ref = {k: v.clone().requires_grad_(True) for k, v in leaves.items()}
+    out_r = R.level_solve(f, ref["m"], ref["u0"], ref["ax"], ref["ay"], ref["a"], W, 13, 5, hx, hy, True)
+    go = torch.randn(out_r.shape, dtype=F64, generator=g)
+    (out_r * go).sum().backward()
+    gd = {k: v.to("cuda").clone().requires_grad_(True) for k, v in leaves.items()}
+    fd, Wd, god = _cuda(f, W, go)
+    out = AG.LevelSolve.apply(fd, gd["m"], gd["u0"], gd["ax"], gd["ay"], gd["a"], Wd, 13, 5, hx, hy, True)
+    (out * god).sum().backward()
+    assert _rel(out, out_r) < 1e-11
+    for k in leaves:
+        assert _rel(gd[k].grad, ref[k].grad) < 1e-8, k
+
+
+# --------------------------------------------------------- whole pyramid
+def test_single_level_forward_matches_oracle(gpu, oracle):
+    """levels=0: no warp and float32-valued inputs, so the differentiable map
+    is the reference's get_displacement(levels=0) up to rounding."""
+    from pyflowreg_amd import autograd as AG
+    fixed, moving = _frames(2, 30, 26, 2, seed=25)
+    w = AG.get_displacement(torch.from_numpy(fixed).cuda(), torch.from_numpy(moving).cuda(), levels=0)
+    for t in range(2):
+        ref = oracle.get_displacement(fixed, moving[t], levels=0)
+        assert np.abs(w[t].cpu().numpy() - ref).max() < 1e-10 * max(1.0, np.abs(ref).max())
+
+
+def test_pyramid_forward_and_gradient_match_torch_autograd(gpu, oracle):
+    from pyflowreg_amd import autograd as AG
+    fixed, moving = _frames(2, 40, 48, 2, seed=26)
+    f, m = _planes(fixed, moving)
+    mask = (torch.rand((40, 48), dtype=F64, generator=torch.Generator().manual_seed(4)) > 0.2).double()
+    # reference
+    alpha_r = torch.tensor([2.0, 2.0], dtype=F64, requires_grad=True)
+    a_r = torch.tensor([0.45, 0.45], dtype=F64, requires_grad=True)
+    m_r = m.clone().requires_grad_(True)
+    flow_r = R.get_displacement(f, m_r, alpha_r, a_r, oracle, iterations=20)
+    loss_r = R.loss(f, m_r, flow_r, mask)
+    loss_r.backward()
+    # GPU
+    alpha = torch.tensor([2.0, 2.0], dtype=F64, device="cuda", requires_grad=True)
+    a = torch.tensor([0.45, 0.45], dtype=F64, device="cuda", requires_grad=True)
+    mv = torch.from_numpy(moving).cuda().requires_grad_(True)
+    fx = torch.from_numpy(fixed).cuda()
+    flow = AG.get_displacement(fx, mv, alpha=alpha, a_data=a, iterations=20)
+    loss = AG.registration_loss(fx, mv, flow, mask.cuda())
+    loss.backward()
+    errs = dict(flow=_rel(flow.permute(0, 3, 1, 2), flow_r), alpha=_rel(alpha.grad, alpha_r.grad),
+                a_data=_rel(a.grad, a_r.grad), moving=_rel(mv.grad.permute(0, 3, 1, 2), m_r.grad))
+    print("relative errors vs torch autograd:", errs, flush=True)
+    assert errs["flow"] < 1e-10
+    assert abs(float(loss.detach()) - float(loss_r.detach())) <= 1e-10 * abs(float(loss_r.detach()))
+    # 12 levels of sums in another order (atomics in the warp and tensor
+    # backward): gradients agree to ~1e-8
+    assert errs["alpha"] < 1e-6 and errs["a_data"] < 1e-6 and errs["moving"] < 1e-6
+
+
+def _fd_check(fixed, moving, mask, directions, steps, a_data=0.45, **opts):
+    """Analytic directional derivatives of the loss and central differences
+    at each step in ``steps[name]``: [(name, analytic, {h: fd})]."""
+    from pyflowreg_amd import autograd as AG
+    fx = torch.from_numpy(fixed).cuda()
+    base = dict(alpha=torch.tensor([2.0, 2.0], dtype=F64, device="cuda"),
+                a_data=torch.tensor([a_data] * fixed.shape[2], dtype=F64, device="cuda"),
+                moving=torch.from_numpy(moving).cuda())
+    mk = torch.as_tensor(mask, device="cuda")
+
+    def loss_of(p):
+        flow = AG.get_displacement(fx, p["moving"], alpha=p["alpha"], a_data=p["a_data"], **opts)
+        return AG.registration_loss(fx, p["moving"], flow, mk)
+
+    leaves = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+    loss_of(leaves).backward()
+    out = []
+    for name, d in directions:
+        d = d.to("cuda")
+        analytic = float((leaves[name].grad * d).sum())
+        fds = {}
+        with torch.no_grad():
+            for h in steps[name]:
+                lp = float(loss_of({**base, name: base[name] + h * d}))
+                lm = float(loss_of({**base, name: base[name] - h * d}))
+                fds[h] = (lp - lm) / (2 * h)
+        out.append((name, analytic, fds))
+    return out
+
+
+def _directions(moving, seed):
+    g = torch.Generator().manual_seed(seed)
+    dirs = [("alpha", torch.tensor([1.0, 0.0], dtype=F64)), ("alpha", torch.tensor([0.0, 1.0], dtype=F64)),
+            ("a_data", torch.tensor([1.0, 0.0], dtype=F64)), ("a_data", torch.tensor([0.0, 1.0], dtype=F64))]
+    for _ in range(2):
+        dirs.append(("moving", torch.randn(moving.shape, dtype=F64, generator=g)))
+    return dirs
+
+
+def _assert_fd(results, tol=1e-3):
+    for name, an, fds in results:
+        rel = {h: abs(an - fd) / max(abs(an), 1e-300) for h, fd in fds.items()}
+        print(f"{name}: analytic {an:.10e} " + " ".join(f"h={h:g}:{r:.1e}" for h, r in rel.items()), flush=True)
+        assert min(rel.values()) <= tol, (name, an, fds)
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_gradient_vs_central_differences_128(gpu):
+    """128x128 crop, 2 channels, full pyramid (12 levels), 50 iterations;
+    loss = masked SSE after registration; derivatives along alpha (both),
+    a_data (both channels) and two random moving-frame directions.
+
+    The map is piecewise smooth: every level's 5x5 median and every warp
+    sample that crosses the image border put kinks into it, spaced far below
+    1e-4 in these directions, so central differences estimate the derivative
+    only for h well below that; the check takes the best of h = 1e-7, 1e-8,
+    1e-9 and needs 1e-3 relative.  a_data = 0.7: at 0.45 the robust weight
+    (E + 1e-5)^(a - 1) is stiff enough (d psi / dE ~ 1e7 where the fit is
+    exact) that no step in FP64 isolates a smooth piece; the default exponent
+    is covered by the torch autograd comparison and the single-level check."""
+    fixed, moving = _frames(2, 128, 128, 2, seed=27)
+    mask = np.zeros((128, 128))
+    mask[8:-8, 8:-8] = 1.0
+    steps = dict(alpha=(1e-7, 1e-8, 1e-9), a_data=(1e-7, 1e-8, 1e-9), moving=(1e-7, 1e-8, 1e-9))
+    _assert_fd(_fd_check(fixed, moving, mask, _directions(moving, 13), steps, a_data=0.7))
+
+
+@pytest.mark.timeout(600)
+def test_adjoint_gradient_vs_central_differences_single_level(gpu):
+    """levels=0 at the default a_data = 0.45 (one 128x128 level, no warp
+    between levels): best of h = 1e-5 .. 1e-7 for alpha and a_data and of
+    h = 1e-7 .. 1e-9 along moving-frame directions, where the stiff robust
+    weight shortens the smooth pieces."""
+    fixed, moving = _frames(2, 128, 128, 2, seed=28)
+    mask = np.zeros((128, 128))
+    mask[8:-8, 8:-8] = 1.0
+    steps = dict(alpha=(1e-5, 1e-6, 1e-7), a_data=(1e-5, 1e-6, 1e-7), moving=(1e-7, 1e-8, 1e-9))
+    _assert_fd(_fd_check(fixed, moving, mask, _directions(moving, 14), steps, levels=0))
+
+
+@pytest.mark.timeout(900)
+def test_gradient_vs_torch_autograd_128(gpu, oracle):
+    """The same 128x128 crop, one moving frame, against the torch autograd
+    restatement of the whole pyramid."""
+    from pyflowreg_amd import autograd as AG
+    fixed, moving = _frames(1, 128, 128, 2, seed=27)
+    f, m = _planes(fixed, moving)
+    mask = torch.zeros((128, 128), dtype=F64)
+    mask[8:-8, 8:-8] = 1.0
+    alpha_r = torch.tensor([2.0, 2.0], dtype=F64, requires_grad=True)
+    a_r = torch.tensor([0.45, 0.45], dtype=F64, requires_grad=True)
+    m_r = m.clone().requires_grad_(True)
+    flow_r = R.get_displacement(f, m_r, alpha_r, a_r, oracle)
+    R.loss(f, m_r, flow_r, mask).backward()
+    alpha = torch.tensor([2.0, 2.0], dtype=F64, device="cuda", requires_grad=True)
+    a = torch.tensor([0.45, 0.45], dtype=F64, device="cuda", requires_grad=True)
+    mv = torch.from_numpy(moving).cuda().requires_grad_(True)
+    fx = torch.from_numpy(fixed).cuda()
+    flow = AG.get_displacement(fx, mv, alpha=alpha, a_data=a)
+    AG.registration_loss(fx, mv, flow, mask.cuda()).backward()
+    errs = dict(flow=_rel(flow.permute(0, 3, 1, 2), flow_r), alpha=_rel(alpha.grad, alpha_r.grad),
+                a_data=_rel(a.grad, a_r.grad), moving=_rel(mv.grad.permute(0, 3, 1, 2), m_r.grad))
+    print("relative errors vs torch autograd:", errs, flush=True)
+    # 12 levels x 50 sweeps of the robust solve amplify rounding: ~4e-9
+    assert errs["flow"] < 1e-7
+    assert errs["alpha"] < 1e-5 and errs["a_data"] < 1e-5 and errs["moving"] < 1e-5
