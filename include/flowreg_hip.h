@@ -19,6 +19,7 @@
 #ifndef FLOWREG_HIP_H
 #define FLOWREG_HIP_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -198,18 +199,28 @@ int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving,
 int frh_solver_profile(frh_solver *s, int enable);
 int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n);
 
-/* ------------------------------------- differentiable single-level solve */
-/* Forward and backward (reverse-mode) kernels of get_displacement(...,
- * levels=0) (core/optical_flow.py:476-617 with one level, zero initial flow,
- * gradient-constancy data term, core/level_solver.py:158-369) and of a
- * continuous bicubic warp (core/warping.py:160-234 without the 1/32-px map
- * rounding), all float64, row-major per frame, for gradients of a
- * registration loss w.r.t. alpha, a_data and the moving frames.  No
- * counterpart entry points exist in the reference (it has no gradients);
- * the forward operations are the reference's.
- *   images (T, C, h, w); states (T, 2, h, w) = (du, dv);
- *   J (T, 6, C, h, w); coefficients (T, 6, h, w) =
- *   (A12, Bu, Bv, omega/Du, omega/Dv, b).  At most 2 channels. */
+/* ------------------------------------------- differentiable registration */
+/* Forward and backward (reverse-mode) kernels of one pass of
+ * get_displacement (core/optical_flow.py:476-617: gradient-constancy data
+ * term, core/level_solver.py:158-369, a_smooth = 1) and of a continuous
+ * bicubic warp (core/warping.py:160-234 without the 1/32-px map rounding),
+ * all float64, for gradients of a registration loss w.r.t. alpha, a_data
+ * and the moving frames.  The reference has no gradients, so these entry
+ * points have no counterparts there; their forward operations are the
+ * reference's.
+ *   images (T, C, h, w) and J (T, 6, C, h, w) row-major;
+ *   solver states (T, 2, plane) = (du, dv), coefficients (T, 6, plane) =
+ *   (A12, Bu, Bv, omega/Du, omega/Dv, b) and their adjoints in the skewed
+ *   layout: cell (r, c) at slot (r + c) * hs + r, hs = 64 * ceil(h / 64),
+ *   plane = (h + w - 1) * hs (frh_grad_skew_plane); padding slots are never
+ *   read.  At most 2 channels and 1024 rows per level. */
+
+/* Doubles per skewed plane of an h x w level. */
+int frh_grad_skew_plane(int h, int w, size_t *plane);
+
+/* N planes (h, w) row-major -> skewed (to_skew = 1, padding slots set to 0)
+ * or skewed -> row-major (to_skew = 0). */
+int frh_grad_skew(const double *src, int N, int h, int w, double *dst, int to_skew, void *stream);
 
 /* get_motion_tensor_gc of fixed (C, h, w) and moving (T, C, h, w): J = f
  * (when J != NULL); g_moving += dJ/dmoving^T gJ (when gJ != NULL). */
@@ -217,24 +228,24 @@ int frh_grad_tensor(const double *fixed, const double *moving, int T, int C, int
                     double hy, double hx, double *J, const double *gJ, double *g_moving,
                     void *stream);
 
-/* Lag-block coefficients from the state x at a psi refresh (psi = 1 when
- * psi_one), the level's initial flow u0 (T, 2, h, w; NULL = zero), J,
+/* Lag-block coefficients from the state x (skewed) at a psi refresh (psi =
+ * 1 when psi_one), the level's initial flow u0 (T, 2, h, w; NULL = zero), J,
  * weight W (C, h, w), ax = alpha_x / hx^2, ay = alpha_y / hy^2 and a_data
- * (C, device): coef (when != NULL).  With g_coef, adds the pull-backs g_x
- * (T, 2, h, w), g_u0 (when u0 and g_u0 are not NULL), g_J and per-cell
- * partials g_par (T, 2 + C, h, w) of ax, ay, a_k. */
+ * (C, device): coef (skewed, when != NULL).  With g_coef (skewed), adds the
+ * pull-backs g_x (skewed), g_u0 (row-major, when u0 and g_u0 are not NULL),
+ * g_J and per-cell partials g_par (T, 2 + C, plane) of ax, ay, a_k. */
 int frh_grad_coef(const double *x, const double *u0, const double *J, const double *W, int T, int C,
                   int h, int w, double ax, double ay, const double *a_data, int psi_one, double *coef,
                   const double *g_coef, double *g_x, double *g_u0, double *g_J, double *g_par,
                   void *stream);
 
-/* One column-major lexicographic SOR sweep x_in -> x_out. */
+/* One column-major lexicographic SOR sweep x_in -> x_out (skewed). */
 int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T, int h, int w,
                    double ax, double ay, void *stream);
 
-/* Exact adjoint of frh_grad_sweep: lam_in = (dx_out/dx_in)^T lam_out;
- * g_coef += (dx_out/dcoef)^T lam_out; g_axy (T, 2, h, w) += per-cell
- * partials of ax, ay.  scratch: 4 T h w doubles. */
+/* Exact adjoint of frh_grad_sweep (all skewed): lam_in = (dx_out/dx_in)^T
+ * lam_out; g_coef += (dx_out/dcoef)^T lam_out; g_axy (T, 2, plane) += per-cell
+ * partials of ax, ay.  scratch: 4 T plane doubles. */
 int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double *x_out,
                            const double *lam_out, double *lam_in, double *g_coef, double *g_axy,
                            double *scratch, int T, int h, int w, double ax, double ay,

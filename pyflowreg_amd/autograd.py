@@ -24,7 +24,8 @@ refresh and each lag block is recomputed from its checkpoint in the backward
 pass.
 
 Layouts (device, float64, contiguous): images (T, C, h, w), flows
-(T, 2, h, w) with plane 0 = u (x) and plane 1 = v (y).
+(T, 2, h, w) with plane 0 = u (x) and plane 1 = v (y); inside a level solve
+the sweeps' states and coefficients use the skewed layout (``skew_plane``).
 """
 
 from __future__ import annotations
@@ -109,6 +110,33 @@ def warp(img, u, v, fill):
 
 
 # ------------------------------------------------------------ level solve
+def skew_plane(h: int, w: int) -> int:
+    """Doubles per plane of the solver's skewed layout: cell (r, c) at slot
+    (r + c) * hs + r, hs = 64 * ceil(h / 64)."""
+    return (h + w - 1) * 64 * ((h + 63) // 64)
+
+
+def to_skew(x: torch.Tensor) -> torch.Tensor:
+    """(..., h, w) row-major planes -> (..., plane) skewed (padding = 0)."""
+    h, w = x.shape[-2:]
+    lead = x.shape[:-2]
+    src = _c(x)
+    out = torch.empty(lead + (skew_plane(h, w),), dtype=_F64, device=x.device)
+    n = int(np.prod(lead)) if lead else 1
+    check(lib().frh_grad_skew(ptr(src), n, h, w, ptr(out), 1, stream_handle()), "frh_grad_skew")
+    return out
+
+
+def from_skew(x: torch.Tensor, h: int, w: int) -> torch.Tensor:
+    """(..., plane) skewed -> (..., h, w) row-major."""
+    lead = x.shape[:-1]
+    src = _c(x)
+    out = torch.empty(lead + (h, w), dtype=_F64, device=x.device)
+    n = int(np.prod(lead)) if lead else 1
+    check(lib().frh_grad_skew(ptr(src), n, h, w, ptr(out), 0, stream_handle()), "frh_grad_skew")
+    return out
+
+
 class LevelSolve(torch.autograd.Function):
     """One pyramid level of core/optical_flow.py:582-613:
     J = get_motion_tensor_gc(f1, f2w); (du, dv) = compute_flow(J, u0) with
@@ -118,7 +146,9 @@ class LevelSolve(torch.autograd.Function):
 
     f1 (C, h, w) fixed level (constant); f2w (T, C, h, w) warped moving
     level; u0 (T, 2, h, w) the level's initial flow; ax = alpha_x / hx^2 and
-    ay = alpha_y / hy^2 (0-dim); a_data (C,); weight (C, h, w) (constant)."""
+    ay = alpha_y / hy^2 (0-dim); a_data (C,); weight (C, h, w) (constant).
+    The sweeps' states, coefficients and adjoints are kept in the skewed
+    layout (``skew_plane``)."""
 
     @staticmethod
     def forward(ctx, f1, f2w, u0, ax, ay, a_data, weight, iterations, update_lag, hx, hy, median):
@@ -132,14 +162,16 @@ class LevelSolve(torch.autograd.Function):
             raise ValueError("LevelSolve: at most 2 channels and 1024 rows")
         L, st = lib(), stream_handle()
         axf, ayf = float(ax), float(ay)
-        J = torch.empty((T, 6, C, h, w), dtype=_F64, device=f2w.device)
+        P = skew_plane(h, w)
+        dev = f2w.device
+        J = torch.empty((T, 6, C, h, w), dtype=_F64, device=dev)
         check(L.frh_grad_tensor(ptr(f1), ptr(f2w), T, C, h, w, hy, hx, ptr(J), None, None, st),
               "frh_grad_tensor")
         blocks = lag_blocks(iterations, update_lag)
-        ckpt = torch.empty((len(blocks), T, 2, h, w), dtype=_F64, device=f2w.device)
-        x = torch.zeros((T, 2, h, w), dtype=_F64, device=f2w.device)
+        ckpt = torch.empty((len(blocks), T, 2, P), dtype=_F64, device=dev)
+        x = torch.zeros((T, 2, P), dtype=_F64, device=dev)
         x_next = torch.empty_like(x)
-        coef = torch.empty((T, 6, h, w), dtype=_F64, device=f2w.device)
+        coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
         for b, (s, e, psi_one) in enumerate(blocks):
             ckpt[b].copy_(x)
             check(L.frh_grad_coef(ptr(x), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
@@ -149,7 +181,7 @@ class LevelSolve(torch.autograd.Function):
                 check(L.frh_grad_sweep(ptr(coef), ptr(x), ptr(x_next), T, h, w, axf, ayf, st),
                       "frh_grad_sweep")
                 x, x_next = x_next, x
-        du = x
+        du = from_skew(x, h, w)
         if median:
             dm = torch.empty_like(du)
             check(L.frh_grad_median(ptr(du), 2 * T, h, w, ptr(dm), None, None, st), "frh_grad_median")
@@ -165,20 +197,23 @@ class LevelSolve(torch.autograd.Function):
         iterations, update_lag, hx, hy, median, axf, ayf = ctx.cfg
         T, C, h, w = f2w.shape
         L, st = lib(), stream_handle()
+        P = skew_plane(h, w)
+        dev = du.device
         g_u = _c(g_u)
         g_u0 = g_u.clone()
         if median:
-            lam = torch.zeros_like(du)
-            check(L.frh_grad_median(ptr(du), 2 * T, h, w, None, ptr(g_u), ptr(lam), st), "frh_grad_median")
+            g_du = torch.zeros_like(du)
+            check(L.frh_grad_median(ptr(du), 2 * T, h, w, None, ptr(g_u), ptr(g_du), st), "frh_grad_median")
         else:
-            lam = g_u.clone()
-        g_J = torch.zeros_like(J)
-        g_par = torch.zeros((T, 2 + C, h, w), dtype=_F64, device=du.device)
-        g_axy = torch.zeros((T, 2, h, w), dtype=_F64, device=du.device)
-        g_coef = torch.empty((T, 6, h, w), dtype=_F64, device=du.device)
-        coef = torch.empty_like(g_coef)
-        scratch = torch.empty((4, T, h, w), dtype=_F64, device=du.device)
+            g_du = g_u
+        lam = to_skew(g_du)
         lam_in = torch.empty_like(lam)
+        g_J = torch.zeros_like(J)
+        g_par = torch.zeros((T, 2 + C, P), dtype=_F64, device=dev)
+        g_axy = torch.zeros((T, 2, P), dtype=_F64, device=dev)
+        g_coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
+        coef = torch.empty_like(g_coef)
+        scratch = torch.empty((T, 4, P), dtype=_F64, device=dev)
         blocks = lag_blocks(iterations, update_lag)
         states = None
         for b in range(len(blocks) - 1, -1, -1):
@@ -189,7 +224,7 @@ class LevelSolve(torch.autograd.Function):
                   "frh_grad_coef")
             n = e - s
             if states is None or states.shape[0] < n + 1:
-                states = torch.empty((n + 1, T, 2, h, w), dtype=_F64, device=du.device)
+                states = torch.empty((n + 1, T, 2, P), dtype=_F64, device=dev)
             states[0].copy_(x0)
             for k in range(n):  # recompute the block from its checkpoint
                 check(L.frh_grad_sweep(ptr(coef), ptr(states[k]), ptr(states[k + 1]), T, h, w, axf, ayf, st),
@@ -213,7 +248,7 @@ class LevelSolve(torch.autograd.Function):
                   "frh_grad_tensor")
         g_ax = (g_par[:, 0].sum() + g_axy[:, 0].sum()).reshape(())
         g_ay = (g_par[:, 1].sum() + g_axy[:, 1].sum()).reshape(())
-        g_a = g_par[:, 2:].sum(dim=(0, 2, 3))
+        g_a = g_par[:, 2:].sum(dim=(0, 2))
         return None, g_f2w, g_u0, g_ax, g_ay, g_a, None, None, None, None, None, None
 
 

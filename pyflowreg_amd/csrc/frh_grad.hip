@@ -18,6 +18,7 @@
 
 #include "../../include/flowreg_hip.h"
 #include "frh_internal.h"
+#include "frh_device.h"
 
 namespace frh {
 namespace grad {
@@ -252,19 +253,67 @@ __device__ inline void flow_diffs(const double *f, int h, int w, int r, int c, d
     sy = (r > 0 ? f[p - w] - fc : 0.0) + (r < h - 1 ? f[p + w] - fc : 0.0);
 }
 
-__global__ __launch_bounds__(64) void k_gcoef_fwd(const double *x, const double *u0, const double *J, const double *W, int T,
-                            int C, int h, int w, double ax, double ay, const double *a_data, int psi_one,
-                            double *coef)
+// Neighbour sums of a skewed plane f at cell (r, c) in slot q: the same
+// operator as flow_diffs ((r, c +- 1) sit at q +- hs, (r +- 1, c) at
+// q +- (hs + 1)).
+__device__ inline void flow_diffs_skew(const double *f, int h, int w, int hs, int r, int c, size_t q,
+                                       double &sx, double &sy)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-    if (c >= w) return;
+    const double fc = f[q];
+    sx = (c > 0 ? f[q - hs] - fc : 0.0) + (c < w - 1 ? f[q + hs] - fc : 0.0);
+    sy = (r > 0 ? f[q - hs - 1] - fc : 0.0) + (r < h - 1 ? f[q + hs + 1] - fc : 0.0);
+}
+
+// The solve's state, coefficients and their adjoints live in the skewed
+// layout of the production solver: cell (r, c) of a frame's h x w level at
+// slot (r + c) * hs + r of a plane of (h + w - 1) * hs doubles, hs = 64 *
+// ceil(h / 64), so that one anti-diagonal is one contiguous run.  Padding
+// slots are never written by the solve's kernels.
+struct GSkew {
+    int h, w, hs;
+    size_t plane;
+};
+
+__device__ inline bool skew_cell(const GSkew &k, size_t q, int &r, int &c)
+{
+    r = (int)(q % (size_t)k.hs);
+    c = (int)(q / (size_t)k.hs) - r;
+    return r < k.h && c >= 0 && c < k.w;
+}
+
+// (N, h, w) row-major <-> (N, plane) skewed (padding slots set to zero)
+__global__ void k_gskew(const double *src, double *dst, int N, GSkew k, int to_skew)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int n = blockIdx.y;
+    if (q >= k.plane) return;
+    int r, c;
+    const bool ok = skew_cell(k, q, r, c);
+    const size_t hw = (size_t)k.h * k.w;
+    if (to_skew)
+        dst[(size_t)n * k.plane + q] = ok ? src[(size_t)n * hw + (size_t)r * k.w + c] : 0.0;
+    else if (ok)
+        dst[(size_t)n * hw + (size_t)r * k.w + c] = src[(size_t)n * k.plane + q];
+}
+
+// One thread per skewed slot: x, coef skewed; u0, J, W row-major.
+__global__ __launch_bounds__(64) void k_gcoef_fwd(const double *x, const double *u0, const double *J,
+                                                   const double *W, int T, int C, GSkew k, double ax,
+                                                   double ay, const double *a_data, int psi_one, double *coef)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (q >= k.plane) return;
+    int r, c;
+    if (!skew_cell(k, q, r, c)) return;
+    const int h = k.h, w = k.w;
     const size_t hw = (size_t)h * w, p = (size_t)r * w + c;
     double Jr[6 * kMaxC], Wr[kMaxC], a[kMaxC], K[6];
     for (int e = 0; e < 6; ++e)
-        for (int k = 0; k < C; ++k) Jr[e * C + k] = J[(((size_t)t * 6 + e) * C + k) * hw + p];
-    for (int k = 0; k < C; ++k) {
-        Wr[k] = W[k * hw + p];
-        a[k] = a_data[k];
+        for (int kk = 0; kk < C; ++kk) Jr[e * C + kk] = J[(((size_t)t * 6 + e) * C + kk) * hw + p];
+    for (int kk = 0; kk < C; ++kk) {
+        Wr[kk] = W[kk * hw + p];
+        a[kk] = a_data[kk];
     }
     double nlr, nud, cu = 0.0, cv = 0.0;
     edge_counts(h, w, r, c, nlr, nud);
@@ -275,47 +324,50 @@ __global__ __launch_bounds__(64) void k_gcoef_fwd(const double *x, const double 
         flow_diffs(u0 + ((size_t)t * 2 + 1) * hw, h, w, r, c, sx, sy);
         cv = ax * sx + ay * sy;
     }
-    coef_cell(x[((size_t)t * 2) * hw + p], x[((size_t)t * 2 + 1) * hw + p], ax, ay, a, Jr, Wr, C, psi_one,
-              nlr, nud, cu, cv, K);
-    for (int e = 0; e < 6; ++e) coef[((size_t)t * 6 + e) * hw + p] = K[e];
+    const double *X = x + (size_t)t * 2 * k.plane;
+    coef_cell(X[q], X[k.plane + q], ax, ay, a, Jr, Wr, C, psi_one, nlr, nud, cu, cv, K);
+    for (int e = 0; e < 6; ++e) coef[((size_t)t * 6 + e) * k.plane + q] = K[e];
 }
 
-// Pull-back of the coefficient adjoints gK (T, 6, h, w) through coef_cell:
-// gx += dK/dx^T gK (T, 2, h, w), gJ += dK/dJ^T gK, per-cell partial sums
-// for ax, ay, a_k in gpar (T, 2 + C, h, w), and (with u0) gu0 += the
-// adjoint of the smoothness terms: cu = ax * sx(u) + ay * sy(u) with the
-// symmetric neighbour-difference operators sx, sy, so gu0 = ax * sx(gBu) +
-// ay * sy(gBu).  Reverse mode by hand:
+// Pull-back of the coefficient adjoints gK (T, 6, plane) through coef_cell:
+// gx += dK/dx^T gK (T, 2, plane), gJ += dK/dJ^T gK (row-major), per-cell
+// partial sums for ax, ay, a_k in gpar (T, 2 + C, plane), and (with u0)
+// gu0 += the adjoint of the smoothness terms (row-major): cu = ax * sx(u) +
+// ay * sy(u) with the symmetric neighbour-difference operators sx, sy, so
+// gu0 = ax * sx(gBu) + ay * sy(gBu).  Reverse mode by hand:
 //   gDu = -omega / Du^2 gK3, gDv = -omega / Dv^2 gK4, gws = gDu + gDv
 //   per channel: g_wp = gDu j11 + gDv j22 + gK0 j12 - gK1 j13 - gK2 j23,
 //   g_psi = W g_wp; psi = a p, p = (e + 1e-5)^(a - 1), e = max(E, 0):
 //   dpsi/da = p (1 + a ln(e + 1e-5)), dpsi/dE = a (a - 1) p / (e + 1e-5)
 //   (zero for E < 0); E is quadratic in (x, y) and linear in J.
 __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double *u0, const double *J,
-                                                   const double *W, const double *gK, int T, int C, int h,
-                                                   int w, double ax, double ay, const double *a_data,
-                                                   int psi_one, double *gx, double *gu0, double *gJ,
-                                                   double *gpar)
+                                                   const double *W, const double *gK, int T, int C, GSkew k,
+                                                   double ax, double ay, const double *a_data, int psi_one,
+                                                   double *gx, double *gu0, double *gJ, double *gpar)
 {
-    const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, t = blockIdx.z;
-    if (c >= w) return;
-    const size_t hw = (size_t)h * w, p = (size_t)r * w + c;
-    const double xv = x[((size_t)t * 2) * hw + p], yv = x[((size_t)t * 2 + 1) * hw + p];
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (q >= k.plane) return;
+    int r, c;
+    if (!skew_cell(k, q, r, c)) return;
+    const int h = k.h, w = k.w;
+    const size_t hw = (size_t)h * w, p = (size_t)r * w + c, P = k.plane;
+    const double xv = x[(size_t)t * 2 * P + q], yv = x[((size_t)t * 2 + 1) * P + q];
     double g[6];
-    for (int e = 0; e < 6; ++e) g[e] = gK[((size_t)t * 6 + e) * hw + p];
+    for (int e = 0; e < 6; ++e) g[e] = gK[((size_t)t * 6 + e) * P + q];
     // Du, Dv at the refresh (recomputed)
     const double ws = (ax + ax) + (ay + ay);
     double Du = ws, Dv = ws;
     double psi[kMaxC], Jc[6 * kMaxC];
-    for (int k = 0; k < C; ++k) {
-        for (int e = 0; e < 6; ++e) Jc[e * kMaxC + k] = J[(((size_t)t * 6 + e) * C + k) * hw + p];
-        const double j11 = Jc[0 * kMaxC + k], j22 = Jc[1 * kMaxC + k], j33 = Jc[2 * kMaxC + k];
-        const double j12 = Jc[3 * kMaxC + k], j13 = Jc[4 * kMaxC + k], j23 = Jc[5 * kMaxC + k];
-        psi[k] = psi_one ? 1.0
-                         : psi_of(j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
-                                      2.0 * (j13 * xv) + j33,
-                                  a_data[k]);
-        const double wp = W[k * hw + p] * psi[k];
+    for (int kk = 0; kk < C; ++kk) {
+        for (int e = 0; e < 6; ++e) Jc[e * kMaxC + kk] = J[(((size_t)t * 6 + e) * C + kk) * hw + p];
+        const double j11 = Jc[0 * kMaxC + kk], j22 = Jc[1 * kMaxC + kk], j33 = Jc[2 * kMaxC + kk];
+        const double j12 = Jc[3 * kMaxC + kk], j13 = Jc[4 * kMaxC + kk], j23 = Jc[5 * kMaxC + kk];
+        psi[kk] = psi_one ? 1.0
+                          : psi_of(j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
+                                       2.0 * (j13 * xv) + j33,
+                                   a_data[kk]);
+        const double wp = W[kk * hw + p] * psi[kk];
         Du += wp * j11;
         Dv += wp * j22;
     }
@@ -325,10 +377,10 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
     edge_counts(h, w, r, c, nlr, nud);
     double g_ax = 2.0 * gws + nlr * g[5], g_ay = 2.0 * gws + nud * g[5];
     double g_x = 0.0, g_y = 0.0;
-    for (int k = 0; k < C; ++k) {
-        const double j11 = Jc[0 * kMaxC + k], j22 = Jc[1 * kMaxC + k], j33 = Jc[2 * kMaxC + k];
-        const double j12 = Jc[3 * kMaxC + k], j13 = Jc[4 * kMaxC + k], j23 = Jc[5 * kMaxC + k];
-        const double Wk = W[k * hw + p], wp = Wk * psi[k];
+    for (int kk = 0; kk < C; ++kk) {
+        const double j11 = Jc[0 * kMaxC + kk], j22 = Jc[1 * kMaxC + kk], j33 = Jc[2 * kMaxC + kk];
+        const double j12 = Jc[3 * kMaxC + kk], j13 = Jc[4 * kMaxC + kk], j23 = Jc[5 * kMaxC + kk];
+        const double Wk = W[kk * hw + p], wp = Wk * psi[kk];
         double gj[6];
         gj[0] = gDu * wp;
         gj[1] = gDv * wp;
@@ -341,7 +393,7 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
             const double g_psi = Wk * (gDu * j11 + gDv * j22 + g[0] * j12 - g[1] * j13 - g[2] * j23);
             const double E = j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
                              2.0 * (j13 * xv) + j33;
-            const double a = a_data[k], base = (E < 0 ? 0.0 : E) + 1e-5;
+            const double a = a_data[kk], base = (E < 0 ? 0.0 : E) + 1e-5;
             const double pw = pow(base, a - 1.0);
             g_a = g_psi * pw * (1.0 + a * log(base));
             const double gE = E < 0 ? 0.0 : g_psi * a * (a - 1.0) * pw / base;
@@ -354,11 +406,11 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
             g_x += gE * 2.0 * (j11 * xv + j12 * yv + j13);
             g_y += gE * 2.0 * (j22 * yv + j12 * xv + j23);
         }
-        gpar[((size_t)t * (2 + C) + 2 + k) * hw + p] += g_a;
-        for (int e = 0; e < 6; ++e) gJ[(((size_t)t * 6 + e) * C + k) * hw + p] += gj[e];
+        gpar[((size_t)t * (2 + C) + 2 + kk) * P + q] += g_a;
+        for (int e = 0; e < 6; ++e) gJ[(((size_t)t * 6 + e) * C + kk) * hw + p] += gj[e];
     }
     if (u0) {
-        const double *gBu = gK + ((size_t)t * 6 + 1) * hw, *gBv = gK + ((size_t)t * 6 + 2) * hw;
+        const double *gBu = gK + ((size_t)t * 6 + 1) * P, *gBv = gK + ((size_t)t * 6 + 2) * P;
         double sx, sy;
         flow_diffs(u0 + (size_t)t * 2 * hw, h, w, r, c, sx, sy);
         g_ax += g[1] * sx;
@@ -367,122 +419,244 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
         g_ax += g[2] * sx;
         g_ay += g[2] * sy;
         if (gu0) {
-            flow_diffs(gBu, h, w, r, c, sx, sy);
+            flow_diffs_skew(gBu, h, w, k.hs, r, c, q, sx, sy);
             gu0[((size_t)t * 2) * hw + p] += ax * sx + ay * sy;
-            flow_diffs(gBv, h, w, r, c, sx, sy);
+            flow_diffs_skew(gBv, h, w, k.hs, r, c, q, sx, sy);
             gu0[((size_t)t * 2 + 1) * hw + p] += ax * sx + ay * sy;
         }
     }
-    gx[((size_t)t * 2) * hw + p] += g_x;
-    gx[((size_t)t * 2 + 1) * hw + p] += g_y;
-    gpar[((size_t)t * (2 + C) + 0) * hw + p] += g_ax;
-    gpar[((size_t)t * (2 + C) + 1) * hw + p] += g_ay;
+    gx[(size_t)t * 2 * P + q] += g_x;
+    gx[((size_t)t * 2 + 1) * P + q] += g_y;
+    gpar[((size_t)t * (2 + C) + 0) * P + q] += g_ax;
+    gpar[((size_t)t * (2 + C) + 1) * P + q] += g_ay;
 }
 
 // --------------------------------------------------------------- sweeps
 // One column-major lexicographic SOR sweep (core/level_solver.py:257-364),
-// one work-group per frame, lane = row, one diagonal r + c = d per step:
+// one work-group per frame, lane = row, one anti-diagonal d = r + c per
+// step:
 //   nu = Bu + ax (l + r) + ay (d + u) - A12 ov
 //   du' = wu nu + (1 - omega + wu b) ou
 //   nv = Bv + ax (lv + rv) + ay (dv + uv) - A12 du'
 //   dv' = wv nv + (1 - omega + wv b) ov
 // (left / up neighbours new, right / down old; neighbours outside the level
-// hold the cell's own start value, folded into b).
-__global__ void k_gsweep_fwd(const double *coef, const double *xin, double *xout, int h, int w,
-                             double ax, double ay)
+// hold the cell's own start value, folded into b).  The left neighbour is
+// the lane's own previous result, the upper one lane r - 1's (a DPP shift
+// inside the wave, LDS across waves); the old values and coefficients do not
+// depend on the sweep, so they are loaded D steps ahead into a register ring
+// with branch-free buffer loads, and the per-step barrier waits for LDS only.
+constexpr int kGMaxWaves = 16;
+
+struct GFwdSlot {
+    double K[6], ou, ov, ru, rv, du, dv;
+};
+
+template <int D, int MAXW>
+__global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, const double *xin, double *xout,
+                                                          GSkew k, double ax, double ay)
 {
-    const int t = blockIdx.x, r = threadIdx.x;
-    const size_t hw = (size_t)h * w;
-    const double *K = coef + (size_t)t * 6 * hw;
-    const double *ui = xin + (size_t)t * 2 * hw, *vi = ui + hw;
-    double *uo = xout + (size_t)t * 2 * hw, *vo = uo + hw;
-    for (int d = 0; d <= h + w - 2; ++d) {
-        const int c = d - r;
-        if (r < h && c >= 0 && c < w) {
-            const size_t p = (size_t)r * w + c;
-            const double A12 = K[p], Bu = K[hw + p], Bv = K[2 * hw + p], wu = K[3 * hw + p],
-                         wv = K[4 * hw + p], b = K[5 * hw + p];
-            const double lu = c > 0 ? uo[p - 1] : 0.0, uu = r > 0 ? uo[p - w] : 0.0;
-            const double ru = c < w - 1 ? ui[p + 1] : 0.0, du = r < h - 1 ? ui[p + w] : 0.0;
-            const double lv = c > 0 ? vo[p - 1] : 0.0, uv = r > 0 ? vo[p - w] : 0.0;
-            const double rv = c < w - 1 ? vi[p + 1] : 0.0, dv = r < h - 1 ? vi[p + w] : 0.0;
-            const double ou = ui[p], ov = vi[p];
-            const double nu = Bu + ax * (lu + ru) + ay * (du + uu) - A12 * ov;
-            const double un = wu * nu + (1.0 - kOmega + wu * b) * ou;
-            const double nv = Bv + ax * (lv + rv) + ay * (dv + uv) - A12 * un;
-            uo[p] = un;
-            vo[p] = wv * nv + (1.0 - kOmega + wv * b) * ov;
+    __shared__ double xb[2][MAXW][2];
+    const int t = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = r >> 6;
+    const int h = k.h, w = k.w, hs = k.hs;
+    const bool row_ok = r < h, down_ok = r + 1 < h;
+    const unsigned pb = (unsigned)(k.plane * 8);
+    const auto rK = make_rsrc(coef + (size_t)t * 6 * k.plane, 6 * pb);
+    const auto rI = make_rsrc(xin + (size_t)t * 2 * k.plane, 2 * pb);
+    const auto rO = make_rsrc(xout + (size_t)t * 2 * k.plane, 2 * pb);
+    static_assert(D >= 2, "a ring slot is refilled while another is in use");
+    GFwdSlot S[D];
+    auto issue = [&](GFwdSlot &s, int d) {
+        const unsigned q = slot_offset(row_ok, d, r, w, hs);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) s.K[e] = buf_load_f64(rK, q, e * pb);
+        s.ou = buf_load_f64(rI, q, 0);
+        s.ov = buf_load_f64(rI, q, pb);
+        const unsigned qr = slot_offset(row_ok, d + 1, r, w, hs);        // (r, c + 1)
+        s.ru = buf_load_f64(rI, qr, 0);
+        s.rv = buf_load_f64(rI, qr, pb);
+        const unsigned qd = slot_offset(down_ok, d + 1, r + 1, w, hs);   // (r + 1, c)
+        s.du = buf_load_f64(rI, qd, 0);
+        s.dv = buf_load_f64(rI, qd, pb);
+    };
+    const int nd = h + w - 1;
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) issue(S[s], s);
+    double pu = 0.0, pv = 0.0;  // this lane's results of the previous step
+    for (int d0 = 0; d0 < nd; d0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int d = d0 + s;
+            issue(S[(s + D - 1) % D], d + D - 1);
+            const GFwdSlot &c = S[s];
+            double uu = wave_from_prev_lane(pu), uv = wave_from_prev_lane(pv);
+            const int wu_src = wv > 0 ? wv - 1 : 0;
+            const double eu = xb[(d + 1) & 1][wu_src][0], ev = xb[(d + 1) & 1][wu_src][1];
+            if (lane == 0) {
+                uu = wv > 0 ? eu : 0.0;
+                uv = wv > 0 ? ev : 0.0;
+            }
+            const unsigned q = slot_offset(row_ok, d, r, w, hs);
+            const double A12 = c.K[0], Bu = c.K[1], Bv = c.K[2], wu = c.K[3], wvv = c.K[4], b = c.K[5];
+            const double nu = Bu + ax * (pu + c.ru) + ay * (c.du + uu) - A12 * c.ov;
+            double un = wu * nu + (1.0 - kOmega + wu * b) * c.ou;
+            const double nv = Bv + ax * (pv + c.rv) + ay * (c.dv + uv) - A12 * un;
+            double vn = wvv * nv + (1.0 - kOmega + wvv * b) * c.ov;
+            const bool ok = q != kOutOfRange;
+            un = ok ? un : 0.0;
+            vn = ok ? vn : 0.0;
+            buf_store_f64(un, rO, q, 0);
+            buf_store_f64(vn, rO, q, pb);
+            pu = un;
+            pv = vn;
+            if (lane == 63) {
+                xb[d & 1][wv][0] = un;
+                xb[d & 1][wv][1] = vn;
+            }
+            lds_barrier();
         }
-        __syncthreads();
     }
 }
 
-// Reverse-mode derivative of k_gsweep_fwd.  lam_out: adjoint of xout;
-// writes lam_in (adjoint of xin), adds the coefficient adjoints to gK and
-// per-cell partials of ax, ay to gaxy (T, 2, h, w).  Scratch anu, anv, adu,
-// adv (T, h, w).  Diagonals run from last to first: a cell's new values
-// are consumed only by its right and lower neighbours (diagonal d + 1) and
-// by the next sweep (lam_out).
-__global__ void k_gsweep_bwd(const double *coef, const double *xin, const double *xout,
-                             const double *lam_out, double *lam_in, double *gK, double *gaxy,
-                             double *anu, double *anv, double *adu, double *adv, int h, int w,
-                             double ax, double ay)
+// Reverse-mode derivative of k_gsweep_fwd.  lam_out: adjoint of xout.
+// Writes, per cell, the adjoints of the new values (adu, adv) and of the
+// update numerators (anu = wu adu, anv = wv adv) into scratch (T, 4, plane),
+// adds the coefficient adjoints to gK and the per-cell partials of ax, ay to
+// gaxy (T, 2, plane).  Diagonals run from last to first: a cell's new values
+// are read only by its right and lower neighbours (diagonal d + 1) and by the
+// next sweep (lam_out), so the right neighbour's adjoint is the lane's own
+// previous result and the lower one lane r + 1's.  k_gsweep_lam_in then forms
+// the adjoint of xin from scratch.
+struct GBwdSlot {
+    double K[6], lu_out, lv_out, un, lu, lv, uu, uv, ou, ov, ru, rv, du, dv;
+};
+
+template <int D, int MAXW>
+__global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, const double *xin,
+                                                          const double *xout, const double *lam_out,
+                                                          double *gK, double *gaxy, double *scratch, GSkew k,
+                                                          double ax, double ay)
 {
-    const int t = blockIdx.x, r = threadIdx.x;
-    const size_t hw = (size_t)h * w;
-    const double *K = coef + (size_t)t * 6 * hw;
-    const double *ui = xin + (size_t)t * 2 * hw, *vi = ui + hw;
-    const double *uo = xout + (size_t)t * 2 * hw, *vo = uo + hw;
-    const double *lu_out = lam_out + (size_t)t * 2 * hw, *lv_out = lu_out + hw;
-    double *Anu = anu + (size_t)t * hw, *Anv = anv + (size_t)t * hw;
-    double *Adu = adu + (size_t)t * hw, *Adv = adv + (size_t)t * hw;
-    double *G = gK + (size_t)t * 6 * hw, *Gxy = gaxy + (size_t)t * 2 * hw;
-    for (int d = h + w - 2; d >= 0; --d) {
-        const int c = d - r;
-        if (r < h && c >= 0 && c < w) {
-            const size_t p = (size_t)r * w + c;
-            const double A12 = K[p], Bu = K[hw + p], Bv = K[2 * hw + p], wu = K[3 * hw + p],
-                         wv = K[4 * hw + p], b = K[5 * hw + p];
-            const bool hr = c < w - 1, hd = r < h - 1, hl = c > 0, hu = r > 0;
-            const double a_dv = lv_out[p] + (hr ? ax * Anv[p + 1] : 0.0) + (hd ? ay * Anv[p + w] : 0.0);
-            const double a_nv = wv * a_dv;
-            const double a_du = lu_out[p] + (hr ? ax * Anu[p + 1] : 0.0) + (hd ? ay * Anu[p + w] : 0.0) -
-                                A12 * a_nv;
-            const double a_nu = wu * a_du;
-            Anu[p] = a_nu;
-            Anv[p] = a_nv;
-            Adu[p] = a_du;
-            Adv[p] = a_dv;
-            // forward intermediates
-            const double lu = hl ? uo[p - 1] : 0.0, uu = hu ? uo[p - w] : 0.0;
-            const double ru = hr ? ui[p + 1] : 0.0, du = hd ? ui[p + w] : 0.0;
-            const double lv = hl ? vo[p - 1] : 0.0, uv = hu ? vo[p - w] : 0.0;
-            const double rv = hr ? vi[p + 1] : 0.0, dv = hd ? vi[p + w] : 0.0;
-            const double ou = ui[p], ov = vi[p], un = uo[p];
-            const double nu = Bu + ax * (lu + ru) + ay * (du + uu) - A12 * ov;
-            const double nv = Bv + ax * (lv + rv) + ay * (dv + uv) - A12 * un;
-            G[p] += -ov * a_nu - un * a_nv;                 // A12
-            G[hw + p] += a_nu;                             // Bu
-            G[2 * hw + p] += a_nv;                         // Bv
-            G[3 * hw + p] += (nu + b * ou) * a_du;         // wu
-            G[4 * hw + p] += (nv + b * ov) * a_dv;         // wv
-            G[5 * hw + p] += wu * ou * a_du + wv * ov * a_dv;  // b
-            Gxy[p] += (lu + ru) * a_nu + (lv + rv) * a_nv;
-            Gxy[hw + p] += (du + uu) * a_nu + (dv + uv) * a_nv;
+    __shared__ double xb[2][MAXW][2];
+    const int t = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = r >> 6, nw = (int)(blockDim.x >> 6);
+    const int h = k.h, w = k.w, hs = k.hs;
+    const bool row_ok = r < h, down_ok = r + 1 < h, up_ok = r >= 1 && r - 1 < h;
+    const unsigned pb = (unsigned)(k.plane * 8);
+    const auto rK = make_rsrc(coef + (size_t)t * 6 * k.plane, 6 * pb);
+    const auto rI = make_rsrc(xin + (size_t)t * 2 * k.plane, 2 * pb);
+    const auto rO = make_rsrc(xout + (size_t)t * 2 * k.plane, 2 * pb);
+    const auto rL = make_rsrc(lam_out + (size_t)t * 2 * k.plane, 2 * pb);
+    const auto rG = make_rsrc(gK + (size_t)t * 6 * k.plane, 6 * pb);
+    const auto rX = make_rsrc(gaxy + (size_t)t * 2 * k.plane, 2 * pb);
+    const auto rS = make_rsrc(scratch + (size_t)t * 4 * k.plane, 4 * pb);
+    static_assert(D >= 2 && D % 2 == 0, "the gK ring alternates with the step parity");
+    GBwdSlot S[D];
+    double Gc[2][8];  // gK (6) and gaxy (2) of the next step's cell, loaded one step ahead
+    auto issue = [&](GBwdSlot &s, int d) {
+        const unsigned q = slot_offset(row_ok, d, r, w, hs);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) s.K[e] = buf_load_f64(rK, q, e * pb);
+        s.lu_out = buf_load_f64(rL, q, 0);
+        s.lv_out = buf_load_f64(rL, q, pb);
+        s.un = buf_load_f64(rO, q, 0);
+        s.ou = buf_load_f64(rI, q, 0);
+        s.ov = buf_load_f64(rI, q, pb);
+        const unsigned ql = slot_offset(row_ok, d - 1, r, w, hs);        // (r, c - 1), new
+        s.lu = buf_load_f64(rO, ql, 0);
+        s.lv = buf_load_f64(rO, ql, pb);
+        const unsigned qu = slot_offset(up_ok, d - 1, r - 1, w, hs);     // (r - 1, c), new
+        s.uu = buf_load_f64(rO, qu, 0);
+        s.uv = buf_load_f64(rO, qu, pb);
+        const unsigned qr = slot_offset(row_ok, d + 1, r, w, hs);        // (r, c + 1), old
+        s.ru = buf_load_f64(rI, qr, 0);
+        s.rv = buf_load_f64(rI, qr, pb);
+        const unsigned qd = slot_offset(down_ok, d + 1, r + 1, w, hs);   // (r + 1, c), old
+        s.du = buf_load_f64(rI, qd, 0);
+        s.dv = buf_load_f64(rI, qd, pb);
+    };
+    auto issue_g = [&](double *g, int d) {
+        const unsigned q = slot_offset(row_ok, d, r, w, hs);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) g[e] = buf_load_f64(rG, q, e * pb);
+        g[6] = buf_load_f64(rX, q, 0);
+        g[7] = buf_load_f64(rX, q, pb);
+    };
+    const int nd = h + w - 1;
+    // step i handles diagonal nd - 1 - i
+#pragma unroll
+    for (int s = 0; s < D - 1; ++s) issue(S[s], nd - 1 - s);
+    issue_g(Gc[0], nd - 1);
+    double pnu = 0.0, pnv = 0.0;  // this lane's anu, anv of the previous step
+    for (int i0 = 0; i0 < nd; i0 += D) {
+#pragma unroll
+        for (int s = 0; s < D; ++s) {
+            const int i = i0 + s, d = nd - 1 - i;
+            issue_g(Gc[(s + 1) & 1], d - 1);  // D is even: the parity of s is the parity of i
+            issue(S[(s + D - 1) % D], d - (D - 1));
+            const GBwdSlot &c = S[s];
+            double dnu = wave_from_next_lane(pnu), dnv = wave_from_next_lane(pnv);
+            const int wd_src = wv + 1 < nw ? wv + 1 : wv;
+            const double eu = xb[(i + 1) & 1][wd_src][0], ev = xb[(i + 1) & 1][wd_src][1];
+            if (lane == 63) {
+                dnu = wv + 1 < nw ? eu : 0.0;
+                dnv = wv + 1 < nw ? ev : 0.0;
+            }
+            const unsigned q = slot_offset(row_ok, d, r, w, hs);
+            const bool ok = q != kOutOfRange;
+            const double A12 = c.K[0], Bu = c.K[1], Bv = c.K[2], wu = c.K[3], wvv = c.K[4], b = c.K[5];
+            double a_dv = c.lv_out + ax * pnv + ay * dnv;
+            double a_nv = wvv * a_dv;
+            double a_du = c.lu_out + ax * pnu + ay * dnu - A12 * a_nv;
+            double a_nu = wu * a_du;
+            a_dv = ok ? a_dv : 0.0;
+            a_nv = ok ? a_nv : 0.0;
+            a_du = ok ? a_du : 0.0;
+            a_nu = ok ? a_nu : 0.0;
+            const double nu = Bu + ax * (c.lu + c.ru) + ay * (c.du + c.uu) - A12 * c.ov;
+            const double nv = Bv + ax * (c.lv + c.rv) + ay * (c.dv + c.uv) - A12 * c.un;
+            const double *g = Gc[s & 1];
+            buf_store_f64(g[0] - c.ov * a_nu - c.un * a_nv, rG, q, 0);         // A12
+            buf_store_f64(g[1] + a_nu, rG, q, pb);                              // Bu
+            buf_store_f64(g[2] + a_nv, rG, q, 2 * pb);                          // Bv
+            buf_store_f64(g[3] + (nu + b * c.ou) * a_du, rG, q, 3 * pb);        // wu
+            buf_store_f64(g[4] + (nv + b * c.ov) * a_dv, rG, q, 4 * pb);        // wv
+            buf_store_f64(g[5] + wu * c.ou * a_du + wvv * c.ov * a_dv, rG, q, 5 * pb);  // b
+            buf_store_f64(g[6] + (c.lu + c.ru) * a_nu + (c.lv + c.rv) * a_nv, rX, q, 0);
+            buf_store_f64(g[7] + (c.du + c.uu) * a_nu + (c.dv + c.uv) * a_nv, rX, q, pb);
+            buf_store_f64(a_nu, rS, q, 0);
+            buf_store_f64(a_nv, rS, q, pb);
+            buf_store_f64(a_du, rS, q, 2 * pb);
+            buf_store_f64(a_dv, rS, q, 3 * pb);
+            pnu = a_nu;
+            pnv = a_nv;
+            if (lane == 0) {
+                xb[i & 1][wv][0] = a_nu;
+                xb[i & 1][wv][1] = a_nv;
+            }
+            lds_barrier();
         }
-        __syncthreads();
     }
-    // adjoint of the sweep's input: own old value, and the old values the
-    // left and upper neighbours read as their right / lower neighbour
-    double *li_u = lam_in + (size_t)t * 2 * hw, *li_v = li_u + hw;
-    for (int c = 0; c < w && r < h; ++c) {
-        const size_t p = (size_t)r * w + c;
-        const double A12 = K[p], wu = K[3 * hw + p], wv = K[4 * hw + p], b = K[5 * hw + p];
-        li_u[p] = (1.0 - kOmega + wu * b) * Adu[p] + (c > 0 ? ax * Anu[p - 1] : 0.0) +
-                  (r > 0 ? ay * Anu[p - w] : 0.0);
-        li_v[p] = (1.0 - kOmega + wv * b) * Adv[p] + (c > 0 ? ax * Anv[p - 1] : 0.0) +
-                  (r > 0 ? ay * Anv[p - w] : 0.0) - A12 * Anu[p];
-    }
+}
+
+// Adjoint of the sweep's input from the scratch of k_gsweep_bwd: each old
+// value is read by its own update and, as a right / lower neighbour, by the
+// updates of the cells to its left and above.
+__global__ __launch_bounds__(64) void k_gsweep_lam_in(const double *coef, const double *scratch, double *lam_in,
+                                                       GSkew k, double ax, double ay)
+{
+    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.y;
+    if (q >= k.plane) return;
+    int r, c;
+    if (!skew_cell(k, q, r, c)) return;
+    const size_t P = k.plane;
+    const double *K = coef + (size_t)t * 6 * P, *S = scratch + (size_t)t * 4 * P;
+    const double *Anu = S, *Anv = S + P, *Adu = S + 2 * P, *Adv = S + 3 * P;
+    const double A12 = K[q], wu = K[3 * P + q], wv = K[4 * P + q], b = K[5 * P + q];
+    const size_t ql = q - k.hs, qu = q - k.hs - 1;  // (r, c - 1), (r - 1, c)
+    double *L = lam_in + (size_t)t * 2 * P;
+    L[q] = (1.0 - kOmega + wu * b) * Adu[q] + (c > 0 ? ax * Anu[ql] : 0.0) + (r > 0 ? ay * Anu[qu] : 0.0);
+    L[P + q] = (1.0 - kOmega + wv * b) * Adv[q] + (c > 0 ? ax * Anv[ql] : 0.0) + (r > 0 ? ay * Anv[qu] : 0.0) -
+               A12 * Anu[q];
 }
 
 // --------------------------------------------------------------- median
@@ -664,23 +838,52 @@ int frh_grad_tensor(const double *fixed, const double *moving, int T, int C, int
     return FRH_OK;
 }
 
+static bool make_gskew(int h, int w, GSkew &k)
+{
+    if (h <= 0 || w <= 0 || h > 1024 || w > 65535) return false;
+    k.h = h;
+    k.w = w;
+    k.hs = 64 * ((h + 63) / 64);
+    k.plane = (size_t)(h + w - 1) * (size_t)k.hs;
+    return 6 * k.plane * 8 < (size_t)0x7fffffff;  // one frame's coefficients within a buffer resource
+}
+
+int frh_grad_skew_plane(int h, int w, size_t *plane)
+{
+    GSkew k;
+    FRH_GCHECK(plane && make_gskew(h, w, k), "frh_grad_skew_plane: bad level size (h <= 1024)");
+    *plane = k.plane;
+    return FRH_OK;
+}
+
+int frh_grad_skew(const double *src, int N, int h, int w, double *dst, int to_skew, void *stream)
+{
+    GSkew k;
+    FRH_GCHECK(src && dst && N > 0 && N <= 65535 && make_gskew(h, w, k), "frh_grad_skew: bad arguments");
+    hipLaunchKernelGGL(k_gskew, dim3((unsigned)((k.plane + 255) / 256), N), dim3(256), 0, (hipStream_t)stream,
+                       src, dst, N, k, to_skew);
+    FRH_GLAUNCH("skew conversion");
+    return FRH_OK;
+}
+
 int frh_grad_coef(const double *x, const double *u0, const double *J, const double *W, int T, int C,
                   int h, int w, double ax, double ay, const double *a_data, int psi_one, double *coef,
                   const double *g_coef, double *g_x, double *g_u0, double *g_J, double *g_par,
                   void *stream)
 {
-    FRH_GCHECK(x && J && W && a_data && T > 0 && C > 0 && C <= kMaxC && h > 0 && w > 0 &&
-                   T <= 65535 && h <= 65535, "frh_grad_coef: bad arguments (at most 2 channels)");
+    GSkew k;
+    FRH_GCHECK(x && J && W && a_data && T > 0 && C > 0 && C <= kMaxC && T <= 65535 && make_gskew(h, w, k),
+               "frh_grad_coef: bad arguments (at most 2 channels and 1024 rows)");
     FRH_GCHECK(!g_coef || (g_x && g_J && g_par), "frh_grad_coef: g_x, g_J and g_par are required with g_coef");
     hipStream_t st = (hipStream_t)stream;
-    dim3 g((w + 63) / 64, h, T), b(64);
+    dim3 g((unsigned)((k.plane + 63) / 64), T), b(64);
     if (coef) {
-        hipLaunchKernelGGL(k_gcoef_fwd, g, b, 0, st, x, u0, J, W, T, C, h, w, ax, ay, a_data, psi_one, coef);
+        hipLaunchKernelGGL(k_gcoef_fwd, g, b, 0, st, x, u0, J, W, T, C, k, ax, ay, a_data, psi_one, coef);
         FRH_GLAUNCH("coefficient forward");
     }
     if (g_coef) {
-        hipLaunchKernelGGL(k_gcoef_bwd, g, b, 0, st, x, u0, J, W, g_coef, T, C, h, w, ax, ay, a_data,
-                           psi_one, g_x, g_u0, g_J, g_par);
+        hipLaunchKernelGGL(k_gcoef_bwd, g, b, 0, st, x, u0, J, W, g_coef, T, C, k, ax, ay, a_data, psi_one,
+                           g_x, g_u0, g_J, g_par);
         FRH_GLAUNCH("coefficient backward");
     }
     return FRH_OK;
@@ -689,10 +892,17 @@ int frh_grad_coef(const double *x, const double *u0, const double *J, const doub
 int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T, int h, int w,
                    double ax, double ay, void *stream)
 {
-    FRH_GCHECK(coef && x_in && x_out && T > 0 && T <= 65535 && h > 0 && h <= 1024 && w > 0,
+    GSkew k;
+    FRH_GCHECK(coef && x_in && x_out && T > 0 && T <= 65535 && make_gskew(h, w, k),
                "frh_grad_sweep: bad arguments (at most 1024 rows)");
-    hipLaunchKernelGGL(k_gsweep_fwd, dim3(T), dim3(64 * ((h + 63) / 64)), 0, (hipStream_t)stream, coef,
-                       x_in, x_out, h, w, ax, ay);
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 b(k.hs);
+    if (k.hs <= 256)
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+    else if (k.hs <= 512)
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+    else
+        hipLaunchKernelGGL((k_gsweep_fwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
     FRH_GLAUNCH("sweep forward");
     return FRH_OK;
 }
@@ -701,14 +911,26 @@ int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double 
                            const double *lam_out, double *lam_in, double *g_coef, double *g_axy,
                            double *scratch, int T, int h, int w, double ax, double ay, void *stream)
 {
+    GSkew k;
     FRH_GCHECK(coef && x_in && x_out && lam_out && lam_in && g_coef && g_axy && scratch && T > 0 &&
-                   T <= 65535 && h > 0 && h <= 1024 && w > 0,
+                   T <= 65535 && make_gskew(h, w, k),
                "frh_grad_sweep_adjoint: bad arguments (at most 1024 rows)");
-    const size_t n = (size_t)T * h * w;
-    hipLaunchKernelGGL(k_gsweep_bwd, dim3(T), dim3(64 * ((h + 63) / 64)), 0, (hipStream_t)stream, coef,
-                       x_in, x_out, lam_out, lam_in, g_coef, g_axy, scratch, scratch + n, scratch + 2 * n,
-                       scratch + 3 * n, h, w, ax, ay);
+    FRH_GCHECK(lam_in != lam_out, "frh_grad_sweep_adjoint: lam_in must not alias lam_out");
+    hipStream_t st = (hipStream_t)stream;
+    const dim3 b(k.hs);
+    if (k.hs <= 256)
+        hipLaunchKernelGGL((k_gsweep_bwd<2, 4>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+                           scratch, k, ax, ay);
+    else if (k.hs <= 512)
+        hipLaunchKernelGGL((k_gsweep_bwd<2, 8>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+                           scratch, k, ax, ay);
+    else
+        hipLaunchKernelGGL((k_gsweep_bwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef,
+                           g_axy, scratch, k, ax, ay);
     FRH_GLAUNCH("sweep adjoint");
+    hipLaunchKernelGGL(k_gsweep_lam_in, dim3((unsigned)((k.plane + 63) / 64), T), dim3(64), 0, st, coef, scratch,
+                       lam_in, k, ax, ay);
+    FRH_GLAUNCH("sweep adjoint input");
     return FRH_OK;
 }
 
