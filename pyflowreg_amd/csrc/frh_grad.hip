@@ -19,6 +19,7 @@
 #include "../../include/flowreg_hip.h"
 #include "frh_internal.h"
 #include "frh_device.h"
+#include "frh_median.h"
 
 namespace frh {
 namespace grad {
@@ -672,43 +673,42 @@ __device__ inline int mirror5(int j, int n)
     return j < n ? j : period - j;
 }
 
-__device__ inline double median25_sorted(double *v)
-{
-    for (int i = 1; i < 25; ++i) {  // insertion sort of 25 values
-        double x = v[i];
-        int j = i - 1;
-        while (j >= 0 && v[j] > x) {
-            v[j + 1] = v[j];
-            --j;
-        }
-        v[j + 1] = x;
-    }
-    return v[12];
-}
-
-__global__ __launch_bounds__(64) void k_gmedian(const double *x, int T, int h, int w, double *out, const double *gout,
-                          double *gx)
+__global__ __launch_bounds__(64) void k_gmedian(const double *x, int T, int h, int w, double *out,
+                                                 const double *gout, double *gx)
 {
     const int c = blockIdx.x * blockDim.x + threadIdx.x, r = blockIdx.y, tq = blockIdx.z;
     if (c >= w) return;
     const size_t hw = (size_t)h * w;
     const double *X = x + (size_t)tq * hw;
-    double v[25];
-    int src[25];
+    int rr[5], cc[5];
+#pragma unroll
+    for (int k = 0; k < 5; ++k) {
+        rr[k] = mirror5(r + k - 2, h);
+        cc[k] = mirror5(c + k - 2, w);
+    }
+    double v[25], s[25];
+#pragma unroll
     for (int i = 0; i < 5; ++i)
-        for (int j = 0; j < 5; ++j) {
-            const int rr = mirror5(r + i - 2, h), cc = mirror5(c + j - 2, w);
-            src[i * 5 + j] = rr * w + cc;
-            v[i * 5 + j] = X[rr * w + cc];
-        }
-    double s[25];
-    for (int i = 0; i < 25; ++i) s[i] = v[i];
-    const double m = median25_sorted(s);
+#pragma unroll
+        for (int j = 0; j < 5; ++j) v[i * 5 + j] = s[i * 5 + j] = X[(size_t)rr[i] * w + cc[j]];
+    const double m = median25(s);
     if (out) out[(size_t)tq * hw + (size_t)r * w + c] = m;
     if (gx) {
-        int k = 0;
-        while (k < 24 && v[k] != m) ++k;  // the first window sample equal to the median
-        atomicAdd(gx + (size_t)tq * hw + src[k], gout[(size_t)tq * hw + (size_t)r * w + c]);
+        // the first window sample equal to the median receives the gradient
+        int ki = 0, kj = 0;
+#pragma unroll
+        for (int k = 24; k >= 0; --k)
+            if (v[k] == m) {
+                ki = k / 5;
+                kj = k % 5;
+            }
+        int sr = rr[0], sc = cc[0];
+#pragma unroll
+        for (int k = 1; k < 5; ++k) {
+            sr = ki == k ? rr[k] : sr;
+            sc = kj == k ? cc[k] : sc;
+        }
+        atomicAdd(gx + (size_t)tq * hw + (size_t)sr * w + sc, gout[(size_t)tq * hw + (size_t)r * w + c]);
     }
 }
 
@@ -919,10 +919,10 @@ int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double 
     hipStream_t st = (hipStream_t)stream;
     const dim3 b(k.hs);
     if (k.hs <= 256)
-        hipLaunchKernelGGL((k_gsweep_bwd<2, 4>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+        hipLaunchKernelGGL((k_gsweep_bwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
                            scratch, k, ax, ay);
     else if (k.hs <= 512)
-        hipLaunchKernelGGL((k_gsweep_bwd<2, 8>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+        hipLaunchKernelGGL((k_gsweep_bwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
                            scratch, k, ax, ay);
     else
         hipLaunchKernelGGL((k_gsweep_bwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef,
