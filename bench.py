@@ -248,6 +248,7 @@ def main():
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--channels", type=int, default=2)
     ap.add_argument("--plain-frames", type=int, default=3, help="0 skips the plain path")
+    ap.add_argument("--graph", action="store_true", help="replay the batched level loop as a hipGraph")
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -303,7 +304,7 @@ def main():
     raw_ref = frames[0].double()
     batch = frames[1:].contiguous()
     ref_hwc = preprocess(raw_ref)[0].permute(1, 2, 0).contiguous()
-    solver = fr.DisplacementSolver(H, W, C, max_batch=T)
+    solver = fr.DisplacementSolver(H, W, C, max_batch=T, graph=args.graph)
     solver.set_reference(ref_hwc, [0.5] * C)
 
     def step():
@@ -348,18 +349,24 @@ def main():
                   "rms_px": float(np.sqrt((d ** 2).mean())), "reference": "FP64 CPU oracle"}
     epe = float(torch.linalg.vector_norm(w64.cpu() - truth_h[1:], dim=-1).mean())
 
-    plain_fps = None
+    plain_fps = plain_fps_eager = None
     if args.plain_frames > 0:
-        plain = fr.DisplacementSolver(H, W, C, max_batch=1)
-        plain.set_reference(ref_hwc, [0.5] * C)
+        # one frame per call; the level loop replayed as a hipGraph (and,
+        # for comparison, launched eagerly)
         pp = preprocess(batch[: args.plain_frames], normalization_ref=raw_ref).permute(0, 2, 3, 1)
-        plain.run(pp[0], out_dtype=torch.float32)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for k in range(args.plain_frames):
-            plain.run(pp[k], out_dtype=torch.float32)
-        torch.cuda.synchronize()
-        plain_fps = args.plain_frames / (time.perf_counter() - t1)
+        rates = {}
+        for use_graph in (True, False):
+            plain = fr.DisplacementSolver(H, W, C, max_batch=1, graph=use_graph)
+            plain.set_reference(ref_hwc, [0.5] * C)
+            plain.run(pp[0], out_dtype=torch.float32)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for k in range(args.plain_frames):
+                plain.run(pp[k], out_dtype=torch.float32)
+            torch.cuda.synchronize()
+            rates[use_graph] = args.plain_frames / (time.perf_counter() - t1)
+            plain.close()
+        plain_fps, plain_fps_eager = rates[True], rates[False]
 
     out = {
         "metric": METRIC,
@@ -378,6 +385,8 @@ def main():
                                "alpha=2, 50 iterations, update_lag 5, eta 0.8, a_data 0.45",
                    "frames_per_step": T, "H": H, "W": W, "channels": C},
         "plain_fps": round(plain_fps, 3) if plain_fps else None,
+        "plain_fps_eager": round(plain_fps_eager, 3) if plain_fps_eager else None,
+        "batched_graph": bool(args.graph),
         "batched_over_plain": round(fps / plain_fps, 2) if plain_fps else None,
         "parity": parity,
         "epe_vs_truth_px": round(epe, 4),

@@ -197,6 +197,15 @@ int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving,
  * + flow), 5 = warp, 6 = median/add.  Requires frh_solver_profile(s, 1)
  * before the run (it synchronizes the stream); returns the count written. */
 int frh_solver_profile(frh_solver *s, int enable);
+
+/* enable != 0: frh_solver_run / frh_solver_run_pairs replay the level loop
+ * as a hipGraph captured on the first call with a given (T, out_f64,
+ * per-frame references), with the frames and flows copied through
+ * solver-owned staging buffers (device-to-device, on the caller's stream).
+ * Results equal the eager launches bit for bit; profiling runs eagerly.
+ * No counterpart in the reference (its solvers are CPU or per-call GPU
+ * code); it replaces per-kernel launch overhead for small batches. */
+int frh_solver_set_graph(frh_solver *s, int enable);
 int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n);
 
 /* ------------------------------------------- differentiable registration */

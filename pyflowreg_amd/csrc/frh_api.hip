@@ -385,6 +385,15 @@ struct frh_solver {
     size_t pool_used = 0;
     double tm[TM_N] = {0};
     int counts[TM_N] = {0};
+    // graph replay of the level loop (frh_solver_set_graph): captured once per
+    // (T, output type, per-frame references) on a solver-owned stream, with
+    // the caller's frames and output staged through solver-owned buffers
+    bool use_graph = false;
+    hipStream_t cap_stream = nullptr;
+    hipGraphExec_t gexec = nullptr;
+    int g_T = 0, g_out_f64 = 0, g_pairs = 0;
+    float *stage_in = nullptr, *stage_fixed = nullptr;
+    void *stage_out = nullptr;
 };
 
 static void free_solver(frh_solver *s)
@@ -400,6 +409,9 @@ static void free_solver(frh_solver *s)
     (void)hipFree(s->pyr1); (void)hipFree(s->f1l);
     (void)hipFree(s->wlow); (void)hipFree(s->wtmp);
     for (auto e : s->pool) (void)hipEventDestroy(e);
+    if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+    if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
+    (void)hipFree(s->stage_in); (void)hipFree(s->stage_fixed); (void)hipFree(s->stage_out);
     delete s;
 }
 
@@ -440,6 +452,8 @@ struct Scope {
 // (fixed == nullptr) or against per-frame references fixed (T, C, H, W).
 static int solver_run(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
                       int out_f64, hipStream_t st);
+static int solver_dispatch(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
+                           int out_f64, hipStream_t st);
 
 extern "C" {
 
@@ -584,7 +598,7 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
     FRH_REQ(s && moving && w_out, "frh_solver_run: null pointer");
     FRH_REQ(s->have_ref && s->have_fixed, "frh_solver_run: frh_solver_set_reference has not set a reference");
     FRH_REQ(T >= 1 && T <= s->max_batch, "frh_solver_run: T=%d outside [1, %d]", T, s->max_batch);
-    return solver_run(s, nullptr, moving, T, w_out, out_f64, as_stream(stream));
+    return solver_dispatch(s, nullptr, moving, T, w_out, out_f64, as_stream(stream));
 }
 
 int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
@@ -599,7 +613,7 @@ int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving,
         if (!rc) rc = dev_alloc(&s->f1l, T_ * C * (size_t)s->lv.back().h * s->lv.back().w, "reference levels");
         if (rc) return rc;
     }
-    return solver_run(s, fixed, moving, T, w_out, out_f64, as_stream(stream));
+    return solver_dispatch(s, fixed, moving, T, w_out, out_f64, as_stream(stream));
 }
 
 }  // extern "C"
@@ -731,7 +745,69 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
     return FRH_OK;
 }
 
+// Eager launches, or a replay of the captured level loop.  The graph's
+// kernels read and write the staging buffers, so one capture serves every
+// call with the same (T, output type, per-frame references); the level
+// images, reference and weights it uses live in the solver and keep their
+// addresses across frh_solver_set_reference calls.
+static int solver_dispatch(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
+                           int out_f64, hipStream_t st)
+{
+    if (!s->use_graph || s->profile) return solver_run(s, fixed, moving, T, w_out, out_f64, st);
+    const size_t in_elems = (size_t)s->max_batch * s->C * s->H * s->W;
+    const size_t n_in = (size_t)T * s->C * s->H * s->W;
+    const size_t out_bytes = (size_t)T * s->H * s->W * 2 * (out_f64 ? sizeof(double) : sizeof(float));
+    if (!s->stage_in) {
+        int rc = dev_alloc(&s->stage_in, in_elems, "graph input staging");
+        if (!rc) rc = dev_alloc((double **)&s->stage_out, (size_t)s->max_batch * s->H * s->W * 2,
+                                "graph output staging");
+        if (rc) return rc;
+    }
+    if (fixed && !s->stage_fixed) {
+        int rc = dev_alloc(&s->stage_fixed, in_elems, "graph reference staging");
+        if (rc) return rc;
+    }
+    const int pairs = fixed != nullptr;
+    if (!s->gexec || s->g_T != T || s->g_out_f64 != out_f64 || s->g_pairs != pairs) {
+        if (s->gexec) {
+            (void)hipGraphExecDestroy(s->gexec);
+            s->gexec = nullptr;
+        }
+        if (!s->cap_stream) FRH_HIP(hipStreamCreateWithFlags(&s->cap_stream, hipStreamNonBlocking), "capture stream");
+        FRH_HIP(hipStreamBeginCapture(s->cap_stream, hipStreamCaptureModeThreadLocal), "begin capture");
+        const int rc = solver_run(s, pairs ? s->stage_fixed : nullptr, s->stage_in, T, s->stage_out, out_f64,
+                                  s->cap_stream);
+        hipGraph_t g = nullptr;
+        const hipError_t e = hipStreamEndCapture(s->cap_stream, &g);
+        if (rc) {
+            if (g) (void)hipGraphDestroy(g);
+            return rc;
+        }
+        FRH_HIP(e, "end capture");
+        const hipError_t ei = hipGraphInstantiate(&s->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        FRH_HIP(ei, "graph instantiate");
+        s->g_T = T;
+        s->g_out_f64 = out_f64;
+        s->g_pairs = pairs;
+    }
+    FRH_HIP(hipMemcpyAsync(s->stage_in, moving, sizeof(float) * n_in, hipMemcpyDeviceToDevice, st), "stage frames");
+    if (pairs)
+        FRH_HIP(hipMemcpyAsync(s->stage_fixed, fixed, sizeof(float) * n_in, hipMemcpyDeviceToDevice, st),
+                "stage references");
+    FRH_HIP(hipGraphLaunch(s->gexec, st), "graph launch");
+    FRH_HIP(hipMemcpyAsync(w_out, s->stage_out, out_bytes, hipMemcpyDeviceToDevice, st), "unstage flows");
+    return FRH_OK;
+}
+
 extern "C" {
+
+int frh_solver_set_graph(frh_solver *s, int enable)
+{
+    FRH_REQ(s, "frh_solver_set_graph: null solver");
+    s->use_graph = enable != 0;
+    return FRH_OK;
+}
 
 int frh_solver_profile(frh_solver *s, int enable)
 {

@@ -224,7 +224,7 @@ class DisplacementSolver:
 
     def __init__(self, H, W, C, *, alpha=(2, 2), update_lag=5, iterations=50, min_level=0,
                  levels=50, eta=0.8, a_smooth=1.0, a_data=0.45, max_batch=1,
-                 const_assumption="gc"):
+                 const_assumption="gc", graph=False):
         if np.isscalar(alpha):
             alpha = (float(alpha), float(alpha))
         alpha = tuple(float(a) for a in alpha)
@@ -265,6 +265,13 @@ class DisplacementSolver:
               "frh_solver_create")
         self._h = h
         self._ref = None
+        if graph:
+            self.set_graph(True)
+
+    def set_graph(self, enable=True):
+        """Replay the level loop as a hipGraph (captured on the first run with
+        a given batch size and output dtype; bit-identical to eager runs)."""
+        check(lib().frh_solver_set_graph(self._h, int(bool(enable))), "frh_solver_set_graph")
 
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
