@@ -68,6 +68,15 @@ int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const doubl
                       const double *den, const double *wy, int ry, const double *wx, int rx,
                       float *out, void *stream);
 
+/* frh_prefilter / frh_prefilter_f64 with the float64 result kept
+ * (out_f64 = 1, out float64): the preprocessed frames before the executors'
+ * float32 cast, as update_reference (compensate_recording.py:680-713) and
+ * FlowRegLive (flow_reg_live.py:216-323) use them.  raw is float64 when
+ * raw_f64 != 0, else float32. */
+int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
+                     const double *den, const double *wy, int ry, const double *wx, int rx,
+                     void *out, int out_f64, void *stream);
+
 /* util/resize_util.py:116-141,232-254 (imresize_fused_gauss_cubic).
  * src: (N, h, w) planes; dst: (N, oh, ow); tmp: (N, h, ow) scratch.
  * ix, wx: (ow, P) and iy, wy: (oh, P) device tables from
@@ -76,21 +85,29 @@ int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow
                const int32_t *ix, const float *wx, const int32_t *iy, const float *wy, int P,
                float *tmp, void *stream);
 
-/* core/warping.py:160-234 (imregister_wrapper, cubic).  f2: (T, C, H, W)
- * planes to warp; f1: (C, H, W) fill for out-of-bounds pixels, or
- * (T, C, H, W) when f1_per_frame != 0.  u, v: (T, H, W) float32
- * displacements, divided by h_col / h_row (float64) before use.
- * out: (T, C, H, W).  OpenCV INTER_CUBIC + BORDER_REPLICATE semantics with
- * 1/32-px map quantization. */
+/* core/warping.py:160-234 (imregister_wrapper, cubic) of float32 images
+ * (cv2.remap's 32F path: float sums).  f2: (T, C, H, W) planes to warp; f1:
+ * (C, H, W) fill for out-of-bounds pixels, or (T, C, H, W) when
+ * f1_per_frame != 0.  u, v: (T, H, W) float32 displacements, divided by
+ * h_col / h_row (float64) before use.  out: (T, C, H, W).  OpenCV
+ * INTER_CUBIC + BORDER_REPLICATE semantics with 1/32-px map quantization. */
 int frh_warp_cubic(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
                    int W, const float *u, const float *v, double h_col, double h_row,
                    float *out, void *stream);
 
-/* Same warp for float64 displacements already divided by the grid spacing
- * (the reference's imregister_wrapper(f2, u / h_col, v / h_row, f1) call
- * with u of any dtype): the map is float32(x + u) formed in float64. */
-int frh_warp_cubic_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
-                       int W, const double *u, const double *v, float *out, void *stream);
+/* The same warp for float64 displacements already divided by the grid
+ * spacing (the reference's imregister_wrapper(f2, u / h_col, v / h_row, f1)
+ * with u of any dtype: the map is float32(x + u) formed in float64), with the
+ * image type the reference passes: float64 images take cv2.remap's 64F path
+ * (float weights, double sums; acc_f64 = 1) -- the level images of
+ * get_displacement are float64 -- and float32 images its 32F path
+ * (acc_f64 = 0).  f2, f1 are float64 when img_f64 != 0 (then acc_f64 must be
+ * 1), else float32 (float64 values exactly representable in float32 may be
+ * passed as float32 with acc_f64 = 1).  The result and the out-of-bounds fill
+ * are rounded to float32, as the reference's float32 output array does. */
+int frh_warp_cubic_ex(const void *f2, const void *f1, int img_f64, int acc_f64, int f1_per_frame,
+                      int T, int C, int H, int W, const double *u, const double *v, float *out,
+                      void *stream);
 
 /* Data terms (core/optical_flow.py:446-473, _resolve_motion_tensor_func). */
 #define FRH_TENSOR_GC 0   /* "gc" / "gradient": get_motion_tensor_gc, :73-160 */

@@ -24,9 +24,10 @@
  *   or_fill_resize_tables   linear-ramp reproduction, partition of unity,
  *                           identity at scale 1
  *   or_resize_tab           scale-1 identity, ramp reproduction, constants
- *   or_warp_cubic           reference warp tests (zero / integer shifts,
+ *   or_warp_cubic(64)       reference warp tests (zero / integer shifts,
  *                           out-of-bounds <- fixed), ramp reproduction with
- *                           1/32-px quantization
+ *                           1/32-px quantization; the 64F sums against a
+ *                           float128 evaluation of the same taps
  *   or_motion_tensor_gc     reference known answer J11=0.8, identical frames
  *                           give J33=J13=J23=0, quadratic closed forms
  *   or_compute_flow         f1=f2 gives zero flow; a_data=1 converges to the
@@ -176,14 +177,55 @@ static void cv_cubic_coeffs(float x, float *c)
  * cv2.remap (opencv-contrib-python>=4.10, third party, restated from its
  * public behaviour; parity with OpenCV itself is unpinned): float maps are
  * rounded to 1/32 px (cvRound, half-to-even), the 4x4 weights are products
- * of two float 1-D tables, sums are float.  Interior pixels sum row by row,
- * border pixels (BORDER_REPLICATE) sum the 16 taps sequentially.
+ * of two float 1-D tables.  The sums follow the image depth: float for
+ * float32 images (or_warp_cubic, OpenCV's 32F path), double for float64
+ * images (or_warp_cubic64, its 64F path: float weights, double products and
+ * sums); get_displacement's level images are float64, so its level warps
+ * take the 64F path.  Interior pixels sum row by row, border pixels
+ * (BORDER_REPLICATE) sum the 16 taps sequentially.  The reference stores the
+ * result, and the OOB fill, in a float32 array.
  * u, v are the per-level displacements already divided by the grid spacing
  * (u / h_col, v / h_row in the reference), as float64. */
 static int g_warp_quantize = 1;
 /* Study switch: 0 evaluates the bicubic weights at the exact fractional
  * position instead of OpenCV's 1/32-px grid (never used for parity). */
 void or_set_warp_quantize(int q) { g_warp_quantize = q; }
+
+/* Map of pixel (x, y): 0 when out of bounds, else 1 with the tap origin
+ * (sx, sy), the 16 weights and whether all taps are inside (interior). */
+static int warp_taps(int x, int y, int H, int W, double u, double v, float tab[32][4], int *sx, int *sy,
+                     float w[16], int *interior)
+{
+    float mx = (float)((double)x + u);
+    float my = (float)((double)y + v);
+    if ((mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H)) return 0;
+    float cx = mx < 0.f ? 0.f : (mx > (float)(W - 1) ? (float)(W - 1) : mx);
+    float cy = my < 0.f ? 0.f : (my > (float)(H - 1) ? (float)(H - 1) : my);
+    int X = (int)lrintf(cx * 32.f), Y = (int)lrintf(cy * 32.f);
+    *sx = (X >> 5) - 1;
+    *sy = (Y >> 5) - 1;
+    const float *wxv = tab[X & 31], *wyv = tab[Y & 31];
+    float cxq[4], cyq[4];
+    if (!g_warp_quantize) {
+        *sx = (int)floorf(cx) - 1; *sy = (int)floorf(cy) - 1;
+        cv_cubic_coeffs(cx - floorf(cx), cxq); cv_cubic_coeffs(cy - floorf(cy), cyq);
+        wxv = cxq; wyv = cyq;
+    }
+    for (int i = 0; i < 4; ++i)
+        for (int j = 0; j < 4; ++j) w[i * 4 + j] = wyv[i] * wxv[j];
+    int width1 = W - 3 > 0 ? W - 3 : 0, height1 = H - 3 > 0 ? H - 3 : 0;
+    *interior = (unsigned)*sx < (unsigned)width1 && (unsigned)*sy < (unsigned)height1;
+    return 1;
+}
+
+static void border_taps(int sx, int sy, int H, int W, int xs[4], int ys[4])
+{
+    for (int i = 0; i < 4; ++i) {
+        int a = sx + i, b = sy + i;
+        xs[i] = a < 0 ? 0 : (a >= W ? W - 1 : a);
+        ys[i] = b < 0 ? 0 : (b >= H ? H - 1 : b);
+    }
+}
 
 void or_warp_cubic(const float *f2, const float *f1, int H, int W, int C,
                    const double *u, const double *v, float *out)
@@ -193,31 +235,15 @@ void or_warp_cubic(const float *f2, const float *f1, int H, int W, int C,
     for (int y = 0; y < H; ++y)
         for (int x = 0; x < W; ++x) {
             size_t p = (size_t)y * W + x;
-            float mx = (float)((double)x + u[p]);
-            float my = (float)((double)y + v[p]);
-            int oob = (mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H);
-            if (oob) {
+            int sx, sy, interior;
+            float w[16];
+            if (!warp_taps(x, y, H, W, u[p], v[p], tab, &sx, &sy, w, &interior)) {
                 for (int c = 0; c < C; ++c) out[p * C + c] = f1[p * C + c];
                 continue;
             }
-            float cx = mx < 0.f ? 0.f : (mx > (float)(W - 1) ? (float)(W - 1) : mx);
-            float cy = my < 0.f ? 0.f : (my > (float)(H - 1) ? (float)(H - 1) : my);
-            int X = (int)lrintf(cx * 32.f), Y = (int)lrintf(cy * 32.f);
-            int sx = (X >> 5) - 1, sy = (Y >> 5) - 1;
-            const float *wxv = tab[X & 31], *wyv = tab[Y & 31];
-            float cxq[4], cyq[4];
-            if (!g_warp_quantize) {
-                sx = (int)floorf(cx) - 1; sy = (int)floorf(cy) - 1;
-                cv_cubic_coeffs(cx - floorf(cx), cxq); cv_cubic_coeffs(cy - floorf(cy), cyq);
-                wxv = cxq; wyv = cyq;
-            }
-            float w[16];
-            for (int i = 0; i < 4; ++i)
-                for (int j = 0; j < 4; ++j) w[i * 4 + j] = wyv[i] * wxv[j];
-            int width1 = W - 3 > 0 ? W - 3 : 0, height1 = H - 3 > 0 ? H - 3 : 0;
             for (int c = 0; c < C; ++c) {
                 float sum;
-                if ((unsigned)sx < (unsigned)width1 && (unsigned)sy < (unsigned)height1) {
+                if (interior) {
                     const float *S = f2 + ((size_t)sy * W + sx) * C + c;
                     size_t rs = (size_t)W * C;
                     sum = S[0] * w[0] + S[C] * w[1] + S[2 * C] * w[2] + S[3 * C] * w[3];
@@ -229,17 +255,56 @@ void or_warp_cubic(const float *f2, const float *f1, int H, int W, int C,
                     sum += S[0] * w[12] + S[C] * w[13] + S[2 * C] * w[14] + S[3 * C] * w[15];
                 } else {
                     int xs[4], ys[4];
-                    for (int i = 0; i < 4; ++i) {
-                        int a = sx + i, b = sy + i;
-                        xs[i] = a < 0 ? 0 : (a >= W ? W - 1 : a);
-                        ys[i] = b < 0 ? 0 : (b >= H ? H - 1 : b);
-                    }
+                    border_taps(sx, sy, H, W, xs, ys);
                     sum = 0.f;
                     for (int i = 0; i < 4; ++i)
                         for (int j = 0; j < 4; ++j)
                             sum += f2[((size_t)ys[i] * W + xs[j]) * C + c] * w[i * 4 + j];
                 }
                 out[p * C + c] = sum;
+            }
+        }
+}
+
+void or_warp_cubic64(const double *f2, const double *f1, int H, int W, int C,
+                     const double *u, const double *v, float *out)
+{
+    float tab[32][4];
+    for (int k = 0; k < 32; ++k) cv_cubic_coeffs(k * (1.f / 32), tab[k]);
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t p = (size_t)y * W + x;
+            int sx, sy, interior;
+            float w[16];
+            if (!warp_taps(x, y, H, W, u[p], v[p], tab, &sx, &sy, w, &interior)) {
+                for (int c = 0; c < C; ++c) out[p * C + c] = (float)f1[p * C + c];
+                continue;
+            }
+            for (int c = 0; c < C; ++c) {
+                double sum;
+                if (interior) {
+                    const double *S = f2 + ((size_t)sy * W + sx) * C + c;
+                    size_t rs = (size_t)W * C;
+                    sum = S[0] * (double)w[0] + S[C] * (double)w[1] + S[2 * C] * (double)w[2] +
+                          S[3 * C] * (double)w[3];
+                    S += rs;
+                    sum += S[0] * (double)w[4] + S[C] * (double)w[5] + S[2 * C] * (double)w[6] +
+                           S[3 * C] * (double)w[7];
+                    S += rs;
+                    sum += S[0] * (double)w[8] + S[C] * (double)w[9] + S[2 * C] * (double)w[10] +
+                           S[3 * C] * (double)w[11];
+                    S += rs;
+                    sum += S[0] * (double)w[12] + S[C] * (double)w[13] + S[2 * C] * (double)w[14] +
+                           S[3 * C] * (double)w[15];
+                } else {
+                    int xs[4], ys[4];
+                    border_taps(sx, sy, H, W, xs, ys);
+                    sum = 0.0;
+                    for (int i = 0; i < 4; ++i)
+                        for (int j = 0; j < 4; ++j)
+                            sum += f2[((size_t)ys[i] * W + xs[j]) * C + c] * (double)w[i * 4 + j];
+                }
+                out[p * C + c] = (float)sum;
             }
         }
 }

@@ -64,6 +64,9 @@ def lib(precision: str = "f64") -> ctypes.CDLL:
     L.or_warp_cubic.argtypes = [_f32p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                 _f64p, _f64p, _f32p]
     L.or_warp_cubic.restype = None
+    L.or_warp_cubic64.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                  _f64p, _f64p, _f32p]
+    L.or_warp_cubic64.restype = None
     tens = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double]
     L.or_motion_tensor_gc.argtypes = tens + [ctypes.c_int, ctypes.c_int] + [rp] * 6
@@ -152,14 +155,20 @@ def resize(img, size, sigma_coeff: float = 0.6):
 
 # -------------------------------------------------------------------- warp
 def imregister_wrapper(f2_level, u, v, f1_level):
-    """core/warping.py:160-234, cubic interpolation (cv2.remap restated)."""
+    """core/warping.py:160-234, cubic interpolation (cv2.remap restated):
+    float64 images take OpenCV's 64F path (double sums), other images are
+    converted to float32 and take its 32F path (float sums); the result is
+    float32."""
     squeeze = f2_level.ndim == 2
-    f2 = np.ascontiguousarray(f2_level if not squeeze else f2_level[:, :, None], dtype=np.float32)
-    f1 = np.ascontiguousarray(f1_level if not squeeze else f1_level[:, :, None], dtype=np.float32)
+    f64 = np.asarray(f2_level).dtype == np.float64
+    dt = np.float64 if f64 else np.float32
+    f2 = np.ascontiguousarray(f2_level if not squeeze else f2_level[:, :, None], dtype=dt)
+    f1 = np.ascontiguousarray(f1_level if not squeeze else f1_level[:, :, None], dtype=dt)
     H, W, C = f2.shape
     out = np.empty((H, W, C), np.float32)
-    lib().or_warp_cubic(f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64),
-                        np.ascontiguousarray(v, dtype=np.float64), out)
+    fn = lib().or_warp_cubic64 if f64 else lib().or_warp_cubic
+    fn(f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64),
+       out)
     return out[:, :, 0] if squeeze else out
 
 
@@ -353,10 +362,9 @@ def _stage(fixed, moving, alpha, update_lag, iterations, min_level, levels, eta,
         else:
             u = np.pad(resize(u[1:-1, 1:-1], level_size), 1, mode="edge").astype(dt)
             v = np.pad(resize(v[1:-1, 1:-1], level_size), 1, mode="edge").astype(dt)
-            tmp = imregister_wrapper(f2_level.astype(np.float32),
-                                     u[1:-1, 1:-1].astype(np.float64) / h_col,
-                                     v[1:-1, 1:-1].astype(np.float64) / h_row,
-                                     f1_level.astype(np.float32))
+            # the level images are float64 (f32-valued): cv2.remap's 64F path
+            tmp = imregister_wrapper(f2_level, u[1:-1, 1:-1].astype(np.float64) / h_col,
+                                     v[1:-1, 1:-1].astype(np.float64) / h_row, f1_level)
         lh, lw = level_size
         J = _level_tensor(tensor_fn, f1_level, tmp, h_row, h_col, C, dt, precision)
         weight_level = _weight_level(weight, lh, lw, C, dt)
@@ -412,10 +420,8 @@ def _stage_gnc(fixed, moving, alpha, update_lag, iterations, min_level, levels, 
         alpha_scaling = 1 if i == min_level else eta ** (-0.5 * i)
         alpha_tmp = [alpha_scaling * alpha[j] for j in range(len(alpha))]
         for _ in range(warping_steps):
-            tmp = imregister_wrapper(f2_level.astype(np.float32),
-                                     (u[1:-1, 1:-1] / h_col).astype(np.float64),
-                                     (v[1:-1, 1:-1] / h_row).astype(np.float64),
-                                     f1_level.astype(np.float32))
+            tmp = imregister_wrapper(f2_level, (u[1:-1, 1:-1] / h_col).astype(np.float64),
+                                     (v[1:-1, 1:-1] / h_row).astype(np.float64), f1_level)
             J = _level_tensor(tensor_fn, f1_level, tmp, h_row, h_col, C, dt, precision)
             args = (J[0], J[1], J[2], J[3], J[4], J[5], weight_level, u.astype(dt), v.astype(dt),
                     alpha_tmp, iterations, update_lag, 0, a_data_arr, a_smooth, h_col, h_row)

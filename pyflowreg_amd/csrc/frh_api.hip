@@ -177,7 +177,7 @@ int frh_warping_depth(double eta, int levels, int m, int n)
 
 static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, const double *lo,
                           const double *den, const double *wy, int ry, const double *wx, int rx,
-                          float *out, void *stream)
+                          void *out, int out_f64, void *stream)
 {
     FRH_REQ(raw && out && lo && den && wy && wx, "frh_prefilter: null pointer");
     FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0 && C <= 32, "frh_prefilter: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
@@ -187,7 +187,9 @@ static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, 
     FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)T * C * H * W, st), "frh_prefilter scratch");
     hipError_t e = f64 ? launch_prefilter_v64((const double *)raw, T, H, W, C, lo, den, wy, ry, tmp, st)
                        : launch_prefilter_v((const float *)raw, T, H, W, C, lo, den, wy, ry, tmp, st);
-    if (e == hipSuccess) e = launch_prefilter_h(tmp, T, H, W, C, wx, rx, out, st);
+    if (e == hipSuccess)
+        e = out_f64 ? launch_prefilter_h64(tmp, T, H, W, C, wx, rx, (double *)out, st)
+                    : launch_prefilter_h(tmp, T, H, W, C, wx, rx, (float *)out, st);
     hipError_t e2 = hipFreeAsync(tmp, st);
     if (e != hipSuccess) return fail(FRH_EHIP, "prefilter kernels: %s", hipGetErrorString(e));
     FRH_HIP(e2, "frh_prefilter free");
@@ -197,14 +199,21 @@ static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, 
 int frh_prefilter(const float *raw, int T, int H, int W, int C, const double *lo, const double *den,
                   const double *wy, int ry, const double *wx, int rx, float *out, void *stream)
 {
-    return prefilter_impl(raw, 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, stream);
+    return prefilter_impl(raw, 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, stream);
 }
 
 int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const double *lo,
                       const double *den, const double *wy, int ry, const double *wx, int rx,
                       float *out, void *stream)
 {
-    return prefilter_impl(raw, 1, T, H, W, C, lo, den, wy, ry, wx, rx, out, stream);
+    return prefilter_impl(raw, 1, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, stream);
+}
+
+int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
+                     const double *den, const double *wy, int ry, const double *wx, int rx, void *out,
+                     int out_f64, void *stream)
+{
+    return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, stream);
 }
 
 int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow, const int32_t *ix,
@@ -224,19 +233,20 @@ int frh_warp_cubic(const float *f2, const float *f1, int f1_per_frame, int T, in
     FRH_REQ(f2 && f1 && u && v && out, "frh_warp_cubic: null pointer");
     FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_cubic: bad shape");
     FRH_REQ(h_col > 0 && h_row > 0, "frh_warp_cubic: grid spacings must be positive");
-    hipError_t e = launch_warp(f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, h_col, h_row, out,
+    hipError_t e = launch_warp(f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, h_col, h_row, 0, out,
                                as_stream(stream));
     FRH_HIP(e, "warp kernel");
     return FRH_OK;
 }
 
-int frh_warp_cubic_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H, int W,
-                       const double *u, const double *v, float *out, void *stream)
+int frh_warp_cubic_ex(const void *f2, const void *f1, int img_f64, int acc_f64, int f1_per_frame, int T, int C,
+                      int H, int W, const double *u, const double *v, float *out, void *stream)
 {
-    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_cubic_f64: null pointer");
-    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_cubic_f64: bad shape");
-    hipError_t e = launch_warp_f64(f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, out,
-                                   as_stream(stream));
+    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_cubic_ex: null pointer");
+    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_cubic_ex: bad shape");
+    FRH_REQ(!img_f64 || acc_f64, "frh_warp_cubic_ex: float64 images take cv2's 64F path (acc_f64 = 1)");
+    hipError_t e = launch_warp_ex(f2, f1, img_f64 != 0, acc_f64 != 0, f1_per_frame, T, C, H, W, u, v,
+                                  (size_t)H * W, out, as_stream(stream));
     FRH_HIP(e, "warp kernel");
     return FRH_OK;
 }
@@ -667,8 +677,9 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
             }
             {
                 Scope sc(s, TM_WARP, st);
+                // the reference's level images are float64: cv2.remap's 64F path
                 FRH_HIP(launch_warp(s->f2l, f1, fixed ? 1 : 0, T, C, h, w, s->uvl, s->uvl + hw, 2 * hw,
-                                    L.h_col, L.h_row, s->f2w, st), "warp");
+                                    L.h_col, L.h_row, 1, s->f2w, st), "warp");
             }
             f2src = s->f2w;
         }

@@ -42,15 +42,19 @@ hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, cons
                               hipStream_t st);
 hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, const double *wx,
                               int rx, float *out, hipStream_t st);
+hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, const double *wx,
+                                int rx, double *out, hipStream_t st);
 hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow,
                          const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
                          int P, float *tmp, hipStream_t st);
+// acc64: cv2.remap's 64F path (double sums), else its 32F path (float sums)
 hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
                        int W, const float *u, const float *v, size_t uv_frame_stride,
-                       double h_col, double h_row, float *out, hipStream_t st);
-hipError_t launch_warp_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
-                           int W, const double *u, const double *v, size_t uv_frame_stride,
-                           float *out, hipStream_t st);
+                       double h_col, double h_row, int acc64, float *out, hipStream_t st);
+// float64 displacements; images float64 (img_f64, double sums) or float32
+hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame, int T,
+                          int C, int H, int W, const double *u, const double *v, size_t uv_frame_stride,
+                          float *out, hipStream_t st);
 hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_stride, const float *f2,
                          int T, int C, int h, int w, double hy, double hx, double eps, const float *u,
                          const float *v, size_t uv_frame_stride, double ax, double ay, double *J,

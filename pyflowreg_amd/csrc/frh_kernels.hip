@@ -71,8 +71,9 @@ __global__ void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, Ga
     tmp[((size_t)tc * H + y) * W + x] = acc;
 }
 
+template <typename O>
 __global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx, int rx,
-                              float *__restrict__ out)
+                              O *__restrict__ out)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
@@ -82,7 +83,7 @@ __global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, Gaus
     double acc = row[x] * wx.w[rx];
     for (int j = rx; j >= 1; --j)
         acc += (row[reflect_half(x - j, W)] + row[reflect_half(x + j, W)]) * wx.w[rx - j];
-    out[((size_t)tc * H + y) * W + x] = (float)acc;
+    out[((size_t)tc * H + y) * W + x] = (O)acc;
 }
 
 hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, const double *lo,
@@ -121,7 +122,17 @@ hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, con
     GaussW g{};
     for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_h, grid, dim3(64), 0, st, tmp, H, W, g, rx, out);
+    hipLaunchKernelGGL(k_prefilter_h<float>, grid, dim3(64), 0, st, tmp, H, W, g, rx, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, const double *wx,
+                                int rx, double *out, hipStream_t st)
+{
+    GaussW g{};
+    for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
+    dim3 grid((W + 63) / 64, H, T * C);
+    hipLaunchKernelGGL(k_prefilter_h<double>, grid, dim3(64), 0, st, tmp, H, W, g, rx, out);
     return hipGetLastError();
 }
 
@@ -278,8 +289,13 @@ __device__ inline void cv_cubic_coeffs(float x, float *c)
 
 // U: float (level flows, divided by h_col / h_row in float64 here) or
 // double (displacements already divided by the caller, h = 1).
-template <typename U>
-__global__ void k_warp(const float *__restrict__ f2, const float *__restrict__ f1,
+// cv2.remap INTER_CUBIC + BORDER_REPLICATE of the image type I: the 1/32-px
+// float weight table of both depths, and sums in Acc -- float for float32
+// images (OpenCV's 32F path), double for float64 images (its 64F path, which
+// the reference takes for its float64 level images); the result is stored as
+// float32, as the reference's float32 output array does.
+template <typename U, typename I, typename Acc>
+__global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
                        int f1_per_frame, int C, int H, int W, const U *__restrict__ u,
                        const U *__restrict__ v, size_t uv_stride, double h_col, double h_row,
                        float *__restrict__ out)
@@ -289,14 +305,14 @@ __global__ void k_warp(const float *__restrict__ f2, const float *__restrict__ f
     int t = blockIdx.z;
     if (x >= W) return;
     size_t HW = (size_t)H * W, p = (size_t)y * W + x;
-    const float *F2 = f2 + (size_t)t * C * HW;
-    const float *F1 = f1 + (f1_per_frame ? (size_t)t * C * HW : 0);
+    const I *F2 = f2 + (size_t)t * C * HW;
+    const I *F1 = f1 + (f1_per_frame ? (size_t)t * C * HW : 0);
     float *O = out + (size_t)t * C * HW;
     float mx = (float)((double)x + (double)u[t * uv_stride + p] / h_col);
     float my = (float)((double)y + (double)v[t * uv_stride + p] / h_row);
     bool oob = (mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H);
     if (oob) {
-        for (int c = 0; c < C; ++c) O[c * HW + p] = F1[c * HW + p];
+        for (int c = 0; c < C; ++c) O[c * HW + p] = (float)F1[c * HW + p];
         return;
     }
     float cx = mx > (float)(W - 1) ? (float)(W - 1) : mx;
@@ -311,15 +327,15 @@ __global__ void k_warp(const float *__restrict__ f2, const float *__restrict__ f
     int width1 = W - 3 > 0 ? W - 3 : 0, height1 = H - 3 > 0 ? H - 3 : 0;
     if ((unsigned)sx < (unsigned)width1 && (unsigned)sy < (unsigned)height1) {
         for (int c = 0; c < C; ++c) {
-            const float *S = F2 + c * HW + (size_t)sy * W + sx;
-            float sum = S[0] * w[0] + S[1] * w[1] + S[2] * w[2] + S[3] * w[3];
+            const I *S = F2 + c * HW + (size_t)sy * W + sx;
+            Acc sum = (Acc)S[0] * w[0] + (Acc)S[1] * w[1] + (Acc)S[2] * w[2] + (Acc)S[3] * w[3];
             S += W;
-            sum += S[0] * w[4] + S[1] * w[5] + S[2] * w[6] + S[3] * w[7];
+            sum += (Acc)S[0] * w[4] + (Acc)S[1] * w[5] + (Acc)S[2] * w[6] + (Acc)S[3] * w[7];
             S += W;
-            sum += S[0] * w[8] + S[1] * w[9] + S[2] * w[10] + S[3] * w[11];
+            sum += (Acc)S[0] * w[8] + (Acc)S[1] * w[9] + (Acc)S[2] * w[10] + (Acc)S[3] * w[11];
             S += W;
-            sum += S[0] * w[12] + S[1] * w[13] + S[2] * w[14] + S[3] * w[15];
-            O[c * HW + p] = sum;
+            sum += (Acc)S[0] * w[12] + (Acc)S[1] * w[13] + (Acc)S[2] * w[14] + (Acc)S[3] * w[15];
+            O[c * HW + p] = (float)sum;
         }
     } else {
         int xs[4], ys[4];
@@ -328,30 +344,43 @@ __global__ void k_warp(const float *__restrict__ f2, const float *__restrict__ f
             ys[i] = clampi(sy + i, 0, H - 1);
         }
         for (int c = 0; c < C; ++c) {
-            const float *S = F2 + c * HW;
-            float sum = 0.f;
+            const I *S = F2 + c * HW;
+            Acc sum = 0;
             for (int i = 0; i < 4; ++i)
-                for (int j = 0; j < 4; ++j) sum += S[(size_t)ys[i] * W + xs[j]] * w[i * 4 + j];
-            O[c * HW + p] = sum;
+                for (int j = 0; j < 4; ++j) sum += (Acc)S[(size_t)ys[i] * W + xs[j]] * w[i * 4 + j];
+            O[c * HW + p] = (float)sum;
         }
     }
 }
 
 hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
                        int W, const float *u, const float *v, size_t uv_frame_stride,
-                       double h_col, double h_row, float *out, hipStream_t st)
+                       double h_col, double h_row, int acc64, float *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_warp<float>, dim3((W + 63) / 64, H, T), dim3(64), 0, st, f2, f1, f1_per_frame,
-                       C, H, W, u, v, uv_frame_stride, h_col, h_row, out);
+    const dim3 g((W + 63) / 64, H, T), b(64);
+    if (acc64)
+        hipLaunchKernelGGL((k_warp<float, float, double>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v,
+                           uv_frame_stride, h_col, h_row, out);
+    else
+        hipLaunchKernelGGL((k_warp<float, float, float>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v,
+                           uv_frame_stride, h_col, h_row, out);
     return hipGetLastError();
 }
 
-hipError_t launch_warp_f64(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
-                           int W, const double *u, const double *v, size_t uv_frame_stride,
-                           float *out, hipStream_t st)
+hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame, int T,
+                          int C, int H, int W, const double *u, const double *v, size_t uv_frame_stride,
+                          float *out, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_warp<double>, dim3((W + 63) / 64, H, T), dim3(64), 0, st, f2, f1, f1_per_frame,
-                       C, H, W, u, v, uv_frame_stride, 1.0, 1.0, out);
+    const dim3 g((W + 63) / 64, H, T), b(64);
+    if (img_f64)
+        hipLaunchKernelGGL((k_warp<double, double, double>), g, b, 0, st, (const double *)f2,
+                           (const double *)f1, f1_per_frame, C, H, W, u, v, uv_frame_stride, 1.0, 1.0, out);
+    else if (acc64)
+        hipLaunchKernelGGL((k_warp<double, float, double>), g, b, 0, st, (const float *)f2, (const float *)f1,
+                           f1_per_frame, C, H, W, u, v, uv_frame_stride, 1.0, 1.0, out);
+    else
+        hipLaunchKernelGGL((k_warp<double, float, float>), g, b, 0, st, (const float *)f2, (const float *)f1,
+                           f1_per_frame, C, H, W, u, v, uv_frame_stride, 1.0, 1.0, out);
     return hipGetLastError();
 }
 

@@ -42,16 +42,25 @@ def _device():
 
 def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor) -> torch.Tensor:
     """core/warping.py:160-234 (cubic) for a batch on the GPU.
-    frames (T, H, W, C) float32, flow (T, H, W, 2), fill (H, W, C) float32
-    -> (T, H, W, C) float32."""
+    frames (T, H, W, C) float32 (cv2.remap's 32F path) or float64 (its 64F
+    path, double sums), flow (T, H, W, 2), fill (H, W, C) -> (T, H, W, C)
+    float32."""
     T, H, W, C = frames.shape
+    f64 = frames.dtype == torch.float64
     f2 = frames.permute(0, 3, 1, 2).contiguous()
-    f1 = fill.permute(2, 0, 1).contiguous()
-    u = flow[..., 0].to(torch.float32).contiguous()
-    v = flow[..., 1].to(torch.float32).contiguous()
-    out = torch.empty_like(f2)
-    check(lib().frh_warp_cubic(ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u), ptr(v), 1.0, 1.0,
-                               ptr(out), stream_handle()), "frh_warp_cubic")
+    f1 = fill.to(frames.dtype).permute(2, 0, 1).contiguous()
+    if f64:
+        u = flow[..., 0].to(torch.float64).contiguous()
+        v = flow[..., 1].to(torch.float64).contiguous()
+        out = torch.empty(f2.shape, dtype=torch.float32, device=f2.device)
+        check(lib().frh_warp_cubic_ex(ptr(f2), ptr(f1), 1, 1, 0, T, C, H, W, ptr(u), ptr(v), ptr(out),
+                                      stream_handle()), "frh_warp_cubic_ex")
+    else:
+        u = flow[..., 0].to(torch.float32).contiguous()
+        v = flow[..., 1].to(torch.float32).contiguous()
+        out = torch.empty_like(f2)
+        check(lib().frh_warp_cubic(ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u), ptr(v), 1.0, 1.0,
+                                   ptr(out), stream_handle()), "frh_warp_cubic")
     return out.permute(0, 2, 3, 1)
 
 
@@ -150,7 +159,6 @@ class HipBatchedExecutor:
         as3 = lambda x: x[..., None] if x.ndim == 2 else x  # noqa: E731
         batch = np.ascontiguousarray(as4(np.asarray(batch)))
         bp = np.ascontiguousarray(as4(np.asarray(batch_proc)).astype(np.float32, copy=False))
-        rr = np.ascontiguousarray(as3(np.asarray(reference_raw)).astype(np.float32, copy=False))
         rp = np.ascontiguousarray(as3(np.asarray(reference_proc)).astype(np.float32, copy=False))
         wi = np.ascontiguousarray(np.asarray(w_init).astype(np.float32, copy=False))
         T, H, W, C = batch.shape
@@ -167,8 +175,10 @@ class HipBatchedExecutor:
             solver = self._get_solver(H, W, C, T, fp, term)
             solver.set_reference(rp, fp.get("weight"), wi)
             flow = solver.run(torch.from_numpy(bp).to(dev), out_dtype=torch.float32)
-        reg = warp_frames(torch.from_numpy(batch.astype(np.float32)).to(dev), flow,
-                          torch.from_numpy(rr).to(dev))
+        # cv2.remap works in the raw frames' depth: float64 frames sum in float64
+        wdt = np.float64 if batch.dtype == np.float64 else np.float32
+        reg = warp_frames(torch.from_numpy(batch.astype(wdt, copy=False)).to(dev), flow,
+                          torch.from_numpy(np.asarray(as3(np.asarray(reference_raw)), dtype=wdt)).to(dev))
         registered = reg.cpu().numpy().astype(batch.dtype, copy=False)
         flows = flow.cpu().numpy()
         if progress_callback is not None:

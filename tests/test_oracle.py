@@ -180,6 +180,70 @@ def test_warp_matches_torch_bicubic_at_quantized_positions(oracle):
     np.testing.assert_allclose(out[3:-3, 3:-3], ref[3:-3, 3:-3], atol=1e-5)
 
 
+def _cv_coeffs(x):
+    """OpenCV's interpolateCubic (A = -0.75) in float32, from its public
+    source: the 4 weights at fraction x."""
+    A = np.float32(-0.75)
+    x = np.float32(x)
+    one = np.float32(1)
+    c0 = ((A * (x + one) - np.float32(5) * A) * (x + one) + np.float32(8) * A) * (x + one) - np.float32(4) * A
+    c1 = ((A + np.float32(2)) * x - (A + np.float32(3))) * x * x + one
+    c2 = ((A + np.float32(2)) * (one - x) - (A + np.float32(3))) * (one - x) * (one - x) + one
+    c3 = one - c0 - c1 - c2
+    return [c0, c1, c2, c3]
+
+
+def test_warp64_reference_warp_tests(oracle):
+    """tests/core/test_warping.py:22-33,250-360 on float64 images (the level
+    images of get_displacement): zero shift is the identity, integer shifts
+    copy, and out-of-bounds pixels take the fixed frame rounded to float32."""
+    rng = np.random.default_rng(21)
+    H, W = 30, 34
+    f2 = rng.random((H, W, 2)).astype(np.float32).astype(np.float64)
+    f1 = rng.random((H, W, 2))  # not float32-valued
+    z = np.zeros((H, W))
+    np.testing.assert_array_equal(oracle.imregister_wrapper(f2, z, z, f1), f2.astype(np.float32))
+    out = oracle.imregister_wrapper(f2, z + 4.0, z - 3.0, f1)
+    np.testing.assert_array_equal(out[3:, :-4], f2[:-3, 4:].astype(np.float32))
+    np.testing.assert_array_equal(out[:3], f1[:3].astype(np.float32))
+    np.testing.assert_array_equal(out[:, -4:], f1[:, -4:].astype(np.float32))
+
+
+def test_warp64_sums_in_float64(oracle):
+    """cv2.remap's 64F path: float weights (products of the float 1-D
+    tables), float64 products and sums (row sums added in turn), one
+    rounding to float32 at the end; evaluated here with numpy float64 at
+    1/32-px positions of interior pixels.  The 32F path (float sums) gives
+    different bits on the same data."""
+    rng = np.random.default_rng(22)
+    H, W = 20, 26
+    f2 = (rng.random((H, W)) * 100).astype(np.float32).astype(np.float64)
+    ku = rng.integers(-40, 40, (H, W))
+    kv = rng.integers(-40, 40, (H, W))
+    u, v = ku / 32.0, kv / 32.0
+    out = oracle.imregister_wrapper(f2, u, v, f2)
+    out32 = oracle.imregister_wrapper(f2.astype(np.float32), u, v, f2.astype(np.float32))
+    checked = 0
+    for y in range(H):
+        for x in range(W):
+            X, Y = x * 32 + ku[y, x], y * 32 + kv[y, x]
+            sx, sy = (X >> 5) - 1, (Y >> 5) - 1
+            if not (0 <= sx < W - 3 and 0 <= sy < H - 3):
+                continue
+            wx, wy = _cv_coeffs((X & 31) / 32.0), _cv_coeffs((Y & 31) / 32.0)
+            total = None
+            for i in range(4):
+                row = None
+                for j in range(4):
+                    term = f2[sy + i, sx + j] * np.float64(np.float32(wy[i] * wx[j]))
+                    row = term if row is None else row + term
+                total = row if total is None else total + row
+            assert out[y, x] == np.float32(total), (y, x)
+            checked += 1
+    assert checked > 200
+    assert np.any(out != out32)
+
+
 # ------------------------------------------------------------------ tensor
 def test_motion_tensor_reference_known_answer(oracle):
     """tests/core/test_optical_flow.py:532-551: f = 1e-3 x^2 with hx = 2 gives
