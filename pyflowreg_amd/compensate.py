@@ -14,7 +14,11 @@ compensate_recording.py):
   ``hip_batched``), then ``w_init`` becomes the float32 mean of the batch's
   last (up to) 20 flows (:786-806);
 * per-frame statistics mean/max displacement, mean divergence and mean
-  translation are accumulated (:808-821).
+  translation are accumulated (:808-821);
+* with ``update_reference`` the processed reference becomes the float64
+  mean of the batch's last (up to) 100 preprocessed float64 frames, each
+  warped by its flow with the current reference as fill (:680-713,
+  :866-867).
 
 Means and statistics are computed with numpy on the host, exactly as the
 reference computes them, so that ``w_init`` is bit-identical.
@@ -28,7 +32,7 @@ from typing import Callable, Optional, Sequence
 import numpy as np
 import torch
 
-from .executor import HipBatchedExecutor
+from .executor import HipBatchedExecutor, warp_frames
 from .preprocess import preprocess
 
 _QUALITY_MIN_LEVEL = {"quality": 0, "balanced": 4, "fast": 6}
@@ -53,6 +57,7 @@ class RegistrationOptions:
     channel_normalization: str = "together"
     interpolation_method: str = "cubic"
     update_initialization_w: bool = True
+    update_reference: bool = False
     constancy_assumption: str = "gc"
     extra: dict = field(default_factory=dict)
 
@@ -80,7 +85,8 @@ class RegistrationOptions:
             return options
         kw = {}
         for f in ("alpha", "weight", "levels", "min_level", "eta", "update_lag", "iterations",
-                  "a_smooth", "a_data", "sigma", "buffer_size", "update_initialization_w"):
+                  "a_smooth", "a_data", "sigma", "buffer_size", "update_initialization_w",
+                  "update_reference"):
             if hasattr(options, f):
                 kw[f] = getattr(options, f)
         for f in ("quality_setting", "channel_normalization", "interpolation_method",
@@ -118,14 +124,15 @@ class MotionCorrector:
         self.shape = (H, W, C)
         self.weight = _channel_weights(self.opt.weight, C)
         self.sigma = _sigma_rows(self.opt.sigma)
-        self.reference_proc = self._preprocess(self.reference_raw, None)[0].permute(1, 2, 0)
+        # float64, as the reference keeps it; the executor casts it to float32
+        self.reference_proc = self._preprocess(self.reference_raw, None, torch.float64)[0].permute(1, 2, 0)
         self.executor = HipBatchedExecutor(max_batch=max_batch)
         self.w_init = None
         self.mean_disp, self.max_disp, self.mean_div, self.mean_translation = [], [], [], []
 
-    def _preprocess(self, frames, norm_ref):
+    def _preprocess(self, frames, norm_ref, out_dtype=torch.float32):
         return preprocess(frames, normalization_ref=norm_ref, sigma=self.sigma[0],
-                          channel_normalization=self.opt.channel_normalization)
+                          channel_normalization=self.opt.channel_normalization, out_dtype=out_dtype)
 
     def _flow_params(self):
         fp = self.opt.flow_params()
@@ -143,7 +150,11 @@ class MotionCorrector:
         """Register one batch (T, H, W, C); returns (registered, w)."""
         batch = np.asarray(batch)
         batch = batch[..., None] if batch.ndim == 3 else batch
-        bp = self._preprocess(batch, self.reference_raw)
+        if self.opt.update_reference:
+            bp64 = self._preprocess(batch, self.reference_raw, torch.float64)
+            bp = bp64.to(torch.float32)  # the same rounding as the float32 prefilter output
+        else:
+            bp = self._preprocess(batch, self.reference_raw)
         H, W, _ = self.shape
         if self.w_init is None:
             n_init = min(22, batch.shape[0])
@@ -163,7 +174,29 @@ class MotionCorrector:
             um = float(np.mean(w[t, :, :, 0]))
             vm = float(np.mean(w[t, :, :, 1]))
             self.mean_translation.append(float(np.sqrt(um ** 2 + vm ** 2)))
+        if self.opt.update_reference:
+            self._update_reference(bp64, w)
         return registered, w
+
+    def _update_reference(self, bp64, w):
+        """compensate_recording.py:680-713: the processed reference becomes
+        the mean of the last min(100, T) preprocessed float64 frames, each
+        warped (cubic, float64 images: cv2.remap's 64F path, float32 result)
+        by its flow with the current processed reference as fill; the mean
+        is float64, summed frame by frame as numpy's axis-0 mean does."""
+        if str(self.opt.interpolation_method).lower() != "cubic":
+            raise ValueError("flowreg_hip: update_reference supports cubic warps only")
+        T = bp64.shape[0]
+        n = min(100, T)
+        if n < 1:
+            return
+        frames = bp64[T - n:].permute(0, 2, 3, 1)
+        flows = torch.from_numpy(np.ascontiguousarray(w[T - n:])).to(frames.device)
+        reg = warp_frames(frames, flows, self.reference_proc)
+        acc = reg[0].to(torch.float64)
+        for k in range(1, n):
+            acc = acc + reg[k].to(torch.float64)
+        self.reference_proc = (acc / n).contiguous()
 
     def close(self):
         self.executor.cleanup()

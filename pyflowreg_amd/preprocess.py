@@ -46,9 +46,12 @@ def _range(src, C, channel_normalization, eps):
 
 
 def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
-               channel_normalization="together", eps=1e-8, truncate=4.0) -> torch.Tensor:
+               channel_normalization="together", eps=1e-8, truncate=4.0,
+               out_dtype=torch.float32) -> torch.Tensor:
     """frames: (H, W, C) or (T, H, W, C) numpy/tensor (float32 or float64).
-    Returns float32 CUDA planes (T, C, H, W) (T = 1 for a single frame)."""
+    Returns CUDA planes (T, C, H, W) (T = 1 for a single frame): float32, the
+    executors' cast, or float64 (``out_dtype``), the preprocessed frames
+    before that cast as update_reference and FlowRegLive use them."""
     if not torch.cuda.is_available():
         raise FlowregHipError("flowreg_hip needs a ROCm GPU; none is visible")
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -74,8 +77,8 @@ def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
     lo, den = _range(ref, C, channel_normalization, eps)
     ry, wy = gaussian_weights(sy, truncate) if sy > 1e-15 else (0, np.ones(1))
     rx, wx = gaussian_weights(sx, truncate) if sx > 1e-15 else (0, np.ones(1))
-    out = torch.empty((T, C, H, W), dtype=torch.float32, device=dev)
-    fn = lib().frh_prefilter_f64 if f64 else lib().frh_prefilter
-    check(fn(ptr(x), T, H, W, C, ptr(lo), ptr(den), ptr(wy), ry, ptr(wx), rx, ptr(out),
-             stream_handle()), "frh_prefilter")
+    out64 = out_dtype == torch.float64
+    out = torch.empty((T, C, H, W), dtype=torch.float64 if out64 else torch.float32, device=dev)
+    check(lib().frh_prefilter_ex(ptr(x), int(f64), T, H, W, C, ptr(lo), ptr(den), ptr(wy), ry, ptr(wx), rx,
+                                 ptr(out), int(out64), stream_handle()), "frh_prefilter_ex")
     return out

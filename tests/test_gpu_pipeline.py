@@ -76,6 +76,53 @@ def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle):
     np.testing.assert_array_equal(reg, np.concatenate(oregs))
 
 
+def test_compensate_update_reference_matches_oracle(gpu, oracle):
+    """compensate_recording.py:680-713,866-867: after each batch the
+    processed reference becomes the float64 mean of the last min(100, T)
+    preprocessed float64 frames warped by their flows (float64 images:
+    cv2.remap's 64F path) with the old reference as fill; the next batch
+    registers against it."""
+    from pyflowreg_amd.compensate import MotionCorrector, RegistrationOptions
+    v = _video(25, 40, 32, 2, 33)
+    ref = v[0].astype(np.float64)
+    video = v[1:]
+    opt = RegistrationOptions(alpha=(2.0, 2.0), buffer_size=8, sigma=(1.0, 1.0, 0.1), update_reference=True)
+    mc = MotionCorrector(ref, opt, max_batch=8)
+    fp = dict(alpha=(2.0, 2.0), weight=np.ones((40, 32, 2)) * 0.5, levels=100, min_level=0,
+              eta=0.8, iterations=50, update_lag=5, a_data=0.45, a_smooth=1.0)
+    ref_proc = oracle.preprocess(ref)  # float64
+    w_init = None
+    try:
+        for s in range(0, video.shape[0], 8):
+            b = video[s:s + 8]
+            reg, w = mc.process(b)
+            rp32 = ref_proc.astype(np.float32)
+            if w_init is None:
+                _, w0 = _oracle_batch(oracle, b, ref, rp32, np.zeros((40, 32, 2)), fp)
+                w_init = np.mean(w0, axis=0)
+            oreg, ow = _oracle_batch(oracle, b, ref, rp32, w_init, fp)
+            w_init = np.mean(ow[-20:], axis=0) if ow.shape[0] > 20 else np.mean(ow, axis=0)
+            np.testing.assert_array_equal(w, ow)
+            np.testing.assert_array_equal(reg, oreg)
+            bp64 = oracle.preprocess(b, ref)
+            n = min(100, b.shape[0])
+            comp = np.stack([oracle.imregister_wrapper(bp64[t], ow[t, :, :, 0], ow[t, :, :, 1], ref_proc)
+                             for t in range(b.shape[0] - n, b.shape[0])]).astype(np.float64)
+            ref_proc = np.mean(comp, axis=0)
+            np.testing.assert_array_equal(mc.reference_proc.cpu().numpy(), ref_proc)
+    finally:
+        mc.close()
+
+
+def test_preprocess_float64_output_matches_oracle(gpu, oracle):
+    from pyflowreg_amd.preprocess import preprocess
+    v = _video(4, 36, 30, 2, 34)
+    ref = v[0].astype(np.float64)
+    out = preprocess(v[1:], normalization_ref=ref, out_dtype=__import__("torch").float64)
+    exp = oracle.preprocess(v[1:], ref)
+    np.testing.assert_array_equal(out.permute(0, 2, 3, 1).cpu().numpy(), exp)
+
+
 def test_min_level_above_zero_matches_oracle(gpu, oracle):
     """quality 'balanced' (min_level 4): the flow stops at level 4 and is
     brought to full resolution with cv2.resize INTER_CUBIC."""
