@@ -72,10 +72,17 @@ int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const doubl
  * (out_f64 = 1, out float64): the preprocessed frames before the executors'
  * float32 cast, as update_reference (compensate_recording.py:680-713) and
  * FlowRegLive (flow_reg_live.py:216-323) use them.  raw is float64 when
- * raw_f64 != 0, else float32. */
+ * raw_f64 != 0, else float32.  semantics: 0 for the float64 arithmetic of
+ * frh_prefilter, or flags for a float32 frame normalised by its own
+ * float32 range (util/image_processing.py:12-98 keeps numpy's float32 there,
+ * FlowRegLive's reference): FRH_PRE_NORM32, (x - lo) / den in float32;
+ * FRH_PRE_FILT32, each Gaussian pass stored as float32, as scipy's
+ * gaussian_filter does for a float32 array. */
+#define FRH_PRE_NORM32 1
+#define FRH_PRE_FILT32 2
 int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
                      const double *den, const double *wy, int ry, const double *wx, int rx,
-                     void *out, int out_f64, void *stream);
+                     void *out, int out_f64, int semantics, void *stream);
 
 /* util/resize_util.py:116-141,232-254 (imresize_fused_gauss_cubic).
  * src: (N, h, w) planes; dst: (N, oh, ow); tmp: (N, h, ow) scratch.

@@ -177,7 +177,7 @@ int frh_warping_depth(double eta, int levels, int m, int n)
 
 static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, const double *lo,
                           const double *den, const double *wy, int ry, const double *wx, int rx,
-                          void *out, int out_f64, void *stream)
+                          void *out, int out_f64, int sem, void *stream)
 {
     FRH_REQ(raw && out && lo && den && wy && wx, "frh_prefilter: null pointer");
     FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0 && C <= 32, "frh_prefilter: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
@@ -185,11 +185,11 @@ static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, 
     hipStream_t st = as_stream(stream);
     double *tmp = nullptr;
     FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)T * C * H * W, st), "frh_prefilter scratch");
-    hipError_t e = f64 ? launch_prefilter_v64((const double *)raw, T, H, W, C, lo, den, wy, ry, tmp, st)
-                       : launch_prefilter_v((const float *)raw, T, H, W, C, lo, den, wy, ry, tmp, st);
+    hipError_t e = f64 ? launch_prefilter_v64((const double *)raw, T, H, W, C, lo, den, wy, ry, sem, tmp, st)
+                       : launch_prefilter_v((const float *)raw, T, H, W, C, lo, den, wy, ry, sem, tmp, st);
     if (e == hipSuccess)
-        e = out_f64 ? launch_prefilter_h64(tmp, T, H, W, C, wx, rx, (double *)out, st)
-                    : launch_prefilter_h(tmp, T, H, W, C, wx, rx, (float *)out, st);
+        e = out_f64 ? launch_prefilter_h64(tmp, T, H, W, C, wx, rx, sem, (double *)out, st)
+                    : launch_prefilter_h(tmp, T, H, W, C, wx, rx, sem, (float *)out, st);
     hipError_t e2 = hipFreeAsync(tmp, st);
     if (e != hipSuccess) return fail(FRH_EHIP, "prefilter kernels: %s", hipGetErrorString(e));
     FRH_HIP(e2, "frh_prefilter free");
@@ -199,21 +199,23 @@ static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, 
 int frh_prefilter(const float *raw, int T, int H, int W, int C, const double *lo, const double *den,
                   const double *wy, int ry, const double *wx, int rx, float *out, void *stream)
 {
-    return prefilter_impl(raw, 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, stream);
+    return prefilter_impl(raw, 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, 0, stream);
 }
 
 int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const double *lo,
                       const double *den, const double *wy, int ry, const double *wx, int rx,
                       float *out, void *stream)
 {
-    return prefilter_impl(raw, 1, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, stream);
+    return prefilter_impl(raw, 1, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, 0, stream);
 }
 
 int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
                      const double *den, const double *wy, int ry, const double *wx, int rx, void *out,
-                     int out_f64, void *stream)
+                     int out_f64, int semantics, void *stream)
 {
-    return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, stream);
+    FRH_REQ((semantics & ~(FRH_PRE_NORM32 | FRH_PRE_FILT32)) == 0, "frh_prefilter_ex: unknown semantics flags");
+    return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, semantics,
+                          stream);
 }
 
 int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow, const int32_t *ix,

@@ -38,12 +38,12 @@ constexpr int kCoefPlanes = 5;  // Du, Dv, A12, Bu, Bv
 
 // ---- launchers (frh_kernels.hip); all return hipError_t of the launch ----
 hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, const double *lo,
-                              const double *den, const double *wy, int ry, double *tmp,
+                              const double *den, const double *wy, int ry, int sem, double *tmp,
                               hipStream_t st);
 hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, const double *wx,
-                              int rx, float *out, hipStream_t st);
+                              int rx, int sem, float *out, hipStream_t st);
 hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, const double *wx,
-                                int rx, double *out, hipStream_t st);
+                                int rx, int sem, double *out, hipStream_t st);
 hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow,
                          const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
                          int P, float *tmp, hipStream_t st);
@@ -77,7 +77,7 @@ hipError_t launch_broadcast_f32(const float *src, size_t n, float *dst, size_t d
 hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *dst,
                               hipStream_t st);
 hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, const double *lo,
-                                const double *den, const double *wy, int ry, double *tmp,
+                                const double *den, const double *wy, int ry, int sem, double *tmp,
                                 hipStream_t st);
 hipError_t launch_padded_to_skew_f64(const double *src, int m, int n, int K, const Skew &s,
                                      double *dst, hipStream_t st);

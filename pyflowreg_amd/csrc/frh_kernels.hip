@@ -53,9 +53,17 @@ struct GaussW {
     double w[65];
 };
 
+// sem (FRH_PRE_* of include/flowreg_hip.h): bit 0, the normalisation
+// (x - lo) / den in float32 arithmetic (numpy with a float32 array and
+// float32 bounds: each operation rounded to float32, which double
+// arithmetic followed by a float32 rounding reproduces exactly); bit 1,
+// scipy's gaussian_filter on a float32 array, which stores each 1-D pass
+// into the float32 output (the double line sums rounded to float32).
+constexpr int kPreNorm32 = 1, kPreFilt32 = 2;
+
 template <typename T>
 __global__ void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, GaussW lo_den,
-                              GaussW wy, int ry, double *__restrict__ tmp)
+                              GaussW wy, int ry, int sem, double *__restrict__ tmp)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
@@ -64,15 +72,19 @@ __global__ void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, Ga
     int t = tc / C, c = tc % C;
     double lo = lo_den.w[2 * c], den = lo_den.w[2 * c + 1];
     const T *base = raw + (size_t)t * H * W * C + c;
-    auto nrm = [&](int yy) { return ((double)base[((size_t)yy * W + x) * C] - lo) / den; };
+    const bool n32 = (sem & kPreNorm32) != 0;
+    auto nrm = [&](int yy) {
+        const double d = (double)base[((size_t)yy * W + x) * C] - lo;
+        return n32 ? (double)(float)((double)(float)d / den) : d / den;
+    };
     double acc = nrm(y) * wy.w[ry];
     for (int j = ry; j >= 1; --j)
         acc += (nrm(reflect_half(y - j, H)) + nrm(reflect_half(y + j, H))) * wy.w[ry - j];
-    tmp[((size_t)tc * H + y) * W + x] = acc;
+    tmp[((size_t)tc * H + y) * W + x] = (sem & kPreFilt32) ? (double)(float)acc : acc;
 }
 
 template <typename O>
-__global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx, int rx,
+__global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx, int rx, int sem,
                               O *__restrict__ out)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
@@ -83,11 +95,11 @@ __global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, Gaus
     double acc = row[x] * wx.w[rx];
     for (int j = rx; j >= 1; --j)
         acc += (row[reflect_half(x - j, W)] + row[reflect_half(x + j, W)]) * wx.w[rx - j];
-    out[((size_t)tc * H + y) * W + x] = (O)acc;
+    out[((size_t)tc * H + y) * W + x] = (sem & kPreFilt32) ? (O)(float)acc : (O)acc;
 }
 
 hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, const double *lo,
-                              const double *den, const double *wy, int ry, double *tmp,
+                              const double *den, const double *wy, int ry, int sem, double *tmp,
                               hipStream_t st)
 {
     GaussW ld{}, g{};
@@ -97,12 +109,12 @@ hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, cons
     }
     for (int k = 0; k < 2 * ry + 1; ++k) g.w[k] = wy[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_v<float>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, tmp);
+    hipLaunchKernelGGL(k_prefilter_v<float>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, sem, tmp);
     return hipGetLastError();
 }
 
 hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, const double *lo,
-                                const double *den, const double *wy, int ry, double *tmp,
+                                const double *den, const double *wy, int ry, int sem, double *tmp,
                                 hipStream_t st)
 {
     GaussW ld{}, g{};
@@ -112,27 +124,27 @@ hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, c
     }
     for (int k = 0; k < 2 * ry + 1; ++k) g.w[k] = wy[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_v<double>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, tmp);
+    hipLaunchKernelGGL(k_prefilter_v<double>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, sem, tmp);
     return hipGetLastError();
 }
 
 hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, const double *wx,
-                              int rx, float *out, hipStream_t st)
+                              int rx, int sem, float *out, hipStream_t st)
 {
     GaussW g{};
     for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_h<float>, grid, dim3(64), 0, st, tmp, H, W, g, rx, out);
+    hipLaunchKernelGGL(k_prefilter_h<float>, grid, dim3(64), 0, st, tmp, H, W, g, rx, sem, out);
     return hipGetLastError();
 }
 
 hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, const double *wx,
-                                int rx, double *out, hipStream_t st)
+                                int rx, int sem, double *out, hipStream_t st)
 {
     GaussW g{};
     for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_h<double>, grid, dim3(64), 0, st, tmp, H, W, g, rx, out);
+    hipLaunchKernelGGL(k_prefilter_h<double>, grid, dim3(64), 0, st, tmp, H, W, g, rx, sem, out);
     return hipGetLastError();
 }
 

@@ -43,24 +43,18 @@ def _device():
 def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor) -> torch.Tensor:
     """core/warping.py:160-234 (cubic) for a batch on the GPU.
     frames (T, H, W, C) float32 (cv2.remap's 32F path) or float64 (its 64F
-    path, double sums), flow (T, H, W, 2), fill (H, W, C) -> (T, H, W, C)
+    path, double sums), flow (T, H, W, 2) of any float dtype (the map
+    float32(x + u) is formed in float64), fill (H, W, C) -> (T, H, W, C)
     float32."""
     T, H, W, C = frames.shape
     f64 = frames.dtype == torch.float64
     f2 = frames.permute(0, 3, 1, 2).contiguous()
     f1 = fill.to(frames.dtype).permute(2, 0, 1).contiguous()
-    if f64:
-        u = flow[..., 0].to(torch.float64).contiguous()
-        v = flow[..., 1].to(torch.float64).contiguous()
-        out = torch.empty(f2.shape, dtype=torch.float32, device=f2.device)
-        check(lib().frh_warp_cubic_ex(ptr(f2), ptr(f1), 1, 1, 0, T, C, H, W, ptr(u), ptr(v), ptr(out),
-                                      stream_handle()), "frh_warp_cubic_ex")
-    else:
-        u = flow[..., 0].to(torch.float32).contiguous()
-        v = flow[..., 1].to(torch.float32).contiguous()
-        out = torch.empty_like(f2)
-        check(lib().frh_warp_cubic(ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u), ptr(v), 1.0, 1.0,
-                                   ptr(out), stream_handle()), "frh_warp_cubic")
+    u = flow[..., 0].to(torch.float64).contiguous()
+    v = flow[..., 1].to(torch.float64).contiguous()
+    out = torch.empty(f2.shape, dtype=torch.float32, device=f2.device)
+    check(lib().frh_warp_cubic_ex(ptr(f2), ptr(f1), int(f64), int(f64), 0, T, C, H, W, ptr(u), ptr(v), ptr(out),
+                                  stream_handle()), "frh_warp_cubic_ex")
     return out.permute(0, 2, 3, 1)
 
 

@@ -15,6 +15,9 @@ import torch
 
 from ._lib import FlowregHipError, check, lib, ptr, stream_handle
 
+PRE_NORM32 = 1  # FRH_PRE_NORM32
+PRE_FILT32 = 2  # FRH_PRE_FILT32
+
 
 def gaussian_weights(sigma: float, truncate: float = 4.0):
     """The 1-D kernel scipy.ndimage.gaussian_filter1d correlates with
@@ -47,11 +50,16 @@ def _range(src, C, channel_normalization, eps):
 
 def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
                channel_normalization="together", eps=1e-8, truncate=4.0,
-               out_dtype=torch.float32) -> torch.Tensor:
+               out_dtype=torch.float32, norm_bounds=None, semantics=0) -> torch.Tensor:
     """frames: (H, W, C) or (T, H, W, C) numpy/tensor (float32 or float64).
     Returns CUDA planes (T, C, H, W) (T = 1 for a single frame): float32, the
     executors' cast, or float64 (``out_dtype``), the preprocessed frames
-    before that cast as update_reference and FlowRegLive use them."""
+    before that cast as update_reference and FlowRegLive use them.
+    ``norm_bounds`` = (lo, den) per channel replaces the range of
+    ``normalization_ref``: x -> (x - lo) / den (FlowRegLive normalises with
+    its filtered reference's min and max, flow_reg_live.py:192-214);
+    ``semantics`` takes FRH_PRE_NORM32 / FRH_PRE_FILT32 for a float32 frame
+    normalised and filtered in float32 as numpy and scipy do."""
     if not torch.cuda.is_available():
         raise FlowregHipError("flowreg_hip needs a ROCm GPU; none is visible")
     dev = torch.device("cuda", torch.cuda.current_device())
@@ -73,12 +81,16 @@ def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
     st = float(sig[2]) if (len(sig) > 2 and not spatial_only) else 0.0
     if st > 1e-15 and int(truncate * st + 0.5) > 0:
         raise ValueError("flowreg_hip: temporal Gaussian radius > 0 is not implemented")
-    ref = normalization_ref if normalization_ref is not None else frames
-    lo, den = _range(ref, C, channel_normalization, eps)
+    if norm_bounds is not None:
+        lo = np.ascontiguousarray(np.broadcast_to(np.asarray(norm_bounds[0], dtype=np.float64), (C,)))
+        den = np.ascontiguousarray(np.broadcast_to(np.asarray(norm_bounds[1], dtype=np.float64), (C,)))
+    else:
+        ref = normalization_ref if normalization_ref is not None else frames
+        lo, den = _range(ref, C, channel_normalization, eps)
     ry, wy = gaussian_weights(sy, truncate) if sy > 1e-15 else (0, np.ones(1))
     rx, wx = gaussian_weights(sx, truncate) if sx > 1e-15 else (0, np.ones(1))
     out64 = out_dtype == torch.float64
     out = torch.empty((T, C, H, W), dtype=torch.float64 if out64 else torch.float32, device=dev)
     check(lib().frh_prefilter_ex(ptr(x), int(f64), T, H, W, C, ptr(lo), ptr(den), ptr(wy), ry, ptr(wx), rx,
-                                 ptr(out), int(out64), stream_handle()), "frh_prefilter_ex")
+                                 ptr(out), int(out64), int(semantics), stream_handle()), "frh_prefilter_ex")
     return out
