@@ -212,34 +212,63 @@ class LevelSolve(torch.autograd.Function):
         g_par = torch.zeros((T, 2 + C, P), dtype=_F64, device=dev)
         g_axy = torch.zeros((T, 2, P), dtype=_F64, device=dev)
         g_coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
-        coef = torch.empty_like(g_coef)
         scratch = torch.empty((T, 4, P), dtype=_F64, device=dev)
         blocks = lag_blocks(iterations, update_lag)
-        states = None
-        for b in range(len(blocks) - 1, -1, -1):
+        nmax = max(e - s for s, e, _ in blocks)
+        # Lag blocks run backwards.  Recomputing a block's sweeps from its
+        # checkpoint does not depend on the later blocks' adjoints, so the
+        # recompute of the next block runs on a side stream while the main
+        # stream runs the current block's adjoint (each uses one work-group
+        # per frame); two state / coefficient buffers alternate.
+        bufs = [torch.empty((nmax, T, 2, P), dtype=_F64, device=dev) for _ in range(2)]
+        coefs = [torch.empty((T, 6, P), dtype=_F64, device=dev) for _ in range(2)]
+        main = torch.cuda.current_stream(dev)
+        side = torch.cuda.Stream(device=dev)
+        side.wait_stream(main)
+        hs = side.cuda_stream
+
+        def x_in(b, slot, k):  # the state entering sweep k of block b
+            return ckpt[b] if k == 0 else bufs[slot][k - 1]
+
+        def recompute(b, slot):
             s, e, psi_one = blocks[b]
-            x0 = ckpt[b]
-            check(L.frh_grad_coef(ptr(x0), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
-                                  int(psi_one), ptr(coef), None, None, None, None, None, st),
+            check(L.frh_grad_coef(ptr(ckpt[b]), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
+                                  int(psi_one), ptr(coefs[slot]), None, None, None, None, None, hs),
                   "frh_grad_coef")
-            n = e - s
-            if states is None or states.shape[0] < n + 1:
-                states = torch.empty((n + 1, T, 2, P), dtype=_F64, device=dev)
-            states[0].copy_(x0)
-            for k in range(n):  # recompute the block from its checkpoint
-                check(L.frh_grad_sweep(ptr(coef), ptr(states[k]), ptr(states[k + 1]), T, h, w, axf, ayf, st),
-                      "frh_grad_sweep")
+            for k in range(e - s):
+                check(L.frh_grad_sweep(ptr(coefs[slot]), ptr(x_in(b, slot, k)), ptr(bufs[slot][k]), T, h, w,
+                                       axf, ayf, hs), "frh_grad_sweep")
+            ev = torch.cuda.Event()
+            ev.record(side)
+            return ev
+
+        order = list(range(len(blocks) - 1, -1, -1))
+        ev_rec = recompute(order[0], 0)
+        ev_adj = None
+        for i, b in enumerate(order):
+            slot = i % 2
+            ev_next = None
+            if i + 1 < len(order):
+                if ev_adj is not None:  # the other buffers served the previous block's adjoint
+                    side.wait_event(ev_adj)
+                ev_next = recompute(order[i + 1], 1 - slot)
+            main.wait_event(ev_rec)
+            s, e, psi_one = blocks[b]
             g_coef.zero_()
-            for k in range(n - 1, -1, -1):
-                check(L.frh_grad_sweep_adjoint(ptr(coef), ptr(states[k]), ptr(states[k + 1]), ptr(lam),
-                                               ptr(lam_in), ptr(g_coef), ptr(g_axy), ptr(scratch), T, h, w,
-                                               axf, ayf, st), "frh_grad_sweep_adjoint")
+            for k in range(e - s - 1, -1, -1):
+                check(L.frh_grad_sweep_adjoint(ptr(coefs[slot]), ptr(x_in(b, slot, k)), ptr(bufs[slot][k]),
+                                               ptr(lam), ptr(lam_in), ptr(g_coef), ptr(g_axy), ptr(scratch), T, h,
+                                               w, axf, ayf, st), "frh_grad_sweep_adjoint")
                 lam, lam_in = lam_in, lam
             # the block's coefficients depend on its start state, J, u0 and
             # the parameters; lam becomes the adjoint of the start state
-            check(L.frh_grad_coef(ptr(x0), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
+            check(L.frh_grad_coef(ptr(ckpt[b]), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
                                   int(psi_one), None, ptr(g_coef), ptr(lam), ptr(g_u0), ptr(g_J), ptr(g_par),
                                   st), "frh_grad_coef")
+            ev_adj = torch.cuda.Event()
+            ev_adj.record(main)
+            ev_rec = ev_next
+        main.wait_stream(side)
         # the solve starts from du = dv = 0: lam (its adjoint) is dropped
         g_f2w = None
         if ctx.needs_input_grad[1]:
