@@ -239,6 +239,42 @@ def grad(args):
     }))
 
 
+def compensate(args):
+    """compensate_arr end to end: a host recording of --frames frames
+    (512x512x2 float32) registered in batches of --batch with w_init updates,
+    batches staged through pinned memory; host arrays in and out.  Prints
+    one JSON line (frames/s)."""
+    import torch
+    from pyflowreg_amd import synth
+    from pyflowreg_amd.compensate import RegistrationOptions, compensate_arr
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1"}))
+        sys.exit(2)
+    dev = torch.device("cuda", 0)
+    H = W = args.size
+    C = args.channels
+    frames, _ = synth.video(args.frames + 1, H, W, n_channels=C, seed=1000, device=dev)
+    video = frames[1:].cpu().numpy()
+    ref = frames[0].double().cpu().numpy()
+    opt = RegistrationOptions(alpha=(2.0, 2.0), buffer_size=args.batch, sigma=(1.0, 1.0, 0.1))
+    compensate_arr(video[: min(args.batch, 32)], ref, opt, max_batch=args.batch)  # warm-up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        reg, w = compensate_arr(video, ref, opt, max_batch=args.batch)
+    dt = time.perf_counter() - t0
+    print(json.dumps({
+        "metric": "frames/s compensate_arr end to end (512x512x2ch, host arrays in and out)",
+        "value": round(args.steps * video.shape[0] / dt, 3), "unit": "frames/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": 1, "ms_per_step": round(dt / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (blob scenes, smooth random displacement + drift, Poisson noise)",
+        "config": {"workload": f"compensate_arr: {video.shape[0]} frames {H}x{W}x{C}, batches of {args.batch}, "
+                               "w_init from the first 22 frames then the mean of each batch's last 20",
+                   "frames": int(video.shape[0])},
+    }))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -253,9 +289,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--workload", choices=("uniform", "uneven", "grad"), default="uniform",
+    ap.add_argument("--frames", type=int, default=512, help="recording length for the compensate workload")
+    ap.add_argument("--workload", choices=("uniform", "uneven", "grad", "compensate"), default="uniform",
                     help="uneven: 512 independent pairs of mixed sizes and channel counts; "
-                         "grad: differentiable registration (forward + backward)")
+                         "grad: differentiable registration (forward + backward); "
+                         "compensate: compensate_arr end to end")
     ap.add_argument("--grad-frames", type=int, default=32, help="moving frames for the grad workload")
     ap.add_argument("--pairs", type=int, default=512)
     ap.add_argument("--streams", type=int, default=4, help="streams for the uneven workload's groups")
@@ -268,6 +306,9 @@ def main():
         return
     if args.workload == "grad":
         grad(args)
+        return
+    if args.workload == "compensate":
+        compensate(args)
         return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus must be 1",

@@ -15,6 +15,9 @@ compensate_recording.py):
   last (up to) 20 flows (:786-806);
 * per-frame statistics mean/max displacement, mean divergence and mean
   translation are accumulated (:808-821);
+* batches are staged in pinned host memory and uploaded on a side stream
+  while the previous batch is solved; preprocessing, solving and warping
+  stay on the device;
 * with ``update_reference`` the processed reference becomes the float64
   mean of the batch's last (up to) 100 preprocessed float64 frames, each
   warped by its flow with the current reference as fill (:680-713,
@@ -33,6 +36,7 @@ import numpy as np
 import torch
 
 from .executor import HipBatchedExecutor, warp_frames
+from .optical_flow import _resolve_const_assumption
 from .preprocess import preprocess
 
 _QUALITY_MIN_LEVEL = {"quality": 0, "balanced": 4, "fast": 6}
@@ -113,8 +117,79 @@ def _sigma_rows(sigma):
     return s if s.ndim == 2 else s[None]
 
 
+_STAT_WORKERS = 8
+_STAT_POOL = None
+
+
+def _stat_pool():
+    global _STAT_POOL
+    if _STAT_POOL is None:
+        from concurrent.futures import ThreadPoolExecutor
+        _STAT_POOL = ThreadPoolExecutor(max_workers=_STAT_WORKERS)
+    return _STAT_POOL
+
+
+class BatchStager:
+    """Pinned host staging of raw frame batches (the batches read_batch
+    returns, util/io/_base.py:283-313) with asynchronous uploads on a side
+    stream: two pinned-host / device buffer pairs alternate, so the host copy
+    and upload of the next batch overlap the GPU work on the current one.
+    Frames are staged as float64 when they are float64, else as float32."""
+
+    def __init__(self, device):
+        self.device = device
+        self.stream = torch.cuda.Stream(device=device)
+        self.slots = [None, None]
+        self.turn = 0
+
+    def upload(self, frames: np.ndarray) -> int:
+        frames = np.asarray(frames)
+        dt = np.float64 if frames.dtype == np.float64 else np.float32
+        key = (frames.shape, dt)
+        k = self.turn
+        self.turn ^= 1
+        slot = self.slots[k]
+        if slot is None or slot["key"] != key:
+            if slot is not None:  # nothing may still read or write the buffers it replaces
+                for ev in (slot["up"], slot["done"]):
+                    if ev is not None:
+                        ev.synchronize()
+            tdt = torch.float64 if dt == np.float64 else torch.float32
+            # allocated from the side stream's pool: a block that kernels on the
+            # main stream freed but may still be using is never handed out here
+            with torch.cuda.stream(self.stream):
+                dev = torch.empty(frames.shape, dtype=tdt, device=self.device)
+            slot = {"key": key, "host": torch.empty(frames.shape, dtype=tdt, pin_memory=True),
+                    "dev": dev, "up": None, "done": None}
+            self.slots[k] = slot
+        if slot["up"] is not None:
+            slot["up"].synchronize()  # the last upload from this host buffer has left it
+        np.copyto(slot["host"].numpy(), frames, casting="unsafe")
+        with torch.cuda.stream(self.stream):
+            if slot["done"] is not None:
+                self.stream.wait_event(slot["done"])  # the last batch in this device buffer is used up
+            slot["dev"].copy_(slot["host"], non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.stream)
+            slot["up"] = ev
+        return k
+
+    def acquire(self, k: int) -> torch.Tensor:
+        slot = self.slots[k]
+        torch.cuda.current_stream(self.device).wait_event(slot["up"])
+        return slot["dev"]
+
+    def release(self, k: int) -> None:
+        ev = torch.cuda.Event()
+        ev.record(torch.cuda.current_stream(self.device))
+        self.slots[k]["done"] = ev
+
+
 class MotionCorrector:
-    """Device-resident BatchMotionCorrector state for one recording."""
+    """Device-resident BatchMotionCorrector state for one recording: a batch
+    is staged and uploaded once, preprocessed, solved and warped on the GPU,
+    and only its outputs come back to the host (where numpy forms the means
+    and statistics exactly as the reference does)."""
 
     def __init__(self, reference, options=None, max_batch: int = 256):
         self.opt = RegistrationOptions.from_any(options)
@@ -127,8 +202,13 @@ class MotionCorrector:
         # float64, as the reference keeps it; the executor casts it to float32
         self.reference_proc = self._preprocess(self.reference_raw, None, torch.float64)[0].permute(1, 2, 0)
         self.executor = HipBatchedExecutor(max_batch=max_batch)
+        self.device = self.reference_proc.device
+        self.stager = BatchStager(self.device)
+        self._ref_raw_dev = {}
         self.w_init = None
         self.mean_disp, self.max_disp, self.mean_div, self.mean_translation = [], [], [], []
+        if str(self.opt.interpolation_method).lower() != "cubic":
+            raise ValueError("flowreg_hip: only cubic registration warps are implemented")
 
     def _preprocess(self, frames, norm_ref, out_dtype=torch.float32):
         return preprocess(frames, normalization_ref=norm_ref, sigma=self.sigma[0],
@@ -140,42 +220,94 @@ class MotionCorrector:
         fp["weight"] = np.ones((H, W, C)) * self.weight.reshape(1, 1, -1)
         return fp
 
-    def _register(self, batch, batch_proc, w_init):
-        return self.executor.process_batch(
-            batch, batch_proc.permute(0, 2, 3, 1).cpu().numpy(), self.reference_raw,
-            self.reference_proc.cpu().numpy(), w_init, interpolation_method=self.opt.interpolation_method,
-            flow_params=self._flow_params())
+    def _reference_raw(self, dtype):
+        if dtype not in self._ref_raw_dev:
+            self._ref_raw_dev[dtype] = torch.as_tensor(self.reference_raw, dtype=dtype, device=self.device)
+        return self._ref_raw_dev[dtype]
+
+    def _register(self, frames, frames_proc, w_init):
+        fp = self._flow_params()
+        term = _resolve_const_assumption(fp.get("const_assumption", "gc"))
+        return self.executor.register_device(frames, frames_proc, self._reference_raw(frames.dtype),
+                                             self.reference_proc, w_init, fp, term)
+
+    def _launch(self, frames):
+        """GPU work of one batch (T, H, W, C) on the device: preprocess,
+        w_init on the first batch, the batch's flows and registered frames."""
+        if self.opt.update_reference:
+            bp64 = self._preprocess(frames, self.reference_raw, torch.float64)
+            bp = bp64.to(torch.float32)  # the same rounding as the float32 prefilter output
+        else:
+            bp64 = None
+            bp = self._preprocess(frames, self.reference_raw)
+        bp = bp.permute(0, 2, 3, 1)
+        H, W, _ = self.shape
+        if self.w_init is None:
+            n_init = min(22, frames.shape[0])
+            _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)))
+            self.w_init = np.mean(w0.cpu().numpy(), axis=0)
+        cur = self.w_init if self.opt.update_initialization_w else np.zeros_like(self.w_init)
+        reg, flow = self._register(frames, bp, cur)
+        return reg, flow, bp64
+
+    def _finish(self, dtype, reg, flow, bp64, out_reg=None, out_w=None):
+        """Host side of one batch that the next batch depends on: outputs
+        (copied into out_reg / out_w when given), w_init and (with
+        update_reference) the new reference."""
+        if out_reg is not None and out_reg.dtype == np.float32:
+            torch.from_numpy(out_reg).copy_(reg)
+            registered = out_reg
+        else:
+            registered = reg.cpu().numpy().astype(dtype, copy=False)
+            if out_reg is not None:
+                out_reg[...] = registered
+        if out_w is not None:
+            torch.from_numpy(out_w).copy_(flow)
+            w = out_w
+        else:
+            w = flow.cpu().numpy()
+        if self.opt.update_initialization_w:
+            self.w_init = np.mean(w[-20:], axis=0) if w.shape[0] > 20 else np.mean(w, axis=0)
+        if self.opt.update_reference:
+            self._update_reference(bp64, w)
+        return registered, w
+
+    def _statistics(self, w):
+        """compensate_recording.py:808-821, per frame with numpy (frame
+        chunks on a thread pool: each frame's reductions are its own, so the
+        values are those of the serial loop)."""
+        def chunk(lo, hi):
+            ww = w[lo:hi]
+            mag = np.sqrt(ww[:, :, :, 0] ** 2 + ww[:, :, :, 1] ** 2)
+            md = np.mean(mag, axis=(1, 2)).tolist()
+            mx = np.max(mag, axis=(1, 2)).tolist()
+            div, tr = [], []
+            for t in range(ww.shape[0]):
+                du_dx = np.gradient(ww[t, :, :, 0], axis=1)
+                dv_dy = np.gradient(ww[t, :, :, 1], axis=0)
+                div.append(float(np.mean(du_dx + dv_dy)))
+                um = float(np.mean(ww[t, :, :, 0]))
+                vm = float(np.mean(ww[t, :, :, 1]))
+                tr.append(float(np.sqrt(um ** 2 + vm ** 2)))
+            return md, mx, div, tr
+        T = w.shape[0]
+        step = max(1, -(-T // _STAT_WORKERS))
+        parts = list(_stat_pool().map(lambda lo: chunk(lo, min(T, lo + step)), range(0, T, step)))
+        for md, mx, div, tr in parts:
+            self.mean_disp.extend(md)
+            self.max_disp.extend(mx)
+            self.mean_div.extend(div)
+            self.mean_translation.extend(tr)
 
     def process(self, batch):
         """Register one batch (T, H, W, C); returns (registered, w)."""
         batch = np.asarray(batch)
         batch = batch[..., None] if batch.ndim == 3 else batch
-        if self.opt.update_reference:
-            bp64 = self._preprocess(batch, self.reference_raw, torch.float64)
-            bp = bp64.to(torch.float32)  # the same rounding as the float32 prefilter output
-        else:
-            bp = self._preprocess(batch, self.reference_raw)
-        H, W, _ = self.shape
-        if self.w_init is None:
-            n_init = min(22, batch.shape[0])
-            _, w0 = self._register(batch[:n_init], bp[:n_init], np.zeros((H, W, 2)))
-            self.w_init = np.mean(w0, axis=0)
-        cur = self.w_init if self.opt.update_initialization_w else np.zeros_like(self.w_init)
-        registered, w = self._register(batch, bp, cur)
-        if self.opt.update_initialization_w:
-            self.w_init = np.mean(w[-20:], axis=0) if w.shape[0] > 20 else np.mean(w, axis=0)
-        mag = np.sqrt(w[:, :, :, 0] ** 2 + w[:, :, :, 1] ** 2)
-        self.mean_disp.extend(np.mean(mag, axis=(1, 2)).tolist())
-        self.max_disp.extend(np.max(mag, axis=(1, 2)).tolist())
-        for t in range(w.shape[0]):
-            du_dx = np.gradient(w[t, :, :, 0], axis=1)
-            dv_dy = np.gradient(w[t, :, :, 1], axis=0)
-            self.mean_div.append(float(np.mean(du_dx + dv_dy)))
-            um = float(np.mean(w[t, :, :, 0]))
-            vm = float(np.mean(w[t, :, :, 1]))
-            self.mean_translation.append(float(np.sqrt(um ** 2 + vm ** 2)))
-        if self.opt.update_reference:
-            self._update_reference(bp64, w)
+        k = self.stager.upload(batch)
+        state = self._launch(self.stager.acquire(k))
+        self.stager.release(k)
+        registered, w = self._finish(batch.dtype, *state)
+        self._statistics(w)
         return registered, w
 
     def _update_reference(self, bp64, w):
@@ -184,8 +316,6 @@ class MotionCorrector:
         warped (cubic, float64 images: cv2.remap's 64F path, float32 result)
         by its flow with the current processed reference as fill; the mean
         is float64, summed frame by frame as numpy's axis-0 mean does."""
-        if str(self.opt.interpolation_method).lower() != "cubic":
-            raise ValueError("flowreg_hip: update_reference supports cubic warps only")
         T = bp64.shape[0]
         n = min(100, T)
         if n < 1:
@@ -216,17 +346,26 @@ def compensate_arr(c1, c_ref, options=None,
     mc = MotionCorrector(c_ref, options, max_batch=max_batch)
     T = video.shape[0]
     bs = int(mc.opt.buffer_size)
-    regs, ws = [], []
+    registered = np.empty(video.shape, dtype=video.dtype)
+    flows = np.empty(video.shape[:3] + (2,), dtype=np.float32)
     try:
+        k = mc.stager.upload(video[0:bs])
+        state = mc._launch(mc.stager.acquire(k))
+        mc.stager.release(k)
         for s in range(0, T, bs):
-            r, w = mc.process(video[s:s + bs])
-            regs.append(r)
-            ws.append(w)
+            nxt = s + bs < T
+            if nxt:  # stage the next batch while the GPU works on this one
+                k = mc.stager.upload(video[s + bs:s + 2 * bs])
+            e = min(T, s + bs)
+            _, w = mc._finish(video.dtype, *state, out_reg=registered[s:e], out_w=flows[s:e])
+            if nxt:  # the next batch's GPU work runs during this batch's statistics
+                state = mc._launch(mc.stager.acquire(k))
+                mc.stager.release(k)
+            mc._statistics(w)
             if progress_callback is not None:
                 progress_callback(min(T, s + bs), T)
     finally:
         mc.close()
-    registered = np.concatenate(regs)
     if squeeze:
         registered = registered[..., 0]
-    return registered, np.concatenate(ws)
+    return registered, flows

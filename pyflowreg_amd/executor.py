@@ -71,6 +71,18 @@ def compute_batch_valid_masks(w) -> np.ndarray:
     return (valid.to(torch.uint8) * 255).cpu().numpy()
 
 
+def _to_numpy32(a):
+    if isinstance(a, torch.Tensor):
+        return a.to(torch.float32).cpu().numpy()
+    return np.ascontiguousarray(np.asarray(a).astype(np.float32, copy=False))
+
+
+def _to_device32(a):
+    if isinstance(a, torch.Tensor):
+        return a.to(torch.float32)
+    return torch.from_numpy(np.ascontiguousarray(np.asarray(a).astype(np.float32, copy=False)))
+
+
 class HipBatchedExecutor:
     """Executor 'hip_batched': whole batches per GPU launch."""
 
@@ -132,6 +144,33 @@ class HipBatchedExecutor:
             self._key = key
         return self._solver
 
+    def register_device(self, frames: torch.Tensor, frames_proc: torch.Tensor, reference_raw: torch.Tensor,
+                        reference_proc, w_init, flow_params: dict, term: str = "gc"):
+        """The batch's registration with every array on the device: frames
+        (T, H, W, C) raw (float32, or float64 for cv2's 64F warp), frames_proc
+        (T, H, W, C) float32 preprocessed, reference_raw (H, W, C) in the
+        frames' dtype, reference_proc (H, W, C) and w_init (H, W, 2) (cast to
+        float32 as the executors do).  Returns (registered float32, flow
+        float32) device tensors."""
+        T, H, W, C = frames.shape
+        fp = flow_params
+        if fp.get("gnc_schedule") is not None:
+            # GNC stages relinearise on the host loop (pyflowreg_amd.stage_loop),
+            # one frame at a time, as the sequential executor does
+            from .optical_flow import get_displacement
+            kw = {k: v for k, v in fp.items() if k != "weight"}
+            rp = _to_numpy32(reference_proc)
+            wi = _to_numpy32(w_init)
+            bp = frames_proc.cpu().numpy()
+            flows_np = np.stack([get_displacement(rp, bp[t], weight=fp.get("weight"), uv=wi.copy(), **kw)
+                                 for t in range(T)]).astype(np.float32)
+            flow = torch.from_numpy(flows_np).to(frames.device)
+        else:
+            solver = self._get_solver(H, W, C, T, fp, term)
+            solver.set_reference(_to_device32(reference_proc), fp.get("weight"), _to_device32(w_init))
+            flow = solver.run(frames_proc, out_dtype=torch.float32)
+        return warp_frames(frames, flow, reference_raw), flow
+
     def process_batch(self, batch: np.ndarray, batch_proc: np.ndarray, reference_raw: np.ndarray,
                       reference_proc: np.ndarray, w_init: np.ndarray,
                       get_displacement_func: Optional[Callable] = None,
@@ -155,28 +194,16 @@ class HipBatchedExecutor:
         bp = np.ascontiguousarray(as4(np.asarray(batch_proc)).astype(np.float32, copy=False))
         rp = np.ascontiguousarray(as3(np.asarray(reference_proc)).astype(np.float32, copy=False))
         wi = np.ascontiguousarray(np.asarray(w_init).astype(np.float32, copy=False))
-        T, H, W, C = batch.shape
         dev = _device()
-        if fp.get("gnc_schedule") is not None:
-            # GNC stages relinearise on the host loop (pyflowreg_amd.stage_loop),
-            # one frame at a time, as the sequential executor does
-            from .optical_flow import get_displacement
-            kw = {k: v for k, v in fp.items() if k != "weight"}
-            flows_np = np.stack([get_displacement(rp, bp[t], weight=fp.get("weight"), uv=wi.copy(), **kw)
-                                 for t in range(T)]).astype(np.float32)
-            flow = torch.from_numpy(flows_np).to(dev)
-        else:
-            solver = self._get_solver(H, W, C, T, fp, term)
-            solver.set_reference(rp, fp.get("weight"), wi)
-            flow = solver.run(torch.from_numpy(bp).to(dev), out_dtype=torch.float32)
         # cv2.remap works in the raw frames' depth: float64 frames sum in float64
         wdt = np.float64 if batch.dtype == np.float64 else np.float32
-        reg = warp_frames(torch.from_numpy(batch.astype(wdt, copy=False)).to(dev), flow,
-                          torch.from_numpy(np.asarray(as3(np.asarray(reference_raw)), dtype=wdt)).to(dev))
+        reg, flow = self.register_device(
+            torch.from_numpy(batch.astype(wdt, copy=False)).to(dev), torch.from_numpy(bp).to(dev),
+            torch.from_numpy(np.asarray(as3(np.asarray(reference_raw)), dtype=wdt)).to(dev), rp, wi, fp, term)
         registered = reg.cpu().numpy().astype(batch.dtype, copy=False)
         flows = flow.cpu().numpy()
         if progress_callback is not None:
-            progress_callback(T)
+            progress_callback(batch.shape[0])
         return registered, flows
 
 
