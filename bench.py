@@ -285,6 +285,8 @@ def main():
     ap.add_argument("--channels", type=int, default=2)
     ap.add_argument("--plain-frames", type=int, default=3, help="0 skips the plain path")
     ap.add_argument("--graph", action="store_true", help="replay the batched level loop as a hipGraph")
+    ap.add_argument("--autotune", action="store_true",
+                    help="tune (or load from the JSON cache) the sweeps per wavefront pass of each level")
     ap.add_argument("--cpu-frames", type=int, default=16)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -347,6 +349,12 @@ def main():
     ref_hwc = preprocess(raw_ref)[0].permute(1, 2, 0).contiguous()
     solver = fr.DisplacementSolver(H, W, C, max_batch=T, graph=args.graph)
     solver.set_reference(ref_hwc, [0.5] * C)
+    tuned = None
+    if args.autotune:
+        from pyflowreg_amd import autotune as at
+        p0 = preprocess(batch, normalization_ref=raw_ref).permute(0, 2, 3, 1)
+        tuned = at.autotune(solver, p0)
+        tuned = {"limits": tuned["limits"], "cached": tuned["cached"]}
 
     def step():
         p = preprocess(batch, normalization_ref=raw_ref)
@@ -428,6 +436,7 @@ def main():
         "plain_fps": round(plain_fps, 3) if plain_fps else None,
         "plain_fps_eager": round(plain_fps_eager, 3) if plain_fps_eager else None,
         "batched_graph": bool(args.graph),
+        "autotune": tuned,
         "batched_over_plain": round(fps / plain_fps, 2) if plain_fps else None,
         "parity": parity,
         "epe_vs_truth_px": round(epe, 4),

@@ -232,6 +232,17 @@ int frh_solver_profile(frh_solver *s, int enable);
 int frh_solver_set_graph(frh_solver *s, int enable);
 int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n);
 
+/* Tuning knobs of the SOR wavefront (no counterpart in the reference, whose
+ * solvers have no launch geometry): each level's sweeps run as wavefront
+ * passes of at most pass_limit sweeps (between coefficient refreshes); more
+ * sweeps per pass read and write the state fewer times but hold more
+ * registers.  The result does not depend on the limit.  level_info returns
+ * a level's size, current and largest allowed limit, and its sweep +
+ * assembly time (ms) in the last profiled run. */
+int frh_solver_level_info(frh_solver *s, int level, int *h, int *w, int *pass_limit, int *pass_limit_max,
+                          double *last_ms);
+int frh_solver_set_pass_limit(frh_solver *s, int level, int pass_limit);
+
 /* ------------------------------------------- differentiable registration */
 /* Forward and backward (reverse-mode) kernels of one pass of
  * get_displacement (core/optical_flow.py:476-617: gradient-constancy data

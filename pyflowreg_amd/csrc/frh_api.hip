@@ -389,9 +389,11 @@ struct frh_solver {
     bool have_ref = false;
     bool profile = false;
     struct Mark {
-        int cat;
+        int cat, level;
         hipEvent_t a, b;
     };
+    int cur_level = -1;                 // level being launched (profile marks)
+    std::vector<double> level_ms;       // sweep + assemble time per level of the last profiled run
     std::vector<Mark> marks;
     std::vector<hipEvent_t> pool;
     size_t pool_used = 0;
@@ -455,7 +457,7 @@ struct Scope {
         if (s->profile) {
             hipEvent_t b = next_event(s);
             (void)hipEventRecord(b, st);
-            s->marks.push_back({cat, a, b});
+            s->marks.push_back({cat, s->cur_level, a, b});
         }
     }
 };
@@ -636,6 +638,7 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
     const int C = s->C;
     s->marks.clear();
     s->pool_used = 0;
+    s->cur_level = -1;
     hipEvent_t t0{}, t1{};
     if (s->profile) {
         t0 = next_event(s);
@@ -651,6 +654,7 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
     }
     for (int li = 0; li < s->nl; ++li) {
         LevelDev &L = s->lv[li];
+        s->cur_level = li;
         const int h = L.h, w = L.w;
         const size_t hw = (size_t)h * w;
         const Skew &sk = L.sk;
@@ -749,10 +753,13 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         (void)hipEventElapsedTime(&ms, t0, t1);
         s->tm[TM_TOTAL] = ms;
         s->counts[TM_TOTAL] = 1;
+        s->level_ms.assign(s->nl, 0.0);
         for (auto &m : s->marks) {
             (void)hipEventElapsedTime(&ms, m.a, m.b);
             s->tm[m.cat] += ms;
             s->counts[m.cat] += 1;
+            if ((m.cat == TM_SWEEP || m.cat == TM_ASSEMBLE) && m.level >= 0 && m.level < s->nl)
+                s->level_ms[m.level] += ms;
         }
     }
     return FRH_OK;
@@ -838,6 +845,36 @@ int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n)
         counts[k] = s->counts[k];
     }
     return k;
+}
+
+int frh_solver_level_info(frh_solver *s, int level, int *h, int *w, int *pass_limit, int *pass_limit_max,
+                          double *last_ms)
+{
+    FRH_REQ(s && level >= 0 && level < s->nl, "frh_solver_level_info: bad level");
+    const LevelDev &L = s->lv[level];
+    int lim = 0;
+    for (const Pass &p : L.passes) lim = p.nsweeps > lim ? p.nsweeps : lim;
+    if (h) *h = L.h;
+    if (w) *w = L.w;
+    if (pass_limit) *pass_limit = lim;
+    if (pass_limit_max) *pass_limit_max = sweep_pmax(L.h);
+    if (last_ms) *last_ms = level < (int)s->level_ms.size() ? s->level_ms[level] : 0.0;
+    return FRH_OK;
+}
+
+int frh_solver_set_pass_limit(frh_solver *s, int level, int pass_limit)
+{
+    FRH_REQ(s && level >= 0 && level < s->nl, "frh_solver_set_pass_limit: bad level");
+    LevelDev &L = s->lv[level];
+    FRH_REQ(pass_limit >= 1 && pass_limit <= sweep_pmax(L.h),
+            "frh_solver_set_pass_limit: %d sweeps per pass outside [1, %d] for a %d-row level", pass_limit,
+            sweep_pmax(L.h), L.h);
+    L.passes = plan_passes(s->iterations, s->update_lag, pass_limit);
+    if (s->gexec) {  // the captured level loop holds the old pass schedule
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+    }
+    return FRH_OK;
 }
 
 }  // extern "C"
