@@ -606,14 +606,17 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
 }
 
 // Tiles of kTensorRows rows x 64 columns, one thread per cell (lanes run
-// along a row, so the image reads are coalesced).  A tile's 16 rows fill
-// whole 128-byte lines of each skewed plane on its inner diagonals.  Tiles
-// are numbered frame-major, then row-major, and dealt to the XCDs in
-// contiguous ranges (xcd_block) so neighbouring tiles share one L2.
-// J planes (component-major, then channel) and cu, cv are written skewed.
-// The level image f1 is float64 data in the reference; P2 marks a float32
-// warped image (every level below the top).
+// along a row, so the image reads are coalesced).  Tiles are numbered
+// frame-major, then row-major, and dealt to the XCDs in contiguous ranges
+// (xcd_block) so neighbouring tiles share one L2.  J planes (component-major,
+// then channel) and cu, cv are written skewed: each output plane of a tile
+// goes through LDS and out diagonal by diagonal, so that 16 consecutive
+// lanes store one diagonal's 16 rows -- a contiguous 128-byte run -- instead
+// of every lane of a row storing to a different diagonal.  The level image
+// f1 is float64 data in the reference; P2 marks a float32 warped image
+// (every level below the top).
 constexpr int kTensorRows = 16;
+constexpr int kTensorDiags = kTensorRows + 63;  // diagonals a tile touches
 
 template <int KIND, bool P2>
 __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
@@ -622,28 +625,44 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
     double *__restrict__ cuv, size_t fstride, int T)
 {
+    __shared__ double stage[kTensorRows][64];
     const int tiles_x = (w + 63) / 64, tiles_y = (h + kTensorRows - 1) / kTensorRows;
     const unsigned nb = (unsigned)tiles_x * tiles_y * T;
     const unsigned L = xcd_block(blockIdx.x, nb);
     const int t = L / (tiles_x * tiles_y), rem = L % (tiles_x * tiles_y);
-    const int r = (rem / tiles_x) * kTensorRows + threadIdx.y;
-    const int c = (rem % tiles_x) * 64 + threadIdx.x;
-    if (r >= h || c >= w) return;
-    size_t hw = (size_t)h * w, q = (size_t)(r + c) * hs + r;
+    const int r0 = (rem / tiles_x) * kTensorRows, c0 = (rem % tiles_x) * 64;
+    const int r = r0 + threadIdx.y, c = c0 + threadIdx.x;
+    const bool ok = r < h && c < w;  // every thread takes part in the staging barriers
+    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const size_t hw = (size_t)h * w, d0 = (size_t)(r0 + c0);
+    // one output plane of the tile: cell (r, c) -> slot (r + c) * hs + r,
+    // stored diagonal by diagonal (slot s = diagonal j of the tile, row i)
+    auto put = [&](double val, double *dst) {
+        stage[threadIdx.y][threadIdx.x] = val;
+        __syncthreads();
+        for (int s = tid; s < kTensorDiags * kTensorRows; s += 64 * kTensorRows) {
+            const int j = s / kTensorRows, i = s % kTensorRows, cl = j - i;
+            if (cl >= 0 && cl < 64 && r0 + i < h && c0 + cl < w)
+                dst[(d0 + j) * hs + r0 + i] = stage[i][cl];
+        }
+        __syncthreads();
+    };
     double *Jt = J + (size_t)t * fstride;
     for (int ch = 0; ch < C; ++ch) {
-        TensorOut o = tensor_cell<KIND, false, P2>(f1 + (size_t)t * f1_fstride + ch * hw,
-                                                  f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx, eps);
-        Jt[(size_t)(0 * C + ch) * plane + q] = o.J11;
-        Jt[(size_t)(1 * C + ch) * plane + q] = o.J22;
-        Jt[(size_t)(2 * C + ch) * plane + q] = o.J33;
-        Jt[(size_t)(3 * C + ch) * plane + q] = o.J12;
-        Jt[(size_t)(4 * C + ch) * plane + q] = o.J13;
-        Jt[(size_t)(5 * C + ch) * plane + q] = o.J23;
+        TensorOut o{};
+        if (ok)
+            o = tensor_cell<KIND, false, P2>(f1 + (size_t)t * f1_fstride + ch * hw, f2 + ((size_t)t * C + ch) * hw,
+                                             h, w, r, c, hy, hx, eps);
+        put(o.J11, Jt + (size_t)(0 * C + ch) * plane);
+        put(o.J22, Jt + (size_t)(1 * C + ch) * plane);
+        put(o.J33, Jt + (size_t)(2 * C + ch) * plane);
+        put(o.J12, Jt + (size_t)(3 * C + ch) * plane);
+        put(o.J13, Jt + (size_t)(4 * C + ch) * plane);
+        put(o.J23, Jt + (size_t)(5 * C + ch) * plane);
     }
     double *ct = cuv + (size_t)t * fstride;
-    ct[q] = smooth_rhs(u + t * uv_stride, h, w, r, c, ax, ay);
-    ct[plane + q] = smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay);
+    put(ok ? smooth_rhs(u + t * uv_stride, h, w, r, c, ax, ay) : 0.0, ct);
+    put(ok ? smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay) : 0.0, ct + plane);
 }
 
 hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_stride, const float *f2,
