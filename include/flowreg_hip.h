@@ -116,6 +116,15 @@ int frh_warp_cubic_ex(const void *f2, const void *f1, int img_f64, int acc_f64, 
                       int T, int C, int H, int W, const double *u, const double *v, float *out,
                       void *stream);
 
+/* imregister_wrapper(f2, u, v, f1, interpolation_method="linear")
+ * (core/warping.py:160-234 with cv2.INTER_LINEAR): the same arguments and
+ * image-type rules as frh_warp_cubic_ex, bilinear taps on the 1/32-px grid
+ * with BORDER_REPLICATE.  Used by the cross-correlation prealignment
+ * (motion_correction/parallelization/sequential.py:97-130). */
+int frh_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc_f64, int f1_per_frame,
+                       int T, int C, int H, int W, const double *u, const double *v, float *out,
+                       void *stream);
+
 /* Data terms (core/optical_flow.py:446-473, _resolve_motion_tensor_func). */
 #define FRH_TENSOR_GC 0   /* "gc" / "gradient": get_motion_tensor_gc, :73-160 */
 #define FRH_TENSOR_GRAY 1 /* "gray" / "brightness": get_motion_tensor_gray, :163-228 */

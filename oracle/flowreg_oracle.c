@@ -25,9 +25,10 @@
  *                           identity at scale 1
  *   or_resize_tab           scale-1 identity, ramp reproduction, constants
  *   or_warp_cubic(64)       reference warp tests (zero / integer shifts,
- *                           out-of-bounds <- fixed), ramp reproduction with
- *                           1/32-px quantization; the 64F sums against a
- *                           float128 evaluation of the same taps
+ *   or_warp_linear(64)      out-of-bounds <- fixed), ramp reproduction with
+ *                           1/32-px quantization (bilinear: exact on a
+ *                           ramp at the quantized position); the 64F sums
+ *                           against a float128 evaluation of the same taps
  *   or_motion_tensor_gc     reference known answer J11=0.8, identical frames
  *                           give J33=J13=J23=0, quadratic closed forms
  *   or_compute_flow         f1=f2 gives zero flow; a_data=1 converges to the
@@ -304,6 +305,80 @@ void or_warp_cubic64(const double *f2, const double *f1, int H, int W, int C,
                         for (int j = 0; j < 4; ++j)
                             sum += f2[((size_t)ys[i] * W + xs[j]) * C + c] * (double)w[i * 4 + j];
                 }
+                out[p * C + c] = (float)sum;
+            }
+        }
+}
+
+/* core/warping.py:160-234 with interpolation_method="linear": cv2.remap
+ * INTER_LINEAR + BORDER_REPLICATE, restated like the cubic path above
+ * (parity with OpenCV itself unpinned).  The map is rounded to 1/32 px; the
+ * tap origin is (X >> 5, Y >> 5) and the 2x2 weights are products
+ * wy[i] * wx[j] of the float table {1 - k/32, k/32} (all exact).  A pixel
+ * whose origin satisfies sx < W-1 and sy < H-1 sums its four taps left to
+ * right; elsewhere the taps are clamped to the image (BORDER_REPLICATE) and
+ * summed in the same order.  Float sums for float32 images, double sums of
+ * double products for float64 images. */
+static int lin_taps(int x, int y, int H, int W, double u, double v, int xs[2], int ys[2], float w[4])
+{
+    float mx = (float)((double)x + u);
+    float my = (float)((double)y + v);
+    if ((mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H)) return 0;
+    float cx = mx > (float)(W - 1) ? (float)(W - 1) : mx;
+    float cy = my > (float)(H - 1) ? (float)(H - 1) : my;
+    int X = (int)lrintf(cx * 32.f), Y = (int)lrintf(cy * 32.f);
+    int sx = X >> 5, sy = Y >> 5;
+    float fx = (float)(X & 31) * (1.f / 32), fy = (float)(Y & 31) * (1.f / 32);
+    float wx[2] = {1.f - fx, fx}, wy[2] = {1.f - fy, fy};
+    for (int i = 0; i < 2; ++i)
+        for (int j = 0; j < 2; ++j) w[i * 2 + j] = wy[i] * wx[j];
+    for (int i = 0; i < 2; ++i) {
+        int a = sx + i, b = sy + i;
+        xs[i] = a < 0 ? 0 : (a >= W ? W - 1 : a);
+        ys[i] = b < 0 ? 0 : (b >= H ? H - 1 : b);
+    }
+    return 1;
+}
+
+void or_warp_linear(const float *f2, const float *f1, int H, int W, int C,
+                    const double *u, const double *v, float *out)
+{
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t p = (size_t)y * W + x;
+            int xs[2], ys[2];
+            float w[4];
+            if (!lin_taps(x, y, H, W, u[p], v[p], xs, ys, w)) {
+                for (int c = 0; c < C; ++c) out[p * C + c] = f1[p * C + c];
+                continue;
+            }
+            for (int c = 0; c < C; ++c) {
+                const float *S = f2 + c;
+                float sum = S[((size_t)ys[0] * W + xs[0]) * C] * w[0] + S[((size_t)ys[0] * W + xs[1]) * C] * w[1] +
+                            S[((size_t)ys[1] * W + xs[0]) * C] * w[2] + S[((size_t)ys[1] * W + xs[1]) * C] * w[3];
+                out[p * C + c] = sum;
+            }
+        }
+}
+
+void or_warp_linear64(const double *f2, const double *f1, int H, int W, int C,
+                      const double *u, const double *v, float *out)
+{
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t p = (size_t)y * W + x;
+            int xs[2], ys[2];
+            float w[4];
+            if (!lin_taps(x, y, H, W, u[p], v[p], xs, ys, w)) {
+                for (int c = 0; c < C; ++c) out[p * C + c] = (float)f1[p * C + c];
+                continue;
+            }
+            for (int c = 0; c < C; ++c) {
+                const double *S = f2 + c;
+                double sum = S[((size_t)ys[0] * W + xs[0]) * C] * (double)w[0] +
+                             S[((size_t)ys[0] * W + xs[1]) * C] * (double)w[1] +
+                             S[((size_t)ys[1] * W + xs[0]) * C] * (double)w[2] +
+                             S[((size_t)ys[1] * W + xs[1]) * C] * (double)w[3];
                 out[p * C + c] = (float)sum;
             }
         }

@@ -67,6 +67,10 @@ def lib(precision: str = "f64") -> ctypes.CDLL:
     L.or_warp_cubic64.argtypes = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
                                   _f64p, _f64p, _f32p]
     L.or_warp_cubic64.restype = None
+    L.or_warp_linear.argtypes = L.or_warp_cubic.argtypes
+    L.or_warp_linear.restype = None
+    L.or_warp_linear64.argtypes = L.or_warp_cubic64.argtypes
+    L.or_warp_linear64.restype = None
     tens = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double]
     L.or_motion_tensor_gc.argtypes = tens + [ctypes.c_int, ctypes.c_int] + [rp] * 6
@@ -154,11 +158,14 @@ def resize(img, size, sigma_coeff: float = 0.6):
 
 
 # -------------------------------------------------------------------- warp
-def imregister_wrapper(f2_level, u, v, f1_level):
-    """core/warping.py:160-234, cubic interpolation (cv2.remap restated):
-    float64 images take OpenCV's 64F path (double sums), other images are
-    converted to float32 and take its 32F path (float sums); the result is
-    float32."""
+def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
+    """core/warping.py:160-234, cubic or linear interpolation (cv2.remap
+    restated): float64 images take OpenCV's 64F path (double sums), other
+    images are converted to float32 and take its 32F path (float sums); the
+    result is float32."""
+    method = interpolation_method.lower()
+    if method not in ("cubic", "linear"):
+        raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
     squeeze = f2_level.ndim == 2
     f64 = np.asarray(f2_level).dtype == np.float64
     dt = np.float64 if f64 else np.float32
@@ -166,7 +173,10 @@ def imregister_wrapper(f2_level, u, v, f1_level):
     f1 = np.ascontiguousarray(f1_level if not squeeze else f1_level[:, :, None], dtype=dt)
     H, W, C = f2.shape
     out = np.empty((H, W, C), np.float32)
-    fn = lib().or_warp_cubic64 if f64 else lib().or_warp_cubic
+    if method == "cubic":
+        fn = lib().or_warp_cubic64 if f64 else lib().or_warp_cubic
+    else:
+        fn = lib().or_warp_linear64 if f64 else lib().or_warp_linear
     fn(f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64),
        out)
     return out[:, :, 0] if squeeze else out

@@ -59,6 +59,8 @@ SIGNATURES = {
                                c_void_p, c_double, c_double, c_void_p, c_void_p]),
     "frh_warp_cubic_ex": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
                                   c_void_p, c_void_p, c_void_p, c_void_p]),
+    "frh_warp_linear_ex": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_int, c_int,
+                                   c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_prefilter_ex": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
     "frh_motion_tensor": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_double, c_double,

@@ -63,7 +63,15 @@ class RegistrationOptions:
     update_initialization_w: bool = True
     update_reference: bool = False
     constancy_assumption: str = "gc"
+    cc_initialization: bool = False
+    cc_hw: object = 256
+    cc_up: int = 1
     extra: dict = field(default_factory=dict)
+
+    def __post_init__(self):
+        # OF_options.py's validation of the prealignment fields
+        if int(self.cc_up) < 1:
+            raise ValueError("cc_up must be >= 1")
 
     @property
     def effective_min_level(self) -> int:
@@ -90,7 +98,7 @@ class RegistrationOptions:
         kw = {}
         for f in ("alpha", "weight", "levels", "min_level", "eta", "update_lag", "iterations",
                   "a_smooth", "a_data", "sigma", "buffer_size", "update_initialization_w",
-                  "update_reference"):
+                  "update_reference", "cc_initialization", "cc_hw", "cc_up"):
             if hasattr(options, f):
                 kw[f] = getattr(options, f)
         for f in ("quality_setting", "channel_normalization", "interpolation_method",
@@ -207,8 +215,8 @@ class MotionCorrector:
         self._ref_raw_dev = {}
         self.w_init = None
         self.mean_disp, self.max_disp, self.mean_div, self.mean_translation = [], [], [], []
-        if str(self.opt.interpolation_method).lower() != "cubic":
-            raise ValueError("flowreg_hip: only cubic registration warps are implemented")
+        if str(self.opt.interpolation_method).lower() not in ("cubic", "linear"):
+            raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
 
     def _preprocess(self, frames, norm_ref, out_dtype=torch.float32):
         return preprocess(frames, normalization_ref=norm_ref, sigma=self.sigma[0],
@@ -225,11 +233,13 @@ class MotionCorrector:
             self._ref_raw_dev[dtype] = torch.as_tensor(self.reference_raw, dtype=dtype, device=self.device)
         return self._ref_raw_dev[dtype]
 
-    def _register(self, frames, frames_proc, w_init):
+    def _register(self, frames, frames_proc, w_init, prealign=True):
         fp = self._flow_params()
         term = _resolve_const_assumption(fp.get("const_assumption", "gc"))
+        cc = (self.opt.cc_hw, int(self.opt.cc_up)) if self.opt.cc_initialization and prealign else None
         return self.executor.register_device(frames, frames_proc, self._reference_raw(frames.dtype),
-                                             self.reference_proc, w_init, fp, term)
+                                             self.reference_proc, w_init, fp, term,
+                                             self.opt.interpolation_method, cc)
 
     def _launch(self, frames):
         """GPU work of one batch (T, H, W, C) on the device: preprocess,
@@ -244,7 +254,10 @@ class MotionCorrector:
         H, W, _ = self.shape
         if self.w_init is None:
             n_init = min(22, frames.shape[0])
-            _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)))
+            # compensate_recording.py:645-678: more than 4 frames go through
+            # the executor (with the prealignment), fewer through
+            # get_displacement directly
+            _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)), prealign=n_init > 4)
             self.w_init = np.mean(w0.cpu().numpy(), axis=0)
         cur = self.w_init if self.opt.update_initialization_w else np.zeros_like(self.w_init)
         reg, flow = self._register(frames, bp, cur)
@@ -313,7 +326,8 @@ class MotionCorrector:
     def _update_reference(self, bp64, w):
         """compensate_recording.py:680-713: the processed reference becomes
         the mean of the last min(100, T) preprocessed float64 frames, each
-        warped (cubic, float64 images: cv2.remap's 64F path, float32 result)
+        warped (interpolation_method, float64 images: cv2.remap's 64F path,
+        float32 result)
         by its flow with the current processed reference as fill; the mean
         is float64, summed frame by frame as numpy's axis-0 mean does."""
         T = bp64.shape[0]
@@ -322,7 +336,7 @@ class MotionCorrector:
             return
         frames = bp64[T - n:].permute(0, 2, 3, 1)
         flows = torch.from_numpy(np.ascontiguousarray(w[T - n:])).to(frames.device)
-        reg = warp_frames(frames, flows, self.reference_proc)
+        reg = warp_frames(frames, flows, self.reference_proc, self.opt.interpolation_method)
         acc = reg[0].to(torch.float64)
         for k in range(1, n):
             acc = acc + reg[k].to(torch.float64)

@@ -253,6 +253,18 @@ int frh_warp_cubic_ex(const void *f2, const void *f1, int img_f64, int acc_f64, 
     return FRH_OK;
 }
 
+int frh_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc_f64, int f1_per_frame, int T,
+                       int C, int H, int W, const double *u, const double *v, float *out, void *stream)
+{
+    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_linear_ex: null pointer");
+    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_linear_ex: bad shape");
+    FRH_REQ(!img_f64 || acc_f64, "frh_warp_linear_ex: float64 images take cv2's 64F path (acc_f64 = 1)");
+    hipError_t e = launch_warp_linear_ex(f2, f1, img_f64 != 0, acc_f64 != 0, f1_per_frame, T, C, H, W, u, v,
+                                         (size_t)H * W, out, as_stream(stream));
+    FRH_HIP(e, "linear warp kernel");
+    return FRH_OK;
+}
+
 int frh_motion_tensor(int kind, const float *f1, const float *f2, int h, int w, double hy, double hx,
                       double eps, int f1_f32, int f2_f32, double *J11, double *J22, double *J33,
                       double *J12, double *J13, double *J23, void *stream)

@@ -52,6 +52,9 @@ hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T
                        int W, const float *u, const float *v, size_t uv_frame_stride,
                        double h_col, double h_row, int acc64, float *out, hipStream_t st);
 // float64 displacements; images float64 (img_f64, double sums) or float32
+hipError_t launch_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame,
+                                 int T, int C, int H, int W, const double *u, const double *v,
+                                 size_t uv_frame_stride, float *out, hipStream_t st);
 hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame, int T,
                           int C, int H, int W, const double *u, const double *v, size_t uv_frame_stride,
                           float *out, hipStream_t st);

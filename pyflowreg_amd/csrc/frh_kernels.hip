@@ -396,6 +396,65 @@ hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64
     return hipGetLastError();
 }
 
+// imregister_wrapper(..., interpolation_method="linear"): cv2.remap with
+// INTER_LINEAR + BORDER_REPLICATE.  The map handling is that of k_warp (the
+// float32 map from a float64 sum, out-of-bounds test on the unclipped map,
+// clip, 1/32-px rounding); the tap origin is (X >> 5, Y >> 5) and the 2x2
+// weights are wy[i] * wx[j] of the float table {1 - k/32, k/32}.  Origins
+// with sx < W - 1 and sy < H - 1 read their 2x2 taps directly, the others
+// clamp them to the image; both sum the four products left to right in Acc
+// (float for float32 images, double for float64 images).
+template <typename U, typename I, typename Acc>
+__global__ void k_warp_linear(const I *__restrict__ f2, const I *__restrict__ f1, int f1_per_frame, int C,
+                              int H, int W, const U *__restrict__ u, const U *__restrict__ v,
+                              size_t uv_stride, float *__restrict__ out)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, t = blockIdx.z;
+    if (x >= W) return;
+    const size_t HW = (size_t)H * W, p = (size_t)y * W + x;
+    const I *F2 = f2 + (size_t)t * C * HW;
+    const I *F1 = f1 + (f1_per_frame ? (size_t)t * C * HW : 0);
+    float *O = out + (size_t)t * C * HW;
+    const float mx = (float)((double)x + (double)u[t * uv_stride + p]);
+    const float my = (float)((double)y + (double)v[t * uv_stride + p]);
+    if ((mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H)) {
+        for (int c = 0; c < C; ++c) O[c * HW + p] = (float)F1[c * HW + p];
+        return;
+    }
+    const float cx = mx > (float)(W - 1) ? (float)(W - 1) : mx;
+    const float cy = my > (float)(H - 1) ? (float)(H - 1) : my;
+    const int X = (int)rintf(cx * 32.f), Y = (int)rintf(cy * 32.f);
+    const int sx = X >> 5, sy = Y >> 5;
+    const float fx = (float)(X & 31) * (1.f / 32), fy = (float)(Y & 31) * (1.f / 32);
+    const float wx0 = 1.f - fx, wy0 = 1.f - fy;
+    const float w0 = wy0 * wx0, w1 = wy0 * fx, w2 = fy * wx0, w3 = fy * fx;
+    const int x0 = clampi(sx, 0, W - 1), x1 = clampi(sx + 1, 0, W - 1);
+    const int y0 = clampi(sy, 0, H - 1), y1 = clampi(sy + 1, 0, H - 1);
+    for (int c = 0; c < C; ++c) {
+        const I *S = F2 + c * HW;
+        const Acc sum = (Acc)S[(size_t)y0 * W + x0] * (Acc)w0 + (Acc)S[(size_t)y0 * W + x1] * (Acc)w1 +
+                        (Acc)S[(size_t)y1 * W + x0] * (Acc)w2 + (Acc)S[(size_t)y1 * W + x1] * (Acc)w3;
+        O[c * HW + p] = (float)sum;
+    }
+}
+
+hipError_t launch_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame,
+                                 int T, int C, int H, int W, const double *u, const double *v,
+                                 size_t uv_frame_stride, float *out, hipStream_t st)
+{
+    const dim3 g((W + 63) / 64, H, T), b(64);
+    if (img_f64)
+        hipLaunchKernelGGL((k_warp_linear<double, double, double>), g, b, 0, st, (const double *)f2,
+                           (const double *)f1, f1_per_frame, C, H, W, u, v, uv_frame_stride, out);
+    else if (acc64)
+        hipLaunchKernelGGL((k_warp_linear<double, float, double>), g, b, 0, st, (const float *)f2,
+                           (const float *)f1, f1_per_frame, C, H, W, u, v, uv_frame_stride, out);
+    else
+        hipLaunchKernelGGL((k_warp_linear<double, float, float>), g, b, 0, st, (const float *)f2,
+                           (const float *)f1, f1_per_frame, C, H, W, u, v, uv_frame_stride, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- motion tensor
 // core/optical_flow.py:73-345 (get_motion_tensor_gc / _gray / _cs), one
 // cell of one channel.  Symmetric 1-px pads are clamped reads; np.gradient

@@ -40,12 +40,17 @@ def _device():
     return torch.device("cuda", torch.cuda.current_device())
 
 
-def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor) -> torch.Tensor:
-    """core/warping.py:160-234 (cubic) for a batch on the GPU.
+def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor,
+                interpolation_method: str = "cubic") -> torch.Tensor:
+    """core/warping.py:160-234 (cubic or linear) for a batch on the GPU.
     frames (T, H, W, C) float32 (cv2.remap's 32F path) or float64 (its 64F
     path, double sums), flow (T, H, W, 2) of any float dtype (the map
     float32(x + u) is formed in float64), fill (H, W, C) -> (T, H, W, C)
     float32."""
+    method = str(interpolation_method).lower()
+    if method not in ("cubic", "linear"):
+        raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
+    name = "frh_warp_cubic_ex" if method == "cubic" else "frh_warp_linear_ex"
     T, H, W, C = frames.shape
     f64 = frames.dtype == torch.float64
     f2 = frames.permute(0, 3, 1, 2).contiguous()
@@ -53,8 +58,8 @@ def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor) ->
     u = flow[..., 0].to(torch.float64).contiguous()
     v = flow[..., 1].to(torch.float64).contiguous()
     out = torch.empty(f2.shape, dtype=torch.float32, device=f2.device)
-    check(lib().frh_warp_cubic_ex(ptr(f2), ptr(f1), int(f64), int(f64), 0, T, C, H, W, ptr(u), ptr(v), ptr(out),
-                                  stream_handle()), "frh_warp_cubic_ex")
+    check(getattr(lib(), name)(ptr(f2), ptr(f1), int(f64), int(f64), 0, T, C, H, W, ptr(u), ptr(v), ptr(out),
+                               stream_handle()), name)
     return out.permute(0, 2, 3, 1)
 
 
@@ -144,17 +149,46 @@ class HipBatchedExecutor:
             self._key = key
         return self._solver
 
+    def _prealigned_flows(self, frames_proc, reference_proc, w_init, fp, term, cc):
+        """The cc_initialization steps of parallelization/sequential.py:97-140
+        for the whole batch: warp every frame by w_init (linear), estimate
+        the rigid residual against the reference by phase correlation, warp
+        the frames again by w_init + residual, solve the non-rigid residual
+        from zero and return (w_init + residual + flow) in float32."""
+        from .xcorr import estimate_rigid_xcorr_batch
+        T, H, W, C = frames_proc.shape
+        hw, up = cc
+        rp = _to_device32(reference_proc).to(frames_proc.device).contiguous()
+        wi = _to_device32(w_init).to(frames_proc.device).contiguous()
+        partial = warp_frames(frames_proc, wi[None].expand(T, H, W, 2), rp, "linear")
+        ref_cc = rp[..., 0] if C == 1 else rp
+        mov_cc = partial[..., 0] if C == 1 else partial
+        target = (hw, hw) if isinstance(hw, int) else hw
+        w_cross = estimate_rigid_xcorr_batch(ref_cc, mov_cc, target_hw=target, up=up, weight=fp.get("weight"))
+        w_comb = wi[None] + w_cross[:, None, None, :]
+        aligned = warp_frames(frames_proc, w_comb, rp, "linear")
+        solver = self._get_solver(H, W, C, T, fp, term)
+        solver.set_reference(rp, fp.get("weight"), None)
+        w_res = solver.run(aligned.contiguous(), out_dtype=torch.float64)
+        return (w_comb.to(torch.float64) + w_res).to(torch.float32)
+
     def register_device(self, frames: torch.Tensor, frames_proc: torch.Tensor, reference_raw: torch.Tensor,
-                        reference_proc, w_init, flow_params: dict, term: str = "gc"):
+                        reference_proc, w_init, flow_params: dict, term: str = "gc",
+                        interpolation_method: str = "cubic", cc=None):
         """The batch's registration with every array on the device: frames
         (T, H, W, C) raw (float32, or float64 for cv2's 64F warp), frames_proc
         (T, H, W, C) float32 preprocessed, reference_raw (H, W, C) in the
         frames' dtype, reference_proc (H, W, C) and w_init (H, W, 2) (cast to
-        float32 as the executors do).  Returns (registered float32, flow
-        float32) device tensors."""
+        float32 as the executors do).  cc = (cc_hw, cc_up) runs the
+        cross-correlation prealignment first.  Returns (registered float32,
+        flow float32) device tensors."""
         T, H, W, C = frames.shape
         fp = flow_params
-        if fp.get("gnc_schedule") is not None:
+        if cc is not None:
+            if fp.get("gnc_schedule") is not None:
+                raise ValueError("hip_batched: cc_initialization with a GNC schedule is not implemented")
+            flow = self._prealigned_flows(frames_proc, reference_proc, w_init, fp, term, cc)
+        elif fp.get("gnc_schedule") is not None:
             # GNC stages relinearise on the host loop (pyflowreg_amd.stage_loop),
             # one frame at a time, as the sequential executor does
             from .optical_flow import get_displacement
@@ -169,7 +203,7 @@ class HipBatchedExecutor:
             solver = self._get_solver(H, W, C, T, fp, term)
             solver.set_reference(_to_device32(reference_proc), fp.get("weight"), _to_device32(w_init))
             flow = solver.run(frames_proc, out_dtype=torch.float32)
-        return warp_frames(frames, flow, reference_raw), flow
+        return warp_frames(frames, flow, reference_raw, interpolation_method), flow
 
     def process_batch(self, batch: np.ndarray, batch_proc: np.ndarray, reference_raw: np.ndarray,
                       reference_proc: np.ndarray, w_init: np.ndarray,
@@ -181,13 +215,14 @@ class HipBatchedExecutor:
         """parallelization/base.py:76-110.  Returns (registered (T, H, W, C)
         in batch's dtype, flow (T, H, W, 2) float32)."""
         flow_params_all = dict(kwargs.get("flow_params", {}))
+        cc = None
         if bool(flow_params_all.get("cc_initialization", False)):
-            raise ValueError("hip_batched: cc_initialization is not implemented")
+            cc = (flow_params_all.get("cc_hw", 256), int(flow_params_all.get("cc_up", 1)))
         fp = {k: v for k, v in flow_params_all.items()
               if k not in ("cc_initialization", "cc_hw", "cc_up")}
         term = _resolve_const_assumption(fp.get("const_assumption", "gc"))
-        if str(interpolation_method).lower() != "cubic":
-            raise ValueError("hip_batched: only cubic registration warps are implemented")
+        if str(interpolation_method).lower() not in ("cubic", "linear"):
+            raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
         as4 = lambda x: x[..., None] if x.ndim == 3 else x  # noqa: E731
         as3 = lambda x: x[..., None] if x.ndim == 2 else x  # noqa: E731
         batch = np.ascontiguousarray(as4(np.asarray(batch)))
@@ -199,7 +234,8 @@ class HipBatchedExecutor:
         wdt = np.float64 if batch.dtype == np.float64 else np.float32
         reg, flow = self.register_device(
             torch.from_numpy(batch.astype(wdt, copy=False)).to(dev), torch.from_numpy(bp).to(dev),
-            torch.from_numpy(np.asarray(as3(np.asarray(reference_raw)), dtype=wdt)).to(dev), rp, wi, fp, term)
+            torch.from_numpy(np.asarray(as3(np.asarray(reference_raw)), dtype=wdt)).to(dev), rp, wi, fp, term,
+            interpolation_method, cc)
         registered = reg.cpu().numpy().astype(batch.dtype, copy=False)
         flows = flow.cpu().numpy()
         if progress_callback is not None:

@@ -9,7 +9,8 @@ Per frame (flow_reg_live.py:216-323):
   sum from the newest frame back);
 * solve get_displacement(reference_proc, frame_proc, uv=last_flow) with a
   one-frame ``DisplacementSolver`` replayed as a hipGraph;
-* warp the raw frame with the raw reference as fill;
+* warp the raw frame with the raw reference as fill (cubic or linear, per
+  ``interpolation_method``);
 * every ``reference_update_interval`` frames, mix the warped processed frame
   into the processed reference and the warped raw frame into the raw
   reference with weight ``reference_update_weight`` (numpy's weak-scalar
@@ -201,9 +202,10 @@ class FlowRegLive:
         flow = solver.run(proc.to(torch.float32)[None], out_dtype=torch.float64)[0]
         self.last_flow = flow
         fwarp = fr.to(torch.float64) if fr.dtype == torch.float64 else fr.to(torch.float32)
-        registered = warp_frames(fwarp[None], flow[None], self.reference_raw)[0]
+        interp = self.options.interpolation_method
+        registered = warp_frames(fwarp[None], flow[None], self.reference_raw, interp)[0]
         if self.frame_count % self.reference_update_interval == 0:
-            reg_proc = warp_frames(proc[None], flow[None], self.reference_proc)[0]
+            reg_proc = warp_frames(proc[None], flow[None], self.reference_proc, interp)[0]
             wgt = self.reference_update_weight
             # numpy's rules, which torch shares: a Python float times a
             # float32 array is a float32 product, float64 + float32 is float64

@@ -523,16 +523,15 @@ def imresize_fused_gauss_cubic(img, size, sigma_coeff=0.6, per_axis=False):
 
 
 def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
-    """core/warping.py:160-234 (cubic) on the GPU; numpy in, numpy (float32)
-    out.  u, v are the displacements in level pixels (the reference's
-    u / h_col, v / h_row) of any dtype: the map float32(x + u) is formed in
-    float64, as numpy forms grid + u.  cv2.remap's arithmetic follows the
-    image dtype: float64 images (get_displacement's level images, the
-    preprocessed frames of update_reference) sum in float64, other images
+    """core/warping.py:160-234 (cubic or linear) on the GPU; numpy in, numpy
+    (float32) out.  u, v are the displacements in level pixels (the
+    reference's u / h_col, v / h_row) of any dtype: the map float32(x + u) is
+    formed in float64, as numpy forms grid + u.  cv2.remap's arithmetic
+    follows the image dtype: float64 images (get_displacement's level images,
+    the preprocessed frames of update_reference) sum in float64, other images
     are converted to float32 and sum in float32."""
-    if interpolation_method.lower() != "cubic":
-        if interpolation_method.lower() == "linear":
-            raise ValueError("flowreg_hip: linear warp is not implemented")
+    method = interpolation_method.lower()
+    if method not in ("cubic", "linear"):
         raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
     squeeze = np.asarray(f2_level).ndim == 2 if not isinstance(f2_level, torch.Tensor) else f2_level.ndim == 2
     f64 = f2_level.dtype in (np.float64, torch.float64)
@@ -548,8 +547,9 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
         return torch.as_tensor(np.asarray(a).astype(np.float64), device=dev).contiguous()
     uu, vv = disp(u), disp(v)
     out = torch.empty((C, H, W), dtype=torch.float32, device=dev)
-    check(lib().frh_warp_cubic_ex(ptr(f2), ptr(f1), int(f64), int(f64), 0, 1, C, H, W, ptr(uu), ptr(vv),
-                                  ptr(out), stream_handle()), "frh_warp_cubic_ex")
+    name = "frh_warp_cubic_ex" if method == "cubic" else "frh_warp_linear_ex"
+    check(getattr(lib(), name)(ptr(f2), ptr(f1), int(f64), int(f64), 0, 1, C, H, W, ptr(uu), ptr(vv),
+                               ptr(out), stream_handle()), name)
     res = out.permute(1, 2, 0)
     res = res[:, :, 0] if squeeze else res
     if isinstance(f2_level, torch.Tensor):
