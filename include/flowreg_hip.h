@@ -147,7 +147,10 @@ int frh_motion_tensor(int kind, const float *f1, const float *f2, int h, int w, 
  * float64; a_data (K) float64 host memory; hx scales column differences,
  * hy rows.  du, dv: (m, n) float64 outputs (interior updated, ring = the
  * last boundary copy).  Sweeps run in the reference's column-major
- * lexicographic order (a diagonal wavefront on the GPU). */
+ * lexicographic order (a diagonal wavefront on the GPU).  a_smooth != 1
+ * refreshes the smoothness weights psi_s (nonlinearity_smoothness_2d,
+ * :69-155) with the data term's psi and weights each edge by
+ * 0.5 (psi_s[c] + psi_s[nb]) alpha / h^2 (:277-316). */
 int frh_level_solver(const double *J11, const double *J22, const double *J33,
                      const double *J12, const double *J13, const double *J23,
                      const double *weight, const double *u, const double *v, int m, int n,

@@ -601,8 +601,11 @@ static void set_boundary_2d(real *f, int m, int n)
     }
 }
 
+static inline real cf_pow(real x, real b);
+
 /* core/level_solver.py:69-155 (nonlinearity_smoothness_2d), eps = 1e-5,
- * one-sided differences on the outer ring. */
+ * one-sided differences on the outer ring; the power with frh_det_pow, as
+ * for the data term's psi (pinned against numpy's pow and np.gradient). */
 static void psi_smooth_2d(real *psi_s, const real *u, const real *du, const real *v, const real *dv,
                           int m, int n, real a, real hx, real hy)
 {
@@ -625,9 +628,16 @@ static void psi_smooth_2d(real *psi_s, const real *u, const real *du, const real
             }
             real t = ux * ux + uy * uy + vx * vx + vy * vy;
             if (t < 0) t = 0;
-            psi_s[q] = a * pow(t + (real)1e-5, a - 1);
+            psi_s[q] = a * cf_pow(t + (real)1e-5, a - 1);
         }
     free(uf); free(vf);
+}
+
+/* Exported for the tests: psi_smooth_2d over (m, n) padded arrays. */
+void or_psi_smooth(const real *u, const real *du, const real *v, const real *dv, int m, int n,
+                   double a, double hx, double hy, real *psi_s)
+{
+    psi_smooth_2d(psi_s, u, du, v, dv, m, n, (real)a, (real)hx, (real)hy);
 }
 
 /* One SOR update of pixel (j, i); core/level_solver.py:262-364. */

@@ -83,6 +83,9 @@ def lib(precision: str = "f64") -> ctypes.CDLL:
         ctypes.c_double, ctypes.c_double, ctypes.c_int, ctypes.c_int, _f64p,
         ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double, rp, rp, ctypes.c_int]
     L.or_compute_flow.restype = None
+    L.or_psi_smooth.argtypes = [rp] * 4 + [ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                           ctypes.c_double, rp]
+    L.or_psi_smooth.restype = None
     L.or_cubic_resize_cv.argtypes = [rp, ctypes.c_int, ctypes.c_int, ctypes.c_int, rp,
                                      ctypes.c_int, ctypes.c_int]
     L.or_cubic_resize_cv.restype = None
@@ -286,6 +289,16 @@ def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, 
                      update_lag, a_data, a_smooth, hx, hy, precision=precision, order=order,
                      gnc_beta=gnc_beta)
     return r[:, :, 0], r[:, :, 1]
+
+
+def psi_smooth(u, du, v, dv, a, hx, hy):
+    """core/level_solver.py:69-155 (nonlinearity_smoothness_2d) on padded
+    (m, n) float64 arrays."""
+    arrs = [np.ascontiguousarray(x, dtype=np.float64) for x in (u, du, v, dv)]
+    m, n = arrs[0].shape
+    out = np.empty((m, n))
+    lib().or_psi_smooth(*arrs, m, n, float(a), float(hx), float(hy), out)
+    return out
 
 
 def cubic_resize_cv(w, m, n):

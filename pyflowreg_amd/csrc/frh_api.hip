@@ -5,6 +5,7 @@
 #include <string.h>
 
 #include <cmath>
+#include <algorithm>
 #include <string>
 #include <vector>
 
@@ -320,18 +321,20 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
     FRH_REQ(m - 2 <= 1024, "frh_level_solver: at most 1024 interior rows (got %d)", m - 2);
     FRH_REQ(iterations >= 0 && update_lag >= 1, "frh_level_solver: iterations >= 0, update_lag >= 1");
     FRH_REQ(hx > 0 && hy > 0, "frh_level_solver: hx, hy must be positive");
-    if (a_smooth != 1.0)
-        return fail(FRH_EUNSUP, "frh_level_solver: a_smooth != 1 is not implemented on the GPU");
     hipStream_t st = as_stream(stream);
     Skew s = make_skew(m - 2, n - 2);
     const double *Js[6] = {J11, J22, J33, J12, J13, J23};
-    size_t npl = (size_t)6 * K + K + 2 + 5 + 2 + 2;
+    // a_smooth != 1: nine coefficients per cell and psi_s on the padded grid
+    const bool nl = a_smooth != 1.0;
+    const size_t ncoef = nl ? 9 : 5, nps = nl ? (size_t)m * n : 0;
+    size_t npl = (size_t)6 * K + K + 2 + ncoef + 2 + 2;
     double *work = nullptr;
-    FRH_HIP(hipMallocAsync((void **)&work, sizeof(double) * npl * s.plane, st), "level solver scratch");
+    FRH_HIP(hipMallocAsync((void **)&work, sizeof(double) * (npl * s.plane + nps), st), "level solver scratch");
     double *J = work, *Wsk = J + 6 * K * s.plane, *cuv = Wsk + K * s.plane, *coef = cuv + 2 * s.plane;
-    double *duv = coef + 5 * s.plane, *ring = duv + 2 * s.plane;
+    double *duv = coef + ncoef * s.plane, *ring = duv + 2 * s.plane, *psis = work + npl * s.plane;
     double ax = alpha_x / (hx * hx), ay = alpha_y / (hy * hy);
     FRH_HIP(launch_fill_f64(work, npl * s.plane, 0.0, st), "level solver clear");
+    if (nl) FRH_HIP(launch_fill_f64(psis, nps, 1.0, st), "psi_s init");
     std::vector<double> am1(K);
     for (int k = 0; k < K; ++k) am1[k] = a_data[k] - 1.0;
     hipError_t e = hipSuccess;
@@ -339,12 +342,27 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
         e = launch_padded_to_skew_f64(Js[c], m, n, K, s, J + (size_t)c * K * s.plane, st);
     if (e == hipSuccess) e = launch_padded_to_skew_f64(weight, m, n, K, s, Wsk, st);
     if (e == hipSuccess) e = launch_cuv_padded(u, v, m, n, ax, ay, s, cuv, st);
-    std::vector<Pass> passes = plan_passes(iterations, update_lag, sweep_pmax(s.h));
+    const int pmax = nl ? sweep_pmax_nl(s.h) : sweep_pmax(s.h);
+    std::vector<Pass> passes = plan_passes(iterations, update_lag, pmax);
     size_t fstride = npl * s.plane;
     if (passes.empty() && e == hipSuccess) e = launch_fill_f64(duv, 4 * s.plane, 0.0, st);
-    const bool fused = fused_sweep_supported(K) && passes_fusable(passes);
+    const bool fused = !nl && fused_sweep_supported(K) && passes_fusable(passes);
     for (size_t k = 0; k < passes.size() && e == hipSuccess; ++k) {
         double *rg = k + 1 == passes.size() ? ring : nullptr;
+        if (nl) {
+            // every pass records the border cells' start values: the ring
+            // that the next psi_s refresh sees
+            const Pass &p = passes[k];
+            if (p.assemble && !p.psi_one)
+                e = launch_psi_smooth(nullptr, nullptr, 0, u, v, duv, ring, fstride, 1, s, k == 0 ? 1 : 0,
+                                      a_smooth, gnc_beta, hx, hy, psis, 0, st);
+            if (e == hipSuccess && p.assemble)
+                e = launch_assemble_nl(J, Wsk, nullptr, nullptr, 0, u, v, psis, 0, duv, coef, fstride, 1, K, s,
+                                       ax, ay, am1.data(), a_data, p.psi_one ? 1 : 0, k == 0 ? 1 : 0, gnc_beta,
+                                       st);
+            if (e == hipSuccess) e = launch_sweep_nl(p.nsweeps, coef, duv, ring, fstride, 1, s, k == 0 ? 1 : 0, st);
+            continue;
+        }
         if (fused) {
             e = launch_sweep_fused(passes[k].nsweeps, J, Wsk, cuv, duv, rg, fstride, 1, K, s, ax, ay,
                                    am1.data(), a_data, passes[k].psi_one ? 1 : 0, k == 0 ? 1 : 0,
@@ -390,6 +408,7 @@ struct frh_solver {
     float *f1lev = nullptr, *uvtop = nullptr, *tmp = nullptr, *f2l = nullptr, *f2w = nullptr;
     float *uvl = nullptr, *uvn = nullptr;
     double *Wlev = nullptr, *work = nullptr;
+    double *psis = nullptr;       // a_smooth != 1: psi_s per frame on the padded level grid
     int Wsum = 0;                 // sum of level widths (multi-level horizontal pass)
     std::vector<int> xoff;        // per level: column offset in the staged rows
     int32_t *hoff = nullptr, *hix = nullptr;
@@ -430,7 +449,7 @@ static void free_solver(frh_solver *s)
         (void)hipFree(L.up_ix); (void)hipFree(L.up_iy); (void)hipFree(L.up_wx); (void)hipFree(L.up_wy);
     }
     (void)hipFree(s->f1lev); (void)hipFree(s->uvtop); (void)hipFree(s->tmp); (void)hipFree(s->f2l); (void)hipFree(s->f2w);
-    (void)hipFree(s->uvl); (void)hipFree(s->uvn); (void)hipFree(s->Wlev); (void)hipFree(s->work);
+    (void)hipFree(s->uvl); (void)hipFree(s->uvn); (void)hipFree(s->Wlev); (void)hipFree(s->work); (void)hipFree(s->psis);
     (void)hipFree(s->hoff); (void)hipFree(s->hix); (void)hipFree(s->hwx); (void)hipFree(s->pyr);
     (void)hipFree(s->pyr1); (void)hipFree(s->f1l);
     (void)hipFree(s->wlow); (void)hipFree(s->wtmp);
@@ -489,7 +508,6 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
     FRH_REQ(d->H > 0 && d->W > 0 && d->C >= 1 && d->C <= 8, "frh_solver_create: bad geometry");
     FRH_REQ(d->n_levels >= 1 && d->max_batch >= 1 && d->max_batch <= 65535, "frh_solver_create: bad level count or batch");
     FRH_REQ(d->iterations >= 0 && d->update_lag >= 1, "frh_solver_create: iterations >= 0, update_lag >= 1");
-    if (d->a_smooth != 1.0) return fail(FRH_EUNSUP, "a_smooth != 1 is not implemented on the GPU");
     FRH_REQ(d->const_assumption >= FRH_TENSOR_GC && d->const_assumption <= FRH_TENSOR_CS,
             "frh_solver_create: unknown data term %d", d->const_assumption);
     frh_solver *s = new frh_solver();
@@ -519,7 +537,8 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
         D.sk = make_skew(L.h, L.w);
         D.f1_off = f1_total;
         D.W_off = W_total;
-        D.passes = plan_passes(d->iterations, d->update_lag, sweep_pmax(L.h));
+        D.passes = plan_passes(d->iterations, d->update_lag,
+                               d->a_smooth != 1.0 ? sweep_pmax_nl(L.h) : sweep_pmax(L.h));
         f1_total += (size_t)d->C * L.h * L.w;
         W_total += (size_t)d->C * D.sk.plane;
         if (D.sk.plane > plane_max) plane_max = D.sk.plane;
@@ -557,7 +576,9 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
         if (!rc) rc = upload(&s->hwx, hwx.data(), hwx.size(), "pyramid tables");
     }
     size_t T = (size_t)d->max_batch, C = d->C, HW = (size_t)d->H * d->W;
-    s->npl = 6 * C + 2 + 5 + 2;
+    const bool nl = d->a_smooth != 1.0;
+    // J, cu/cv, coefficients (9 with a_smooth != 1), du/dv, border ring
+    s->npl = 6 * C + 2 + (nl ? 9 : 5) + 2 + (nl ? 2 : 0);
     size_t cmax = C > 2 ? C : 2;
     if (!rc) rc = dev_alloc(&s->f1lev, f1_total, "reference levels");
     if (!rc) rc = dev_alloc(&s->Wlev, W_total, "weight levels");
@@ -568,6 +589,7 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
     if (!rc) rc = dev_alloc(&s->uvl, T * 2 * (size_t)hmax * wmax, "level flow");
     if (!rc) rc = dev_alloc(&s->uvn, T * 2 * (size_t)hmax * wmax, "next flow");
     if (!rc) rc = dev_alloc(&s->work, T * s->npl * plane_max, "solver workspace");
+    if (!rc && nl) rc = dev_alloc(&s->psis, T * (size_t)(hmax + 2) * (wmax + 2), "smoothness weights");
     if (!rc) rc = dev_alloc(&s->pyr, T * C * (size_t)d->H * s->Wsum, "pyramid rows");
     if (!rc && (hmax != d->H || wmax != d->W)) {
         rc = dev_alloc(&s->wlow, T * 2 * (size_t)hmax * wmax, "coarse flow");
@@ -671,7 +693,9 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         const size_t hw = (size_t)h * w;
         const Skew &sk = L.sk;
         const size_t fs = s->npl * sk.plane;
-        double *J = s->work, *cuv = J + 6 * C * sk.plane, *coef = cuv + 2 * sk.plane, *duv = coef + 5 * sk.plane;
+        const bool nl = s->a_smooth != 1.0;
+        double *J = s->work, *cuv = J + 6 * C * sk.plane, *coef = cuv + 2 * sk.plane;
+        double *duv = coef + (nl ? 9 : 5) * sk.plane, *ring = duv + 2 * sk.plane;
         {
             Scope sc(s, TM_RESIZE, st);
             FRH_HIP(launch_pyr_v(s->pyr, T * C, s->H, s->Wsum, s->xoff[li], s->f2l, h, w, L.img_iy,
@@ -714,8 +738,30 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         // inside the wavefront (measured on MI355X), so the pyramid solver
         // uses it; frh_level_solver keeps the fused form.
         const bool fused = false;
+        const size_t ps_stride = (size_t)(h + 2) * (w + 2);
+        if (nl && !L.passes.empty() && L.passes[0].psi_one)
+            FRH_HIP(launch_fill_f64(s->psis, (size_t)T * ps_stride, 1.0, st), "psi_s init");
         for (size_t k = 0; k < L.passes.size(); ++k) {
             const Pass &p = L.passes[k];
+            if (nl) {
+                // a_smooth != 1: psi_s refresh, nine-coefficient assembly and
+                // sweeps that record the border ring for the next refresh
+                if (p.assemble) {
+                    Scope sc(s, TM_ASSEMBLE, st);
+                    if (!p.psi_one)
+                        FRH_HIP(launch_psi_smooth(s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr, duv, ring, fs, T, sk,
+                                                  k == 0 ? 1 : 0, s->a_smooth, -1.0, L.h_col, L.h_row, s->psis,
+                                                  ps_stride, st), "smoothness weights");
+                    FRH_HIP(launch_assemble_nl(J, s->Wlev + L.W_off, s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr,
+                                               s->psis, ps_stride, duv, coef, fs, T, C, sk, L.axw, L.ayw,
+                                               s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
+                                               -1.0, st), "assemble (nonlinear smoothness)");
+                }
+                Scope sc(s, TM_SWEEP, st);
+                FRH_HIP(launch_sweep_nl(p.nsweeps, coef, duv, ring, fs, T, sk, k == 0 ? 1 : 0, st),
+                        "sor sweep (nonlinear smoothness)");
+                continue;
+            }
             if (fused) {
                 Scope sc(s, TM_SWEEP, st);
                 FRH_HIP(launch_sweep_fused(p.nsweeps, J, s->Wlev + L.W_off, cuv, duv, nullptr, fs, T, C, sk,

@@ -62,6 +62,23 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
                          int T, int C, int h, int w, double hy, double hx, double eps, const float *u,
                          const float *v, size_t uv_frame_stride, double ax, double ay, double *J,
                          double *cuv, size_t frame_stride, const Skew &s, hipStream_t st);
+// a_smooth != 1: psi_s over the padded grid at a refresh, nine-coefficient
+// assembly, sweeps with per-cell smoothness weights (P <= 2).
+hipError_t launch_psi_smooth(const float *u32, const float *v32, size_t fs32, const double *u64,
+                             const double *v64, const double *duv, const double *ring, size_t fstride,
+                             int T, const Skew &s, int zero, double a_smooth, double gnc_beta, double hx,
+                             double hy, double *psi_s, size_t ps_stride, hipStream_t st);
+hipError_t launch_assemble_nl(const double *J, const double *Wsk, const float *u32, const float *v32,
+                              size_t fs32, const double *u64, const double *v64, const double *psi_s,
+                              size_t ps_stride, const double *duv, double *coef, size_t frame_stride, int T,
+                              int C, const Skew &s, double ax, double ay, const double *a_data_minus1,
+                              const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                              hipStream_t st);
+hipError_t launch_sweep_nl(int P, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
+                           const Skew &s, int zero_init, hipStream_t st);
+// sweeps per pass with nine coefficients: 2 up to 512 rows, 1 above (fewer
+// registers per lane in the 1024-lane work-groups)
+inline int sweep_pmax_nl(int h) { return 64 * ((h + 63) / 64) <= 512 ? 2 : 1; }
 hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
                            const double *duv, double *coef, size_t frame_stride, int T, int C,
                            const Skew &s, double ax, double ay, const double *a_data_minus1,

@@ -830,13 +830,13 @@ inline AData make_adata(int C, const double *a_data, const double *a_data_minus1
 }
 
 // Coefficients of one cell from its raw data: J (6 components x C channels,
-// component-major), W (C), cu, cv, and the cell's start values (x, y).
+// component-major), W (C), cu, cv, the sum ws of its four smoothness
+// weights and the cell's start values (x, y).
 template <int C>
-__device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc, double cu,
-                                              double cv, double x, double y, double ax, double ay,
-                                              const AData &ad, int psi_one, double *K)
+__device__ __forceinline__ void coef_from_raw_ws(const double *J, const double *Wc, double cu,
+                                                 double cv, double x, double y, double ws,
+                                                 const AData &ad, int psi_one, double *K)
 {
-    double ws = ((ax + ax) + ay) + ay;
     double Du = ws, Dv = ws, A12 = 0.0, Bu = cu, Bv = cv;
 #pragma unroll
     for (int k = 0; k < C; ++k) {
@@ -863,6 +863,15 @@ __device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc,
     K[2] = Bv;
     K[3] = Du != 0 ? omega / Du : 0.0;
     K[4] = Dv != 0 ? omega / Dv : 0.0;
+}
+
+// The same with the constant weights ax (left, right) and ay (down, up).
+template <int C>
+__device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc, double cu,
+                                              double cv, double x, double y, double ax, double ay,
+                                              const AData &ad, int psi_one, double *K)
+{
+    coef_from_raw_ws<C>(J, Wc, cu, cv, x, y, ((ax + ax) + ay) + ay, ad, psi_one, K);
 }
 
 template <int C>
@@ -917,6 +926,173 @@ hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv
     default: return hipErrorInvalidValue;
     }
 #undef FRH_ASM
+    return hipGetLastError();
+}
+
+// -------------------------------- nonlinear smoothness (a_smooth != 1)
+// core/level_solver.py:69-155,248-255,262-316.  The smoothness weight of
+// the edge between a cell and its neighbour is 0.5 (psi_s[c] + psi_s[nb])
+// alpha / h^2, with psi_s refreshed over the padded grid at every psi
+// refresh, so each lag block carries nine coefficients per cell:
+// {A12, Bu, Bv, omega/Du, omega/Dv, w_left, w_right, w_down, w_up}.
+
+// The level's initial flow on the padded (h+2) x (w+2) grid: either the
+// caller's padded float64 planes (u64, v64; one frame) or the level's
+// float32 flow (u32, v32; frame stride fs32) read with the edge pad of
+// add_boundary (clamped indices).
+struct PadFlow {
+    const float *u32, *v32;
+    size_t fs32;
+    const double *u64, *v64;
+};
+
+__device__ inline double pad_flow(const PadFlow &f, int comp, int t, int h, int w, int jp, int ip)
+{
+    if (f.u64) return (comp ? f.v64 : f.u64)[(size_t)jp * (w + 2) + ip];
+    const int r = clampi(jp - 1, 0, h - 1), c = clampi(ip - 1, 0, w - 1);
+    return (double)(comp ? f.v32 : f.u32)[(size_t)t * f.fs32 + (size_t)r * w + c];
+}
+
+// nonlinearity_smoothness_2d at a psi refresh: psi_s over the padded grid
+// of u + du, v + dv (one thread per padded cell).  du, dv of the interior
+// come from the skewed state; the padded ring holds what set_boundary_2d
+// copied before the previous sweep -- the start values of that sweep at
+// the nearest border cell, which the sweep recorded in the ring planes.
+// zero = 1 before the first sweep (du = 0 everywhere).  One-sided
+// differences on the padded edge, central inside; psi_s = a (t + 1e-5)^
+// (a - 1) with frh_det_pow, then (1 - beta) + beta psi_s when mix is set
+// (compute_flow_gnc).
+// u + du (comp 0) or v + dv (comp 1) at padded cell (j, i) of frame t
+__device__ __forceinline__ double psi_full(const PadFlow &fl, const double *D, const double *R, int comp, int t,
+                                           int h, int w, int hs, size_t plane, int zero, int j, int i)
+{
+    double d = 0.0;
+    if (!zero) {
+        const bool inner = j >= 1 && j <= h && i >= 1 && i <= w;
+        const int r = clampi(j - 1, 0, h - 1), c = clampi(i - 1, 0, w - 1);
+        d = (inner ? D : R)[(size_t)comp * plane + (size_t)(r + c) * hs + r];
+    }
+    return pad_flow(fl, comp, t, h, w, j, i) + d;
+}
+
+__global__ void k_psi_smooth(PadFlow fl, const double *__restrict__ duv, const double *__restrict__ ring,
+                             size_t fstride, int h, int w, int hs, size_t plane, int zero, double a,
+                             double am1, int mix, double beta, double hx, double hy,
+                             double *__restrict__ psi_s, size_t ps_stride)
+{
+    const int ip = blockIdx.x * blockDim.x + threadIdx.x, jp = blockIdx.y, t = blockIdx.z;
+    const int m = h + 2, n = w + 2;
+    if (ip >= n) return;
+    const double *D = duv + (size_t)t * fstride, *R = ring + (size_t)t * fstride;
+    // one-sided on the padded edge, central inside
+    const int il = ip == 0 ? 0 : ip - 1, ir = ip == n - 1 ? n - 1 : ip + 1;
+    const int ju = jp == 0 ? 0 : jp - 1, jd = jp == m - 1 ? m - 1 : jp + 1;
+    const bool cx = ip > 0 && ip < n - 1, cy = jp > 0 && jp < m - 1;
+    double g[4];  // ux, uy, vx, vy
+#pragma unroll
+    for (int comp = 0; comp < 2; ++comp) {
+        const double dx = psi_full(fl, D, R, comp, t, h, w, hs, plane, zero, jp, ir) -
+                          psi_full(fl, D, R, comp, t, h, w, hs, plane, zero, jp, il);
+        const double dy = psi_full(fl, D, R, comp, t, h, w, hs, plane, zero, jd, ip) -
+                          psi_full(fl, D, R, comp, t, h, w, hs, plane, zero, ju, ip);
+        g[2 * comp] = cx ? dx / (2 * hx) : dx / hx;
+        g[2 * comp + 1] = cy ? dy / (2 * hy) : dy / hy;
+    }
+    double s = g[0] * g[0] + g[1] * g[1] + g[2] * g[2] + g[3] * g[3];
+    if (s < 0) s = 0;
+    double ps = a * frh_det_pow(s + 1e-5, am1);
+    if (mix) ps = (1.0 - beta) + beta * ps;
+    psi_s[(size_t)t * ps_stride + (size_t)jp * n + ip] = ps;
+}
+
+hipError_t launch_psi_smooth(const float *u32, const float *v32, size_t fs32, const double *u64,
+                             const double *v64, const double *duv, const double *ring, size_t fstride,
+                             int T, const Skew &s, int zero, double a_smooth, double gnc_beta, double hx,
+                             double hy, double *psi_s, size_t ps_stride, hipStream_t st)
+{
+    PadFlow fl{u32, v32, fs32, u64, v64};
+    const int mix = gnc_beta >= 0 && a_smooth > 0.0 && a_smooth < 1.0 && gnc_beta < 1.0;
+    const dim3 g((s.w + 2 + 63) / 64, s.h + 2, T), b(64);
+    hipLaunchKernelGGL(k_psi_smooth, g, b, 0, st, fl, duv, ring, fstride, s.h, s.w, s.hs, s.plane, zero,
+                       a_smooth, a_smooth - 1.0, mix, gnc_beta, hx, hy, psi_s, ps_stride);
+    return hipGetLastError();
+}
+
+// Lag-block coefficients with the psi_s weights (the operations of
+// assemble_pixel in the oracle): per cell the four edge weights, cu, cv
+// from the padded initial flow, then the data terms as in k_assemble; nine
+// planes out.
+template <int C>
+__global__ void k_assemble_nl(const double *__restrict__ J, const double *__restrict__ Wsk, PadFlow fl,
+                              const double *__restrict__ psi_s, size_t ps_stride,
+                              const double *__restrict__ duv, double *__restrict__ coef, size_t fstride,
+                              int h, int w, int hs, size_t plane, double ax, double ay, AData ad,
+                              int psi_one, int zero_init)
+{
+    const int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
+    const int c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    const size_t q = (size_t)d * hs + r;
+    const int n = w + 2, jp = r + 1, ip = c + 1;
+    const double *ps = psi_s + (size_t)t * ps_stride;
+    const double pc = ps[(size_t)jp * n + ip];
+    const int nj[4] = {jp, jp, jp + 1, jp - 1}, ni[4] = {ip - 1, ip + 1, ip, ip};  // left, right, down, up
+    const double wd[4] = {ax, ax, ay, ay};
+    double wn[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) wn[k] = 0.5 * (pc + ps[(size_t)nj[k] * n + ni[k]]) * wd[k];
+    const double uc = pad_flow(fl, 0, t, h, w, jp, ip), vc = pad_flow(fl, 1, t, h, w, jp, ip);
+    double cu = wn[0] * (pad_flow(fl, 0, t, h, w, nj[0], ni[0]) - uc);
+    double cv = wn[0] * (pad_flow(fl, 1, t, h, w, nj[0], ni[0]) - vc);
+#pragma unroll
+    for (int k = 1; k < 4; ++k) {
+        cu = cu + wn[k] * (pad_flow(fl, 0, t, h, w, nj[k], ni[k]) - uc);
+        cv = cv + wn[k] * (pad_flow(fl, 1, t, h, w, nj[k], ni[k]) - vc);
+    }
+    const double *Jt = J + (size_t)t * fstride;
+    double x = 0.0, y = 0.0;
+    if (!psi_one && !zero_init) {
+        x = duv[(size_t)t * fstride + q];
+        y = duv[(size_t)t * fstride + plane + q];
+    }
+    double Jr[6 * C], Wr[C], K[5];
+#pragma unroll
+    for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
+#pragma unroll
+    for (int k = 0; k < C; ++k) Wr[k] = Wsk[(size_t)k * plane + q];
+    coef_from_raw_ws<C>(Jr, Wr, cu, cv, x, y, ((wn[0] + wn[1]) + wn[2]) + wn[3], ad, psi_one, K);
+    double *co = coef + (size_t)t * fstride;
+#pragma unroll
+    for (int e = 0; e < 5; ++e) co[e * plane + q] = K[e];
+#pragma unroll
+    for (int k = 0; k < 4; ++k) co[(5 + k) * plane + q] = wn[k];
+}
+
+hipError_t launch_assemble_nl(const double *J, const double *Wsk, const float *u32, const float *v32,
+                              size_t fs32, const double *u64, const double *v64, const double *psi_s,
+                              size_t ps_stride, const double *duv, double *coef, size_t frame_stride, int T,
+                              int C, const Skew &s, double ax, double ay, const double *a_data_minus1,
+                              const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                              hipStream_t st)
+{
+    const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
+    PadFlow fl{u32, v32, fs32, u64, v64};
+    dim3 g(s.h + s.w - 1, T), b(s.hs);
+#define FRH_ASMNL(CC)                                                                               \
+    hipLaunchKernelGGL(k_assemble_nl<CC>, g, b, 0, st, J, Wsk, fl, psi_s, ps_stride, duv, coef, frame_stride, \
+                       s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
+    switch (C) {
+    case 1: FRH_ASMNL(1); break;
+    case 2: FRH_ASMNL(2); break;
+    case 3: FRH_ASMNL(3); break;
+    case 4: FRH_ASMNL(4); break;
+    case 5: FRH_ASMNL(5); break;
+    case 6: FRH_ASMNL(6); break;
+    case 7: FRH_ASMNL(7); break;
+    case 8: FRH_ASMNL(8); break;
+    default: return hipErrorInvalidValue;
+    }
+#undef FRH_ASMNL
     return hipGetLastError();
 }
 
@@ -975,15 +1151,20 @@ constexpr int sweep_ring(int nmin)
     return n;
 }
 
+// C = kSmoothNL: nine precomputed coefficients per cell (per-cell smoothness
+// weights, a_smooth != 1).
+constexpr int kSmoothNL = -1;
+
 template <int P, int C>
 struct SweepState {
     static constexpr int NR = 2 * P - 1;  // diagonals a lane's P sweeps span
-    static constexpr int NK = sweep_ring(C == 0 ? NR + 3 : NR);
-    static constexpr int DK = NK - NR;    // coefficient prefetch distance (C = 0)
+    static constexpr int NK = sweep_ring(C <= 0 ? NR + 3 : NR);
+    static constexpr int DK = NK - NR;    // coefficient prefetch distance (C <= 0)
     static constexpr int NL = kSweepNL, DL = kSweepDL;
     static constexpr int U = sweep_lcm(NK, NL);
     static constexpr int NRAW = C > 0 ? 7 * C + 2 : 1;
-    double K[NK][5];
+    static constexpr int NKC = C == kSmoothNL ? 9 : 5;  // coefficients per cell
+    double K[NK][NKC];
     double Ru[2][P];      // results of step t-1 / t-2, slot by step parity
     double Rv[2][P];
     double Lu[NL], Lv[NL];
@@ -1026,12 +1207,12 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
     const int h = a.h, w = a.w, hs = a.hs;
     const bool row_ok = r < h;
     {
-        if constexpr (C == 0) {
+        if constexpr (C <= 0) {
             const int d = t + DK;
             const unsigned q = slot_offset(row_ok, d, r, w, hs);
             double *k = st.K[(S + DK) % NK];
 #pragma unroll
-            for (int e = 0; e < 5; ++e) k[e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
+            for (int e = 0; e < St::NKC; ++e) k[e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
         } else {
             const int d = t + 1, c = d - r;
             if (row_ok && c >= 0 && c < w) {
@@ -1093,18 +1274,29 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C> &st, const SweepArgs
             double ru = p == 0 ? ldu : st.Ru[PV][p > 0 ? p - 1 : 0];
             double rv = p == 0 ? ldv : st.Rv[PV][p > 0 ? p - 1 : 0];
             double lu = st.Ru[PV][p], lv = st.Rv[PV][p];
-            double b = fma(ax, (double)((c == 0) + (c == w - 1)), brow);
             double A12 = Kp[0], Bu = Kp[1], Bv = Kp[2], wu = Kp[3], wvv = Kp[4];
-            double nu = fma(ax, lu + ru, Bu);
-            nu = fma(ay, dnu[p] + upu[p], nu);
+            double b, nu, nv;
+            if constexpr (C == kSmoothNL) {
+                // per-cell edge weights; an out-of-level neighbour's weight
+                // moves to the cell's own term, summed left, right, up, down
+                const double wl = Kp[5], wr = Kp[6], wdn = Kp[7], wup = Kp[8];
+                b = (((c == 0 ? wl : 0.0) + (c == w - 1 ? wr : 0.0)) + (r == 0 ? wup : 0.0)) +
+                    (r == h - 1 ? wdn : 0.0);
+                nu = fma(wup, upu[p], fma(wdn, dnu[p], fma(wr, ru, fma(wl, lu, Bu))));
+                nv = fma(wup, upv[p], fma(wdn, dnv[p], fma(wr, rv, fma(wl, lv, Bv))));
+            } else {
+                b = fma(ax, (double)((c == 0) + (c == w - 1)), brow);
+                nu = fma(ax, lu + ru, Bu);
+                nu = fma(ay, dnu[p] + upu[p], nu);
+                nv = fma(ax, lv + rv, Bv);
+                nv = fma(ay, dnv[p] + upv[p], nv);
+            }
             nu = fma(-A12, ov, nu);
             double dun = fma(wu, nu, fma(wu, b, om1) * ou);
-            double nv = fma(ax, lv + rv, Bv);
-            nv = fma(ay, dnv[p] + upv[p], nv);
             nv = fma(-A12, dun, nv);
             double dvn = fma(wvv, nv, fma(wvv, b, om1) * ov);
             const bool ok = row_ok && (unsigned)c < (unsigned)w;
-            if constexpr (C == 0) {
+            if constexpr (C <= 0) {
                 // a cell outside the level has zero coefficients and a zero
                 // start value, so it computes 0 (up to sign)
                 cu[p] = dun;
@@ -1193,7 +1385,7 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
 #pragma unroll
     for (int k = 0; k < NK; ++k)
 #pragma unroll
-        for (int e = 0; e < 5; ++e) st.K[k][e] = 0.0;
+        for (int e = 0; e < St::NKC; ++e) st.K[k][e] = 0.0;
 #pragma unroll
     for (int p = 0; p < P; ++p) st.Ru[0][p] = st.Ru[1][p] = st.Rv[0][p] = st.Rv[1][p] = 0.0;
 #pragma unroll
@@ -1205,7 +1397,7 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     auto cell_ok = [&](int d) { int c = d - r; return row_ok && c >= 0 && c < w; };
     SweepBufs bf;
     bf.plane_bytes = (unsigned)(a.plane * sizeof(double));
-    bf.coef = make_rsrc(cb, C == 0 ? 5 * bf.plane_bytes : 0u);
+    bf.coef = make_rsrc(cb, C <= 0 ? St::NKC * bf.plane_bytes : 0u);
     bf.duv = make_rsrc(du, 2 * bf.plane_bytes);
 #pragma unroll
     for (int d = 0; d < DL; ++d) {
@@ -1213,12 +1405,12 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
         st.Lu[d] = buf_load_f64(bf.duv, q, 0);
         st.Lv[d] = buf_load_f64(bf.duv, q, bf.plane_bytes);
     }
-    if constexpr (C == 0) {
+    if constexpr (C <= 0) {
 #pragma unroll
         for (int d = 0; d < DK; ++d) {
             const unsigned q = slot_offset(row_ok, d, r, w, hs);
 #pragma unroll
-            for (int e = 0; e < 5; ++e) st.K[d][e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
+            for (int e = 0; e < St::NKC; ++e) st.K[d][e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
         }
     } else {
         if (cell_ok(0)) {
@@ -1239,7 +1431,7 @@ template <int P, int C>
 static hipError_t launch_sweep_pc(const SweepArgs &a, int T, hipStream_t st)
 {
     // buffer offsets are 32-bit and must stay below kOutOfRange
-    if (a.plane * sizeof(double) * 5 >= kOutOfRange) return hipErrorInvalidValue;
+    if (a.plane * sizeof(double) * SweepState<P, C>::NKC >= kOutOfRange) return hipErrorInvalidValue;
     if (a.hs <= 512)
         hipLaunchKernelGGL((k_sweep<P, 512, C>), dim3(T), dim3(a.hs), 0, st, a);
     else if (P <= 2)
@@ -1272,6 +1464,19 @@ hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, si
     case 3: return launch_sweep_p<3>(a, 0, T, st);
     case 4: return launch_sweep_p<4>(a, 0, T, st);
     case 5: return launch_sweep_p<5>(a, 0, T, st);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sweep_nl(int P, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
+                           const Skew &s, int zero_init, hipStream_t st)
+{
+    SweepArgs a{};
+    a.coef = coef; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
+    a.h = s.h; a.w = s.w; a.hs = s.hs; a.zero_init = zero_init;
+    switch (P) {
+    case 1: return launch_sweep_pc<1, kSmoothNL>(a, T, st);
+    case 2: return launch_sweep_pc<2, kSmoothNL>(a, T, st);
     default: return hipErrorInvalidValue;
     }
 }

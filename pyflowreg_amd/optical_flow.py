@@ -232,8 +232,6 @@ class DisplacementSolver:
         self.max_batch = int(max_batch)
         self.iterations, self.update_lag = int(iterations), int(update_lag)
         self.plan = pyramid_plan(self.H, self.W, float(eta), int(levels), int(min_level), alpha)
-        if float(a_smooth) != 1.0:
-            raise ValueError("flowreg_hip: a_smooth != 1 is not implemented")
         ad = np.broadcast_to(np.asarray(a_data, dtype=np.float64), (self.C,))
         self._a_data = np.ascontiguousarray(ad)
         keep = []

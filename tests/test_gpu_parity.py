@@ -144,3 +144,62 @@ def test_batched_solver_equals_single(gpu):
     for t in range(3):
         w1 = gpu.get_displacement(ref, fr[1 + t], weight=[0.5, 0.5])
         np.testing.assert_array_equal(wb[t], w1)
+
+
+# a_smooth != 1: psi_s refreshes, nine-coefficient lag blocks, border ring
+@pytest.mark.parametrize("h,w,K,its,lag,a_s", [(12, 15, 1, 50, 5, 0.5), (40, 33, 2, 50, 5, 0.75),
+                                               (20, 11, 2, 3, 1, 0.5), (30, 25, 2, 16, 7, 1.5),
+                                               (9, 70, 3, 11, 2, 0.6), (530, 12, 1, 7, 5, 0.5)])
+def test_level_solver_nonlinear_smoothness_bit_exact(gpu, oracle, h, w, K, its, lag, a_s):
+    rng = _rng(6)
+    J, W, u, v = _level_inputs(oracle, rng, h, w, K)
+    ad = np.full(K, 0.45)
+    args = (J[0], J[1], J[2], J[3], J[4], J[5], W, u, v, [2.0, 2.5], its, lag, 0, ad, a_s, 1.3, 0.8)
+    du, dv = gpu.level_solver(*args)
+    ou, ov = oracle.level_solver(*args)
+    np.testing.assert_array_equal(du, ou)
+    np.testing.assert_array_equal(dv, ov)
+
+
+@pytest.mark.parametrize("beta", [0.0, 0.4, 1.0])
+def test_level_solver_gnc_nonlinear_smoothness_bit_exact(gpu, oracle, beta):
+    """compute_flow_gnc blends psi_s with the stage weight when 0 < a_s < 1."""
+    rng = _rng(7)
+    J, W, u, v = _level_inputs(oracle, rng, 18, 23, 2)
+    ad = np.full(2, 0.45)
+    args = (J[0], J[1], J[2], J[3], J[4], J[5], W, u, v, [2.0, 2.0], 20, 5, 0, ad, 0.5, 1.0, 1.0)
+    du, dv = gpu.level_solver(*args, gnc_beta=beta)
+    ou, ov = oracle.level_solver(*args, gnc_beta=beta)
+    np.testing.assert_array_equal(du, ou)
+    np.testing.assert_array_equal(dv, ov)
+
+
+@pytest.mark.parametrize("H,W,C,a_s", [(64, 64, 1, 0.6), (48, 80, 2, 0.5)])
+def test_get_displacement_nonlinear_smoothness_bit_exact(gpu, oracle, H, W, C, a_s):
+    f, m = _pair(H, W, C, 30 + H)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(m, f.astype(np.float64)).astype(np.float32)
+    uv = np.zeros((H, W, 2), np.float32)
+    uv[..., 0] = 0.3
+    kw = dict(weight=[0.5, 0.5][:C] if C > 1 else None, uv=uv, a_smooth=a_s)
+    a = gpu.get_displacement(ref, mov, **kw)
+    b = oracle.get_displacement(ref, mov, **kw)
+    d = np.abs(a - b)
+    assert d.max() == 0.0, f"max |dflow| {d.max():.3e}"
+
+
+def test_batched_solver_nonlinear_smoothness(gpu, oracle):
+    """Three frames in one launch per kernel, each equal to the oracle."""
+    import torch
+    from pyflowreg_amd.optical_flow import DisplacementSolver
+    H, W, C = 40, 52, 2
+    f, _ = _pair(H, W, C, 77)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    movs = np.stack([oracle.preprocess(_pair(H, W, C, 78 + k)[1], f.astype(np.float64)).astype(np.float32)
+                     for k in range(3)])
+    s = DisplacementSolver(H, W, C, max_batch=3, a_smooth=0.7)
+    s.set_reference(ref, np.array([0.5, 0.5]), None)
+    out = s.run(torch.from_numpy(movs).cuda(), out_dtype=torch.float64).cpu().numpy()
+    for t in range(3):
+        b = oracle.get_displacement(ref, movs[t], weight=[0.5, 0.5], a_smooth=0.7)
+        np.testing.assert_array_equal(out[t], b)

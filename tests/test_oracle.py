@@ -509,3 +509,111 @@ def test_preprocess_normalizes_with_reference_range(oracle):
     ref = np.array([[[2.0], [6.0]]])
     arr = np.array([[[4.0], [8.0]]])
     np.testing.assert_allclose(oracle.normalize(arr, ref), (arr - 2) / (4 + 1e-8))
+
+
+# ------------------------------------------- nonlinear smoothness (a_smooth != 1)
+def test_psi_smooth_matches_numpy_gradient(oracle):
+    """core/level_solver.py:69-155: one-sided differences on the padded edge
+    and central ones inside are numpy's np.gradient (edge_order 1); psi_s =
+    a (t + 1e-5)^(a - 1) with numpy's pow."""
+    rng = np.random.default_rng(30)
+    m, n = 9, 12
+    u, du, v, dv = (rng.standard_normal((m, n)) for _ in range(4))
+    a, hx, hy = 0.6, 1.3, 0.7
+    ps = oracle.psi_smooth(u, du, v, dv, a, hx, hy)
+    uy, ux = np.gradient(u + du, hy, hx)
+    vy, vx = np.gradient(v + dv, hy, hx)
+    ref = a * np.power(ux * ux + uy * uy + vx * vx + vy * vy + 1e-5, a - 1)
+    np.testing.assert_allclose(ps, ref, rtol=4e-15)
+
+
+def _dense_system_nl(J, W, u, v, ax, ay, psi, psi_s):
+    """_dense_system with the edge weights 0.5 (psi_s[c] + psi_s[nb]) w of
+    core/level_solver.py:277-316."""
+    m, n, K = J[0].shape
+    h, w = m - 2, n - 2
+    N = h * w
+    A = np.zeros((2 * N, 2 * N))
+    Dp = np.zeros(2 * N)
+    b = np.zeros(2 * N)
+    for c in range(w):
+        for r in range(h):
+            p = c * h + r
+            jr, jc = r + 1, c + 1
+            wp = W[jr, jc] * psi[jr, jc]
+            a11 = (wp * J[0][jr, jc]).sum()
+            a22 = (wp * J[1][jr, jc]).sum()
+            a12 = (wp * J[3][jr, jc]).sum()
+            b1 = -(wp * J[4][jr, jc]).sum()
+            b2 = -(wp * J[5][jr, jc]).sum()
+            ws = 0.0
+            for (dr, dc, wd) in [(0, -1, ax), (0, 1, ax), (1, 0, ay), (-1, 0, ay)]:
+                wt = 0.5 * (psi_s[jr, jc] + psi_s[jr + dr, jc + dc]) * wd
+                rr, cc = r + dr, c + dc
+                ws += wt
+                b1 += wt * (u[jr + dr, jc + dc] - u[jr, jc])
+                b2 += wt * (v[jr + dr, jc + dc] - v[jr, jc])
+                if 0 <= rr < h and 0 <= cc < w:
+                    q = cc * h + rr
+                    A[2 * p, 2 * q] -= wt
+                    A[2 * p + 1, 2 * q + 1] -= wt
+                    A[2 * p, 2 * p] += wt
+                    A[2 * p + 1, 2 * p + 1] += wt
+            A[2 * p, 2 * p] += a11
+            A[2 * p + 1, 2 * p + 1] += a22
+            A[2 * p, 2 * p + 1] = a12
+            A[2 * p + 1, 2 * p] = a12
+            Dp[2 * p] = ws + a11
+            Dp[2 * p + 1] = ws + a22
+            b[2 * p] = b1
+            b[2 * p + 1] = b2
+    return A, Dp, b
+
+
+@pytest.mark.parametrize("sweeps", [1, 2])
+def test_solver_nonlinear_smoothness_matrix_form(oracle, sweeps):
+    """update_lag = 1 with a_smooth = 0.6: psi_s is refreshed before sweep 0
+    from du = 0 (psi_smooth of u, v alone) and every later sweep refreshes
+    again; the first sweep equals one matrix-form SOR step with the edge
+    weights 0.5 (psi_s[c] + psi_s[nb]) alpha / h^2."""
+    rng = np.random.default_rng(31)
+    h, w = 5, 6
+    J, W, u, v = _random_level(oracle, rng, h, w, 1)
+    a_s, hx, hy = 0.6, 1.1, 0.9
+    ax, ay = 2.0 / hx ** 2, 2.0 / hy ** 2
+    psi = np.ones_like(W)
+    z = np.zeros_like(u)
+    ps = oracle.psi_smooth(u, z, v, z, a_s, hx, hy)
+    A, Dp, b = _dense_system_nl(J, W, u, v, ax, ay, psi, ps)
+    M = np.tril(A, -1) + np.diag(Dp / 1.95)
+    x = np.linalg.solve(M, b)
+    f = oracle.compute_flow(*J, W, u, v, 2.0, 2.0, 1, 1, [1.0], a_s, hx, hy)
+    du, dv = _to_field(x, h, w)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 0], du, rtol=1e-10, atol=1e-12)
+    np.testing.assert_allclose(f[1:-1, 1:-1, 1], dv, rtol=1e-10, atol=1e-12)
+    if sweeps == 2:
+        # the second refresh sees the padded ring that set_boundary_2d copied
+        # before sweep 0 (zeros) and the interior after sweep 0
+        du_p = np.zeros_like(u)
+        dv_p = np.zeros_like(u)
+        du_p[1:-1, 1:-1], dv_p[1:-1, 1:-1] = du, dv
+        ps2 = oracle.psi_smooth(u, du_p, v, dv_p, a_s, hx, hy)
+        A2, Dp2, b2 = _dense_system_nl(J, W, u, v, ax, ay, psi, ps2)
+        M2 = np.tril(A2, -1) + np.diag(Dp2 / 1.95)
+        x2 = x + np.linalg.solve(M2, b2 - A2 @ x)
+        f2 = oracle.compute_flow(*J, W, u, v, 2.0, 2.0, 2, 1, [1.0], a_s, hx, hy)
+        du2, dv2 = _to_field(x2, h, w)
+        np.testing.assert_allclose(f2[1:-1, 1:-1, 0], du2, rtol=1e-9, atol=1e-12)
+        np.testing.assert_allclose(f2[1:-1, 1:-1, 1], dv2, rtol=1e-9, atol=1e-12)
+
+
+def test_solver_nonlinear_smoothness_literal_order_agrees(oracle):
+    """With a_smooth = 0.5 and lag 5 (several psi_s refreshes over stale
+    set_boundary rings) the lag-block coefficient form equals the reference's
+    per-sweep expression order to rounding."""
+    rng = np.random.default_rng(32)
+    J, W, u, v = _random_level(oracle, rng, 14, 17, 2)
+    args = (*J, W, u, v, 2.0, 2.0, 50, 5, [0.45, 0.45], 0.5, 1.2, 0.9)
+    a = oracle.compute_flow(*args)
+    b = oracle.compute_flow(*args, order=2)
+    np.testing.assert_allclose(a, b, atol=1e-11)
