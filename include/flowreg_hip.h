@@ -84,6 +84,17 @@ int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, c
                      const double *den, const double *wy, int ry, const double *wx, int rx,
                      void *out, int out_f64, int semantics, void *stream);
 
+/* apply_gaussian_filter's 4-D branch with a temporal kernel
+ * (util/image_processing.py:147-161; scipy's gaussian_filter filters the
+ * frame axis first, then rows, then columns): raw (T, H, W, C) normalised
+ * with (lo, den) per channel, correlated along T with wt (2 rt + 1 taps,
+ * mode "reflect" over the T frames of the call), then the row and column
+ * passes of frh_prefilter_ex (float64 semantics).  rt = 0 is frh_prefilter_ex
+ * with semantics 0. */
+int frh_prefilter_3d(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
+                     const double *den, const double *wt, int rt, const double *wy, int ry,
+                     const double *wx, int rx, void *out, int out_f64, void *stream);
+
 /* util/resize_util.py:116-141,232-254 (imresize_fused_gauss_cubic).
  * src: (N, h, w) planes; dst: (N, oh, ow); tmp: (N, h, ow) scratch.
  * ix, wx: (ow, P) and iy, wy: (oh, P) device tables from

@@ -63,6 +63,8 @@ SIGNATURES = {
                                    c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_prefilter_ex": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
+    "frh_prefilter_3d": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
+                                 c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "frh_motion_tensor": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_double, c_double,
                                   c_double, c_int, c_int] + [c_void_p] * 6 + [c_void_p]),
     "frh_level_solver": (c_int, [c_void_p] * 9 + [c_int, c_int, c_int, c_double, c_double, c_int,

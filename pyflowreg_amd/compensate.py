@@ -219,7 +219,7 @@ class MotionCorrector:
             raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
 
     def _preprocess(self, frames, norm_ref, out_dtype=torch.float32):
-        return preprocess(frames, normalization_ref=norm_ref, sigma=self.sigma[0],
+        return preprocess(frames, normalization_ref=norm_ref, sigma=self.sigma,
                           channel_normalization=self.opt.channel_normalization, out_dtype=out_dtype)
 
     def _flow_params(self):

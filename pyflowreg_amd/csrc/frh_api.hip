@@ -219,6 +219,33 @@ int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, c
                           stream);
 }
 
+int frh_prefilter_3d(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
+                     const double *den, const double *wt, int rt, const double *wy, int ry, const double *wx,
+                     int rx, void *out, int out_f64, void *stream)
+{
+    FRH_REQ(raw && out && lo && den && wt && wy && wx, "frh_prefilter_3d: null pointer");
+    FRH_REQ(rt >= 0 && rt <= 32, "frh_prefilter_3d: temporal radius must be in [0, 32]");
+    FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0 && C <= 32 && H <= 65535 && T <= 65535,
+            "frh_prefilter_3d: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
+    if (rt == 0)  // the temporal kernel is [1]: an exact identity
+        return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, 0, stream);
+    hipStream_t st = as_stream(stream);
+    double *tf = nullptr;
+    FRH_HIP(hipMallocAsync((void **)&tf, sizeof(double) * (size_t)T * H * W * C, st), "frh_prefilter_3d scratch");
+    hipError_t e = launch_prefilter_t(raw, raw_f64 != 0, T, H, W, C, lo, den, wt, rt, tf, st);
+    int rc = FRH_OK;
+    if (e == hipSuccess) {
+        // the frames are normalised now: x -> (x - 0) / 1 is exact
+        std::vector<double> zero(C, 0.0), one(C, 1.0);
+        rc = prefilter_impl(tf, 1, T, H, W, C, zero.data(), one.data(), wy, ry, wx, rx, out, out_f64 != 0, 0, stream);
+    }
+    hipError_t e2 = hipFreeAsync(tf, st);
+    if (e != hipSuccess) return fail(FRH_EHIP, "temporal prefilter kernel: %s", hipGetErrorString(e));
+    if (rc) return rc;
+    FRH_HIP(e2, "frh_prefilter_3d free");
+    return FRH_OK;
+}
+
 int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow, const int32_t *ix,
                const float *wx, const int32_t *iy, const float *wy, int P, float *tmp, void *stream)
 {

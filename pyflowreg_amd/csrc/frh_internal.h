@@ -40,6 +40,8 @@ constexpr int kCoefPlanes = 5;  // Du, Dv, A12, Bu, Bv
 hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, const double *lo,
                               const double *den, const double *wy, int ry, int sem, double *tmp,
                               hipStream_t st);
+hipError_t launch_prefilter_t(const void *raw, int raw_f64, int Tn, int H, int W, int C, const double *lo,
+                              const double *den, const double *wt, int rt, double *out, hipStream_t st);
 hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, const double *wx,
                               int rx, int sem, float *out, hipStream_t st);
 hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, const double *wx,

@@ -83,6 +83,47 @@ __global__ void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, Ga
     tmp[((size_t)tc * H + y) * W + x] = (sem & kPreFilt32) ? (double)(float)acc : acc;
 }
 
+// Temporal pass of apply_gaussian_filter's 4-D branch (scipy's
+// gaussian_filter filters axis 0, the frames of the batch, first): the
+// normalised frames (x - lo) / den correlated with the temporal kernel
+// along T with mode "reflect", in the operation order of the row pass;
+// float64 (T, H, W, C) out.  One thread per element of a frame row.
+template <typename T>
+__global__ void k_prefilter_t(const T *__restrict__ raw, int Tn, int H, int W, int C, GaussW lo_den, GaussW wt,
+                              int rt, double *__restrict__ out)
+{
+    const int e = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, t = blockIdx.z;
+    const int WC = W * C;
+    if (e >= WC) return;
+    const int c = e % C;
+    const double lo = lo_den.w[2 * c], den = lo_den.w[2 * c + 1];
+    const size_t frame = (size_t)H * WC, off = (size_t)y * WC + e;
+    auto nrm = [&](int tt) { return ((double)raw[(size_t)tt * frame + off] - lo) / den; };
+    double acc = nrm(t) * wt.w[rt];
+    for (int j = rt; j >= 1; --j)
+        acc += (nrm(reflect_half(t - j, Tn)) + nrm(reflect_half(t + j, Tn))) * wt.w[rt - j];
+    out[(size_t)t * frame + off] = acc;
+}
+
+hipError_t launch_prefilter_t(const void *raw, int raw_f64, int Tn, int H, int W, int C, const double *lo,
+                              const double *den, const double *wt, int rt, double *out, hipStream_t st)
+{
+    GaussW ld{}, g{};
+    for (int c = 0; c < C; ++c) {
+        ld.w[2 * c] = lo[c];
+        ld.w[2 * c + 1] = den[c];
+    }
+    for (int k = 0; k < 2 * rt + 1; ++k) g.w[k] = wt[k];
+    const dim3 grid((W * C + 63) / 64, H, Tn);
+    if (raw_f64)
+        hipLaunchKernelGGL(k_prefilter_t<double>, grid, dim3(64), 0, st, (const double *)raw, Tn, H, W, C, ld, g,
+                           rt, out);
+    else
+        hipLaunchKernelGGL(k_prefilter_t<float>, grid, dim3(64), 0, st, (const float *)raw, Tn, H, W, C, ld, g,
+                           rt, out);
+    return hipGetLastError();
+}
+
 template <typename O>
 __global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx, int rx, int sem,
                               O *__restrict__ out)

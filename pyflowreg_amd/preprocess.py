@@ -72,25 +72,39 @@ def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
     f64 = x.dtype == torch.float64
     x = x.to(device=dev, dtype=torch.float64 if f64 else torch.float32).contiguous()
     T, H, W, C = x.shape
-    sig = np.asarray(sigma, dtype=np.float64)
-    if sig.ndim == 2:
-        if not np.all(sig == sig[0]):
-            raise ValueError("flowreg_hip: per-channel sigmas must be equal")
-        sig = sig[0]
-    sy, sx = float(sig[0]), float(sig[1])
-    st = float(sig[2]) if (len(sig) > 2 and not spatial_only) else 0.0
-    if st > 1e-15 and int(truncate * st + 0.5) > 0:
-        raise ValueError("flowreg_hip: temporal Gaussian radius > 0 is not implemented")
     if norm_bounds is not None:
         lo = np.ascontiguousarray(np.broadcast_to(np.asarray(norm_bounds[0], dtype=np.float64), (C,)))
         den = np.ascontiguousarray(np.broadcast_to(np.asarray(norm_bounds[1], dtype=np.float64), (C,)))
     else:
         ref = normalization_ref if normalization_ref is not None else frames
         lo, den = _range(ref, C, channel_normalization, eps)
+    sig = np.asarray(sigma, dtype=np.float64)
+    out64 = out_dtype == torch.float64
+    if sig.ndim == 2 and not np.all(sig == sig[0]):
+        # per-channel sigmas (row min(c, rows - 1) for channel c): each
+        # channel filtered on its own with its range
+        out = torch.empty((T, C, H, W), dtype=torch.float64 if out64 else torch.float32, device=dev)
+        for c in range(C):
+            xc = x[..., c:c + 1].contiguous()
+            out[:, c] = preprocess(xc if not spatial_only else xc[0], sigma=sig[min(c, len(sig) - 1)],
+                                   truncate=truncate, out_dtype=out_dtype, norm_bounds=(lo[c:c + 1], den[c:c + 1]),
+                                   semantics=semantics)[:, 0]
+        return out
+    if sig.ndim == 2:
+        sig = sig[0]
+    sy, sx = float(sig[0]), float(sig[1])
+    st = float(sig[2]) if (len(sig) > 2 and not spatial_only) else 0.0
     ry, wy = gaussian_weights(sy, truncate) if sy > 1e-15 else (0, np.ones(1))
     rx, wx = gaussian_weights(sx, truncate) if sx > 1e-15 else (0, np.ones(1))
-    out64 = out_dtype == torch.float64
+    rt, wt = gaussian_weights(st, truncate) if st > 1e-15 else (0, np.ones(1))
     out = torch.empty((T, C, H, W), dtype=torch.float64 if out64 else torch.float32, device=dev)
+    if rt > 0:
+        # scipy filters the frame axis first (apply_gaussian_filter's 4-D branch)
+        if semantics:
+            raise ValueError("flowreg_hip: float32 prefilter semantics with a temporal kernel are not implemented")
+        check(lib().frh_prefilter_3d(ptr(x), int(f64), T, H, W, C, ptr(lo), ptr(den), ptr(wt), rt, ptr(wy), ry,
+                                     ptr(wx), rx, ptr(out), int(out64), stream_handle()), "frh_prefilter_3d")
+        return out
     check(lib().frh_prefilter_ex(ptr(x), int(f64), T, H, W, C, ptr(lo), ptr(den), ptr(wy), ry, ptr(wx), rx,
                                  ptr(out), int(out64), int(semantics), stream_handle()), "frh_prefilter_ex")
     return out

@@ -167,3 +167,61 @@ def test_valid_masks():
     assert m.dtype == np.uint8
     assert np.all(m[0][:, :9] == 255) and np.all(m[0][:, 10:] == 0)
     assert np.all(m[1][:3] == 0) and np.all(m[1][3:] == 255)
+
+
+@pytest.mark.parametrize("sigma", [(1.0, 1.0, 0.6), (1.2, 0.8, 1.5), ((1.0, 1.0, 0.1), (2.0, 0.5, 0.7))])
+def test_preprocess_temporal_and_per_channel_sigmas(gpu, oracle, sigma):
+    """apply_gaussian_filter's 4-D branch (util/image_processing.py:147-161):
+    the temporal kernel over the frames of the call (radius 2, and 6 > T = 5
+    with repeated reflections), per-channel sigma rows; a single frame is
+    filtered spatially only."""
+    import torch
+    from pyflowreg_amd.preprocess import preprocess
+    v = _video(6, 36, 30, 2, 35)
+    ref = v[0].astype(np.float64)
+    sig = np.asarray(sigma, dtype=np.float64)
+    exp = oracle.preprocess(v[1:], ref, sigma=sig)
+    out64 = preprocess(v[1:], normalization_ref=ref, sigma=sig, out_dtype=torch.float64)
+    np.testing.assert_array_equal(out64.permute(0, 2, 3, 1).cpu().numpy(), exp)
+    out32 = preprocess(v[1:], normalization_ref=ref, sigma=sig)
+    np.testing.assert_array_equal(out32.permute(0, 2, 3, 1).cpu().numpy(), exp.astype(np.float32))
+    one = preprocess(ref, sigma=sig, out_dtype=torch.float64)
+    np.testing.assert_array_equal(one.permute(0, 2, 3, 1)[0].cpu().numpy(), oracle.preprocess(ref, sigma=sig))
+
+
+def test_compensate_temporal_prefilter_matches_oracle(gpu, oracle):
+    """compensate_arr with sigma_t = 0.6 (temporal radius 2): each batch of
+    5 frames is filtered along its own frames, as _preprocess_frames does."""
+    from pyflowreg_amd.compensate import RegistrationOptions, compensate_arr
+    v = _video(11, 32, 28, 2, 36)
+    ref = v[0].astype(np.float64)
+    video = v[1:]
+    sig = (1.0, 1.0, 0.6)
+    opt = RegistrationOptions(alpha=(2.0, 2.0), buffer_size=5, sigma=sig)
+    reg, w = compensate_arr(video, ref, opt)
+    ref_proc = oracle.preprocess(ref, sigma=sig).astype(np.float32)
+    fp = dict(alpha=(2.0, 2.0), weight=np.ones((32, 28, 2)) * 0.5, levels=100, min_level=0,
+              eta=0.8, iterations=50, update_lag=5, a_data=0.45, a_smooth=1.0)
+
+    def batch(b, w_init):
+        bp = oracle.preprocess(b, ref, sigma=sig).astype(np.float32)
+        flows, regs = [], []
+        for t in range(b.shape[0]):
+            f = oracle.get_displacement(ref_proc, bp[t], uv=w_init.astype(np.float32).copy(), **fp).astype(np.float32)
+            regs.append(oracle.imregister_wrapper(b[t].astype(np.float32), f[..., 0].astype(np.float64),
+                                                  f[..., 1].astype(np.float64), ref.astype(np.float32)))
+            flows.append(f)
+        return np.stack(regs), np.stack(flows)
+    w_init = None
+    oregs, ows = [], []
+    for s in range(0, video.shape[0], 5):
+        b = video[s:s + 5]
+        if w_init is None:
+            _, w0 = batch(b, np.zeros((32, 28, 2)))
+            w_init = np.mean(w0, axis=0)
+        r, ww = batch(b, w_init)
+        w_init = np.mean(ww[-20:], axis=0)
+        oregs.append(r)
+        ows.append(ww)
+    np.testing.assert_array_equal(w, np.concatenate(ows))
+    np.testing.assert_array_equal(reg, np.concatenate(oregs))
