@@ -1566,7 +1566,30 @@ __global__ __launch_bounds__(64) void k_median_add(
     if (r >= h || c < 0 || c >= w) return;
     const double *du = duv + (size_t)t * fstride, *dv = du + plane;
     double mu, mv;
-    if (do_median) {
+    // rows of this wave that hold level cells, and their columns
+    const int r0 = (rem % nchunk) * 64;
+    const int ra = max(r0, d - w + 1), rb = min(min(r0 + 63, d), h - 1);
+    const bool interior = ra >= 2 && rb <= h - 3 && d - rb >= 2 && d - ra <= w - 3;
+    if (do_median && interior) {
+        // uniform: no window of the wave is mirrored, so window sample
+        // (i, j) sits at slot q0 + (i + j) hs + i from the window's
+        // top-left slot q0 = (d - 4) hs + r - 2: one vector offset and
+        // scalar offsets for all 50 loads
+        const __amdgpu_buffer_rsrc_t rs = make_rsrc(du, (unsigned)(2 * plane * sizeof(double)));
+        const unsigned vq = (unsigned)(((size_t)(d - 4) * hs + (r - 2)) * sizeof(double));
+        const unsigned pb = (unsigned)(plane * sizeof(double));
+        double a[25], b[25];
+#pragma unroll
+        for (int i = 0; i < 5; ++i)
+#pragma unroll
+            for (int j = 0; j < 5; ++j) {
+                const unsigned so = (unsigned)(((i + j) * hs + i) * sizeof(double));
+                a[i * 5 + j] = buf_load_f64(rs, vq, so);
+                b[i * 5 + j] = buf_load_f64(rs, vq, so + pb);
+            }
+        mu = median25(a);
+        mv = median25(b);
+    } else if (do_median) {
         double a[25], b[25];
         int rr[5], cc[5];
 #pragma unroll
