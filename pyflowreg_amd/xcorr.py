@@ -116,54 +116,77 @@ def _upsampled_cc(R: torch.Tensor, region: int, up: float, offsets: torch.Tensor
     return cc.abs()
 
 
-def _periodic_shift(img: torch.Tensor, shift) -> torch.Tensor:
-    """img (h, w) shifted by shift = (dy, dx) with wrap-around:
-    out[y, x] = img[y - dy, x - dx]."""
-    dy, dx = float(shift[0]), float(shift[1])
-    if dy == round(dy) and dx == round(dx):
-        return torch.roll(img, shifts=(int(round(dy)), int(round(dx))), dims=(0, 1))
-    h, w = img.shape
-    dev = img.device
-    fy = _freqs(h, dev)[:, None] / h
-    fx = _freqs(w, dev)[None, :] / w
-    ramp = torch.exp(-2j * math.pi * (fy * dy + fx * dx))
-    return torch.fft.ifft2(torch.fft.fft2(img.to(torch.float64)) * ramp).real.to(img.dtype)
+def _periodic_shift(imgs: torch.Tensor, shifts: np.ndarray) -> torch.Tensor:
+    """imgs (T, h, w) each shifted by its shifts[t] = (dy, dx) with
+    wrap-around: out[t, y, x] = imgs[t, y - dy, x - dx].  Integer shifts
+    gather; a frame with a fractional shift takes the Fourier shift."""
+    T, h, w = imgs.shape
+    dev = imgs.device
+    integer = np.all(shifts == np.round(shifts), axis=1)
+    out = torch.empty_like(imgs)
+    if integer.any():
+        idx = torch.from_numpy(np.nonzero(integer)[0]).to(dev)
+        s = torch.from_numpy(np.round(shifts[integer]).astype(np.int64)).to(dev)
+        ys = (torch.arange(h, device=dev)[None, :] - s[:, 0:1]) % h
+        xs = (torch.arange(w, device=dev)[None, :] - s[:, 1:2]) % w
+        src = imgs[idx]
+        out[idx] = src[torch.arange(len(idx), device=dev)[:, None, None], ys[:, :, None], xs[:, None, :]]
+    if (~integer).any():
+        idx = torch.from_numpy(np.nonzero(~integer)[0]).to(dev)
+        s = torch.from_numpy(shifts[~integer].astype(np.float64)).to(dev)
+        fy = _freqs(h, dev)[None, :, None] / h
+        fx = _freqs(w, dev)[None, None, :] / w
+        ramp = torch.exp(-2j * math.pi * (fy * s[:, 0, None, None] + fx * s[:, 1, None, None]))
+        out[idx] = torch.fft.ifft2(torch.fft.fft2(imgs[idx].to(torch.float64)) * ramp).real.to(imgs.dtype)
+    return out
 
 
-def _pearson(a: torch.Tensor, b: torch.Tensor) -> float:
-    a = a.reshape(-1).double()
-    b = b.reshape(-1).double()
-    da = a - a.mean()
-    db = b - b.mean()
-    den = torch.sqrt((da * da).sum() * (db * db).sum())
-    return float((da * db).sum() / den)
-
-
-def _disambiguate(ref: torch.Tensor, mov: torch.Tensor, shift) -> np.ndarray:
-    h, w = ref.shape
-    pos = [float(shift[0]) % h, float(shift[1]) % w]
-    neg = [pos[0] - h, pos[1] - w]
-    shifted = _periodic_shift(mov, shift)
-    iy, ix = int(round(pos[0])), int(round(pos[1]))
-    # per axis: part 0 = rows/columns before the split (the negative alias),
-    # part 1 = from the split on (the positive alias)
-    parts_y = ((0, iy), (iy, h))
-    parts_x = ((0, ix), (ix, w))
-    best, pick = -1.0, None
-    for ky in range(2):
-        for kx in range(2):
-            (y0, y1), (x0, x1) = parts_y[ky], parts_x[kx]
-            rt = ref[y0:y1, x0:x1]
-            corr = -1.0
-            if rt.numel() > 2:
-                corr = _pearson(rt, shifted[y0:y1, x0:x1])
-            if corr > best:
-                best, pick = corr, (ky, kx)
-    if pick is None:
-        warnings.warn("phase_cross_correlation: could not disambiguate the shift (no overlap tile "
-                      "correlates); returning the periodic estimate")
-        return np.asarray([float(shift[0]), float(shift[1])])
-    return np.asarray([pos[0] if pick[0] == 1 else neg[0], pos[1] if pick[1] == 1 else neg[1]])
+def _disambiguate(ref: torch.Tensor, movs: torch.Tensor, shifts: np.ndarray) -> np.ndarray:
+    """The periodic alias of each frame's shift whose overlap tile
+    correlates best with the reference (Pearson, centred per tile); one
+    host transfer for the whole batch."""
+    T, h, w = movs.shape
+    dev = movs.device
+    pos = np.stack([shifts[:, 0] % h, shifts[:, 1] % w], axis=1)
+    neg = pos - np.array([h, w], dtype=np.float64)
+    split = np.round(pos).astype(np.int64)
+    shifted = _periodic_shift(movs, shifts).to(torch.float64)
+    a = ref.to(torch.float64)[None].expand(T, h, w)
+    sy = torch.from_numpy(split[:, 0]).to(dev)[:, None, None]
+    sx = torch.from_numpy(split[:, 1]).to(dev)[:, None, None]
+    rows = torch.arange(h, device=dev)[None, :, None]
+    cols = torch.arange(w, device=dev)[None, None, :]
+    corr = torch.empty((T, 4), dtype=torch.float64, device=dev)
+    count = torch.empty((T, 4), dtype=torch.float64, device=dev)
+    # tiles in the order (rows before/after the split) x (columns before/after)
+    for k, (ky, kx) in enumerate(((0, 0), (0, 1), (1, 0), (1, 1))):
+        my = (rows >= sy) if ky else (rows < sy)
+        mx = (cols >= sx) if kx else (cols < sx)
+        m = (my & mx).to(torch.float64)
+        n = m.sum(dim=(1, 2))
+        na = n.clamp(min=1)
+        ma = (a * m).sum(dim=(1, 2)) / na
+        mb = (shifted * m).sum(dim=(1, 2)) / na
+        da = (a - ma[:, None, None]) * m
+        db = (shifted - mb[:, None, None]) * m
+        corr[:, k] = (da * db).sum(dim=(1, 2)) / torch.sqrt((da * da).sum(dim=(1, 2)) * (db * db).sum(dim=(1, 2)))
+        count[:, k] = n
+    corr = corr.cpu().numpy()
+    count = count.cpu().numpy()
+    out = np.array(shifts, dtype=np.float64)
+    for t in range(T):
+        best, pick = -1.0, None
+        for k in range(4):
+            c = corr[t, k] if count[t, k] > 2 else -1.0
+            if c > best:  # NaN (a constant tile) never wins
+                best, pick = c, k
+        if pick is None:
+            warnings.warn("phase_cross_correlation: could not disambiguate the shift (no overlap tile "
+                          "correlates); returning the periodic estimate")
+            continue
+        ky, kx = pick // 2, pick % 2
+        out[t] = [pos[t, 0] if ky == 1 else neg[t, 0], pos[t, 1] if kx == 1 else neg[t, 1]]
+    return out
 
 
 def phase_cross_correlation(ref: torch.Tensor, movs: torch.Tensor, upsample_factor: int = 1,
@@ -198,8 +221,7 @@ def phase_cross_correlation(ref: torch.Tensor, movs: torch.Tensor, upsample_fact
         shifts = (shifts + maxima / up).astype(np.float32)
     out = shifts.astype(np.float64)
     if disambiguate:
-        for t in range(T):
-            out[t] = _disambiguate(ref, movs[t], out[t])
+        out = _disambiguate(ref, movs, out)
     return out
 
 

@@ -172,6 +172,25 @@ def test_batch_matches_per_frame_and_oracle(gpu):
     assert isinstance(estimate_rigid_xcorr_batch(ref, movs[:1], target_hw=256), torch.Tensor)
 
 
+def test_disambiguation_batch_matches_oracle(gpu):
+    """Zero-filled translations beyond half the image: the batched GPU
+    disambiguation picks the same aliases as the oracle (integer and
+    subpixel refinement)."""
+    import torch
+    from pyflowreg_amd.xcorr import phase_cross_correlation
+    from test_xcorr_oracle import _zero_filled_translation
+    rng = np.random.default_rng(48)
+    ref = rng.random((64, 64)).astype(np.float32)
+    movs = np.stack([_zero_filled_translation(ref, dy, dx) for dy, dx in
+                     [(40, -45), (-36, 10), (5, 50), (20, -25), (0, 0)]])
+    for up in (1, 4):
+        g = phase_cross_correlation(torch.from_numpy(ref).cuda(), torch.from_numpy(movs).cuda(),
+                                    upsample_factor=up, disambiguate=True)
+        for t in range(movs.shape[0]):
+            np.testing.assert_array_equal(g[t], XO.phase_cross_correlation(ref, movs[t], upsample_factor=up,
+                                                                             disambiguate=True))
+
+
 # ------------------------------------------------------- cc in the executor
 def _rigid_video(T, H, W, seed):
     """tests/util/test_xcorr_prealignment.py:142-180's generator (seeded)."""

@@ -92,16 +92,26 @@ def test_upsampled_dft_peak_of_fractional_ramp(sy, sx, n):
     assert abs(mag.max() - n * n) < 1e-6 * n * n
 
 
-def test_disambiguation_picks_the_overlapping_alias():
-    """A non-periodic translation (zero fill) by (+20, -25) on 64x64: the
-    periodic estimate is ambiguous; the disambiguated shift is the true
-    one."""
+def _zero_filled_translation(ref, dy, dx):
+    h, w = ref.shape
+    mov = np.zeros_like(ref)
+    mov[max(0, dy):h + min(0, dy), max(0, dx):w + min(0, dx)] = \
+        ref[max(0, -dy):h - max(0, dy), max(0, -dx):w - max(0, dx)]
+    return mov
+
+
+@pytest.mark.parametrize("dy,dx", [(40, -45), (-36, 10), (5, 50), (20, -25)])
+def test_disambiguation_picks_the_overlapping_alias(dy, dx):
+    """A non-periodic translation (zero fill) of a 64x64 image: the periodic
+    estimate wraps into (-32, 32]; the disambiguated shift is the true one,
+    also beyond half the image."""
     rng = np.random.default_rng(3)
     ref = rng.random((64, 64)).astype(np.float32)
-    mov = np.zeros_like(ref)
-    mov[20:, :-25] = ref[:-20, 25:]
+    mov = _zero_filled_translation(ref, dy, dx)
     s = X.phase_cross_correlation(ref, mov, disambiguate=True)
-    assert tuple(s) == (-20.0, 25.0)
+    assert tuple(s) == (-dy, -dx)
+    if abs(dy) > 32 or abs(dx) > 32:
+        assert tuple(X.phase_cross_correlation(ref, mov)) != (-dy, -dx)
 
 
 # --------------------------------- reference tests (test_xcorr_prealignment)
