@@ -448,8 +448,13 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
 // with branch-free buffer loads, and the per-step barrier waits for LDS only.
 constexpr int kGMaxWaves = 16;
 
+// Ring slot of one diagonal: its coefficients and old values; du, dv (the
+// old values of row r + 1) are loaded by lane 63 only -- the other lanes
+// take them from lane r + 1's slot of the next diagonal with a DPP shift,
+// and the old right neighbour (r, c + 1) is the lane's own slot of the next
+// diagonal, so each old value is read from memory once.
 struct GFwdSlot {
-    double K[6], ou, ov, ru, rv, du, dv;
+    double K[6], ou, ov, du, dv;
 };
 
 template <int D, int MAXW>
@@ -472,10 +477,9 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
         for (int e = 0; e < 6; ++e) s.K[e] = buf_load_f64(rK, q, e * pb);
         s.ou = buf_load_f64(rI, q, 0);
         s.ov = buf_load_f64(rI, q, pb);
-        const unsigned qr = slot_offset(row_ok, d + 1, r, w, hs);        // (r, c + 1)
-        s.ru = buf_load_f64(rI, qr, 0);
-        s.rv = buf_load_f64(rI, qr, pb);
-        const unsigned qd = slot_offset(down_ok, d + 1, r + 1, w, hs);   // (r + 1, c)
+        // (r + 1, c): lane 63 reads it (the next wave's lane 0 holds it),
+        // the other lanes' offset is out of range (no memory access)
+        const unsigned qd = lane == 63 ? slot_offset(down_ok, d + 1, r + 1, w, hs) : kOutOfRange;
         s.du = buf_load_f64(rI, qd, 0);
         s.dv = buf_load_f64(rI, qd, pb);
     };
@@ -489,6 +493,10 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
             const int d = d0 + s;
             issue(S[(s + D - 1) % D], d + D - 1);
             const GFwdSlot &c = S[s];
+            const GFwdSlot &nx = S[(s + 1) % D];  // diagonal d + 1
+            const double ru = nx.ou, rv = nx.ov;  // (r, c + 1), old
+            const double du_n = wave_from_next_lane(nx.ou), dv_n = wave_from_next_lane(nx.ov);
+            const double cdu = lane == 63 ? c.du : du_n, cdv = lane == 63 ? c.dv : dv_n;  // (r + 1, c), old
             double uu = wave_from_prev_lane(pu), uv = wave_from_prev_lane(pv);
             const int wu_src = wv > 0 ? wv - 1 : 0;
             const double eu = xb[(d + 1) & 1][wu_src][0], ev = xb[(d + 1) & 1][wu_src][1];
@@ -498,9 +506,9 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
             }
             const unsigned q = slot_offset(row_ok, d, r, w, hs);
             const double A12 = c.K[0], Bu = c.K[1], Bv = c.K[2], wu = c.K[3], wvv = c.K[4], b = c.K[5];
-            const double nu = Bu + ax * (pu + c.ru) + ay * (c.du + uu) - A12 * c.ov;
+            const double nu = Bu + ax * (pu + ru) + ay * (cdu + uu) - A12 * c.ov;
             double un = wu * nu + (1.0 - kOmega + wu * b) * c.ou;
-            const double nv = Bv + ax * (pv + c.rv) + ay * (c.dv + uv) - A12 * un;
+            const double nv = Bv + ax * (pv + rv) + ay * (cdv + uv) - A12 * un;
             double vn = wvv * nv + (1.0 - kOmega + wvv * b) * c.ov;
             const bool ok = q != kOutOfRange;
             un = ok ? un : 0.0;
@@ -527,8 +535,14 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
 // next sweep (lam_out), so the right neighbour's adjoint is the lane's own
 // previous result and the lower one lane r + 1's.  k_gsweep_lam_in then forms
 // the adjoint of xin from scratch.
+// Ring slot of one diagonal (diagonals run last to first).  The new values
+// of the left and upper neighbours are the lane's own / lane r - 1's slot
+// of the next diagonal processed (d - 1), the old values of the right and
+// lower neighbours the lane's own / lane r + 1's old values of the previous
+// step (d + 1), carried in registers; only the wave's edge lanes load them
+// (uu, uv by lane 0, du, dv by lane 63), so each value is read once.
 struct GBwdSlot {
-    double K[6], lu_out, lv_out, un, lu, lv, uu, uv, ou, ov, ru, rv, du, dv;
+    double K[6], lu_out, lv_out, un, vn, uu, uv, ou, ov, du, dv;
 };
 
 template <int D, int MAXW>
@@ -559,18 +573,13 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
         s.lu_out = buf_load_f64(rL, q, 0);
         s.lv_out = buf_load_f64(rL, q, pb);
         s.un = buf_load_f64(rO, q, 0);
+        s.vn = buf_load_f64(rO, q, pb);
         s.ou = buf_load_f64(rI, q, 0);
         s.ov = buf_load_f64(rI, q, pb);
-        const unsigned ql = slot_offset(row_ok, d - 1, r, w, hs);        // (r, c - 1), new
-        s.lu = buf_load_f64(rO, ql, 0);
-        s.lv = buf_load_f64(rO, ql, pb);
-        const unsigned qu = slot_offset(up_ok, d - 1, r - 1, w, hs);     // (r - 1, c), new
+        const unsigned qu = lane == 0 ? slot_offset(up_ok, d - 1, r - 1, w, hs) : kOutOfRange;     // (r - 1, c), new
         s.uu = buf_load_f64(rO, qu, 0);
         s.uv = buf_load_f64(rO, qu, pb);
-        const unsigned qr = slot_offset(row_ok, d + 1, r, w, hs);        // (r, c + 1), old
-        s.ru = buf_load_f64(rI, qr, 0);
-        s.rv = buf_load_f64(rI, qr, pb);
-        const unsigned qd = slot_offset(down_ok, d + 1, r + 1, w, hs);   // (r + 1, c), old
+        const unsigned qd = lane == 63 ? slot_offset(down_ok, d + 1, r + 1, w, hs) : kOutOfRange;  // (r + 1, c), old
         s.du = buf_load_f64(rI, qd, 0);
         s.dv = buf_load_f64(rI, qd, pb);
     };
@@ -587,6 +596,7 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
     for (int s = 0; s < D - 1; ++s) issue(S[s], nd - 1 - s);
     issue_g(Gc[0], nd - 1);
     double pnu = 0.0, pnv = 0.0;  // this lane's anu, anv of the previous step
+    double pou = 0.0, pov = 0.0;  // this lane's old values of the previous step's diagonal (d + 1)
     for (int i0 = 0; i0 < nd; i0 += D) {
 #pragma unroll
         for (int s = 0; s < D; ++s) {
@@ -594,6 +604,15 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
             issue_g(Gc[(s + 1) & 1], d - 1);  // D is even: the parity of s is the parity of i
             issue(S[(s + D - 1) % D], d - (D - 1));
             const GBwdSlot &c = S[s];
+            const GBwdSlot &nx = S[(s + 1) % D];  // diagonal d - 1
+            const double lu = nx.un, lv = nx.vn;  // (r, c - 1), new
+            const double uu_p = wave_from_prev_lane(nx.un), uv_p = wave_from_prev_lane(nx.vn);
+            const double cuu = lane == 0 ? c.uu : uu_p, cuv = lane == 0 ? c.uv : uv_p;  // (r - 1, c), new
+            const double ru = pou, rv = pov;  // (r, c + 1), old
+            const double du_n = wave_from_next_lane(pou), dv_n = wave_from_next_lane(pov);
+            const double cdu = lane == 63 ? c.du : du_n, cdv = lane == 63 ? c.dv : dv_n;  // (r + 1, c), old
+            pou = c.ou;
+            pov = c.ov;
             double dnu = wave_from_next_lane(pnu), dnv = wave_from_next_lane(pnv);
             const int wd_src = wv + 1 < nw ? wv + 1 : wv;
             const double eu = xb[(i + 1) & 1][wd_src][0], ev = xb[(i + 1) & 1][wd_src][1];
@@ -612,8 +631,8 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
             a_nv = ok ? a_nv : 0.0;
             a_du = ok ? a_du : 0.0;
             a_nu = ok ? a_nu : 0.0;
-            const double nu = Bu + ax * (c.lu + c.ru) + ay * (c.du + c.uu) - A12 * c.ov;
-            const double nv = Bv + ax * (c.lv + c.rv) + ay * (c.dv + c.uv) - A12 * c.un;
+            const double nu = Bu + ax * (lu + ru) + ay * (cdu + cuu) - A12 * c.ov;
+            const double nv = Bv + ax * (lv + rv) + ay * (cdv + cuv) - A12 * c.un;
             const double *g = Gc[s & 1];
             buf_store_f64(g[0] - c.ov * a_nu - c.un * a_nv, rG, q, 0);         // A12
             buf_store_f64(g[1] + a_nu, rG, q, pb);                              // Bu
@@ -621,8 +640,8 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
             buf_store_f64(g[3] + (nu + b * c.ou) * a_du, rG, q, 3 * pb);        // wu
             buf_store_f64(g[4] + (nv + b * c.ov) * a_dv, rG, q, 4 * pb);        // wv
             buf_store_f64(g[5] + wu * c.ou * a_du + wvv * c.ov * a_dv, rG, q, 5 * pb);  // b
-            buf_store_f64(g[6] + (c.lu + c.ru) * a_nu + (c.lv + c.rv) * a_nv, rX, q, 0);
-            buf_store_f64(g[7] + (c.du + c.uu) * a_nu + (c.dv + c.uv) * a_nv, rX, q, pb);
+            buf_store_f64(g[6] + (lu + ru) * a_nu + (lv + rv) * a_nv, rX, q, 0);
+            buf_store_f64(g[7] + (cdu + cuu) * a_nu + (cdv + cuv) * a_nv, rX, q, pb);
             buf_store_f64(a_nu, rS, q, 0);
             buf_store_f64(a_nv, rS, q, pb);
             buf_store_f64(a_du, rS, q, 2 * pb);
