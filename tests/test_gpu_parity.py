@@ -203,3 +203,21 @@ def test_batched_solver_nonlinear_smoothness(gpu, oracle):
     for t in range(3):
         b = oracle.get_displacement(ref, movs[t], weight=[0.5, 0.5], a_smooth=0.7)
         np.testing.assert_array_equal(out[t], b)
+
+
+@pytest.mark.parametrize("C", [3, 4])
+def test_get_displacement_more_channels_bit_exact(gpu, oracle, C):
+    """Tensor, assembly and median kernels with 3 and 4 channels (channel
+    weights normalised to sum 1)."""
+    rng = _rng(40 + C)
+    H, W = 40, 52
+    f = np.stack([_pair(H, W, 1, 50 + c)[0][..., 0] for c in range(C)], -1)
+    m = np.stack([_pair(H, W, 1, 50 + c)[1][..., 0] for c in range(C)], -1)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(m, f.astype(np.float64)).astype(np.float32)
+    weight = list(rng.uniform(0.2, 1.0, C))
+    kw = dict(weight=weight, a_data=list(rng.uniform(0.3, 0.6, C)))
+    a = gpu.get_displacement(ref, mov, **kw)
+    b = oracle.get_displacement(ref, mov, **kw)
+    d = np.abs(a - b)
+    assert d.max() == 0.0, f"max |dflow| {d.max():.3e}"
