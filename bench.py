@@ -15,7 +15,9 @@ Prints one JSON line.  Beside the batched path it reports
 * ``roofline`` for the dominant kernel (algorithmic bytes over HIP-event
   time on the launch stream; HBM traffic per launch from the rocprofv3
   FETCH_SIZE/WRITE_SIZE summary in profiles/, when recorded for this
-  workload).
+  workload), with the measured bandwidth of a plain 1 GiB device copy on
+  the same GPU beside the 8 TB/s peak (``measured_copy_gbs``,
+  ``frac_of_measured_copy``).
 """
 
 from __future__ import annotations
@@ -112,6 +114,25 @@ def measured_traffic(H, W, C, T):
             if len(parts) == 3 and parts[1] == "traffic_bytes_per_launch:":
                 out[parts[0]] = int(float(parts[2]))
     return out
+
+
+def device_copy_gbs(mib=1024, reps=10):
+    """Measured HBM bandwidth of a plain device copy (read + write bytes per
+    second) on this GPU, timed with events around reps copies of mib MiB."""
+    import torch
+    n = mib * (1 << 20) // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    dst.copy_(src)
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(reps):
+        dst.copy_(src)
+    b.record()
+    b.synchronize()
+    gbs = 2 * n * 4 * reps / (a.elapsed_time(b) * 1e-3) / 1e9
+    del src, dst
+    return round(gbs, 1)
 
 
 def copy_probe(mib):
@@ -388,7 +409,10 @@ def main():
                          "launches": n, "avg_launch_ms": round(ms / max(n, 1), 4),
                          "total_ms": round(ms, 3)}
     dominant = max(kernels, key=lambda k: kernels[k]["total_ms"])
-    roofline = dict(kernels[dominant], kernel=dominant)
+    copy_gbs = device_copy_gbs()
+    for k in kernels.values():
+        k["frac_of_measured_copy"] = round(k["achieved"] / copy_gbs, 4)
+    roofline = dict(kernels[dominant], kernel=dominant, measured_copy_gbs=copy_gbs)
 
     parity = None
     if cpu_w is not None:
