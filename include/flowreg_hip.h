@@ -136,6 +136,18 @@ int frh_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc_f64,
                        int T, int C, int H, int W, const double *u, const double *v, float *out,
                        void *stream);
 
+/* imregister_wrapper(f2, u, v, f1, ...) with uint16 (is_signed = 0) or int16
+ * (is_signed = 1) raw frames f2 (core/warping.py:219-227: cv2.remap keeps
+ * the integer depth).  OpenCV's 16U / 16S remap sums the integer samples with
+ * its float weights, like its 32F path, and stores saturate_cast<T>(sum):
+ * rounded half to even and clamped to the type's range.  f2 holds the samples
+ * converted to float32 (exact); f1, the out-of-bounds fill, is float32 and
+ * stored unrounded (the reference writes it into its float32 output array).
+ * linear = 1: INTER_LINEAR, else INTER_CUBIC; other arguments as
+ * frh_warp_cubic_ex. */
+int frh_warp_int16(int linear, int is_signed, const float *f2, const float *f1, int f1_per_frame, int T,
+                   int C, int H, int W, const double *u, const double *v, float *out, void *stream);
+
 /* Data terms (core/optical_flow.py:446-473, _resolve_motion_tensor_func). */
 #define FRH_TENSOR_GC 0   /* "gc" / "gradient": get_motion_tensor_gc, :73-160 */
 #define FRH_TENSOR_GRAY 1 /* "gray" / "brightness": get_motion_tensor_gray, :163-228 */

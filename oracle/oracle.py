@@ -21,7 +21,10 @@ precision study.  ``order=1`` switches the SOR sweep to red-black ordering
 Parity status per function (the checks live in tests/test_oracle.py):
 all functions are pinned except the fractional-position arithmetic of the
 OpenCV bicubic remap and of ``cv2.resize`` (third-party, restated from its
-public behaviour, cv2 absent here): "parity unpinned" for those two.
+public behaviour, cv2 absent here): "parity unpinned" for those two.  The
+16-bit remap's rounding and clamping are pinned by closed forms; that
+OpenCV's 16U / 16S path uses the 32F path's float weights and sums is
+restated from OpenCV's source and likewise unpinned against cv2 outputs.
 """
 
 from __future__ import annotations
@@ -165,7 +168,12 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
     """core/warping.py:160-234, cubic or linear interpolation (cv2.remap
     restated): float64 images take OpenCV's 64F path (double sums), other
     images are converted to float32 and take its 32F path (float sums); the
-    result is float32."""
+    result is float32.  uint16 / int16 images: OpenCV's 16U / 16S remap
+    (remapBicubic / remapBilinear with float weights and Cast<float, T>)
+    computes the 32F path's float sum on the integer samples and stores
+    saturate_cast<T>(sum), i.e. rounded half to even and clamped to T's
+    range, before the reference writes it into its float32 array; the
+    out-of-bounds fill is not rounded."""
     method = interpolation_method.lower()
     if method not in ("cubic", "linear"):
         raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
@@ -182,6 +190,16 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
         fn = lib().or_warp_linear64 if f64 else lib().or_warp_linear
     fn(f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64),
        out)
+    raw = np.asarray(f2_level).dtype
+    if raw in (np.dtype(np.uint16), np.dtype(np.int16)):
+        gy, gx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
+        mx = (gx + np.asarray(u, dtype=np.float64)).astype(np.float32)
+        my = (gy + np.asarray(v, dtype=np.float64)).astype(np.float32)
+        inb = ~((mx < 0) | (mx >= W) | (my < 0) | (my >= H))
+        info = np.iinfo(raw)
+        for c in range(C):
+            oc = out[:, :, c]
+            oc[inb] = np.clip(np.rint(oc[inb]), info.min, info.max)
     return out[:, :, 0] if squeeze else out
 
 

@@ -233,17 +233,19 @@ class MotionCorrector:
             self._ref_raw_dev[dtype] = torch.as_tensor(self.reference_raw, dtype=dtype, device=self.device)
         return self._ref_raw_dev[dtype]
 
-    def _register(self, frames, frames_proc, w_init, prealign=True):
+    def _register(self, frames, frames_proc, w_init, prealign=True, raw_dtype=None):
         fp = self._flow_params()
         term = _resolve_const_assumption(fp.get("const_assumption", "gc"))
         cc = (self.opt.cc_hw, int(self.opt.cc_up)) if self.opt.cc_initialization and prealign else None
         return self.executor.register_device(frames, frames_proc, self._reference_raw(frames.dtype),
                                              self.reference_proc, w_init, fp, term,
-                                             self.opt.interpolation_method, cc)
+                                             self.opt.interpolation_method, cc, raw_dtype)
 
-    def _launch(self, frames):
+    def _launch(self, frames, raw_dtype=None):
         """GPU work of one batch (T, H, W, C) on the device: preprocess,
-        w_init on the first batch, the batch's flows and registered frames."""
+        w_init on the first batch, the batch's flows and registered frames.
+        raw_dtype: the host batch's dtype (the staged frames are float32 or
+        float64)."""
         if self.opt.update_reference:
             bp64 = self._preprocess(frames, self.reference_raw, torch.float64)
             bp = bp64.to(torch.float32)  # the same rounding as the float32 prefilter output
@@ -257,10 +259,11 @@ class MotionCorrector:
             # compensate_recording.py:645-678: more than 4 frames go through
             # the executor (with the prealignment), fewer through
             # get_displacement directly
-            _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)), prealign=n_init > 4)
+            _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)), prealign=n_init > 4,
+                                   raw_dtype=raw_dtype)
             self.w_init = np.mean(w0.cpu().numpy(), axis=0)
         cur = self.w_init if self.opt.update_initialization_w else np.zeros_like(self.w_init)
-        reg, flow = self._register(frames, bp, cur)
+        reg, flow = self._register(frames, bp, cur, raw_dtype=raw_dtype)
         return reg, flow, bp64
 
     def _finish(self, dtype, reg, flow, bp64, out_reg=None, out_w=None):
@@ -317,7 +320,7 @@ class MotionCorrector:
         batch = np.asarray(batch)
         batch = batch[..., None] if batch.ndim == 3 else batch
         k = self.stager.upload(batch)
-        state = self._launch(self.stager.acquire(k))
+        state = self._launch(self.stager.acquire(k), batch.dtype)
         self.stager.release(k)
         registered, w = self._finish(batch.dtype, *state)
         self._statistics(w)
@@ -364,7 +367,7 @@ def compensate_arr(c1, c_ref, options=None,
     flows = np.empty(video.shape[:3] + (2,), dtype=np.float32)
     try:
         k = mc.stager.upload(video[0:bs])
-        state = mc._launch(mc.stager.acquire(k))
+        state = mc._launch(mc.stager.acquire(k), video.dtype)
         mc.stager.release(k)
         for s in range(0, T, bs):
             nxt = s + bs < T
@@ -373,7 +376,7 @@ def compensate_arr(c1, c_ref, options=None,
             e = min(T, s + bs)
             _, w = mc._finish(video.dtype, *state, out_reg=registered[s:e], out_w=flows[s:e])
             if nxt:  # the next batch's GPU work runs during this batch's statistics
-                state = mc._launch(mc.stager.acquire(k))
+                state = mc._launch(mc.stager.acquire(k), video.dtype)
                 mc.stager.release(k)
             mc._statistics(w)
             if progress_callback is not None:

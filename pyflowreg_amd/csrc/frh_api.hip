@@ -293,6 +293,18 @@ int frh_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc_f64,
     return FRH_OK;
 }
 
+int frh_warp_int16(int linear, int is_signed, const float *f2, const float *f1, int f1_per_frame, int T, int C,
+                   int H, int W, const double *u, const double *v, float *out, void *stream)
+{
+    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_int16: null pointer");
+    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_int16: bad shape");
+    const float lo = is_signed ? -32768.f : 0.f, hi = is_signed ? 32767.f : 65535.f;
+    hipError_t e = launch_warp_saturate(linear != 0, f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, lo,
+                                        hi, out, as_stream(stream));
+    FRH_HIP(e, "16-bit warp kernel");
+    return FRH_OK;
+}
+
 int frh_motion_tensor(int kind, const float *f1, const float *f2, int h, int w, double hy, double hx,
                       double eps, int f1_f32, int f2_f32, double *J11, double *J22, double *J33,
                       double *J12, double *J13, double *J23, void *stream)

@@ -340,6 +340,17 @@ __device__ inline void cv_cubic_coeffs(float x, float *c)
     c[3] = 1.f - c[0] - c[1] - c[2];
 }
 
+// OpenCV's 16U / 16S remap paths use the float weights and float sums of
+// its 32F path on the integer samples (exact in float) and store
+// saturate_cast<T>(sum): round half to even, then clamp to T's range.  With
+// sat set, an in-bounds result is rounded and clamped to [lo, hi] that way;
+// the out-of-bounds fill is stored as it is (the reference writes it into its
+// float32 output array).
+__device__ __forceinline__ float warp_store_value(float s, int sat, float lo, float hi)
+{
+    return sat ? fminf(fmaxf(rintf(s), lo), hi) : s;
+}
+
 // U: float (level flows, divided by h_col / h_row in float64 here) or
 // double (displacements already divided by the caller, h = 1).
 // cv2.remap INTER_CUBIC + BORDER_REPLICATE of the image type I: the 1/32-px
@@ -351,7 +362,7 @@ template <typename U, typename I, typename Acc>
 __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
                        int f1_per_frame, int C, int H, int W, const U *__restrict__ u,
                        const U *__restrict__ v, size_t uv_stride, double h_col, double h_row,
-                       float *__restrict__ out)
+                       float *__restrict__ out, int sat = 0, float lo = 0.f, float hi = 0.f)
 {
     int x = blockIdx.x * blockDim.x + threadIdx.x;
     int y = blockIdx.y;
@@ -388,7 +399,7 @@ __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
             sum += (Acc)S[0] * w[8] + (Acc)S[1] * w[9] + (Acc)S[2] * w[10] + (Acc)S[3] * w[11];
             S += W;
             sum += (Acc)S[0] * w[12] + (Acc)S[1] * w[13] + (Acc)S[2] * w[14] + (Acc)S[3] * w[15];
-            O[c * HW + p] = (float)sum;
+            O[c * HW + p] = warp_store_value((float)sum, sat, lo, hi);
         }
     } else {
         int xs[4], ys[4];
@@ -401,7 +412,7 @@ __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
             Acc sum = 0;
             for (int i = 0; i < 4; ++i)
                 for (int j = 0; j < 4; ++j) sum += (Acc)S[(size_t)ys[i] * W + xs[j]] * w[i * 4 + j];
-            O[c * HW + p] = (float)sum;
+            O[c * HW + p] = warp_store_value((float)sum, sat, lo, hi);
         }
     }
 }
@@ -448,7 +459,8 @@ hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64
 template <typename U, typename I, typename Acc>
 __global__ void k_warp_linear(const I *__restrict__ f2, const I *__restrict__ f1, int f1_per_frame, int C,
                               int H, int W, const U *__restrict__ u, const U *__restrict__ v,
-                              size_t uv_stride, float *__restrict__ out)
+                              size_t uv_stride, float *__restrict__ out, int sat = 0, float lo = 0.f,
+                              float hi = 0.f)
 {
     const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, t = blockIdx.z;
     if (x >= W) return;
@@ -475,7 +487,7 @@ __global__ void k_warp_linear(const I *__restrict__ f2, const I *__restrict__ f1
         const I *S = F2 + c * HW;
         const Acc sum = (Acc)S[(size_t)y0 * W + x0] * (Acc)w0 + (Acc)S[(size_t)y0 * W + x1] * (Acc)w1 +
                         (Acc)S[(size_t)y1 * W + x0] * (Acc)w2 + (Acc)S[(size_t)y1 * W + x1] * (Acc)w3;
-        O[c * HW + p] = (float)sum;
+        O[c * HW + p] = warp_store_value((float)sum, sat, lo, hi);
     }
 }
 
@@ -493,6 +505,20 @@ hipError_t launch_warp_linear_ex(const void *f2, const void *f1, int img_f64, in
     else
         hipLaunchKernelGGL((k_warp_linear<double, float, float>), g, b, 0, st, (const float *)f2,
                            (const float *)f1, f1_per_frame, C, H, W, u, v, uv_frame_stride, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_warp_saturate(int linear, const float *f2, const float *f1, int f1_per_frame, int T, int C,
+                                int H, int W, const double *u, const double *v, size_t uv_frame_stride,
+                                float lo, float hi, float *out, hipStream_t st)
+{
+    const dim3 g((W + 63) / 64, H, T), b(64);
+    if (linear)
+        hipLaunchKernelGGL((k_warp_linear<double, float, float>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u,
+                           v, uv_frame_stride, out, 1, lo, hi);
+    else
+        hipLaunchKernelGGL((k_warp<double, float, float>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v,
+                           uv_frame_stride, 1.0, 1.0, out, 1, lo, hi);
     return hipGetLastError();
 }
 

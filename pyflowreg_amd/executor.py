@@ -41,16 +41,21 @@ def _device():
 
 
 def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor,
-                interpolation_method: str = "cubic") -> torch.Tensor:
+                interpolation_method: str = "cubic", raw_dtype=None) -> torch.Tensor:
     """core/warping.py:160-234 (cubic or linear) for a batch on the GPU.
     frames (T, H, W, C) float32 (cv2.remap's 32F path) or float64 (its 64F
     path, double sums), flow (T, H, W, 2) of any float dtype (the map
     float32(x + u) is formed in float64), fill (H, W, C) -> (T, H, W, C)
-    float32."""
+    float32.  raw_dtype uint16 / int16: frames hold those integer samples
+    (as float32) and cv2.remap works in their depth, rounding each in-bounds
+    result half to even and clamping it to the type's range
+    (frh_warp_int16)."""
     method = str(interpolation_method).lower()
     if method not in ("cubic", "linear"):
         raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
-    name = "frh_warp_cubic_ex" if method == "cubic" else "frh_warp_linear_ex"
+    int16 = raw_dtype is not None and np.dtype(raw_dtype) in (np.dtype(np.uint16), np.dtype(np.int16))
+    if int16 and frames.dtype != torch.float32:
+        raise ValueError("warp_frames: 16-bit raw frames are passed as float32")
     T, H, W, C = frames.shape
     f64 = frames.dtype == torch.float64
     f2 = frames.permute(0, 3, 1, 2).contiguous()
@@ -58,6 +63,12 @@ def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor,
     u = flow[..., 0].to(torch.float64).contiguous()
     v = flow[..., 1].to(torch.float64).contiguous()
     out = torch.empty(f2.shape, dtype=torch.float32, device=f2.device)
+    if int16:
+        signed = int(np.dtype(raw_dtype) == np.dtype(np.int16))
+        check(lib().frh_warp_int16(int(method == "linear"), signed, ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u),
+                                   ptr(v), ptr(out), stream_handle()), "frh_warp_int16")
+        return out.permute(0, 2, 3, 1)
+    name = "frh_warp_cubic_ex" if method == "cubic" else "frh_warp_linear_ex"
     check(getattr(lib(), name)(ptr(f2), ptr(f1), int(f64), int(f64), 0, T, C, H, W, ptr(u), ptr(v), ptr(out),
                                stream_handle()), name)
     return out.permute(0, 2, 3, 1)
@@ -174,14 +185,16 @@ class HipBatchedExecutor:
 
     def register_device(self, frames: torch.Tensor, frames_proc: torch.Tensor, reference_raw: torch.Tensor,
                         reference_proc, w_init, flow_params: dict, term: str = "gc",
-                        interpolation_method: str = "cubic", cc=None):
+                        interpolation_method: str = "cubic", cc=None, raw_dtype=None):
         """The batch's registration with every array on the device: frames
         (T, H, W, C) raw (float32, or float64 for cv2's 64F warp), frames_proc
         (T, H, W, C) float32 preprocessed, reference_raw (H, W, C) in the
         frames' dtype, reference_proc (H, W, C) and w_init (H, W, 2) (cast to
         float32 as the executors do).  cc = (cc_hw, cc_up) runs the
-        cross-correlation prealignment first.  Returns (registered float32,
-        flow float32) device tensors."""
+        cross-correlation prealignment first.  raw_dtype: the raw frames'
+        dtype before staging (uint16 / int16 frames warp in their integer
+        depth, see warp_frames).  Returns (registered float32, flow float32)
+        device tensors."""
         T, H, W, C = frames.shape
         fp = flow_params
         if cc is not None:
@@ -203,7 +216,7 @@ class HipBatchedExecutor:
             solver = self._get_solver(H, W, C, T, fp, term)
             solver.set_reference(_to_device32(reference_proc), fp.get("weight"), _to_device32(w_init))
             flow = solver.run(frames_proc, out_dtype=torch.float32)
-        return warp_frames(frames, flow, reference_raw, interpolation_method), flow
+        return warp_frames(frames, flow, reference_raw, interpolation_method, raw_dtype), flow
 
     def process_batch(self, batch: np.ndarray, batch_proc: np.ndarray, reference_raw: np.ndarray,
                       reference_proc: np.ndarray, w_init: np.ndarray,
@@ -235,7 +248,7 @@ class HipBatchedExecutor:
         reg, flow = self.register_device(
             torch.from_numpy(batch.astype(wdt, copy=False)).to(dev), torch.from_numpy(bp).to(dev),
             torch.from_numpy(np.asarray(as3(np.asarray(reference_raw)), dtype=wdt)).to(dev), rp, wi, fp, term,
-            interpolation_method, cc)
+            interpolation_method, cc, batch.dtype)
         registered = reg.cpu().numpy().astype(batch.dtype, copy=False)
         flows = flow.cpu().numpy()
         if progress_callback is not None:

@@ -201,9 +201,14 @@ class FlowRegLive:
         solver.set_reference(self.reference_proc, weight, self.last_flow)
         flow = solver.run(proc.to(torch.float32)[None], out_dtype=torch.float64)[0]
         self.last_flow = flow
-        fwarp = fr.to(torch.float64) if fr.dtype == torch.float64 else fr.to(torch.float32)
+        # cv2.remap keeps 16-bit frames in their depth (float sums, rounded
+        # results); other integer frames take the float64 path
+        if frame.dtype in (np.uint16, np.int16):
+            fwarp = fr.to(torch.float32)
+        else:
+            fwarp = fr.to(torch.float64) if fr.dtype == torch.float64 else fr.to(torch.float32)
         interp = self.options.interpolation_method
-        registered = warp_frames(fwarp[None], flow[None], self.reference_raw, interp)[0]
+        registered = warp_frames(fwarp[None], flow[None], self.reference_raw, interp, frame.dtype)[0]
         if self.frame_count % self.reference_update_interval == 0:
             reg_proc = warp_frames(proc[None], flow[None], self.reference_proc, interp)[0]
             wgt = self.reference_update_weight

@@ -74,6 +74,27 @@ def test_flow_reg_live_matches_oracle(gpu, oracle):
     assert live.frame_count == 10
 
 
+@pytest.mark.timeout(600)
+def test_flow_reg_live_uint16_frames_match_oracle(gpu, oracle):
+    """uint16 raw frames (flow_reg_live.py:216-323 with an integer
+    recording): normalised in float64, registered with cv2's 16U remap
+    (float sums, results rounded half to even and clamped), the raw
+    reference becoming float64 at its first update."""
+    from pyflowreg_amd.compensate import RegistrationOptions
+    from pyflowreg_amd.live import FlowRegLive
+    v = np.rint(_video(9, 40, 36, 2, 64) * 3000.0).astype(np.uint16)
+    sigma = (1.0, 1.0, 0.0)
+    live = FlowRegLive(RegistrationOptions(alpha=(2.0, 2.0), sigma=sigma), reference_update_interval=3)
+    live.set_reference(v[0])
+    exp = _oracle_live(oracle, v[0], v[1:], sigma, 3)
+    for k in range(1, v.shape[0]):
+        reg, flow = live(v[k])
+        oreg, oflow = exp[k - 1]
+        np.testing.assert_array_equal(flow, oflow, err_msg=f"flow of frame {k}")
+        assert reg.dtype == np.float32
+        np.testing.assert_array_equal(reg, oreg, err_msg=f"registered frame {k}")
+
+
 def test_flow_reg_live_buffers_frames_before_a_reference(gpu):
     from pyflowreg_amd.live import FlowRegLive
     v = _video(3, 24, 20, 1, 62)

@@ -21,7 +21,8 @@ def _oracle_batch(oracle, batch, ref_raw, ref_proc, w_init, fp):
     for t in range(batch.shape[0]):
         f = oracle.get_displacement(ref_proc, bp[t], uv=w_init.astype(np.float32).copy(),
                                     **fp).astype(np.float32)
-        reg = oracle.imregister_wrapper(batch[t].astype(np.float32), f[..., 0].astype(np.float64),
+        raw = batch[t] if batch.dtype in (np.uint16, np.int16) else batch[t].astype(np.float32)
+        reg = oracle.imregister_wrapper(raw, f[..., 0].astype(np.float64),
                                         f[..., 1].astype(np.float64), ref_raw.astype(np.float32))
         flows.append(f)
         regs.append(reg)
@@ -47,12 +48,72 @@ def test_executor_matches_sequential_oracle(gpu, oracle):
     assert reg.dtype == batch.dtype and w.dtype == np.float32
 
 
-def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle):
+def _int_video(T, H, W, C, seed, dt):
+    """Synthetic frames scaled to 16-bit counts (signed frames straddle 0)."""
+    v = _video(T, H, W, C, seed)
+    off = 0.0 if dt == np.uint16 else -1200.0
+    return np.rint(v * 3000.0 + off).astype(dt)
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.int16])
+@pytest.mark.parametrize("method", ["cubic", "linear"])
+def test_warp_frames_int16_matches_oracle(gpu, oracle, dt, method):
+    """16-bit raw frames warp in their depth (cv2.remap 16U / 16S): float
+    sums of the integer samples, in-bounds results rounded half to even and
+    clamped, the float64 fill rounded to float32 only."""
+    import torch
+    from pyflowreg_amd.executor import warp_frames
+    rng = np.random.default_rng(25)
+    T, H, W, C = 3, 31, 37, 2
+    info = np.iinfo(dt)
+    frames = rng.integers(info.min, info.max, (T, H, W, C), endpoint=True).astype(dt)
+    frames[0, 10:20, 10:20] = info.max  # flat patches with steps: cubic overshoot is clamped
+    frames[1, 5:15, 20:30] = info.min
+    fill = rng.random((H, W, C)) * 1000.0
+    flow = rng.uniform(-3, 3, (T, H, W, 2)).astype(np.float32)
+    dev = torch.device("cuda")
+    out = warp_frames(torch.from_numpy(frames.astype(np.float32)).to(dev), torch.from_numpy(flow).to(dev),
+                      torch.from_numpy(fill).to(dev), method, raw_dtype=dt).cpu().numpy()
+    for t in range(T):
+        ref = oracle.imregister_wrapper(frames[t], flow[t, ..., 0].astype(np.float64),
+                                        flow[t, ..., 1].astype(np.float64), fill, method)
+        np.testing.assert_array_equal(out[t], ref, err_msg=f"frame {t}")
+    f32 = oracle.imregister_wrapper(frames[0].astype(np.float32), flow[0, ..., 0].astype(np.float64),
+                                    flow[0, ..., 1].astype(np.float64), fill, method)
+    assert np.any(out[0] != f32)  # the rounding and clamping did something
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.int16])
+def test_executor_int16_frames_match_oracle(gpu, oracle, dt):
+    """process_batch with 16-bit raw frames: registered frames in the
+    batch's dtype, equal to the reference's float32 result of cv2's 16-bit
+    remap (and of the float fill) cast to that dtype."""
+    from pyflowreg_amd.executor import HipBatchedExecutor
+    v = _int_video(5, 40, 36, 2, 26, dt)
+    ref_raw = v[0].astype(np.float64) + 0.375  # a non-integer out-of-bounds fill
+    ref_proc = oracle.preprocess(ref_raw).astype(np.float32)
+    batch = v[1:]
+    w_init = np.zeros((40, 36, 2), np.float32)
+    w_init[..., 0] = 1.5
+    fp = dict(alpha=(2.0, 2.0), weight=np.ones((40, 36, 2)) * 0.5, levels=50, min_level=0,
+              eta=0.8, iterations=50, update_lag=5, a_data=0.45, a_smooth=1.0)
+    ex = HipBatchedExecutor(max_batch=4)
+    bp = oracle.preprocess(batch, ref_raw).astype(np.float32)
+    reg, w = ex.process_batch(batch, bp, ref_raw, ref_proc, w_init, flow_params=fp)
+    oreg, ow = _oracle_batch(oracle, batch, ref_raw, ref_proc, w_init, fp)
+    np.testing.assert_array_equal(w, ow)
+    assert reg.dtype == dt
+    np.testing.assert_array_equal(reg, oreg.astype(dt))
+
+
+@pytest.mark.parametrize("dt", [np.float32, np.uint16])
+def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle, dt):
     """compensate_recording.py:645-678,786-806 with buffer_size 10 over 26
     frames: w_init from the first min(22, 10) frames, then the mean of each
-    batch's last (up to) 20 flows."""
+    batch's last (up to) 20 flows.  uint16 frames: staged as float32,
+    registered in cv2's 16U depth and returned as uint16."""
     from pyflowreg_amd.compensate import RegistrationOptions, compensate_arr
-    v = _video(27, 40, 32, 2, 31)
+    v = _video(27, 40, 32, 2, 31) if dt == np.float32 else _int_video(27, 40, 32, 2, 31, dt)
     ref = v[0].astype(np.float64)
     video = v[1:]
     opt = RegistrationOptions(alpha=(2.0, 2.0), buffer_size=10, sigma=(1.0, 1.0, 0.1))
@@ -73,7 +134,8 @@ def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle):
         oregs.append(r)
         ows.append(ww)
     np.testing.assert_array_equal(w, np.concatenate(ows))
-    np.testing.assert_array_equal(reg, np.concatenate(oregs))
+    assert reg.dtype == dt
+    np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(dt))
 
 
 def test_compensate_update_reference_matches_oracle(gpu, oracle):

@@ -244,6 +244,60 @@ def test_warp64_sums_in_float64(oracle):
     assert np.any(out != out32)
 
 
+@pytest.mark.parametrize("dt", [np.uint16, np.int16])
+@pytest.mark.parametrize("method", ["cubic", "linear"])
+def test_warp_int16_identity(oracle, dt, method):
+    """16-bit raw frames: zero displacement reproduces every sample exactly
+    (the weights at fraction 0 are 0, 1, 0, 0), including the type's
+    extremes."""
+    info = np.iinfo(dt)
+    rng = np.random.default_rng(23)
+    f2 = rng.integers(info.min, info.max, (14, 17, 2), endpoint=True).astype(dt)
+    f2[0, 0, 0], f2[0, 1, 0] = info.min, info.max
+    z = np.zeros((14, 17))
+    out = oracle.imregister_wrapper(f2, z, z, f2.astype(np.float64), method)
+    assert out.dtype == np.float32
+    np.testing.assert_array_equal(out, f2.astype(np.float32))
+
+
+def test_warp_uint16_rounds_half_to_even(oracle):
+    """A half-pixel shift with linear taps gives (a + b) / 2, exact in float;
+    cv2's saturate_cast rounds it half to even (1.5 -> 2, 2.5 -> 2), where a
+    truncating cast would give 1 and 2 and rounding half up 2 and 3.  The
+    last column's map (W - 0.5) is clipped to W - 1 and reads that sample."""
+    H, W = 5, 8
+    row = np.array([1, 2, 2, 3, 7, 8, 100, 101], dtype=np.uint16)
+    f2 = np.tile(row, (H, 1))
+    u, v = np.full((H, W), 0.5), np.zeros((H, W))
+    out = oracle.imregister_wrapper(f2, u, v, np.zeros((H, W)), "linear")
+    exp = np.array([2, 2, 2, 5, 8, 54, 100, 101], dtype=np.float32)  # 1.5, 2, 2.5, 5, 7.5, 54, 100.5, clip
+    np.testing.assert_array_equal(out, np.tile(exp, (H, 1)))
+
+
+@pytest.mark.parametrize("dt", [np.uint16, np.int16])
+def test_warp_int16_cubic_saturates(oracle, dt):
+    """Keys' cubic (A = -0.75) at fraction 1/2 has weights -3/32, 19/32,
+    19/32, -3/32: next to a step from lo to hi it overshoots by 3/32 of the
+    step on both sides, which the 16-bit cast clamps to the type's range;
+    away from the step it reproduces lo, hi and their midpoint."""
+    info = np.iinfo(dt)
+    lo, hi = int(info.min), int(info.max)
+    H, W = 8, 12
+    row = np.array([lo] * 6 + [hi] * 6, dtype=dt)
+    f2 = np.tile(row, (H, 1))
+    u, v = np.full((H, W), 0.5), np.zeros((H, W))
+    out = oracle.imregister_wrapper(f2, u, v, np.zeros((H, W)), "cubic")
+    ys = slice(1, 5)  # interior rows: sy = y - 1 in [0, H - 3)
+    assert np.all(out[ys, 4] == lo)   # taps lo, lo, lo, hi: below lo, clamped
+    assert np.all(out[ys, 6] == hi)   # taps lo, hi, hi, hi: above hi, clamped
+    mid = np.float32(np.rint(np.float32(lo) * np.float32(0.5) + np.float32(hi) * np.float32(0.5)))
+    assert np.all(out[ys, 5] == mid)  # taps lo, lo, hi, hi
+    assert np.all(out[ys, 2] == lo) and np.all(out[ys, 8] == hi)
+    # the float32 path on the same samples keeps the overshoot
+    f32 = oracle.imregister_wrapper(f2.astype(np.float32), u, v, np.zeros((H, W)), "cubic")
+    assert np.all(f32[ys, 4] < lo) and np.all(f32[ys, 6] > hi)
+
+
 # ------------------------------------------------------------------ tensor
 def test_motion_tensor_reference_known_answer(oracle):
     """tests/core/test_optical_flow.py:532-551: f = 1e-3 x^2 with hx = 2 gives
