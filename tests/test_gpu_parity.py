@@ -221,3 +221,32 @@ def test_get_displacement_more_channels_bit_exact(gpu, oracle, C):
     b = oracle.get_displacement(ref, mov, **kw)
     d = np.abs(a - b)
     assert d.max() == 0.0, f"max |dflow| {d.max():.3e}"
+
+
+@pytest.mark.parametrize("opts", [
+    dict(iterations=2, update_lag=5, levels=1, eta=0.95, alpha=(50.0, 50.0), a_data=1.0),
+    dict(iterations=5, update_lag=5, levels=2, eta=0.95, alpha=(100.0, 100.0), a_data=1.0),
+    dict(iterations=7, update_lag=3, levels=50, eta=0.8, alpha=(1.5, 2.5), a_data=0.45),
+    dict(iterations=13, update_lag=1, levels=50, eta=0.7, alpha=(3.0, 3.0), a_data=0.3),
+    dict(iterations=9, update_lag=4, levels=4, eta=0.9, alpha=(2.0, 2.0), a_data=0.6, min_level=1),
+])
+def test_batched_solver_options_bit_exact(gpu, oracle, opts):
+    """The batched solver under non-default options (the reference test
+    suite's fast/minimal presets, lag blocks that do not divide the
+    iterations, lag 1, a shallow pyramid stopped above the finest level):
+    every frame equal to the oracle's get_displacement."""
+    import torch
+    from pyflowreg_amd.optical_flow import DisplacementSolver
+    H, W, C = 36, 50, 2
+    f, _ = _pair(H, W, C, 91)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    movs = np.stack([oracle.preprocess(_pair(H, W, C, 92 + k)[1], f.astype(np.float64)).astype(np.float32)
+                     for k in range(3)])
+    s = DisplacementSolver(H, W, C, max_batch=3, **opts)
+    s.set_reference(ref, np.array([0.5, 0.5]), None)
+    out = s.run(torch.from_numpy(movs).cuda(), out_dtype=torch.float64).cpu().numpy()
+    for t in range(3):
+        b = oracle.get_displacement(ref, movs[t], weight=[0.5, 0.5], **opts)
+        d = np.abs(out[t] - b)
+        assert d.max() == 0.0, f"frame {t}: max |dflow| {d.max():.3e}"
+    s.close()

@@ -287,3 +287,71 @@ def test_compensate_temporal_prefilter_matches_oracle(gpu, oracle):
         ows.append(ww)
     np.testing.assert_array_equal(w, np.concatenate(ows))
     np.testing.assert_array_equal(reg, np.concatenate(oregs))
+
+
+def _moving_disc_uint16(T, H, W, C, seed):
+    """The reference's test recording (tests/fixtures.py:14-87, pattern
+    "motion"): a bright disc moving on a circle over a textured, noisy
+    background, clipped to [0, 1] and stored as uint16 counts."""
+    rng = np.random.default_rng(seed)
+    frames = np.zeros((T, H, W, C), np.float32)
+    cy, cx, radius = H // 2, W // 2, min(H, W) // 8
+    y, x = np.ogrid[:H, :W]
+    for t in range(T):
+        for c in range(C):
+            ox = int(radius * 0.5 * np.cos(2 * np.pi * t / T))
+            oy = int(radius * 0.5 * np.sin(2 * np.pi * t / T))
+            mask = (x - (cx + ox)) ** 2 + (y - (cy + oy)) ** 2 <= radius ** 2
+            frames[t, :, :, c] = mask.astype(np.float32) * 0.8
+            frames[t, :, :, c] += 0.2 * rng.random((H, W))
+            frames[t, :, :, c] += 0.1 * rng.random((H, W))
+    return (np.clip(frames, 0.0, 1.0) * 65535).astype(np.uint16)
+
+
+@pytest.mark.parametrize("preset", ["minimal", "fast"])
+def test_compensate_arr_reference_test_options(gpu, oracle, preset):
+    """The options of the reference's own test suite (tests/fixtures.py:
+    133-165 get_minimal_of_options: alpha 100, levels 2, iterations 5, eta
+    0.95, a_data 1, per-channel sigma rows with sigma_t 0.5, weights (1, 1),
+    joint normalisation; tests/conftest.py:98-106 fast_of_options: levels 1,
+    iterations 2, alpha 50) on its uint16 moving-disc recording, against the
+    oracle's batch loop: flows and registered uint16 frames bit for bit."""
+    from pyflowreg_amd.compensate import RegistrationOptions, compensate_arr
+    T, H, W, C, bs = 12, 32, 64, 2, 6
+    video = _moving_disc_uint16(T, H, W, C, 37)
+    ref = np.mean(video[:5], axis=0)
+    alpha, levels, iters = ((100.0, 100.0), 2, 5) if preset == "minimal" else ((50.0, 50.0), 1, 2)
+    sig = ((1.0, 1.0, 0.5), (1.0, 1.0, 0.5))
+    opt = RegistrationOptions(alpha=alpha, levels=levels, iterations=iters, eta=0.95, update_lag=5,
+                              a_smooth=1.0, a_data=1.0, sigma=sig, weight=(1.0, 1.0), min_level=0,
+                              quality_setting="fast", channel_normalization="joint", buffer_size=bs)
+    reg, w = compensate_arr(video, ref, opt)
+    ref_proc = oracle.preprocess(ref, sigma=sig).astype(np.float32)
+    fp = dict(alpha=alpha, weight=np.ones((H, W, C)) * 0.5, levels=levels, min_level=0, eta=0.95,
+              iterations=iters, update_lag=5, a_data=1.0, a_smooth=1.0)
+
+    def batch(b, w_init):
+        bp = oracle.preprocess(b, ref, sigma=sig).astype(np.float32)
+        flows, regs = [], []
+        for t in range(b.shape[0]):
+            f = oracle.get_displacement(ref_proc, bp[t], uv=w_init.astype(np.float32).copy(),
+                                        **fp).astype(np.float32)
+            regs.append(oracle.imregister_wrapper(b[t], f[..., 0].astype(np.float64),
+                                                  f[..., 1].astype(np.float64), ref))
+            flows.append(f)
+        return np.stack(regs), np.stack(flows)
+    w_init = None
+    oregs, ows = [], []
+    for s in range(0, T, bs):
+        b = video[s:s + bs]
+        if w_init is None:
+            _, w0 = batch(b, np.zeros((H, W, 2)))
+            w_init = np.mean(w0, axis=0)
+        r, ww = batch(b, w_init)
+        w_init = np.mean(ww[-20:], axis=0)
+        oregs.append(r)
+        ows.append(ww)
+    np.testing.assert_array_equal(w, np.concatenate(ows))
+    assert reg.dtype == np.uint16
+    np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(np.uint16))
+    assert np.abs(w).max() > 0.1  # the disc's motion is found
