@@ -250,3 +250,53 @@ def test_batched_solver_options_bit_exact(gpu, oracle, opts):
         d = np.abs(out[t] - b)
         assert d.max() == 0.0, f"frame {t}: max |dflow| {d.max():.3e}"
     s.close()
+
+
+@pytest.mark.parametrize("kind", ["hw", "hwc"])
+def test_get_displacement_spatial_weight_map_bit_exact(gpu, oracle, kind):
+    """Per-pixel weights (core/optical_flow.py:856-879): an (H, W) map
+    broadcast to every channel, or an (H, W, C) map, resized to each level
+    (:565-569) -- here a smooth map with a zero-weight mask."""
+    H, W, C = 56, 44, 2
+    f, m = _pair(H, W, C, 121)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(m, f.astype(np.float64)).astype(np.float32)
+    yy, xx = np.mgrid[0:H, 0:W]
+    base = 0.25 + 0.5 * np.exp(-((yy - H / 3) ** 2 + (xx - W / 2) ** 2) / 300.0)
+    base[:8, :12] = 0.0  # masked corner
+    wmap = base if kind == "hw" else np.stack([base, 1.0 - 0.5 * base], axis=-1)
+    a = gpu.get_displacement(ref, mov, weight=wmap)
+    b = oracle.get_displacement(ref, mov, weight=wmap)
+    d = np.abs(a - b)
+    assert d.max() == 0.0, f"max |dflow| {d.max():.3e}"
+    # the batched solver takes the same map
+    import torch
+    from pyflowreg_amd.optical_flow import DisplacementSolver
+    s = DisplacementSolver(H, W, C, max_batch=2)
+    s.set_reference(ref, wmap, None)
+    out = s.run(torch.from_numpy(np.stack([mov, mov])).cuda(), out_dtype=torch.float64).cpu().numpy()
+    np.testing.assert_array_equal(out[0], b)
+    np.testing.assert_array_equal(out[1], b)
+    s.close()
+
+
+def test_get_displacement_varying_initial_flow_bit_exact(gpu, oracle):
+    """A spatially varying initial flow uv (resized to the coarsest level,
+    core/optical_flow.py:531-533) through the plain and batched solvers."""
+    H, W, C = 60, 52, 2
+    f, m = _pair(H, W, C, 131)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    mov = oracle.preprocess(m, f.astype(np.float64)).astype(np.float32)
+    yy, xx = np.mgrid[0:H, 0:W]
+    uv = np.stack([1.5 * np.sin(xx / 9.0) * np.cos(yy / 13.0), -0.8 + 0.02 * (yy - xx)], axis=-1).astype(np.float32)
+    a = gpu.get_displacement(ref, mov, weight=[0.5, 0.5], uv=uv)
+    b = oracle.get_displacement(ref, mov, weight=[0.5, 0.5], uv=uv)
+    d = np.abs(a - b)
+    assert d.max() == 0.0, f"max |dflow| {d.max():.3e}"
+    import torch
+    from pyflowreg_amd.optical_flow import DisplacementSolver
+    s = DisplacementSolver(H, W, C, max_batch=1)
+    s.set_reference(ref, np.array([0.5, 0.5]), uv)
+    out = s.run(torch.from_numpy(mov[None]).cuda(), out_dtype=torch.float64).cpu().numpy()
+    np.testing.assert_array_equal(out[0], b)
+    s.close()
