@@ -300,3 +300,25 @@ def test_get_displacement_varying_initial_flow_bit_exact(gpu, oracle):
     out = s.run(torch.from_numpy(mov[None]).cuda(), out_dtype=torch.float64).cpu().numpy()
     np.testing.assert_array_equal(out[0], b)
     s.close()
+
+
+def test_get_displacement_argument_contract(gpu, oracle):
+    """core/optical_flow.py:842-858: 2-D frames give an (H, W, 2) flow equal
+    to the single-channel result; frames of different ndim fail the
+    reference's assertion; an unknown const_assumption raises ValueError;
+    levels taller than the sweep's 1024 rows are rejected before any launch."""
+    from pyflowreg_amd._lib import FlowregHipError
+    f, m = _pair(40, 36, 1, 141)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)[..., 0]
+    mov = oracle.preprocess(m, f.astype(np.float64)).astype(np.float32)[..., 0]
+    a = gpu.get_displacement(ref, mov)
+    assert a.shape == (40, 36, 2)
+    np.testing.assert_array_equal(a, oracle.get_displacement(ref, mov))
+    np.testing.assert_array_equal(a, gpu.get_displacement(ref[..., None], mov[..., None]))
+    with pytest.raises(AssertionError):
+        gpu.get_displacement(ref, mov[..., None])
+    with pytest.raises(ValueError):
+        gpu.get_displacement(ref, mov, const_assumption="unknown")
+    tall = np.zeros((1100, 16), np.float32)
+    with pytest.raises(FlowregHipError, match="1024 rows"):
+        gpu.get_displacement(tall, tall, levels=1)
