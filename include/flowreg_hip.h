@@ -191,6 +191,12 @@ int frh_resize_cubic_cv(const double *src, int T, int h, int w, int C, double *d
  * core/optical_flow.py:607-609.  f: (N, h, w) float64 planes, in place. */
 int frh_median5_mirror(double *f, int N, int h, int w, void *stream);
 
+/* np.mean(w, axis=0) of n float32 frames of len values each, w C-contiguous
+ * (n, len): the rows summed in order in float32, then divided by n
+ * (motion_correction/compensate_recording.py:674,801-806, the w_init
+ * means of the batch loop).  out holds len float32 values. */
+int frh_mean_frames_f32(const float *w, int n, size_t len, float *out, void *stream);
+
 /* ----------------------------------------------------- whole pyramid */
 
 /* One pyramid level as planned by the host (core/optical_flow.py:514-577).

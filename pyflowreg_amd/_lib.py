@@ -73,6 +73,7 @@ SIGNATURES = {
                                                   c_int, c_void_p, c_double, c_double, c_double,
                                                   c_double, c_void_p, c_void_p, c_void_p]),
     "frh_median5_mirror": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+    "frh_mean_frames_f32": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, c_void_p]),
     "frh_resize_cubic_cv": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
                                     c_void_p]),
     "frh_solver_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(SolverDesc)]),

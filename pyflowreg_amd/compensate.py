@@ -17,14 +17,18 @@ compensate_recording.py):
   translation are accumulated (:808-821);
 * batches are staged in pinned host memory and uploaded on a side stream
   while the previous batch is solved; preprocessing, solving and warping
-  stay on the device;
+  stay on the device; ``w_init`` (and, with ``update_reference``, the new
+  reference) is formed on the device, so the next batch's GPU work is
+  queued before the current batch's results are copied to the host on a
+  side stream;
 * with ``update_reference`` the processed reference becomes the float64
   mean of the batch's last (up to) 100 preprocessed float64 frames, each
   warped by its flow with the current reference as fill (:680-713,
   :866-867).
 
-Means and statistics are computed with numpy on the host, exactly as the
-reference computes them, so that ``w_init`` is bit-identical.
+The ``w_init`` means are numpy's float32 axis-0 means, bit for bit
+(``frh_mean_frames_f32``); the statistics are computed with numpy on the
+host, exactly as the reference computes them.
 """
 
 from __future__ import annotations
@@ -35,9 +39,9 @@ from typing import Callable, Optional, Sequence
 import numpy as np
 import torch
 
-from .executor import HipBatchedExecutor, warp_frames
+from .executor import HipBatchedExecutor, mean_frames, warp_frames
 from .optical_flow import _resolve_const_assumption
-from .preprocess import preprocess
+from .preprocess import _range, preprocess
 
 _QUALITY_MIN_LEVEL = {"quality": 0, "balanced": 4, "fast": 6}
 
@@ -137,6 +141,15 @@ def _stat_pool():
     return _STAT_POOL
 
 
+def _parallel_copy(dst: np.ndarray, src: np.ndarray) -> None:
+    """np.copyto(dst, src, casting="unsafe") in frame chunks on the worker
+    threads (numpy releases the GIL while it copies)."""
+    n = dst.shape[0]
+    step = max(1, -(-n // _STAT_WORKERS))
+    list(_stat_pool().map(lambda lo: np.copyto(dst[lo:lo + step], src[lo:lo + step], casting="unsafe"),
+                          range(0, n, step)))
+
+
 class BatchStager:
     """Pinned host staging of raw frame batches (the batches read_batch
     returns, util/io/_base.py:283-313) with asynchronous uploads on a side
@@ -210,15 +223,26 @@ class MotionCorrector:
         # float64, as the reference keeps it; the executor casts it to float32
         self.reference_proc = self._preprocess(self.reference_raw, None, torch.float64)[0].permute(1, 2, 0)
         self.executor = HipBatchedExecutor(max_batch=max_batch)
+        self.executor.reserve = min(int(self.opt.buffer_size), max_batch)  # one solver for every batch
         self.device = self.reference_proc.device
         self.stager = BatchStager(self.device)
         self._ref_raw_dev = {}
-        self.w_init = None
+        self.w_init = None  # (H, W, 2) float32 device tensor once the first batch is queued
+        self.copy_stream = torch.cuda.Stream(device=self.device)
+        self._pin = {}  # pinned host buffers of the outputs, reused batch after batch
+        self._norm_bounds = None
         self.mean_disp, self.max_disp, self.mean_div, self.mean_translation = [], [], [], []
         if str(self.opt.interpolation_method).lower() not in ("cubic", "linear"):
             raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
 
     def _preprocess(self, frames, norm_ref, out_dtype=torch.float32):
+        if norm_ref is self.reference_raw:
+            # the raw reference's range, formed once per recording
+            if self._norm_bounds is None:
+                self._norm_bounds = _range(self.reference_raw, self.shape[2], self.opt.channel_normalization,
+                                           1e-8)
+            return preprocess(frames, sigma=self.sigma, channel_normalization=self.opt.channel_normalization,
+                              out_dtype=out_dtype, norm_bounds=self._norm_bounds)
         return preprocess(frames, normalization_ref=norm_ref, sigma=self.sigma,
                           channel_normalization=self.opt.channel_normalization, out_dtype=out_dtype)
 
@@ -242,10 +266,12 @@ class MotionCorrector:
                                              self.opt.interpolation_method, cc, raw_dtype)
 
     def _launch(self, frames, raw_dtype=None):
-        """GPU work of one batch (T, H, W, C) on the device: preprocess,
-        w_init on the first batch, the batch's flows and registered frames.
-        raw_dtype: the host batch's dtype (the staged frames are float32 or
-        float64)."""
+        """Queue the GPU work of one batch (T, H, W, C) on the device:
+        preprocess, w_init on the first batch, the batch's flows and
+        registered frames, then the next batch's w_init (and reference, with
+        update_reference).  raw_dtype: the host batch's dtype (the staged
+        frames are float32 or float64).  Returns (registered, flow, event
+        recorded after the batch's work)."""
         if self.opt.update_reference:
             bp64 = self._preprocess(frames, self.reference_raw, torch.float64)
             bp = bp64.to(torch.float32)  # the same rounding as the float32 prefilter output
@@ -261,31 +287,48 @@ class MotionCorrector:
             # get_displacement directly
             _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)), prealign=n_init > 4,
                                    raw_dtype=raw_dtype)
-            self.w_init = np.mean(w0.cpu().numpy(), axis=0)
-        cur = self.w_init if self.opt.update_initialization_w else np.zeros_like(self.w_init)
+            self.w_init = mean_frames(w0)
+        cur = self.w_init if self.opt.update_initialization_w else torch.zeros_like(self.w_init)
         reg, flow = self._register(frames, bp, cur, raw_dtype=raw_dtype)
-        return reg, flow, bp64
-
-    def _finish(self, dtype, reg, flow, bp64, out_reg=None, out_w=None):
-        """Host side of one batch that the next batch depends on: outputs
-        (copied into out_reg / out_w when given), w_init and (with
-        update_reference) the new reference."""
-        if out_reg is not None and out_reg.dtype == np.float32:
-            torch.from_numpy(out_reg).copy_(reg)
-            registered = out_reg
-        else:
-            registered = reg.cpu().numpy().astype(dtype, copy=False)
-            if out_reg is not None:
-                out_reg[...] = registered
-        if out_w is not None:
-            torch.from_numpy(out_w).copy_(flow)
-            w = out_w
-        else:
-            w = flow.cpu().numpy()
+        # (T, H, W, C) in memory here, so that the copy to the host is one DMA
+        reg = reg.contiguous()
+        # compensate_recording.py:801-806 and :866-867, on the device
         if self.opt.update_initialization_w:
-            self.w_init = np.mean(w[-20:], axis=0) if w.shape[0] > 20 else np.mean(w, axis=0)
+            self.w_init = mean_frames(flow[-20:] if flow.shape[0] > 20 else flow)
         if self.opt.update_reference:
-            self._update_reference(bp64, w)
+            self._update_reference(bp64, flow)
+        done = torch.cuda.Event()
+        done.record(torch.cuda.current_stream(self.device))
+        return reg, flow, done
+
+    def _pinned(self, name, shape, dtype):
+        key = (tuple(shape), dtype)
+        buf = self._pin.get(name)
+        if buf is None or buf[0] != key:
+            buf = (key, torch.empty(tuple(shape), dtype=dtype, pin_memory=True))
+            self._pin[name] = buf
+        return buf[1]
+
+    def _finish(self, dtype, reg, flow, done, out_reg=None, out_w=None):
+        """Copy one batch's outputs to the host (into out_reg / out_w when
+        given): DMA into pinned buffers on the copy stream after the batch's
+        work (event done), while the next batch's work, if queued, keeps
+        running; then host threads copy them into the result arrays (the
+        registered frames cast to dtype with numpy's unsafe cast, as the
+        reference's array assignment does)."""
+        with torch.cuda.stream(self.copy_stream):
+            self.copy_stream.wait_event(done)
+            hr = self._pinned("reg", reg.shape, reg.dtype)
+            hw = self._pinned("flow", flow.shape, flow.dtype)
+            hr.copy_(reg, non_blocking=True)
+            hw.copy_(flow, non_blocking=True)
+            ev = torch.cuda.Event()
+            ev.record(self.copy_stream)
+        ev.synchronize()
+        registered = out_reg if out_reg is not None else np.empty(tuple(reg.shape), dtype=dtype)
+        w = out_w if out_w is not None else np.empty(tuple(flow.shape), dtype=np.float32)
+        _parallel_copy(registered, hr.numpy())
+        _parallel_copy(w, hw.numpy())
         return registered, w
 
     def _statistics(self, w):
@@ -338,7 +381,7 @@ class MotionCorrector:
         if n < 1:
             return
         frames = bp64[T - n:].permute(0, 2, 3, 1)
-        flows = torch.from_numpy(np.ascontiguousarray(w[T - n:])).to(frames.device)
+        flows = w[T - n:]
         reg = warp_frames(frames, flows, self.reference_proc, self.opt.interpolation_method)
         acc = reg[0].to(torch.float64)
         for k in range(1, n):
@@ -370,14 +413,17 @@ def compensate_arr(c1, c_ref, options=None,
         state = mc._launch(mc.stager.acquire(k), video.dtype)
         mc.stager.release(k)
         for s in range(0, T, bs):
-            nxt = s + bs < T
-            if nxt:  # stage the next batch while the GPU works on this one
-                k = mc.stager.upload(video[s + bs:s + 2 * bs])
             e = min(T, s + bs)
-            _, w = mc._finish(video.dtype, *state, out_reg=registered[s:e], out_w=flows[s:e])
-            if nxt:  # the next batch's GPU work runs during this batch's statistics
-                state = mc._launch(mc.stager.acquire(k), video.dtype)
+            nxt = None
+            if e < T:
+                # stage the next batch and queue its GPU work (its w_init is
+                # already on the device) before this batch's outputs are
+                # copied out and its statistics computed on the host
+                k = mc.stager.upload(video[e:e + bs])
+                nxt = mc._launch(mc.stager.acquire(k), video.dtype)
                 mc.stager.release(k)
+            _, w = mc._finish(video.dtype, *state, out_reg=registered[s:e], out_w=flows[s:e])
+            state = nxt
             mc._statistics(w)
             if progress_callback is not None:
                 progress_callback(min(T, s + bs), T)

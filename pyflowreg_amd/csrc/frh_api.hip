@@ -335,6 +335,13 @@ int frh_resize_cubic_cv(const double *src, int T, int h, int w, int C, double *d
     return FRH_OK;
 }
 
+int frh_mean_frames_f32(const float *w, int n, size_t len, float *out, void *stream)
+{
+    FRH_REQ(w && out && n > 0 && len > 0, "frh_mean_frames_f32: bad arguments");
+    FRH_HIP(launch_mean_frames_f32(w, n, len, out, as_stream(stream)), "frame mean kernel");
+    return FRH_OK;
+}
+
 int frh_median5_mirror(double *f, int N, int h, int w, void *stream)
 {
     FRH_REQ(f && N > 0 && h > 0 && w > 0 && h <= 65535 && N <= 65535, "frh_median5_mirror: bad arguments");

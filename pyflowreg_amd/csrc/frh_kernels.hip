@@ -522,6 +522,29 @@ hipError_t launch_warp_saturate(int linear, const float *f2, const float *f1, in
     return hipGetLastError();
 }
 
+// numpy's np.mean(w, axis=0) of a C-contiguous float32 (n, len) array: the
+// reduction over the outer axis adds the rows one after another in float32
+// (pairwise summation applies only along a contiguous inner axis), and the
+// float32 sum divided by the count rounds as a float32 division would (the
+// quotient computed in float64 and rounded to float32 is the correctly
+// rounded float32 quotient, since 53 >= 2 * 24 + 2).
+__global__ void k_mean_frames_f32(const float *__restrict__ w, int n, size_t len, float *__restrict__ out)
+{
+    const size_t j = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= len) return;
+    float s = w[j];
+    for (int i = 1; i < n; ++i) s += w[(size_t)i * len + j];
+    out[j] = s / (float)n;
+}
+
+hipError_t launch_mean_frames_f32(const float *w, int n, size_t len, float *out, hipStream_t st)
+{
+    const size_t nb = (len + 255) / 256;
+    if (nb == 0 || nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_mean_frames_f32, dim3((unsigned)nb), dim3(256), 0, st, w, n, len, out);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------- motion tensor
 // core/optical_flow.py:73-345 (get_motion_tensor_gc / _gray / _cs), one
 // cell of one channel.  Symmetric 1-px pads are clamped reads; np.gradient

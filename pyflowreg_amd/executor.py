@@ -74,6 +74,19 @@ def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor,
     return out.permute(0, 2, 3, 1)
 
 
+def mean_frames(w: torch.Tensor) -> torch.Tensor:
+    """np.mean(w, axis=0) of a float32 (n, ...) device tensor, bit for bit
+    (frh_mean_frames_f32: the frames summed in order in float32, then
+    divided by n)."""
+    if w.dtype != torch.float32 or w.shape[0] < 1:
+        raise ValueError("mean_frames: a float32 tensor with at least one frame")
+    w = w.contiguous()
+    out = torch.empty(w.shape[1:], dtype=torch.float32, device=w.device)
+    check(lib().frh_mean_frames_f32(ptr(w), int(w.shape[0]), int(out.numel()), ptr(out), stream_handle()),
+          "frh_mean_frames_f32")
+    return out
+
+
 def compute_batch_valid_masks(w) -> np.ndarray:
     """core/warping.py:14-60,128-157: uint8 masks (0/255) of pixels whose
     backward-warp position x + u, y + v stays inside the frame."""
@@ -107,6 +120,7 @@ class HipBatchedExecutor:
     def __init__(self, n_workers: Optional[int] = None, max_batch: int = 256):
         self.n_workers = 1
         self.max_batch = int(max_batch)
+        self.reserve = 0  # frames a new solver makes room for at least
         self._key = None
         self._solver = None
 
@@ -149,14 +163,18 @@ class HipBatchedExecutor:
                int(fp.get("iterations", 50)), int(fp.get("min_level", 0)),
                int(fp.get("levels", 50)), float(fp.get("eta", 0.8)),
                float(fp.get("a_smooth", 1.0)),
-               tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist()),
-               min(T, self.max_batch), term)
-        if key != self._key:
+               tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist()), term)
+        # the cached solver is kept for any batch it has room for; a new one
+        # gets room for max(T, reserve) frames (at most max_batch), so a
+        # recording's first, short batch does not force a second allocation
+        need = min(T, self.max_batch)
+        if key != self._key or self._solver.max_batch < need:
             self.cleanup()
+            nb = min(max(need, int(self.reserve)), self.max_batch)
             self._solver = DisplacementSolver(
                 H, W, C, alpha=key[3], update_lag=key[4], iterations=key[5], min_level=key[6],
                 levels=key[7], eta=key[8], a_smooth=key[9], a_data=np.array(key[10]),
-                max_batch=key[11], const_assumption=term)
+                max_batch=nb, const_assumption=term)
             self._key = key
         return self._solver
 

@@ -355,3 +355,15 @@ def test_compensate_arr_reference_test_options(gpu, oracle, preset):
     assert reg.dtype == np.uint16
     np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(np.uint16))
     assert np.abs(w).max() > 0.1  # the disc's motion is found
+
+
+@pytest.mark.parametrize("n", [1, 2, 7, 20, 22])
+def test_mean_frames_equals_numpy(gpu, n):
+    """frh_mean_frames_f32 against np.mean(w, axis=0) on float32 flows of
+    mixed magnitudes, bit for bit."""
+    import torch
+    from pyflowreg_amd.executor import mean_frames
+    rng = np.random.default_rng(40 + n)
+    w = (rng.standard_normal((n, 37, 41, 2)) * rng.choice([1e-3, 1.0, 1e3], (n, 1, 1, 1))).astype(np.float32)
+    out = mean_frames(torch.from_numpy(w).cuda()).cpu().numpy()
+    np.testing.assert_array_equal(out, np.mean(w, axis=0))
