@@ -574,18 +574,25 @@ __device__ inline double rnd(double x)
         return x;
 }
 
-// np.gradient interior value (x[+1] - x[-1]) / (2. * h)
-template <bool F32>
+// np.gradient interior value (x[+1] - x[-1]) / (2. * h).  UNIT: h == 1 (the
+// finest level), where dividing by 2 is the exact scaling by 0.5.
+template <bool F32, bool UNIT = false>
 __device__ inline double dgrad(double xp, double xm, double h)
 {
-    return rnd<F32>(rnd<F32>(xp - xm) / rnd<F32>(2.0 * h));
+    if constexpr (UNIT)
+        return rnd<F32>(rnd<F32>(xp - xm) * 0.5);
+    else
+        return rnd<F32>(rnd<F32>(xp - xm) / rnd<F32>(2.0 * h));
 }
 
-// (x[-1] - 2 * x[0] + x[+1]) / (h ** 2)
-template <bool F32>
+// (x[-1] - 2 * x[0] + x[+1]) / (h ** 2); UNIT: h == 1, the division is exact
+template <bool F32, bool UNIT = false>
 __device__ inline double dsecond(double xm, double x0, double xp, double h)
 {
-    return rnd<F32>(rnd<F32>(rnd<F32>(xm - 2 * x0) + xp) / rnd<F32>(h * h));
+    if constexpr (UNIT)
+        return rnd<F32>(rnd<F32>(xm - 2 * x0) + xp);
+    else
+        return rnd<F32>(rnd<F32>(rnd<F32>(xm - 2 * x0) + xp) / rnd<F32>(h * h));
 }
 
 __device__ inline double img(const float *f, int w, int r, int c) { return (double)f[(size_t)r * w + c]; }
@@ -604,7 +611,7 @@ __device__ inline Clamp3 clamp3(int h, int w, int r, int c)
     return k;
 }
 
-template <bool P1, bool P2>
+template <bool P1, bool P2, bool UNIT = false>
 __device__ inline TensorOut tensor_gc(const float *f1, const float *f2, int h, int w, int r, int c,
                                       double hy, double hx)
 {
@@ -612,19 +619,19 @@ __device__ inline TensorOut tensor_gc(const float *f1, const float *f2, int h, i
     const Clamp3 k = clamp3(h, w, r, c);
     // fx = 0.5 (d/dx f1 + d/dx f2) on rows r-1, r+1 (clamped: the re-pad)
     auto fx_at = [&](int rr) {
-        double g1 = dgrad<P1>(img(f1, w, rr, k.cr), img(f1, w, rr, k.cl), hx);
-        double g2 = dgrad<P2>(img(f2, w, rr, k.cr), img(f2, w, rr, k.cl), hx);
+        double g1 = dgrad<P1, UNIT>(img(f1, w, rr, k.cr), img(f1, w, rr, k.cl), hx);
+        double g2 = dgrad<P2, UNIT>(img(f2, w, rr, k.cr), img(f2, w, rr, k.cl), hx);
         return rnd<PM>(0.5 * rnd<PM>(g1 + g2));
     };
     auto ft_at = [&](int rr, int cc) { return rnd<PM>(img(f2, w, rr, cc) - img(f1, w, rr, cc)); };
-    const double fxy = dgrad<PM>(fx_at(k.rd), fx_at(k.ru), hy);
-    const double fxt = dgrad<PM>(ft_at(r, k.cr), ft_at(r, k.cl), hx);
-    const double fyt = dgrad<PM>(ft_at(k.rd, c), ft_at(k.ru, c), hy);
+    const double fxy = dgrad<PM, UNIT>(fx_at(k.rd), fx_at(k.ru), hy);
+    const double fxt = dgrad<PM, UNIT>(ft_at(r, k.cr), ft_at(r, k.cl), hx);
+    const double fyt = dgrad<PM, UNIT>(ft_at(k.rd, c), ft_at(k.ru, c), hy);
     const double a0 = img(f1, w, r, c), b0 = img(f2, w, r, c);
-    const double fxx1 = dsecond<P1>(img(f1, w, r, k.cl), a0, img(f1, w, r, k.cr), hx);
-    const double fyy1 = dsecond<P1>(img(f1, w, k.ru, c), a0, img(f1, w, k.rd, c), hy);
-    const double fxx2 = dsecond<P2>(img(f2, w, r, k.cl), b0, img(f2, w, r, k.cr), hx);
-    const double fyy2 = dsecond<P2>(img(f2, w, k.ru, c), b0, img(f2, w, k.rd, c), hy);
+    const double fxx1 = dsecond<P1, UNIT>(img(f1, w, r, k.cl), a0, img(f1, w, r, k.cr), hx);
+    const double fyy1 = dsecond<P1, UNIT>(img(f1, w, k.ru, c), a0, img(f1, w, k.rd, c), hy);
+    const double fxx2 = dsecond<P2, UNIT>(img(f2, w, r, k.cl), b0, img(f2, w, r, k.cr), hx);
+    const double fyy2 = dsecond<P2, UNIT>(img(f2, w, k.ru, c), b0, img(f2, w, k.rd, c), hy);
     const double fxx = rnd<PM>(0.5 * rnd<PM>(fxx1 + fxx2));
     const double fyy = rnd<PM>(0.5 * rnd<PM>(fyy1 + fyy2));
     // reg = 1.0 / ((np.sqrt(a**2 + b**2) ** 2) + 1e-6)
@@ -727,12 +734,12 @@ __device__ inline TensorOut tensor_cs(const float *f1, const float *f2, int h, i
     return o;
 }
 
-template <int KIND, bool P1, bool P2>
+template <int KIND, bool P1, bool P2, bool UNIT = false>
 __device__ inline TensorOut tensor_cell(const float *f1, const float *f2, int h, int w, int r, int c,
                                         double hy, double hx, double eps)
 {
     if constexpr (KIND == kTensorGC)
-        return tensor_gc<P1, P2>(f1, f2, h, w, r, c, hy, hx);
+        return tensor_gc<P1, P2, UNIT>(f1, f2, h, w, r, c, hy, hx);
     else if constexpr (KIND == kTensorGray)
         return tensor_gray<P1, P2>(f1, f2, h, w, r, c, hy, hx);
     else
@@ -763,11 +770,12 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
 // lanes store one diagonal's 16 rows -- a contiguous 128-byte run -- instead
 // of every lane of a row storing to a different diagonal.  The level image
 // f1 is float64 data in the reference; P2 marks a float32 warped image
-// (every level below the top).
+// (every level below the top).  UNIT: unit grid spacing (hy = hx = 1, the
+// finest level), gradient-constancy term only.
 constexpr int kTensorRows = 16;
 constexpr int kTensorDiags = kTensorRows + 63;  // diagonals a tile touches
 
-template <int KIND, bool P2>
+template <int KIND, bool P2, bool UNIT = false>
 __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
     size_t plane, double hy, double hx, double eps, size_t f1_fstride, const float *__restrict__ u,
@@ -800,8 +808,8 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     for (int ch = 0; ch < C; ++ch) {
         TensorOut o{};
         if (ok)
-            o = tensor_cell<KIND, false, P2>(f1 + (size_t)t * f1_fstride + ch * hw, f2 + ((size_t)t * C + ch) * hw,
-                                             h, w, r, c, hy, hx, eps);
+            o = tensor_cell<KIND, false, P2, UNIT>(f1 + (size_t)t * f1_fstride + ch * hw,
+                                                   f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx, eps);
         put(o.J11, Jt + (size_t)(0 * C + ch) * plane);
         put(o.J22, Jt + (size_t)(1 * C + ch) * plane);
         put(o.J33, Jt + (size_t)(2 * C + ch) * plane);
@@ -825,9 +833,20 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
     hipLaunchKernelGGL((k_tensor<K, P>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, C, \
                        h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
                        ay, J, cuv, frame_stride, T)
+#define FRH_TEN_U(K, P)                                                                           \
+    hipLaunchKernelGGL((k_tensor<K, P, true>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, \
+                       C, h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
+                       ay, J, cuv, frame_stride, T)
+    const bool unit = kind == kTensorGC && hy == 1.0 && hx == 1.0;
     switch (kind * 2 + (f2_f32 ? 1 : 0)) {
-    case 0: FRH_TEN(kTensorGC, false); break;
-    case 1: FRH_TEN(kTensorGC, true); break;
+    case 0:
+        if (unit) FRH_TEN_U(kTensorGC, false);
+        else FRH_TEN(kTensorGC, false);
+        break;
+    case 1:
+        if (unit) FRH_TEN_U(kTensorGC, true);
+        else FRH_TEN(kTensorGC, true);
+        break;
     case 2: FRH_TEN(kTensorGray, false); break;
     case 3: FRH_TEN(kTensorGray, true); break;
     case 4: FRH_TEN(kTensorCS, false); break;
@@ -835,6 +854,7 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
     default: return hipErrorInvalidValue;
     }
 #undef FRH_TEN
+#undef FRH_TEN_U
     return hipGetLastError();
 }
 
