@@ -33,6 +33,8 @@ extern "C" {
 #define FRH_EUNSUP (-4)   /* valid request this build does not implement */
 
 const char *frh_last_error(void);
+/* Incremented whenever entry points are added or change; 3 adds
+ * frh_warp_int16 and frh_mean_frames_f32. */
 int frh_abi_version(void);
 
 /* ------------------------------------------------------------ host only */
