@@ -651,16 +651,16 @@ __device__ inline TensorOut tensor_gc(const float *f1, const float *f2, int h, i
 }
 
 // core/optical_flow.py:163-228
-template <bool P1, bool P2>
+template <bool P1, bool P2, bool UNIT = false>
 __device__ inline TensorOut tensor_gray(const float *f1, const float *f2, int h, int w, int r,
                                         int c, double hy, double hx)
 {
     constexpr bool PM = P1 && P2;
     const Clamp3 k = clamp3(h, w, r, c);
-    const double fx1 = dgrad<P1>(img(f1, w, r, k.cr), img(f1, w, r, k.cl), hx);
-    const double fy1 = dgrad<P1>(img(f1, w, k.rd, c), img(f1, w, k.ru, c), hy);
-    const double fx2 = dgrad<P2>(img(f2, w, r, k.cr), img(f2, w, r, k.cl), hx);
-    const double fy2 = dgrad<P2>(img(f2, w, k.rd, c), img(f2, w, k.ru, c), hy);
+    const double fx1 = dgrad<P1, UNIT>(img(f1, w, r, k.cr), img(f1, w, r, k.cl), hx);
+    const double fy1 = dgrad<P1, UNIT>(img(f1, w, k.rd, c), img(f1, w, k.ru, c), hy);
+    const double fx2 = dgrad<P2, UNIT>(img(f2, w, r, k.cr), img(f2, w, r, k.cl), hx);
+    const double fy2 = dgrad<P2, UNIT>(img(f2, w, k.rd, c), img(f2, w, k.ru, c), hy);
     const double fx = rnd<PM>(0.5 * rnd<PM>(fx1 + fx2));
     const double fy = rnd<PM>(0.5 * rnd<PM>(fy1 + fy2));
     const double ft = rnd<PM>(img(f2, w, r, c) - img(f1, w, r, c));
@@ -741,7 +741,7 @@ __device__ inline TensorOut tensor_cell(const float *f1, const float *f2, int h,
     if constexpr (KIND == kTensorGC)
         return tensor_gc<P1, P2, UNIT>(f1, f2, h, w, r, c, hy, hx);
     else if constexpr (KIND == kTensorGray)
-        return tensor_gray<P1, P2>(f1, f2, h, w, r, c, hy, hx);
+        return tensor_gray<P1, P2, UNIT>(f1, f2, h, w, r, c, hy, hx);
     else
         return tensor_cs<P1, P2>(f1, f2, h, w, r, c, hy, hx, eps);
 }
@@ -771,7 +771,7 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
 // of every lane of a row storing to a different diagonal.  The level image
 // f1 is float64 data in the reference; P2 marks a float32 warped image
 // (every level below the top).  UNIT: unit grid spacing (hy = hx = 1, the
-// finest level), gradient-constancy term only.
+// finest level), gradient- and gray-constancy terms.
 constexpr int kTensorRows = 16;
 constexpr int kTensorDiags = kTensorRows + 63;  // diagonals a tile touches
 
@@ -837,7 +837,7 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
     hipLaunchKernelGGL((k_tensor<K, P, true>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, \
                        C, h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
                        ay, J, cuv, frame_stride, T)
-    const bool unit = kind == kTensorGC && hy == 1.0 && hx == 1.0;
+    const bool unit = kind != kTensorCS && hy == 1.0 && hx == 1.0;
     switch (kind * 2 + (f2_f32 ? 1 : 0)) {
     case 0:
         if (unit) FRH_TEN_U(kTensorGC, false);
@@ -847,8 +847,14 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
         if (unit) FRH_TEN_U(kTensorGC, true);
         else FRH_TEN(kTensorGC, true);
         break;
-    case 2: FRH_TEN(kTensorGray, false); break;
-    case 3: FRH_TEN(kTensorGray, true); break;
+    case 2:
+        if (unit) FRH_TEN_U(kTensorGray, false);
+        else FRH_TEN(kTensorGray, false);
+        break;
+    case 3:
+        if (unit) FRH_TEN_U(kTensorGray, true);
+        else FRH_TEN(kTensorGray, true);
+        break;
     case 4: FRH_TEN(kTensorCS, false); break;
     case 5: FRH_TEN(kTensorCS, true); break;
     default: return hipErrorInvalidValue;
