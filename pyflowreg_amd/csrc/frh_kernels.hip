@@ -679,7 +679,8 @@ __device__ inline TensorOut tensor_gray(const float *f1, const float *f2, int h,
 template <bool F32>
 __device__ inline double census_s(double nb, double ce, double dist, double eps2)
 {
-    const double rr = rnd<F32>(rnd<F32>(nb - ce) / rnd<F32>(dist));
+    // a unit distance (axis neighbours at unit spacing) divides exactly
+    const double rr = dist == 1.0 ? rnd<F32>(nb - ce) : rnd<F32>(rnd<F32>(nb - ce) / rnd<F32>(dist));
     const double d = rnd<F32>(sqrt(rnd<F32>(rnd<F32>(rr * rr) + rnd<F32>(eps2))));
     return rnd<F32>(0.5 * rnd<F32>(1.0 + rnd<F32>(rr / d)));
 }
@@ -688,7 +689,7 @@ __device__ inline double census_s(double nb, double ce, double dist, double eps2
 // order; per offset, the census images of both frames on the h x w core
 // (edge-padded, i.e. clamped) and their np.gradient derivatives at the
 // cell; float64 accumulation, then the scale 2 / 8.
-template <bool P1, bool P2>
+template <bool P1, bool P2, bool UNIT = false>
 __device__ inline TensorOut tensor_cs(const float *f1, const float *f2, int h, int w, int r, int c,
                                       double hy, double hx, double eps)
 {
@@ -709,10 +710,10 @@ __device__ inline TensorOut tensor_cs(const float *f1, const float *f2, int h, i
                 const double nb = img(f, w, nr, nc), ce = img(f, w, rr, cc);
                 return f32 ? census_s<true>(nb, ce, dist, eps2) : census_s<false>(nb, ce, dist, eps2);
             };
-            const double sx1 = dgrad<P1>(s_at(f1, P1, r, k.cr), s_at(f1, P1, r, k.cl), hx);
-            const double sx2 = dgrad<P2>(s_at(f2, P2, r, k.cr), s_at(f2, P2, r, k.cl), hx);
-            const double sy1 = dgrad<P1>(s_at(f1, P1, k.rd, c), s_at(f1, P1, k.ru, c), hy);
-            const double sy2 = dgrad<P2>(s_at(f2, P2, k.rd, c), s_at(f2, P2, k.ru, c), hy);
+            const double sx1 = dgrad<P1, UNIT>(s_at(f1, P1, r, k.cr), s_at(f1, P1, r, k.cl), hx);
+            const double sx2 = dgrad<P2, UNIT>(s_at(f2, P2, r, k.cr), s_at(f2, P2, r, k.cl), hx);
+            const double sy1 = dgrad<P1, UNIT>(s_at(f1, P1, k.rd, c), s_at(f1, P1, k.ru, c), hy);
+            const double sy2 = dgrad<P2, UNIT>(s_at(f2, P2, k.rd, c), s_at(f2, P2, k.ru, c), hy);
             const double sx = rnd<PM>(0.5 * rnd<PM>(sx1 + sx2));
             const double sy = rnd<PM>(0.5 * rnd<PM>(sy1 + sy2));
             const double st = rnd<PM>(s_at(f2, P2, r, c) - s_at(f1, P1, r, c));
@@ -743,7 +744,7 @@ __device__ inline TensorOut tensor_cell(const float *f1, const float *f2, int h,
     else if constexpr (KIND == kTensorGray)
         return tensor_gray<P1, P2, UNIT>(f1, f2, h, w, r, c, hy, hx);
     else
-        return tensor_cs<P1, P2>(f1, f2, h, w, r, c, hy, hx, eps);
+        return tensor_cs<P1, P2, UNIT>(f1, f2, h, w, r, c, hy, hx, eps);
 }
 
 // cu = sum_nb w_nb (u_nb - u_c) over (left, right, down, up); u is
@@ -771,7 +772,7 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
 // of every lane of a row storing to a different diagonal.  The level image
 // f1 is float64 data in the reference; P2 marks a float32 warped image
 // (every level below the top).  UNIT: unit grid spacing (hy = hx = 1, the
-// finest level), gradient- and gray-constancy terms.
+// finest level), where the gradients' divisions by 2h and h^2 are exact.
 constexpr int kTensorRows = 16;
 constexpr int kTensorDiags = kTensorRows + 63;  // diagonals a tile touches
 
@@ -837,7 +838,7 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
     hipLaunchKernelGGL((k_tensor<K, P, true>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, \
                        C, h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
                        ay, J, cuv, frame_stride, T)
-    const bool unit = kind != kTensorCS && hy == 1.0 && hx == 1.0;
+    const bool unit = hy == 1.0 && hx == 1.0;
     switch (kind * 2 + (f2_f32 ? 1 : 0)) {
     case 0:
         if (unit) FRH_TEN_U(kTensorGC, false);
@@ -855,8 +856,14 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
         if (unit) FRH_TEN_U(kTensorGray, true);
         else FRH_TEN(kTensorGray, true);
         break;
-    case 4: FRH_TEN(kTensorCS, false); break;
-    case 5: FRH_TEN(kTensorCS, true); break;
+    case 4:
+        if (unit) FRH_TEN_U(kTensorCS, false);
+        else FRH_TEN(kTensorCS, false);
+        break;
+    case 5:
+        if (unit) FRH_TEN_U(kTensorCS, true);
+        else FRH_TEN(kTensorCS, true);
+        break;
     default: return hipErrorInvalidValue;
     }
 #undef FRH_TEN
