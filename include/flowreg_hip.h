@@ -33,8 +33,8 @@ extern "C" {
 #define FRH_EUNSUP (-4)   /* valid request this build does not implement */
 
 const char *frh_last_error(void);
-/* Incremented whenever entry points are added or change; 3 adds
- * frh_warp_int16 and frh_mean_frames_f32. */
+/* Incremented whenever entry points are added or change; 3 added
+ * frh_warp_int16 and frh_mean_frames_f32, 4 adds frh_warp_u8. */
 int frh_abi_version(void);
 
 /* ------------------------------------------------------------ host only */
@@ -149,6 +149,18 @@ int frh_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc_f64,
  * frh_warp_cubic_ex. */
 int frh_warp_int16(int linear, int is_signed, const float *f2, const float *f1, int f1_per_frame, int T,
                    int C, int H, int W, const double *u, const double *v, float *out, void *stream);
+
+/* imregister_wrapper(f2, u, v, f1, ...) with uint8 raw frames f2
+ * (core/warping.py:219-227: cv2.remap keeps the 8-bit depth).  OpenCV's 8U
+ * remap weighs the taps with its 15-bit fixed-point table (the float weight
+ * products times 2^15, rounded, their sum fixed up to 2^15 on one central
+ * cell), sums the integer samples exactly and stores
+ * saturate_cast<uchar>((sum + 2^14) >> 15).  f2 holds the samples converted
+ * to float32 (exact); f1, the out-of-bounds fill, is float32 and stored as
+ * it is.  linear = 1: INTER_LINEAR, else INTER_CUBIC; other arguments as
+ * frh_warp_cubic_ex. */
+int frh_warp_u8(int linear, const float *f2, const float *f1, int f1_per_frame, int T, int C, int H, int W,
+                const double *u, const double *v, float *out, void *stream);
 
 /* Data terms (core/optical_flow.py:446-473, _resolve_motion_tensor_func). */
 #define FRH_TENSOR_GC 0   /* "gc" / "gradient": get_motion_tensor_gc, :73-160 */

@@ -19,26 +19,40 @@
  * the pyflowreg repository root).  Arrays are C-contiguous, interleaved
  * (rows, cols, channels) like the reference's numpy arrays.
  *
- * Pinning (what checks each function, see tests/test_oracle.py):
- *   or_warping_depth        closed form + level table (19 levels at 512^2)
- *   or_fill_resize_tables   linear-ramp reproduction, partition of unity,
- *                           identity at scale 1
- *   or_resize_tab           scale-1 identity, ramp reproduction, constants
+ * Pinning (what checks each function; tests/test_oracle.py unless named):
+ *   or_warping_depth        closed form + level tables (19 levels at 512^2,
+ *                           16 / 21 / 22 at 256^2, 796x512, 1024^2)
+ *   or_fill_resize_tables   partition of unity, indices in range
+ *   or_resize_tab           identity at scale 1; upsampling = torch's Keys
+ *                           weights on the reference's 3-tap window;
+ *                           downsampling = scipy-style float32 Gaussian then
+ *                           torch's Keys resampling, minus the dropped tap
  *   or_warp_cubic(64)       reference warp tests (zero / integer shifts,
- *   or_warp_linear(64)      out-of-bounds <- fixed), ramp reproduction with
- *                           1/32-px quantization (bilinear: exact on a
- *                           ramp at the quantized position); the 64F sums
- *                           against a float128 evaluation of the same taps
+ *                           out-of-bounds <- fixed); torch bicubic at the
+ *                           1/32-px-rounded positions (a 0.3 px shift lands
+ *                           at 0.3125); the 64F sums against numpy float64
+ *   or_warp_linear(64)      the same reference tests; torch bilinear at the
+ *                           rounded positions (tests/test_xcorr_oracle.py)
+ *   16-bit frames           (oracle.py imregister_wrapper over or_warp_cubic /
+ *                           or_warp_linear) identity at the type's extremes,
+ *                           half-pixel rounding half to even, clamped cubic
+ *                           overshoot
  *   or_motion_tensor_gc     reference known answer J11=0.8, identical frames
- *                           give J33=J13=J23=0, quadratic closed forms
+ *                           give J33=J13=J23=0, quadratic closed forms,
+ *                           numpy's float32 arithmetic for the warped image
  *   or_compute_flow         f1=f2 gives zero flow; a_data=1 converges to the
- *                           dense direct solve; one sweep equals the matrix
- *                           form of SOR in column-major order
+ *                           dense direct solve; 1-3 sweeps equal the matrix
+ *                           form of SOR in column-major order; the literal
+ *                           per-sweep expression order of the reference
+ *   or_cubic_resize_cv      torch's Keys (A = -0.75) weights at OpenCV's
+ *                           float32 source coordinates (512^2, 256^2, 796x512
+ *                           from min_level 4 and 6), constants, identity at
+ *                           scale 1.  Which table cell OpenCV's own float
+ *                           coefficient evaluation rounds differently is
+ *                           parity unpinned (cv2 absent).
  *   (the level loop, the median and the Gaussian prefilter live in
  *   oracle/oracle.py; the median and Gaussian call scipy, the reference's
  *   own dependency, directly)
- *   or_cubic_resize_cv      partially pinned (ramp reproduction, identity);
- *                           OpenCV resize tables are parity unpinned
  */
 #include <tgmath.h>
 #include <stdint.h>
@@ -306,6 +320,86 @@ void or_warp_cubic64(const double *f2, const double *f1, int H, int W, int C,
                             sum += f2[((size_t)ys[i] * W + xs[j]) * C + c] * (double)w[i * 4 + j];
                 }
                 out[p * C + c] = (float)sum;
+            }
+        }
+}
+
+/* cv2.remap on 8-bit images (opencv-contrib-python>=4.10, third party;
+ * imgwarp.cpp initInterTab2D / remapBicubic / remapBilinear with
+ * FixedPtCast<int, uchar, 15>, restated; parity with OpenCV itself is
+ * unpinned).  The 2-D weights of table cell (fx, fy) are the float products
+ * wy[k1] * wx[k2] of the 1-D float table, scaled by 2^15 and rounded to
+ * int16 (cvRound, saturated: a weight of exactly 1 becomes 32767).  When
+ * they do not sum to 2^15, the difference is taken off the largest (sum too
+ * small) or put on the smallest (sum too large) of the cells
+ * (k1, k2) in [ks/2, ks/2 + 2)^2, scanned row by row, first extreme wins.
+ * For the bilinear 2x2 table that window reaches past the block; OpenCV
+ * reads the next, still unfilled (zero) table cell there, restated as
+ * zeros.  The sample is the exact integer sum of the taps (clamped
+ * indices, BORDER_REPLICATE), stored as saturate((sum + 2^14) >> 15). */
+void or_remap_u8_weights(int linear, int fx, int fy, int *itab)
+{
+    const int ks = linear ? 2 : 4;
+    float tx[4], ty[4];
+    if (linear) {
+        tx[1] = (float)fx * (1.f / 32); tx[0] = 1.f - tx[1];
+        ty[1] = (float)fy * (1.f / 32); ty[0] = 1.f - ty[1];
+    } else {
+        cv_cubic_coeffs((float)fx * (1.f / 32), tx);
+        cv_cubic_coeffs((float)fy * (1.f / 32), ty);
+    }
+    int buf[24] = {0}, isum = 0;
+    for (int k1 = 0; k1 < ks; ++k1)
+        for (int k2 = 0; k2 < ks; ++k2) {
+            float wv = ty[k1] * tx[k2];
+            long r = lrintf(wv * 32768.f);
+            r = r < -32768 ? -32768 : (r > 32767 ? 32767 : r);
+            buf[k1 * ks + k2] = (int)r;
+            isum += (int)r;
+        }
+    if (isum != 32768) {
+        int diff = isum - 32768, h = ks / 2, M1 = h, M2 = h, m1 = h, m2 = h;
+        for (int k1 = h; k1 < h + 2; ++k1)
+            for (int k2 = h; k2 < h + 2; ++k2) {
+                if (buf[k1 * ks + k2] < buf[m1 * ks + m2]) m1 = k1, m2 = k2;
+                else if (buf[k1 * ks + k2] > buf[M1 * ks + M2]) M1 = k1, M2 = k2;
+            }
+        if (diff < 0) buf[M1 * ks + M2] = (int16_t)(buf[M1 * ks + M2] - diff);
+        else buf[m1 * ks + m2] = (int16_t)(buf[m1 * ks + m2] - diff);
+    }
+    for (int k = 0; k < ks * ks; ++k) itab[k] = buf[k];
+}
+
+void or_warp_u8(int linear, const uint8_t *f2, const float *f1, int H, int W, int C,
+                const double *u, const double *v, float *out)
+{
+    const int ks = linear ? 2 : 4, off = linear ? 0 : 1;
+    for (int y = 0; y < H; ++y)
+        for (int x = 0; x < W; ++x) {
+            size_t p = (size_t)y * W + x;
+            float mx = (float)((double)x + u[p]);
+            float my = (float)((double)y + v[p]);
+            if ((mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H)) {
+                for (int c = 0; c < C; ++c) out[p * C + c] = f1[p * C + c];
+                continue;
+            }
+            float cx = mx > (float)(W - 1) ? (float)(W - 1) : mx;
+            float cy = my > (float)(H - 1) ? (float)(H - 1) : my;
+            int X = (int)lrintf(cx * 32.f), Y = (int)lrintf(cy * 32.f);
+            int sx = (X >> 5) - off, sy = (Y >> 5) - off, itab[16], xs[4], ys[4];
+            or_remap_u8_weights(linear, X & 31, Y & 31, itab);
+            for (int i = 0; i < ks; ++i) {
+                int a = sx + i, b = sy + i;
+                xs[i] = a < 0 ? 0 : (a >= W ? W - 1 : a);
+                ys[i] = b < 0 ? 0 : (b >= H ? H - 1 : b);
+            }
+            for (int c = 0; c < C; ++c) {
+                int sum = 0;
+                for (int i = 0; i < ks; ++i)
+                    for (int j = 0; j < ks; ++j)
+                        sum += (int)f2[((size_t)ys[i] * W + xs[j]) * C + c] * itab[i * ks + j];
+                int r = (sum + (1 << 14)) >> 15;
+                out[p * C + c] = (float)(r < 0 ? 0 : (r > 255 ? 255 : r));
             }
         }
 }

@@ -18,13 +18,16 @@ code with float32 tensor/solver arithmetic; it exists only for the
 precision study.  ``order=1`` switches the SOR sweep to red-black ordering
 (ordering study only).
 
-Parity status per function (the checks live in tests/test_oracle.py):
-all functions are pinned except the fractional-position arithmetic of the
-OpenCV bicubic remap and of ``cv2.resize`` (third-party, restated from its
-public behaviour, cv2 absent here): "parity unpinned" for those two.  The
-16-bit remap's rounding and clamping are pinned by closed forms; that
-OpenCV's 16U / 16S path uses the 32F path's float weights and sums is
-restated from OpenCV's source and likewise unpinned against cv2 outputs.
+Parity status per function (the checks live in tests/test_oracle.py): every
+function has an independent pin.  The OpenCV remap (bicubic and bilinear) and
+``cv2.resize`` INTER_CUBIC are pinned against torch's Keys / bilinear weights
+at OpenCV's rounded positions (1/32 px for the remap, float32 source
+coordinates for the resize) to the float32 rounding of OpenCV's weights;
+the last bit of OpenCV's own float weight evaluation and summation order is
+restated from OpenCV's source and unpinned against cv2 outputs (cv2 absent
+here).  The 16-bit remap's rounding and clamping are pinned by closed forms;
+the 8-bit remap's fixed-point weights (15-bit integer table, rounded sums)
+by closed forms on integer shifts, half-pixel shifts and saturation.
 """
 
 from __future__ import annotations
@@ -74,6 +77,12 @@ def lib(precision: str = "f64") -> ctypes.CDLL:
     L.or_warp_linear.restype = None
     L.or_warp_linear64.argtypes = L.or_warp_cubic64.argtypes
     L.or_warp_linear64.restype = None
+    _u8p = np.ctypeslib.ndpointer(np.uint8, flags="C_CONTIGUOUS")
+    L.or_warp_u8.argtypes = [ctypes.c_int, _u8p, _f32p, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                             _f64p, _f64p, _f32p]
+    L.or_warp_u8.restype = None
+    L.or_remap_u8_weights.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, _i32p]
+    L.or_remap_u8_weights.restype = None
     tens = [_f64p, _f64p, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_double, ctypes.c_double]
     L.or_motion_tensor_gc.argtypes = tens + [ctypes.c_int, ctypes.c_int] + [rp] * 6
@@ -173,11 +182,25 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
     computes the 32F path's float sum on the integer samples and stores
     saturate_cast<T>(sum), i.e. rounded half to even and clamped to T's
     range, before the reference writes it into its float32 array; the
-    out-of-bounds fill is not rounded."""
+    out-of-bounds fill is not rounded.  uint8 images: OpenCV's 8U path
+    (15-bit fixed-point weight table, exact integer sums, rounded and
+    saturated to [0, 255]; or_warp_u8).  Other integer dtypes have no
+    cv2.remap path (the reference's cv2 call raises): ValueError."""
     method = interpolation_method.lower()
     if method not in ("cubic", "linear"):
         raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
+    raw = np.asarray(f2_level).dtype
+    if raw.kind in "iub" and raw not in (np.dtype(np.uint8), np.dtype(np.uint16), np.dtype(np.int16)):
+        raise ValueError(f"cv2.remap has no {raw} path")
     squeeze = f2_level.ndim == 2
+    if raw == np.dtype(np.uint8):
+        f2 = np.ascontiguousarray(f2_level if not squeeze else f2_level[:, :, None])
+        f1 = np.ascontiguousarray(f1_level if not squeeze else f1_level[:, :, None], dtype=np.float32)
+        H, W, C = f2.shape
+        out = np.empty((H, W, C), np.float32)
+        lib().or_warp_u8(int(method == "linear"), f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64),
+                         np.ascontiguousarray(v, dtype=np.float64), out)
+        return out[:, :, 0] if squeeze else out
     f64 = np.asarray(f2_level).dtype == np.float64
     dt = np.float64 if f64 else np.float32
     f2 = np.ascontiguousarray(f2_level if not squeeze else f2_level[:, :, None], dtype=dt)
@@ -190,7 +213,6 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
         fn = lib().or_warp_linear64 if f64 else lib().or_warp_linear
     fn(f2, f1, H, W, C, np.ascontiguousarray(u, dtype=np.float64), np.ascontiguousarray(v, dtype=np.float64),
        out)
-    raw = np.asarray(f2_level).dtype
     if raw in (np.dtype(np.uint16), np.dtype(np.int16)):
         gy, gx = np.meshgrid(np.arange(H), np.arange(W), indexing="ij")
         mx = (gx + np.asarray(u, dtype=np.float64)).astype(np.float32)
@@ -201,6 +223,15 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
             oc = out[:, :, c]
             oc[inb] = np.clip(np.rint(oc[inb]), info.min, info.max)
     return out[:, :, 0] if squeeze else out
+
+
+def remap_u8_weights(linear: bool, fx: int, fy: int) -> np.ndarray:
+    """OpenCV's 8-bit remap weight table cell (fx, fy) in 1/32 px, int16
+    values as int32: (2, 2) for linear, (4, 4) for cubic."""
+    ks = 2 if linear else 4
+    out = np.zeros(16, np.int32)
+    lib().or_remap_u8_weights(int(bool(linear)), int(fx), int(fy), out)
+    return out[:ks * ks].reshape(ks, ks)
 
 
 # ------------------------------------------------------------------ tensor

@@ -137,7 +137,7 @@ extern "C" {
 
 const char *frh_last_error(void) { return g_err; }
 
-int frh_abi_version(void) { return 3; }
+int frh_abi_version(void) { return 4; }
 
 int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
 {
@@ -302,6 +302,17 @@ int frh_warp_int16(int linear, int is_signed, const float *f2, const float *f1, 
     hipError_t e = launch_warp_saturate(linear != 0, f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, lo,
                                         hi, out, as_stream(stream));
     FRH_HIP(e, "16-bit warp kernel");
+    return FRH_OK;
+}
+
+int frh_warp_u8(int linear, const float *f2, const float *f1, int f1_per_frame, int T, int C, int H, int W,
+                const double *u, const double *v, float *out, void *stream)
+{
+    FRH_REQ(f2 && f1 && u && v && out, "frh_warp_u8: null pointer");
+    FRH_REQ(T > 0 && C > 0 && H > 0 && W > 0 && H <= 65535 && T <= 65535, "frh_warp_u8: bad shape");
+    FRH_HIP(launch_warp_u8(linear != 0, f2, f1, f1_per_frame, T, C, H, W, u, v, (size_t)H * W, out,
+                           as_stream(stream)),
+            "8-bit warp kernel");
     return FRH_OK;
 }
 

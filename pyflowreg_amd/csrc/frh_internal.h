@@ -57,14 +57,19 @@ hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T
 hipError_t launch_warp_linear_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame,
                                  int T, int C, int H, int W, const double *u, const double *v,
                                  size_t uv_frame_stride, float *out, hipStream_t st);
+// np.mean(w, axis=0) of n float32 frames of len values each
+hipError_t launch_mean_frames_f32(const float *w, int n, size_t len, float *out, hipStream_t st);
 // float32 images holding 16-bit integer samples: float sums, in-bounds
 // results rounded half to even and clamped to [lo, hi] (cv2's 16U / 16S
 // remap); linear selects INTER_LINEAR, else INTER_CUBIC
-// np.mean(w, axis=0) of n float32 frames of len values each
-hipError_t launch_mean_frames_f32(const float *w, int n, size_t len, float *out, hipStream_t st);
 hipError_t launch_warp_saturate(int linear, const float *f2, const float *f1, int f1_per_frame, int T, int C,
                                 int H, int W, const double *u, const double *v, size_t uv_frame_stride,
                                 float lo, float hi, float *out, hipStream_t st);
+// float32 images holding uint8 samples: cv2's 8U remap (15-bit fixed-point
+// weights, integer sums, rounded and saturated to [0, 255])
+hipError_t launch_warp_u8(int linear, const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                          int W, const double *u, const double *v, size_t uv_frame_stride, float *out,
+                          hipStream_t st);
 hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64, int f1_per_frame, int T,
                           int C, int H, int W, const double *u, const double *v, size_t uv_frame_stride,
                           float *out, hipStream_t st);

@@ -522,6 +522,108 @@ hipError_t launch_warp_saturate(int linear, const float *f2, const float *f1, in
     return hipGetLastError();
 }
 
+// cv2.remap on 8-bit images, OpenCV's 8U path (imgwarp.cpp: initInterTab2D,
+// remapBicubic / remapBilinear with FixedPtCast<int, uchar, 15>).  The map
+// handling is that of k_warp / k_warp_linear.  The weights of table cell
+// (X & 31, Y & 31) are the float products wy[k1] * wx[k2] of the 1-D float
+// table scaled by 2^15 and rounded to int16 (round half to even, saturated:
+// a weight of exactly 1 becomes 32767); when they do not sum to 2^15 the
+// difference is taken off the largest (sum too small) or put on the
+// smallest (sum too large) cell of the window (k1, k2) in [KS/2, KS/2 + 2)^2,
+// scanned row by row, first extreme kept.  For the 2x2 bilinear table the
+// window reaches past the block, where OpenCV reads the next, still unfilled
+// table cell: zeros here.  Each sample is the exact integer sum of its taps
+// (clamped indices: BORDER_REPLICATE), stored as clamp((sum + 2^14) >> 15,
+// 0, 255).  Samples arrive as float32 holding the uint8 values (exact).
+template <bool LINEAR>
+__global__ void k_warp_u8(const float *__restrict__ f2, const float *__restrict__ f1, int f1_per_frame, int C,
+                          int H, int W, const double *__restrict__ u, const double *__restrict__ v,
+                          size_t uv_stride, float *__restrict__ out)
+{
+    constexpr int KS = LINEAR ? 2 : 4, H2 = KS / 2;
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, t = blockIdx.z;
+    if (x >= W) return;
+    const size_t HW = (size_t)H * W, p = (size_t)y * W + x;
+    const float *F2 = f2 + (size_t)t * C * HW;
+    const float *F1 = f1 + (f1_per_frame ? (size_t)t * C * HW : 0);
+    float *O = out + (size_t)t * C * HW;
+    const float mx = (float)((double)x + u[t * uv_stride + p]);
+    const float my = (float)((double)y + v[t * uv_stride + p]);
+    if ((mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H)) {
+        for (int c = 0; c < C; ++c) O[c * HW + p] = F1[c * HW + p];
+        return;
+    }
+    const float cx = mx > (float)(W - 1) ? (float)(W - 1) : mx;
+    const float cy = my > (float)(H - 1) ? (float)(H - 1) : my;
+    const int X = (int)rintf(cx * 32.f), Y = (int)rintf(cy * 32.f);
+    const int sx = (X >> 5) - (LINEAR ? 0 : 1), sy = (Y >> 5) - (LINEAR ? 0 : 1);
+    float tx[4], ty[4];
+    if (LINEAR) {
+        tx[1] = (float)(X & 31) * (1.f / 32); tx[0] = 1.f - tx[1];
+        ty[1] = (float)(Y & 31) * (1.f / 32); ty[0] = 1.f - ty[1];
+    } else {
+        cv_cubic_coeffs((float)(X & 31) * (1.f / 32), tx);
+        cv_cubic_coeffs((float)(Y & 31) * (1.f / 32), ty);
+    }
+    int wt[KS * KS];
+    int isum = 0;
+#pragma unroll
+    for (int k1 = 0; k1 < KS; ++k1)
+#pragma unroll
+        for (int k2 = 0; k2 < KS; ++k2) {
+            const float wv = ty[k1] * tx[k2];
+            const int r = clampi((int)rintf(wv * 32768.f), -32768, 32767);
+            wt[k1 * KS + k2] = r;
+            isum += r;
+        }
+    if (isum != 32768) {
+        // window cells past the KS x KS block read as 0
+        int mi = H2 * KS + H2, Mi = mi, mval = wt[mi], Mval = wt[Mi];
+#pragma unroll
+        for (int k1 = H2; k1 < H2 + 2; ++k1)
+#pragma unroll
+            for (int k2 = H2; k2 < H2 + 2; ++k2) {
+                const int k = k1 * KS + k2;
+                const int val = (k1 < KS && k2 < KS) ? wt[(k1 < KS && k2 < KS) ? k : 0] : 0;
+                if (val < mval) { mi = k; mval = val; }
+                else if (val > Mval) { Mi = k; Mval = val; }
+            }
+        const int diff = isum - 32768, tgt = diff < 0 ? Mi : mi;
+#pragma unroll
+        for (int k = 0; k < KS * KS; ++k)
+            if (k == tgt) wt[k] = (int)(short)(wt[k] - diff);
+    }
+    int xs[KS], ys[KS];
+#pragma unroll
+    for (int i = 0; i < KS; ++i) {
+        xs[i] = clampi(sx + i, 0, W - 1);
+        ys[i] = clampi(sy + i, 0, H - 1);
+    }
+    for (int c = 0; c < C; ++c) {
+        const float *S = F2 + c * HW;
+        int sum = 0;
+#pragma unroll
+        for (int i = 0; i < KS; ++i)
+#pragma unroll
+            for (int j = 0; j < KS; ++j) sum += (int)S[(size_t)ys[i] * W + xs[j]] * wt[i * KS + j];
+        O[c * HW + p] = (float)clampi((sum + (1 << 14)) >> 15, 0, 255);
+    }
+}
+
+hipError_t launch_warp_u8(int linear, const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
+                          int W, const double *u, const double *v, size_t uv_frame_stride, float *out,
+                          hipStream_t st)
+{
+    const dim3 g((W + 63) / 64, H, T), b(64);
+    if (linear)
+        hipLaunchKernelGGL(k_warp_u8<true>, g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v, uv_frame_stride,
+                           out);
+    else
+        hipLaunchKernelGGL(k_warp_u8<false>, g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v, uv_frame_stride,
+                           out);
+    return hipGetLastError();
+}
+
 // numpy's np.mean(w, axis=0) of a C-contiguous float32 (n, len) array: the
 // reduction over the outer axis adds the rows one after another in float32
 // (pairwise summation applies only along a contiguous inner axis), and the

@@ -19,7 +19,7 @@ import torch
 
 from . import _lib
 from ._lib import check, lib, ptr, stream_handle
-from .optical_flow import DisplacementSolver, _resolve_const_assumption
+from .optical_flow import DisplacementSolver, _resolve_const_assumption, remap_depth
 
 _EXECUTORS = {}
 
@@ -46,16 +46,19 @@ def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor,
     frames (T, H, W, C) float32 (cv2.remap's 32F path) or float64 (its 64F
     path, double sums), flow (T, H, W, 2) of any float dtype (the map
     float32(x + u) is formed in float64), fill (H, W, C) -> (T, H, W, C)
-    float32.  raw_dtype uint16 / int16: frames hold those integer samples
-    (as float32) and cv2.remap works in their depth, rounding each in-bounds
-    result half to even and clamping it to the type's range
-    (frh_warp_int16)."""
+    float32.  raw_dtype: the frames' dtype before staging.  uint16 / int16:
+    frames hold those integer samples (as float32) and cv2.remap works in
+    their depth, rounding each in-bounds result half to even and clamping
+    it to the type's range (frh_warp_int16).  uint8: its 8U path, 15-bit
+    fixed-point weights and results rounded and saturated to [0, 255]
+    (frh_warp_u8).  Dtypes cv2.remap has no path for raise ValueError."""
     method = str(interpolation_method).lower()
     if method not in ("cubic", "linear"):
         raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
-    int16 = raw_dtype is not None and np.dtype(raw_dtype) in (np.dtype(np.uint16), np.dtype(np.int16))
-    if int16 and frames.dtype != torch.float32:
-        raise ValueError("warp_frames: 16-bit raw frames are passed as float32")
+    depth = remap_depth(raw_dtype) if raw_dtype is not None else None
+    integer = depth in ("u8", "u16", "s16")
+    if integer and frames.dtype != torch.float32:
+        raise ValueError("warp_frames: integer raw frames are passed as float32")
     T, H, W, C = frames.shape
     f64 = frames.dtype == torch.float64
     f2 = frames.permute(0, 3, 1, 2).contiguous()
@@ -63,8 +66,12 @@ def warp_frames(frames: torch.Tensor, flow: torch.Tensor, fill: torch.Tensor,
     u = flow[..., 0].to(torch.float64).contiguous()
     v = flow[..., 1].to(torch.float64).contiguous()
     out = torch.empty(f2.shape, dtype=torch.float32, device=f2.device)
-    if int16:
-        signed = int(np.dtype(raw_dtype) == np.dtype(np.int16))
+    if depth == "u8":
+        check(lib().frh_warp_u8(int(method == "linear"), ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u), ptr(v),
+                                ptr(out), stream_handle()), "frh_warp_u8")
+        return out.permute(0, 2, 3, 1)
+    if integer:
+        signed = int(depth == "s16")
         check(lib().frh_warp_int16(int(method == "linear"), signed, ptr(f2), ptr(f1), 0, T, C, H, W, ptr(u),
                                    ptr(v), ptr(out), stream_handle()), "frh_warp_int16")
         return out.permute(0, 2, 3, 1)
@@ -210,8 +217,8 @@ class HipBatchedExecutor:
         frames' dtype, reference_proc (H, W, C) and w_init (H, W, 2) (cast to
         float32 as the executors do).  cc = (cc_hw, cc_up) runs the
         cross-correlation prealignment first.  raw_dtype: the raw frames'
-        dtype before staging (uint16 / int16 frames warp in their integer
-        depth, see warp_frames).  Returns (registered float32, flow float32)
+        dtype before staging (uint8 / uint16 / int16 frames warp in their
+        integer depth, see warp_frames).  Returns (registered float32, flow float32)
         device tensors."""
         T, H, W, C = frames.shape
         fp = flow_params
@@ -257,6 +264,7 @@ class HipBatchedExecutor:
         as4 = lambda x: x[..., None] if x.ndim == 3 else x  # noqa: E731
         as3 = lambda x: x[..., None] if x.ndim == 2 else x  # noqa: E731
         batch = np.ascontiguousarray(as4(np.asarray(batch)))
+        remap_depth(batch.dtype)
         bp = np.ascontiguousarray(as4(np.asarray(batch_proc)).astype(np.float32, copy=False))
         rp = np.ascontiguousarray(as3(np.asarray(reference_proc)).astype(np.float32, copy=False))
         wi = np.ascontiguousarray(np.asarray(w_init).astype(np.float32, copy=False))

@@ -33,7 +33,7 @@ import torch
 
 from .compensate import RegistrationOptions, compensate_arr
 from .executor import warp_frames
-from .optical_flow import DisplacementSolver, _device
+from .optical_flow import DisplacementSolver, _device, remap_depth
 from .preprocess import PRE_FILT32, PRE_NORM32, preprocess
 
 
@@ -179,6 +179,7 @@ class FlowRegLive:
             self.reference_buffer.append(np.array(frame, copy=True))
             return frame, np.zeros(frame.shape[:2] + (2,), dtype=np.float32)
         frame = np.asarray(frame)
+        remap_depth(frame.dtype)
         if frame.ndim == 2:
             frame = frame[..., None]
         self.frame_count += 1
@@ -201,12 +202,10 @@ class FlowRegLive:
         solver.set_reference(self.reference_proc, weight, self.last_flow)
         flow = solver.run(proc.to(torch.float32)[None], out_dtype=torch.float64)[0]
         self.last_flow = flow
-        # cv2.remap keeps 16-bit frames in their depth (float sums, rounded
-        # results); other integer frames take the float64 path
-        if frame.dtype in (np.uint16, np.int16):
-            fwarp = fr.to(torch.float32)
-        else:
-            fwarp = fr.to(torch.float64) if fr.dtype == torch.float64 else fr.to(torch.float32)
+        # cv2.remap keeps 8-bit and 16-bit frames in their depth (fixed-point
+        # or float sums, rounded and saturated results) and has no path for
+        # other integer dtypes (warp_frames raises for those)
+        fwarp = fr.to(torch.float64) if frame.dtype == np.float64 else fr.to(torch.float32)
         interp = self.options.interpolation_method
         registered = warp_frames(fwarp[None], flow[None], self.reference_raw, interp, frame.dtype)[0]
         if self.frame_count % self.reference_update_interval == 0:

@@ -529,19 +529,39 @@ def imresize_fused_gauss_cubic(img, size, sigma_coeff=0.6, per_axis=False):
     return res.astype(in_dtype, copy=False)
 
 
+def remap_depth(raw_dtype) -> str:
+    """The cv2.remap code path (core/warping.py:219-227 calls it on the raw
+    frame) for a frame dtype: "f32", "f64", "u8", "u16" or "s16".  cv2.remap
+    has no path for other integer depths (8-bit signed, 32-bit, 64-bit) or
+    half floats, where the reference's call raises; so does this one
+    (ValueError)."""
+    d = np.dtype(raw_dtype)
+    table = {np.dtype(np.float32): "f32", np.dtype(np.float64): "f64", np.dtype(np.uint8): "u8",
+             np.dtype(np.uint16): "u16", np.dtype(np.int16): "s16"}
+    if d not in table:
+        raise ValueError(f"imregister_wrapper: cv2.remap has no {d} path (uint8, uint16, int16, float32 "
+                         "and float64 frames are registered)")
+    return table[d]
+
+
 def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
     """core/warping.py:160-234 (cubic or linear) on the GPU; numpy in, numpy
     (float32) out.  u, v are the displacements in level pixels (the
     reference's u / h_col, v / h_row) of any dtype: the map float32(x + u) is
     formed in float64, as numpy forms grid + u.  cv2.remap's arithmetic
     follows the image dtype: float64 images (get_displacement's level images,
-    the preprocessed frames of update_reference) sum in float64, other images
-    are converted to float32 and sum in float32."""
+    the preprocessed frames of update_reference) sum in float64, float32
+    images in float32; uint8, uint16 and int16 images take OpenCV's 8U and
+    16U / 16S paths (frh_warp_u8, frh_warp_int16); other dtypes raise
+    ValueError, as cv2.remap does."""
     method = interpolation_method.lower()
     if method not in ("cubic", "linear"):
         raise ValueError("Unsupported interpolation method. Use 'linear' or 'cubic'.")
     squeeze = np.asarray(f2_level).ndim == 2 if not isinstance(f2_level, torch.Tensor) else f2_level.ndim == 2
-    f64 = f2_level.dtype in (np.float64, torch.float64)
+    raw = (torch.empty((), dtype=f2_level.dtype).numpy().dtype if isinstance(f2_level, torch.Tensor)
+           else np.asarray(f2_level).dtype)
+    depth = remap_depth(raw)
+    f64 = depth == "f64"
     dt = torch.float64 if f64 else torch.float32
     f2 = _as_planes(f2_level, dt)
     f1 = _as_planes(f1_level, dt)
@@ -554,9 +574,17 @@ def imregister_wrapper(f2_level, u, v, f1_level, interpolation_method="cubic"):
         return torch.as_tensor(np.asarray(a).astype(np.float64), device=dev).contiguous()
     uu, vv = disp(u), disp(v)
     out = torch.empty((C, H, W), dtype=torch.float32, device=dev)
-    name = "frh_warp_cubic_ex" if method == "cubic" else "frh_warp_linear_ex"
-    check(getattr(lib(), name)(ptr(f2), ptr(f1), int(f64), int(f64), 0, 1, C, H, W, ptr(uu), ptr(vv),
-                               ptr(out), stream_handle()), name)
+    lin = int(method == "linear")
+    if depth == "u8":
+        check(lib().frh_warp_u8(lin, ptr(f2), ptr(f1), 0, 1, C, H, W, ptr(uu), ptr(vv), ptr(out),
+                                stream_handle()), "frh_warp_u8")
+    elif depth in ("u16", "s16"):
+        check(lib().frh_warp_int16(lin, int(depth == "s16"), ptr(f2), ptr(f1), 0, 1, C, H, W, ptr(uu), ptr(vv),
+                                   ptr(out), stream_handle()), "frh_warp_int16")
+    else:
+        name = "frh_warp_cubic_ex" if method == "cubic" else "frh_warp_linear_ex"
+        check(getattr(lib(), name)(ptr(f2), ptr(f1), int(f64), int(f64), 0, 1, C, H, W, ptr(uu), ptr(vv),
+                                   ptr(out), stream_handle()), name)
     res = out.permute(1, 2, 0)
     res = res[:, :, 0] if squeeze else res
     if isinstance(f2_level, torch.Tensor):

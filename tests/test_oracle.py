@@ -113,6 +113,100 @@ def test_resize_downsample_is_gaussian_then_cubic(oracle):
     assert checked > 20
 
 
+# ------------------------------------------- cv2.resize INTER_CUBIC (min_level > 0)
+def _torch_keys_at(n_in, coords):
+    """K[i, j] = torch's bicubic (Keys, A = -0.75, taps clamped to the edge)
+    weight of input j at source coordinate coords[i], from grid_sample of
+    unit impulses (align_corners=False: coordinate c <-> grid (2c + 1)/n - 1)."""
+    eye = torch.eye(n_in, dtype=torch.float64)[:, None, None, :]
+    g = (2.0 * np.asarray(coords, np.float64) + 1.0) / n_in - 1.0
+    grid = torch.zeros((n_in, 1, len(g), 2), dtype=torch.float64)
+    grid[..., 0] = torch.from_numpy(g)
+    out = torch.nn.functional.grid_sample(eye, grid, mode="bicubic", padding_mode="border",
+                                          align_corners=False)
+    return out[:, 0, 0, :].numpy().T
+
+
+def _cv_source_coords(n_in, n_out):
+    """OpenCV's resize source coordinate, formed in double and rounded to
+    float32: float((i + 0.5) * (1 / (n_out / n_in)) - 0.5)."""
+    return ((np.arange(n_out) + 0.5) * (1.0 / (n_out / n_in)) - 0.5).astype(np.float32)
+
+
+def _keys_resize(w, m, n, coords_y, coords_x):
+    Ky = _torch_keys_at(w.shape[0], coords_y)
+    Kx = _torch_keys_at(w.shape[1], coords_x)
+    return np.stack([Ky @ w[..., c] @ Kx.T for c in range(w.shape[2])], axis=-1)
+
+
+@pytest.mark.parametrize("size,min_level", [((512, 512), 4), ((512, 512), 6),
+                                            ((256, 256), 4), ((256, 256), 6),
+                                            ((796, 512), 4)])
+def test_cubic_resize_cv_final_upsample_matches_torch_bicubic(oracle, size, min_level):
+    """or_cubic_resize_cv: the final upsample of core/optical_flow.py:612-617
+    from the level that min_level stops at, against torch's Keys weights at
+    OpenCV's float32 source coordinates.  The remaining difference is the
+    float32 rounding of OpenCV's coefficients: at most 5.6e-6 px for flows of
+    a few pixels.  At 796 rows the scale 326/796 is not dyadic, so there a
+    coordinate kept in float64 misses by 1.7e-4; align_corners or a transposed
+    axis miss by more than 1e-3."""
+    m, n = size
+    plan, ml = oracle.level_plan(m, n, 0.8, 50, min_level)
+    assert ml == min_level
+    lh, lw = plan[-1][1]
+    assert (lh, lw) != (m, n)
+    rng = np.random.default_rng(min_level + m)
+    w = rng.uniform(-4.0, 4.0, (lh, lw, 2))
+    got = oracle.cubic_resize_cv(w, m, n)
+    assert got.shape == (m, n, 2)
+    want = _keys_resize(w, m, n, _cv_source_coords(lh, m), _cv_source_coords(lw, n))
+    d = np.abs(got - want).max()
+    assert d < 1e-5, d
+    # torch's own resize forms the coordinate in float64: equal where the scale
+    # is dyadic (512 -> 210, 256 -> 105 ...), about 1e-4 apart at 326/796
+    t = torch.from_numpy(np.ascontiguousarray(w.transpose(2, 0, 1)))[None]
+    plain = torch.nn.functional.interpolate(t, size=(m, n), mode="bicubic",
+                                            align_corners=False)[0].numpy().transpose(1, 2, 0)
+    if m == 796:
+        assert np.abs(got - plain).max() > 1e-4
+    else:
+        assert np.abs(got - plain).max() < 1e-5
+    aligned = torch.nn.functional.interpolate(t, size=(m, n), mode="bicubic",
+                                              align_corners=True)[0].numpy().transpose(1, 2, 0)
+    assert np.abs(got - aligned).max() > 1e-3
+    if m != n:
+        swapped = _keys_resize(w, m, n, _cv_source_coords(lh, m)[::-1].copy(),
+                               _cv_source_coords(lw, n))
+        assert np.abs(got - swapped).max() > 1e-3
+
+
+def test_cubic_resize_cv_constants_and_a(oracle):
+    """Constants pass through (OpenCV's last coefficient is 1 - the other
+    three, so the weights sum to one up to float32 rounding), and the kernel
+    is Keys' A = -0.75, not Catmull-Rom's A = -0.5: A = -0.5 reproduces a
+    linear ramp exactly, A = -0.75 does not, and the restatement follows the
+    latter by the amount torch's A = -0.75 weights predict."""
+    const = np.full((23, 31, 2), 2.5)
+    np.testing.assert_allclose(oracle.cubic_resize_cv(const, 57, 64), 2.5, rtol=0, atol=1e-6)
+    h, w, m, n = 41, 67, 96, 160
+    x = np.arange(w, dtype=np.float64)
+    field = np.ascontiguousarray(np.broadcast_to((0.25 * x + 3.0)[None, :, None], (h, w, 1)))
+    got = oracle.cubic_resize_cv(field, m, n)[:, :, 0]
+    fx = _cv_source_coords(w, n).astype(np.float64)
+    ci = (fx >= 2) & (fx < w - 3)
+    ramp = 0.25 * fx[ci] + 3.0
+    keys = (_torch_keys_at(w, fx) @ (0.25 * x + 3.0))[ci]
+    assert np.abs(got[0, ci] - ramp).max() > 1e-3
+    np.testing.assert_allclose(got[:, ci], np.broadcast_to(keys, (m, ci.sum())), rtol=0, atol=2e-6)
+
+
+def test_cubic_resize_cv_identity_at_scale_one(oracle):
+    """At scale 1 every source coordinate is an integer and the Keys
+    coefficients are (0, 1, 0, 0): the resize returns its input bit for bit."""
+    w = np.random.default_rng(7).uniform(-3, 3, (37, 53, 2))
+    np.testing.assert_array_equal(oracle.cubic_resize_cv(w, 37, 53), w)
+
+
 # -------------------------------------------------------------------- warp
 def test_warp_zero_displacement_is_identity(oracle):
     """tests/core/test_warping.py:22-33 (zero shift preserves frames)."""
@@ -671,3 +765,106 @@ def test_solver_nonlinear_smoothness_literal_order_agrees(oracle):
     a = oracle.compute_flow(*args)
     b = oracle.compute_flow(*args, order=2)
     np.testing.assert_allclose(a, b, atol=1e-11)
+
+
+# ------------------------------------------------ 8-bit frames (cv2's 8U remap)
+def test_remap_u8_weight_tables(oracle):
+    """or_remap_u8_weights: every table cell sums to exactly 2^15; the
+    bilinear cells are the exact products (32 - fx)(32 - fy) * 32 except cell
+    (0, 0), whose weight 1 saturates to 32767 and gets the missing unit at
+    (1, 1); the bicubic cells are torch's Keys (A = -0.75) weight products
+    times 2^15 to within 2 units (float32 table, rounding, the sum fix)."""
+    for fy in range(32):
+        for fx in range(32):
+            lin = oracle.remap_u8_weights(True, fx, fy)
+            assert lin.sum() == 32768
+            want = np.outer([32 - fy, fy], [32 - fx, fx]) * 32
+            if fx == 0 and fy == 0:
+                np.testing.assert_array_equal(lin, [[32767, 0], [0, 1]])
+            else:
+                np.testing.assert_array_equal(lin, want)
+    kx = _torch_keys_at(8, 3.0 + np.arange(32) / 32.0)[:, 2:6]   # taps 2..5 of source 3 + k/32
+    fixed = 0
+    for fy in range(32):
+        for fx in range(32):
+            cub = oracle.remap_u8_weights(False, fx, fy)
+            assert cub.sum() == 32768
+            if fx == 0 and fy == 0:
+                continue                        # the saturated cell, below
+            off = np.abs(cub - np.outer(kx[fy], kx[fx]) * 32768)
+            # rounding leaves every cell within half a unit (plus the float32
+            # table's error); the sum fix moves one cell of the lower-right
+            # central window by the rounding residue
+            big = np.argwhere(off > 0.52)
+            assert len(big) <= 1
+            if len(big):
+                assert tuple(big[0]) in {(2, 2), (2, 3), (3, 2), (3, 3)} and off.max() < 4
+                fixed += 1
+    assert fixed > 50
+    np.testing.assert_array_equal(oracle.remap_u8_weights(False, 0, 0),
+                                  [[0, 0, 0, 0], [0, 32767, 0, 0], [0, 0, 1, 0], [0, 0, 0, 0]])
+
+
+@pytest.mark.parametrize("method", ["cubic", "linear"])
+def test_warp_u8_identity_and_integer_shift(oracle, method):
+    """Reference warp tests (tests/core/test_warping.py:232-447) on uint8:
+    zero shift returns every sample, including 0 and 255; an integer shift
+    copies the content; positions mapped outside take the fixed frame."""
+    rng = np.random.default_rng(21)
+    H, W = 20, 24
+    f2 = rng.integers(0, 256, (H, W, 2), dtype=np.uint8)
+    f2[0, 0] = 0
+    f2[1, 1] = 255
+    f1 = rng.integers(0, 256, (H, W, 2), dtype=np.uint8).astype(np.float32)
+    z = np.zeros((H, W))
+    np.testing.assert_array_equal(oracle.imregister_wrapper(f2, z, z, f1, method), f2.astype(np.float32))
+    out = oracle.imregister_wrapper(f2, z + 3.0, z - 2.0, f1, method)
+    np.testing.assert_array_equal(out[2:, :W - 3], f2[:H - 2, 3:].astype(np.float32))
+    np.testing.assert_array_equal(out[:2], f1[:2])
+    np.testing.assert_array_equal(out[:, W - 3:], f1[:, W - 3:])
+
+
+def test_warp_u8_linear_half_pixel_rounds_half_up(oracle):
+    """A half-pixel linear shift weighs both taps 2^14: the 8U path stores
+    (a + b + 1) >> 1, i.e. halves round up (2.5 -> 3), where the 16-bit
+    path rounds half to even (2.5 -> 2) and the float path keeps 2.5."""
+    H, W = 4, 8
+    f2 = np.tile(np.array([2, 3, 1, 2, 254, 255, 7, 7], np.uint8), (H, 1))
+    u = np.full((H, W), 0.5)
+    out = oracle.imregister_wrapper(f2, u, np.zeros((H, W)), f2, "linear")
+    want = (f2[:, :-1].astype(np.int64) + f2[:, 1:] + 1) // 2
+    np.testing.assert_array_equal(out[:, :-1], want.astype(np.float32))
+    assert out[0, 0] == 3 and out[0, 4] == 255
+
+
+def test_warp_u8_cubic_saturates_and_tracks_torch(oracle):
+    """Keys' cubic next to a 0 -> 255 step overshoots on both sides; the 8U
+    path clamps to [0, 255] instead of wrapping, and elsewhere it is torch's
+    bicubic at the 1/32-px-rounded positions to within half a count."""
+    H, W = 16, 32
+    f2 = np.zeros((H, W), np.uint8)
+    f2[:, W // 2:] = 255
+    u = np.full((H, W), 0.4)
+    out = oracle.imregister_wrapper(f2, u, np.zeros((H, W)), f2, "cubic")
+    assert out.min() >= 0 and out.max() <= 255
+    f32 = oracle.imregister_wrapper(f2.astype(np.float32), u, np.zeros((H, W)), f2.astype(np.float32), "cubic")
+    assert f32.min() < -1 and f32.max() > 256          # the float path keeps the overshoot
+    inner = (slice(3, -3), slice(3, -4))
+    np.testing.assert_array_equal(out[inner], np.clip(np.floor(f32[inner] + 0.5), 0, 255))
+    rng = np.random.default_rng(5)
+    img = rng.integers(0, 256, (24, 28), dtype=np.uint8)
+    uu = rng.uniform(-0.9, 0.9, (24, 28))
+    vv = rng.uniform(-0.9, 0.9, (24, 28))
+    got = oracle.imregister_wrapper(img, uu, vv, img, "cubic")
+    yy, xx = np.meshgrid(np.arange(24), np.arange(28), indexing="ij")
+    mx = np.round((xx + uu).astype(np.float32) * 32) / 32
+    my = np.round((yy + vv).astype(np.float32) * 32) / 32
+    ref = np.clip(_torch_warp(img.astype(np.float32), mx, my), 0, 255)
+    assert np.abs(got[3:-3, 3:-3] - ref[3:-3, 3:-3]).max() <= 0.5 + 0.02
+
+
+@pytest.mark.parametrize("dt", [np.int8, np.int32, np.uint32])
+def test_warp_rejects_dtypes_cv2_remap_lacks(oracle, dt):
+    f2 = np.zeros((8, 8), dt)
+    with pytest.raises(ValueError):
+        oracle.imregister_wrapper(f2, np.zeros((8, 8)), np.zeros((8, 8)), f2)
