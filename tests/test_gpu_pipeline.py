@@ -21,7 +21,7 @@ def _oracle_batch(oracle, batch, ref_raw, ref_proc, w_init, fp):
     for t in range(batch.shape[0]):
         f = oracle.get_displacement(ref_proc, bp[t], uv=w_init.astype(np.float32).copy(),
                                     **fp).astype(np.float32)
-        raw = batch[t] if batch.dtype in (np.uint16, np.int16) else batch[t].astype(np.float32)
+        raw = batch[t] if batch.dtype in (np.uint8, np.uint16, np.int16) else batch[t].astype(np.float32)
         reg = oracle.imregister_wrapper(raw, f[..., 0].astype(np.float64),
                                         f[..., 1].astype(np.float64), ref_raw.astype(np.float32))
         flows.append(f)
@@ -81,6 +81,84 @@ def test_warp_frames_int16_matches_oracle(gpu, oracle, dt, method):
     f32 = oracle.imregister_wrapper(frames[0].astype(np.float32), flow[0, ..., 0].astype(np.float64),
                                     flow[0, ..., 1].astype(np.float64), fill, method)
     assert np.any(out[0] != f32)  # the rounding and clamping did something
+
+
+@pytest.mark.parametrize("method", ["cubic", "linear"])
+def test_warp_frames_u8_matches_oracle_and_stays_in_range(gpu, oracle, method):
+    """uint8 raw frames warp in cv2's 8U depth: 15-bit fixed-point weights,
+    exact integer sums, rounded and saturated results.  Step edges 0 -> 255
+    make the cubic overshoot on both sides; every in-bounds sample must stay
+    in [0, 255] and be an integer (a float path cast to uint8 would wrap)."""
+    import torch
+    from pyflowreg_amd.executor import warp_frames
+    rng = np.random.default_rng(27)
+    T, H, W, C = 3, 33, 41, 2
+    frames = rng.integers(0, 256, (T, H, W, C), dtype=np.uint8)
+    frames[0, :, : W // 2] = 0
+    frames[0, :, W // 2:] = 255
+    frames[1, 8:24, 8:24] = 255
+    frames[1, 12:20, 12:20] = 0
+    fill = rng.integers(0, 256, (H, W, C)).astype(np.float64) + 0.25
+    flow = rng.uniform(-2.5, 2.5, (T, H, W, 2)).astype(np.float32)
+    flow[0, ..., 0] = 0.4
+    flow[0, ..., 1] = 0.0
+    dev = torch.device("cuda")
+    out = warp_frames(torch.from_numpy(frames.astype(np.float32)).to(dev), torch.from_numpy(flow).to(dev),
+                      torch.from_numpy(fill).to(dev), method, raw_dtype=np.uint8).cpu().numpy()
+    for t in range(T):
+        ref = oracle.imregister_wrapper(frames[t], flow[t, ..., 0].astype(np.float64),
+                                        flow[t, ..., 1].astype(np.float64), fill.astype(np.float32), method)
+        np.testing.assert_array_equal(out[t], ref, err_msg=f"frame {t}")
+    inb = out[0][:, :W - 1]
+    assert inb.min() >= 0 and inb.max() <= 255 and np.all(inb == np.rint(inb))
+    if method == "cubic":
+        f32 = oracle.imregister_wrapper(frames[0].astype(np.float32), flow[0, ..., 0].astype(np.float64),
+                                        flow[0, ..., 1].astype(np.float64), fill.astype(np.float32), method)
+        assert f32.min() < 0 and f32.max() > 255   # what the float path would have stored
+
+
+def test_imregister_wrapper_integer_depths(gpu, oracle):
+    """The stand-alone imregister_wrapper keeps uint8 / uint16 / int16
+    frames in their cv2.remap depth and rejects depths cv2.remap lacks."""
+    rng = np.random.default_rng(28)
+    H, W = 21, 26
+    u = rng.uniform(-1.5, 1.5, (H, W))
+    v = rng.uniform(-1.5, 1.5, (H, W))
+    for dt in (np.uint8, np.uint16, np.int16):
+        info = np.iinfo(dt)
+        f2 = rng.integers(info.min, info.max, (H, W, 2), endpoint=True).astype(dt)
+        f1 = rng.random((H, W, 2)).astype(np.float32) * 100
+        for method in ("cubic", "linear"):
+            a = gpu.imregister_wrapper(f2, u, v, f1, method)
+            b = oracle.imregister_wrapper(f2, u, v, f1, method)
+            np.testing.assert_array_equal(a, b, err_msg=f"{np.dtype(dt)} {method}")
+    for dt in (np.int8, np.int32):
+        with pytest.raises(ValueError):
+            gpu.imregister_wrapper(np.zeros((H, W), dt), u, v, np.zeros((H, W), np.float32))
+
+
+def test_executor_u8_frames_match_oracle(gpu, oracle):
+    """process_batch with uint8 raw frames: registered frames are uint8,
+    equal to cv2's 8U remap result (and the fill) as the reference stores
+    them."""
+    from pyflowreg_amd.executor import HipBatchedExecutor
+    v = np.clip(np.rint(_video(5, 40, 36, 2, 29) * 200.0), 0, 255).astype(np.uint8)
+    ref_raw = v[0].astype(np.float64)
+    ref_proc = oracle.preprocess(ref_raw).astype(np.float32)
+    batch = v[1:]
+    w_init = np.zeros((40, 36, 2), np.float32)
+    w_init[..., 1] = -1.25
+    fp = dict(alpha=(2.0, 2.0), weight=np.ones((40, 36, 2)) * 0.5, levels=50, min_level=0,
+              eta=0.8, iterations=50, update_lag=5, a_data=0.45, a_smooth=1.0)
+    ex = HipBatchedExecutor(max_batch=4)
+    bp = oracle.preprocess(batch, ref_raw).astype(np.float32)
+    reg, w = ex.process_batch(batch, bp, ref_raw, ref_proc, w_init, flow_params=fp)
+    oreg, ow = _oracle_batch(oracle, batch, ref_raw, ref_proc, w_init, fp)
+    np.testing.assert_array_equal(w, ow)
+    assert reg.dtype == np.uint8
+    np.testing.assert_array_equal(reg, oreg.astype(np.uint8))
+    with pytest.raises(ValueError):
+        ex.process_batch(batch.astype(np.int8), bp, ref_raw, ref_proc, w_init, flow_params=fp)
 
 
 @pytest.mark.parametrize("dt", [np.uint16, np.int16])
