@@ -44,6 +44,9 @@ from .optical_flow import _resolve_const_assumption
 from .preprocess import _range, preprocess
 
 _QUALITY_MIN_LEVEL = {"quality": 0, "balanced": 4, "fast": 6}
+_BUILTIN_BACKENDS = ("flowreg", "flowreg_cuda", "flowreg_torch")
+_OUTPUT_TYPES = {"single": np.float32, "double": np.float64, "uint8": np.uint8, "uint16": np.uint16,
+                 "int16": np.int16, "int32": np.int32}
 
 
 @dataclass
@@ -70,6 +73,11 @@ class RegistrationOptions:
     cc_initialization: bool = False
     cc_hw: object = 256
     cc_up: int = 1
+    output_typename: Optional[str] = "double"
+    flow_backend: Optional[str] = None
+    backend_params: dict = field(default_factory=dict)
+    get_displacement_impl: Optional[Callable] = None
+    get_displacement_factory: Optional[Callable] = None
     extra: dict = field(default_factory=dict)
 
     def __post_init__(self):
@@ -110,6 +118,19 @@ class RegistrationOptions:
             if hasattr(options, f):
                 v = getattr(options, f)
                 kw[f] = getattr(v, "value", v)
+        for f in ("output_typename", "get_displacement_impl", "get_displacement_factory"):
+            if hasattr(options, f):
+                kw[f] = getattr(options, f)
+        if getattr(options, "backend_params", None):
+            kw["backend_params"] = dict(options.backend_params)
+        # OFOptions' flow_backend names pyflowreg's own backends ("flowreg",
+        # the default, and its CuPy / torch variants): this pipeline is the
+        # flowreg_hip backend, so those names select it; any other name must
+        # be registered in this package's registry
+        fb = getattr(options, "flow_backend", None)
+        fb = getattr(fb, "value", fb)
+        if fb is not None and fb not in _BUILTIN_BACKENDS:
+            kw["flow_backend"] = fb
         if kw.get("channel_normalization") in ("joint",):
             kw["channel_normalization"] = "together"
         return cls(**kw)
@@ -212,8 +233,11 @@ class MotionCorrector:
     and only its outputs come back to the host (where numpy forms the means
     and statistics exactly as the reference does)."""
 
-    def __init__(self, reference, options=None, max_batch: int = 256):
+    def __init__(self, reference, options=None, max_batch: int = 256, get_disp: Optional[Callable] = None):
         self.opt = RegistrationOptions.from_any(options)
+        # None: this backend's batched solver; else a displacement callable
+        # called frame by frame (see _register_custom)
+        self.get_disp = get_disp
         ref = np.asarray(reference, dtype=np.float64)
         self.reference_raw = ref[..., None] if ref.ndim == 2 else ref
         H, W, C = self.reference_raw.shape
@@ -257,13 +281,43 @@ class MotionCorrector:
             self._ref_raw_dev[dtype] = torch.as_tensor(self.reference_raw, dtype=dtype, device=self.device)
         return self._ref_raw_dev[dtype]
 
-    def _register(self, frames, frames_proc, w_init, prealign=True, raw_dtype=None):
+    def _register(self, frames, frames_proc, w_init, prealign=True, raw_dtype=None, with_uv=True):
         fp = self._flow_params()
+        if self.get_disp is not None:
+            return self._register_custom(frames, frames_proc, w_init, fp, raw_dtype, with_uv)
         term = _resolve_const_assumption(fp.get("const_assumption", "gc"))
         cc = (self.opt.cc_hw, int(self.opt.cc_up)) if self.opt.cc_initialization and prealign else None
         return self.executor.register_device(frames, frames_proc, self._reference_raw(frames.dtype),
                                              self.reference_proc, w_init, fp, term,
                                              self.opt.interpolation_method, cc, raw_dtype)
+
+    def _register_custom(self, frames, frames_proc, w_init, fp, raw_dtype, with_uv):
+        """A displacement callable that is not this backend's solver
+        (get_displacement / get_displacement_factory / another registered
+        backend): called frame by frame on host arrays as the sequential
+        executor calls it (parallelization/sequential.py:141-145, float32
+        inputs, uv = a copy of w_init, the flow cast to float32); the
+        registration warp stays on the GPU.  with_uv=False is the reference's
+        serial w_init path for at most 4 frames (compensate_recording.py:
+        664-671), which passes no uv."""
+        if self.opt.cc_initialization:
+            raise ValueError("compensate_arr: cc_initialization with a custom displacement callable is not "
+                             "implemented by the flowreg_hip pipeline")
+        rp = np.ascontiguousarray(self.reference_proc.to(torch.float32).cpu().numpy())
+        bp = frames_proc.cpu().numpy()
+        wi = w_init.cpu().numpy() if isinstance(w_init, torch.Tensor) else np.asarray(w_init)
+        wi = np.ascontiguousarray(wi.astype(np.float32, copy=False))
+        flows = []
+        for t in range(bp.shape[0]):
+            if with_uv:
+                f = self.get_disp(rp, bp[t], uv=wi.copy(), **fp)
+            else:
+                f = self.get_disp(rp, bp[t], **fp)
+            flows.append(np.asarray(f).astype(np.float32, copy=False))
+        flow = torch.from_numpy(np.stack(flows)).to(frames.device)
+        reg = warp_frames(frames, flow, self._reference_raw(frames.dtype), self.opt.interpolation_method,
+                          raw_dtype)
+        return reg, flow
 
     def _launch(self, frames, raw_dtype=None):
         """Queue the GPU work of one batch (T, H, W, C) on the device:
@@ -286,7 +340,7 @@ class MotionCorrector:
             # the executor (with the prealignment), fewer through
             # get_displacement directly
             _, w0 = self._register(frames[:n_init], bp[:n_init], np.zeros((H, W, 2)), prealign=n_init > 4,
-                                   raw_dtype=raw_dtype)
+                                   raw_dtype=raw_dtype, with_uv=n_init > 4)
             self.w_init = mean_frames(w0)
         cur = self.w_init if self.opt.update_initialization_w else torch.zeros_like(self.w_init)
         reg, flow = self._register(frames, bp, cur, raw_dtype=raw_dtype)
@@ -392,22 +446,130 @@ class MotionCorrector:
         self.executor.cleanup()
 
 
+def _resolve_displacement(opt: RegistrationOptions):
+    """OFOptions.resolve_get_displacement (OF_options.py:1021-1082):
+    get_displacement_impl, else get_displacement_factory(**backend_params),
+    else the registered backend's factory(**backend_params).  Returns None
+    when the result is this backend's own get_displacement (the batched
+    GPU solver then registers whole batches), else the callable."""
+    from .backend_registry import get_backend
+    from .optical_flow import get_displacement as own
+    if opt.get_displacement_impl is not None:
+        fn = opt.get_displacement_impl
+    elif opt.get_displacement_factory is not None:
+        fn = opt.get_displacement_factory(**opt.backend_params)
+    else:
+        fn = get_backend(opt.flow_backend or "flowreg_hip")(**opt.backend_params)
+    return None if fn is own else fn
+
+
+def _resolve_executor(opt: RegistrationOptions, registration_config) -> str:
+    """compensate_recording.py:200-267 for this pipeline: the requested
+    parallelization must be one the backend supports, else a warning names
+    the fallback; None selects hip_batched (the reference auto-selects
+    among its built-in executors; flowreg_hip declares sequential and
+    hip_batched, which give identical results here)."""
+    import warnings
+    from .backend_registry import get_backend_executors
+    backend = opt.flow_backend or "flowreg_hip"
+    supported = get_backend_executors(backend)
+    name = getattr(registration_config, "parallelization", None)
+    if name is None and isinstance(registration_config, dict):
+        name = registration_config.get("parallelization")
+    if name is None:
+        return "hip_batched" if "hip_batched" in supported else "sequential"
+    if name in supported and name in ("sequential", "hip_batched"):
+        return name
+    reason = (f"Backend '{backend}' does not support '{name}' executor" if name not in supported
+              else f"Executor '{name}' is not available on this system")
+    warnings.warn(f"{reason}. Supported executors: {sorted(supported)}. Falling back to 'sequential'.")
+    return "sequential"
+
+
+def _fire(callbacks, what, *args):
+    """compensate_recording.py:823-867: a callback's exception becomes a
+    warning and the run goes on."""
+    import warnings
+    for cb in callbacks:
+        try:
+            cb(*args)
+        except Exception as e:  # noqa: BLE001 -- the reference catches every exception here
+            warnings.warn(f"{what} callback error: {e}")
+
+
 def compensate_arr(c1, c_ref, options=None,
-                   progress_callback: Optional[Callable[[int, int], None]] = None,
+                   progress_callback: Optional[Callable[[int, int], None]] = None, *,
+                   flow_backend: Optional[str] = None,
+                   backend_params: Optional[dict] = None,
+                   get_displacement: Optional[Callable] = None,
+                   get_displacement_factory: Optional[Callable[..., Callable]] = None,
+                   w_callback: Optional[Callable[[np.ndarray, int, int], None]] = None,
+                   registered_callback: Optional[Callable[[np.ndarray, int, int], None]] = None,
+                   registration_config=None,
                    max_batch: int = 256):
     """compensate_arr (motion_correction/compensate_arr.py:16-231) on the
-    GPU: returns (registered (T, H, W[, C]) in c1's dtype, w (T, H, W, 2)
-    float32)."""
+    GPU, with the reference's arguments:
+
+    * shapes: (T, H, W, C); (T, H, W) with a 2-D c_ref (returned as
+      (T, H, W)); (H, W) (returned as (H, W), the flow as (H, W, 2));
+      (H, W, C) with a 3-D c_ref is one frame (returned as (1, H, W, C)),
+      unless its last axis is longer than 4 (then (T, H, W), one channel);
+    * the registered frames are cast per ``options.output_typename``
+      (default "double": float64; "single", "uint8", "uint16", "int16",
+      "int32"; other values keep the frames' dtype);
+    * flow_backend / backend_params / get_displacement /
+      get_displacement_factory override the options' displacement callable
+      (resolved as OFOptions.resolve_get_displacement does).  This
+      backend's own get_displacement runs batched on the GPU; any other
+      callable is called frame by frame, with the warp on the GPU;
+    * progress_callback(current, total) after each batch (per frame with
+      registration_config.parallelization = "sequential"),
+      registered_callback(batch, start, end) and w_callback(w, start, end)
+      after it, in that order, as compensate_recording.py:786-867 fires
+      them; a raising callback is turned into a warning;
+    * registration_config: RegistrationConfig (or a dict) whose
+      parallelization names the executor ("hip_batched", default, or
+      "sequential"; others fall back to "sequential" with a warning).
+
+    Returns (registered, w float32 (T, H, W, 2))."""
     c1 = np.asarray(c1)
-    squeeze = c1.ndim == 3 and np.asarray(c_ref).ndim == 2
-    video = c1[..., None] if squeeze else c1
-    if video.ndim == 3:
-        video = video[None]
-    mc = MotionCorrector(c_ref, options, max_batch=max_batch)
+    c_ref = np.asarray(c_ref)
+    if c1.size == 0:
+        raise ValueError("Input array cannot be empty")
+    opt = RegistrationOptions.from_any(options)
+    over = {}
+    if flow_backend is not None:
+        over["flow_backend"] = flow_backend
+    if backend_params is not None:
+        over["backend_params"] = {**opt.backend_params, **dict(backend_params)}
+    if get_displacement is not None:
+        over["get_displacement_impl"] = get_displacement
+    if get_displacement_factory is not None:
+        over["get_displacement_factory"] = get_displacement_factory
+    if over:
+        from dataclasses import replace as _replace
+        opt = _replace(opt, **over)
+    get_disp = _resolve_displacement(opt)
+    executor = _resolve_executor(opt, registration_config)
+    orig_ndim = c1.ndim
+    squeeze = None
+    if c1.ndim == 3 and c_ref.ndim == 2:
+        video, ref, squeeze = c1[..., None], c_ref[..., None], "channel"
+    elif c1.ndim == 2:
+        video, ref, squeeze = c1[None, :, :, None], (c_ref[..., None] if c_ref.ndim == 2 else c_ref), "all"
+    elif c1.ndim == 3:
+        video = c1[None] if c1.shape[-1] <= 4 else c1[..., None]
+        ref = c_ref
+    else:
+        video, ref = c1, c_ref
+    if video.ndim != 4:
+        raise ValueError(f"compensate_arr: cannot read frames of shape {c1.shape}")
+    mc = MotionCorrector(ref, opt, max_batch=max_batch, get_disp=get_disp)
     T = video.shape[0]
     bs = int(mc.opt.buffer_size)
     registered = np.empty(video.shape, dtype=video.dtype)
     flows = np.empty(video.shape[:3] + (2,), dtype=np.float32)
+    progress = [progress_callback] if progress_callback is not None else []
     try:
         k = mc.stager.upload(video[0:bs])
         state = mc._launch(mc.stager.acquire(k), video.dtype)
@@ -424,11 +586,22 @@ def compensate_arr(c1, c_ref, options=None,
                 mc.stager.release(k)
             _, w = mc._finish(video.dtype, *state, out_reg=registered[s:e], out_w=flows[s:e])
             state = nxt
+            if executor == "sequential":
+                for cur in range(s + 1, e + 1):
+                    _fire(progress, "Progress", cur, T)
+            else:
+                _fire(progress, "Progress", e, T)
             mc._statistics(w)
-            if progress_callback is not None:
-                progress_callback(min(T, s + bs), T)
+            _fire([registered_callback] if registered_callback is not None else [], "Registered frames",
+                  registered[s:e], s, e)
+            _fire([w_callback] if w_callback is not None else [], "Displacement field", flows[s:e], s, e)
     finally:
         mc.close()
-    if squeeze:
-        registered = registered[..., 0]
+    if opt.output_typename in _OUTPUT_TYPES:
+        registered = registered.astype(_OUTPUT_TYPES[opt.output_typename])
+    if squeeze == "all":
+        registered = np.squeeze(registered)
+        flows = np.squeeze(flows, axis=0)
+    elif squeeze == "channel" and orig_ndim == 3:
+        registered = np.squeeze(registered, axis=-1)
     return registered, flows

@@ -5,9 +5,19 @@ the whole-pyramid gradient against central FP64 finite differences.
 
 Tolerances: stage forwards agree with torch to 1e-11 relative (the same
 float64 formulas in another evaluation order); gradients of one level solve
-to 1e-8 and of a whole pyramid to 1e-6 relative (rounding through 50 sweeps
-per level and sums in another order); finite differences (h = 1e-4) to 1e-3
-relative, the stated target for the adjoint gradient.
+to 1e-8 and of a whole pyramid to 1e-5 relative (rounding through 50 sweeps
+per level and sums in another order); central differences to 1e-3
+relative, the stated target for the adjoint gradient, at EVERY step of a
+decreasing sequence spanning a factor 10 (per direction, inside the window
+where the differences have converged: above it kinks of the map, below it
+float64 rounding of the loss dominate), so that one lucky step cannot pass a
+wrong adjoint.  h = 1e-4 is not usable: the
+registration map is piecewise smooth (medians, border crossings, the stiff
+robust weight) with kinks far closer than 1e-4 in these directions, so
+differences at that step do not estimate a derivative of this map
+(whatever computes the gradient); the pyramid check therefore runs at
+a_data = 0.7, and the default a_data = 0.45 is checked by finite
+differences on one level and against torch autograd on the full pyramid.
 """
 
 import numpy as np
@@ -404,10 +414,17 @@ def _directions(moving, seed):
 
 
 def _assert_fd(results, tol=1e-3):
+    """Every step of the sequence must agree with the analytic derivative,
+    and the differences must agree with each other (a converged sequence)."""
     for name, an, fds in results:
         rel = {h: abs(an - fd) / max(abs(an), 1e-300) for h, fd in fds.items()}
         print(f"{name}: analytic {an:.10e} " + " ".join(f"h={h:g}:{r:.1e}" for h, r in rel.items()), flush=True)
-        assert min(rel.values()) <= tol, (name, an, fds)
+    for name, an, fds in results:
+        rel = {h: abs(an - fd) / max(abs(an), 1e-300) for h, fd in fds.items()}
+        assert max(rel.values()) <= tol, (name, an, fds)
+        vals = list(fds.values())
+        spread = (max(vals) - min(vals)) / max(abs(an), 1e-300)
+        assert spread <= tol, (name, "finite differences not converged", fds)
 
 
 @pytest.mark.timeout(600)
@@ -419,28 +436,32 @@ def test_adjoint_gradient_vs_central_differences_128(gpu):
     The map is piecewise smooth: every level's 5x5 median and every warp
     sample that crosses the image border put kinks into it, spaced far below
     1e-4 in these directions, so central differences estimate the derivative
-    only for h well below that; the check takes the best of h = 1e-7, 1e-8,
-    1e-9 and needs 1e-3 relative.  a_data = 0.7: at 0.45 the robust weight
+    only for h well below that.  Measured convergence (relative error of
+    the difference, h = 1e-5 ... 1e-10): alpha converges for h in
+    [1e-6, 3e-8], a_data in [1e-7, 3e-9], moving-frame directions in
+    [1e-9, 1e-10] (kinks of the map are that close there); every step of
+    the sequence taken from the window must agree to 1e-3.  a_data = 0.7: at 0.45 the robust weight
     (E + 1e-5)^(a - 1) is stiff enough (d psi / dE ~ 1e7 where the fit is
     exact) that no step in FP64 isolates a smooth piece; the default exponent
     is covered by the torch autograd comparison and the single-level check."""
     fixed, moving = _frames(2, 128, 128, 2, seed=27)
     mask = np.zeros((128, 128))
     mask[8:-8, 8:-8] = 1.0
-    steps = dict(alpha=(1e-7, 1e-8, 1e-9), a_data=(1e-7, 1e-8, 1e-9), moving=(1e-7, 1e-8, 1e-9))
+    steps = dict(alpha=(3e-7, 1e-7, 3e-8), a_data=(1e-7, 3e-8, 1e-8), moving=(1e-9, 3e-10, 1e-10))
     _assert_fd(_fd_check(fixed, moving, mask, _directions(moving, 13), steps, a_data=0.7))
 
 
 @pytest.mark.timeout(600)
 def test_adjoint_gradient_vs_central_differences_single_level(gpu):
     """levels=0 at the default a_data = 0.45 (one 128x128 level, no warp
-    between levels): best of h = 1e-5 .. 1e-7 for alpha and a_data and of
-    h = 1e-7 .. 1e-9 along moving-frame directions, where the stiff robust
-    weight shortens the smooth pieces."""
+    between levels): each of h = 1e-6, 3e-7, 1e-7 for alpha and a_data and
+    of h = 1e-9, 3e-10, 1e-10 along moving-frame directions, where the stiff
+    robust weight shortens the smooth pieces (measured windows: alpha and
+    a_data [3e-6, 1e-8], moving [1e-9, 1e-10])."""
     fixed, moving = _frames(2, 128, 128, 2, seed=28)
     mask = np.zeros((128, 128))
     mask[8:-8, 8:-8] = 1.0
-    steps = dict(alpha=(1e-5, 1e-6, 1e-7), a_data=(1e-5, 1e-6, 1e-7), moving=(1e-7, 1e-8, 1e-9))
+    steps = dict(alpha=(1e-6, 3e-7, 1e-7), a_data=(1e-6, 3e-7, 1e-7), moving=(1e-9, 3e-10, 1e-10))
     _assert_fd(_fd_check(fixed, moving, mask, _directions(moving, 14), steps, levels=0))
 
 

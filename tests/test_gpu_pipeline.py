@@ -189,7 +189,8 @@ def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle, dt):
     """compensate_recording.py:645-678,786-806 with buffer_size 10 over 26
     frames: w_init from the first min(22, 10) frames, then the mean of each
     batch's last (up to) 20 flows.  uint16 frames: staged as float32,
-    registered in cv2's 16U depth and returned as uint16."""
+    registered in cv2's 16U depth, stored as uint16 and returned as float64
+    (output_typename "double", compensate_arr.py:194-207)."""
     from pyflowreg_amd.compensate import RegistrationOptions, compensate_arr
     v = _video(27, 40, 32, 2, 31) if dt == np.float32 else _int_video(27, 40, 32, 2, 31, dt)
     ref = v[0].astype(np.float64)
@@ -212,8 +213,10 @@ def test_compensate_arr_matches_oracle_batch_loop(gpu, oracle, dt):
         oregs.append(r)
         ows.append(ww)
     np.testing.assert_array_equal(w, np.concatenate(ows))
-    assert reg.dtype == dt
-    np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(dt))
+    # the pipeline stores the frames in the batch's dtype; compensate_arr
+    # then casts them per output_typename, "double" by default
+    assert reg.dtype == np.float64
+    np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(dt).astype(np.float64))
 
 
 def test_compensate_update_reference_matches_oracle(gpu, oracle):
@@ -430,8 +433,8 @@ def test_compensate_arr_reference_test_options(gpu, oracle, preset):
         oregs.append(r)
         ows.append(ww)
     np.testing.assert_array_equal(w, np.concatenate(ows))
-    assert reg.dtype == np.uint16
-    np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(np.uint16))
+    assert reg.dtype == np.float64  # output_typename "double"
+    np.testing.assert_array_equal(reg, np.concatenate(oregs).astype(np.uint16).astype(np.float64))
     assert np.abs(w).max() > 0.1  # the disc's motion is found
 
 
