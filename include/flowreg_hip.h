@@ -34,7 +34,9 @@ extern "C" {
 
 const char *frh_last_error(void);
 /* Incremented whenever entry points are added or change; 3 added
- * frh_warp_int16 and frh_mean_frames_f32, 4 adds frh_warp_u8. */
+ * frh_warp_int16 and frh_mean_frames_f32, 4 frh_warp_u8; 5 moved every
+ * stage's scratch to the caller (scratch arguments and *_scratch_bytes
+ * queries; frh_prefilter / frh_prefilter_f64 folded into frh_prefilter_ex). */
 int frh_abi_version(void);
 
 /* ------------------------------------------------------------ host only */
@@ -50,41 +52,42 @@ int frh_warping_depth(double eta, int levels, int m, int n);
 
 /* ------------------------------------------------------------ stages */
 
+/* Scratch.  The stage entry points below that need device memory beyond
+ * their inputs and outputs take it from the caller (scratch, at least
+ * scratch_bytes as returned by the matching *_scratch_bytes query; a
+ * smaller or missing buffer is rejected with FRH_EINVAL): no stage entry
+ * point allocates.  The scratch is used on `stream` only, so a buffer may be
+ * reused by the next call on the same stream. */
+
+/* Bytes of scratch frh_prefilter_ex (rt = 0) and frh_prefilter_3d need for
+ * T frames of H x W x C: the float64 planes between the two Gaussian
+ * passes, and with a temporal kernel (rt > 0) the frames after it. */
+size_t frh_prefilter_scratch_bytes(int T, int H, int W, int C, int rt);
+
 /* util/image_processing.py:12-98 (normalize) followed by :101-165
  * (apply_gaussian_filter, mode "reflect"), then the executors' float32 cast
  * (motion_correction/parallelization/base.py:63-73).
- * raw: (T, H, W, C) float32.  lo[c], den[c]: normalisation offset and
+ * raw: (T, H, W, C), float64 when raw_f64 != 0 (the reference frame is the
+ * float64 mean of raw frames, motion_correction/compensate_recording.py:
+ * 491-519), else float32.  lo[c], den[c]: normalisation offset and
  * denominator (max - min + eps) per channel, float64, host memory.
  * wy (2*ry+1), wx (2*rx+1): Gaussian weights, float64, host memory, the
- * values scipy.ndimage uses.  out: (T, C, H, W) float32 planes.
- * Arithmetic is float64 in scipy's summation order (vertical pass, then
- * horizontal). */
-int frh_prefilter(const float *raw, int T, int H, int W, int C, const double *lo,
-                  const double *den, const double *wy, int ry, const double *wx, int rx,
-                  float *out, void *stream);
-
-/* Same as frh_prefilter for float64 raw input (the reference frame is the
- * float64 mean of raw frames, motion_correction/compensate_recording.py:
- * 491-519). */
-int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const double *lo,
-                      const double *den, const double *wy, int ry, const double *wx, int rx,
-                      float *out, void *stream);
-
-/* frh_prefilter / frh_prefilter_f64 with the float64 result kept
- * (out_f64 = 1, out float64): the preprocessed frames before the executors'
+ * values scipy.ndimage uses.  out: (T, C, H, W) planes, float32, or with
+ * out_f64 = 1 float64: the preprocessed frames before the executors'
  * float32 cast, as update_reference (compensate_recording.py:680-713) and
- * FlowRegLive (flow_reg_live.py:216-323) use them.  raw is float64 when
- * raw_f64 != 0, else float32.  semantics: 0 for the float64 arithmetic of
- * frh_prefilter, or flags for a float32 frame normalised by its own
- * float32 range (util/image_processing.py:12-98 keeps numpy's float32 there,
- * FlowRegLive's reference): FRH_PRE_NORM32, (x - lo) / den in float32;
- * FRH_PRE_FILT32, each Gaussian pass stored as float32, as scipy's
- * gaussian_filter does for a float32 array. */
+ * FlowRegLive (flow_reg_live.py:216-323) use them.  Arithmetic is float64
+ * in scipy's summation order (vertical pass, then horizontal).
+ * semantics: 0 for that float64 arithmetic, or flags for a float32 frame
+ * normalised by its own float32 range (util/image_processing.py:12-98 keeps
+ * numpy's float32 there, FlowRegLive's reference): FRH_PRE_NORM32,
+ * (x - lo) / den in float32; FRH_PRE_FILT32, each Gaussian pass stored as
+ * float32, as scipy's gaussian_filter does for a float32 array. */
 #define FRH_PRE_NORM32 1
 #define FRH_PRE_FILT32 2
 int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
                      const double *den, const double *wy, int ry, const double *wx, int rx,
-                     void *out, int out_f64, int semantics, void *stream);
+                     void *out, int out_f64, int semantics, void *scratch, size_t scratch_bytes,
+                     void *stream);
 
 /* apply_gaussian_filter's 4-D branch with a temporal kernel
  * (util/image_processing.py:147-161; scipy's gaussian_filter filters the
@@ -95,7 +98,8 @@ int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, c
  * with semantics 0. */
 int frh_prefilter_3d(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
                      const double *den, const double *wt, int rt, const double *wy, int ry,
-                     const double *wx, int rx, void *out, int out_f64, void *stream);
+                     const double *wx, int rx, void *out, int out_f64, void *scratch,
+                     size_t scratch_bytes, void *stream);
 
 /* util/resize_util.py:116-141,232-254 (imresize_fused_gauss_cubic).
  * src: (N, h, w) planes; dst: (N, oh, ow); tmp: (N, h, ow) scratch.
@@ -193,17 +197,27 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33,
                      const double *weight, const double *u, const double *v, int m, int n,
                      int K, double alpha_x, double alpha_y, int iterations, int update_lag,
                      const double *a_data, double a_smooth, double hx, double hy,
-                     double gnc_beta, double *du, double *dv, void *stream);
+                     double gnc_beta, double *du, double *dv, void *scratch, size_t scratch_bytes,
+                     void *stream);
+/* Bytes of scratch frh_level_solver needs for (m, n, K) arrays: the skewed
+ * copies of J, the weights and the initial flow, the coefficients, du / dv
+ * and the boundary ring (and psi_s when a_smooth != 1). */
+size_t frh_level_solver_scratch_bytes(int m, int n, int K, double a_smooth);
 
 /* cv2.resize(w, (ow, oh), interpolation=cv2.INTER_CUBIC) of float64 flows
  * (core/optical_flow.py:615-616, only when min_level > 0), restated from
  * OpenCV's public behaviour.  src: (T, h, w, C), dst: (T, oh, ow, C). */
 int frh_resize_cubic_cv(const double *src, int T, int h, int w, int C, double *dst, int oh,
-                        int ow, void *stream);
+                        int ow, void *scratch, size_t scratch_bytes, void *stream);
+/* Bytes of scratch of frh_resize_cubic_cv: the horizontal pass's output. */
+size_t frh_resize_cubic_cv_scratch_bytes(int T, int h, int ow, int C);
 
 /* scipy.ndimage.median_filter(size=(5, 5), mode="mirror") as called at
  * core/optical_flow.py:607-609.  f: (N, h, w) float64 planes, in place. */
-int frh_median5_mirror(double *f, int N, int h, int w, void *stream);
+int frh_median5_mirror(double *f, int N, int h, int w, void *scratch, size_t scratch_bytes,
+                       void *stream);
+/* Bytes of scratch of frh_median5_mirror: a copy of the planes. */
+size_t frh_median5_mirror_scratch_bytes(int N, int h, int w);
 
 /* np.mean(w, axis=0) of n float32 frames of len values each, w C-contiguous
  * (n, len): the rows summed in order in float32, then divided by n

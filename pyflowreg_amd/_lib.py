@@ -15,6 +15,7 @@ LIB_PATH = os.path.join(_HERE, "libflowreg_hip.so")
 _lib = None
 
 c_int, c_double, c_void_p = ctypes.c_int, ctypes.c_double, ctypes.c_void_p
+c_size = ctypes.c_size_t
 c_i32p = ctypes.POINTER(ctypes.c_int32)
 c_f32p = ctypes.POINTER(ctypes.c_float)
 c_f64p = ctypes.POINTER(ctypes.c_double)
@@ -49,10 +50,6 @@ SIGNATURES = {
     "frh_abi_version": (c_int, []),
     "frh_fill_resize_tables": (c_int, [c_int, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     "frh_warping_depth": (c_int, [c_double, c_int, c_int, c_int]),
-    "frh_prefilter": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
-                              c_int, c_void_p, c_int, c_void_p, c_void_p]),
-    "frh_prefilter_f64": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
-                                  c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p]),
     "frh_resize": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_int, c_void_p,
                            c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     "frh_warp_cubic": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p,
@@ -65,19 +62,25 @@ SIGNATURES = {
                                c_void_p, c_void_p, c_void_p]),
     "frh_warp_u8": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
                             c_void_p, c_void_p]),
+    "frh_prefilter_scratch_bytes": (c_size, [c_int, c_int, c_int, c_int, c_int]),
     "frh_prefilter_ex": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
-                                 c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
+                                 c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_size, c_void_p]),
     "frh_prefilter_3d": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
-                                 c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+                                 c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_size,
+                                 c_void_p]),
     "frh_motion_tensor": (c_int, [c_int, c_void_p, c_void_p, c_int, c_int, c_double, c_double,
                                   c_double, c_int, c_int] + [c_void_p] * 6 + [c_void_p]),
     "frh_level_solver": (c_int, [c_void_p] * 9 + [c_int, c_int, c_int, c_double, c_double, c_int,
                                                   c_int, c_void_p, c_double, c_double, c_double,
-                                                  c_double, c_void_p, c_void_p, c_void_p]),
-    "frh_median5_mirror": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p]),
+                                                  c_double, c_void_p, c_void_p, c_void_p, c_size,
+                                                  c_void_p]),
+    "frh_level_solver_scratch_bytes": (c_size, [c_int, c_int, c_int, c_double]),
+    "frh_median5_mirror": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_size, c_void_p]),
+    "frh_median5_mirror_scratch_bytes": (c_size, [c_int, c_int, c_int]),
     "frh_mean_frames_f32": (c_int, [c_void_p, c_int, ctypes.c_size_t, c_void_p, c_void_p]),
     "frh_resize_cubic_cv": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_int,
-                                    c_void_p]),
+                                    c_void_p, c_size, c_void_p]),
+    "frh_resize_cubic_cv_scratch_bytes": (c_size, [c_int, c_int, c_int, c_int]),
     "frh_solver_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(SolverDesc)]),
     "frh_solver_destroy": (None, [c_void_p]),
     "frh_solver_set_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
@@ -138,6 +141,16 @@ def ptr(t) -> int:
     if hasattr(t, "data_ptr"):
         return t.data_ptr()
     return t.ctypes.data
+
+
+def scratch(nbytes: int, device=None):
+    """Caller-owned scratch for an entry point that takes one: a device
+    byte tensor from torch's caching allocator (stream-ordered reuse, no
+    device allocation once the pool holds a block of that size); keep it
+    alive until the call's work is queued."""
+    import torch
+    return torch.empty((max(int(nbytes), 1),), dtype=torch.uint8,
+                       device=device if device is not None else torch.device("cuda", torch.cuda.current_device()))
 
 
 def stream_handle(device=None) -> int:

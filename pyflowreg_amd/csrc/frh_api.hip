@@ -137,7 +137,7 @@ extern "C" {
 
 const char *frh_last_error(void) { return g_err; }
 
-int frh_abi_version(void) { return 4; }
+int frh_abi_version(void) { return 5; }
 
 int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
 {
@@ -176,74 +176,68 @@ int frh_warping_depth(double eta, int levels, int m, int n)
 
 // ------------------------------------------------------------------ stages
 
+// Scratch of the prefilter: the float64 planes between the vertical and the
+// horizontal pass, plus (temporal kernel) the float64 frames after the pass
+// along T.
+size_t frh_prefilter_scratch_bytes(int T, int H, int W, int C, int rt)
+{
+    if (T <= 0 || H <= 0 || W <= 0 || C <= 0 || rt < 0) return 0;
+    const size_t plane = sizeof(double) * (size_t)T * C * H * W;
+    return rt > 0 ? 2 * plane : plane;
+}
+
+#define FRH_SCRATCH(ptr_, have, need, what)                                                        \
+    FRH_REQ((ptr_) != nullptr && (have) >= (need), "%s: scratch of %zu bytes needed, %zu given", what, \
+            (size_t)(need), (size_t)((ptr_) ? (have) : 0))
+
 static int prefilter_impl(const void *raw, int f64, int T, int H, int W, int C, const double *lo,
                           const double *den, const double *wy, int ry, const double *wx, int rx,
-                          void *out, int out_f64, int sem, void *stream)
+                          void *out, int out_f64, int sem, double *tmp, void *stream)
 {
     FRH_REQ(raw && out && lo && den && wy && wx, "frh_prefilter: null pointer");
     FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0 && C <= 32, "frh_prefilter: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
     FRH_REQ(ry >= 0 && ry <= 32 && rx >= 0 && rx <= 32, "frh_prefilter: Gaussian radius must be in [0, 32]");
     hipStream_t st = as_stream(stream);
-    double *tmp = nullptr;
-    FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)T * C * H * W, st), "frh_prefilter scratch");
     hipError_t e = f64 ? launch_prefilter_v64((const double *)raw, T, H, W, C, lo, den, wy, ry, sem, tmp, st)
                        : launch_prefilter_v((const float *)raw, T, H, W, C, lo, den, wy, ry, sem, tmp, st);
     if (e == hipSuccess)
         e = out_f64 ? launch_prefilter_h64(tmp, T, H, W, C, wx, rx, sem, (double *)out, st)
                     : launch_prefilter_h(tmp, T, H, W, C, wx, rx, sem, (float *)out, st);
-    hipError_t e2 = hipFreeAsync(tmp, st);
     if (e != hipSuccess) return fail(FRH_EHIP, "prefilter kernels: %s", hipGetErrorString(e));
-    FRH_HIP(e2, "frh_prefilter free");
     return FRH_OK;
-}
-
-int frh_prefilter(const float *raw, int T, int H, int W, int C, const double *lo, const double *den,
-                  const double *wy, int ry, const double *wx, int rx, float *out, void *stream)
-{
-    return prefilter_impl(raw, 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, 0, stream);
-}
-
-int frh_prefilter_f64(const double *raw, int T, int H, int W, int C, const double *lo,
-                      const double *den, const double *wy, int ry, const double *wx, int rx,
-                      float *out, void *stream)
-{
-    return prefilter_impl(raw, 1, T, H, W, C, lo, den, wy, ry, wx, rx, out, 0, 0, stream);
 }
 
 int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
                      const double *den, const double *wy, int ry, const double *wx, int rx, void *out,
-                     int out_f64, int semantics, void *stream)
+                     int out_f64, int semantics, void *scratch, size_t scratch_bytes, void *stream)
 {
     FRH_REQ((semantics & ~(FRH_PRE_NORM32 | FRH_PRE_FILT32)) == 0, "frh_prefilter_ex: unknown semantics flags");
+    FRH_SCRATCH(scratch, scratch_bytes, frh_prefilter_scratch_bytes(T, H, W, C, 0), "frh_prefilter_ex");
     return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, semantics,
-                          stream);
+                          (double *)scratch, stream);
 }
 
 int frh_prefilter_3d(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
                      const double *den, const double *wt, int rt, const double *wy, int ry, const double *wx,
-                     int rx, void *out, int out_f64, void *stream)
+                     int rx, void *out, int out_f64, void *scratch, size_t scratch_bytes, void *stream)
 {
     FRH_REQ(raw && out && lo && den && wt && wy && wx, "frh_prefilter_3d: null pointer");
     FRH_REQ(rt >= 0 && rt <= 32, "frh_prefilter_3d: temporal radius must be in [0, 32]");
     FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0 && C <= 32 && H <= 65535 && T <= 65535,
             "frh_prefilter_3d: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
+    FRH_SCRATCH(scratch, scratch_bytes, frh_prefilter_scratch_bytes(T, H, W, C, rt), "frh_prefilter_3d");
+    double *tmp = (double *)scratch;
     if (rt == 0)  // the temporal kernel is [1]: an exact identity
-        return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, 0, stream);
+        return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, 0, tmp,
+                              stream);
     hipStream_t st = as_stream(stream);
-    double *tf = nullptr;
-    FRH_HIP(hipMallocAsync((void **)&tf, sizeof(double) * (size_t)T * H * W * C, st), "frh_prefilter_3d scratch");
+    double *tf = tmp + (size_t)T * C * H * W;
     hipError_t e = launch_prefilter_t(raw, raw_f64 != 0, T, H, W, C, lo, den, wt, rt, tf, st);
-    int rc = FRH_OK;
-    if (e == hipSuccess) {
-        // the frames are normalised now: x -> (x - 0) / 1 is exact
-        std::vector<double> zero(C, 0.0), one(C, 1.0);
-        rc = prefilter_impl(tf, 1, T, H, W, C, zero.data(), one.data(), wy, ry, wx, rx, out, out_f64 != 0, 0, stream);
-    }
-    hipError_t e2 = hipFreeAsync(tf, st);
     if (e != hipSuccess) return fail(FRH_EHIP, "temporal prefilter kernel: %s", hipGetErrorString(e));
-    if (rc) return rc;
-    FRH_HIP(e2, "frh_prefilter_3d free");
-    return FRH_OK;
+    // the frames are normalised now: x -> (x - 0) / 1 is exact
+    std::vector<double> zero(C, 0.0), one(C, 1.0);
+    return prefilter_impl(tf, 1, T, H, W, C, zero.data(), one.data(), wy, ry, wx, rx, out, out_f64 != 0, 0, tmp,
+                          stream);
 }
 
 int frh_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow, const int32_t *ix,
@@ -331,18 +325,20 @@ int frh_motion_tensor(int kind, const float *f1, const float *f2, int h, int w, 
     return FRH_OK;
 }
 
+size_t frh_resize_cubic_cv_scratch_bytes(int T, int h, int ow, int C)
+{
+    if (T <= 0 || h <= 0 || ow <= 0 || C <= 0) return 0;
+    return sizeof(double) * (size_t)T * h * ow * C;
+}
+
 int frh_resize_cubic_cv(const double *src, int T, int h, int w, int C, double *dst, int oh, int ow,
-                        void *stream)
+                        void *scratch, size_t scratch_bytes, void *stream)
 {
     FRH_REQ(src && dst && T > 0 && h > 0 && w > 0 && C > 0 && oh > 0 && ow > 0, "frh_resize_cubic_cv: bad arguments");
     FRH_REQ(T <= 65535 && oh <= 65535 && h <= 65535, "frh_resize_cubic_cv: grid limits exceeded");
-    hipStream_t st = as_stream(stream);
-    double *tmp = nullptr;
-    FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)T * h * ow * C, st), "cubic resize scratch");
-    hipError_t e = launch_cv_resize(src, T, h, w, C, tmp, dst, oh, ow, st);
-    hipError_t e2 = hipFreeAsync(tmp, st);
-    FRH_HIP(e, "cubic resize kernels");
-    FRH_HIP(e2, "cubic resize free");
+    FRH_SCRATCH(scratch, scratch_bytes, frh_resize_cubic_cv_scratch_bytes(T, h, ow, C), "frh_resize_cubic_cv");
+    FRH_HIP(launch_cv_resize(src, T, h, w, C, (double *)scratch, dst, oh, ow, as_stream(stream)),
+            "cubic resize kernels");
     return FRH_OK;
 }
 
@@ -353,24 +349,40 @@ int frh_mean_frames_f32(const float *w, int n, size_t len, float *out, void *str
     return FRH_OK;
 }
 
-int frh_median5_mirror(double *f, int N, int h, int w, void *stream)
+size_t frh_median5_mirror_scratch_bytes(int N, int h, int w)
+{
+    if (N <= 0 || h <= 0 || w <= 0) return 0;
+    return sizeof(double) * (size_t)N * h * w;
+}
+
+int frh_median5_mirror(double *f, int N, int h, int w, void *scratch, size_t scratch_bytes, void *stream)
 {
     FRH_REQ(f && N > 0 && h > 0 && w > 0 && h <= 65535 && N <= 65535, "frh_median5_mirror: bad arguments");
-    hipStream_t st = as_stream(stream);
-    double *tmp = nullptr;
-    FRH_HIP(hipMallocAsync((void **)&tmp, sizeof(double) * (size_t)N * h * w, st), "median scratch");
-    hipError_t e = launch_median_plane(f, N, h, w, tmp, st);
-    hipError_t e2 = hipFreeAsync(tmp, st);
-    FRH_HIP(e, "median kernel");
-    FRH_HIP(e2, "median free");
+    FRH_SCRATCH(scratch, scratch_bytes, frh_median5_mirror_scratch_bytes(N, h, w), "frh_median5_mirror");
+    FRH_HIP(launch_median_plane(f, N, h, w, (double *)scratch, as_stream(stream)), "median kernel");
     return FRH_OK;
+}
+
+// Scratch of the stand-alone level solver: the skewed planes of J (6 K),
+// the weights (K), cu / cv, the coefficients (5, or 9 with a_smooth != 1),
+// du / dv and the border ring, plus psi_s on the padded grid when
+// a_smooth != 1.
+size_t frh_level_solver_scratch_bytes(int m, int n, int K, double a_smooth)
+{
+    if (m < 3 || n < 3 || K < 1) return 0;
+    Skew s = make_skew(m - 2, n - 2);
+    const bool nl = a_smooth != 1.0;
+    const size_t ncoef = nl ? 9 : 5, nps = nl ? (size_t)m * n : 0;
+    const size_t npl = (size_t)6 * K + K + 2 + ncoef + 2 + 2;
+    return sizeof(double) * (npl * s.plane + nps);
 }
 
 int frh_level_solver(const double *J11, const double *J22, const double *J33, const double *J12,
                      const double *J13, const double *J23, const double *weight, const double *u,
                      const double *v, int m, int n, int K, double alpha_x, double alpha_y,
                      int iterations, int update_lag, const double *a_data, double a_smooth,
-                     double hx, double hy, double gnc_beta, double *du, double *dv, void *stream)
+                     double hx, double hy, double gnc_beta, double *du, double *dv, void *scratch,
+                     size_t scratch_bytes, void *stream)
 {
     FRH_REQ(J11 && J22 && J33 && J12 && J13 && J23 && weight && u && v && a_data && du && dv,
             "frh_level_solver: null pointer");
@@ -385,8 +397,8 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
     const bool nl = a_smooth != 1.0;
     const size_t ncoef = nl ? 9 : 5, nps = nl ? (size_t)m * n : 0;
     size_t npl = (size_t)6 * K + K + 2 + ncoef + 2 + 2;
-    double *work = nullptr;
-    FRH_HIP(hipMallocAsync((void **)&work, sizeof(double) * (npl * s.plane + nps), st), "level solver scratch");
+    FRH_SCRATCH(scratch, scratch_bytes, frh_level_solver_scratch_bytes(m, n, K, a_smooth), "frh_level_solver");
+    double *work = (double *)scratch;
     double *J = work, *Wsk = J + 6 * K * s.plane, *cuv = Wsk + K * s.plane, *coef = cuv + 2 * s.plane;
     double *duv = coef + ncoef * s.plane, *ring = duv + 2 * s.plane, *psis = work + npl * s.plane;
     double ax = alpha_x / (hx * hx), ay = alpha_y / (hy * hy);
@@ -433,9 +445,7 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
             e = launch_sweep(passes[k].nsweeps, coef, duv, rg, fstride, 1, s, ax, ay, k == 0 ? 1 : 0, st);
     }
     if (e == hipSuccess) e = launch_skew_to_padded_f64(duv, ring, s, du, dv, st);
-    hipError_t e2 = hipFreeAsync(work, st);
     FRH_HIP(e, "level solver kernels");
-    FRH_HIP(e2, "level solver free");
     return FRH_OK;
 }
 

@@ -26,7 +26,7 @@ import numpy as np
 import torch
 
 from . import _lib
-from ._lib import check, lib, ptr, stream_handle
+from ._lib import check, lib, ptr, scratch, stream_handle
 
 _SUPPORTED_DATA_TERMS = {
     "gc": "gc", "gradient": "gc",
@@ -651,11 +651,13 @@ def level_solver(J11, J22, J33, J12, J13, J23, weight, u, v, alpha, iterations, 
     ad = np.ascontiguousarray(np.broadcast_to(np.asarray(a_data, np.float64), (K,)))
     du = torch.empty((m, n), dtype=torch.float64, device=dev)
     dv = torch.empty((m, n), dtype=torch.float64, device=dev)
+    nb = lib().frh_level_solver_scratch_bytes(m, n, K, float(a_smooth))
+    work = scratch(nb, dev)
     check(lib().frh_level_solver(*[ptr(j) for j in Js], ptr(W_), ptr(uu), ptr(vv), m, n, K,
                                  float(alpha[0]), float(alpha[1]), int(iterations),
                                  int(update_lag), ptr(ad), float(a_smooth), float(hx), float(hy),
                                  -1.0 if gnc_beta is None else float(gnc_beta),
-                                 ptr(du), ptr(dv), stream_handle()), "frh_level_solver")
+                                 ptr(du), ptr(dv), ptr(work), nb, stream_handle()), "frh_level_solver")
     return du.cpu().numpy(), dv.cpu().numpy()
 
 
@@ -665,7 +667,9 @@ def median5_mirror(f):
     dev = _device()
     t = torch.as_tensor(np.ascontiguousarray(f, dtype=np.float64), device=dev).clone()
     h, w = t.shape
-    check(lib().frh_median5_mirror(ptr(t), 1, h, w, stream_handle()), "frh_median5_mirror")
+    nb = lib().frh_median5_mirror_scratch_bytes(1, h, w)
+    work = scratch(nb, dev)
+    check(lib().frh_median5_mirror(ptr(t), 1, h, w, ptr(work), nb, stream_handle()), "frh_median5_mirror")
     return t.cpu().numpy()
 
 
@@ -676,6 +680,8 @@ def cv_resize_cubic(w, m, n):
     src = torch.as_tensor(np.ascontiguousarray(w, dtype=np.float64), device=dev)
     h, ww, C = src.shape
     dst = torch.empty((m, n, C), dtype=torch.float64, device=dev)
-    check(lib().frh_resize_cubic_cv(ptr(src), 1, h, ww, C, ptr(dst), m, n, stream_handle()),
+    nb = lib().frh_resize_cubic_cv_scratch_bytes(1, h, n, C)
+    work = scratch(nb, dev)
+    check(lib().frh_resize_cubic_cv(ptr(src), 1, h, ww, C, ptr(dst), m, n, ptr(work), nb, stream_handle()),
           "frh_resize_cubic_cv")
     return dst.cpu().numpy()

@@ -13,7 +13,7 @@ from __future__ import annotations
 import numpy as np
 import torch
 
-from ._lib import FlowregHipError, check, lib, ptr, stream_handle
+from ._lib import FlowregHipError, check, lib, ptr, scratch, stream_handle
 
 PRE_NORM32 = 1  # FRH_PRE_NORM32
 PRE_FILT32 = 2  # FRH_PRE_FILT32
@@ -102,9 +102,15 @@ def preprocess(frames, normalization_ref=None, sigma=(1.0, 1.0, 0.1),
         # scipy filters the frame axis first (apply_gaussian_filter's 4-D branch)
         if semantics:
             raise ValueError("flowreg_hip: float32 prefilter semantics with a temporal kernel are not implemented")
+        nb = lib().frh_prefilter_scratch_bytes(T, H, W, C, rt)
+        work = scratch(nb, dev)
         check(lib().frh_prefilter_3d(ptr(x), int(f64), T, H, W, C, ptr(lo), ptr(den), ptr(wt), rt, ptr(wy), ry,
-                                     ptr(wx), rx, ptr(out), int(out64), stream_handle()), "frh_prefilter_3d")
+                                     ptr(wx), rx, ptr(out), int(out64), ptr(work), nb, stream_handle()),
+              "frh_prefilter_3d")
         return out
+    nb = lib().frh_prefilter_scratch_bytes(T, H, W, C, 0)
+    work = scratch(nb, dev)
     check(lib().frh_prefilter_ex(ptr(x), int(f64), T, H, W, C, ptr(lo), ptr(den), ptr(wy), ry, ptr(wx), rx,
-                                 ptr(out), int(out64), int(semantics), stream_handle()), "frh_prefilter_ex")
+                                 ptr(out), int(out64), int(semantics), ptr(work), nb, stream_handle()),
+          "frh_prefilter_ex")
     return out

@@ -35,7 +35,7 @@ def test_host_entry_points_without_gpu():
     import numpy as np
     from pyflowreg_amd import _lib
     L = _lib.lib()
-    assert L.frh_abi_version() == 4
+    assert L.frh_abi_version() == 5
     assert L.frh_warping_depth(0.8, 50, 512, 512) == 18
     g = np.array([1.0], np.float32)
     idx = np.zeros((5, 4), np.int32)
@@ -44,6 +44,26 @@ def test_host_entry_points_without_gpu():
     np.testing.assert_array_equal(wt[:, 2], 1.0)  # scale 1 is the identity
     rc = L.frh_fill_resize_tables(0, 5, 0, g.ctypes.data, idx.ctypes.data, wt.ctypes.data)
     assert rc == -1 and b"bad arguments" in L.frh_last_error()
+
+
+def test_scratch_queries_and_rejection_without_gpu():
+    """The stage entry points take caller scratch: the queries give its size
+    and a missing or short buffer is rejected before any device work."""
+    from pyflowreg_amd import _lib
+    L = _lib.lib()
+    assert L.frh_prefilter_scratch_bytes(4, 16, 20, 2, 0) == 8 * 4 * 16 * 20 * 2
+    assert L.frh_prefilter_scratch_bytes(4, 16, 20, 2, 3) == 2 * 8 * 4 * 16 * 20 * 2
+    assert L.frh_median5_mirror_scratch_bytes(3, 10, 12) == 8 * 3 * 10 * 12
+    assert L.frh_resize_cubic_cv_scratch_bytes(2, 10, 30, 2) == 8 * 2 * 10 * 30 * 2
+    # (m-2) x (n-2) = 14 x 18 interior: 64-row skew, 25 planes at K = 2 (12 J,
+    # 2 weights, cu/cv, 5 coefficients, du/dv, the ring)
+    assert L.frh_level_solver_scratch_bytes(16, 20, 2, 1.0) == 8 * 25 * 64 * (14 + 18 - 1)
+    assert L.frh_level_solver_scratch_bytes(16, 20, 2, 0.5) == 8 * (29 * 64 * (14 + 18 - 1) + 16 * 20)
+    dummy = 1 << 20  # never dereferenced: the checks fail first
+    rc = L.frh_median5_mirror(dummy, 3, 10, 12, None, 0, None)
+    assert rc == -1 and b"scratch of 2880 bytes needed, 0 given" in L.frh_last_error()
+    rc = L.frh_median5_mirror(dummy, 3, 10, 12, dummy, 2879, None)
+    assert rc == -1 and b"2879 given" in L.frh_last_error()
 
 
 def test_get_displacement_raises_without_gpu():
