@@ -36,7 +36,8 @@ const char *frh_last_error(void);
 /* Incremented whenever entry points are added or change; 3 added
  * frh_warp_int16 and frh_mean_frames_f32, 4 frh_warp_u8; 5 moved every
  * stage's scratch to the caller (scratch arguments and *_scratch_bytes
- * queries; frh_prefilter / frh_prefilter_f64 folded into frh_prefilter_ex). */
+ * queries; frh_prefilter / frh_prefilter_f64 folded into frh_prefilter_ex);
+ * 6 adds frh_prefilter_noalloc. */
 int frh_abi_version(void);
 
 /* ------------------------------------------------------------ host only */
@@ -88,6 +89,22 @@ int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, c
                      const double *den, const double *wy, int ry, const double *wx, int rx,
                      void *out, int out_f64, int semantics, void *scratch, size_t scratch_bytes,
                      void *stream);
+
+/* frh_prefilter_ex without a scratch argument and without any device
+ * allocation.  No counterpart in the reference: it is for callers that
+ * cannot allocate device memory (code under HIP graph capture, tight
+ * per-frame serving loops such as a live registration, harnesses that
+ * forbid hipMalloc).  Its scratch is a static device array of
+ * frh_prefilter_noalloc_max_bytes() bytes (32 MiB) that each GPU holds until
+ * the process ends; calls needing more (T*C*H*W float64 values) are rejected
+ * with FRH_EINVAL.  Every call on one GPU shares that GPU's array: calls that
+ * overlap in time on one GPU (different streams, or threads) race.  The
+ * stream must belong to the current device.  Results equal frh_prefilter_ex's
+ * bit for bit. */
+int frh_prefilter_noalloc(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
+                          const double *den, const double *wy, int ry, const double *wx, int rx,
+                          void *out, int out_f64, int semantics, void *stream);
+size_t frh_prefilter_noalloc_max_bytes(void);
 
 /* apply_gaussian_filter's 4-D branch with a temporal kernel
  * (util/image_processing.py:147-161; scipy's gaussian_filter filters the

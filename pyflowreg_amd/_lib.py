@@ -65,6 +65,9 @@ SIGNATURES = {
     "frh_prefilter_scratch_bytes": (c_size, [c_int, c_int, c_int, c_int, c_int]),
     "frh_prefilter_ex": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p, c_size, c_void_p]),
+    "frh_prefilter_noalloc": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p,
+                                      c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_int, c_void_p]),
+    "frh_prefilter_noalloc_max_bytes": (c_size, []),
     "frh_prefilter_3d": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p,
                                  c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_size,
                                  c_void_p]),

@@ -6,6 +6,7 @@
 
 #include <cmath>
 #include <algorithm>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -133,11 +134,51 @@ int frh::frh_report_error(int code, const char *fmt, ...)
     return code;
 }
 
+// ------------------------------------------------ no-allocation prefilter
+// frh_prefilter_noalloc takes its scratch from this file-scope device array
+// instead of a caller buffer: every GPU the library runs on holds its own
+// copy (32 MiB) until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in); it uses the device
+// address hipGetSymbolAddress returns, looked up once per device with that
+// device current and kept in g_noalloc_ptr.
+static constexpr size_t kNoAllocScratchBytes = (size_t)32 << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_noalloc_mu;
+void *g_noalloc_ptr[kMaxDevices] = {};
+
+// The current device's copy of the scratch array (looked up on the first
+// call on that device); the stream must belong to the current device.
+int noalloc_scratch(void *stream, void **out)
+{
+    int dev = -1;
+    FRH_HIP(hipGetDevice(&dev), "frh_prefilter_noalloc: current device");
+    FRH_REQ(dev >= 0 && dev < kMaxDevices, "frh_prefilter_noalloc: device %d out of range", dev);
+    if (stream) {
+        hipDevice_t sdev = -1;
+        FRH_HIP(hipStreamGetDevice(as_stream(stream), &sdev), "frh_prefilter_noalloc: stream device");
+        FRH_REQ((int)sdev == dev, "frh_prefilter_noalloc: the stream belongs to device %d, device %d is current",
+                (int)sdev, dev);
+    }
+    std::lock_guard<std::mutex> lock(g_noalloc_mu);
+    if (!g_noalloc_ptr[dev]) {
+        void *p = nullptr;
+        FRH_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)), "frh_prefilter_noalloc: scratch address");
+        FRH_REQ(p != nullptr, "frh_prefilter_noalloc: no device address for the scratch array");
+        g_noalloc_ptr[dev] = p;
+    }
+    *out = g_noalloc_ptr[dev];
+    return FRH_OK;
+}
+}  // namespace
+
 extern "C" {
 
 const char *frh_last_error(void) { return g_err; }
 
-int frh_abi_version(void) { return 5; }
+int frh_abi_version(void) { return 6; }
 
 int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
 {
@@ -215,6 +256,29 @@ int frh_prefilter_ex(const void *raw, int raw_f64, int T, int H, int W, int C, c
     FRH_SCRATCH(scratch, scratch_bytes, frh_prefilter_scratch_bytes(T, H, W, C, 0), "frh_prefilter_ex");
     return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, semantics,
                           (double *)scratch, stream);
+}
+
+size_t frh_prefilter_noalloc_max_bytes(void) { return kNoAllocScratchBytes; }
+
+int frh_prefilter_noalloc(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
+                          const double *den, const double *wy, int ry, const double *wx, int rx, void *out,
+                          int out_f64, int semantics, void *stream)
+{
+    FRH_REQ((semantics & ~(FRH_PRE_NORM32 | FRH_PRE_FILT32)) == 0,
+            "frh_prefilter_noalloc: unknown semantics flags");
+    FRH_REQ(T > 0 && H > 0 && W > 0 && C > 0, "frh_prefilter_noalloc: bad shape T=%d H=%d W=%d C=%d", T, H, W, C);
+    const size_t need = frh_prefilter_scratch_bytes(T, H, W, C, 0);
+    FRH_REQ(need <= kNoAllocScratchBytes,
+            "frh_prefilter_noalloc: %zu bytes of scratch needed (T*C*H*W float64 values), the static scratch "
+            "holds %zu; split the frames over several calls or use frh_prefilter_ex",
+            need, kNoAllocScratchBytes);
+    void *p = nullptr;
+    const int rc = noalloc_scratch(stream, &p);
+    if (rc) return rc;
+    // the vertical pass writes every scratch value the horizontal pass reads:
+    // nothing of an earlier call's contents is used
+    return prefilter_impl(raw, raw_f64 != 0, T, H, W, C, lo, den, wy, ry, wx, rx, out, out_f64 != 0, semantics,
+                          (double *)p, stream);
 }
 
 int frh_prefilter_3d(const void *raw, int raw_f64, int T, int H, int W, int C, const double *lo,
