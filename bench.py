@@ -48,11 +48,23 @@ def _cpu_frame(args):
     return time.perf_counter() - t, w
 
 
+def _cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
 def cpu_baseline(raw, n_frames, workers):
     """FP64 CPU restatement of the reference path (the oracle) on the first
     n_frames moving frames, one frame per worker process with 1 thread each,
-    like pyflowreg's multiprocessing executor.  Returns (frames/s, seconds
-    of CPU work, flows)."""
+    like pyflowreg's multiprocessing executor
+    (motion_correction/parallelization/multiprocessing.py:38-45,322-335).
+    Returns (frames/s, seconds of CPU work, flows)."""
     os.environ["OMP_NUM_THREADS"] = "1"
     import numpy as np
     from oracle import oracle as O
@@ -64,16 +76,43 @@ def cpu_baseline(raw, n_frames, workers):
     ctx = mp.get_context("fork")
     t = time.perf_counter()
     with ctx.Pool(workers) as pool:
-        res = pool.map(_cpu_frame, jobs)
+        res = pool.map(_cpu_frame, jobs, chunksize=1)
     wall = time.perf_counter() - t
     return n_frames / wall, float(sum(r[0] for r in res)), np.stack([r[1] for r in res])
+
+
+def cpu_latency(raw, frames_0):
+    """Single-thread per-frame latency of the CPU restatement (in this
+    process, 1 thread): the 256x256x1 single pair (preprocessed with the
+    fixed frame's range) over frames_0 runs, and one 512x512x2 frame of the
+    recording."""
+    os.environ["OMP_NUM_THREADS"] = "1"
+    import numpy as np
+    from oracle import oracle as O
+    from pyflowreg_amd import synth
+    fixed, moving, _ = synth.single_pair(256, 256, seed=0)
+    r64 = fixed[..., None].astype(np.float64)
+    ref0 = O.preprocess(r64).astype(np.float32)
+    mov0 = O.preprocess(moving[None, ..., None], r64).astype(np.float32)[0]
+    t = time.perf_counter()
+    for _ in range(frames_0):
+        O.get_displacement(ref0, mov0)
+    lat0 = (time.perf_counter() - t) / frames_0
+    ref64 = raw[0].astype(np.float64)
+    ref1 = O.preprocess(ref64).astype(np.float32)
+    mov1 = O.preprocess(raw[1], ref64).astype(np.float32)
+    t = time.perf_counter()
+    O.get_displacement(ref1, mov1, weight=[0.5, 0.5])
+    lat1 = time.perf_counter() - t
+    return lat0, lat1
 
 
 def _passes(solver, h):
     """Sweep counts and assemble flags of a level's passes (frh_api.hip
     plan_passes with the sweep kernel's pass-size limit)."""
     its, lag = solver.iterations, solver.update_lag
-    pmax = 5 if 64 * ((h + 63) // 64) <= 512 else 2
+    hs = 64 * ((h + 63) // 64)
+    pmax = 5 if hs <= 512 else (2 if hs <= 1024 else 1)
     starts = [it for it in range(its) if it == 0 or (it + 1) % lag == 0]
     out = []
     for k, b in enumerate(starts):
@@ -308,7 +347,8 @@ def main():
     ap.add_argument("--graph", action="store_true", help="replay the batched level loop as a hipGraph")
     ap.add_argument("--autotune", action="store_true",
                     help="tune (or load from the JSON cache) the sweeps per wavefront pass of each level")
-    ap.add_argument("--cpu-frames", type=int, default=16)
+    ap.add_argument("--cpu-frames", type=int, default=32,
+                    help="frames of the CPU throughput sample (at least twice the worker count)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a device copy of MIB MiB (counter calibration)")
@@ -352,12 +392,21 @@ def main():
     cpu = None
     cpu_w = None
     if not args.no_cpu:
-        n_cpu = min(args.cpu_frames, T)
-        workers = min(16, os.cpu_count() or 1, n_cpu)  # one GPU's share of the host
+        n_host = os.cpu_count() or 1
+        # one GPU's share of the host: at most 16 worker processes, whatever
+        # os.cpu_count() reports on a shared multi-GPU host
+        workers = min(16, n_host)
+        n_cpu = min(max(args.cpu_frames, 2 * workers), T)
         fps_cpu, work, cpu_w = cpu_baseline(raw, n_cpu, workers)
+        lat0, lat1 = cpu_latency(raw, 3) if H == 512 and C == 2 else (None, None)
         cpu = {"value": round(fps_cpu, 4), "unit": "frames/s", "cores": workers, "kind": "port",
-               "sample": f"first {n_cpu} frames of the batch ({H}x{W}x{C}), one per worker "
-                         f"process, {work:.1f} s of CPU work, FP64 oracle (C + scipy median)"}
+               "sample": f"first {n_cpu} frames of the batch ({H}x{W}x{C}), {workers} worker processes "
+                         f"with 1 thread each (one frame per task, as the multiprocessing executor), "
+                         f"{work:.1f} s of CPU work, FP64 oracle (C + scipy median)",
+               "os_cpu_count": n_host, "cpu_model": _cpu_model(),
+               "latency_s_per_frame_1_thread": {
+                   "256x256x1 single pair": None if lat0 is None else round(lat0, 4),
+                   "512x512x2": None if lat1 is None else round(lat1, 4)}}
 
     import pyflowreg_amd as fr
     from pyflowreg_amd.preprocess import preprocess

@@ -102,7 +102,13 @@ std::vector<Pass> plan_passes(int iterations, int lag, int pmax)
     return out;
 }
 
-int sweep_pmax(int h) { return 64 * ((h + 63) / 64) <= 512 ? 5 : 2; }
+// Sweeps per wavefront pass: 5 for 512-lane work-groups, 2 for 1024-lane
+// ones, 1 for levels over 1024 rows (band mode, one launch per 1024 rows).
+int sweep_pmax(int h)
+{
+    const int hs = 64 * ((h + 63) / 64);
+    return hs <= 512 ? 5 : (hs <= 1024 ? 2 : 1);
+}
 
 template <typename T>
 int dev_alloc(T **p, size_t n, const char *what)
@@ -451,7 +457,6 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
     FRH_REQ(J11 && J22 && J33 && J12 && J13 && J23 && weight && u && v && a_data && du && dv,
             "frh_level_solver: null pointer");
     FRH_REQ(m >= 3 && n >= 3 && K >= 1 && K <= 8, "frh_level_solver: need m, n >= 3 and 1 <= K <= 8 (m=%d n=%d K=%d)", m, n, K);
-    FRH_REQ(m - 2 <= 1024, "frh_level_solver: at most 1024 interior rows (got %d)", m - 2);
     FRH_REQ(iterations >= 0 && update_lag >= 1, "frh_level_solver: iterations >= 0, update_lag >= 1");
     FRH_REQ(hx > 0 && hy > 0, "frh_level_solver: hx, hy must be positive");
     hipStream_t st = as_stream(stream);
@@ -652,8 +657,8 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
     int rc = FRH_OK;
     for (int i = 0; i < d->n_levels && rc == FRH_OK; ++i) {
         const frh_level_desc &L = d->levels[i];
-        if (L.h < 1 || L.w < 1 || L.h > 1024 || L.P_img < 4 || (i > 0 && L.P_up < 4)) {
-            rc = fail(FRH_EINVAL, "frh_solver_create: level %d invalid (h=%d w=%d, at most 1024 rows)", i, L.h, L.w);
+        if (L.h < 1 || L.w < 1 || L.h > 65535 || L.P_img < 4 || (i > 0 && L.P_up < 4)) {
+            rc = fail(FRH_EINVAL, "frh_solver_create: level %d invalid (h=%d w=%d)", i, L.h, L.w);
             break;
         }
         if (i > 0 && (L.h < d->levels[i - 1].h || L.w < d->levels[i - 1].w)) {

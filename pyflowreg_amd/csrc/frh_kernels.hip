@@ -1106,7 +1106,7 @@ __global__ void k_assemble(const double *__restrict__ J, const double *__restric
                            size_t plane, double ax, double ay, AData ad, int psi_one,
                            int zero_init)
 {
-    int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
+    int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x, t = blockIdx.y;
     int c = d - r;
     if (r >= h || c < 0 || c >= w) return;
     size_t q = (size_t)d * hs + r;
@@ -1135,7 +1135,8 @@ hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv
                            hipStream_t st)
 {
     const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
-    dim3 g(s.h + s.w - 1, T), b(s.hs);
+    // one work-group per diagonal and frame (and per 1024 rows of taller levels)
+    dim3 g(s.h + s.w - 1, T, (s.hs + 1023) / 1024), b(s.hs < 1024 ? s.hs : 1024);
 #define FRH_ASM(CC)                                                                            \
     hipLaunchKernelGGL(k_assemble<CC>, g, b, 0, st, J, Wsk, cuv, duv, coef, frame_stride, s.h, s.w, \
                        s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
@@ -1254,7 +1255,7 @@ __global__ void k_assemble_nl(const double *__restrict__ J, const double *__rest
                               int h, int w, int hs, size_t plane, double ax, double ay, AData ad,
                               int psi_one, int zero_init)
 {
-    const int d = blockIdx.x, r = threadIdx.x, t = blockIdx.y;
+    const int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x, t = blockIdx.y;
     const int c = d - r;
     if (r >= h || c < 0 || c >= w) return;
     const size_t q = (size_t)d * hs + r;
@@ -1302,7 +1303,7 @@ hipError_t launch_assemble_nl(const double *J, const double *Wsk, const float *u
 {
     const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
     PadFlow fl{u32, v32, fs32, u64, v64};
-    dim3 g(s.h + s.w - 1, T), b(s.hs);
+    dim3 g(s.h + s.w - 1, T, (s.hs + 1023) / 1024), b(s.hs < 1024 ? s.hs : 1024);
 #define FRH_ASMNL(CC)                                                                               \
     hipLaunchKernelGGL(k_assemble_nl<CC>, g, b, 0, st, J, Wsk, fl, psi_s, ps_stride, duv, coef, frame_stride, \
                        s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
@@ -1344,9 +1345,10 @@ struct SweepArgs {
     double *duv;
     double *ring;
     size_t fstride, plane;
-    int h, w, hs;
+    int h, w, hs;         // h: rows of this launch (a band of the level in band mode)
     double ax, ay;
     int zero_init;
+    int r0, hlev;         // band mode: the band's first row and the level's rows
     // fused passes: coefficients formed on entry from the raw planes
     const double *J, *Wsk, *cuv;
     AData ad;
@@ -1385,6 +1387,17 @@ constexpr int kSweepNLTight = 4, kSweepDLTight = 3;
 // weights, a_smooth != 1).
 constexpr int kSmoothNL = -1;
 
+// Band mode (levels of more than 1024 rows, one sweep per pass): a level is
+// swept in bands of at most 512 rows, one launch per band from the top.
+// One sweep's new values depend only on the new values of the cell above and
+// of the cell to the left, so finishing a band before starting the next
+// gives the column-major loop's values: the band's top row takes the new
+// values of the row above from the previous band's launch, and its bottom
+// row the start values of the row below, which the next band has not
+// touched yet.  Both come from memory through a small ring, kBandNR slots,
+// loaded kBandDR steps ahead.
+constexpr int kBandNR = 4, kBandDR = 3;
+
 template <int P, int C, bool TIGHT = false>
 struct SweepState {
     static constexpr int NR = 2 * P - 1;  // diagonals a lane's P sweeps span
@@ -1399,7 +1412,27 @@ struct SweepState {
     double Rv[2][P];
     double Lu[NL], Lv[NL];
     double raw[NRAW];     // raw data of diag t+1 loaded at step t, C >= 1
+    double Bu[kBandNR], Bv[kBandNR];  // band mode: neighbour rows of the band's edge lanes
 };
+
+// Band mode: byte offset of the value the band's top lane (new value of the
+// row above) or bottom lane (start value of the row below) needs at step t,
+// or kOutOfRange.
+__device__ inline unsigned band_offset(const SweepArgs &a, int r, int t)
+{
+    const int h = a.h, w = a.w;
+    unsigned q = kOutOfRange;
+    if (r == 0 && a.r0 > 0) {
+        const int c = t;
+        if ((unsigned)c < (unsigned)w) q = ((unsigned)(a.r0 - 1 + c) * (unsigned)a.hs + (unsigned)(a.r0 - 1)) * 8u;
+    } else if (r == h - 1 && a.r0 + h < a.hlev && !a.zero_init) {
+        // (on a level's first pass every start value is 0: the memory still
+        // holds the previous level's values)
+        const int c = t - (h - 1);
+        if ((unsigned)c < (unsigned)w) q = ((unsigned)(a.r0 + h + c) * (unsigned)a.hs + (unsigned)(a.r0 + h)) * 8u;
+    }
+    return q;
+}
 
 template <int C>
 __device__ __forceinline__ void load_raw(double *raw, const SweepArgs &a, size_t fo, size_t q)
@@ -1418,7 +1451,7 @@ struct SweepBufs {
     unsigned plane_bytes;
 };
 
-template <int P, int MAXW, int C, int S, bool TT>
+template <int P, int MAXW, int C, int S, bool TT, bool BAND>
 __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const SweepArgs &a, SweepBufs &bf,
                                            double *rdu, double *rdv,
                                            double (*xb)[MAXW][2][P + 1][2], int t, int r,
@@ -1426,6 +1459,10 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
 {
     using St = SweepState<P, C, TT>;
     constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL;
+    static_assert(!BAND || (P == 1 && C <= 0 && St::U % kBandNR == 0), "band mode: one sweep per pass");
+    // the lane's row and the diagonal offset in the level (band mode: the
+    // launch's row r is level row r + R0, its diagonal d is level diagonal d + R0)
+    const int R0 = BAND ? a.r0 : 0;
     // Every step, also the ones past the last diagonal (t > Tend, where no
     // lane has a cell), issues the same vector memory instructions, with no
     // branch around them, so the compiler can count the loads in flight and
@@ -1439,7 +1476,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     {
         if constexpr (C <= 0) {
             const int d = t + DK;
-            const unsigned q = slot_offset(row_ok, d, r, w, hs);
+            const unsigned q = slot_offset(row_ok, d + R0, r + R0, w, hs);
             double *k = st.K[(S + DK) % NK];
 #pragma unroll
             for (int e = 0; e < St::NKC; ++e) k[e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
@@ -1453,9 +1490,14 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
             }
         }
         const int d = t + DL;
-        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d, r, w, hs);
+        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
         st.Lu[(S + DL) % NL] = buf_load_f64(bf.duv, q, 0);
         st.Lv[(S + DL) % NL] = buf_load_f64(bf.duv, q, bf.plane_bytes);
+        if constexpr (BAND) {
+            const unsigned qb = band_offset(a, r, t + kBandDR);
+            st.Bu[(S + kBandDR) % kBandNR] = buf_load_f64(bf.duv, qb, 0);
+            st.Bv[(S + kBandDR) % kBandNR] = buf_load_f64(bf.duv, qb, bf.plane_bytes);
+        }
     }
     const double lpu = st.Lu[S % NL], lpv = st.Lv[S % NL];              // own start value
     const double ldu = st.Lu[(S + 1) % NL], ldv = st.Lv[(S + 1) % NL];  // right neighbour's
@@ -1486,9 +1528,19 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
             dnv[p] = xb[pb][wv + 1][0][p - 1][1];
         }
     }
+    if constexpr (BAND) {
+        // the rows above and below the band (0 at the level's own edges)
+        if ((r == 0 && a.r0 > 0) || (r == h - 1 && a.r0 + h < a.hlev)) {
+            upu[0] = r == 0 ? st.Bu[S % kBandNR] : upu[0];
+            upv[0] = r == 0 ? st.Bv[S % kBandNR] : upv[0];
+            dnu[0] = r == 0 ? dnu[0] : st.Bu[S % kBandNR];
+            dnv[0] = r == 0 ? dnv[0] : st.Bv[S % kBandNR];
+        }
+    }
     const double ax = a.ax, ay = a.ay, om1 = 1 - 1.95;
     // out-of-level neighbour weight b = ax * (#left/right outside) + ay * (#up/down outside)
-    const double brow = ay * (double)((r == 0) + (r == h - 1));
+    const int rg = r + R0, HL = BAND ? a.hlev : h;  // level row, level rows
+    const double brow = ay * (double)((rg == 0) + (rg == HL - 1));
     const int rlo = wv * 64, rhi = (rlo + 63 < h - 1 ? rlo + 63 : h - 1);
     double cu[P], cv[P];
 #pragma unroll
@@ -1510,8 +1562,8 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
                 // per-cell edge weights; an out-of-level neighbour's weight
                 // moves to the cell's own term, summed left, right, up, down
                 const double wl = Kp[5], wr = Kp[6], wdn = Kp[7], wup = Kp[8];
-                b = (((c == 0 ? wl : 0.0) + (c == w - 1 ? wr : 0.0)) + (r == 0 ? wup : 0.0)) +
-                    (r == h - 1 ? wdn : 0.0);
+                b = (((c == 0 ? wl : 0.0) + (c == w - 1 ? wr : 0.0)) + (rg == 0 ? wup : 0.0)) +
+                    (rg == HL - 1 ? wdn : 0.0);
                 nu = fma(wup, upu[p], fma(wdn, dnu[p], fma(wr, ru, fma(wl, lu, Bu))));
                 nv = fma(wup, upv[p], fma(wdn, dnv[p], fma(wr, rv, fma(wl, lv, Bv))));
             } else {
@@ -1535,8 +1587,8 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
                 cu[p] = ok ? dun : 0.0;
                 cv[p] = ok ? dvn : 0.0;
             }
-            if (p == P - 1 && rdu && ok && (r == 0 || r == h - 1 || c == 0 || c == w - 1)) {
-                size_t q = (size_t)(t - 2 * (P - 1)) * hs + r;
+            if (p == P - 1 && rdu && ok && (rg == 0 || rg == HL - 1 || c == 0 || c == w - 1)) {
+                size_t q = (size_t)(t - 2 * (P - 1) + R0) * hs + rg;
                 rdu[q] = ou;
                 rdv[q] = ov;
             }
@@ -1544,7 +1596,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     }
     {
         // sweep P-1 leaves diag t - 2(P-1): store its results
-        const unsigned q = slot_offset(row_ok, t - 2 * (P - 1), r, w, hs);
+        const unsigned q = slot_offset(row_ok, t - 2 * (P - 1) + R0, rg, w, hs);
         buf_store_f64(cu[P - 1], bf.duv, q, 0);
         buf_store_f64(cv[P - 1], bf.duv, q, bf.plane_bytes);
     }
@@ -1581,17 +1633,17 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     __builtin_amdgcn_sched_barrier(0);  // keep each step's loads within the step
 }
 
-template <int P, int MAXW, int C, bool TT, int... S>
+template <int P, int MAXW, int C, bool TT, bool BAND, int... S>
 __device__ __forceinline__ void sweep_block(SweepState<P, C, TT> &st, const SweepArgs &a,
                                             SweepBufs &bf, double *rdu,
                                             double *rdv, double (*xb)[MAXW][2][P + 1][2], int t0,
                                             int r, int lane, int wv, int nw, int Tend,
                                             std::integer_sequence<int, S...>)
 {
-    (sweep_step<P, MAXW, C, S, TT>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
+    (sweep_step<P, MAXW, C, S, TT, BAND>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
 }
 
-template <int P, int MAXT, int C>
+template <int P, int MAXT, int C, bool BAND = false>
 __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
 {
     constexpr bool TT = MAXT > 512 && C == 0 && P >= 2;  // the 1-sweep form fits without it
@@ -1626,20 +1678,29 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     // prologue: start values of diags 0 .. DL-1; coefficients of diags
     // 0 .. DK-1 (C = 0) or of diag 0 (C >= 1)
     auto cell_ok = [&](int d) { int c = d - r; return row_ok && c >= 0 && c < w; };
+    const int R0 = BAND ? a.r0 : 0;
     SweepBufs bf;
     bf.plane_bytes = (unsigned)(a.plane * sizeof(double));
     bf.coef = make_rsrc(cb, C <= 0 ? St::NKC * bf.plane_bytes : 0u);
     bf.duv = make_rsrc(du, 2 * bf.plane_bytes);
 #pragma unroll
     for (int d = 0; d < DL; ++d) {
-        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d, r, w, hs);
+        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
         st.Lu[d] = buf_load_f64(bf.duv, q, 0);
         st.Lv[d] = buf_load_f64(bf.duv, q, bf.plane_bytes);
+    }
+    if constexpr (BAND) {
+#pragma unroll
+        for (int k = 0; k < kBandDR; ++k) {
+            const unsigned qb = band_offset(a, r, k);
+            st.Bu[k] = buf_load_f64(bf.duv, qb, 0);
+            st.Bv[k] = buf_load_f64(bf.duv, qb, bf.plane_bytes);
+        }
     }
     if constexpr (C <= 0) {
 #pragma unroll
         for (int d = 0; d < DK; ++d) {
-            const unsigned q = slot_offset(row_ok, d, r, w, hs);
+            const unsigned q = slot_offset(row_ok, d + R0, r + R0, w, hs);
 #pragma unroll
             for (int e = 0; e < St::NKC; ++e) st.K[d][e] = buf_load_f64(bf.coef, q, e * bf.plane_bytes);
         }
@@ -1654,21 +1715,37 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     __syncthreads();
     const int Tend = (h + w - 2) + 2 * (P - 1);
     for (int t0 = 0; t0 <= Tend; t0 += U)
-        sweep_block<P, MAXW, C, TT>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
-                                std::make_integer_sequence<int, U>{});
+        sweep_block<P, MAXW, C, TT, BAND>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
+                                          std::make_integer_sequence<int, U>{});
 }
 
 template <int P, int C>
-static hipError_t launch_sweep_pc(const SweepArgs &a, int T, hipStream_t st)
+static hipError_t launch_sweep_pc(SweepArgs a, int T, hipStream_t st)
 {
     // buffer offsets are 32-bit and must stay below kOutOfRange
     if (a.plane * sizeof(double) * SweepState<P, C>::NKC >= kOutOfRange) return hipErrorInvalidValue;
-    if (a.hs <= 512)
+    a.r0 = 0;
+    a.hlev = a.h;
+    if (a.hs <= 512) {
         hipLaunchKernelGGL((k_sweep<P, 512, C>), dim3(T), dim3(a.hs), 0, st, a);
-    else if (P <= 2)
+    } else if (a.hs <= 1024 && P <= 2) {
         hipLaunchKernelGGL((k_sweep<(P <= 2 ? P : 2), 1024, C>), dim3(T), dim3(a.hs), 0, st, a);
-    else
+    } else if (a.hs > 1024 && P == 1 && C <= 0) {
+        // band mode: bands of 512 rows from the top, one launch each (512-lane
+        // work-groups keep the band rings in registers)
+        const int hl = a.h;
+        for (int r0 = 0; r0 < hl; r0 += 512) {
+            SweepArgs b = a;
+            b.r0 = r0;
+            b.h = hl - r0 < 512 ? hl - r0 : 512;
+            hipLaunchKernelGGL((k_sweep<1, 512, (C <= 0 ? C : 0), true>), dim3(T), dim3(64 * ((b.h + 63) / 64)), 0,
+                               st, b);
+            const hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+        }
+    } else {
         return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
 
@@ -1952,7 +2029,7 @@ hipError_t launch_broadcast_f32(const float *src, size_t n, float *dst, size_t d
 __global__ void k_to_skew_f32(const float *__restrict__ src, int h, int w, int hs, size_t plane,
                               double *__restrict__ dst)
 {
-    int d = blockIdx.x, r = threadIdx.x, n = blockIdx.y;
+    int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x, n = blockIdx.y;
     int c = d - r;
     if (r >= h || c < 0 || c >= w) return;
     dst[(size_t)n * plane + (size_t)d * hs + r] = (double)src[((size_t)n * h + r) * w + c];
@@ -1960,8 +2037,8 @@ __global__ void k_to_skew_f32(const float *__restrict__ src, int h, int w, int h
 
 hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *dst, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_to_skew_f32, dim3(s.h + s.w - 1, N), dim3(s.hs), 0, st, src, s.h, s.w,
-                       s.hs, s.plane, dst);
+    hipLaunchKernelGGL(k_to_skew_f32, dim3(s.h + s.w - 1, N, (s.hs + 1023) / 1024),
+                       dim3(s.hs < 1024 ? s.hs : 1024), 0, st, src, s.h, s.w, s.hs, s.plane, dst);
     return hipGetLastError();
 }
 
@@ -1969,7 +2046,7 @@ hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *ds
 __global__ void k_padded_to_skew_f64(const double *__restrict__ src, int m, int n, int K, int hs,
                                      size_t plane, double *__restrict__ dst)
 {
-    int d = blockIdx.x, r = threadIdx.x, k = blockIdx.y;
+    int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x, k = blockIdx.y;
     int h = m - 2, w = n - 2, c = d - r;
     if (r >= h || c < 0 || c >= w) return;
     dst[(size_t)k * plane + (size_t)d * hs + r] = src[((size_t)(r + 1) * n + (c + 1)) * K + k];
@@ -1978,8 +2055,8 @@ __global__ void k_padded_to_skew_f64(const double *__restrict__ src, int m, int 
 hipError_t launch_padded_to_skew_f64(const double *src, int m, int n, int K, const Skew &s,
                                      double *dst, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_padded_to_skew_f64, dim3(s.h + s.w - 1, K), dim3(s.hs), 0, st, src, m, n,
-                       K, s.hs, s.plane, dst);
+    hipLaunchKernelGGL(k_padded_to_skew_f64, dim3(s.h + s.w - 1, K, (s.hs + 1023) / 1024),
+                       dim3(s.hs < 1024 ? s.hs : 1024), 0, st, src, m, n, K, s.hs, s.plane, dst);
     return hipGetLastError();
 }
 
@@ -2015,7 +2092,7 @@ __global__ void k_cuv_padded(const double *__restrict__ u, const double *__restr
                              int n, double ax, double ay, int hs, size_t plane,
                              double *__restrict__ cuv)
 {
-    int d = blockIdx.x, r = threadIdx.x;
+    int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x;
     int h = m - 2, w = n - 2, c = d - r;
     if (r >= h || c < 0 || c >= w) return;
     size_t cc = (size_t)(r + 1) * n + (c + 1);
@@ -2034,8 +2111,8 @@ __global__ void k_cuv_padded(const double *__restrict__ u, const double *__restr
 hipError_t launch_cuv_padded(const double *u, const double *v, int m, int n, double ax,
                              double ay, const Skew &s, double *cuv, hipStream_t st)
 {
-    hipLaunchKernelGGL(k_cuv_padded, dim3(s.h + s.w - 1), dim3(s.hs), 0, st, u, v, m, n, ax, ay,
-                       s.hs, s.plane, cuv);
+    hipLaunchKernelGGL(k_cuv_padded, dim3(s.h + s.w - 1, 1, (s.hs + 1023) / 1024), dim3(s.hs < 1024 ? s.hs : 1024),
+                       0, st, u, v, m, n, ax, ay, s.hs, s.plane, cuv);
     return hipGetLastError();
 }
 
