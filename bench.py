@@ -128,8 +128,9 @@ def algorithmic_bytes(solver, T):
     """Algorithmic HBM bytes of one run's k_sweep and k_assemble launches.
     k_sweep, per cell and pass: 5 float64 coefficients and 2 float64 start
     values read (not on a level's first pass) and 2 float64 results written.
-    k_assemble, per cell: 6C J + C W + cu, cv read (+ du, dv after the
-    level's first pass) and 5 coefficients written, all float64."""
+    k_assemble, per cell: 6C float64 J, C float32 weights and float64
+    cu, cv read, + float64 du, dv after the level's first pass, and 5
+    float64 coefficients written."""
     C = solver.C
     sweep = assemble = 0
     for (_, h, w, *_rest) in solver.plan.levels:
@@ -137,7 +138,7 @@ def algorithmic_bytes(solver, T):
         for k, (n, asm) in enumerate(_passes(solver, h)):
             sweep += cells * 8 * (7 + 2 if k > 0 else 5 + 2)
             if asm:
-                assemble += cells * 8 * (7 * C + 2 + (2 if k > 0 else 0) + 5)
+                assemble += cells * (8 * 6 * C + 4 * C + 16 + (16 if k > 0 else 0) + 40)
     return {"sweep": sweep, "assemble": assemble}
 
 

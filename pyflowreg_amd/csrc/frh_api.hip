@@ -495,7 +495,7 @@ int frh_level_solver(const double *J11, const double *J22, const double *J33, co
                 e = launch_psi_smooth(nullptr, nullptr, 0, u, v, duv, ring, fstride, 1, s, k == 0 ? 1 : 0,
                                       a_smooth, gnc_beta, hx, hy, psis, 0, st);
             if (e == hipSuccess && p.assemble)
-                e = launch_assemble_nl(J, Wsk, nullptr, nullptr, 0, u, v, psis, 0, duv, coef, fstride, 1, K, s,
+                e = launch_assemble_nl(J, Wsk, 0, nullptr, nullptr, 0, u, v, psis, 0, duv, coef, fstride, 1, K, s,
                                        ax, ay, am1.data(), a_data, p.psi_one ? 1 : 0, k == 0 ? 1 : 0, gnc_beta,
                                        st);
             if (e == hipSuccess) e = launch_sweep_nl(p.nsweeps, coef, duv, ring, fstride, 1, s, k == 0 ? 1 : 0, st);
@@ -543,7 +543,8 @@ struct frh_solver {
     std::vector<LevelDev> lv;
     float *f1lev = nullptr, *uvtop = nullptr, *tmp = nullptr, *f2l = nullptr, *f2w = nullptr;
     float *uvl = nullptr, *uvn = nullptr;
-    double *Wlev = nullptr, *work = nullptr;
+    float *Wlev = nullptr;  // weight levels, skewed float32 (float32 values)
+    double *work = nullptr;
     double *psis = nullptr;       // a_smooth != 1: psi_s per frame on the padded level grid
     int Wsum = 0;                 // sum of level widths (multi-level horizontal pass)
     std::vector<int> xoff;        // per level: column offset in the staged rows
@@ -763,7 +764,7 @@ int frh_solver_set_reference(frh_solver *s, const float *fixed, const float *wei
         float *wl = s->f2l;  // scratch for the resized weight planes
         FRH_HIP(launch_pyr_v(s->pyr, C, s->H, s->Wsum, s->xoff[i], wl, L.h, L.w, L.img_iy, L.img_wy,
                              L.P_img, st), "weight pyramid");
-        FRH_HIP(launch_to_skew_f32(wl, C, L.sk, s->Wlev + L.W_off, st), "weight skew");
+        FRH_HIP(launch_to_skew_f32f(wl, C, L.sk, s->Wlev + L.W_off, st), "weight skew");
     }
     LevelDev &T0 = s->lv[0];
     if (uv) {
@@ -873,7 +874,6 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         // The separate assemble kernel is faster than forming coefficients
         // inside the wavefront (measured on MI355X), so the pyramid solver
         // uses it; frh_level_solver keeps the fused form.
-        const bool fused = false;
         const size_t ps_stride = (size_t)(h + 2) * (w + 2);
         if (nl && !L.passes.empty() && L.passes[0].psi_one)
             FRH_HIP(launch_fill_f64(s->psis, (size_t)T * ps_stride, 1.0, st), "psi_s init");
@@ -888,7 +888,7 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
                         FRH_HIP(launch_psi_smooth(s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr, duv, ring, fs, T, sk,
                                                   k == 0 ? 1 : 0, s->a_smooth, -1.0, L.h_col, L.h_row, s->psis,
                                                   ps_stride, st), "smoothness weights");
-                    FRH_HIP(launch_assemble_nl(J, s->Wlev + L.W_off, s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr,
+                    FRH_HIP(launch_assemble_nl(J, s->Wlev + L.W_off, 1, s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr,
                                                s->psis, ps_stride, duv, coef, fs, T, C, sk, L.axw, L.ayw,
                                                s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
                                                -1.0, st), "assemble (nonlinear smoothness)");
@@ -898,16 +898,9 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
                         "sor sweep (nonlinear smoothness)");
                 continue;
             }
-            if (fused) {
-                Scope sc(s, TM_SWEEP, st);
-                FRH_HIP(launch_sweep_fused(p.nsweeps, J, s->Wlev + L.W_off, cuv, duv, nullptr, fs, T, C, sk,
-                                           L.axw, L.ayw, s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0,
-                                           k == 0 ? 1 : 0, -1.0, st), "sor sweep (fused)");
-                continue;
-            }
             if (p.assemble) {
                 Scope sc(s, TM_ASSEMBLE, st);
-                FRH_HIP(launch_assemble(J, s->Wlev + L.W_off, cuv, duv, coef, fs, T, C, sk, L.axw, L.ayw,
+                FRH_HIP(launch_assemble_w32(J, s->Wlev + L.W_off, cuv, duv, coef, fs, T, C, sk, L.axw, L.ayw,
                                         s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
                                         -1.0, st), "assemble");
             }

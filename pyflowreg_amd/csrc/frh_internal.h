@@ -83,7 +83,7 @@ hipError_t launch_psi_smooth(const float *u32, const float *v32, size_t fs32, co
                              const double *v64, const double *duv, const double *ring, size_t fstride,
                              int T, const Skew &s, int zero, double a_smooth, double gnc_beta, double hx,
                              double hy, double *psi_s, size_t ps_stride, hipStream_t st);
-hipError_t launch_assemble_nl(const double *J, const double *Wsk, const float *u32, const float *v32,
+hipError_t launch_assemble_nl(const double *J, const void *Wsk, int w_f32, const float *u32, const float *v32,
                               size_t fs32, const double *u64, const double *v64, const double *psi_s,
                               size_t ps_stride, const double *duv, double *coef, size_t frame_stride, int T,
                               int C, const Skew &s, double ax, double ay, const double *a_data_minus1,
@@ -99,6 +99,11 @@ hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv
                            const Skew &s, double ax, double ay, const double *a_data_minus1,
                            const double *a_data, int psi_one, int zero_init, double gnc_beta,
                            hipStream_t st);
+// the pyramid solver's assembly: float32 weight planes
+hipError_t launch_assemble_w32(const double *J, const float *Wsk, const double *cuv, const double *duv,
+                               double *coef, size_t frame_stride, int T, int C, const Skew &s, double ax,
+                               double ay, const double *a_data_minus1, const double *a_data, int psi_one,
+                               int zero_init, double gnc_beta, hipStream_t st);
 hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
                         int T, const Skew &s, double ax, double ay, int zero_init,
                         hipStream_t st);
@@ -109,6 +114,7 @@ hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew 
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st);
 hipError_t launch_broadcast_f32(const float *src, size_t n, float *dst, size_t dst_stride, int T,
                                 hipStream_t st);
+hipError_t launch_to_skew_f32f(const float *src, int N, const Skew &s, float *dst, hipStream_t st);
 hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *dst,
                               hipStream_t st);
 hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, const double *lo,

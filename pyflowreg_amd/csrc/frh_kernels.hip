@@ -4,6 +4,7 @@
 // operation order (compiled with -ffp-contract=off, IEEE division and
 // square root) so that results are bit-identical to the CPU oracle given the
 // same inputs.  Reference citations are pyflowreg source paths.
+#include <type_traits>
 #include <utility>
 
 #include "frh_internal.h"
@@ -897,13 +898,13 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const size_t hw = (size_t)h * w, d0 = (size_t)(r0 + c0);
     // one output plane of the tile: cell (r, c) -> slot (r + c) * hs + r,
     // stored diagonal by diagonal (slot s = diagonal j of the tile, row i)
-    auto put = [&](double val, double *dst) {
+    auto put = [&](double val, auto *dst) {
         stage[threadIdx.y][threadIdx.x] = val;
         __syncthreads();
         for (int s = tid; s < kTensorDiags * kTensorRows; s += 64 * kTensorRows) {
             const int j = s / kTensorRows, i = s % kTensorRows, cl = j - i;
             if (cl >= 0 && cl < 64 && r0 + i < h && c0 + cl < w)
-                dst[(d0 + j) * hs + r0 + i] = stage[i][cl];
+                dst[(d0 + j) * hs + r0 + i] = (std::remove_pointer_t<decltype(dst)>)stage[i][cl];
         }
         __syncthreads();
     };
@@ -1099,8 +1100,11 @@ __device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc,
     coef_from_raw_ws<C>(J, Wc, cu, cv, x, y, ((ax + ax) + ay) + ay, ad, psi_one, K);
 }
 
-template <int C>
-__global__ void k_assemble(const double *__restrict__ J, const double *__restrict__ Wsk,
+// WT: the weight planes' type (float32 in the pyramid solver, whose weight
+// levels are float32 values; float64 for the stand-alone level solver's
+// caller-given weights).
+template <int C, typename WT>
+__global__ void k_assemble(const double *__restrict__ J, const WT *__restrict__ Wsk,
                            const double *__restrict__ cuv, const double *__restrict__ duv,
                            double *__restrict__ coef, size_t fstride, int h, int w, int hs,
                            size_t plane, double ax, double ay, AData ad, int psi_one,
@@ -1121,25 +1125,23 @@ __global__ void k_assemble(const double *__restrict__ J, const double *__restric
 #pragma unroll
     for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
 #pragma unroll
-    for (int k = 0; k < C; ++k) Wr[k] = Wsk[(size_t)k * plane + q];
+    for (int k = 0; k < C; ++k) Wr[k] = (double)Wsk[(size_t)k * plane + q];
     coef_from_raw<C>(Jr, Wr, ct[q], ct[plane + q], x, y, ax, ay, ad, psi_one, K);
     double *co = coef + (size_t)t * fstride;
 #pragma unroll
     for (int e = 0; e < 5; ++e) co[e * plane + q] = K[e];
 }
 
-hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
-                           const double *duv, double *coef, size_t frame_stride, int T, int C,
-                           const Skew &s, double ax, double ay, const double *a_data_minus1,
-                           const double *a_data, int psi_one, int zero_init, double gnc_beta,
-                           hipStream_t st)
+template <typename WT>
+static hipError_t launch_assemble_t(const double *J, const WT *Wsk, const double *cuv, const double *duv,
+                                    double *coef, size_t frame_stride, int T, int C, const Skew &s, double ax,
+                                    double ay, const AData &ad, int psi_one, int zero_init, hipStream_t st)
 {
-    const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
     // one work-group per diagonal and frame (and per 1024 rows of taller levels)
     dim3 g(s.h + s.w - 1, T, (s.hs + 1023) / 1024), b(s.hs < 1024 ? s.hs : 1024);
 #define FRH_ASM(CC)                                                                            \
-    hipLaunchKernelGGL(k_assemble<CC>, g, b, 0, st, J, Wsk, cuv, duv, coef, frame_stride, s.h, s.w, \
-                       s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
+    hipLaunchKernelGGL((k_assemble<CC, WT>), g, b, 0, st, J, Wsk, cuv, duv, coef, frame_stride, s.h, \
+                       s.w, s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
     switch (C) {
     case 1: FRH_ASM(1); break;
     case 2: FRH_ASM(2); break;
@@ -1153,6 +1155,27 @@ hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv
     }
 #undef FRH_ASM
     return hipGetLastError();
+}
+
+hipError_t launch_assemble(const double *J, const double *Wsk, const double *cuv,
+                           const double *duv, double *coef, size_t frame_stride, int T, int C,
+                           const Skew &s, double ax, double ay, const double *a_data_minus1,
+                           const double *a_data, int psi_one, int zero_init, double gnc_beta,
+                           hipStream_t st)
+{
+    const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
+    return launch_assemble_t<double>(J, Wsk, cuv, duv, coef, frame_stride, T, C, s, ax, ay, ad, psi_one,
+                                            zero_init, st);
+}
+
+hipError_t launch_assemble_w32(const double *J, const float *Wsk, const double *cuv, const double *duv,
+                               double *coef, size_t frame_stride, int T, int C, const Skew &s, double ax,
+                               double ay, const double *a_data_minus1, const double *a_data, int psi_one,
+                               int zero_init, double gnc_beta, hipStream_t st)
+{
+    const AData ad = make_adata(C, a_data, a_data_minus1, gnc_beta);
+    return launch_assemble_t<float>(J, Wsk, cuv, duv, coef, frame_stride, T, C, s, ax, ay, ad, psi_one,
+                                    zero_init, st);
 }
 
 // -------------------------------- nonlinear smoothness (a_smooth != 1)
@@ -1248,8 +1271,8 @@ hipError_t launch_psi_smooth(const float *u32, const float *v32, size_t fs32, co
 // assemble_pixel in the oracle): per cell the four edge weights, cu, cv
 // from the padded initial flow, then the data terms as in k_assemble; nine
 // planes out.
-template <int C>
-__global__ void k_assemble_nl(const double *__restrict__ J, const double *__restrict__ Wsk, PadFlow fl,
+template <int C, typename WT>
+__global__ void k_assemble_nl(const double *__restrict__ J, const WT *__restrict__ Wsk, PadFlow fl,
                               const double *__restrict__ psi_s, size_t ps_stride,
                               const double *__restrict__ duv, double *__restrict__ coef, size_t fstride,
                               int h, int w, int hs, size_t plane, double ax, double ay, AData ad,
@@ -1285,7 +1308,7 @@ __global__ void k_assemble_nl(const double *__restrict__ J, const double *__rest
 #pragma unroll
     for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
 #pragma unroll
-    for (int k = 0; k < C; ++k) Wr[k] = Wsk[(size_t)k * plane + q];
+    for (int k = 0; k < C; ++k) Wr[k] = (double)Wsk[(size_t)k * plane + q];
     coef_from_raw_ws<C>(Jr, Wr, cu, cv, x, y, ((wn[0] + wn[1]) + wn[2]) + wn[3], ad, psi_one, K);
     double *co = coef + (size_t)t * fstride;
 #pragma unroll
@@ -1294,7 +1317,7 @@ __global__ void k_assemble_nl(const double *__restrict__ J, const double *__rest
     for (int k = 0; k < 4; ++k) co[(5 + k) * plane + q] = wn[k];
 }
 
-hipError_t launch_assemble_nl(const double *J, const double *Wsk, const float *u32, const float *v32,
+hipError_t launch_assemble_nl(const double *J, const void *Wsk, int w_f32, const float *u32, const float *v32,
                               size_t fs32, const double *u64, const double *v64, const double *psi_s,
                               size_t ps_stride, const double *duv, double *coef, size_t frame_stride, int T,
                               int C, const Skew &s, double ax, double ay, const double *a_data_minus1,
@@ -1305,8 +1328,13 @@ hipError_t launch_assemble_nl(const double *J, const double *Wsk, const float *u
     PadFlow fl{u32, v32, fs32, u64, v64};
     dim3 g(s.h + s.w - 1, T, (s.hs + 1023) / 1024), b(s.hs < 1024 ? s.hs : 1024);
 #define FRH_ASMNL(CC)                                                                               \
-    hipLaunchKernelGGL(k_assemble_nl<CC>, g, b, 0, st, J, Wsk, fl, psi_s, ps_stride, duv, coef, frame_stride, \
-                       s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one, zero_init)
+    if (w_f32)                                                                                      \
+        hipLaunchKernelGGL((k_assemble_nl<CC, float>), g, b, 0, st, J, (const float *)Wsk, fl, psi_s, ps_stride, \
+                           duv, coef, frame_stride, s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one, zero_init); \
+    else                                                                                            \
+        hipLaunchKernelGGL((k_assemble_nl<CC, double>), g, b, 0, st, J, (const double *)Wsk, fl, psi_s,     \
+                           ps_stride, duv, coef, frame_stride, s.h, s.w, s.hs, s.plane, ax, ay, ad, psi_one, \
+                           zero_init)
     switch (C) {
     case 1: FRH_ASMNL(1); break;
     case 2: FRH_ASMNL(2); break;
@@ -2038,6 +2066,23 @@ __global__ void k_to_skew_f32(const float *__restrict__ src, int h, int w, int h
 hipError_t launch_to_skew_f32(const float *src, int N, const Skew &s, double *dst, hipStream_t st)
 {
     hipLaunchKernelGGL(k_to_skew_f32, dim3(s.h + s.w - 1, N, (s.hs + 1023) / 1024),
+                       dim3(s.hs < 1024 ? s.hs : 1024), 0, st, src, s.h, s.w, s.hs, s.plane, dst);
+    return hipGetLastError();
+}
+
+// Row-major float32 planes (N, h, w) -> skewed float32 planes.
+__global__ void k_to_skew_f32f(const float *__restrict__ src, int h, int w, int hs, size_t plane,
+                               float *__restrict__ dst)
+{
+    int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x, n = blockIdx.y;
+    int c = d - r;
+    if (r >= h || c < 0 || c >= w) return;
+    dst[(size_t)n * plane + (size_t)d * hs + r] = src[((size_t)n * h + r) * w + c];
+}
+
+hipError_t launch_to_skew_f32f(const float *src, int N, const Skew &s, float *dst, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_to_skew_f32f, dim3(s.h + s.w - 1, N, (s.hs + 1023) / 1024),
                        dim3(s.hs < 1024 ? s.hs : 1024), 0, st, src, s.h, s.w, s.hs, s.plane, dst);
     return hipGetLastError();
 }
