@@ -40,6 +40,7 @@ import numpy as np
 import torch
 
 from .executor import HipBatchedExecutor, mean_frames, warp_frames
+from .of_options import normalize_weight
 from .optical_flow import _resolve_const_assumption
 from .preprocess import _range, preprocess
 
@@ -73,17 +74,24 @@ class RegistrationOptions:
     cc_initialization: bool = False
     cc_hw: object = 256
     cc_up: int = 1
+    gnc_schedule: Optional[Sequence[float]] = None
+    warping_steps: Optional[int] = None
     output_typename: Optional[str] = "double"
     flow_backend: Optional[str] = None
     backend_params: dict = field(default_factory=dict)
     get_displacement_impl: Optional[Callable] = None
     get_displacement_factory: Optional[Callable] = None
     extra: dict = field(default_factory=dict)
+    # the weight has had OFOptions' normalisation (from_any of an OFOptions)
+    weight_normalized: bool = False
 
     def __post_init__(self):
         # OF_options.py's validation of the prealignment fields
         if int(self.cc_up) < 1:
             raise ValueError("cc_up must be >= 1")
+        if not self.weight_normalized:
+            self.weight = normalize_weight(self.weight)
+            self.weight_normalized = True
 
     @property
     def effective_min_level(self) -> int:
@@ -98,8 +106,8 @@ class RegistrationOptions:
         return {"alpha": tuple(alpha), "levels": self.levels,
                 "min_level": self.effective_min_level, "eta": self.eta,
                 "iterations": self.iterations, "update_lag": self.update_lag,
-                "a_data": self.a_data, "a_smooth": self.a_smooth, "gnc_schedule": None,
-                "warping_steps": None, "const_assumption": self.constancy_assumption}
+                "a_data": self.a_data, "a_smooth": self.a_smooth, "gnc_schedule": self.gnc_schedule,
+                "warping_steps": self.warping_steps, "const_assumption": self.constancy_assumption}
 
     @classmethod
     def from_any(cls, options) -> "RegistrationOptions":
@@ -110,7 +118,8 @@ class RegistrationOptions:
         kw = {}
         for f in ("alpha", "weight", "levels", "min_level", "eta", "update_lag", "iterations",
                   "a_smooth", "a_data", "sigma", "buffer_size", "update_initialization_w",
-                  "update_reference", "cc_initialization", "cc_hw", "cc_up"):
+                  "update_reference", "cc_initialization", "cc_hw", "cc_up", "gnc_schedule",
+                  "warping_steps"):
             if hasattr(options, f):
                 kw[f] = getattr(options, f)
         for f in ("quality_setting", "channel_normalization", "interpolation_method",
@@ -121,6 +130,8 @@ class RegistrationOptions:
         for f in ("output_typename", "get_displacement_impl", "get_displacement_factory"):
             if hasattr(options, f):
                 kw[f] = getattr(options, f)
+        # OFOptions normalises its weight when it is set (OF_options.py:547-584)
+        kw["weight_normalized"] = hasattr(options, "get_weight_at")
         if getattr(options, "backend_params", None):
             kw["backend_params"] = dict(options.backend_params)
         # OFOptions' flow_backend names pyflowreg's own backends ("flowreg",
@@ -133,16 +144,41 @@ class RegistrationOptions:
             kw["flow_backend"] = fb
         if kw.get("channel_normalization") in ("joint",):
             kw["channel_normalization"] = "together"
+        # the options' own file and reader features this in-memory pipeline
+        # does not have
+        for f, default in (("bin_size", 1), ("n_references", 1), ("channel_idx", None)):
+            if getattr(options, f, default) != default:
+                raise ValueError(f"flowreg_hip compensate_arr: {f}={getattr(options, f)!r} is not implemented")
         return cls(**kw)
 
 
-def _channel_weights(weight, C):
-    """compensate_recording.py:504-516: per-channel constants summing to 1."""
-    w1 = np.asarray(weight, dtype=np.float64).reshape(-1)
-    s = w1.sum()
-    if s > 0:
-        w1 = w1 / s
-    return np.array([w1[c] if c < len(w1) else 1.0 / C for c in range(C)])
+def _weight_maps(weight, H, W, C):
+    """compensate_recording.py:504-516 with OFOptions.get_weight_at
+    (OF_options.py:679-724) on an already normalised weight: the (H, W, C)
+    float64 weight of each channel.  A single value applies to every
+    channel; more values than channels are cut to C and renormalised; a
+    missing channel gets 1/C; an (H, W) map is channel 0's (the others 1/C)
+    and an (H, W, C) map gives each channel its plane."""
+    w = np.asarray(weight, dtype=np.float64)
+    if w.ndim <= 1:
+        w = w.reshape(-1)
+        if w.size == 1:
+            vals = [float(w[0])] * C
+        else:
+            if w.size > C:
+                w = w[:C] / w[:C].sum()
+            vals = [float(w[c]) if c < w.size else 1.0 / C for c in range(C)]
+        return np.ones((H, W, C)) * np.asarray(vals).reshape(1, 1, -1)
+    out = np.empty((H, W, C))
+    if w.ndim == 2:
+        for c in range(C):
+            out[:, :, c] = w if c == 0 else 1.0 / C
+        return out
+    if w.ndim == 3:
+        for c in range(C):
+            out[:, :, c] = w[:, :, c] if c < w.shape[2] else 1.0 / C
+        return out
+    raise ValueError(f"Unexpected weight array with {w.ndim} dimensions")
 
 
 def _sigma_rows(sigma):
@@ -242,7 +278,7 @@ class MotionCorrector:
         self.reference_raw = ref[..., None] if ref.ndim == 2 else ref
         H, W, C = self.reference_raw.shape
         self.shape = (H, W, C)
-        self.weight = _channel_weights(self.opt.weight, C)
+        self.weight = _weight_maps(self.opt.weight, H, W, C)
         self.sigma = _sigma_rows(self.opt.sigma)
         # float64, as the reference keeps it; the executor casts it to float32
         self.reference_proc = self._preprocess(self.reference_raw, None, torch.float64)[0].permute(1, 2, 0)
@@ -273,7 +309,7 @@ class MotionCorrector:
     def _flow_params(self):
         fp = self.opt.flow_params()
         H, W, C = self.shape
-        fp["weight"] = np.ones((H, W, C)) * self.weight.reshape(1, 1, -1)
+        fp["weight"] = self.weight
         return fp
 
     def _reference_raw(self, dtype):

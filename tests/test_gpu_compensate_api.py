@@ -170,3 +170,25 @@ def test_output_typename_and_shapes(gpu):
     assert one.shape == (H, W) and w1.shape == (H, W, 2)
     with pytest.raises(ValueError, match="empty"):
         compensate_arr(np.zeros((0, H, W), np.float32), v[0], _opt())
+
+
+def test_ofoptions_drive_compensate_arr(gpu):
+    """An OFOptions (the reference's field names and defaults) gives the same
+    result as the equivalent RegistrationOptions; its gnc_schedule reaches
+    the displacement calls (the GNC stages change the flows)."""
+    from pyflowreg_amd import OFOptions
+    from pyflowreg_amd.compensate import compensate_arr
+    v = _video(9, 48)
+    o = OFOptions(alpha=2, buffer_size=10, iterations=20, sigma=[1.0, 1.0, 0.1], weight=[1, 1])
+    reg_o, w_o = compensate_arr(v[1:], v[0], o)
+    reg_r, w_r = compensate_arr(v[1:], v[0], _opt())
+    np.testing.assert_array_equal(w_o, w_r)
+    np.testing.assert_array_equal(reg_o, reg_r)
+    assert reg_o.dtype == np.float64
+    g = OFOptions(alpha=2, buffer_size=10, iterations=6, levels=3, sigma=[1.0, 1.0, 0.1],
+                  gnc_schedule=[0.0, 1.0], warping_steps=1)
+    _, w_g = compensate_arr(v[1:4], v[0], g)
+    n = OFOptions(alpha=2, buffer_size=10, iterations=6, levels=3, sigma=[1.0, 1.0, 0.1])
+    _, w_n = compensate_arr(v[1:4], v[0], n)
+    assert np.isfinite(w_g).all() and w_g.shape == w_n.shape
+    assert np.abs(w_g - w_n).max() > 1e-6
