@@ -851,8 +851,12 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
             const LevelDev &Lp = s->lv[li - 1];
             {
                 Scope sc(s, TM_RESIZE, st);
-                FRH_HIP(launch_resize(s->uvn, T * 2, Lp.h, Lp.w, s->uvl, h, w, L.up_ix, L.up_wx, L.up_iy,
-                                      L.up_wy, L.P_up, s->tmp, st), "flow upsample");
+                if (L.P_up == 4)
+                    FRH_HIP(launch_resize_fused4(s->uvn, T * 2, Lp.h, Lp.w, s->uvl, h, w, L.up_ix, L.up_wx, L.up_iy,
+                                                 L.up_wy, st), "flow upsample");
+                else
+                    FRH_HIP(launch_resize(s->uvn, T * 2, Lp.h, Lp.w, s->uvl, h, w, L.up_ix, L.up_wx, L.up_iy,
+                                          L.up_wy, L.P_up, s->tmp, st), "flow upsample");
             }
             {
                 Scope sc(s, TM_WARP, st);

@@ -46,6 +46,11 @@ hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, con
                               int rx, int sem, float *out, hipStream_t st);
 hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, const double *wx,
                                 int rx, int sem, double *out, hipStream_t st);
+// launch_resize for 4-tap tables (flow upsampling) in one kernel, the same
+// results without the intermediate plane
+hipError_t launch_resize_fused4(const float *src, int N, int h, int w, float *dst, int oh, int ow,
+                                const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
+                                hipStream_t st);
 hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int oh, int ow,
                          const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
                          int P, float *tmp, hipStream_t st);

@@ -244,6 +244,52 @@ hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int 
     return hipGetLastError();
 }
 
+// Both passes in one kernel for short tap lists (the flow upsampling
+// between levels, P = 4): each output sample forms the horizontal sums of
+// the P rows its vertical taps name -- each the same float32 products and
+// float64 running sum as k_resize_h, rounded to float32 -- and combines them
+// as k_resize_v does, so the result is the two-pass result bit for bit,
+// without the intermediate plane.
+template <int P>
+__global__ void k_resize_fused(const float *__restrict__ src, int h, int w, float *__restrict__ dst, int oh,
+                               int ow, const int32_t *__restrict__ ix, const float *__restrict__ wx,
+                               const int32_t *__restrict__ iy, const float *__restrict__ wy)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, n = blockIdx.z;
+    if (x >= ow) return;
+    int cx[P];
+    float kx[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+        cx[q] = ix[(size_t)x * P + q];
+        kx[q] = wx[(size_t)x * P + q];
+    }
+    const float *plane = src + (size_t)n * h * w;
+    double s = 0.0;
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const float *row = plane + (size_t)iy[(size_t)y * P + p] * w;
+        double hs = 0.0;
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+            const float prod = row[cx[q]] * kx[q];
+            hs += (double)prod;
+        }
+        const float prod = (float)hs * wy[(size_t)y * P + p];
+        s += (double)prod;
+    }
+    dst[((size_t)n * oh + y) * ow + x] = (float)s;
+}
+
+hipError_t launch_resize_fused4(const float *src, int N, int h, int w, float *dst, int oh, int ow,
+                                const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
+                                hipStream_t st)
+{
+    hipLaunchKernelGGL(k_resize_fused<4>, dim3((ow + 63) / 64, oh, N), dim3(64), 0, st, src, h, w, dst, oh, ow,
+                       ix, wx, iy, wy);
+    return hipGetLastError();
+}
+
 // Multi-level pyramid from full resolution (core/optical_flow.py:519-520:
 // every level is resized from the full-resolution frame).  Horizontal pass:
 // one work-group per (plane, block of R rows) stages the rows in LDS and
