@@ -575,7 +575,8 @@ def compensate_arr(c1, c_ref, options=None,
     opt = RegistrationOptions.from_any(options)
     over = {}
     if flow_backend is not None:
-        over["flow_backend"] = flow_backend
+        # the reference's own backend names select this backend, as in the options
+        over["flow_backend"] = None if flow_backend in _BUILTIN_BACKENDS else flow_backend
     if backend_params is not None:
         over["backend_params"] = {**opt.backend_params, **dict(backend_params)}
     if get_displacement is not None:

@@ -148,6 +148,9 @@ def test_factory_and_backend_resolution(gpu):
     assert made == [{"device": "cuda:0"}]
     _, w_b = compensate_arr(v[1:], v[0], _opt(), flow_backend="flowreg_hip")
     np.testing.assert_array_equal(w_f, w_b)
+    # the reference's own backend name selects this backend (a drop-in for its default)
+    _, w_r = compensate_arr(v[1:], v[0], _opt(), flow_backend="flowreg")
+    np.testing.assert_array_equal(w_r, w_b)
     with pytest.raises(ValueError, match="Unknown flow backend"):
         compensate_arr(v[1:], v[0], _opt(), flow_backend="no_such_backend")
     register_backend("test_zero_flow", lambda **p: (lambda f, m, **kw: np.zeros(f.shape[:2] + (2,))),

@@ -448,3 +448,34 @@ def test_mean_frames_equals_numpy(gpu, n):
     w = (rng.standard_normal((n, 37, 41, 2)) * rng.choice([1e-3, 1.0, 1e3], (n, 1, 1, 1))).astype(np.float32)
     out = mean_frames(torch.from_numpy(w).cuda()).cpu().numpy()
     np.testing.assert_array_equal(out, np.mean(w, axis=0))
+
+
+def _median_tie_inputs():
+    """2-D float64 arrays that a median run on float32 keys gets wrong unless
+    it resolves ties: distinct values one float64 ulp apart (one float32
+    key), exact duplicates, signed zeros, a wide exponent range, subnormals."""
+    rng = np.random.default_rng(11)
+    h, w = 37, 41
+    base = rng.choice([0.1, 1.0, -2.5, 3.0e5, 7.0e-6], size=(h, w))
+    ulps = base + rng.integers(-3, 4, size=(h, w)) * np.spacing(np.abs(base))
+    mixed = rng.standard_normal((h, w))
+    mixed[rng.random((h, w)) < 0.3] = 0.25
+    mixed[rng.random((h, w)) < 0.1] = 0.0
+    mixed[rng.random((h, w)) < 0.1] = -0.0
+    wide = np.sign(rng.standard_normal((h, w))) * 10.0 ** rng.uniform(-300, 300, (h, w))
+    sub = rng.integers(-6, 7, size=(h, w)) * 5e-324 + rng.integers(0, 2, size=(h, w)) * 1e-310
+    same = np.full((h, w), 3.0)
+    # one float32 key everywhere, distinct doubles: every window takes the exact path
+    one_key = 1.0 + rng.integers(0, 9, size=(h, w)) * 2.0 ** -52
+    return {"ulps": ulps, "mixed": mixed, "wide": wide, "sub": sub, "same": same, "one_key": one_key}
+
+
+@pytest.mark.parametrize("case", ["ulps", "mixed", "wide", "sub", "same", "one_key"])
+def test_median_float32_key_ties_match_scipy(gpu, case):
+    """The 5x5 mirror median (float32-keyed selection, float64 network for
+    windows whose median key several distinct samples share) equals
+    scipy.ndimage.median_filter on inputs built to break a keyed selection."""
+    from scipy.ndimage import median_filter
+    from pyflowreg_amd.optical_flow import median5_mirror
+    f = _median_tie_inputs()[case]
+    np.testing.assert_array_equal(median5_mirror(f), median_filter(f, size=(5, 5), mode="mirror"))
