@@ -65,12 +65,22 @@ def factory(**backend_params) -> Callable:
     return get_displacement
 
 
-def register(pyflowreg_core=None) -> None:
+def register(pyflowreg_core=None) -> bool:
     """Register ``flowreg_hip`` into pyflowreg's registry when given its
-    ``pyflowreg.core`` module (or into this module's registry otherwise)."""
+    ``pyflowreg.core`` module (or into this module's registry otherwise),
+    if libflowreg_hip.so loads: like the reference's GPU backends, which are
+    registered only when their library imports (core/__init__.py:97-104,
+    199-213), a backend whose library is missing is simply not registered.
+    Returns whether it was registered."""
+    from ._lib import lib
+    try:
+        lib()
+    except (OSError, RuntimeError):
+        return False
     target = pyflowreg_core if pyflowreg_core is not None else None
     reg = getattr(target, "register_backend", None) if target is not None else register_backend
     reg("flowreg_hip", factory, supported_executors=set(SUPPORTED_EXECUTORS))
+    return True
 
 
 register()
