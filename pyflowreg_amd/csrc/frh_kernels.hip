@@ -932,7 +932,7 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
     double *__restrict__ cuv, size_t fstride, int T)
 {
-    __shared__ double stage[kTensorRows][64];
+    __shared__ double stage[6][kTensorRows][64];
     const int tiles_x = (w + 63) / 64, tiles_y = (h + kTensorRows - 1) / kTensorRows;
     const unsigned nb = (unsigned)tiles_x * tiles_y * T;
     const unsigned L = xcd_block(blockIdx.x, nb);
@@ -942,15 +942,21 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
     const bool ok = r < h && c < w;  // every thread takes part in the staging barriers
     const int tid = threadIdx.y * 64 + threadIdx.x;
     const size_t hw = (size_t)h * w, d0 = (size_t)(r0 + c0);
-    // one output plane of the tile: cell (r, c) -> slot (r + c) * hs + r,
-    // stored diagonal by diagonal (slot s = diagonal j of the tile, row i)
-    auto put = [&](double val, auto *dst) {
-        stage[threadIdx.y][threadIdx.x] = val;
+    // N output planes of the tile at once: cell (r, c) -> slot (r + c) * hs + r,
+    // stored diagonal by diagonal (slot s = diagonal j of the tile, row i);
+    // plane e of the N goes to base + e * step
+    auto put = [&](auto n_planes, const double *vals, double *base, size_t step) {
+        constexpr int N = decltype(n_planes)::value;
+#pragma unroll
+        for (int e = 0; e < N; ++e) stage[e][threadIdx.y][threadIdx.x] = vals[e];
         __syncthreads();
         for (int s = tid; s < kTensorDiags * kTensorRows; s += 64 * kTensorRows) {
             const int j = s / kTensorRows, i = s % kTensorRows, cl = j - i;
-            if (cl >= 0 && cl < 64 && r0 + i < h && c0 + cl < w)
-                dst[(d0 + j) * hs + r0 + i] = (std::remove_pointer_t<decltype(dst)>)stage[i][cl];
+            if (cl >= 0 && cl < 64 && r0 + i < h && c0 + cl < w) {
+                const size_t q = (d0 + j) * hs + r0 + i;
+#pragma unroll
+                for (int e = 0; e < N; ++e) base[e * step + q] = stage[e][i][cl];
+            }
         }
         __syncthreads();
     };
@@ -960,16 +966,14 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
         if (ok)
             o = tensor_cell<KIND, false, P2, UNIT>(f1 + (size_t)t * f1_fstride + ch * hw,
                                                    f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx, eps);
-        put(o.J11, Jt + (size_t)(0 * C + ch) * plane);
-        put(o.J22, Jt + (size_t)(1 * C + ch) * plane);
-        put(o.J33, Jt + (size_t)(2 * C + ch) * plane);
-        put(o.J12, Jt + (size_t)(3 * C + ch) * plane);
-        put(o.J13, Jt + (size_t)(4 * C + ch) * plane);
-        put(o.J23, Jt + (size_t)(5 * C + ch) * plane);
+        // components J11, J22, J33, J12, J13, J23 -> planes (e * C + ch)
+        const double vals[6] = {o.J11, o.J22, o.J33, o.J12, o.J13, o.J23};
+        put(std::integral_constant<int, 6>{}, vals, Jt + (size_t)ch * plane, (size_t)C * plane);
     }
     double *ct = cuv + (size_t)t * fstride;
-    put(ok ? smooth_rhs(u + t * uv_stride, h, w, r, c, ax, ay) : 0.0, ct);
-    put(ok ? smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay) : 0.0, ct + plane);
+    const double cvals[2] = {ok ? smooth_rhs(u + t * uv_stride, h, w, r, c, ax, ay) : 0.0,
+                             ok ? smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay) : 0.0};
+    put(std::integral_constant<int, 2>{}, cvals, ct, plane);
 }
 
 hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_stride, const float *f2,
