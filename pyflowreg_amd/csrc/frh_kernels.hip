@@ -911,36 +911,38 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
     return s;
 }
 
-// Tiles of kTensorRows rows x 64 columns, one thread per cell (lanes run
-// along a row, so the image reads are coalesced).  Tiles are numbered
-// frame-major, then row-major, and dealt to the XCDs in contiguous ranges
-// (xcd_block) so neighbouring tiles share one L2.  J planes (component-major,
-// then channel) and cu, cv are written skewed: each output plane of a tile
-// goes through LDS and out diagonal by diagonal, so that 16 consecutive
-// lanes store one diagonal's 16 rows -- a contiguous 128-byte run -- instead
-// of every lane of a row storing to a different diagonal.  The level image
+// Tiles of kTensorRows rows x kTensorCols columns, one thread per cell
+// (lanes run along a row, so the image reads are coalesced).  Tiles are
+// numbered frame-major, then row-major, and dealt to the XCDs in contiguous
+// ranges (xcd_block) so neighbouring tiles share one L2.  J planes
+// (component-major, then channel) and cu, cv are written skewed: the output
+// planes of a tile go through LDS (a channel's six at once) and out diagonal
+// by diagonal, so that 16 consecutive lanes store one diagonal's 16 rows --
+// a contiguous 128-byte run -- instead of every lane of a row storing to a
+// different diagonal.  512-thread work-groups (24 KB of LDS) let several
+// tiles share a CU, so one tile's staging barriers do not stall the CU.  The level image
 // f1 is float64 data in the reference; P2 marks a float32 warped image
 // (every level below the top).  UNIT: unit grid spacing (hy = hx = 1, the
 // finest level), where the gradients' divisions by 2h and h^2 are exact.
-constexpr int kTensorRows = 16;
-constexpr int kTensorDiags = kTensorRows + 63;  // diagonals a tile touches
+constexpr int kTensorRows = 16, kTensorCols = 32;
+constexpr int kTensorDiags = kTensorRows + kTensorCols - 1;  // diagonals a tile touches
 
 template <int KIND, bool P2, bool UNIT = false>
-__global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
+__global__ __launch_bounds__(kTensorCols * kTensorRows) void k_tensor(
     const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
     size_t plane, double hy, double hx, double eps, size_t f1_fstride, const float *__restrict__ u,
     const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
     double *__restrict__ cuv, size_t fstride, int T)
 {
-    __shared__ double stage[6][kTensorRows][64];
-    const int tiles_x = (w + 63) / 64, tiles_y = (h + kTensorRows - 1) / kTensorRows;
+    __shared__ double stage[6][kTensorRows][kTensorCols];
+    const int tiles_x = (w + kTensorCols - 1) / kTensorCols, tiles_y = (h + kTensorRows - 1) / kTensorRows;
     const unsigned nb = (unsigned)tiles_x * tiles_y * T;
     const unsigned L = xcd_block(blockIdx.x, nb);
     const int t = L / (tiles_x * tiles_y), rem = L % (tiles_x * tiles_y);
-    const int r0 = (rem / tiles_x) * kTensorRows, c0 = (rem % tiles_x) * 64;
+    const int r0 = (rem / tiles_x) * kTensorRows, c0 = (rem % tiles_x) * kTensorCols;
     const int r = r0 + threadIdx.y, c = c0 + threadIdx.x;
     const bool ok = r < h && c < w;  // every thread takes part in the staging barriers
-    const int tid = threadIdx.y * 64 + threadIdx.x;
+    const int tid = threadIdx.y * kTensorCols + threadIdx.x;
     const size_t hw = (size_t)h * w, d0 = (size_t)(r0 + c0);
     // N output planes of the tile at once: cell (r, c) -> slot (r + c) * hs + r,
     // stored diagonal by diagonal (slot s = diagonal j of the tile, row i);
@@ -950,9 +952,9 @@ __global__ __launch_bounds__(64 * kTensorRows) void k_tensor(
 #pragma unroll
         for (int e = 0; e < N; ++e) stage[e][threadIdx.y][threadIdx.x] = vals[e];
         __syncthreads();
-        for (int s = tid; s < kTensorDiags * kTensorRows; s += 64 * kTensorRows) {
+        for (int s = tid; s < kTensorDiags * kTensorRows; s += kTensorCols * kTensorRows) {
             const int j = s / kTensorRows, i = s % kTensorRows, cl = j - i;
-            if (cl >= 0 && cl < 64 && r0 + i < h && c0 + cl < w) {
+            if (cl >= 0 && cl < kTensorCols && r0 + i < h && c0 + cl < w) {
                 const size_t q = (d0 + j) * hs + r0 + i;
 #pragma unroll
                 for (int e = 0; e < N; ++e) base[e * step + q] = stage[e][i][cl];
@@ -981,14 +983,14 @@ hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_
                          const float *v, size_t uv_frame_stride, double ax, double ay, double *J,
                          double *cuv, size_t frame_stride, const Skew &s, hipStream_t st)
 {
-    const size_t nb = (size_t)((w + 63) / 64) * ((h + kTensorRows - 1) / kTensorRows) * T;
+    const size_t nb = (size_t)((w + kTensorCols - 1) / kTensorCols) * ((h + kTensorRows - 1) / kTensorRows) * T;
     if (nb > 0x7fffffffu) return hipErrorInvalidValue;
 #define FRH_TEN(K, P)                                                                             \
-    hipLaunchKernelGGL((k_tensor<K, P>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, C, \
+    hipLaunchKernelGGL((k_tensor<K, P>), dim3((unsigned)nb), dim3(kTensorCols, kTensorRows), 0, st, f1, f2, C, \
                        h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
                        ay, J, cuv, frame_stride, T)
 #define FRH_TEN_U(K, P)                                                                           \
-    hipLaunchKernelGGL((k_tensor<K, P, true>), dim3((unsigned)nb), dim3(64, kTensorRows), 0, st, f1, f2, \
+    hipLaunchKernelGGL((k_tensor<K, P, true>), dim3((unsigned)nb), dim3(kTensorCols, kTensorRows), 0, st, f1, f2, \
                        C, h, w, s.hs, s.plane, hy, hx, eps, f1_frame_stride, u, v, uv_frame_stride, ax, \
                        ay, J, cuv, frame_stride, T)
     const bool unit = hy == 1.0 && hx == 1.0;
