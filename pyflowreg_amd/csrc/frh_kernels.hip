@@ -193,6 +193,37 @@ hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, c
 // ----------------------------------------------------------------- resize
 // util/resize_util.py:116-141: float32 products, float64 running sums,
 // float32 stores; horizontal pass then vertical pass.
+
+// Sum over the P taps of a table row: the float32 product of the tap's
+// value (load(index)) and weight, added in tap order to a float64 running
+// sum.  The table entries and values of eight taps are loaded before any of
+// their products is added, so eight memory round trips overlap instead of
+// following one another; the arithmetic and its order are unchanged.
+template <typename Load>
+__device__ __forceinline__ double tap_sum(int P, const int32_t *ii, const float *ww, Load load)
+{
+    double s = 0.0;
+    int p = 0;
+    for (; p + 8 <= P; p += 8) {
+        float v[8], wt[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            v[k] = load(ii[p + k]);
+            wt[k] = ww[p + k];
+        }
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const float prod = v[k] * wt[k];
+            s += (double)prod;
+        }
+    }
+    for (; p < P; ++p) {
+        const float prod = load(ii[p]) * ww[p];
+        s += (double)prod;
+    }
+    return s;
+}
+
 __global__ void k_resize_h(const float *__restrict__ src, int h, int w, float *__restrict__ tmp,
                            int ow, const int32_t *__restrict__ ix, const float *__restrict__ wx,
                            int P)
@@ -202,13 +233,7 @@ __global__ void k_resize_h(const float *__restrict__ src, int h, int w, float *_
     int n = blockIdx.z;
     if (x >= ow) return;
     const float *row = src + ((size_t)n * h + y) * w;
-    const int32_t *ii = ix + (size_t)x * P;
-    const float *ww = wx + (size_t)x * P;
-    double s = 0.0;
-    for (int p = 0; p < P; ++p) {
-        float prod = row[ii[p]] * ww[p];
-        s += (double)prod;
-    }
+    const double s = tap_sum(P, ix + (size_t)x * P, wx + (size_t)x * P, [&](int j) { return row[j]; });
     tmp[((size_t)n * h + y) * ow + x] = (float)s;
 }
 
@@ -221,13 +246,8 @@ __global__ void k_resize_v(const float *__restrict__ tmp, int h, float *__restri
     int n = blockIdx.z;
     if (x >= ow) return;
     const float *col = tmp + (size_t)n * h * ow + x;
-    const int32_t *ii = iy + (size_t)y * P;
-    const float *ww = wy + (size_t)y * P;
-    double s = 0.0;
-    for (int p = 0; p < P; ++p) {
-        float prod = col[(size_t)ii[p] * ow] * ww[p];
-        s += (double)prod;
-    }
+    const double s = tap_sum(P, iy + (size_t)y * P, wy + (size_t)y * P,
+                             [&](int i) { return col[(size_t)i * ow]; });
     dst[((size_t)n * oh + y) * ow + x] = (float)s;
 }
 
@@ -315,7 +335,26 @@ __global__ void k_pyr_h(const float *__restrict__ src, int H, int W, int Wsum,
         double acc[R];
 #pragma unroll
         for (int i = 0; i < R; ++i) acc[i] = 0.0;
-        for (int k = b; k < e; ++k) {
+        // the table entries of eight taps are loaded before their products
+        // are added (each row's sum keeps the tap order)
+        int k = b;
+        for (; k + 8 <= e; k += 8) {
+            int jj[8];
+            float wt[8];
+#pragma unroll
+            for (int q = 0; q < 8; ++q) {
+                jj[q] = hix[k + q];
+                wt[q] = hwx[k + q];
+            }
+#pragma unroll
+            for (int q = 0; q < 8; ++q)
+#pragma unroll
+                for (int i = 0; i < R; ++i) {
+                    float prod = rows[i * W + jj[q]] * wt[q];
+                    acc[i] += (double)prod;
+                }
+        }
+        for (; k < e; ++k) {
             const int j = hix[k];
             const float wt = hwx[k];
 #pragma unroll
@@ -339,13 +378,8 @@ __global__ void k_pyr_v(const float *__restrict__ tmp, int H, int Wsum, int off,
     int n = blockIdx.z;
     if (x >= ow) return;
     const float *col = tmp + (size_t)n * H * Wsum + off + x;
-    const int32_t *ii = iy + (size_t)y * P;
-    const float *ww = wy + (size_t)y * P;
-    double acc = 0.0;
-    for (int p = 0; p < P; ++p) {
-        float prod = col[(size_t)ii[p] * Wsum] * ww[p];
-        acc += (double)prod;
-    }
+    const double acc = tap_sum(P, iy + (size_t)y * P, wy + (size_t)y * P,
+                               [&](int i) { return col[(size_t)i * Wsum]; });
     dst[((size_t)n * oh + y) * ow + x] = (float)acc;
 }
 
