@@ -471,16 +471,35 @@ __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
         for (int j = 0; j < 4; ++j) w[i * 4 + j] = wyv[i] * wxv[j];
     int width1 = W - 3 > 0 ? W - 3 : 0, height1 = H - 3 > 0 ? H - 3 : 0;
     if ((unsigned)sx < (unsigned)width1 && (unsigned)sy < (unsigned)height1) {
-        for (int c = 0; c < C; ++c) {
+        // OpenCV's order: the four products of a row summed left to right,
+        // the rows added top to bottom
+        auto sum16 = [&](const I *a) {
+            Acc sum = (Acc)a[0] * w[0] + (Acc)a[1] * w[1] + (Acc)a[2] * w[2] + (Acc)a[3] * w[3];
+            sum += (Acc)a[4] * w[4] + (Acc)a[5] * w[5] + (Acc)a[6] * w[6] + (Acc)a[7] * w[7];
+            sum += (Acc)a[8] * w[8] + (Acc)a[9] * w[9] + (Acc)a[10] * w[10] + (Acc)a[11] * w[11];
+            sum += (Acc)a[12] * w[12] + (Acc)a[13] * w[13] + (Acc)a[14] * w[14] + (Acc)a[15] * w[15];
+            return sum;
+        };
+        auto load16 = [&](int c, I *a) {
             const I *S = F2 + c * HW + (size_t)sy * W + sx;
-            Acc sum = (Acc)S[0] * w[0] + (Acc)S[1] * w[1] + (Acc)S[2] * w[2] + (Acc)S[3] * w[3];
-            S += W;
-            sum += (Acc)S[0] * w[4] + (Acc)S[1] * w[5] + (Acc)S[2] * w[6] + (Acc)S[3] * w[7];
-            S += W;
-            sum += (Acc)S[0] * w[8] + (Acc)S[1] * w[9] + (Acc)S[2] * w[10] + (Acc)S[3] * w[11];
-            S += W;
-            sum += (Acc)S[0] * w[12] + (Acc)S[1] * w[13] + (Acc)S[2] * w[14] + (Acc)S[3] * w[15];
-            O[c * HW + p] = warp_store_value((float)sum, sat, lo, hi);
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) a[i * 4 + j] = S[(size_t)i * W + j];
+        };
+        int c = 0;
+        // two channels' 32 samples are loaded before either sum is formed
+        for (; c + 2 <= C; c += 2) {
+            I a0[16], a1[16];
+            load16(c, a0);
+            load16(c + 1, a1);
+            O[c * HW + p] = warp_store_value((float)sum16(a0), sat, lo, hi);
+            O[(c + 1) * HW + p] = warp_store_value((float)sum16(a1), sat, lo, hi);
+        }
+        for (; c < C; ++c) {
+            I a0[16];
+            load16(c, a0);
+            O[c * HW + p] = warp_store_value((float)sum16(a0), sat, lo, hi);
         }
     } else {
         int xs[4], ys[4];
