@@ -45,6 +45,16 @@ __device__ inline double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, u
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
 }
 
+// two consecutive float64 values in one 16-byte load
+struct F64x2 {
+    double lo, hi;
+};
+
+__device__ inline F64x2 buf_load_f64x2(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    return __builtin_bit_cast(F64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
+}
+
 __device__ inline void buf_store_f64(double x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
     using u2 = __attribute__((ext_vector_type(2))) unsigned;

@@ -1978,15 +1978,27 @@ __global__ __launch_bounds__(64) void k_median_add(
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(du, (unsigned)(2 * plane * sizeof(double)));
         const unsigned vq = (unsigned)(((size_t)(d - 4) * hs + (r - 2)) * sizeof(double));
         const unsigned pb = (unsigned)(plane * sizeof(double));
+        // the samples of one window diagonal k = i + j sit in consecutive
+        // slots (rows i of that diagonal): pairs of them in one 16-byte load
         double a[25], b[25];
 #pragma unroll
-        for (int i = 0; i < 5; ++i)
+        for (int k = 0; k <= 8; ++k) {
+            const int ilo = k > 4 ? k - 4 : 0, ihi = k < 4 ? k : 4;
 #pragma unroll
-            for (int j = 0; j < 5; ++j) {
-                const unsigned so = (unsigned)(((i + j) * hs + i) * sizeof(double));
-                a[i * 5 + j] = buf_load_f64(rs, vq, so);
-                b[i * 5 + j] = buf_load_f64(rs, vq, so + pb);
+            for (int i = ilo; i <= ihi; i += 2) {
+                const unsigned so = (unsigned)((k * hs + i) * sizeof(double));
+                if (i + 1 <= ihi) {
+                    const F64x2 pu = buf_load_f64x2(rs, vq, so), pv = buf_load_f64x2(rs, vq, so + pb);
+                    a[i * 5 + (k - i)] = pu.lo;
+                    a[(i + 1) * 5 + (k - i - 1)] = pu.hi;
+                    b[i * 5 + (k - i)] = pv.lo;
+                    b[(i + 1) * 5 + (k - i - 1)] = pv.hi;
+                } else {
+                    a[i * 5 + (k - i)] = buf_load_f64(rs, vq, so);
+                    b[i * 5 + (k - i)] = buf_load_f64(rs, vq, so + pb);
+                }
             }
+        }
         mu = median25(a);
         mv = median25(b);
     } else if (do_median) {
