@@ -62,26 +62,38 @@ struct GaussW {
 // into the float32 output (the double line sums rounded to float32).
 constexpr int kPreNorm32 = 1, kPreFilt32 = 2;
 
+// Column pass: a work-group of 64 lanes takes kPreRows output rows of 64
+// columns; each lane normalises the input rows its outputs need once, into
+// its own column of LDS (no lane reads another's, so no barrier), and forms
+// each output from them with the operations and order of the row sum below.
+constexpr int kPreRows = 16;
+
 template <typename T>
-__global__ void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, GaussW lo_den,
-                              GaussW wy, int ry, int sem, double *__restrict__ tmp)
+__global__ __launch_bounds__(64) void k_prefilter_v(const T *__restrict__ raw, int H, int W, int C, GaussW lo_den,
+                                                   GaussW wy, int ry, int sem, double *__restrict__ tmp)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    int tc = blockIdx.z;
+    extern __shared__ double win[];  // (kPreRows + 2 ry) rows x 64 lanes
+    const int x = blockIdx.x * blockDim.x + threadIdx.x;
+    const int y0 = blockIdx.y * kPreRows, n = H - y0 < kPreRows ? H - y0 : kPreRows;
+    const int tc = blockIdx.z;
     if (x >= W) return;
-    int t = tc / C, c = tc % C;
-    double lo = lo_den.w[2 * c], den = lo_den.w[2 * c + 1];
+    const int t = tc / C, c = tc % C;
+    const double lo = lo_den.w[2 * c], den = lo_den.w[2 * c + 1];
     const T *base = raw + (size_t)t * H * W * C + c;
     const bool n32 = (sem & kPreNorm32) != 0;
     auto nrm = [&](int yy) {
         const double d = (double)base[((size_t)yy * W + x) * C] - lo;
         return n32 ? (double)(float)((double)(float)d / den) : d / den;
     };
-    double acc = nrm(y) * wy.w[ry];
-    for (int j = ry; j >= 1; --j)
-        acc += (nrm(reflect_half(y - j, H)) + nrm(reflect_half(y + j, H))) * wy.w[ry - j];
-    tmp[((size_t)tc * H + y) * W + x] = (sem & kPreFilt32) ? (double)(float)acc : acc;
+    // row k of the window is input row y0 - ry + k (reflected into [0, H))
+    double *col = win + threadIdx.x;
+    for (int k = 0; k < n + 2 * ry; ++k) col[k * 64] = nrm(reflect_half(y0 - ry + k, H));
+    for (int i = 0; i < n; ++i) {
+        const double *cw = col + (i + ry) * 64;  // output row y0 + i
+        double acc = cw[0] * wy.w[ry];
+        for (int j = ry; j >= 1; --j) acc += (cw[-j * 64] + cw[j * 64]) * wy.w[ry - j];
+        tmp[((size_t)tc * H + y0 + i) * W + x] = (sem & kPreFilt32) ? (double)(float)acc : acc;
+    }
 }
 
 // Temporal pass of apply_gaussian_filter's 4-D branch (scipy's
@@ -125,18 +137,23 @@ hipError_t launch_prefilter_t(const void *raw, int raw_f64, int Tn, int H, int W
     return hipGetLastError();
 }
 
+// Row pass: the work-group's 64 outputs read their 64 + 2 rx inputs
+// (reflected at the row's ends) from LDS, loaded once.
 template <typename O>
-__global__ void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx, int rx, int sem,
-                              O *__restrict__ out)
+__global__ __launch_bounds__(64) void k_prefilter_h(const double *__restrict__ tmp, int H, int W, GaussW wx,
+                                                   int rx, int sem, O *__restrict__ out)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    int tc = blockIdx.z;
-    if (x >= W) return;
+    extern __shared__ double seg[];  // inputs x0 - rx .. x0 + 63 + rx
+    const int x0 = blockIdx.x * blockDim.x, x = x0 + threadIdx.x;
+    const int y = blockIdx.y, tc = blockIdx.z;
     const double *row = tmp + ((size_t)tc * H + y) * W;
-    double acc = row[x] * wx.w[rx];
-    for (int j = rx; j >= 1; --j)
-        acc += (row[reflect_half(x - j, W)] + row[reflect_half(x + j, W)]) * wx.w[rx - j];
+    const int nx = W - x0 < 64 ? W - x0 : 64;  // outputs of this work-group
+    for (int k = threadIdx.x; k < nx + 2 * rx; k += blockDim.x) seg[k] = row[reflect_half(x0 - rx + k, W)];
+    __syncthreads();
+    if (x >= W) return;
+    const double *cw = seg + threadIdx.x + rx;
+    double acc = cw[0] * wx.w[rx];
+    for (int j = rx; j >= 1; --j) acc += (cw[-j] + cw[j]) * wx.w[rx - j];
     out[((size_t)tc * H + y) * W + x] = (sem & kPreFilt32) ? (O)(float)acc : (O)acc;
 }
 
@@ -150,8 +167,9 @@ hipError_t launch_prefilter_v(const float *raw, int T, int H, int W, int C, cons
         ld.w[2 * c + 1] = den[c];
     }
     for (int k = 0; k < 2 * ry + 1; ++k) g.w[k] = wy[k];
-    dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_v<float>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, sem, tmp);
+    dim3 grid((W + 63) / 64, (H + kPreRows - 1) / kPreRows, T * C);
+    hipLaunchKernelGGL(k_prefilter_v<float>, grid, dim3(64), sizeof(double) * 64 * (kPreRows + 2 * ry), st, raw, H,
+                       W, C, ld, g, ry, sem, tmp);
     return hipGetLastError();
 }
 
@@ -165,8 +183,9 @@ hipError_t launch_prefilter_v64(const double *raw, int T, int H, int W, int C, c
         ld.w[2 * c + 1] = den[c];
     }
     for (int k = 0; k < 2 * ry + 1; ++k) g.w[k] = wy[k];
-    dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_v<double>, grid, dim3(64), 0, st, raw, H, W, C, ld, g, ry, sem, tmp);
+    dim3 grid((W + 63) / 64, (H + kPreRows - 1) / kPreRows, T * C);
+    hipLaunchKernelGGL(k_prefilter_v<double>, grid, dim3(64), sizeof(double) * 64 * (kPreRows + 2 * ry), st, raw,
+                       H, W, C, ld, g, ry, sem, tmp);
     return hipGetLastError();
 }
 
@@ -176,7 +195,8 @@ hipError_t launch_prefilter_h(const double *tmp, int T, int H, int W, int C, con
     GaussW g{};
     for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_h<float>, grid, dim3(64), 0, st, tmp, H, W, g, rx, sem, out);
+    hipLaunchKernelGGL(k_prefilter_h<float>, grid, dim3(64), sizeof(double) * (64 + 2 * rx), st, tmp, H, W, g, rx,
+                       sem, out);
     return hipGetLastError();
 }
 
@@ -186,7 +206,8 @@ hipError_t launch_prefilter_h64(const double *tmp, int T, int H, int W, int C, c
     GaussW g{};
     for (int k = 0; k < 2 * rx + 1; ++k) g.w[k] = wx[k];
     dim3 grid((W + 63) / 64, H, T * C);
-    hipLaunchKernelGGL(k_prefilter_h<double>, grid, dim3(64), 0, st, tmp, H, W, g, rx, sem, out);
+    hipLaunchKernelGGL(k_prefilter_h<double>, grid, dim3(64), sizeof(double) * (64 + 2 * rx), st, tmp, H, W, g, rx,
+                       sem, out);
     return hipGetLastError();
 }
 
