@@ -192,8 +192,10 @@ def copy_probe(mib):
 def uneven(args):
     """Independent pairs with sizes drawn from {256^2, 512^2, 796x512,
     1024^2} and 1-2 channels, max displacement U[2, 8] px, Poisson peak
-    U[100, 1000]: grouped by size and solved in batched launches; one step =
-    all pairs.  Prints one JSON line (pairs/s)."""
+    U[100, 1000]: one solver per size and channel count, all solved as one
+    work list (or, with --uneven-mode streams, each group's own batched
+    launches on concurrent streams); one step = all pairs.  Prints one JSON
+    line (pairs/s)."""
     import numpy as np
     import torch
     import pyflowreg_amd as fr
@@ -215,8 +217,15 @@ def uneven(args):
         s.set_reference(None, None)
         solvers.append((s, fx.permute(0, 2, 3, 1).contiguous(), mv.permute(0, 2, 3, 1).contiguous()))
 
-    def step():
-        return fr.optical_flow.run_groups(solvers, n_streams=args.streams, out_dtype=torch.float32)
+    wl = None
+    if args.uneven_mode == "worklist":
+        wl = fr.WorkList([s for s, _, _ in solvers])
+
+        def step():
+            return wl.run([fx for _, fx, _ in solvers], [mv for _, _, mv in solvers], out_dtype=torch.float32)
+    else:
+        def step():
+            return fr.optical_flow.run_groups(solvers, n_streams=args.streams, out_dtype=torch.float32)
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
@@ -232,7 +241,9 @@ def uneven(args):
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f64",
         "data": "synthetic pairs (blob scenes, smooth displacement max U[2,8] px, Poisson U[100,1000])",
         "config": {"workload": f"batched uneven: {len(pairs)} independent pairs per step, "
-                               f"{len(groups)} size/channel groups on {args.streams} streams",
+                               + (f"{len(groups)} size/channel groups in one work list (one launch per "
+                                  "stage and level step)" if wl is not None else
+                                  f"{len(groups)} size/channel groups on {args.streams} streams"),
                    "groups": {f"{H}x{W}x{C}": len(v) for (H, W, C), v in sorted(groups.items())}},
     }))
 
@@ -361,6 +372,9 @@ def main():
     ap.add_argument("--grad-frames", type=int, default=32, help="moving frames for the grad workload")
     ap.add_argument("--pairs", type=int, default=512)
     ap.add_argument("--streams", type=int, default=4, help="streams for the uneven workload's groups")
+    ap.add_argument("--uneven-mode", choices=("worklist", "streams"), default="worklist",
+                    help="uneven workload: one work list over all groups, or each group's own launches "
+                         "on --streams streams")
     args = ap.parse_args()
     if args.copy_probe:
         copy_probe(args.copy_probe)

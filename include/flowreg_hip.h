@@ -300,6 +300,29 @@ int frh_solver_run(frh_solver *s, const float *moving, int T, void *w_out, int o
 int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving, int T,
                          void *w_out, int out_f64, void *stream);
 
+/* Work list of uneven pairs (one launch per stage and level step for all
+ * frames of all solvers): solvers[0..n) (n <= 32, one per frame size and
+ * channel count; equal iterations, update_lag and data term, a_smooth = 1,
+ * at most 4 channels, levels of at most 1024 rows, every solver's weight and
+ * initial flow set by frh_solver_set_reference) run the level loop of
+ * frh_solver_run_pairs in lock-step, aligned at the finest level.
+ * frh_worklist_run takes, per solver k, T[k] pairs fixed[k], moving[k]
+ * ((T[k], C, H, W) float32 planes) and writes w_out[k] ((T[k], H, W, 2),
+ * float64 or float32).  Each pair's result equals its own get_displacement
+ * call (core/optical_flow.py:773-929) bit for bit; the batching replaces
+ * the reference's per-pair calls of its executors
+ * (motion_correction/parallelization/sequential.py:141-145).  The solvers'
+ * workspaces are used, so a solver runs in one work list or alone at a
+ * time.  The passes follow the default plan of at most 5 sweeps per pass
+ * (levels of up to 512 rows) or 2 (513 .. 1024 rows); limits set with
+ * frh_solver_set_pass_limit do not apply.  The work list keeps the solver
+ * handles; destroy it before them. */
+typedef struct frh_worklist frh_worklist;
+int frh_worklist_create(frh_worklist **out, frh_solver *const *solvers, int n);
+void frh_worklist_destroy(frh_worklist *wl);
+int frh_worklist_run(frh_worklist *wl, const float *const *fixed, const float *const *moving, const int *T,
+                     void *const *w_out, int out_f64, void *stream);
+
 /* Kernel-time accounting of the last frh_solver_run, measured with HIP
  * events on the run's stream: ms[k] is the summed duration and counts[k]
  * the number of launch groups of category k: 0 = whole run, 1 = SOR

@@ -7,13 +7,13 @@ from .live import FlowRegLive
 from .of_options import OFOptions
 from .compensate import compensate_arr
 from .xcorr import estimate_rigid_xcorr_2d, estimate_rigid_xcorr_batch
-from .optical_flow import (DisplacementSolver, get_displacement, get_displacements, get_motion_tensor_cs,
+from .optical_flow import (DisplacementSolver, WorkList, get_displacement, get_displacements, get_motion_tensor_cs,
                            get_motion_tensor_gc, get_motion_tensor_gray, imregister_wrapper,
                            imresize_fused_gauss_cubic, level_solver, normalize_gnc_schedule,
                            normalize_warping_steps, warpingDepth)
 
 __all__ = [
-    "DisplacementSolver", "FlowRegLive", "OFOptions", "compensate_arr", "get_displacement", "get_displacements", "get_motion_tensor_gc", "get_motion_tensor_gray",
+    "DisplacementSolver", "WorkList", "FlowRegLive", "OFOptions", "compensate_arr", "get_displacement", "get_displacements", "get_motion_tensor_gc", "get_motion_tensor_gray",
     "get_motion_tensor_cs", "normalize_gnc_schedule", "normalize_warping_steps", "imregister_wrapper",
     "imresize_fused_gauss_cubic", "level_solver", "warpingDepth", "factory", "register",
     "register_backend", "get_backend", "get_backend_executors", "list_backends",

@@ -89,6 +89,10 @@ SIGNATURES = {
     "frh_solver_set_reference": (c_int, [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_solver_run": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
     "frh_solver_run_pairs": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_void_p, c_int, c_void_p]),
+    "frh_worklist_create": (c_int, [ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p), c_int]),
+    "frh_worklist_destroy": (None, [c_void_p]),
+    "frh_worklist_run": (c_int, [c_void_p, ctypes.POINTER(c_void_p), ctypes.POINTER(c_void_p),
+                                 ctypes.POINTER(c_int), ctypes.POINTER(c_void_p), c_int, c_void_p]),
     "frh_solver_profile": (c_int, [c_void_p, c_int]),
     "frh_solver_set_graph": (c_int, [c_void_p, c_int]),
     "frh_solver_level_info": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),

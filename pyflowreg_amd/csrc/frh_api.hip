@@ -184,7 +184,7 @@ extern "C" {
 
 const char *frh_last_error(void) { return g_err; }
 
-int frh_abi_version(void) { return 6; }
+int frh_abi_version(void) { return 7; }
 
 int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
 {
@@ -1065,6 +1065,246 @@ int frh_solver_set_pass_limit(frh_solver *s, int level, int pass_limit)
         (void)hipGraphExecDestroy(s->gexec);
         s->gexec = nullptr;
     }
+    return FRH_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------- work list
+// Uneven batches: several solvers (one per frame size and channel count,
+// each holding its own workspace, weight levels and initial flow) run the
+// level loop of frh_solver_run_pairs in lock-step.  Their pyramids are
+// aligned at the finest level (level step k holds level k - (K - levels) of
+// a solver with `levels` levels, K the deepest pyramid), and every stage of a
+// level step is one launch over the frames of all solvers at that step: the
+// level images, the flow initialisation or upsampling, the warp, the motion
+// tensor, each pass's assembly and sweep, and the median.  The passes follow
+// one plan per kernel form: levels of at most 512 rows in 512-lane
+// work-groups (5 sweeps per pass), levels of 513 .. 1024 rows in 1024-lane
+// work-groups (2 sweeps per pass) on a second stream.
+struct frh_worklist {
+    std::vector<frh_solver *> g;
+    int K = 0;                       // level steps
+    std::vector<Pass> plan512, plan1024;
+    WlGroup *dev = nullptr;          // descriptors of every step (device)
+    WlGroup *host = nullptr;         // their staging (pinned host memory)
+    size_t cap = 0;                  // descriptors the buffers hold
+    hipEvent_t staged = nullptr;     // the last descriptor upload has been read
+    hipEvent_t done = nullptr;       // the last run's launches have finished
+    bool pending = false;
+    hipStream_t side = nullptr;      // 1024-lane passes
+    hipEvent_t fork = nullptr, join = nullptr;
+};
+
+static void free_worklist(frh_worklist *wl)
+{
+    if (!wl) return;
+    if (wl->pending && wl->staged) (void)hipEventSynchronize(wl->staged);
+    (void)hipFree(wl->dev);
+    (void)hipHostFree(wl->host);
+    if (wl->staged) (void)hipEventDestroy(wl->staged);
+    if (wl->done) (void)hipEventDestroy(wl->done);
+    if (wl->fork) (void)hipEventDestroy(wl->fork);
+    if (wl->join) (void)hipEventDestroy(wl->join);
+    if (wl->side) (void)hipStreamDestroy(wl->side);
+    delete wl;
+}
+
+// One launch's descriptor array: its groups and item prefix sums.
+struct WlLaunch {
+    size_t off = 0;   // first descriptor in the step buffers
+    WlStarts s{};
+    int max_h = 0, max_w = 0, max_hs = 0, max_tiles = 0, max_median = 0;
+    bool any_warp = false;
+};
+
+extern "C" {
+
+int frh_worklist_create(frh_worklist **out, frh_solver *const *solvers, int n)
+{
+    FRH_REQ(out && solvers, "frh_worklist_create: null pointer");
+    FRH_REQ(n >= 1 && n <= kWlMaxGroups, "frh_worklist_create: %d solvers outside [1, %d]", n, kWlMaxGroups);
+    const frh_solver *s0 = solvers[0];
+    FRH_REQ(s0, "frh_worklist_create: null solver");
+    for (int k = 0; k < n; ++k) {
+        const frh_solver *s = solvers[k];
+        FRH_REQ(s, "frh_worklist_create: null solver %d", k);
+        FRH_REQ(s->iterations == s0->iterations && s->update_lag == s0->update_lag,
+                "frh_worklist_create: solvers differ in iterations or update_lag");
+        FRH_REQ(s->const_assumption == s0->const_assumption, "frh_worklist_create: solvers differ in data term");
+        FRH_REQ(s->a_smooth == 1.0, "frh_worklist_create: a_smooth != 1 is not supported in a work list");
+        FRH_REQ(s->C <= 4, "frh_worklist_create: more than 4 channels");
+        for (int i = 0; i < s->nl; ++i) {
+            const LevelDev &L = s->lv[i];
+            FRH_REQ(L.sk.hs <= 1024, "frh_worklist_create: a level of %d rows (at most 1024 in a work list)", L.h);
+            // the sweep's buffer offsets are 32-bit
+            FRH_REQ(L.sk.plane * sizeof(double) * kCoefPlanes < 0x80000000ull,
+                    "frh_worklist_create: a %d x %d level is too large for 32-bit offsets", L.h, L.w);
+            FRH_REQ(i == 0 || L.P_up == 4, "frh_worklist_create: flow upsampling with %d taps", L.P_up);
+        }
+        for (int j = 0; j < k; ++j) FRH_REQ(solvers[j] != s, "frh_worklist_create: solver %d listed twice", k);
+    }
+    frh_worklist *wl = new frh_worklist();
+    for (int k = 0; k < n; ++k) {
+        wl->g.push_back(solvers[k]);
+        wl->K = std::max(wl->K, solvers[k]->nl);
+    }
+    wl->plan512 = plan_passes(s0->iterations, s0->update_lag, 5);
+    wl->plan1024 = plan_passes(s0->iterations, s0->update_lag, 2);
+    // per step: all active groups, those of the 512-lane form, those of the 1024-lane form
+    wl->cap = (size_t)3 * wl->K * n;
+    hipError_t e = hipMalloc((void **)&wl->dev, wl->cap * sizeof(WlGroup));
+    if (e == hipSuccess) e = hipHostMalloc((void **)&wl->host, wl->cap * sizeof(WlGroup), hipHostMallocDefault);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->staged, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->done, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->fork, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&wl->side, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        free_worklist(wl);
+        return fail(FRH_EHIP, "frh_worklist_create: %s", hipGetErrorString(e));
+    }
+    *out = wl;
+    return FRH_OK;
+}
+
+void frh_worklist_destroy(frh_worklist *wl) { free_worklist(wl); }
+
+int frh_worklist_run(frh_worklist *wl, const float *const *fixed, const float *const *moving, const int *T,
+                     void *const *w_out, int out_f64, void *stream)
+{
+    FRH_REQ(wl && fixed && moving && T && w_out, "frh_worklist_run: null pointer");
+    const int n = (int)wl->g.size();
+    hipStream_t st = as_stream(stream);
+    for (int k = 0; k < n; ++k) {
+        frh_solver *s = wl->g[k];
+        FRH_REQ(fixed[k] && moving[k] && w_out[k], "frh_worklist_run: null pointer for solver %d", k);
+        FRH_REQ(s->have_ref, "frh_worklist_run: frh_solver_set_reference has not been called for solver %d", k);
+        FRH_REQ(T[k] >= 1 && T[k] <= s->max_batch, "frh_worklist_run: T=%d outside [1, %d] for solver %d", T[k],
+                s->max_batch, k);
+        FRH_REQ(!s->wlow || out_f64, "frh_worklist_run: min_level > 0 needs float64 output");
+        if (!s->pyr1) {
+            size_t T_ = (size_t)s->max_batch, C = s->C;
+            int rc = dev_alloc(&s->pyr1, T_ * C * (size_t)s->H * s->Wsum, "reference pyramid rows");
+            if (!rc) rc = dev_alloc(&s->f1l, T_ * C * (size_t)s->lv.back().h * s->lv.back().w, "reference levels");
+            if (rc) return rc;
+        }
+    }
+    // descriptors of every step, staged in pinned memory: the previous run's
+    // upload must have been read before they are overwritten, and its
+    // kernels must have finished before the device copy is replaced
+    if (wl->pending) {
+        FRH_HIP(hipEventSynchronize(wl->staged), "work list staging");
+        FRH_HIP(hipStreamWaitEvent(st, wl->done, 0), "work list order");
+    }
+    const int K = wl->K;
+    std::vector<WlLaunch> all(K), f512(K), f1024(K);
+    size_t used = 0;
+    auto add = [&](WlLaunch &L, const WlGroup &G) {
+        if (L.s.n == 0) L.off = used;
+        wl->host[used++] = G;
+        L.s.start[L.s.n + 1] = L.s.start[L.s.n] + G.T;
+        L.s.n += 1;
+        L.max_h = std::max(L.max_h, G.h);
+        L.max_w = std::max(L.max_w, G.w);
+        L.max_hs = std::max(L.max_hs, G.hs);
+        L.max_tiles = std::max(L.max_tiles, ((G.w + 31) / 32) * ((G.h + 15) / 16));
+        L.max_median = std::max(L.max_median, (G.hs / 64) * (G.h + G.w - 1));
+        L.any_warp = L.any_warp || G.li > 0;
+    };
+    for (int pass = 0; pass < 3; ++pass) {
+        for (int k = 0; k < K; ++k) {
+            for (int gi = 0; gi < n; ++gi) {
+                frh_solver *s = wl->g[gi];
+                const int li = k - (K - s->nl);
+                if (li < 0) continue;
+                const LevelDev &L = s->lv[li];
+                if (pass == 1 && L.sk.hs > 512) continue;
+                if (pass == 2 && L.sk.hs <= 512) continue;
+                WlGroup G{};
+                G.T = T[gi];
+                G.C = s->C;
+                G.li = li;
+                G.h = L.h; G.w = L.w; G.hs = L.sk.hs;
+                G.plane = L.sk.plane;
+                G.fs = s->npl * L.sk.plane;
+                G.H = s->H; G.Wsum = s->Wsum; G.xoff = s->xoff[li];
+                G.P_img = L.P_img; G.img_iy = L.img_iy; G.img_wy = L.img_wy;
+                if (li > 0) {
+                    G.ph = s->lv[li - 1].h; G.pw = s->lv[li - 1].w;
+                    G.up_ix = L.up_ix; G.up_iy = L.up_iy; G.up_wx = L.up_wx; G.up_wy = L.up_wy;
+                }
+                G.h_row = L.h_row; G.h_col = L.h_col; G.axw = L.axw; G.ayw = L.ayw;
+                G.pyr = s->pyr; G.pyr1 = s->pyr1;
+                G.f2l = s->f2l; G.f1l = s->f1l; G.f2w = s->f2w; G.uvl = s->uvl; G.uvn = s->uvn;
+                G.uvtop = s->uvtop;
+                G.Wlev = s->Wlev + L.W_off;
+                G.work = s->work;
+                G.do_median = (L.h < L.w ? L.h : L.w) > 5;
+                if (li + 1 < s->nl) G.out_kind = 0;
+                else if (s->wlow) { G.out_kind = 3; G.out = s->wlow; }
+                else { G.out_kind = out_f64 ? 2 : 1; G.out = w_out[gi]; }
+                G.ad = make_adata(s->C, s->a_data.data(), s->am1.data(), -1.0);
+                add(pass == 0 ? all[k] : (pass == 1 ? f512[k] : f1024[k]), G);
+            }
+        }
+    }
+    FRH_HIP(hipMemcpyAsync(wl->dev, wl->host, used * sizeof(WlGroup), hipMemcpyHostToDevice, st),
+            "work list descriptors");
+    FRH_HIP(hipEventRecord(wl->staged, st), "work list staging");
+    wl->pending = true;
+    // pyramid rows of every frame (per solver: its frame size)
+    for (int gi = 0; gi < n; ++gi) {
+        frh_solver *s = wl->g[gi];
+        const int NC = T[gi] * s->C;
+        FRH_HIP(launch_pyr_h(moving[gi], NC, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr, st),
+                "moving pyramid rows");
+        FRH_HIP(launch_pyr_h(fixed[gi], NC, s->H, s->W, s->Wsum, s->hoff, s->hix, s->hwx, s->pyr1, st),
+                "reference pyramid rows");
+    }
+    const int kind = wl->g[0]->const_assumption;
+    for (int k = 0; k < K; ++k) {
+        const WlLaunch &A = all[k];
+        const WlGroup *ga = wl->dev + A.off;
+        FRH_HIP(launch_wl_pyr_v(ga, A.s, A.max_h, A.max_w, st), "work list: level images");
+        FRH_HIP(launch_wl_flow(ga, A.s, A.max_h, A.max_w, st), "work list: level flow");
+        if (A.any_warp) FRH_HIP(launch_wl_warp(ga, A.s, A.max_h, A.max_w, st), "work list: warp");
+        FRH_HIP(launch_wl_tensor(kind, ga, A.s, A.max_tiles, kCensusEps, st), "work list: motion tensor");
+        const bool tall = f1024[k].s.n > 0;
+        hipStream_t sst = st;
+        if (tall) {
+            FRH_HIP(hipEventRecord(wl->fork, st), "work list fork");
+            FRH_HIP(hipStreamWaitEvent(wl->side, wl->fork, 0), "work list fork");
+            sst = wl->side;
+        }
+        for (int form = 0; form < 2; ++form) {
+            const WlLaunch &F = form == 0 ? f512[k] : f1024[k];
+            if (F.s.n == 0) continue;
+            const std::vector<Pass> &plan = form == 0 ? wl->plan512 : wl->plan1024;
+            hipStream_t fs = form == 0 ? st : sst;
+            const WlGroup *gf = wl->dev + F.off;
+            for (size_t p = 0; p < plan.size(); ++p) {
+                if (plan[p].assemble)
+                    FRH_HIP(launch_wl_assemble(gf, F.s, F.max_h + F.max_w - 1, F.max_hs, plan[p].psi_one ? 1 : 0,
+                                               p == 0 ? 1 : 0, fs), "work list: assemble");
+                FRH_HIP(launch_wl_sweep(plan[p].nsweeps, form, gf, F.s, p == 0 ? 1 : 0, fs), "work list: sweep");
+            }
+        }
+        if (tall) {
+            FRH_HIP(hipEventRecord(wl->join, wl->side), "work list join");
+            FRH_HIP(hipStreamWaitEvent(st, wl->join, 0), "work list join");
+        }
+        FRH_HIP(launch_wl_median(ga, A.s, A.max_median, st), "work list: median");
+    }
+    // min_level > 0: cv2.resize of the coarse float64 flow to full resolution
+    for (int gi = 0; gi < n; ++gi) {
+        frh_solver *s = wl->g[gi];
+        if (!s->wlow) continue;
+        const LevelDev &L = s->lv.back();
+        FRH_HIP(launch_cv_resize(s->wlow, T[gi], L.h, L.w, 2, s->wtmp, (double *)w_out[gi], s->H, s->W, st),
+                "final cubic resize");
+    }
+    FRH_HIP(hipEventRecord(wl->done, st), "work list done");
     return FRH_OK;
 }
 

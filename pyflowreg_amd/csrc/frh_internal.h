@@ -33,6 +33,29 @@ __host__ __device__ inline size_t sk(int hs, int r, int c) { return (size_t)(r +
 // Stores a formatted message for frh_last_error() and returns code.
 int frh_report_error(int code, const char *fmt, ...);
 
+// Data-term exponents of the coefficient formation: a_k, a_k - 1 per
+// channel; GNC (core/level_solver.py:372-555): when the stage weight beta is
+// set and 0 < a_k < 1, beta < 1, psi_k becomes (1 - beta) + beta * robust.
+struct AData {
+    double a[8];
+    double am1[8];
+    int gnc[8];
+    double mix, beta;
+};
+
+inline AData make_adata(int C, const double *a_data, const double *a_data_minus1, double gnc_beta)
+{
+    AData ad{};
+    for (int k = 0; k < C && k < 8; ++k) {
+        ad.a[k] = a_data[k];
+        ad.am1[k] = a_data_minus1[k];
+        ad.gnc[k] = gnc_beta >= 0 && a_data[k] > 0.0 && a_data[k] < 1.0 && gnc_beta < 1.0;
+    }
+    ad.mix = 1.0 - gnc_beta;
+    ad.beta = gnc_beta;
+    return ad;
+}
+
 constexpr int kMaxSweepsPerPass = 8;
 constexpr int kCoefPlanes = 5;  // Du, Dv, A12, Bu, Bv
 
@@ -151,5 +174,55 @@ hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const d
 
 hipError_t launch_cv_resize(const double *src, int T, int h, int w, int C, double *tmp,
                             double *dst, int oh, int ow, hipStream_t st);
+
+// ------------------------------------------------------------- work lists
+// Uneven batches (pairs of different sizes, channel counts and pyramid
+// depths) run their level loops in lock-step, aligned at the finest level:
+// at level step k every group whose pyramid has reached its level
+// li = k - (K - levels) is one entry of a descriptor array, and each stage
+// of the level is one launch over all frames of all these groups (work item
+// = frame; items of group g are start[g] .. start[g] + T - 1).
+constexpr int kWlMaxGroups = 32;
+
+struct WlGroup {
+    int T, C, li;        // frames, channels, level index (0 = coarsest)
+    int h, w, hs;        // level size and skewed row stride
+    size_t plane, fs;    // skewed plane (values), workspace values per frame
+    int H, Wsum, xoff;   // full-resolution rows, staged pyramid rows
+    int P_img;
+    const int32_t *img_iy;
+    const float *img_wy;
+    int ph, pw;          // previous level (flow upsampling), li > 0
+    const int32_t *up_ix, *up_iy;
+    const float *up_wx, *up_wy;
+    double h_row, h_col, axw, ayw;
+    const float *pyr, *pyr1;     // staged rows of the moving / fixed frames
+    float *f2l, *f1l, *f2w;      // level images: moving, fixed, warped moving
+    float *uvl, *uvn;            // level flow, next level's flow (planes)
+    const float *uvtop;          // initial flow at the coarsest level
+    const float *Wlev;           // the level's skewed float32 weight planes
+    double *work;                // per frame: J (6C), cu/cv, 5 coefficients, du/dv
+    int do_median;
+    int out_kind;                // 0: uvn planes, 1: float32 / 2: float64 interleaved output, 3: float64 coarse flow
+    void *out;                   // output of frame 0 for out_kind 1..3
+    AData ad;
+};
+
+struct WlStarts {
+    int n;                        // groups in the launch
+    int start[kWlMaxGroups + 1];  // first item of each group, start[n] = items
+};
+
+hipError_t launch_wl_pyr_v(const WlGroup *g, const WlStarts &s, int max_oh, int max_ow, hipStream_t st);
+hipError_t launch_wl_flow(const WlGroup *g, const WlStarts &s, int max_h, int max_w, hipStream_t st);
+hipError_t launch_wl_warp(const WlGroup *g, const WlStarts &s, int max_h, int max_w, hipStream_t st);
+hipError_t launch_wl_tensor(int kind, const WlGroup *g, const WlStarts &s, int max_tiles, double eps,
+                            hipStream_t st);
+hipError_t launch_wl_assemble(const WlGroup *g, const WlStarts &s, int max_diag, int max_hs, int psi_one,
+                              int zero_init, hipStream_t st);
+// form1024: 1024-lane work-groups (levels of 513 .. 1024 rows, P <= 2)
+hipError_t launch_wl_sweep(int P, int form1024, const WlGroup *g, const WlStarts &s, int zero_init,
+                           hipStream_t st);
+hipError_t launch_wl_median(const WlGroup *g, const WlStarts &s, int max_blocks, hipStream_t st);
 
 }  // namespace frh

@@ -291,13 +291,13 @@ hipError_t launch_resize(const float *src, int N, int h, int w, float *dst, int 
 // float64 running sum as k_resize_h, rounded to float32 -- and combines them
 // as k_resize_v does, so the result is the two-pass result bit for bit,
 // without the intermediate plane.
+// Output sample (x, y) of one plane of width w.
 template <int P>
-__global__ void k_resize_fused(const float *__restrict__ src, int h, int w, float *__restrict__ dst, int oh,
-                               int ow, const int32_t *__restrict__ ix, const float *__restrict__ wx,
-                               const int32_t *__restrict__ iy, const float *__restrict__ wy)
+__device__ __forceinline__ float resize_fused_px(const float *__restrict__ plane, int w,
+                                                 const int32_t *__restrict__ ix, const float *__restrict__ wx,
+                                                 const int32_t *__restrict__ iy, const float *__restrict__ wy,
+                                                 int x, int y)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, n = blockIdx.z;
-    if (x >= ow) return;
     int cx[P];
     float kx[P];
 #pragma unroll
@@ -305,7 +305,6 @@ __global__ void k_resize_fused(const float *__restrict__ src, int h, int w, floa
         cx[q] = ix[(size_t)x * P + q];
         kx[q] = wx[(size_t)x * P + q];
     }
-    const float *plane = src + (size_t)n * h * w;
     double s = 0.0;
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -319,7 +318,17 @@ __global__ void k_resize_fused(const float *__restrict__ src, int h, int w, floa
         const float prod = (float)hs * wy[(size_t)y * P + p];
         s += (double)prod;
     }
-    dst[((size_t)n * oh + y) * ow + x] = (float)s;
+    return (float)s;
+}
+
+template <int P>
+__global__ void k_resize_fused(const float *__restrict__ src, int h, int w, float *__restrict__ dst, int oh,
+                               int ow, const int32_t *__restrict__ ix, const float *__restrict__ wx,
+                               const int32_t *__restrict__ iy, const float *__restrict__ wy)
+{
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, n = blockIdx.z;
+    if (x >= ow) return;
+    dst[((size_t)n * oh + y) * ow + x] = resize_fused_px<P>(src + (size_t)n * h * w, w, ix, wx, iy, wy, x, y);
 }
 
 hipError_t launch_resize_fused4(const float *src, int N, int h, int w, float *dst, int oh, int ow,
@@ -460,22 +469,17 @@ __device__ __forceinline__ float warp_store_value(float s, int sat, float lo, fl
 // images (OpenCV's 32F path), double for float64 images (its 64F path, which
 // the reference takes for its float64 level images); the result is stored as
 // float32, as the reference's float32 output array does.
+// One output pixel (x, y) of one frame: F2, F1 and O point at the frame's
+// C planes of H x W, (ux, vy) is the pixel's displacement.
 template <typename U, typename I, typename Acc>
-__global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
-                       int f1_per_frame, int C, int H, int W, const U *__restrict__ u,
-                       const U *__restrict__ v, size_t uv_stride, double h_col, double h_row,
-                       float *__restrict__ out, int sat = 0, float lo = 0.f, float hi = 0.f)
+__device__ __forceinline__ void warp_cubic_px(const I *__restrict__ F2, const I *__restrict__ F1,
+                                              float *__restrict__ O, int C, int H, int W, int x, int y,
+                                              U ux, U vy, double h_col, double h_row, int sat, float lo,
+                                              float hi)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    int t = blockIdx.z;
-    if (x >= W) return;
-    size_t HW = (size_t)H * W, p = (size_t)y * W + x;
-    const I *F2 = f2 + (size_t)t * C * HW;
-    const I *F1 = f1 + (f1_per_frame ? (size_t)t * C * HW : 0);
-    float *O = out + (size_t)t * C * HW;
-    float mx = (float)((double)x + (double)u[t * uv_stride + p] / h_col);
-    float my = (float)((double)y + (double)v[t * uv_stride + p] / h_row);
+    const size_t HW = (size_t)H * W, p = (size_t)y * W + x;
+    float mx = (float)((double)x + (double)ux / h_col);
+    float my = (float)((double)y + (double)vy / h_row);
     bool oob = (mx < 0.f) || (mx >= (float)W) || (my < 0.f) || (my >= (float)H);
     if (oob) {
         for (int c = 0; c < C; ++c) O[c * HW + p] = (float)F1[c * HW + p];
@@ -536,6 +540,22 @@ __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
             O[c * HW + p] = warp_store_value((float)sum, sat, lo, hi);
         }
     }
+}
+
+template <typename U, typename I, typename Acc>
+__global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
+                       int f1_per_frame, int C, int H, int W, const U *__restrict__ u,
+                       const U *__restrict__ v, size_t uv_stride, double h_col, double h_row,
+                       float *__restrict__ out, int sat = 0, float lo = 0.f, float hi = 0.f)
+{
+    int x = blockIdx.x * blockDim.x + threadIdx.x;
+    int y = blockIdx.y;
+    int t = blockIdx.z;
+    if (x >= W) return;
+    const size_t HW = (size_t)H * W, p = (size_t)y * W + x;
+    warp_cubic_px<U, I, Acc>(f2 + (size_t)t * C * HW, f1 + (f1_per_frame ? (size_t)t * C * HW : 0),
+                             out + (size_t)t * C * HW, C, H, W, x, y, u[t * uv_stride + p],
+                             v[t * uv_stride + p], h_col, h_row, sat, lo, hi);
 }
 
 hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
@@ -1001,19 +1021,15 @@ __device__ inline double smooth_rhs(const float *u, int h, int w, int r, int c, 
 constexpr int kTensorRows = 16, kTensorCols = 32;
 constexpr int kTensorDiags = kTensorRows + kTensorCols - 1;  // diagonals a tile touches
 
+// One tile (rows r0.., columns c0..) of one frame: f1t, f2t its C level
+// planes, ut, vt its level flow, Jt, ct its skewed output planes.
 template <int KIND, bool P2, bool UNIT = false>
-__global__ __launch_bounds__(kTensorCols * kTensorRows) void k_tensor(
-    const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
-    size_t plane, double hy, double hx, double eps, size_t f1_fstride, const float *__restrict__ u,
-    const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
-    double *__restrict__ cuv, size_t fstride, int T)
+__device__ __forceinline__ void tensor_tile(double (*stage)[kTensorRows][kTensorCols], const float *__restrict__ f1t,
+                                            const float *__restrict__ f2t, int C, int h, int w, int hs, size_t plane,
+                                            double hy, double hx, double eps, const float *__restrict__ ut,
+                                            const float *__restrict__ vt, double ax, double ay, double *__restrict__ Jt,
+                                            double *__restrict__ ct, int r0, int c0)
 {
-    __shared__ double stage[6][kTensorRows][kTensorCols];
-    const int tiles_x = (w + kTensorCols - 1) / kTensorCols, tiles_y = (h + kTensorRows - 1) / kTensorRows;
-    const unsigned nb = (unsigned)tiles_x * tiles_y * T;
-    const unsigned L = xcd_block(blockIdx.x, nb);
-    const int t = L / (tiles_x * tiles_y), rem = L % (tiles_x * tiles_y);
-    const int r0 = (rem / tiles_x) * kTensorRows, c0 = (rem % tiles_x) * kTensorCols;
     const int r = r0 + threadIdx.y, c = c0 + threadIdx.x;
     const bool ok = r < h && c < w;  // every thread takes part in the staging barriers
     const int tid = threadIdx.y * kTensorCols + threadIdx.x;
@@ -1036,20 +1052,35 @@ __global__ __launch_bounds__(kTensorCols * kTensorRows) void k_tensor(
         }
         __syncthreads();
     };
-    double *Jt = J + (size_t)t * fstride;
     for (int ch = 0; ch < C; ++ch) {
         TensorOut o{};
         if (ok)
-            o = tensor_cell<KIND, false, P2, UNIT>(f1 + (size_t)t * f1_fstride + ch * hw,
-                                                   f2 + ((size_t)t * C + ch) * hw, h, w, r, c, hy, hx, eps);
+            o = tensor_cell<KIND, false, P2, UNIT>(f1t + ch * hw, f2t + ch * hw, h, w, r, c, hy, hx, eps);
         // components J11, J22, J33, J12, J13, J23 -> planes (e * C + ch)
         const double vals[6] = {o.J11, o.J22, o.J33, o.J12, o.J13, o.J23};
         put(std::integral_constant<int, 6>{}, vals, Jt + (size_t)ch * plane, (size_t)C * plane);
     }
-    double *ct = cuv + (size_t)t * fstride;
-    const double cvals[2] = {ok ? smooth_rhs(u + t * uv_stride, h, w, r, c, ax, ay) : 0.0,
-                             ok ? smooth_rhs(v + t * uv_stride, h, w, r, c, ax, ay) : 0.0};
+    const double cvals[2] = {ok ? smooth_rhs(ut, h, w, r, c, ax, ay) : 0.0,
+                             ok ? smooth_rhs(vt, h, w, r, c, ax, ay) : 0.0};
     put(std::integral_constant<int, 2>{}, cvals, ct, plane);
+}
+
+template <int KIND, bool P2, bool UNIT = false>
+__global__ __launch_bounds__(kTensorCols * kTensorRows) void k_tensor(
+    const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
+    size_t plane, double hy, double hx, double eps, size_t f1_fstride, const float *__restrict__ u,
+    const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
+    double *__restrict__ cuv, size_t fstride, int T)
+{
+    __shared__ double stage[6][kTensorRows][kTensorCols];
+    const int tiles_x = (w + kTensorCols - 1) / kTensorCols, tiles_y = (h + kTensorRows - 1) / kTensorRows;
+    const unsigned nb = (unsigned)tiles_x * tiles_y * T;
+    const unsigned L = xcd_block(blockIdx.x, nb);
+    const int t = L / (tiles_x * tiles_y), rem = L % (tiles_x * tiles_y);
+    const int r0 = (rem / tiles_x) * kTensorRows, c0 = (rem % tiles_x) * kTensorCols;
+    tensor_tile<KIND, P2, UNIT>(stage, f1 + (size_t)t * f1_fstride, f2 + (size_t)t * C * h * w, C, h, w, hs, plane,
+                                hy, hx, eps, u + t * uv_stride, v + t * uv_stride, ax, ay, J + (size_t)t * fstride,
+                                cuv + (size_t)t * fstride, r0, c0);
 }
 
 hipError_t launch_tensor(int kind, int f2_f32, const float *f1, size_t f1_frame_stride, const float *f2,
@@ -1158,28 +1189,6 @@ hipError_t launch_tensor_rowmajor(int kind, int f1_f32, int f2_f32, const float 
 //   Du = (((wl+wr)+wd)+wu) + sum_k Wpsi_k J11_k,  Dv likewise with J22,
 //   A12 = sum_k Wpsi_k J12_k, Bu = cu - sum_k Wpsi_k J13_k, Bv = cv - ...J23,
 //   stored as {A12, Bu, Bv, omega/Du, omega/Dv}.
-// a_k, a_k - 1 per channel; GNC (core/level_solver.py:372-555): when the
-// stage weight beta is set and 0 < a_k < 1, beta < 1, psi_k becomes
-// (1 - beta) + beta * robust.
-struct AData {
-    double a[8];
-    double am1[8];
-    int gnc[8];
-    double mix, beta;
-};
-
-inline AData make_adata(int C, const double *a_data, const double *a_data_minus1, double gnc_beta)
-{
-    AData ad{};
-    for (int k = 0; k < C && k < 8; ++k) {
-        ad.a[k] = a_data[k];
-        ad.am1[k] = a_data_minus1[k];
-        ad.gnc[k] = gnc_beta >= 0 && a_data[k] > 0.0 && a_data[k] < 1.0 && gnc_beta < 1.0;
-    }
-    ad.mix = 1.0 - gnc_beta;
-    ad.beta = gnc_beta;
-    return ad;
-}
 
 // Coefficients of one cell from its raw data: J (6 components x C channels,
 // component-major), W (C), cu, cv, the sum ws of its four smoothness
@@ -1229,6 +1238,29 @@ __device__ __forceinline__ void coef_from_raw(const double *J, const double *Wc,
 // WT: the weight planes' type (float32 in the pyramid solver, whose weight
 // levels are float32 values; float64 for the stand-alone level solver's
 // caller-given weights).
+// Slot q of one frame: Jt, ct, dt, co its J, cu/cv, du/dv and coefficient
+// planes; Wsk the level's weight planes.
+template <int C, typename WT>
+__device__ __forceinline__ void assemble_cell(const double *__restrict__ Jt, const WT *__restrict__ Wsk,
+                                              const double *__restrict__ ct, const double *__restrict__ dt,
+                                              double *__restrict__ co, size_t plane, size_t q, double ax,
+                                              double ay, const AData &ad, int psi_one, int zero_init)
+{
+    double x = 0.0, y = 0.0;
+    if (!psi_one && !zero_init) {
+        x = dt[q];
+        y = dt[plane + q];
+    }
+    double Jr[6 * C], Wr[C], K[5];
+#pragma unroll
+    for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
+#pragma unroll
+    for (int k = 0; k < C; ++k) Wr[k] = (double)Wsk[(size_t)k * plane + q];
+    coef_from_raw<C>(Jr, Wr, ct[q], ct[plane + q], x, y, ax, ay, ad, psi_one, K);
+#pragma unroll
+    for (int e = 0; e < 5; ++e) co[e * plane + q] = K[e];
+}
+
 template <int C, typename WT>
 __global__ void k_assemble(const double *__restrict__ J, const WT *__restrict__ Wsk,
                            const double *__restrict__ cuv, const double *__restrict__ duv,
@@ -1239,23 +1271,9 @@ __global__ void k_assemble(const double *__restrict__ J, const WT *__restrict__ 
     int d = blockIdx.x, r = blockIdx.z * blockDim.x + threadIdx.x, t = blockIdx.y;
     int c = d - r;
     if (r >= h || c < 0 || c >= w) return;
-    size_t q = (size_t)d * hs + r;
-    const double *Jt = J + (size_t)t * fstride;
-    const double *ct = cuv + (size_t)t * fstride;
-    double x = 0.0, y = 0.0;
-    if (!psi_one && !zero_init) {
-        x = duv[(size_t)t * fstride + q];
-        y = duv[(size_t)t * fstride + plane + q];
-    }
-    double Jr[6 * C], Wr[C], K[5];
-#pragma unroll
-    for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
-#pragma unroll
-    for (int k = 0; k < C; ++k) Wr[k] = (double)Wsk[(size_t)k * plane + q];
-    coef_from_raw<C>(Jr, Wr, ct[q], ct[plane + q], x, y, ax, ay, ad, psi_one, K);
-    double *co = coef + (size_t)t * fstride;
-#pragma unroll
-    for (int e = 0; e < 5; ++e) co[e * plane + q] = K[e];
+    const size_t fo = (size_t)t * fstride;
+    assemble_cell<C, WT>(J + fo, Wsk, cuv + fo, duv + fo, coef + fo, plane, (size_t)d * hs + r, ax, ay, ad,
+                         psi_one, zero_init);
 }
 
 template <typename WT>
@@ -1609,7 +1627,7 @@ template <int P, int MAXW, int C, int S, bool TT, bool BAND>
 __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const SweepArgs &a, SweepBufs &bf,
                                            double *rdu, double *rdv,
                                            double (*xb)[MAXW][2][P + 1][2], int t, int r,
-                                           int lane, int wv, int nw, int Tend)
+                                           int lane, int wv, int nw, int Tend, size_t fo)
 {
     using St = SweepState<P, C, TT>;
     constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL;
@@ -1637,7 +1655,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
         } else {
             const int d = t + 1, c = d - r;
             if (row_ok && c >= 0 && c < w) {
-                load_raw<C>(st.raw, a, (size_t)blockIdx.x * a.fstride, (size_t)d * hs + r);
+                load_raw<C>(st.raw, a, fo, (size_t)d * hs + r);
             } else {
 #pragma unroll
                 for (int e = 0; e < 7 * C + 2; ++e) st.raw[e] = 0.0;
@@ -1791,30 +1809,32 @@ template <int P, int MAXW, int C, bool TT, bool BAND, int... S>
 __device__ __forceinline__ void sweep_block(SweepState<P, C, TT> &st, const SweepArgs &a,
                                             SweepBufs &bf, double *rdu,
                                             double *rdv, double (*xb)[MAXW][2][P + 1][2], int t0,
-                                            int r, int lane, int wv, int nw, int Tend,
+                                            int r, int lane, int wv, int nw, int Tend, size_t fo,
                                             std::integer_sequence<int, S...>)
 {
-    (sweep_step<P, MAXW, C, S, TT, BAND>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend), ...);
+    (sweep_step<P, MAXW, C, S, TT, BAND>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend, fo), ...);
 }
 
-template <int P, int MAXT, int C, bool BAND = false>
-__global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
+// One wavefront pass over frame f by the work-group's first nthr threads
+// (a multiple of 64; threads past them have left the kernel).
+template <int P, int MAXT, int C, bool BAND>
+__device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
+                                            double (*xb)[MAXT / 64][2][P + 1][2])
 {
     constexpr bool TT = MAXT > 512 && C == 0 && P >= 2;  // the 1-sweep form fits without it
     using St = SweepState<P, C, TT>;
     constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL, U = St::U;
     constexpr int MAXW = MAXT / 64;
-    const int f = blockIdx.x;
     const int r = threadIdx.x;
     const int lane = r & 63;
     const int wv = __builtin_amdgcn_readfirstlane(r >> 6);
-    const int nw = blockDim.x >> 6;
-    const double *cb = a.coef + (size_t)f * a.fstride;
-    double *du = a.duv + (size_t)f * a.fstride;
-    double *rdu = a.ring ? a.ring + (size_t)f * a.fstride : nullptr;
+    const int nw = nthr >> 6;
+    const size_t fo = (size_t)f * a.fstride;
+    const double *cb = a.coef + fo;
+    double *du = a.duv + fo;
+    double *rdu = a.ring ? a.ring + fo : nullptr;
     double *rdv = a.ring ? rdu + a.plane : nullptr;
-    __shared__ double xb[2][MAXW][2][P + 1][2];
-    for (int k = threadIdx.x; k < 2 * MAXW * 2 * (P + 1) * 2; k += blockDim.x)
+    for (int k = threadIdx.x; k < 2 * MAXW * 2 * (P + 1) * 2; k += nthr)
         (&xb[0][0][0][0][0])[k] = 0.0;
     const int h = a.h, w = a.w, hs = a.hs;
     const bool row_ok = r < h;
@@ -1861,7 +1881,7 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     } else {
         if (cell_ok(0)) {
             double raw0[7 * C + 2];
-            load_raw<C>(raw0, a, (size_t)f * a.fstride, (size_t)r);
+            load_raw<C>(raw0, a, fo, (size_t)r);
             coef_from_raw<C>(raw0, raw0 + 6 * C, raw0[7 * C], raw0[7 * C + 1], st.Lu[0], st.Lv[0],
                              a.ax, a.ay, a.ad, a.psi_one, st.K[0]);
         }
@@ -1869,8 +1889,15 @@ __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
     __syncthreads();
     const int Tend = (h + w - 2) + 2 * (P - 1);
     for (int t0 = 0; t0 <= Tend; t0 += U)
-        sweep_block<P, MAXW, C, TT, BAND>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend,
+        sweep_block<P, MAXW, C, TT, BAND>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend, fo,
                                           std::make_integer_sequence<int, U>{});
+}
+
+template <int P, int MAXT, int C, bool BAND = false>
+__global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
+{
+    __shared__ double xb[2][MAXT / 64][2][P + 1][2];
+    sweep_frame<P, MAXT, C, BAND>(a, blockIdx.x, blockDim.x, xb);
 }
 
 template <int P, int C>
@@ -1975,20 +2002,20 @@ hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const d
 // numbered frame-major, diagonal, row chunk and dealt to the XCDs in
 // contiguous ranges (neighbouring diagonals share the lines of the
 // row-major flow planes).
-__global__ __launch_bounds__(64) void k_median_add(
-    const double *__restrict__ duv, size_t fstride, int h, int w, int hs, size_t plane,
-    int do_median, const float *__restrict__ u, const float *__restrict__ v, size_t uv_stride,
-    float *out_f32, double *out_f64, int out_interleaved, int T)
+// Diagonal d, rows r0 .. r0 + 63 of one frame: du (and dv = du + plane) its
+// skewed increments, ut / vt its row-major level flow; the sum goes to
+// out_f32 / out_f64 (frame's (h, w, 2) interleaved output) or to the planes
+// out_f32, out_f32 + h w.
+__device__ __forceinline__ void median_add_px(const double *__restrict__ du, int h, int w, int hs, size_t plane,
+                                              int do_median, const float *__restrict__ ut,
+                                              const float *__restrict__ vt, float *out_f32, double *out_f64,
+                                              int out_interleaved, int d, int r0)
 {
-    const int nchunk = hs / 64, ndiag = h + w - 1;
-    const unsigned L = xcd_block(blockIdx.x, (unsigned)nchunk * ndiag * T);
-    const int t = L / (nchunk * ndiag), rem = L % (nchunk * ndiag);
-    const int d = rem / nchunk, r = (rem % nchunk) * 64 + threadIdx.x, c = d - r;
+    const int r = r0 + threadIdx.x, c = d - r;
     if (r >= h || c < 0 || c >= w) return;
-    const double *du = duv + (size_t)t * fstride, *dv = du + plane;
+    const double *dv = du + plane;
     double mu, mv;
     // rows of this wave that hold level cells, and their columns
-    const int r0 = (rem % nchunk) * 64;
     const int ra = max(r0, d - w + 1), rb = min(min(r0 + 63, d), h - 1);
     const bool interior = ra >= 2 && rb <= h - 3 && d - rb >= 2 && d - ra <= w - 3;
     if (do_median && interior) {
@@ -2046,10 +2073,10 @@ __global__ __launch_bounds__(64) void k_median_add(
         mv = dv[q];
     }
     size_t p = (size_t)r * w + c;
-    double un = (double)u[t * uv_stride + p] + mu;
-    double vn = (double)v[t * uv_stride + p] + mv;
+    double un = (double)ut[p] + mu;
+    double vn = (double)vt[p] + mv;
     if (out_interleaved) {
-        size_t o = ((size_t)t * h * w + p) * 2;
+        size_t o = p * 2;
         if (out_f64) {
             out_f64[o] = un;
             out_f64[o + 1] = vn;
@@ -2058,9 +2085,24 @@ __global__ __launch_bounds__(64) void k_median_add(
             out_f32[o + 1] = (float)vn;
         }
     } else {
-        out_f32[t * uv_stride + p] = (float)un;
-        out_f32[t * uv_stride + (size_t)h * w + p] = (float)vn;
+        out_f32[p] = (float)un;
+        out_f32[(size_t)h * w + p] = (float)vn;
     }
+}
+
+__global__ __launch_bounds__(64) void k_median_add(
+    const double *__restrict__ duv, size_t fstride, int h, int w, int hs, size_t plane,
+    int do_median, const float *__restrict__ u, const float *__restrict__ v, size_t uv_stride,
+    float *out_f32, double *out_f64, int out_interleaved, int T)
+{
+    const int nchunk = hs / 64, ndiag = h + w - 1;
+    const unsigned L = xcd_block(blockIdx.x, (unsigned)nchunk * ndiag * T);
+    const int t = L / (nchunk * ndiag), rem = L % (nchunk * ndiag);
+    const int d = rem / nchunk;
+    const size_t o = out_interleaved ? (size_t)t * h * w * 2 : t * uv_stride;
+    median_add_px(duv + (size_t)t * fstride, h, w, hs, plane, do_median, u + t * uv_stride, v + t * uv_stride,
+                  out_f32 ? out_f32 + o : nullptr, out_f64 ? out_f64 + o : nullptr, out_interleaved, d,
+                  (rem % nchunk) * 64);
 }
 
 hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew &s, int T,
@@ -2296,6 +2338,248 @@ hipError_t launch_cuv_padded(const double *u, const double *v, int m, int n, dou
 {
     hipLaunchKernelGGL(k_cuv_padded, dim3(s.h + s.w - 1, 1, (s.hs + 1023) / 1024), dim3(s.hs < 1024 ? s.hs : 1024),
                        0, st, u, v, m, n, ax, ay, s.hs, s.plane, cuv);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------- work lists
+// One launch per stage over the frames of every group of the level step
+// (descriptors gs, item prefix sums s).  Each kernel forms its frames'
+// values with the same device functions, in the same order, as the
+// launches of one group's level loop.
+__device__ __forceinline__ int wl_group(const WlStarts &s, int item)
+{
+    int g = 0;
+#pragma unroll
+    for (int k = 1; k < kWlMaxGroups; ++k)
+        if (k < s.n && item >= s.start[k]) g = k;
+    return g;
+}
+
+// level images of the moving and fixed frames (k_pyr_v)
+__global__ void k_wl_pyr_v(const WlGroup *__restrict__ gs, WlStarts s)
+{
+    const int item = blockIdx.z, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= G.w || y >= G.h) return;
+    const int t = item - s.start[gi], P = G.P_img;
+    const int32_t *iy = G.img_iy + (size_t)y * P;
+    const float *wy = G.img_wy + (size_t)y * P;
+    for (int c = 0; c < G.C; ++c) {
+        const size_t n = (size_t)t * G.C + c, o = (n * G.h + y) * G.w + x;
+        const float *col = G.pyr + n * G.H * G.Wsum + G.xoff + x;
+        G.f2l[o] = (float)tap_sum(P, iy, wy, [&](int i) { return col[(size_t)i * G.Wsum]; });
+        const float *col1 = G.pyr1 + n * G.H * G.Wsum + G.xoff + x;
+        G.f1l[o] = (float)tap_sum(P, iy, wy, [&](int i) { return col1[(size_t)i * G.Wsum]; });
+    }
+}
+
+hipError_t launch_wl_pyr_v(const WlGroup *g, const WlStarts &s, int max_oh, int max_ow, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wl_pyr_v, dim3((max_ow + 63) / 64, max_oh, s.start[s.n]), dim3(64), 0, st, g, s);
+    return hipGetLastError();
+}
+
+// the level's initial flow: the initial flow at the coarsest level
+// (k_broadcast_f32), the previous level's flow upsampled below it
+// (k_resize_fused<4>)
+__global__ void k_wl_flow(const WlGroup *__restrict__ gs, WlStarts s)
+{
+    const int item = blockIdx.z, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (x >= G.w || y >= G.h) return;
+    const int t = item - s.start[gi];
+    const size_t hw = (size_t)G.h * G.w, p = (size_t)y * G.w + x;
+    float *dst = G.uvl + (size_t)t * 2 * hw;
+    if (G.li == 0) {
+        dst[p] = G.uvtop[p];
+        dst[hw + p] = G.uvtop[hw + p];
+    } else {
+        const size_t phw = (size_t)G.ph * G.pw;
+        for (int k = 0; k < 2; ++k)
+            dst[k * hw + p] = resize_fused_px<4>(G.uvn + ((size_t)t * 2 + k) * phw, G.pw, G.up_ix, G.up_wx,
+                                                 G.up_iy, G.up_wy, x, y);
+    }
+}
+
+hipError_t launch_wl_flow(const WlGroup *g, const WlStarts &s, int max_h, int max_w, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wl_flow, dim3((max_w + 63) / 64, max_h, s.start[s.n]), dim3(64), 0, st, g, s);
+    return hipGetLastError();
+}
+
+// warp of the moving level images below the coarsest level (k_warp, the
+// 64F path of the float64 level images)
+__global__ void k_wl_warp(const WlGroup *__restrict__ gs, WlStarts s)
+{
+    const int item = blockIdx.z, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y;
+    if (G.li == 0 || x >= G.w || y >= G.h) return;
+    const int t = item - s.start[gi];
+    const size_t hw = (size_t)G.h * G.w, p = (size_t)y * G.w + x, fo = (size_t)t * G.C * hw;
+    const float *u = G.uvl + (size_t)t * 2 * hw;
+    warp_cubic_px<float, float, double>(G.f2l + fo, G.f1l + fo, G.f2w + fo, G.C, G.h, G.w, x, y, u[p], u[hw + p],
+                                        G.h_col, G.h_row, 0, 0.f, 0.f);
+}
+
+hipError_t launch_wl_warp(const WlGroup *g, const WlStarts &s, int max_h, int max_w, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wl_warp, dim3((max_w + 63) / 64, max_h, s.start[s.n]), dim3(64), 0, st, g, s);
+    return hipGetLastError();
+}
+
+// motion tensor and cu, cv (k_tensor); the warped image is float32 below the
+// coarsest level, the grid spacing is unit at full resolution
+template <int KIND>
+__global__ __launch_bounds__(kTensorCols * kTensorRows) void k_wl_tensor(const WlGroup *__restrict__ gs,
+                                                                          WlStarts s, double eps)
+{
+    __shared__ double stage[6][kTensorRows][kTensorCols];
+    const int item = blockIdx.y, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int h = G.h, w = G.w, C = G.C;
+    const int tiles_x = (w + kTensorCols - 1) / kTensorCols, tiles_y = (h + kTensorRows - 1) / kTensorRows;
+    if ((int)blockIdx.x >= tiles_x * tiles_y) return;
+    const int r0 = (blockIdx.x / tiles_x) * kTensorRows, c0 = (blockIdx.x % tiles_x) * kTensorCols;
+    const int t = item - s.start[gi];
+    const size_t hw = (size_t)h * w;
+    const float *f1t = G.f1l + (size_t)t * C * hw;
+    const float *f2t = (G.li == 0 ? G.f2l : G.f2w) + (size_t)t * C * hw;
+    const float *ut = G.uvl + (size_t)t * 2 * hw;
+    double *Jt = G.work + (size_t)t * G.fs, *ct = Jt + (size_t)6 * C * G.plane;
+    const bool unit = G.h_row == 1.0 && G.h_col == 1.0;
+#define FRH_WLT(P2, U)                                                                                    \
+    tensor_tile<KIND, P2, U>(stage, f1t, f2t, C, h, w, G.hs, G.plane, G.h_row, G.h_col, eps, ut, ut + hw, G.axw, \
+                             G.ayw, Jt, ct, r0, c0)
+    if (G.li == 0) {
+        if (unit) FRH_WLT(false, true);
+        else FRH_WLT(false, false);
+    } else {
+        if (unit) FRH_WLT(true, true);
+        else FRH_WLT(true, false);
+    }
+#undef FRH_WLT
+}
+
+hipError_t launch_wl_tensor(int kind, const WlGroup *g, const WlStarts &s, int max_tiles, double eps,
+                            hipStream_t st)
+{
+    const dim3 grid(max_tiles, s.start[s.n]), b(kTensorCols, kTensorRows);
+    switch (kind) {
+    case kTensorGC: hipLaunchKernelGGL(k_wl_tensor<kTensorGC>, grid, b, 0, st, g, s, eps); break;
+    case kTensorGray: hipLaunchKernelGGL(k_wl_tensor<kTensorGray>, grid, b, 0, st, g, s, eps); break;
+    case kTensorCS: hipLaunchKernelGGL(k_wl_tensor<kTensorCS>, grid, b, 0, st, g, s, eps); break;
+    default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// lag-block coefficients (k_assemble with the float32 weight planes)
+__global__ void k_wl_assemble(const WlGroup *__restrict__ gs, WlStarts s, int psi_one, int zero_init)
+{
+    const int item = blockIdx.y, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int d = blockIdx.x, r = threadIdx.x, c = d - r;
+    if (r >= G.h || c < 0 || c >= G.w) return;
+    const size_t pl = G.plane, q = (size_t)d * G.hs + r;
+    const double *Jt = G.work + (size_t)(item - s.start[gi]) * G.fs;
+    const double *ct = Jt + (size_t)6 * G.C * pl;
+    double *co = const_cast<double *>(ct) + 2 * pl;
+    const double *dt = co + 5 * pl;
+#define FRH_WLA(CC) assemble_cell<CC, float>(Jt, G.Wlev, ct, dt, co, pl, q, G.axw, G.ayw, G.ad, psi_one, zero_init)
+    switch (G.C) {
+    case 1: FRH_WLA(1); break;
+    case 2: FRH_WLA(2); break;
+    case 3: FRH_WLA(3); break;
+    case 4: FRH_WLA(4); break;
+    default: break;
+    }
+#undef FRH_WLA
+}
+
+hipError_t launch_wl_assemble(const WlGroup *g, const WlStarts &s, int max_diag, int max_hs, int psi_one,
+                              int zero_init, hipStream_t st)
+{
+    if (max_hs < 64 || max_hs > 1024) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_wl_assemble, dim3(max_diag, s.start[s.n]), dim3(max_hs), 0, st, g, s, psi_one,
+                       zero_init);
+    return hipGetLastError();
+}
+
+// one wavefront pass per frame (k_sweep); the work-group's waves past the
+// frame's rows (hs / 64 waves) leave at once, and the barriers of the pass
+// wait for the waves that remain
+template <int P, int MAXT>
+__global__ __launch_bounds__(MAXT) void k_wl_sweep(const WlGroup *__restrict__ gs, WlStarts s, int zero_init)
+{
+    __shared__ double xb[2][MAXT / 64][2][P + 1][2];
+    const int item = blockIdx.x, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int nthr = G.hs;
+    if ((int)threadIdx.x >= nthr) return;
+    SweepArgs a{};
+    a.coef = G.work + (size_t)(6 * G.C + 2) * G.plane;
+    a.duv = const_cast<double *>(a.coef) + 5 * G.plane;
+    a.ring = nullptr;
+    a.fstride = G.fs;
+    a.plane = G.plane;
+    a.h = G.h; a.w = G.w; a.hs = G.hs;
+    a.ax = G.axw; a.ay = G.ayw;
+    a.zero_init = zero_init;
+    a.r0 = 0;
+    a.hlev = G.h;
+    sweep_frame<P, MAXT, 0, false>(a, item - s.start[gi], nthr, xb);
+}
+
+hipError_t launch_wl_sweep(int P, int form1024, const WlGroup *g, const WlStarts &s, int zero_init,
+                           hipStream_t st)
+{
+    const dim3 grid(s.start[s.n]);
+    if (form1024) {
+        switch (P) {
+        case 1: hipLaunchKernelGGL((k_wl_sweep<1, 1024>), grid, dim3(1024), 0, st, g, s, zero_init); break;
+        case 2: hipLaunchKernelGGL((k_wl_sweep<2, 1024>), grid, dim3(1024), 0, st, g, s, zero_init); break;
+        default: return hipErrorInvalidValue;
+        }
+    } else {
+        switch (P) {
+        case 1: hipLaunchKernelGGL((k_wl_sweep<1, 512>), grid, dim3(512), 0, st, g, s, zero_init); break;
+        case 2: hipLaunchKernelGGL((k_wl_sweep<2, 512>), grid, dim3(512), 0, st, g, s, zero_init); break;
+        case 3: hipLaunchKernelGGL((k_wl_sweep<3, 512>), grid, dim3(512), 0, st, g, s, zero_init); break;
+        case 4: hipLaunchKernelGGL((k_wl_sweep<4, 512>), grid, dim3(512), 0, st, g, s, zero_init); break;
+        case 5: hipLaunchKernelGGL((k_wl_sweep<5, 512>), grid, dim3(512), 0, st, g, s, zero_init); break;
+        default: return hipErrorInvalidValue;
+        }
+    }
+    return hipGetLastError();
+}
+
+// median + u += du (k_median_add): next level's flow planes, the output
+// (float32 / float64, (h, w, 2) per frame) or the coarse float64 flow
+__global__ __launch_bounds__(64) void k_wl_median(const WlGroup *__restrict__ gs, WlStarts s)
+{
+    const int item = blockIdx.y, gi = wl_group(s, item);
+    const WlGroup &G = gs[gi];
+    const int nchunk = G.hs / 64, L = blockIdx.x;
+    if (L >= nchunk * (G.h + G.w - 1)) return;
+    const int t = item - s.start[gi];
+    const size_t hw = (size_t)G.h * G.w;
+    const double *du = G.work + (size_t)t * G.fs + (size_t)(6 * G.C + 7) * G.plane;
+    const float *ut = G.uvl + (size_t)t * 2 * hw;
+    float *o32 = nullptr;
+    double *o64 = nullptr;
+    if (G.out_kind == 0) o32 = G.uvn + (size_t)t * 2 * hw;
+    else if (G.out_kind == 1) o32 = (float *)G.out + (size_t)t * 2 * hw;
+    else o64 = (double *)G.out + (size_t)t * 2 * hw;
+    median_add_px(du, G.h, G.w, G.hs, G.plane, G.do_median, ut, ut + hw, o32, o64, G.out_kind != 0, L / nchunk,
+                  (L % nchunk) * 64);
+}
+
+hipError_t launch_wl_median(const WlGroup *g, const WlStarts &s, int max_blocks, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_wl_median, dim3(max_blocks, s.start[s.n]), dim3(64), 0, st, g, s);
     return hipGetLastError();
 }
 

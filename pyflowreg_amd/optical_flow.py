@@ -231,6 +231,7 @@ class DisplacementSolver:
         self.H, self.W, self.C = int(H), int(W), int(C)
         self.max_batch = int(max_batch)
         self.iterations, self.update_lag = int(iterations), int(update_lag)
+        self.a_smooth = float(a_smooth)
         self.plan = pyramid_plan(self.H, self.W, float(eta), int(levels), int(min_level), alpha)
         ad = np.broadcast_to(np.asarray(a_data, dtype=np.float64), (self.C,))
         self._a_data = np.ascontiguousarray(ad)
@@ -339,15 +340,7 @@ class DisplacementSolver:
         """Independent pairs: fixed, moving (T, H, W, C) -> flow (T, H, W, 2),
         each pair solved as its own get_displacement call (shared weight and
         initial flow from set_reference)."""
-        def planes_of(x):
-            t = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(x))
-            if t.ndim == 3:
-                t = t[..., None]
-            p = _as_planes(t)
-            if p.shape[1:] != (self.C, self.H, self.W):
-                raise ValueError(f"frames have shape {tuple(p.shape[1:])}, expected {(self.C, self.H, self.W)}")
-            return p
-        fp, mp = planes_of(fixed), planes_of(moving)
+        fp, mp = self._pair_planes(fixed), self._pair_planes(moving)
         if fp.shape[0] != mp.shape[0]:
             raise ValueError("fixed and moving hold different numbers of frames")
         T = fp.shape[0]
@@ -360,6 +353,16 @@ class DisplacementSolver:
             check(lib().frh_solver_run_pairs(self._h, ptr(fp[s:e]), ptr(mp[s:e]), e - s, ptr(out[s:e]),
                                              1 if f64 else 0, stream_handle()), "frh_solver_run_pairs")
         return out.to(out_dtype)
+
+    def _pair_planes(self, x):
+        """(T, H, W[, C]) frames -> (T, C, H, W) float32 device planes."""
+        t = x if isinstance(x, torch.Tensor) else torch.from_numpy(np.ascontiguousarray(x))
+        if t.ndim == 3:
+            t = t[..., None]
+        p = _as_planes(t)
+        if p.shape[1:] != (self.C, self.H, self.W):
+            raise ValueError(f"frames have shape {tuple(p.shape[1:])}, expected {(self.C, self.H, self.W)}")
+        return p
 
     def timings(self):
         """(ms, counts) per category of the last profiled run."""
@@ -434,14 +437,83 @@ def get_displacement(fixed, moving, alpha=(2, 2), update_lag=5, iterations=50, m
     return w.cpu().numpy()
 
 
+def worklist_eligible(solvers) -> bool:
+    """Whether frh_worklist accepts these solvers: at most 32, a_smooth = 1,
+    at most 4 channels, levels of at most 1024 rows."""
+    if not 1 <= len(solvers) <= 32:
+        return False
+    for s in solvers:
+        if s.a_smooth != 1.0 or s.C > 4 or any(h > 1024 for _, h, *_ in s.plan.levels):
+            return False
+    return True
+
+
+class WorkList:
+    """Independent pairs of uneven sizes, channel counts and pyramid depths,
+    one DisplacementSolver per (H, W, C), solved together: every stage of
+    every level step is one launch over the frames of all solvers
+    (frh_worklist_*; pyramids aligned at the finest level).  Each pair's
+    flow equals its own get_displacement call bit for bit.  The solvers'
+    weights and initial flows come from their set_reference calls."""
+
+    def __init__(self, solvers):
+        self.solvers = list(solvers)
+        n = len(self.solvers)
+        if not worklist_eligible(self.solvers):
+            raise ValueError("frh_worklist needs 1-32 solvers with a_smooth = 1, at most 4 channels "
+                             "and levels of at most 1024 rows")
+        hs = (_lib.c_void_p * n)(*[s._h.value for s in self.solvers])
+        h = _lib.c_void_p()
+        check(lib().frh_worklist_create(_lib.ctypes.byref(h), hs, n), "frh_worklist_create")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().frh_worklist_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def run(self, fixed, moving, out_dtype=torch.float64):
+        """fixed, moving: one (T_k, H_k, W_k, C_k) array or tensor per solver
+        -> list of (T_k, H_k, W_k, 2) flows on the GPU."""
+        n = len(self.solvers)
+        if len(fixed) != n or len(moving) != n:
+            raise ValueError(f"expected {n} fixed and moving batches")
+        fps, mps, outs, Ts = [], [], [], []
+        f64 = out_dtype == torch.float64 or any(s.plan.min_level > 0 for s in self.solvers)
+        for s, f, m in zip(self.solvers, fixed, moving):
+            fp, mp = s._pair_planes(f), s._pair_planes(m)
+            if fp.shape[0] != mp.shape[0]:
+                raise ValueError("fixed and moving hold different numbers of frames")
+            if not 1 <= fp.shape[0] <= s.max_batch:
+                raise ValueError(f"{fp.shape[0]} pairs for a solver of max_batch {s.max_batch}")
+            fps.append(fp)
+            mps.append(mp)
+            Ts.append(fp.shape[0])
+            outs.append(torch.empty((fp.shape[0], s.H, s.W, 2), dtype=torch.float64 if f64 else out_dtype,
+                                    device=mp.device))
+        arr = lambda xs: (_lib.c_void_p * n)(*[ptr(x) for x in xs])  # noqa: E731
+        check(lib().frh_worklist_run(self._h, arr(fps), arr(mps), (_lib.c_int * n)(*Ts), arr(outs),
+                                     1 if f64 else 0, stream_handle()), "frh_worklist_run")
+        return [o.to(out_dtype) for o in outs]
+
+
 def get_displacements(fixed_seq, moving_seq, alpha=(2, 2), update_lag=5, iterations=50,
                       min_level=0, levels=50, eta=0.8, a_smooth=1.0, a_data=0.45,
-                      const_assumption="gc", weight=None, max_batch=256):
+                      const_assumption="gc", weight=None, max_batch=256, worklist=None):
     """get_displacement for many independent (fixed, moving) pairs of uneven
-    sizes and channel counts: pairs are grouped by (H, W, C) and each group is
-    solved in batched launches (DisplacementSolver.run_pairs).  Returns a list
-    of (H, W, 2) float64 numpy flows in input order; each equals the
-    corresponding get_displacement(fixed, moving, ...) call."""
+    sizes and channel counts.  Pairs are grouped by (H, W, C), one solver
+    per group; with worklist=None (where frh_worklist accepts the groups) or
+    True all groups run as one work list (one launch per stage and level
+    step), with worklist=False each group runs its own batched launches on
+    one of 4 streams.  Returns a list of (H, W, 2) float64 numpy flows in
+    input order; each equals the corresponding get_displacement(fixed,
+    moving, ...) call."""
     term = _resolve_const_assumption(const_assumption)
     if np.isscalar(alpha):
         alpha = (alpha, alpha)
@@ -454,23 +526,35 @@ def get_displacements(fixed_seq, moving_seq, alpha=(2, 2), update_lag=5, iterati
         groups.setdefault(key, []).append(k)
     out = [None] * len(pairs)
     jobs = []
+    wl = None
     try:
         for (H, W, C), idx in groups.items():
             ad = tuple(np.broadcast_to(np.asarray(a_data, np.float64), (C,)).tolist())
-            solver = DisplacementSolver(H, W, C, alpha=alpha, update_lag=update_lag,
-                                        iterations=iterations, min_level=min_level, levels=levels,
-                                        eta=eta, a_smooth=a_smooth, a_data=np.array(ad),
-                                        max_batch=min(max_batch, len(idx)), const_assumption=term)
-            solver.set_reference(None, weight)
-            fx = np.stack([np.asarray(pairs[k][0], np.float32).reshape(H, W, C) for k in idx])
-            mv = np.stack([np.asarray(pairs[k][1], np.float32).reshape(H, W, C) for k in idx])
-            jobs.append((solver, fx, mv, idx))
-        results = run_groups([(s, fx, mv) for s, fx, mv, _ in jobs])
+            for s0 in range(0, len(idx), max_batch):
+                part = idx[s0:s0 + max_batch]
+                solver = DisplacementSolver(H, W, C, alpha=alpha, update_lag=update_lag,
+                                            iterations=iterations, min_level=min_level, levels=levels,
+                                            eta=eta, a_smooth=a_smooth, a_data=np.array(ad),
+                                            max_batch=len(part), const_assumption=term)
+                jobs.append((solver, None, None, part))
+                solver.set_reference(None, weight)
+                fx = np.stack([np.asarray(pairs[k][0], np.float32).reshape(H, W, C) for k in part])
+                mv = np.stack([np.asarray(pairs[k][1], np.float32).reshape(H, W, C) for k in part])
+                jobs[-1] = (solver, fx, mv, part)
+        solvers = [j[0] for j in jobs]
+        use_wl = worklist_eligible(solvers) if worklist is None else bool(worklist)
+        if use_wl:
+            wl = WorkList(solvers)
+            results = wl.run([j[1] for j in jobs], [j[2] for j in jobs])
+        else:
+            results = run_groups([(s, fx, mv) for s, fx, mv, _ in jobs])
         for (_, _, _, idx), w in zip(jobs, results):
             w = w.cpu().numpy()
             for j, k in enumerate(idx):
                 out[k] = w[j]
     finally:
+        if wl is not None:
+            wl.close()
         for s, *_ in jobs:
             s.close()
     return out
@@ -484,7 +568,6 @@ def run_groups(groups, n_streams=4, out_dtype=torch.float64):
     dev = _device()
     cur = torch.cuda.current_stream(dev)
     streams = [torch.cuda.Stream(dev) for _ in range(max(1, min(n_streams, len(groups))))]
-    outs = []
     # largest groups first, dealt round-robin
     order = sorted(range(len(groups)), key=lambda g: -groups[g][0].H * groups[g][0].W * groups[g][0].C)
     res = [None] * len(groups)
