@@ -1212,9 +1212,19 @@ int frh_worklist_run(frh_worklist *wl, const float *const *fixed, const float *c
         L.max_median = std::max(L.max_median, (G.hs / 64) * (G.h + G.w - 1));
         L.any_warp = L.any_warp || G.li > 0;
     };
+    // within a launch, the groups with the largest levels come first, so
+    // that the work-groups of the longest frames are dispatched first
+    std::vector<int> order(n);
     for (int pass = 0; pass < 3; ++pass) {
         for (int k = 0; k < K; ++k) {
-            for (int gi = 0; gi < n; ++gi) {
+            for (int gi = 0; gi < n; ++gi) order[gi] = gi;
+            auto cells = [&](int gi) {
+                const frh_solver *s = wl->g[gi];
+                const int li = k - (K - s->nl);
+                return li < 0 ? 0.0 : (double)s->lv[li].h * s->lv[li].w;
+            };
+            std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return cells(a) > cells(b); });
+            for (int gi : order) {
                 frh_solver *s = wl->g[gi];
                 const int li = k - (K - s->nl);
                 if (li < 0) continue;

@@ -2356,7 +2356,7 @@ __device__ __forceinline__ int wl_group(const WlStarts &s, int item)
 }
 
 // level images of the moving and fixed frames (k_pyr_v)
-__global__ void k_wl_pyr_v(const WlGroup *__restrict__ gs, WlStarts s)
+__global__ __launch_bounds__(64) void k_wl_pyr_v(const WlGroup *__restrict__ gs, WlStarts s)
 {
     const int item = blockIdx.z, gi = wl_group(s, item);
     const WlGroup &G = gs[gi];
@@ -2383,7 +2383,7 @@ hipError_t launch_wl_pyr_v(const WlGroup *g, const WlStarts &s, int max_oh, int 
 // the level's initial flow: the initial flow at the coarsest level
 // (k_broadcast_f32), the previous level's flow upsampled below it
 // (k_resize_fused<4>)
-__global__ void k_wl_flow(const WlGroup *__restrict__ gs, WlStarts s)
+__global__ __launch_bounds__(64) void k_wl_flow(const WlGroup *__restrict__ gs, WlStarts s)
 {
     const int item = blockIdx.z, gi = wl_group(s, item);
     const WlGroup &G = gs[gi];
@@ -2411,7 +2411,7 @@ hipError_t launch_wl_flow(const WlGroup *g, const WlStarts &s, int max_h, int ma
 
 // warp of the moving level images below the coarsest level (k_warp, the
 // 64F path of the float64 level images)
-__global__ void k_wl_warp(const WlGroup *__restrict__ gs, WlStarts s)
+__global__ __launch_bounds__(64) void k_wl_warp(const WlGroup *__restrict__ gs, WlStarts s)
 {
     const int item = blockIdx.z, gi = wl_group(s, item);
     const WlGroup &G = gs[gi];
