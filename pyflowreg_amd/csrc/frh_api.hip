@@ -1093,7 +1093,7 @@ struct frh_worklist {
     hipEvent_t done = nullptr;       // the last run's launches have finished
     bool pending = false;
     hipStream_t side = nullptr;      // 1024-lane passes
-    hipEvent_t fork = nullptr, join = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr, first = nullptr;
 };
 
 static void free_worklist(frh_worklist *wl)
@@ -1106,6 +1106,7 @@ static void free_worklist(frh_worklist *wl)
     if (wl->done) (void)hipEventDestroy(wl->done);
     if (wl->fork) (void)hipEventDestroy(wl->fork);
     if (wl->join) (void)hipEventDestroy(wl->join);
+    if (wl->first) (void)hipEventDestroy(wl->first);
     if (wl->side) (void)hipStreamDestroy(wl->side);
     delete wl;
 }
@@ -1159,6 +1160,7 @@ int frh_worklist_create(frh_worklist **out, frh_solver *const *solvers, int n)
     if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->join, hipEventDisableTiming);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&wl->first, hipEventDisableTiming);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&wl->side, hipStreamNonBlocking);
     if (e != hipSuccess) {
         free_worklist(wl);
@@ -1280,23 +1282,29 @@ int frh_worklist_run(frh_worklist *wl, const float *const *fixed, const float *c
         FRH_HIP(launch_wl_flow(ga, A.s, A.max_h, A.max_w, st), "work list: level flow");
         if (A.any_warp) FRH_HIP(launch_wl_warp(ga, A.s, A.max_h, A.max_w, st), "work list: warp");
         FRH_HIP(launch_wl_tensor(kind, ga, A.s, A.max_tiles, kCensusEps, st), "work list: motion tensor");
+        // The 1024-lane passes (the step's longest chain) go first, on the
+        // second stream; the 512-lane passes start after its first
+        // assembly, so that their work-groups fill the CUs that the
+        // shorter tall frames leave idle during the 1024-lane sweeps.
         const bool tall = f1024[k].s.n > 0;
-        hipStream_t sst = st;
         if (tall) {
             FRH_HIP(hipEventRecord(wl->fork, st), "work list fork");
             FRH_HIP(hipStreamWaitEvent(wl->side, wl->fork, 0), "work list fork");
-            sst = wl->side;
         }
-        for (int form = 0; form < 2; ++form) {
+        for (int form = 1; form >= 0; --form) {
             const WlLaunch &F = form == 0 ? f512[k] : f1024[k];
             if (F.s.n == 0) continue;
             const std::vector<Pass> &plan = form == 0 ? wl->plan512 : wl->plan1024;
-            hipStream_t fs = form == 0 ? st : sst;
+            hipStream_t fs = form == 0 ? st : wl->side;
             const WlGroup *gf = wl->dev + F.off;
             for (size_t p = 0; p < plan.size(); ++p) {
                 if (plan[p].assemble)
                     FRH_HIP(launch_wl_assemble(gf, F.s, F.max_h + F.max_w - 1, F.max_hs, plan[p].psi_one ? 1 : 0,
                                                p == 0 ? 1 : 0, fs), "work list: assemble");
+                if (form == 1 && p == 0) {
+                    FRH_HIP(hipEventRecord(wl->first, wl->side), "work list first assembly");
+                    FRH_HIP(hipStreamWaitEvent(st, wl->first, 0), "work list first assembly");
+                }
                 FRH_HIP(launch_wl_sweep(plan[p].nsweeps, form, gf, F.s, p == 0 ? 1 : 0, fs), "work list: sweep");
             }
         }

@@ -2476,12 +2476,17 @@ hipError_t launch_wl_tensor(int kind, const WlGroup *g, const WlStarts &s, int m
     return hipGetLastError();
 }
 
-// lag-block coefficients (k_assemble with the float32 weight planes)
-__global__ void k_wl_assemble(const WlGroup *__restrict__ gs, WlStarts s, int psi_one, int zero_init)
+// lag-block coefficients (k_assemble with the float32 weight planes); one
+// work-group per 256 rows of a diagonal, small enough to be placed on CUs
+// that the other form's sweeps occupy in part
+constexpr int kWlAsmRows = 256;
+
+__global__ __launch_bounds__(kWlAsmRows) void k_wl_assemble(const WlGroup *__restrict__ gs, WlStarts s,
+                                                            int psi_one, int zero_init)
 {
     const int item = blockIdx.y, gi = wl_group(s, item);
     const WlGroup &G = gs[gi];
-    const int d = blockIdx.x, r = threadIdx.x, c = d - r;
+    const int d = blockIdx.x, r = blockIdx.z * kWlAsmRows + threadIdx.x, c = d - r;
     if (r >= G.h || c < 0 || c >= G.w) return;
     const size_t pl = G.plane, q = (size_t)d * G.hs + r;
     const double *Jt = G.work + (size_t)(item - s.start[gi]) * G.fs;
@@ -2503,8 +2508,9 @@ hipError_t launch_wl_assemble(const WlGroup *g, const WlStarts &s, int max_diag,
                               int zero_init, hipStream_t st)
 {
     if (max_hs < 64 || max_hs > 1024) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_wl_assemble, dim3(max_diag, s.start[s.n]), dim3(max_hs), 0, st, g, s, psi_one,
-                       zero_init);
+    const int bx = max_hs < kWlAsmRows ? max_hs : kWlAsmRows;
+    hipLaunchKernelGGL(k_wl_assemble, dim3(max_diag, s.start[s.n], (max_hs + bx - 1) / bx), dim3(bx), 0, st, g, s,
+                       psi_one, zero_init);
     return hipGetLastError();
 }
 
