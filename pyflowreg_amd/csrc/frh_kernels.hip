@@ -1066,7 +1066,7 @@ __device__ __forceinline__ void tensor_tile(double (*stage)[kTensorRows][kTensor
 }
 
 template <int KIND, bool P2, bool UNIT = false>
-__global__ __launch_bounds__(kTensorCols * kTensorRows) void k_tensor(
+__global__ __launch_bounds__(kTensorCols * kTensorRows, KIND == kTensorCS ? 2 : 6) void k_tensor(
     const float *__restrict__ f1, const float *__restrict__ f2, int C, int h, int w, int hs,
     size_t plane, double hy, double hx, double eps, size_t f1_fstride, const float *__restrict__ u,
     const float *__restrict__ v, size_t uv_stride, double ax, double ay, double *__restrict__ J,
@@ -2002,14 +2002,20 @@ hipError_t launch_sweep_fused(int P, const double *J, const double *Wsk, const d
 // numbered frame-major, diagonal, row chunk and dealt to the XCDs in
 // contiguous ranges (neighbouring diagonals share the lines of the
 // row-major flow planes).
+// Rows staged per window diagonal by the median's interior path: the 64
+// rows of a wave, 2 above, 2 below, and the second half of a 16-byte load
+// past the end.
+constexpr int kMedStageRows = 70;
+using MedStage = double[2][9][kMedStageRows];
+
 // Diagonal d, rows r0 .. r0 + 63 of one frame: du (and dv = du + plane) its
 // skewed increments, ut / vt its row-major level flow; the sum goes to
 // out_f32 / out_f64 (frame's (h, w, 2) interleaved output) or to the planes
-// out_f32, out_f32 + h w.
+// out_f32, out_f32 + h w.  stage: the wave's LDS (one-wave work-groups).
 __device__ __forceinline__ void median_add_px(const double *__restrict__ du, int h, int w, int hs, size_t plane,
                                               int do_median, const float *__restrict__ ut,
                                               const float *__restrict__ vt, float *out_f32, double *out_f64,
-                                              int out_interleaved, int d, int r0)
+                                              int out_interleaved, int d, int r0, MedStage &stage)
 {
     const int r = r0 + threadIdx.x, c = d - r;
     if (r >= h || c < 0 || c >= w) return;
@@ -2019,32 +2025,35 @@ __device__ __forceinline__ void median_add_px(const double *__restrict__ du, int
     const int ra = max(r0, d - w + 1), rb = min(min(r0 + 63, d), h - 1);
     const bool interior = ra >= 2 && rb <= h - 3 && d - rb >= 2 && d - ra <= w - 3;
     if (do_median && interior) {
-        // uniform: no window of the wave is mirrored, so window sample
-        // (i, j) sits at slot q0 + (i + j) hs + i from the window's
-        // top-left slot q0 = (d - 4) hs + r - 2: one vector offset and
-        // scalar offsets for all 50 loads
+        // uniform: no window of the wave is mirrored.  Window sample (i, j)
+        // of the cell in row r lies on diagonal d + k - 4 (k = i + j), row
+        // r - 2 + i, so the wave's windows cover rows ra - 2 .. rb + 2 of
+        // the nine diagonals d - 4 .. d + 4.  Each of these runs (per field)
+        // is read once, in 16-byte pairs, into the wave's LDS, and every
+        // lane takes its 25 samples per field from there.
         const __amdgpu_buffer_rsrc_t rs = make_rsrc(du, (unsigned)(2 * plane * sizeof(double)));
-        const unsigned vq = (unsigned)(((size_t)(d - 4) * hs + (r - 2)) * sizeof(double));
         const unsigned pb = (unsigned)(plane * sizeof(double));
-        // the samples of one window diagonal k = i + j sit in consecutive
-        // slots (rows i of that diagonal): pairs of them in one 16-byte load
+        const int li = r - ra, m = rb - ra + 1, n = m + 4;  // lane's row in the run, active lanes, rows per run
+        for (int q = li; 2 * q < n; q += m) {
+#pragma unroll
+            for (int k = 0; k <= 8; ++k) {
+                const unsigned off = (unsigned)(((size_t)(d + k - 4) * hs + (ra - 2) + 2 * q) * sizeof(double));
+                const F64x2 pu = buf_load_f64x2(rs, off, 0), pv = buf_load_f64x2(rs, off, pb);
+                stage[0][k][2 * q] = pu.lo;
+                stage[0][k][2 * q + 1] = pu.hi;
+                stage[1][k][2 * q] = pv.lo;
+                stage[1][k][2 * q + 1] = pv.hi;
+            }
+        }
+        __syncthreads();
         double a[25], b[25];
 #pragma unroll
         for (int k = 0; k <= 8; ++k) {
             const int ilo = k > 4 ? k - 4 : 0, ihi = k < 4 ? k : 4;
 #pragma unroll
-            for (int i = ilo; i <= ihi; i += 2) {
-                const unsigned so = (unsigned)((k * hs + i) * sizeof(double));
-                if (i + 1 <= ihi) {
-                    const F64x2 pu = buf_load_f64x2(rs, vq, so), pv = buf_load_f64x2(rs, vq, so + pb);
-                    a[i * 5 + (k - i)] = pu.lo;
-                    a[(i + 1) * 5 + (k - i - 1)] = pu.hi;
-                    b[i * 5 + (k - i)] = pv.lo;
-                    b[(i + 1) * 5 + (k - i - 1)] = pv.hi;
-                } else {
-                    a[i * 5 + (k - i)] = buf_load_f64(rs, vq, so);
-                    b[i * 5 + (k - i)] = buf_load_f64(rs, vq, so + pb);
-                }
+            for (int i = ilo; i <= ihi; ++i) {
+                a[i * 5 + (k - i)] = stage[0][k][li + i];
+                b[i * 5 + (k - i)] = stage[1][k][li + i];
             }
         }
         mu = median25(a);
@@ -2100,9 +2109,10 @@ __global__ __launch_bounds__(64) void k_median_add(
     const int t = L / (nchunk * ndiag), rem = L % (nchunk * ndiag);
     const int d = rem / nchunk;
     const size_t o = out_interleaved ? (size_t)t * h * w * 2 : t * uv_stride;
+    __shared__ MedStage stage;
     median_add_px(duv + (size_t)t * fstride, h, w, hs, plane, do_median, u + t * uv_stride, v + t * uv_stride,
                   out_f32 ? out_f32 + o : nullptr, out_f64 ? out_f64 + o : nullptr, out_interleaved, d,
-                  (rem % nchunk) * 64);
+                  (rem % nchunk) * 64, stage);
 }
 
 hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew &s, int T,
@@ -2433,7 +2443,7 @@ hipError_t launch_wl_warp(const WlGroup *g, const WlStarts &s, int max_h, int ma
 // motion tensor and cu, cv (k_tensor); the warped image is float32 below the
 // coarsest level, the grid spacing is unit at full resolution
 template <int KIND>
-__global__ __launch_bounds__(kTensorCols * kTensorRows) void k_wl_tensor(const WlGroup *__restrict__ gs,
+__global__ __launch_bounds__(kTensorCols * kTensorRows, KIND == kTensorCS ? 2 : 6) void k_wl_tensor(const WlGroup *__restrict__ gs,
                                                                           WlStarts s, double eps)
 {
     __shared__ double stage[6][kTensorRows][kTensorCols];
@@ -2579,8 +2589,9 @@ __global__ __launch_bounds__(64) void k_wl_median(const WlGroup *__restrict__ gs
     if (G.out_kind == 0) o32 = G.uvn + (size_t)t * 2 * hw;
     else if (G.out_kind == 1) o32 = (float *)G.out + (size_t)t * 2 * hw;
     else o64 = (double *)G.out + (size_t)t * 2 * hw;
+    __shared__ MedStage stage;
     median_add_px(du, G.h, G.w, G.hs, G.plane, G.do_median, ut, ut + hw, o32, o64, G.out_kind != 0, L / nchunk,
-                  (L % nchunk) * 64);
+                  (L % nchunk) * 64, stage);
 }
 
 hipError_t launch_wl_median(const WlGroup *g, const WlStarts &s, int max_blocks, hipStream_t st)
