@@ -40,9 +40,14 @@ __device__ inline __amdgpu_buffer_rsrc_t make_rsrc(const void *p, unsigned bytes
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(p), 0, (int)bytes, 0x00020000);
 }
 
+// Cache policy of a raw buffer access: 0, or kNT (non-temporal: data that
+// streams through once is not kept in the caches).
+constexpr int kNT = 2;
+
+template <int AUX = 0>
 __device__ inline double buf_load_f64(__amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
-    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, AUX));
 }
 
 // two consecutive float64 values in one 16-byte load
@@ -55,10 +60,11 @@ __device__ inline F64x2 buf_load_f64x2(__amdgpu_buffer_rsrc_t r, unsigned voff, 
     return __builtin_bit_cast(F64x2, __builtin_amdgcn_raw_buffer_load_b128(r, (int)voff, (int)soff, 0));
 }
 
+template <int AUX = 0>
 __device__ inline void buf_store_f64(double x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
 {
     using u2 = __attribute__((ext_vector_type(2))) unsigned;
-    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, (int)soff, 0);
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, (int)soff, AUX);
 }
 
 // byte offset of cell (r, d - r) of the skewed plane, or kOutOfRange when

@@ -1246,19 +1246,23 @@ __device__ __forceinline__ void assemble_cell(const double *__restrict__ Jt, con
                                               double *__restrict__ co, size_t plane, size_t q, double ax,
                                               double ay, const AData &ad, int psi_one, int zero_init)
 {
+    // J, cu/cv, du/dv and the coefficients stream through once per pass
+    // (gigabytes apart): non-temporal accesses; the weight planes, shared
+    // by every frame, stay cacheable
     double x = 0.0, y = 0.0;
     if (!psi_one && !zero_init) {
-        x = dt[q];
-        y = dt[plane + q];
+        x = __builtin_nontemporal_load(&dt[q]);
+        y = __builtin_nontemporal_load(&dt[plane + q]);
     }
     double Jr[6 * C], Wr[C], K[5];
 #pragma unroll
-    for (int e = 0; e < 6 * C; ++e) Jr[e] = Jt[(size_t)e * plane + q];
+    for (int e = 0; e < 6 * C; ++e) Jr[e] = __builtin_nontemporal_load(&Jt[(size_t)e * plane + q]);
 #pragma unroll
     for (int k = 0; k < C; ++k) Wr[k] = (double)Wsk[(size_t)k * plane + q];
-    coef_from_raw<C>(Jr, Wr, ct[q], ct[plane + q], x, y, ax, ay, ad, psi_one, K);
+    coef_from_raw<C>(Jr, Wr, __builtin_nontemporal_load(&ct[q]), __builtin_nontemporal_load(&ct[plane + q]), x, y,
+                     ax, ay, ad, psi_one, K);
 #pragma unroll
-    for (int e = 0; e < 5; ++e) co[e * plane + q] = K[e];
+    for (int e = 0; e < 5; ++e) __builtin_nontemporal_store(K[e], &co[e * plane + q]);
 }
 
 template <int C, typename WT>
@@ -1769,8 +1773,10 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     {
         // sweep P-1 leaves diag t - 2(P-1): store its results
         const unsigned q = slot_offset(row_ok, t - 2 * (P - 1) + R0, rg, w, hs);
-        buf_store_f64(cu[P - 1], bf.duv, q, 0);
-        buf_store_f64(cv[P - 1], bf.duv, q, bf.plane_bytes);
+        // (the pass's results are read again only by the next pass or the
+        // assembly, after every frame's pass: non-temporal stores)
+        buf_store_f64<kNT>(cu[P - 1], bf.duv, q, 0);
+        buf_store_f64<kNT>(cv[P - 1], bf.duv, q, bf.plane_bytes);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
