@@ -32,6 +32,21 @@ __device__ inline unsigned xcd_block(unsigned b, unsigned nb)
     return (x < rem ? x * (per + 1) : rem * (per + 1) + (x - rem) * per) + k;
 }
 
+// The logical block (x, y, z) of a gx x gy x gz grid launched as
+// gx * gy * gz one-dimensional blocks (x fastest, then y, then z), numbered
+// with xcd_block: each XCD runs a contiguous range, i.e. whole frames, so
+// rows that the taps of neighbouring output rows share stay in one L2.
+struct Blk3 {
+    int x, y, z;
+};
+
+__device__ inline Blk3 xcd_block3(int gx, int gy)
+{
+    const unsigned gxy = (unsigned)gx * (unsigned)gy;
+    const unsigned L = xcd_block(blockIdx.x, gridDim.x);
+    return Blk3{(int)(L % (unsigned)gx), (int)((L / (unsigned)gx) % (unsigned)gy), (int)(L / gxy)};
+}
+
 __device__ inline int mirror_edge(int j, int n)  // ...c b | a b c...  (scipy "mirror")
 {
     if (n >= 3 && j >= -(n - 1) && j <= 2 * n - 2) {  // one reflection suffices
@@ -326,7 +341,8 @@ __global__ void k_resize_fused(const float *__restrict__ src, int h, int w, floa
                                int ow, const int32_t *__restrict__ ix, const float *__restrict__ wx,
                                const int32_t *__restrict__ iy, const float *__restrict__ wy)
 {
-    const int x = blockIdx.x * blockDim.x + threadIdx.x, y = blockIdx.y, n = blockIdx.z;
+    const Blk3 b = xcd_block3((ow + blockDim.x - 1) / blockDim.x, oh);
+    const int x = b.x * blockDim.x + threadIdx.x, y = b.y, n = b.z;
     if (x >= ow) return;
     dst[((size_t)n * oh + y) * ow + x] = resize_fused_px<P>(src + (size_t)n * h * w, w, ix, wx, iy, wy, x, y);
 }
@@ -335,8 +351,10 @@ hipError_t launch_resize_fused4(const float *src, int N, int h, int w, float *ds
                                 const int32_t *ix, const float *wx, const int32_t *iy, const float *wy,
                                 hipStream_t st)
 {
-    hipLaunchKernelGGL(k_resize_fused<4>, dim3((ow + 63) / 64, oh, N), dim3(64), 0, st, src, h, w, dst, oh, ow,
-                       ix, wx, iy, wy);
+    const size_t nb = (size_t)((ow + 63) / 64) * oh * N;
+    if (nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_resize_fused<4>, dim3((unsigned)nb), dim3(64), 0, st, src, h, w, dst, oh, ow, ix, wx, iy,
+                       wy);
     return hipGetLastError();
 }
 
@@ -403,9 +421,8 @@ __global__ void k_pyr_v(const float *__restrict__ tmp, int H, int Wsum, int off,
                         float *__restrict__ dst, int oh, int ow, const int32_t *__restrict__ iy,
                         const float *__restrict__ wy, int P)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    int n = blockIdx.z;
+    const Blk3 b = xcd_block3((ow + blockDim.x - 1) / blockDim.x, oh);
+    const int x = b.x * blockDim.x + threadIdx.x, y = b.y, n = b.z;
     if (x >= ow) return;
     const float *col = tmp + (size_t)n * H * Wsum + off + x;
     const double acc = tap_sum(P, iy + (size_t)y * P, wy + (size_t)y * P,
@@ -432,8 +449,9 @@ hipError_t launch_pyr_v(const float *tmp, int N, int H, int Wsum, int off, float
                         int ow, const int32_t *iy, const float *wy, int P, hipStream_t st)
 {
     int bx = ow >= 64 ? 64 : (ow >= 32 ? 32 : 16);
-    hipLaunchKernelGGL(k_pyr_v, dim3((ow + bx - 1) / bx, oh, N), dim3(bx), 0, st, tmp, H, Wsum, off,
-                       dst, oh, ow, iy, wy, P);
+    const size_t nb = (size_t)((ow + bx - 1) / bx) * oh * N;
+    if (nb > 0x7fffffffu) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_pyr_v, dim3((unsigned)nb), dim3(bx), 0, st, tmp, H, Wsum, off, dst, oh, ow, iy, wy, P);
     return hipGetLastError();
 }
 
@@ -548,9 +566,8 @@ __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
                        const U *__restrict__ v, size_t uv_stride, double h_col, double h_row,
                        float *__restrict__ out, int sat = 0, float lo = 0.f, float hi = 0.f)
 {
-    int x = blockIdx.x * blockDim.x + threadIdx.x;
-    int y = blockIdx.y;
-    int t = blockIdx.z;
+    const Blk3 b = xcd_block3((W + blockDim.x - 1) / blockDim.x, H);
+    const int x = b.x * blockDim.x + threadIdx.x, y = b.y, t = b.z;
     if (x >= W) return;
     const size_t HW = (size_t)H * W, p = (size_t)y * W + x;
     warp_cubic_px<U, I, Acc>(f2 + (size_t)t * C * HW, f1 + (f1_per_frame ? (size_t)t * C * HW : 0),
@@ -558,11 +575,15 @@ __global__ void k_warp(const I *__restrict__ f2, const I *__restrict__ f1,
                              v[t * uv_stride + p], h_col, h_row, sat, lo, hi);
 }
 
+// k_warp's one-dimensional grid of (W / 64) x H x T blocks (xcd_block3)
+static dim3 warp_grid(int W, int H, int T) { return dim3((unsigned)((size_t)((W + 63) / 64) * H * T)); }
+
 hipError_t launch_warp(const float *f2, const float *f1, int f1_per_frame, int T, int C, int H,
                        int W, const float *u, const float *v, size_t uv_frame_stride,
                        double h_col, double h_row, int acc64, float *out, hipStream_t st)
 {
-    const dim3 g((W + 63) / 64, H, T), b(64);
+    if ((size_t)((W + 63) / 64) * H * T > 0x7fffffffu) return hipErrorInvalidValue;
+    const dim3 g = warp_grid(W, H, T), b(64);
     if (acc64)
         hipLaunchKernelGGL((k_warp<float, float, double>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v,
                            uv_frame_stride, h_col, h_row, out);
@@ -576,7 +597,8 @@ hipError_t launch_warp_ex(const void *f2, const void *f1, int img_f64, int acc64
                           int C, int H, int W, const double *u, const double *v, size_t uv_frame_stride,
                           float *out, hipStream_t st)
 {
-    const dim3 g((W + 63) / 64, H, T), b(64);
+    if ((size_t)((W + 63) / 64) * H * T > 0x7fffffffu) return hipErrorInvalidValue;
+    const dim3 g = warp_grid(W, H, T), b(64);
     if (img_f64)
         hipLaunchKernelGGL((k_warp<double, double, double>), g, b, 0, st, (const double *)f2,
                            (const double *)f1, f1_per_frame, C, H, W, u, v, uv_frame_stride, 1.0, 1.0, out);
@@ -657,9 +679,11 @@ hipError_t launch_warp_saturate(int linear, const float *f2, const float *f1, in
     if (linear)
         hipLaunchKernelGGL((k_warp_linear<double, float, float>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u,
                            v, uv_frame_stride, out, 1, lo, hi);
+    else if ((size_t)((W + 63) / 64) * H * T > 0x7fffffffu)
+        return hipErrorInvalidValue;
     else
-        hipLaunchKernelGGL((k_warp<double, float, float>), g, b, 0, st, f2, f1, f1_per_frame, C, H, W, u, v,
-                           uv_frame_stride, 1.0, 1.0, out, 1, lo, hi);
+        hipLaunchKernelGGL((k_warp<double, float, float>), warp_grid(W, H, T), b, 0, st, f2, f1, f1_per_frame, C, H,
+                           W, u, v, uv_frame_stride, 1.0, 1.0, out, 1, lo, hi);
     return hipGetLastError();
 }
 
