@@ -386,7 +386,8 @@ int frh_grad_tensor(const double *fixed, const double *moving, int T, int C, int
  * weight W (C, h, w), ax = alpha_x / hx^2, ay = alpha_y / hy^2 and a_data
  * (C, device): coef (skewed, when != NULL).  With g_coef (skewed), adds the
  * pull-backs g_x (skewed), g_u0 (row-major, when u0 and g_u0 are not NULL),
- * g_J and per-cell partials g_par (T, 2 + C, plane) of ax, ay, a_k. */
+ * g_J and per-cell partials g_par (T, 2 + C, plane) of ax, ay, a_k (cell
+ * (r, c) at element r w + c of its plane; the rest untouched). */
 int frh_grad_coef(const double *x, const double *u0, const double *J, const double *W, int T, int C,
                   int h, int w, double ax, double ay, const double *a_data, int psi_one, double *coef,
                   const double *g_coef, double *g_x, double *g_u0, double *g_J, double *g_par,
@@ -397,10 +398,12 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
                    double ax, double ay, void *stream);
 
 /* Exact adjoint of frh_grad_sweep (all skewed): lam_in = (dx_out/dx_in)^T
- * lam_out; g_coef += (dx_out/dcoef)^T lam_out; g_axy (T, 2, plane) += per-cell
- * partials of ax, ay.  scratch: 4 T plane doubles. */
+ * lam_out; g_coef += (dx_out/dcoef)^T lam_out (g_coef = ... when
+ * g_coef_set != 0: the first adjoint sweep of a lag block needs no cleared
+ * buffer); g_axy (T, 2, plane) += per-cell partials of ax, ay.  scratch:
+ * 4 T plane doubles. */
 int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double *x_out,
-                           const double *lam_out, double *lam_in, double *g_coef, double *g_axy,
+                           const double *lam_out, double *lam_in, double *g_coef, int g_coef_set, double *g_axy,
                            double *scratch, int T, int h, int w, double ax, double ay,
                            void *stream);
 

@@ -254,11 +254,13 @@ class LevelSolve(torch.autograd.Function):
                 ev_next = recompute(order[i + 1], 1 - slot)
             main.wait_event(ev_rec)
             s, e, psi_one = blocks[b]
-            g_coef.zero_()
+            # the block's last sweep (its first adjoint) sets g_coef, the
+            # others add to it
             for k in range(e - s - 1, -1, -1):
                 check(L.frh_grad_sweep_adjoint(ptr(coefs[slot]), ptr(x_in(b, slot, k)), ptr(bufs[slot][k]),
-                                               ptr(lam), ptr(lam_in), ptr(g_coef), ptr(g_axy), ptr(scratch), T, h,
-                                               w, axf, ayf, st), "frh_grad_sweep_adjoint")
+                                               ptr(lam), ptr(lam_in), ptr(g_coef), int(k == e - s - 1),
+                                               ptr(g_axy), ptr(scratch), T, h, w, axf, ayf, st),
+                      "frh_grad_sweep_adjoint")
                 lam, lam_in = lam_in, lam
             # the block's coefficients depend on its start state, J, u0 and
             # the parameters; lam becomes the adjoint of the start state

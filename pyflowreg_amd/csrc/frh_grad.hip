@@ -297,18 +297,19 @@ __global__ void k_gskew(const double *src, double *dst, int N, GSkew k, int to_s
         dst[(size_t)n * hw + (size_t)r * k.w + c] = src[(size_t)n * k.plane + q];
 }
 
-// One thread per skewed slot: x, coef skewed; u0, J, W row-major.
+// One thread per cell in row-major order (the row-major J, W and u0 reads
+// are the many coalesced ones): x, coef skewed; u0, J, W row-major.
 __global__ __launch_bounds__(64) void k_gcoef_fwd(const double *x, const double *u0, const double *J,
                                                    const double *W, int T, int C, GSkew k, double ax,
                                                    double ay, const double *a_data, int psi_one, double *coef)
 {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
-    if (q >= k.plane) return;
-    int r, c;
-    if (!skew_cell(k, q, r, c)) return;
     const int h = k.h, w = k.w;
-    const size_t hw = (size_t)h * w, p = (size_t)r * w + c;
+    const size_t hw = (size_t)h * w;
+    if (p >= hw) return;
+    const int r = (int)(p / (size_t)w), c = (int)(p % (size_t)w);
+    const size_t q = (size_t)(r + c) * k.hs + r;
     double Jr[6 * kMaxC], Wr[kMaxC], a[kMaxC], K[6];
     for (int e = 0; e < 6; ++e)
         for (int kk = 0; kk < C; ++kk) Jr[e * C + kk] = J[(((size_t)t * 6 + e) * C + kk) * hw + p];
@@ -346,13 +347,13 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
                                                    double ax, double ay, const double *a_data, int psi_one,
                                                    double *gx, double *gu0, double *gJ, double *gpar)
 {
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
-    if (q >= k.plane) return;
-    int r, c;
-    if (!skew_cell(k, q, r, c)) return;
     const int h = k.h, w = k.w;
-    const size_t hw = (size_t)h * w, p = (size_t)r * w + c, P = k.plane;
+    const size_t hw = (size_t)h * w, P = k.plane;
+    if (p >= hw) return;
+    const int r = (int)(p / (size_t)w), c = (int)(p % (size_t)w);
+    const size_t q = (size_t)(r + c) * k.hs + r;
     const double xv = x[(size_t)t * 2 * P + q], yv = x[((size_t)t * 2 + 1) * P + q];
     double g[6];
     for (int e = 0; e < 6; ++e) g[e] = gK[((size_t)t * 6 + e) * P + q];
@@ -407,7 +408,7 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
             g_x += gE * 2.0 * (j11 * xv + j12 * yv + j13);
             g_y += gE * 2.0 * (j22 * yv + j12 * xv + j23);
         }
-        gpar[((size_t)t * (2 + C) + 2 + kk) * P + q] += g_a;
+        gpar[((size_t)t * (2 + C) + 2 + kk) * P + p] += g_a;
         for (int e = 0; e < 6; ++e) gJ[(((size_t)t * 6 + e) * C + kk) * hw + p] += gj[e];
     }
     if (u0) {
@@ -428,8 +429,8 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
     }
     gx[(size_t)t * 2 * P + q] += g_x;
     gx[((size_t)t * 2 + 1) * P + q] += g_y;
-    gpar[((size_t)t * (2 + C) + 0) * P + q] += g_ax;
-    gpar[((size_t)t * (2 + C) + 1) * P + q] += g_ay;
+    gpar[((size_t)t * (2 + C) + 0) * P + p] += g_ax;
+    gpar[((size_t)t * (2 + C) + 1) * P + p] += g_ay;
 }
 
 // --------------------------------------------------------------- sweeps
@@ -549,7 +550,7 @@ template <int D, int MAXW>
 __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, const double *xin,
                                                           const double *xout, const double *lam_out,
                                                           double *gK, double *gaxy, double *scratch, GSkew k,
-                                                          double ax, double ay)
+                                                          double ax, double ay, int gK_set)
 {
     __shared__ double xb[2][MAXW][2];
     const int t = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = r >> 6, nw = (int)(blockDim.x >> 6);
@@ -585,8 +586,11 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
     };
     auto issue_g = [&](double *g, int d) {
         const unsigned q = slot_offset(row_ok, d, r, w, hs);
+        // gK_set: the sweep stores its coefficient adjoints instead of
+        // adding them (out-of-range loads read 0, without a memory access)
+        const unsigned qk = gK_set ? kOutOfRange : q;
 #pragma unroll
-        for (int e = 0; e < 6; ++e) g[e] = buf_load_f64(rG, q, e * pb);
+        for (int e = 0; e < 6; ++e) g[e] = buf_load_f64(rG, qk, e * pb);
         g[6] = buf_load_f64(rX, q, 0);
         g[7] = buf_load_f64(rX, q, pb);
     };
@@ -895,7 +899,7 @@ int frh_grad_coef(const double *x, const double *u0, const double *J, const doub
                "frh_grad_coef: bad arguments (at most 2 channels and 1024 rows)");
     FRH_GCHECK(!g_coef || (g_x && g_J && g_par), "frh_grad_coef: g_x, g_J and g_par are required with g_coef");
     hipStream_t st = (hipStream_t)stream;
-    dim3 g((unsigned)((k.plane + 63) / 64), T), b(64);
+    dim3 g((unsigned)(((size_t)h * w + 63) / 64), T), b(64);
     if (coef) {
         hipLaunchKernelGGL(k_gcoef_fwd, g, b, 0, st, x, u0, J, W, T, C, k, ax, ay, a_data, psi_one, coef);
         FRH_GLAUNCH("coefficient forward");
@@ -927,7 +931,7 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
 }
 
 int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double *x_out,
-                           const double *lam_out, double *lam_in, double *g_coef, double *g_axy,
+                           const double *lam_out, double *lam_in, double *g_coef, int g_coef_set, double *g_axy,
                            double *scratch, int T, int h, int w, double ax, double ay, void *stream)
 {
     GSkew k;
@@ -939,13 +943,13 @@ int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double 
     const dim3 b(k.hs);
     if (k.hs <= 256)
         hipLaunchKernelGGL((k_gsweep_bwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           scratch, k, ax, ay);
+                           scratch, k, ax, ay, g_coef_set ? 1 : 0);
     else if (k.hs <= 512)
         hipLaunchKernelGGL((k_gsweep_bwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           scratch, k, ax, ay);
+                           scratch, k, ax, ay, g_coef_set ? 1 : 0);
     else
         hipLaunchKernelGGL((k_gsweep_bwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef,
-                           g_axy, scratch, k, ax, ay);
+                           g_axy, scratch, k, ax, ay, g_coef_set ? 1 : 0);
     FRH_GLAUNCH("sweep adjoint");
     hipLaunchKernelGGL(k_gsweep_lam_in, dim3((unsigned)((k.plane + 63) / 64), T), dim3(64), 0, st, coef, scratch,
                        lam_in, k, ax, ay);

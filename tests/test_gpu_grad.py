@@ -181,9 +181,17 @@ def test_sweep_forward_and_adjoint(gpu):
     st = _lib.stream_handle()
     _lib.check(L.frh_grad_sweep(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), T, h, w, 2.0, 1.5, st), "sweep")
     _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
-                                        lam_in.data_ptr(), g_coef.data_ptr(), g_axy.data_ptr(),
+                                        lam_in.data_ptr(), g_coef.data_ptr(), 0, g_axy.data_ptr(),
                                         scratch.data_ptr(), T, h, w, 2.0, 1.5, st), "adjoint")
+    # g_coef_set: the coefficient adjoints are stored, whatever the buffer held
+    g_set = torch.full_like(cd, 7.0)
+    g_axy2 = torch.zeros_like(g_axy)
+    _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
+                                        lam_in.data_ptr(), g_set.data_ptr(), 1, g_axy2.data_ptr(),
+                                        scratch.data_ptr(), T, h, w, 2.0, 1.5, st), "adjoint (set)")
     torch.cuda.synchronize()
+    assert torch.equal(_unsk(g_set, h, w), _unsk(g_coef, h, w))
+    assert torch.equal(g_axy2, g_axy)
     assert _rel(_unsk(out, h, w), y) < 1e-12
     assert _rel(_unsk(lam_in, h, w), x.grad) < 1e-11
     assert _rel(_unsk(g_coef, h, w), coef_t.grad) < 1e-11
@@ -231,7 +239,7 @@ def test_sweep_adjoint_several_waves(gpu):
         st = _lib.stream_handle()
         _lib.check(L.frh_grad_sweep(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), 1, h, w, 1.1, 0.9, st), "sweep")
         _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
-                                            lam_in.data_ptr(), g_coef.data_ptr(), g_axy.data_ptr(),
+                                            lam_in.data_ptr(), g_coef.data_ptr(), 1, g_axy.data_ptr(),
                                             scratch.data_ptr(), 1, h, w, 1.1, 0.9, st), "adjoint")
         torch.cuda.synchronize()
         assert _rel(_unsk(out, h, w), y) < 1e-12, (h, w)
