@@ -400,12 +400,11 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
 /* Exact adjoint of frh_grad_sweep (all skewed): lam_in = (dx_out/dx_in)^T
  * lam_out; g_coef += (dx_out/dcoef)^T lam_out (g_coef = ... when
  * g_coef_set != 0: the first adjoint sweep of a lag block needs no cleared
- * buffer); g_axy (T, 2, plane) += per-cell partials of ax, ay.  scratch:
- * 4 T plane doubles. */
+ * buffer); g_axy (T, 2, plane) += partials of ax, ay (one per row r, at
+ * element r of each plane). */
 int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double *x_out,
                            const double *lam_out, double *lam_in, double *g_coef, int g_coef_set, double *g_axy,
-                           double *scratch, int T, int h, int w, double ax, double ay,
-                           void *stream);
+                           int T, int h, int w, double ax, double ay, void *stream);
 
 /* 5x5 mirror median of N (h, w) planes: out (when != NULL); g_x += the
  * output gradient g_out routed to the selected samples (when g_x != NULL). */

@@ -105,7 +105,7 @@ SIGNATURES = {
                       + [c_void_p] * 7),
     "frh_grad_sweep": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
                                c_void_p]),
-    "frh_grad_sweep_adjoint": (c_int, [c_void_p] * 6 + [c_int] + [c_void_p] * 2
+    "frh_grad_sweep_adjoint": (c_int, [c_void_p] * 6 + [c_int, c_void_p]
                                + [c_int, c_int, c_int, c_double, c_double, c_void_p]),
     "frh_grad_median": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_grad_warp": (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, c_int, c_int,

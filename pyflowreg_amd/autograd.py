@@ -212,7 +212,6 @@ class LevelSolve(torch.autograd.Function):
         g_par = torch.zeros((T, 2 + C, P), dtype=_F64, device=dev)
         g_axy = torch.zeros((T, 2, P), dtype=_F64, device=dev)
         g_coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
-        scratch = torch.empty((T, 4, P), dtype=_F64, device=dev)
         blocks = lag_blocks(iterations, update_lag)
         nmax = max(e - s for s, e, _ in blocks)
         # Lag blocks run backwards.  Recomputing a block's sweeps from its
@@ -259,7 +258,7 @@ class LevelSolve(torch.autograd.Function):
             for k in range(e - s - 1, -1, -1):
                 check(L.frh_grad_sweep_adjoint(ptr(coefs[slot]), ptr(x_in(b, slot, k)), ptr(bufs[slot][k]),
                                                ptr(lam), ptr(lam_in), ptr(g_coef), int(k == e - s - 1),
-                                               ptr(g_axy), ptr(scratch), T, h, w, axf, ayf, st),
+                                               ptr(g_axy), T, h, w, axf, ayf, st),
                       "frh_grad_sweep_adjoint")
                 lam, lam_in = lam_in, lam
             # the block's coefficients depend on its start state, J, u0 and

@@ -528,14 +528,17 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
 }
 
 // Reverse-mode derivative of k_gsweep_fwd.  lam_out: adjoint of xout.
-// Writes, per cell, the adjoints of the new values (adu, adv) and of the
-// update numerators (anu = wu adu, anv = wv adv) into scratch (T, 4, plane),
-// adds the coefficient adjoints to gK and the per-cell partials of ax, ay to
-// gaxy (T, 2, plane).  Diagonals run from last to first: a cell's new values
-// are read only by its right and lower neighbours (diagonal d + 1) and by the
-// next sweep (lam_out), so the right neighbour's adjoint is the lane's own
-// previous result and the lower one lane r + 1's.  k_gsweep_lam_in then forms
-// the adjoint of xin from scratch.
+// Adds the coefficient adjoints to gK (or stores them, gK_set) and the
+// partials of ax, ay to gaxy (T, 2, plane; one sum per row, element r of
+// each plane), and writes lam_in, the adjoint of xin.  Diagonals run from
+// last to first: a cell's new values are read only by its right and lower
+// neighbours (diagonal d + 1) and by the next sweep (lam_out), so the right
+// neighbour's adjoint is the lane's own previous result and the lower one
+// lane r + 1's.  A cell's old values are read by its own update and by the
+// updates of its left and upper neighbours (diagonal d - 1, the next step),
+// so its lam_in is complete once the lane and lane r - 1 have passed the
+// next diagonal: it is formed two steps later (the upper neighbour's
+// adjoint of the wave's lane 0 comes through LDS after a barrier).
 // Ring slot of one diagonal (diagonals run last to first).  The new values
 // of the left and upper neighbours are the lane's own / lane r - 1's slot
 // of the next diagonal processed (d - 1), the old values of the right and
@@ -546,13 +549,20 @@ struct GBwdSlot {
     double K[6], lu_out, lv_out, un, vn, uu, uv, ou, ov, du, dv;
 };
 
+// What a cell's lam_in needs of its own step: the adjoints of its update
+// (anu, anv, adu, adv) and its coefficients wu, wv, b, A12.
+struct GBwdHist {
+    double anu, anv, adu, adv, wu, wv, b, A12;
+};
+
 template <int D, int MAXW>
 __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, const double *xin,
                                                           const double *xout, const double *lam_out,
-                                                          double *gK, double *gaxy, double *scratch, GSkew k,
+                                                          double *gK, double *gaxy, double *lam_in, GSkew k,
                                                           double ax, double ay, int gK_set)
 {
-    __shared__ double xb[2][MAXW][2];
+    __shared__ double xb[2][MAXW][2];  // lane 0's anu, anv, for the wave above's lane 63
+    __shared__ double xu[2][MAXW][2];  // lane 63's anu, anv, for the wave below's lane 0
     const int t = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = r >> 6, nw = (int)(blockDim.x >> 6);
     const int h = k.h, w = k.w, hs = k.hs;
     const bool row_ok = r < h, down_ok = r + 1 < h, up_ok = r >= 1 && r - 1 < h;
@@ -562,11 +572,12 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
     const auto rO = make_rsrc(xout + (size_t)t * 2 * k.plane, 2 * pb);
     const auto rL = make_rsrc(lam_out + (size_t)t * 2 * k.plane, 2 * pb);
     const auto rG = make_rsrc(gK + (size_t)t * 6 * k.plane, 6 * pb);
-    const auto rX = make_rsrc(gaxy + (size_t)t * 2 * k.plane, 2 * pb);
-    const auto rS = make_rsrc(scratch + (size_t)t * 4 * k.plane, 4 * pb);
+    const auto rN = make_rsrc(lam_in + (size_t)t * 2 * k.plane, 2 * pb);
     static_assert(D >= 2 && D % 2 == 0, "the gK ring alternates with the step parity");
     GBwdSlot S[D];
-    double Gc[2][8];  // gK (6) and gaxy (2) of the next step's cell, loaded one step ahead
+    double Gc[2][6];  // gK of the next step's cell, loaded one step ahead
+    GBwdHist Hs[2];   // the cells of the last two steps, by step parity
+    Hs[0] = Hs[1] = GBwdHist{0, 0, 0, 0, 0, 0, 0, 0};
     auto issue = [&](GBwdSlot &s, int d) {
         const unsigned q = slot_offset(row_ok, d, r, w, hs);
 #pragma unroll
@@ -585,26 +596,46 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
         s.dv = buf_load_f64(rI, qd, pb);
     };
     auto issue_g = [&](double *g, int d) {
-        const unsigned q = slot_offset(row_ok, d, r, w, hs);
         // gK_set: the sweep stores its coefficient adjoints instead of
         // adding them (out-of-range loads read 0, without a memory access)
-        const unsigned qk = gK_set ? kOutOfRange : q;
+        const unsigned q = gK_set ? kOutOfRange : slot_offset(row_ok, d, r, w, hs);
 #pragma unroll
-        for (int e = 0; e < 6; ++e) g[e] = buf_load_f64(rG, qk, e * pb);
-        g[6] = buf_load_f64(rX, q, 0);
-        g[7] = buf_load_f64(rX, q, pb);
+        for (int e = 0; e < 6; ++e) g[e] = buf_load_f64(rG, q, e * pb);
     };
     const int nd = h + w - 1;
-    // step i handles diagonal nd - 1 - i
+    // step i handles diagonal nd - 1 - i; steps nd and nd + 1 only finish
+    // the lam_in of the last two diagonals
 #pragma unroll
     for (int s = 0; s < D - 1; ++s) issue(S[s], nd - 1 - s);
     issue_g(Gc[0], nd - 1);
     double pnu = 0.0, pnv = 0.0;  // this lane's anu, anv of the previous step
     double pou = 0.0, pov = 0.0;  // this lane's old values of the previous step's diagonal (d + 1)
-    for (int i0 = 0; i0 < nd; i0 += D) {
+    double g_axr = 0.0, g_ayr = 0.0;  // this row's partials of ax, ay
+    for (int i0 = 0; i0 < nd + 2; i0 += D) {
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             const int i = i0 + s, d = nd - 1 - i;
+            // lam_in of the cell of step i - 2 (diagonal d + 2, column c2):
+            // its left neighbour's anu is this lane's of step i - 1 (pnu),
+            // its upper neighbour's lane r - 1's of step i - 1
+            {
+                const GBwdHist &H = Hs[s & 1];  // step i - 2 (D is even)
+                double unu = wave_from_prev_lane(pnu), unv = wave_from_prev_lane(pnv);
+                const int wu_src = wv > 0 ? wv - 1 : 0;
+                const double eu = xu[(i + 1) & 1][wu_src][0], ev = xu[(i + 1) & 1][wu_src][1];
+                if (lane == 0) {
+                    unu = wv > 0 ? eu : 0.0;
+                    unv = wv > 0 ? ev : 0.0;
+                }
+                const int c2 = d + 2 - r;
+                const unsigned q2 = slot_offset(row_ok, d + 2, r, w, hs);
+                const double Lu = (1.0 - kOmega + H.wu * H.b) * H.adu + (c2 > 0 ? ax * pnu : 0.0) +
+                                  (r > 0 ? ay * unu : 0.0);
+                const double Lv = (1.0 - kOmega + H.wv * H.b) * H.adv + (c2 > 0 ? ax * pnv : 0.0) +
+                                  (r > 0 ? ay * unv : 0.0) - H.A12 * H.anu;
+                buf_store_f64(Lu, rN, q2, 0);
+                buf_store_f64(Lv, rN, q2, pb);
+            }
             issue_g(Gc[(s + 1) & 1], d - 1);  // D is even: the parity of s is the parity of i
             issue(S[(s + D - 1) % D], d - (D - 1));
             const GBwdSlot &c = S[s];
@@ -644,43 +675,27 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
             buf_store_f64(g[3] + (nu + b * c.ou) * a_du, rG, q, 3 * pb);        // wu
             buf_store_f64(g[4] + (nv + b * c.ov) * a_dv, rG, q, 4 * pb);        // wv
             buf_store_f64(g[5] + wu * c.ou * a_du + wvv * c.ov * a_dv, rG, q, 5 * pb);  // b
-            buf_store_f64(g[6] + (lu + ru) * a_nu + (lv + rv) * a_nv, rX, q, 0);
-            buf_store_f64(g[7] + (cdu + cuu) * a_nu + (cdv + cuv) * a_nv, rX, q, pb);
-            buf_store_f64(a_nu, rS, q, 0);
-            buf_store_f64(a_nv, rS, q, pb);
-            buf_store_f64(a_du, rS, q, 2 * pb);
-            buf_store_f64(a_dv, rS, q, 3 * pb);
+            g_axr += (lu + ru) * a_nu + (lv + rv) * a_nv;
+            g_ayr += (cdu + cuu) * a_nu + (cdv + cuv) * a_nv;
+            Hs[s & 1] = GBwdHist{a_nu, a_nv, a_du, a_dv, wu, wvv, b, A12};  // read again at step i + 2
             pnu = a_nu;
             pnv = a_nv;
             if (lane == 0) {
                 xb[i & 1][wv][0] = a_nu;
                 xb[i & 1][wv][1] = a_nv;
             }
+            if (lane == 63) {
+                xu[i & 1][wv][0] = a_nu;
+                xu[i & 1][wv][1] = a_nv;
+            }
             lds_barrier();
         }
     }
-}
-
-// Adjoint of the sweep's input from the scratch of k_gsweep_bwd: each old
-// value is read by its own update and, as a right / lower neighbour, by the
-// updates of the cells to its left and above.
-__global__ __launch_bounds__(64) void k_gsweep_lam_in(const double *coef, const double *scratch, double *lam_in,
-                                                       GSkew k, double ax, double ay)
-{
-    const size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int t = blockIdx.y;
-    if (q >= k.plane) return;
-    int r, c;
-    if (!skew_cell(k, q, r, c)) return;
-    const size_t P = k.plane;
-    const double *K = coef + (size_t)t * 6 * P, *S = scratch + (size_t)t * 4 * P;
-    const double *Anu = S, *Anv = S + P, *Adu = S + 2 * P, *Adv = S + 3 * P;
-    const double A12 = K[q], wu = K[3 * P + q], wv = K[4 * P + q], b = K[5 * P + q];
-    const size_t ql = q - k.hs, qu = q - k.hs - 1;  // (r, c - 1), (r - 1, c)
-    double *L = lam_in + (size_t)t * 2 * P;
-    L[q] = (1.0 - kOmega + wu * b) * Adu[q] + (c > 0 ? ax * Anu[ql] : 0.0) + (r > 0 ? ay * Anu[qu] : 0.0);
-    L[P + q] = (1.0 - kOmega + wv * b) * Adv[q] + (c > 0 ? ax * Anv[ql] : 0.0) + (r > 0 ? ay * Anv[qu] : 0.0) -
-               A12 * Anu[q];
+    if (row_ok) {
+        double *G = gaxy + (size_t)t * 2 * k.plane;
+        G[r] += g_axr;
+        G[k.plane + r] += g_ayr;
+    }
 }
 
 // --------------------------------------------------------------- median
@@ -932,28 +947,26 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
 
 int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double *x_out,
                            const double *lam_out, double *lam_in, double *g_coef, int g_coef_set, double *g_axy,
-                           double *scratch, int T, int h, int w, double ax, double ay, void *stream)
+                           int T, int h, int w, double ax, double ay, void *stream)
 {
     GSkew k;
-    FRH_GCHECK(coef && x_in && x_out && lam_out && lam_in && g_coef && g_axy && scratch && T > 0 &&
-                   T <= 65535 && make_gskew(h, w, k),
+    FRH_GCHECK(coef && x_in && x_out && lam_out && lam_in && g_coef && g_axy && T > 0 && T <= 65535 &&
+                   make_gskew(h, w, k),
                "frh_grad_sweep_adjoint: bad arguments (at most 1024 rows)");
     FRH_GCHECK(lam_in != lam_out, "frh_grad_sweep_adjoint: lam_in must not alias lam_out");
     hipStream_t st = (hipStream_t)stream;
     const dim3 b(k.hs);
+    const int set = g_coef_set ? 1 : 0;
     if (k.hs <= 256)
         hipLaunchKernelGGL((k_gsweep_bwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           scratch, k, ax, ay, g_coef_set ? 1 : 0);
+                           lam_in, k, ax, ay, set);
     else if (k.hs <= 512)
         hipLaunchKernelGGL((k_gsweep_bwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           scratch, k, ax, ay, g_coef_set ? 1 : 0);
+                           lam_in, k, ax, ay, set);
     else
-        hipLaunchKernelGGL((k_gsweep_bwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef,
-                           g_axy, scratch, k, ax, ay, g_coef_set ? 1 : 0);
+        hipLaunchKernelGGL((k_gsweep_bwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+                           lam_in, k, ax, ay, set);
     FRH_GLAUNCH("sweep adjoint");
-    hipLaunchKernelGGL(k_gsweep_lam_in, dim3((unsigned)((k.plane + 63) / 64), T), dim3(64), 0, st, coef, scratch,
-                       lam_in, k, ax, ay);
-    FRH_GLAUNCH("sweep adjoint input");
     return FRH_OK;
 }
 

@@ -176,19 +176,18 @@ def test_sweep_forward_and_adjoint(gpu):
     lam_in = torch.empty_like(xd)
     g_coef = torch.zeros_like(cd)
     g_axy = torch.zeros((T, 2, P), dtype=F64, device="cuda")
-    scratch = torch.empty((T, 4, P), dtype=F64, device="cuda")
     L = _lib.lib()
     st = _lib.stream_handle()
     _lib.check(L.frh_grad_sweep(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), T, h, w, 2.0, 1.5, st), "sweep")
     _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
                                         lam_in.data_ptr(), g_coef.data_ptr(), 0, g_axy.data_ptr(),
-                                        scratch.data_ptr(), T, h, w, 2.0, 1.5, st), "adjoint")
+                                        T, h, w, 2.0, 1.5, st), "adjoint")
     # g_coef_set: the coefficient adjoints are stored, whatever the buffer held
     g_set = torch.full_like(cd, 7.0)
     g_axy2 = torch.zeros_like(g_axy)
     _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
                                         lam_in.data_ptr(), g_set.data_ptr(), 1, g_axy2.data_ptr(),
-                                        scratch.data_ptr(), T, h, w, 2.0, 1.5, st), "adjoint (set)")
+                                        T, h, w, 2.0, 1.5, st), "adjoint (set)")
     torch.cuda.synchronize()
     assert torch.equal(_unsk(g_set, h, w), _unsk(g_coef, h, w))
     assert torch.equal(g_axy2, g_axy)
@@ -235,12 +234,11 @@ def test_sweep_adjoint_several_waves(gpu):
         out, lam_in = torch.empty_like(xd), torch.empty_like(xd)
         g_coef = torch.zeros_like(cd)
         g_axy = torch.zeros((1, 2, P), dtype=F64, device="cuda")
-        scratch = torch.empty((1, 4, P), dtype=F64, device="cuda")
         st = _lib.stream_handle()
         _lib.check(L.frh_grad_sweep(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), 1, h, w, 1.1, 0.9, st), "sweep")
         _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), lamd.data_ptr(),
                                             lam_in.data_ptr(), g_coef.data_ptr(), 1, g_axy.data_ptr(),
-                                            scratch.data_ptr(), 1, h, w, 1.1, 0.9, st), "adjoint")
+                                            1, h, w, 1.1, 0.9, st), "adjoint")
         torch.cuda.synchronize()
         assert _rel(_unsk(out, h, w), y) < 1e-12, (h, w)
         assert _rel(_unsk(lam_in, h, w), x.grad) < 1e-11, (h, w)
