@@ -397,6 +397,15 @@ int frh_grad_coef(const double *x, const double *u0, const double *J, const doub
 int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T, int h, int w,
                    double ax, double ay, void *stream);
 
+/* n_sweeps consecutive frh_grad_sweep calls with one coefficient set (the
+ * sweeps of a lag block), pipelined in one wavefront pass per at most 5
+ * (levels of up to 256 rows) or 4 (512) sweeps (single sweeps above 512
+ * rows); the results equal the single sweeps'.  write_all != 0: the state
+ * after sweep j goes to x_out + j out_stride (values; x_in must not overlap
+ * them); else only the last state, to x_out (which may equal x_in). */
+int frh_grad_sweeps(const double *coef, const double *x_in, double *x_out, size_t out_stride, int n_sweeps,
+                    int write_all, int T, int h, int w, double ax, double ay, void *stream);
+
 /* Exact adjoint of frh_grad_sweep (all skewed): lam_in = (dx_out/dx_in)^T
  * lam_out; g_coef += (dx_out/dcoef)^T lam_out (g_coef = ... when
  * g_coef_set != 0: the first adjoint sweep of a lag block needs no cleared

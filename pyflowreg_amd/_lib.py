@@ -105,6 +105,8 @@ SIGNATURES = {
                       + [c_void_p] * 7),
     "frh_grad_sweep": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
                                c_void_p]),
+    "frh_grad_sweeps": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_int, c_int, c_int, c_int,
+                                c_int, c_double, c_double, c_void_p]),
     "frh_grad_sweep_adjoint": (c_int, [c_void_p] * 6 + [c_int, c_void_p]
                                + [c_int, c_int, c_int, c_double, c_double, c_void_p]),
     "frh_grad_median": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),

@@ -177,10 +177,10 @@ class LevelSolve(torch.autograd.Function):
             check(L.frh_grad_coef(ptr(x), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
                                   int(psi_one), ptr(coef), None, None, None, None, None, st),
                   "frh_grad_coef")
-            for _ in range(s, e):
-                check(L.frh_grad_sweep(ptr(coef), ptr(x), ptr(x_next), T, h, w, axf, ayf, st),
-                      "frh_grad_sweep")
-                x, x_next = x_next, x
+            # the block's sweeps share the coefficients: pipelined launches
+            check(L.frh_grad_sweeps(ptr(coef), ptr(x), ptr(x_next), 0, e - s, 0, T, h, w, axf, ayf, st),
+                  "frh_grad_sweeps")
+            x, x_next = x_next, x
         du = from_skew(x, h, w)
         if median:
             dm = torch.empty_like(du)
@@ -234,9 +234,8 @@ class LevelSolve(torch.autograd.Function):
             check(L.frh_grad_coef(ptr(ckpt[b]), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
                                   int(psi_one), ptr(coefs[slot]), None, None, None, None, None, hs),
                   "frh_grad_coef")
-            for k in range(e - s):
-                check(L.frh_grad_sweep(ptr(coefs[slot]), ptr(x_in(b, slot, k)), ptr(bufs[slot][k]), T, h, w,
-                                       axf, ayf, hs), "frh_grad_sweep")
+            check(L.frh_grad_sweeps(ptr(coefs[slot]), ptr(ckpt[b]), ptr(bufs[slot]), bufs[slot][0].numel(), e - s,
+                                    1, T, h, w, axf, ayf, hs), "frh_grad_sweeps")
             ev = torch.cuda.Event()
             ev.record(side)
             return ev

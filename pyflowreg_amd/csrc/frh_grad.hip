@@ -15,6 +15,7 @@
 
 #include <cmath>
 #include <type_traits>
+#include <utility>
 
 #include "../../include/flowreg_hip.h"
 #include "frh_internal.h"
@@ -527,6 +528,180 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
     }
 }
 
+// P consecutive sweeps of k_gsweep_fwd in one wavefront pass (the sweeps
+// of a lag block share their coefficients): at step t lane r updates cell
+// (r, t - r - 2p) in sweep p.  Sweep p reads sweep p - 1's values of the cell
+// (this lane, step t - 2), of its right neighbour (this lane, step t - 1) and
+// of its lower neighbour (lane r + 1, step t - 1), and sweep p's values of its
+// left (this lane) and upper (lane r - 1) neighbours from step t - 1; sweep 0
+// reads the pass input.  Each update is k_gsweep_fwd's arithmetic, so every
+// sweep's values equal those of P single-sweep launches.  Coefficients of
+// diagonal x sit in ring slot x % NK and the input values of diagonal x in
+// slot x % NX, loaded A steps ahead; the step loop is unrolled over NK
+// steps (a multiple of NX) so that every slot index is static.
+template <int P>
+struct GMulti {
+    static constexpr int A = 3;                                  // load distance
+    static constexpr int NX = A + 1;                             // input ring: diagonals t .. t + A
+    static constexpr int NK = (2 * P + A - 1 + NX - 1) / NX * NX;  // coefficients: t - 2(P - 1) .. t + A
+    double K[NK][6];
+    double X[NX][4];      // ou, ov, and (lane 63) du, dv of the lower neighbour
+    double R1[P][2], R2[P][2];  // sweep p's values of steps t - 1 and t - 2
+};
+
+template <int P, int MAXW, bool ALL, int S>
+__device__ __forceinline__ void gmulti_step(GMulti<P> &st, int t, int r, int lane, int wv, int nw, const GSkew &k,
+                                            double ax, double ay, __amdgpu_buffer_rsrc_t rK,
+                                            __amdgpu_buffer_rsrc_t rI, const __amdgpu_buffer_rsrc_t *rO, unsigned pb,
+                                            double (*xu)[MAXW][P][2], double (*xd)[MAXW][P][2])
+{
+    using M = GMulti<P>;
+    constexpr int NK = M::NK, NX = M::NX, A = M::A;
+    const int h = k.h, w = k.w, hs = k.hs;
+    const bool row_ok = r < h, down_ok = r + 1 < h;
+    {
+        // diagonal t + A into the rings
+        const int d = t + A;
+        const unsigned q = slot_offset(row_ok, d, r, w, hs);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) st.K[(S + A) % NK][e] = buf_load_f64(rK, q, e * pb);
+        double *x = st.X[(S + A) % NX];
+        x[0] = buf_load_f64(rI, q, 0);
+        x[1] = buf_load_f64(rI, q, pb);
+        const unsigned qd = lane == 63 ? slot_offset(down_ok, d + 1, r + 1, w, hs) : kOutOfRange;
+        x[2] = buf_load_f64(rI, qd, 0);
+        x[3] = buf_load_f64(rI, qd, pb);
+    }
+    const double *x0 = st.X[S % NX], *x1 = st.X[(S + 1) % NX];  // input of diagonals t, t + 1
+    const int pr = (t + 1) & 1;  // LDS half written at step t - 1
+    const int wu_src = wv > 0 ? wv - 1 : 0, wd_src = wv + 1 < nw ? wv + 1 : wv;
+    double nw_u[P], nw_v[P];
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        const double *Kp = st.K[((S - 2 * p) % NK + NK) % NK];
+        double ou, ov, ru, rv, cdu, cdv;
+        if (p == 0) {
+            ou = x0[0];
+            ov = x0[1];
+            ru = x1[0];
+            rv = x1[1];
+            const double du_n = wave_from_next_lane(x1[0]), dv_n = wave_from_next_lane(x1[1]);
+            cdu = lane == 63 ? x0[2] : du_n;
+            cdv = lane == 63 ? x0[3] : dv_n;
+        } else {
+            ou = st.R2[p > 0 ? p - 1 : 0][0];
+            ov = st.R2[p > 0 ? p - 1 : 0][1];
+            ru = st.R1[p > 0 ? p - 1 : 0][0];
+            rv = st.R1[p > 0 ? p - 1 : 0][1];
+            cdu = wave_from_next_lane(ru);
+            cdv = wave_from_next_lane(rv);
+            if (lane == 63) {
+                cdu = wv + 1 < nw ? xd[pr][wd_src][p - 1][0] : 0.0;
+                cdv = wv + 1 < nw ? xd[pr][wd_src][p - 1][1] : 0.0;
+            }
+        }
+        const double pu = st.R1[p][0], pv = st.R1[p][1];  // (r, c - 1), new
+        double uu = wave_from_prev_lane(pu), uv = wave_from_prev_lane(pv);
+        if (lane == 0) {
+            uu = wv > 0 ? xu[pr][wu_src][p][0] : 0.0;
+            uv = wv > 0 ? xu[pr][wu_src][p][1] : 0.0;
+        }
+        const int d = t - 2 * p;
+        const unsigned q = slot_offset(row_ok, d, r, w, hs);
+        const double A12 = Kp[0], Bu = Kp[1], Bv = Kp[2], wu = Kp[3], wvv = Kp[4], b = Kp[5];
+        const double nu = Bu + ax * (pu + ru) + ay * (cdu + uu) - A12 * ov;
+        double un = wu * nu + (1.0 - kOmega + wu * b) * ou;
+        const double nv = Bv + ax * (pv + rv) + ay * (cdv + uv) - A12 * un;
+        double vn = wvv * nv + (1.0 - kOmega + wvv * b) * ov;
+        const bool ok = q != kOutOfRange;
+        un = ok ? un : 0.0;
+        vn = ok ? vn : 0.0;
+        if (ALL || p == P - 1) {
+            buf_store_f64(un, rO[ALL ? p : 0], q, 0);
+            buf_store_f64(vn, rO[ALL ? p : 0], q, pb);
+        }
+        nw_u[p] = un;
+        nw_v[p] = vn;
+    }
+#pragma unroll
+    for (int p = 0; p < P; ++p) {
+        st.R2[p][0] = st.R1[p][0];
+        st.R2[p][1] = st.R1[p][1];
+        st.R1[p][0] = nw_u[p];
+        st.R1[p][1] = nw_v[p];
+    }
+    const int pw = t & 1;
+    if (lane == 63) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            xu[pw][wv][p][0] = nw_u[p];
+            xu[pw][wv][p][1] = nw_v[p];
+        }
+    }
+    if (lane == 0) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            xd[pw][wv][p][0] = nw_u[p];
+            xd[pw][wv][p][1] = nw_v[p];
+        }
+    }
+    lds_barrier();
+}
+
+template <int P, int MAXW, bool ALL, int... S>
+__device__ __forceinline__ void gmulti_block(GMulti<P> &st, int t0, int r, int lane, int wv, int nw, const GSkew &k,
+                                             double ax, double ay, __amdgpu_buffer_rsrc_t rK,
+                                             __amdgpu_buffer_rsrc_t rI, const __amdgpu_buffer_rsrc_t *rO, unsigned pb,
+                                             double (*xu)[MAXW][P][2], double (*xd)[MAXW][P][2],
+                                             std::integer_sequence<int, S...>)
+{
+    (gmulti_step<P, MAXW, ALL, S>(st, t0 + S, r, lane, wv, nw, k, ax, ay, rK, rI, rO, pb, xu, xd), ...);
+}
+
+// ALL: sweep p's values go to xout + p ostride (frame t at t * 2 plane
+// within each); else only the last sweep's, to xout.
+template <int P, int MAXW, bool ALL>
+__global__ __launch_bounds__(64 * MAXW) void k_gsweeps_fwd(const double *coef, const double *xin, double *xout,
+                                                           size_t ostride, GSkew k, double ax, double ay)
+{
+    __shared__ double xu[2][MAXW][P][2];  // lane 63's new values, for the wave below's lane 0
+    __shared__ double xd[2][MAXW][P][2];  // lane 0's new values, for the wave above's lane 63
+    using M = GMulti<P>;
+    const int f = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = __builtin_amdgcn_readfirstlane(r >> 6);
+    const int nw = (int)(blockDim.x >> 6);
+    const unsigned pb = (unsigned)(k.plane * 8);
+    const auto rK = make_rsrc(coef + (size_t)f * 6 * k.plane, 6 * pb);
+    const auto rI = make_rsrc(xin + (size_t)f * 2 * k.plane, 2 * pb);
+    __amdgpu_buffer_rsrc_t rO[ALL ? P : 1];
+#pragma unroll
+    for (int p = 0; p < (ALL ? P : 1); ++p) rO[p] = make_rsrc(xout + p * ostride + (size_t)f * 2 * k.plane, 2 * pb);
+    for (int i = threadIdx.x; i < 2 * MAXW * P * 2; i += blockDim.x) {
+        (&xu[0][0][0][0])[i] = 0.0;
+        (&xd[0][0][0][0])[i] = 0.0;
+    }
+    M st;
+#pragma unroll
+    for (int p = 0; p < P; ++p) st.R1[p][0] = st.R1[p][1] = st.R2[p][0] = st.R2[p][1] = 0.0;
+    const bool row_ok = r < k.h, down_ok = r + 1 < k.h;
+    // prologue: diagonals 0 .. A - 1
+#pragma unroll
+    for (int d = 0; d < M::A; ++d) {
+        const unsigned q = slot_offset(row_ok, d, r, k.w, k.hs);
+#pragma unroll
+        for (int e = 0; e < 6; ++e) st.K[d][e] = buf_load_f64(rK, q, e * pb);
+        st.X[d][0] = buf_load_f64(rI, q, 0);
+        st.X[d][1] = buf_load_f64(rI, q, pb);
+        const unsigned qd = lane == 63 ? slot_offset(down_ok, d + 1, r + 1, k.w, k.hs) : kOutOfRange;
+        st.X[d][2] = buf_load_f64(rI, qd, 0);
+        st.X[d][3] = buf_load_f64(rI, qd, pb);
+    }
+    __syncthreads();
+    const int Tend = (k.h + k.w - 2) + 2 * (P - 1);
+    for (int t0 = 0; t0 <= Tend; t0 += M::NK)
+        gmulti_block<P, MAXW, ALL>(st, t0, r, lane, wv, nw, k, ax, ay, rK, rI, rO, pb, xu, xd,
+                              std::make_integer_sequence<int, M::NK>{});
+}
+
 // Reverse-mode derivative of k_gsweep_fwd.  lam_out: adjoint of xout.
 // Adds the coefficient adjoints to gK (or stores them, gK_set) and the
 // partials of ax, ay to gaxy (T, 2, plane; one sum per row, element r of
@@ -840,6 +1015,45 @@ __global__ __launch_bounds__(64) void k_gwarp(const double *img, const double *f
 using namespace frh;
 using namespace frh::grad;
 
+// Sweeps per pipelined launch; a lag block of more sweeps takes several.
+// At 512 lanes (two waves per SIMD, 256 registers) 5 sweeps spill
+// registers and run slower than 3 + 2.
+__host__ __device__ constexpr int gmax_pipe(int maxw) { return maxw <= 4 ? 5 : 4; }
+
+template <int MAXW, bool ALL>
+static void launch_gsweeps(int P, const double *coef, const double *x_in, double *x_out, size_t ostride,
+                           int T, const GSkew &k, double ax, double ay, hipStream_t st)
+{
+    const dim3 g(T), b(k.hs);
+    switch (P) {
+    case 2:
+        hipLaunchKernelGGL((k_gsweeps_fwd<2, MAXW, ALL>), g, b, 0, st, coef, x_in, x_out, ostride, k, ax, ay);
+        break;
+    case 3:
+        hipLaunchKernelGGL((k_gsweeps_fwd<3, MAXW, ALL>), g, b, 0, st, coef, x_in, x_out, ostride, k, ax, ay);
+        break;
+    case 4:
+        hipLaunchKernelGGL((k_gsweeps_fwd<4, MAXW, ALL>), g, b, 0, st, coef, x_in, x_out, ostride, k, ax, ay);
+        break;
+    default:
+        if constexpr (gmax_pipe(MAXW) >= 5)
+            hipLaunchKernelGGL((k_gsweeps_fwd<5, MAXW, ALL>), g, b, 0, st, coef, x_in, x_out, ostride, k, ax, ay);
+        break;
+    }
+}
+
+static void launch_gsweep1(const double *coef, const double *x_in, double *x_out, int T, const GSkew &k,
+                           double ax, double ay, hipStream_t st)
+{
+    const dim3 b(k.hs);
+    if (k.hs <= 256)
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+    else if (k.hs <= 512)
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+    else
+        hipLaunchKernelGGL((k_gsweep_fwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+}
+
 extern "C" {
 
 static int grad_fail(const char *what, hipError_t e)
@@ -933,15 +1147,44 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
     GSkew k;
     FRH_GCHECK(coef && x_in && x_out && T > 0 && T <= 65535 && make_gskew(h, w, k),
                "frh_grad_sweep: bad arguments (at most 1024 rows)");
-    hipStream_t st = (hipStream_t)stream;
-    const dim3 b(k.hs);
-    if (k.hs <= 256)
-        hipLaunchKernelGGL((k_gsweep_fwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
-    else if (k.hs <= 512)
-        hipLaunchKernelGGL((k_gsweep_fwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
-    else
-        hipLaunchKernelGGL((k_gsweep_fwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+    launch_gsweep1(coef, x_in, x_out, T, k, ax, ay, (hipStream_t)stream);
     FRH_GLAUNCH("sweep forward");
+    return FRH_OK;
+}
+
+int frh_grad_sweeps(const double *coef, const double *x_in, double *x_out, size_t out_stride, int n_sweeps,
+                    int write_all, int T, int h, int w, double ax, double ay, void *stream)
+{
+    GSkew k;
+    FRH_GCHECK(coef && x_in && x_out && n_sweeps > 0 && T > 0 && T <= 65535 && make_gskew(h, w, k),
+               "frh_grad_sweeps: bad arguments (at most 1024 rows)");
+    const size_t state = (size_t)T * 2 * k.plane;
+    FRH_GCHECK(!write_all || n_sweeps == 1 || out_stride >= state,
+               "frh_grad_sweeps: out_stride is smaller than one state (T * 2 * plane values)");
+    FRH_GCHECK(!write_all || x_in + state <= x_out || x_in >= x_out + (size_t)(n_sweeps - 1) * out_stride + state,
+               "frh_grad_sweeps: x_in must not overlap the written states");
+    hipStream_t st = (hipStream_t)stream;
+    const double *src = x_in;
+    for (int j = 0; j < n_sweeps;) {
+        int P = n_sweeps - j;
+        const int pmax = k.hs > 512 ? 1 : gmax_pipe(k.hs <= 256 ? 4 : 8);  // 1024 lanes: single sweeps
+        if (P > pmax) {  // even chunks: 9 sweeps as 5 + 4
+            const int nch = (P + pmax - 1) / pmax;
+            P = (P + nch - 1) / nch;
+        }
+        double *dst = write_all ? x_out + (size_t)j * out_stride : x_out;
+        if (P == 1)
+            launch_gsweep1(coef, src, dst, T, k, ax, ay, st);
+        else if (k.hs <= 256)
+            write_all ? launch_gsweeps<4, true>(P, coef, src, dst, out_stride, T, k, ax, ay, st)
+                      : launch_gsweeps<4, false>(P, coef, src, dst, 0, T, k, ax, ay, st);
+        else
+            write_all ? launch_gsweeps<8, true>(P, coef, src, dst, out_stride, T, k, ax, ay, st)
+                      : launch_gsweeps<8, false>(P, coef, src, dst, 0, T, k, ax, ay, st);
+        FRH_GLAUNCH("sweeps forward");
+        src = write_all ? dst + (size_t)(P - 1) * out_stride : dst;
+        j += P;
+    }
     return FRH_OK;
 }
 

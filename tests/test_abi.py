@@ -35,7 +35,7 @@ def test_host_entry_points_without_gpu():
     import numpy as np
     from pyflowreg_amd import _lib
     L = _lib.lib()
-    assert L.frh_abi_version() == 8
+    assert L.frh_abi_version() == 9
     assert L.frh_warping_depth(0.8, 50, 512, 512) == 18
     g = np.array([1.0], np.float32)
     idx = np.zeros((5, 4), np.int32)
@@ -64,6 +64,22 @@ def test_scratch_queries_and_rejection_without_gpu():
     assert rc == -1 and b"scratch of 2880 bytes needed, 0 given" in L.frh_last_error()
     rc = L.frh_median5_mirror(dummy, 3, 10, 12, dummy, 2879, None)
     assert rc == -1 and b"2879 given" in L.frh_last_error()
+
+
+def test_pipelined_sweeps_reject_overlap_without_gpu():
+    """frh_grad_sweeps with every state written: a stride under one state or
+    an input inside the outputs is rejected before any device work."""
+    from pyflowreg_amd import _lib
+    L = _lib.lib()
+    base = 1 << 24  # never dereferenced: the checks fail first
+    state = 2 * 2 * 64 * (16 + 20 - 1)  # T = 2 frames, 2 planes of the 16 x 20 skew
+    rc = L.frh_grad_sweeps(base, base + 8 * 10**6, base + 16 * 10**6, state - 1, 3, 1, 2, 16, 20, 1.0, 1.0, None)
+    assert rc == -1 and b"out_stride" in L.frh_last_error()
+    out = base + 8 * 10**6
+    rc = L.frh_grad_sweeps(base, out + 8 * state, out, state, 3, 1, 2, 16, 20, 1.0, 1.0, None)
+    assert rc == -1 and b"overlap" in L.frh_last_error()
+    rc = L.frh_grad_sweeps(base, out, out, state, 0, 0, 2, 16, 20, 1.0, 1.0, None)
+    assert rc == -1 and b"bad arguments" in L.frh_last_error()
 
 
 def test_get_displacement_raises_without_gpu():

@@ -245,6 +245,62 @@ def test_sweep_adjoint_several_waves(gpu):
         assert _rel(_unsk(g_coef, h, w), ct.grad) < 1e-11, (h, w)
 
 
+def _sweeps_case(L, _lib, g, T, h, w, ax, ay):
+    coef = torch.rand((T, 6, h, w), dtype=F64, generator=g) * 0.5
+    x = torch.randn((T, 2, h, w), dtype=F64, generator=g)
+    cd, xd = _cuda(_sk(coef), _sk(x))
+    return cd, xd
+
+
+def test_pipelined_sweeps_equal_single_sweeps(gpu):
+    """frh_grad_sweeps (up to 5 sweeps per wavefront pass, LDS exchange of
+    every sweep's edge values between waves) against the same number of
+    single-sweep launches: bit-exact, for every state when write_all, for the
+    last when not (also in place), at 1..9 sweeps and levels of 1..700 rows
+    (the 1024-lane form runs single sweeps)."""
+    from pyflowreg_amd import _lib
+    g = torch.Generator().manual_seed(31)
+    L = _lib.lib()
+    st = _lib.stream_handle()
+    ax, ay = 1.2, 0.8
+    for (h, w, T) in ((17, 21, 2), (1, 9, 2), (9, 1, 1), (1, 1, 1), (65, 7, 1), (130, 5, 2), (300, 40, 1),
+                      (512, 37, 1), (700, 12, 1)):
+        cd, xd = _sweeps_case(L, _lib, g, T, h, w, ax, ay)
+        for n in (1, 2, 3, 4, 5, 9):
+            ref = torch.empty((n,) + xd.shape, dtype=F64, device="cuda")
+            src = xd
+            for j in range(n):
+                _lib.check(L.frh_grad_sweep(cd.data_ptr(), src.data_ptr(), ref[j].data_ptr(), T, h, w, ax, ay,
+                                            st), "sweep")
+                src = ref[j]
+            allx = torch.full_like(ref, float("nan"))
+            _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), allx.data_ptr(), xd.numel(), n, 1, T, h, w,
+                                         ax, ay, st), "sweeps (all)")
+            last = torch.full_like(xd, float("nan"))
+            _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), last.data_ptr(), 0, n, 0, T, h, w,
+                                         ax, ay, st), "sweeps (last)")
+            inplace = xd.clone()
+            _lib.check(L.frh_grad_sweeps(cd.data_ptr(), inplace.data_ptr(), inplace.data_ptr(), 0, n, 0, T, h, w,
+                                         ax, ay, st), "sweeps (in place)")
+            torch.cuda.synchronize()
+            # (padding slots are never written)
+            assert torch.equal(_unsk(allx, h, w), _unsk(ref, h, w)), (h, w, n)
+            assert torch.equal(_unsk(last, h, w), _unsk(ref[-1], h, w)), (h, w, n)
+            assert torch.equal(_unsk(inplace, h, w), _unsk(ref[-1], h, w)), (h, w, n)
+    # against the torch sweep, 5 sweeps in one pass
+    coef = torch.rand((1, 6, 70, 33), dtype=F64, generator=g) * 0.5
+    x = torch.randn((1, 2, 70, 33), dtype=F64, generator=g)
+    y = x
+    for _ in range(5):
+        y = R.sor_sweep(coef, y, ax, ay)
+    cd, xd = _cuda(_sk(coef), _sk(x))
+    out = torch.empty_like(xd)
+    _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), 0, 5, 0, 1, 70, 33, ax, ay, st),
+               "sweeps")
+    torch.cuda.synchronize()
+    assert _rel(_unsk(out, 70, 33), y) < 1e-12
+
+
 def test_skew_conversion_round_trip(gpu):
     from pyflowreg_amd import autograd as AG
     x = torch.randn((3, 2, 37, 70), dtype=F64, device="cuda")
