@@ -415,6 +415,27 @@ int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double 
                            const double *lam_out, double *lam_in, double *g_coef, int g_coef_set, double *g_axy,
                            int T, int h, int w, double ax, double ay, void *stream);
 
+/* The adjoints of a lag block's n_sweeps sweeps with one coefficient set
+ * (frh_grad_sweeps) in one launch per at most 32 sweeps: a chain of
+ * work-groups per frame, one per sweep, each passing its lam_in to the next
+ * through global memory as it goes (agent-scope release / acquire of
+ * per-stage progress flags); bit-identical to frh_grad_sweep_adjoint
+ * called from the block's last sweep to its first.  x0: the block's start
+ * state; the state after sweep j at xs + j state_stride (values); lam_top:
+ * adjoint of the last state; lam_bottom (output, not lam_top): adjoint of
+ * x0; g_coef and g_axy as in frh_grad_sweep_adjoint (g_coef_set: the
+ * block's last sweep stores).  scratch: frh_grad_sweeps_adjoint_scratch_bytes
+ * bytes; its first int is set nonzero when a stage gave up waiting for its
+ * predecessor (bounded waits: the launch ends, its results are invalid) --
+ * the caller zeroes it once and checks it after the launches.  Relies on
+ * work-groups being dispatched in index order (a stage's predecessor has the
+ * lower index). */
+size_t frh_grad_sweeps_adjoint_scratch_bytes(int n_sweeps, int T, int h, int w);
+int frh_grad_sweeps_adjoint(const double *coef, const double *x0, const double *xs, size_t state_stride,
+                            int n_sweeps, const double *lam_top, double *lam_bottom, double *g_coef, int g_coef_set,
+                            double *g_axy, int T, int h, int w, double ax, double ay, void *scratch,
+                            size_t scratch_bytes, void *stream);
+
 /* 5x5 mirror median of N (h, w) planes: out (when != NULL); g_x += the
  * output gradient g_out routed to the selected samples (when g_x != NULL). */
 int frh_grad_median(const double *x, int N, int h, int w, double *out, const double *g_out,

@@ -220,14 +220,12 @@ class LevelSolve(torch.autograd.Function):
         # stream runs the current block's adjoint (each uses one work-group
         # per frame); two state / coefficient buffers alternate.
         bufs = [torch.empty((nmax, T, 2, P), dtype=_F64, device=dev) for _ in range(2)]
+        chain = torch.zeros(L.frh_grad_sweeps_adjoint_scratch_bytes(nmax, T, h, w), dtype=torch.uint8, device=dev)
         coefs = [torch.empty((T, 6, P), dtype=_F64, device=dev) for _ in range(2)]
         main = torch.cuda.current_stream(dev)
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(main)
         hs = side.cuda_stream
-
-        def x_in(b, slot, k):  # the state entering sweep k of block b
-            return ckpt[b] if k == 0 else bufs[slot][k - 1]
 
         def recompute(b, slot):
             s, e, psi_one = blocks[b]
@@ -252,14 +250,13 @@ class LevelSolve(torch.autograd.Function):
                 ev_next = recompute(order[i + 1], 1 - slot)
             main.wait_event(ev_rec)
             s, e, psi_one = blocks[b]
+            # the block's adjoint sweeps as one chain of work-groups per frame;
             # the block's last sweep (its first adjoint) sets g_coef, the
             # others add to it
-            for k in range(e - s - 1, -1, -1):
-                check(L.frh_grad_sweep_adjoint(ptr(coefs[slot]), ptr(x_in(b, slot, k)), ptr(bufs[slot][k]),
-                                               ptr(lam), ptr(lam_in), ptr(g_coef), int(k == e - s - 1),
-                                               ptr(g_axy), T, h, w, axf, ayf, st),
-                      "frh_grad_sweep_adjoint")
-                lam, lam_in = lam_in, lam
+            check(L.frh_grad_sweeps_adjoint(ptr(coefs[slot]), ptr(ckpt[b]), ptr(bufs[slot]), bufs[slot][0].numel(),
+                                            e - s, ptr(lam), ptr(lam_in), ptr(g_coef), 1, ptr(g_axy), T, h, w,
+                                            axf, ayf, ptr(chain), chain.numel(), st), "frh_grad_sweeps_adjoint")
+            lam, lam_in = lam_in, lam
             # the block's coefficients depend on its start state, J, u0 and
             # the parameters; lam becomes the adjoint of the start state
             check(L.frh_grad_coef(ptr(ckpt[b]), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
@@ -269,6 +266,8 @@ class LevelSolve(torch.autograd.Function):
             ev_adj.record(main)
             ev_rec = ev_next
         main.wait_stream(side)
+        if int(chain[:4].view(torch.int32)[0]) != 0:
+            raise RuntimeError("LevelSolve backward: a chained adjoint stage gave up waiting for its predecessor")
         # the solve starts from du = dv = 0: lam (its adjoint) is dropped
         g_f2w = None
         if ctx.needs_input_grad[1]:

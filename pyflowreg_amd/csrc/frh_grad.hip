@@ -730,15 +730,86 @@ struct GBwdHist {
     double anu, anv, adu, adv, wu, wv, b, A12;
 };
 
-template <int D, int MAXW>
+// The adjoint sweeps of one lag block as a chain of work-groups (CHAIN):
+// work-group (stage s, frame t) runs the adjoint of the block's sweep
+// n - 1 - s, the same code as a single launch, with its lam_out being stage
+// s - 1's lam_in and its coefficient adjoints added to what stage s - 1
+// stored.  Stage s - 1 publishes how many steps it has completed (a release
+// store of an agent-scope flag every kChainPublish steps, after its waves'
+// stores have completed); stage s waits, before it loads the lam_out of
+// diagonal x, for the step after which that diagonal's lam_in is complete,
+// and acquires.  Stages 0 .. n - 1 of all frames are numbered stage-major,
+// so every producer is dispatched before its consumer (work-groups are
+// dispatched in index order) and the frame count is padded to a multiple of
+// 8 so that a frame's stages share an XCD (the index modulo 8) and its L2.
+// A wait is bounded: one that gives up sets *err and lets the work-group
+// finish (its results are then invalid), so a launch always ends.  The
+// row partials of ax, ay are added stage after stage (a last flag value),
+// so every sum is formed in the order of single launches: the chain's
+// results are bit-identical to n single adjoint launches.
+struct GChain {
+    const double *x0;       // the block's start state, x_in of its first sweep
+    const double *xs;       // state after the block's sweep j at xs + j sstride
+    size_t sstride;
+    const double *lam_top;  // adjoint of the block's last state
+    double *lam_mid;        // lam_in of stage s < n - 1 at lam_mid + s lstride
+    size_t lstride;
+    double *lam_bottom;     // adjoint of the block's start state
+    int *flags;             // stage s, frame t: flags[(s Tpad + t) kChainFlagStride]
+    int *err;
+    int n, T, Tpad;
+};
+constexpr int kChainPublish = 8;
+constexpr int kChainFlagStride = 64;  // ints: one flag per 256 bytes
+constexpr unsigned kChainSpin = 1u << 18;
+
+// Waits until the producer's flag is at least need (pre: the flag as
+// loaded one step earlier) and acquires; returns the flag value seen.
+__device__ inline int chain_acquire(int *flag, int need, int pre, int *err)
+{
+    int v = pre;
+    if (v < need) {
+        unsigned it = 0;
+        for (;;) {
+            v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (v >= need || ++it >= kChainSpin) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (v < need) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = 0x7fffffff;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return v;
+}
+
+// s_waitcnt vmcnt(0): this wave's stores have been performed
+__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
+template <int D, int MAXW, bool CHAIN>
 __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, const double *xin,
                                                           const double *xout, const double *lam_out,
                                                           double *gK, double *gaxy, double *lam_in, GSkew k,
-                                                          double ax, double ay, int gK_set)
+                                                          double ax, double ay, int gK_set, GChain ch)
 {
     __shared__ double xb[2][MAXW][2];  // lane 0's anu, anv, for the wave above's lane 63
     __shared__ double xu[2][MAXW][2];  // lane 63's anu, anv, for the wave below's lane 0
-    const int t = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = r >> 6, nw = (int)(blockDim.x >> 6);
+    int t = blockIdx.x;
+    int *fl_in = nullptr, *fl_out = nullptr;  // producer's / own progress flag
+    if constexpr (CHAIN) {
+        const int stage = (int)blockIdx.x / ch.Tpad, sw = ch.n - 1 - stage;
+        t = (int)blockIdx.x - stage * ch.Tpad;
+        if (t >= ch.T) return;
+        xin = sw == 0 ? ch.x0 : ch.xs + (size_t)(sw - 1) * ch.sstride;
+        xout = ch.xs + (size_t)sw * ch.sstride;
+        lam_out = stage == 0 ? ch.lam_top : ch.lam_mid + (size_t)(stage - 1) * ch.lstride;
+        lam_in = stage == ch.n - 1 ? ch.lam_bottom : ch.lam_mid + (size_t)stage * ch.lstride;
+        gK_set = gK_set && stage == 0;
+        if (stage > 0) fl_in = ch.flags + (size_t)((stage - 1) * ch.Tpad + t) * kChainFlagStride;
+        if (stage < ch.n - 1) fl_out = ch.flags + (size_t)(stage * ch.Tpad + t) * kChainFlagStride;
+    }
+    const int r = threadIdx.x, lane = r & 63, wv = r >> 6, nw = (int)(blockDim.x >> 6);
     const int h = k.h, w = k.w, hs = k.hs;
     const bool row_ok = r < h, down_ok = r + 1 < h, up_ok = r >= 1 && r - 1 < h;
     const unsigned pb = (unsigned)(k.plane * 8);
@@ -778,6 +849,14 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
         for (int e = 0; e < 6; ++e) g[e] = buf_load_f64(rG, q, e * pb);
     };
     const int nd = h + w - 1;
+    // Chain: after its step i the producer has stored the lam_in of
+    // diagonals >= nd + 1 - i and the coefficient adjoints of diagonals
+    // >= nd - 1 - i; step i loads lam_out of diagonal nd - i - D, so it
+    // needs i + D + 2 completed steps (of nd + 2 in all).
+    int avail = 0, fpre = 0;
+    if constexpr (CHAIN) {
+        if (fl_in) avail = chain_acquire(fl_in, min(D + 1, nd + 2), 0, ch.err);
+    }
     // step i handles diagonal nd - 1 - i; steps nd and nd + 1 only finish
     // the lam_in of the last two diagonals
 #pragma unroll
@@ -810,6 +889,13 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
                                   (r > 0 ? ay * unv : 0.0) - H.A12 * H.anu;
                 buf_store_f64(Lu, rN, q2, 0);
                 buf_store_f64(Lv, rN, q2, pb);
+            }
+            if constexpr (CHAIN) {
+                if (fl_in) {
+                    const int need = min(i + D + 2, nd + 2);
+                    if (avail < need) avail = chain_acquire(fl_in, need, __builtin_amdgcn_readfirstlane(fpre), ch.err);
+                    fpre = __hip_atomic_load(fl_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
             }
             issue_g(Gc[(s + 1) & 1], d - 1);  // D is even: the parity of s is the parity of i
             issue(S[(s + D - 1) % D], d - (D - 1));
@@ -863,13 +949,31 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
                 xu[i & 1][wv][0] = a_nu;
                 xu[i & 1][wv][1] = a_nv;
             }
-            lds_barrier();
+            if constexpr (CHAIN) {
+                const bool pub = fl_out && ((i + 1) % kChainPublish == 0 || i + 1 == nd + 2);
+                if (pub) wait_stores();
+                lds_barrier();
+                if (pub && r == 0)
+                    __hip_atomic_store(fl_out, min(i + 1, nd + 2), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                lds_barrier();
+            }
         }
+    }
+    if constexpr (CHAIN) {
+        if (fl_in) chain_acquire(fl_in, nd + 3, __builtin_amdgcn_readfirstlane(fpre), ch.err);  // producer's row sums added
     }
     if (row_ok) {
         double *G = gaxy + (size_t)t * 2 * k.plane;
         G[r] += g_axr;
         G[k.plane + r] += g_ayr;
+    }
+    if constexpr (CHAIN) {
+        if (fl_out) {
+            wait_stores();
+            __syncthreads();
+            if (r == 0) __hip_atomic_store(fl_out, nd + 3, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        }
     }
 }
 
@@ -1201,15 +1305,107 @@ int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double 
     const dim3 b(k.hs);
     const int set = g_coef_set ? 1 : 0;
     if (k.hs <= 256)
-        hipLaunchKernelGGL((k_gsweep_bwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           lam_in, k, ax, ay, set);
+        hipLaunchKernelGGL((k_gsweep_bwd<4, 4, false>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+                           lam_in, k, ax, ay, set, GChain{});
     else if (k.hs <= 512)
-        hipLaunchKernelGGL((k_gsweep_bwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           lam_in, k, ax, ay, set);
+        hipLaunchKernelGGL((k_gsweep_bwd<4, 8, false>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+                           lam_in, k, ax, ay, set, GChain{});
     else
-        hipLaunchKernelGGL((k_gsweep_bwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
-                           lam_in, k, ax, ay, set);
+        hipLaunchKernelGGL((k_gsweep_bwd<2, 16, false>), dim3(T), b, 0, st, coef, x_in, x_out, lam_out, g_coef, g_axy,
+                           lam_in, k, ax, ay, set, GChain{});
     FRH_GLAUNCH("sweep adjoint");
+    return FRH_OK;
+}
+
+// Chained adjoint: at most kChainMax sweeps per launch (longer blocks take
+// several launches).  Scratch: the failure word, the flags, then the lam_in
+// of every stage but the last.
+constexpr int kChainMax = 32;
+
+static size_t chain_tpad(int T) { return ((size_t)T + 7) / 8 * 8; }
+
+static size_t chain_scratch_bytes(int n, int T, const GSkew &k)
+{
+    const int nc = n < kChainMax ? n : kChainMax;
+    const int planes = nc - 1 + (n > kChainMax ? 1 : 0);  // + the hand-over between launches
+    return 256 + (size_t)nc * chain_tpad(T) * kChainFlagStride * 4 + (size_t)planes * T * 2 * k.plane * 8;
+}
+
+__global__ void k_gfill_int(int *p, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0;
+}
+
+size_t frh_grad_sweeps_adjoint_scratch_bytes(int n_sweeps, int T, int h, int w)
+{
+    GSkew k;
+    if (n_sweeps <= 0 || T <= 0 || !make_gskew(h, w, k)) return 0;
+    return chain_scratch_bytes(n_sweeps, T, k);
+}
+
+int frh_grad_sweeps_adjoint(const double *coef, const double *x0, const double *xs, size_t state_stride,
+                            int n_sweeps, const double *lam_top, double *lam_bottom, double *g_coef, int g_coef_set,
+                            double *g_axy, int T, int h, int w, double ax, double ay, void *scratch,
+                            size_t scratch_bytes, void *stream)
+{
+    GSkew k;
+    FRH_GCHECK(coef && x0 && xs && lam_top && lam_bottom && g_coef && g_axy && scratch && n_sweeps > 0 && T > 0 &&
+                   T <= 65535 && make_gskew(h, w, k),
+               "frh_grad_sweeps_adjoint: bad arguments (at most 1024 rows)");
+    const size_t state = (size_t)T * 2 * k.plane;
+    FRH_GCHECK(n_sweeps == 1 || state_stride >= state, "frh_grad_sweeps_adjoint: state_stride is smaller than one state");
+    FRH_GCHECK(lam_top != lam_bottom, "frh_grad_sweeps_adjoint: lam_bottom must not alias lam_top");
+    const size_t need = chain_scratch_bytes(n_sweeps, T, k);
+    if (scratch_bytes < need)
+        return frh_report_error(FRH_EINVAL, "frh_grad_sweeps_adjoint: scratch of %zu bytes needed, %zu given", need,
+                                scratch_bytes);
+    hipStream_t st = (hipStream_t)stream;
+    const int Tpad = (int)chain_tpad(T);
+    int *err = (int *)scratch;
+    int *flags = (int *)((char *)scratch + 256);
+    const int nmax = n_sweeps < kChainMax ? n_sweeps : kChainMax;
+    double *lam_mid = (double *)((char *)flags + (size_t)nmax * Tpad * kChainFlagStride * 4);
+    const dim3 b(k.hs);
+    // launches from the block's last sweeps to its first; a launch's
+    // lam_bottom is the next one's lam_top, alternating between the caller's
+    // lam_bottom and a scratch plane so that the last launch writes lam_bottom
+    const int L = (n_sweeps + nmax - 1) / nmax;
+    double *hand = lam_mid + (size_t)(nmax - 1) * state;
+    const double *top = lam_top;
+    for (int li = 0; li < L; ++li) {
+        const int hi = n_sweeps - li * nmax, lo = hi - nmax > 0 ? hi - nmax : 0, n = hi - lo;
+        double *bottom = (L - 1 - li) % 2 == 0 ? lam_bottom : hand;
+        GChain ch{};
+        ch.x0 = lo == 0 ? x0 : xs + (size_t)(lo - 1) * state_stride;
+        ch.xs = xs + (size_t)lo * state_stride;
+        ch.sstride = state_stride;
+        ch.lam_top = top;
+        ch.lam_mid = lam_mid;
+        ch.lstride = state;
+        ch.lam_bottom = bottom;
+        ch.flags = flags;
+        ch.err = err;
+        ch.n = n;
+        ch.T = T;
+        ch.Tpad = Tpad;
+        const int set = g_coef_set && li == 0 ? 1 : 0;
+        const int nflags = n * Tpad * kChainFlagStride;
+        hipLaunchKernelGGL(k_gfill_int, dim3((nflags + 255) / 256), dim3(256), 0, st, flags, nflags);
+        FRH_GLAUNCH("chained adjoint flags");
+        const dim3 g(n * Tpad);
+        if (k.hs <= 256)
+            hipLaunchKernelGGL((k_gsweep_bwd<4, 4, true>), g, b, 0, st, coef, nullptr, nullptr, nullptr, g_coef,
+                               g_axy, nullptr, k, ax, ay, set, ch);
+        else if (k.hs <= 512)
+            hipLaunchKernelGGL((k_gsweep_bwd<4, 8, true>), g, b, 0, st, coef, nullptr, nullptr, nullptr, g_coef,
+                               g_axy, nullptr, k, ax, ay, set, ch);
+        else
+            hipLaunchKernelGGL((k_gsweep_bwd<2, 16, true>), g, b, 0, st, coef, nullptr, nullptr, nullptr, g_coef,
+                               g_axy, nullptr, k, ax, ay, set, ch);
+        FRH_GLAUNCH("chained adjoint");
+        top = bottom;
+    }
     return FRH_OK;
 }
 

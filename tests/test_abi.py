@@ -82,6 +82,27 @@ def test_pipelined_sweeps_reject_overlap_without_gpu():
     assert rc == -1 and b"bad arguments" in L.frh_last_error()
 
 
+def test_chained_adjoint_scratch_without_gpu():
+    """frh_grad_sweeps_adjoint's scratch: a failure word, one 256-byte flag per
+    stage and padded frame, the lam planes between stages (and one hand-over
+    plane above 32 sweeps); a short buffer is rejected before device work."""
+    from pyflowreg_amd import _lib
+    L = _lib.lib()
+    state = 3 * 2 * 64 * (16 + 20 - 1)  # T = 3 frames (8 padded), 2 planes of the 16 x 20 skew
+    assert L.frh_grad_sweeps_adjoint_scratch_bytes(5, 3, 16, 20) == 256 + 5 * 8 * 256 + 4 * state * 8
+    assert L.frh_grad_sweeps_adjoint_scratch_bytes(1, 3, 16, 20) == 256 + 8 * 256
+    assert L.frh_grad_sweeps_adjoint_scratch_bytes(40, 3, 16, 20) == 256 + 32 * 8 * 256 + 32 * state * 8
+    assert L.frh_grad_sweeps_adjoint_scratch_bytes(5, 3, 2000, 20) == 0
+    base = 1 << 24  # never dereferenced: the checks fail first
+    need = L.frh_grad_sweeps_adjoint_scratch_bytes(5, 3, 16, 20)
+    args = [base] * 3 + [state, 5, base, base + 8 * state, base, 1, base, 3, 16, 20, 1.0, 1.0, base]
+    rc = L.frh_grad_sweeps_adjoint(*args, need - 1, None)
+    assert rc == -1 and b"scratch of" in L.frh_last_error()
+    args[6] = base  # lam_bottom aliases lam_top
+    rc = L.frh_grad_sweeps_adjoint(*args, need, None)
+    assert rc == -1 and b"alias" in L.frh_last_error()
+
+
 def test_get_displacement_raises_without_gpu():
     import numpy as np
     import torch

@@ -301,6 +301,63 @@ def test_pipelined_sweeps_equal_single_sweeps(gpu):
     assert _rel(_unsk(out, 70, 33), y) < 1e-12
 
 
+def test_chained_adjoint_equals_single_adjoints(gpu):
+    """frh_grad_sweeps_adjoint (one work-group per sweep and frame, lam_in
+    handed on through global memory under progress flags) against the same
+    adjoints as single launches from the block's last sweep to its first:
+    bit-exact lam, coefficient adjoints (set and accumulated) and row partials,
+    for frame counts that are not multiples of 8, levels of 1..700 rows (the
+    1024-lane form at 700) and blocks longer than one launch (33, 65 sweeps:
+    hand-over planes between launches); no stage gave up waiting."""
+    from pyflowreg_amd import _lib
+    g = torch.Generator().manual_seed(41)
+    L = _lib.lib()
+    st = _lib.stream_handle()
+    ax, ay = 1.1, 0.9
+    nan = float("nan")
+    for (h, w, T, ns) in ((17, 21, 3, (1, 2, 3, 5)), (1, 1, 1, (2, 5)), (1, 9, 2, (3,)), (9, 1, 2, (3,)),
+                          (65, 7, 9, (2, 5)), (300, 40, 2, (5,)), (512, 37, 1, (4, 5)), (700, 12, 1, (3,)),
+                          (17, 5, 2, (33, 65))):
+        coef = torch.rand((T, 6, h, w), dtype=F64, generator=g) * 0.5
+        x = torch.randn((T, 2, h, w), dtype=F64, generator=g)
+        lam = torch.randn((T, 2, h, w), dtype=F64, generator=g)
+        cd, xd, lamd = _cuda(_sk(coef), _sk(x), _sk(lam))
+        _, P = _skew_index(h, w)
+        for n in ns:
+            xs = torch.empty((n,) + xd.shape, dtype=F64, device="cuda")
+            src = xd
+            for j in range(n):
+                _lib.check(L.frh_grad_sweep(cd.data_ptr(), src.data_ptr(), xs[j].data_ptr(), T, h, w, ax, ay, st),
+                           "sweep")
+                src = xs[j]
+            for gset in (0, 1):
+                g_ref = torch.full_like(cd, 0.25)
+                gaxy_ref = torch.full((T, 2, P), 0.5, dtype=F64, device="cuda")
+                lam_a, lam_b = lamd.clone(), torch.empty_like(lamd)
+                for j in range(n - 1, -1, -1):
+                    x_in = xd if j == 0 else xs[j - 1]
+                    _lib.check(L.frh_grad_sweep_adjoint(cd.data_ptr(), x_in.data_ptr(), xs[j].data_ptr(),
+                                                        lam_a.data_ptr(), lam_b.data_ptr(), g_ref.data_ptr(),
+                                                        int(gset and j == n - 1), gaxy_ref.data_ptr(), T, h, w,
+                                                        ax, ay, st), "adjoint")
+                    lam_a, lam_b = lam_b, lam_a
+                nb = L.frh_grad_sweeps_adjoint_scratch_bytes(n, T, h, w)
+                scratch = torch.zeros(nb, dtype=torch.uint8, device="cuda")
+                g_ch = torch.full_like(cd, 0.25)
+                gaxy_ch = torch.full((T, 2, P), 0.5, dtype=F64, device="cuda")
+                lam_out = torch.full_like(lamd, nan)
+                _lib.check(L.frh_grad_sweeps_adjoint(cd.data_ptr(), xd.data_ptr(), xs.data_ptr(), xd.numel(), n,
+                                                     lamd.data_ptr(), lam_out.data_ptr(), g_ch.data_ptr(), gset,
+                                                     gaxy_ch.data_ptr(), T, h, w, ax, ay, scratch.data_ptr(), nb,
+                                                     st), "chained adjoint")
+                torch.cuda.synchronize()
+                case = (h, w, T, n, gset)
+                assert int(scratch[:4].view(torch.int32)[0]) == 0, case
+                assert torch.equal(_unsk(lam_out, h, w), _unsk(lam_a, h, w)), case
+                assert torch.equal(g_ch, g_ref), case
+                assert torch.equal(gaxy_ch, gaxy_ref), case
+
+
 def test_skew_conversion_round_trip(gpu):
     from pyflowreg_amd import autograd as AG
     x = torch.randn((3, 2, 37, 70), dtype=F64, device="cuda")
