@@ -398,13 +398,21 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
                    double ax, double ay, void *stream);
 
 /* n_sweeps consecutive frh_grad_sweep calls with one coefficient set (the
- * sweeps of a lag block), pipelined in one wavefront pass per at most 5
- * (levels of up to 256 rows) or 4 (512) sweeps (single sweeps above 512
- * rows); the results equal the single sweeps'.  write_all != 0: the state
- * after sweep j goes to x_out + j out_stride (values; x_in must not overlap
- * them); else only the last state, to x_out (which may equal x_in). */
+ * sweeps of a lag block); the results equal the single sweeps'.
+ * write_all != 0: the state after sweep j goes to x_out + j out_stride
+ * (values; x_in must not overlap them); else only the last state, to x_out
+ * (which may equal x_in).  Without scratch (NULL): pipelined in one
+ * wavefront pass per at most 5 (levels of up to 256 rows) or 4 (512) sweeps
+ * (single sweeps above 512 rows), one work-group per frame.  With scratch
+ * (frh_grad_sweeps_scratch_bytes bytes): one launch per at most 32 sweeps
+ * as a chain of work-groups per frame, one per sweep, each reading its
+ * predecessor's output as it is published (as frh_grad_sweeps_adjoint; the
+ * first int of scratch is the failure word) -- for batches that leave most
+ * compute units idle. */
+size_t frh_grad_sweeps_scratch_bytes(int n_sweeps, int T);
 int frh_grad_sweeps(const double *coef, const double *x_in, double *x_out, size_t out_stride, int n_sweeps,
-                    int write_all, int T, int h, int w, double ax, double ay, void *stream);
+                    int write_all, int T, int h, int w, double ax, double ay, void *scratch, size_t scratch_bytes,
+                    void *stream);
 
 /* Exact adjoint of frh_grad_sweep (all skewed): lam_in = (dx_out/dx_in)^T
  * lam_out; g_coef += (dx_out/dcoef)^T lam_out (g_coef = ... when

@@ -105,8 +105,9 @@ SIGNATURES = {
                       + [c_void_p] * 7),
     "frh_grad_sweep": (c_int, [c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_double, c_double,
                                c_void_p]),
+    "frh_grad_sweeps_scratch_bytes": (ctypes.c_size_t, [c_int, c_int]),
     "frh_grad_sweeps": (c_int, [c_void_p, c_void_p, c_void_p, ctypes.c_size_t, c_int, c_int, c_int, c_int,
-                                c_int, c_double, c_double, c_void_p]),
+                                c_int, c_double, c_double, c_void_p, ctypes.c_size_t, c_void_p]),
     "frh_grad_sweep_adjoint": (c_int, [c_void_p] * 6 + [c_int, c_void_p]
                                + [c_int, c_int, c_int, c_double, c_double, c_void_p]),
     "frh_grad_sweeps_adjoint_scratch_bytes": (ctypes.c_size_t, [c_int, c_int, c_int, c_int]),

@@ -45,6 +45,26 @@ def _c(t: torch.Tensor) -> torch.Tensor:
     return t.contiguous()
 
 
+def _chain_scratch(L, nmax, T, dev):
+    """Flags (zeroed) for a lag block's sweeps as a chain of work-groups, one
+    per sweep and frame, when those fit on the device at once (batches that
+    would leave most compute units idle); None: one work-group per frame."""
+    if T * nmax > torch.cuda.get_device_properties(dev).multi_processor_count:
+        return None
+    return torch.zeros(L.frh_grad_sweeps_scratch_bytes(nmax, T), dtype=torch.uint8, device=dev)
+
+
+def _scratch_args(scr):
+    return (ptr(scr), scr.numel()) if scr is not None else (None, 0)
+
+
+def _check_chain(scr, what):
+    """The failure word of a chain's scratch: set when a stage gave up waiting
+    for its predecessor (its results are invalid)."""
+    if scr is not None and int(scr[:4].view(torch.int32)[0]) != 0:
+        raise RuntimeError(f"{what}: a chained stage gave up waiting for its predecessor")
+
+
 def lag_blocks(iterations: int, update_lag: int):
     """Coefficient refresh schedule of core/level_solver.py:229-255: a block
     starts at iteration 0 and at every iteration with (it + 1) % update_lag
@@ -172,15 +192,18 @@ class LevelSolve(torch.autograd.Function):
         x = torch.zeros((T, 2, P), dtype=_F64, device=dev)
         x_next = torch.empty_like(x)
         coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
+        scr = _chain_scratch(L, max(e - s for s, e, _ in blocks), T, dev)
         for b, (s, e, psi_one) in enumerate(blocks):
             ckpt[b].copy_(x)
             check(L.frh_grad_coef(ptr(x), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
                                   int(psi_one), ptr(coef), None, None, None, None, None, st),
                   "frh_grad_coef")
-            # the block's sweeps share the coefficients: pipelined launches
-            check(L.frh_grad_sweeps(ptr(coef), ptr(x), ptr(x_next), 0, e - s, 0, T, h, w, axf, ayf, st),
-                  "frh_grad_sweeps")
+            # the block's sweeps share the coefficients: one chain of
+            # work-groups or pipelined launches
+            check(L.frh_grad_sweeps(ptr(coef), ptr(x), ptr(x_next), 0, e - s, 0, T, h, w, axf, ayf,
+                                    *_scratch_args(scr), st), "frh_grad_sweeps")
             x, x_next = x_next, x
+        _check_chain(scr, "LevelSolve forward")
         du = from_skew(x, h, w)
         if median:
             dm = torch.empty_like(du)
@@ -221,6 +244,7 @@ class LevelSolve(torch.autograd.Function):
         # per frame); two state / coefficient buffers alternate.
         bufs = [torch.empty((nmax, T, 2, P), dtype=_F64, device=dev) for _ in range(2)]
         chain = torch.zeros(L.frh_grad_sweeps_adjoint_scratch_bytes(nmax, T, h, w), dtype=torch.uint8, device=dev)
+        scr = _chain_scratch(L, nmax, T, dev)  # the recompute's (side stream)
         coefs = [torch.empty((T, 6, P), dtype=_F64, device=dev) for _ in range(2)]
         main = torch.cuda.current_stream(dev)
         side = torch.cuda.Stream(device=dev)
@@ -233,7 +257,7 @@ class LevelSolve(torch.autograd.Function):
                                   int(psi_one), ptr(coefs[slot]), None, None, None, None, None, hs),
                   "frh_grad_coef")
             check(L.frh_grad_sweeps(ptr(coefs[slot]), ptr(ckpt[b]), ptr(bufs[slot]), bufs[slot][0].numel(), e - s,
-                                    1, T, h, w, axf, ayf, hs), "frh_grad_sweeps")
+                                    1, T, h, w, axf, ayf, *_scratch_args(scr), hs), "frh_grad_sweeps")
             ev = torch.cuda.Event()
             ev.record(side)
             return ev
@@ -266,8 +290,8 @@ class LevelSolve(torch.autograd.Function):
             ev_adj.record(main)
             ev_rec = ev_next
         main.wait_stream(side)
-        if int(chain[:4].view(torch.int32)[0]) != 0:
-            raise RuntimeError("LevelSolve backward: a chained adjoint stage gave up waiting for its predecessor")
+        _check_chain(chain, "LevelSolve backward")
+        _check_chain(scr, "LevelSolve backward (recompute)")
         # the solve starts from du = dv = 0: lam (its adjoint) is dropped
         g_f2w = None
         if ctx.needs_input_grad[1]:

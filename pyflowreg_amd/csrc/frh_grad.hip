@@ -434,6 +434,53 @@ __global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double 
     gpar[((size_t)t * (2 + C) + 1) * P + p] += g_ay;
 }
 
+// --------------------------------------------------------------- chains
+// Chains of work-groups.  A lag block's sweeps (or their adjoints) can run
+// as one work-group per sweep and frame, each stage consuming its
+// predecessor's output while that is still being produced: the producer
+// publishes how many steps it has completed in an agent-scope flag (a
+// release store every few steps, after its waves' stores have
+// completed), the consumer waits for the steps it needs before it loads
+// their results, and acquires.  Stages are numbered stage-major, so every
+// producer has a lower work-group index than its consumers and is
+// dispatched first; frames are padded to a multiple of 8 so that a frame's
+// stages share an XCD (work-group index modulo 8) and its L2.  A wait is
+// bounded: one that gives up sets *err and proceeds (its results are then
+// invalid), so a launch always ends.
+// Steps between two publications: each costs the producer a write-back of
+// its L2, so fewer are faster, but each adds to the lag between stages;
+// measured at 32 frames (512 x 512: 32 best, 8 takes 1.3x as long).
+__host__ __device__ constexpr int chain_publish_interval(int nd)
+{
+    return nd / 32 < 4 ? 4 : nd / 32 > 32 ? 32 : nd / 32;
+}
+constexpr int kChainFlagStride = 64;  // ints: one flag per 256 bytes
+constexpr unsigned kChainSpin = 1u << 18;
+
+// Waits until the producer's flag is at least need (pre: the flag as
+// loaded one step earlier) and acquires; returns the flag value seen.
+__device__ inline int chain_acquire(int *flag, int need, int pre, int *err)
+{
+    int v = pre;
+    if (v < need) {
+        unsigned it = 0;
+        for (;;) {
+            v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (v >= need || ++it >= kChainSpin) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (v < need) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = 0x7fffffff;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return v;
+}
+
+// s_waitcnt vmcnt(0): this wave's stores have been performed
+__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 // --------------------------------------------------------------- sweeps
 // One column-major lexicographic SOR sweep (core/level_solver.py:257-364),
 // one work-group per frame, lane = row, one anti-diagonal d = r + c per
@@ -459,12 +506,39 @@ struct GFwdSlot {
     double K[6], ou, ov, du, dv;
 };
 
-template <int D, int MAXW>
+// The sweeps of a lag block as a chain (CHAIN): work-group (stage s, frame
+// t) runs sweep s of the block, the same code as a single launch, reading
+// stage s - 1's output as it is published.  Stage s writes xs + s sstride;
+// with sstride 0 all stages work in place on xs: a stage reads a cell of
+// its input (its predecessor's output, published) before it overwrites it,
+// and its successor reads the cell only after it is published.
+struct GFChain {
+    const double *x0;  // stage 0's input
+    double *xs;        // stage s's output at xs + s sstride
+    size_t sstride;
+    int *flags;        // stage s, frame t: flags[(s Tpad + t) kChainFlagStride]
+    int *err;
+    int n, T, Tpad;
+    int pub;           // steps between two publications
+};
+
+template <int D, int MAXW, bool CHAIN>
 __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, const double *xin, double *xout,
-                                                          GSkew k, double ax, double ay)
+                                                          GSkew k, double ax, double ay, GFChain ch)
 {
     __shared__ double xb[2][MAXW][2];
-    const int t = blockIdx.x, r = threadIdx.x, lane = r & 63, wv = r >> 6;
+    int t = blockIdx.x;
+    int *fl_in = nullptr, *fl_out = nullptr;  // predecessor's / own progress flag
+    if constexpr (CHAIN) {
+        const int stage = (int)blockIdx.x / ch.Tpad;
+        t = (int)blockIdx.x - stage * ch.Tpad;
+        if (t >= ch.T) return;
+        xin = stage == 0 ? ch.x0 : ch.xs + (size_t)(stage - 1) * ch.sstride;
+        xout = ch.xs + (size_t)stage * ch.sstride;
+        if (stage > 0) fl_in = ch.flags + (size_t)((stage - 1) * ch.Tpad + t) * kChainFlagStride;
+        if (stage < ch.n - 1) fl_out = ch.flags + (size_t)(stage * ch.Tpad + t) * kChainFlagStride;
+    }
+    const int r = threadIdx.x, lane = r & 63, wv = r >> 6;
     const int h = k.h, w = k.w, hs = k.hs;
     const bool row_ok = r < h, down_ok = r + 1 < h;
     const unsigned pb = (unsigned)(k.plane * 8);
@@ -486,6 +560,14 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
         s.dv = buf_load_f64(rI, qd, pb);
     };
     const int nd = h + w - 1;
+    // Chain: the predecessor stores diagonal e at its step e; step d loads
+    // diagonals d + D - 1 and (lane 63) d + D, so it needs d + D + 1
+    // completed steps (of nd).
+    int avail = 0, fpre = 0, pcount = 0;
+    if constexpr (CHAIN) {
+        pcount = ch.pub;
+        if (fl_in) avail = chain_acquire(fl_in, min(D, nd), 0, ch.err);
+    }
 #pragma unroll
     for (int s = 0; s < D - 1; ++s) issue(S[s], s);
     double pu = 0.0, pv = 0.0;  // this lane's results of the previous step
@@ -493,6 +575,13 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
 #pragma unroll
         for (int s = 0; s < D; ++s) {
             const int d = d0 + s;
+            if constexpr (CHAIN) {
+                if (fl_in) {
+                    const int need = min(d + D + 1, nd);
+                    if (avail < need) avail = chain_acquire(fl_in, need, __builtin_amdgcn_readfirstlane(fpre), ch.err);
+                    fpre = __hip_atomic_load(fl_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
             issue(S[(s + D - 1) % D], d + D - 1);
             const GFwdSlot &c = S[s];
             const GFwdSlot &nx = S[(s + 1) % D];  // diagonal d + 1
@@ -523,7 +612,16 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_fwd(const double *coef, co
                 xb[d & 1][wv][0] = un;
                 xb[d & 1][wv][1] = vn;
             }
-            lds_barrier();
+            if constexpr (CHAIN) {
+                const bool pub = fl_out && (--pcount == 0 || d + 1 == nd);
+                if (pcount == 0) pcount = ch.pub;
+                if (pub) wait_stores();
+                lds_barrier();
+                if (pub && r == 0)
+                    __hip_atomic_store(fl_out, min(d + 1, nd), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            } else {
+                lds_barrier();
+            }
         }
     }
 }
@@ -730,23 +828,15 @@ struct GBwdHist {
     double anu, anv, adu, adv, wu, wv, b, A12;
 };
 
-// The adjoint sweeps of one lag block as a chain of work-groups (CHAIN):
-// work-group (stage s, frame t) runs the adjoint of the block's sweep
-// n - 1 - s, the same code as a single launch, with its lam_out being stage
-// s - 1's lam_in and its coefficient adjoints added to what stage s - 1
-// stored.  Stage s - 1 publishes how many steps it has completed (a release
-// store of an agent-scope flag every kChainPublish steps, after its waves'
-// stores have completed); stage s waits, before it loads the lam_out of
-// diagonal x, for the step after which that diagonal's lam_in is complete,
-// and acquires.  Stages 0 .. n - 1 of all frames are numbered stage-major,
-// so every producer is dispatched before its consumer (work-groups are
-// dispatched in index order) and the frame count is padded to a multiple of
-// 8 so that a frame's stages share an XCD (the index modulo 8) and its L2.
-// A wait is bounded: one that gives up sets *err and lets the work-group
-// finish (its results are then invalid), so a launch always ends.  The
-// row partials of ax, ay are added stage after stage (a last flag value),
-// so every sum is formed in the order of single launches: the chain's
-// results are bit-identical to n single adjoint launches.
+// The adjoint sweeps of one lag block as a chain (CHAIN): work-group
+// (stage s, frame t) runs the adjoint of the block's sweep n - 1 - s, the
+// same code as a single launch, with its lam_out being stage s - 1's lam_in
+// and its coefficient adjoints added to what stage s - 1 stored.  Stage s
+// waits, before it loads the lam_out of diagonal x, for the step after which
+// that diagonal's lam_in is complete.  The row partials of ax, ay are added
+// stage after stage (a last flag value), so every sum is formed in the order
+// of single launches: the chain's results are bit-identical to n single
+// adjoint launches.
 struct GChain {
     const double *x0;       // the block's start state, x_in of its first sweep
     const double *xs;       // state after the block's sweep j at xs + j sstride
@@ -758,34 +848,8 @@ struct GChain {
     int *flags;             // stage s, frame t: flags[(s Tpad + t) kChainFlagStride]
     int *err;
     int n, T, Tpad;
+    int pub;                // steps between two publications
 };
-constexpr int kChainPublish = 8;
-constexpr int kChainFlagStride = 64;  // ints: one flag per 256 bytes
-constexpr unsigned kChainSpin = 1u << 18;
-
-// Waits until the producer's flag is at least need (pre: the flag as
-// loaded one step earlier) and acquires; returns the flag value seen.
-__device__ inline int chain_acquire(int *flag, int need, int pre, int *err)
-{
-    int v = pre;
-    if (v < need) {
-        unsigned it = 0;
-        for (;;) {
-            v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (v >= need || ++it >= kChainSpin) break;
-            __builtin_amdgcn_s_sleep(2);
-        }
-        if (v < need) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = 0x7fffffff;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return v;
-}
-
-// s_waitcnt vmcnt(0): this wave's stores have been performed
-__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 template <int D, int MAXW, bool CHAIN>
 __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, const double *xin,
@@ -853,8 +917,9 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
     // diagonals >= nd + 1 - i and the coefficient adjoints of diagonals
     // >= nd - 1 - i; step i loads lam_out of diagonal nd - i - D, so it
     // needs i + D + 2 completed steps (of nd + 2 in all).
-    int avail = 0, fpre = 0;
+    int avail = 0, fpre = 0, pcount = 0;
     if constexpr (CHAIN) {
+        pcount = ch.pub;
         if (fl_in) avail = chain_acquire(fl_in, min(D + 1, nd + 2), 0, ch.err);
     }
     // step i handles diagonal nd - 1 - i; steps nd and nd + 1 only finish
@@ -950,7 +1015,8 @@ __global__ __launch_bounds__(64 * MAXW) void k_gsweep_bwd(const double *coef, co
                 xu[i & 1][wv][1] = a_nv;
             }
             if constexpr (CHAIN) {
-                const bool pub = fl_out && ((i + 1) % kChainPublish == 0 || i + 1 == nd + 2);
+                const bool pub = fl_out && (--pcount == 0 || i + 1 == nd + 2);
+                if (pcount == 0) pcount = ch.pub;
                 if (pub) wait_stores();
                 lds_barrier();
                 if (pub && r == 0)
@@ -1119,6 +1185,28 @@ __global__ __launch_bounds__(64) void k_gwarp(const double *img, const double *f
 using namespace frh;
 using namespace frh::grad;
 
+// Chains: at most kChainMax sweeps per launch (longer blocks take several
+// launches).  Scratch of the chained adjoint: the failure word, the flags,
+// then the lam_in of every stage but the last; of the chained sweeps: the
+// failure word and the flags.
+constexpr int kChainMax = 32;
+
+static size_t chain_tpad(int T) { return ((size_t)T + 7) / 8 * 8; }
+
+static size_t chain_scratch_bytes(int n, int T, const GSkew &k)
+{
+    const int nc = n < kChainMax ? n : kChainMax;
+    const int planes = nc - 1 + (n > kChainMax ? 1 : 0);  // + the hand-over between launches
+    return 256 + (size_t)nc * chain_tpad(T) * kChainFlagStride * 4 + (size_t)planes * T * 2 * k.plane * 8;
+}
+
+// clears the progress flags (one int per kChainFlagStride)
+__global__ void k_gclear_flags(int *flags, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[(size_t)i * kChainFlagStride] = 0;
+}
+
 // Sweeps per pipelined launch; a lag block of more sweeps takes several.
 // At 512 lanes (two waves per SIMD, 256 registers) 5 sweeps spill
 // registers and run slower than 3 + 2.
@@ -1151,11 +1239,24 @@ static void launch_gsweep1(const double *coef, const double *x_in, double *x_out
 {
     const dim3 b(k.hs);
     if (k.hs <= 256)
-        hipLaunchKernelGGL((k_gsweep_fwd<4, 4>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 4, false>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay, GFChain{});
     else if (k.hs <= 512)
-        hipLaunchKernelGGL((k_gsweep_fwd<4, 8>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 8, false>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay, GFChain{});
     else
-        hipLaunchKernelGGL((k_gsweep_fwd<2, 16>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay);
+        hipLaunchKernelGGL((k_gsweep_fwd<2, 16, false>), dim3(T), b, 0, st, coef, x_in, x_out, k, ax, ay, GFChain{});
+}
+
+// A lag block's sweeps as one chain launch (at most kChainMax sweeps).
+static void launch_gsweep_chain(const double *coef, const GFChain &ch, const GSkew &k, double ax, double ay,
+                                hipStream_t st)
+{
+    const dim3 g(ch.n * ch.Tpad), b(k.hs);
+    if (k.hs <= 256)
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 4, true>), g, b, 0, st, coef, nullptr, nullptr, k, ax, ay, ch);
+    else if (k.hs <= 512)
+        hipLaunchKernelGGL((k_gsweep_fwd<4, 8, true>), g, b, 0, st, coef, nullptr, nullptr, k, ax, ay, ch);
+    else
+        hipLaunchKernelGGL((k_gsweep_fwd<2, 16, true>), g, b, 0, st, coef, nullptr, nullptr, k, ax, ay, ch);
 }
 
 extern "C" {
@@ -1256,8 +1357,16 @@ int frh_grad_sweep(const double *coef, const double *x_in, double *x_out, int T,
     return FRH_OK;
 }
 
+size_t frh_grad_sweeps_scratch_bytes(int n_sweeps, int T)
+{
+    if (n_sweeps <= 0 || T <= 0) return 0;
+    const int nc = n_sweeps < kChainMax ? n_sweeps : kChainMax;
+    return 256 + (size_t)nc * chain_tpad(T) * kChainFlagStride * 4;
+}
+
 int frh_grad_sweeps(const double *coef, const double *x_in, double *x_out, size_t out_stride, int n_sweeps,
-                    int write_all, int T, int h, int w, double ax, double ay, void *stream)
+                    int write_all, int T, int h, int w, double ax, double ay, void *scratch, size_t scratch_bytes,
+                    void *stream)
 {
     GSkew k;
     FRH_GCHECK(coef && x_in && x_out && n_sweeps > 0 && T > 0 && T <= 65535 && make_gskew(h, w, k),
@@ -1268,6 +1377,35 @@ int frh_grad_sweeps(const double *coef, const double *x_in, double *x_out, size_
     FRH_GCHECK(!write_all || x_in + state <= x_out || x_in >= x_out + (size_t)(n_sweeps - 1) * out_stride + state,
                "frh_grad_sweeps: x_in must not overlap the written states");
     hipStream_t st = (hipStream_t)stream;
+    if (scratch && n_sweeps > 1) {
+        // one chain launch per at most kChainMax sweeps
+        const size_t need = frh_grad_sweeps_scratch_bytes(n_sweeps, T);
+        if (scratch_bytes < need)
+            return frh_report_error(FRH_EINVAL, "frh_grad_sweeps: scratch of %zu bytes needed, %zu given", need,
+                                    scratch_bytes);
+        const int Tpad = (int)chain_tpad(T);
+        const double *src = x_in;
+        for (int lo = 0; lo < n_sweeps; lo += kChainMax) {
+            const int n = n_sweeps - lo < kChainMax ? n_sweeps - lo : kChainMax;
+            GFChain ch{};
+            ch.x0 = src;
+            ch.xs = write_all ? x_out + (size_t)lo * out_stride : x_out;
+            ch.sstride = write_all ? out_stride : 0;
+            ch.err = (int *)scratch;
+            ch.flags = (int *)((char *)scratch + 256);
+            ch.n = n;
+            ch.T = T;
+            ch.Tpad = Tpad;
+            ch.pub = chain_publish_interval(k.h + k.w - 1);
+            const int nflags = n * Tpad;
+            hipLaunchKernelGGL(k_gclear_flags, dim3((nflags + 255) / 256), dim3(256), 0, st, ch.flags, nflags);
+            FRH_GLAUNCH("chained sweeps flags");
+            launch_gsweep_chain(coef, ch, k, ax, ay, st);
+            FRH_GLAUNCH("chained sweeps");
+            src = ch.xs + (size_t)(n - 1) * ch.sstride;
+        }
+        return FRH_OK;
+    }
     const double *src = x_in;
     for (int j = 0; j < n_sweeps;) {
         int P = n_sweeps - j;
@@ -1315,26 +1453,6 @@ int frh_grad_sweep_adjoint(const double *coef, const double *x_in, const double 
                            lam_in, k, ax, ay, set, GChain{});
     FRH_GLAUNCH("sweep adjoint");
     return FRH_OK;
-}
-
-// Chained adjoint: at most kChainMax sweeps per launch (longer blocks take
-// several launches).  Scratch: the failure word, the flags, then the lam_in
-// of every stage but the last.
-constexpr int kChainMax = 32;
-
-static size_t chain_tpad(int T) { return ((size_t)T + 7) / 8 * 8; }
-
-static size_t chain_scratch_bytes(int n, int T, const GSkew &k)
-{
-    const int nc = n < kChainMax ? n : kChainMax;
-    const int planes = nc - 1 + (n > kChainMax ? 1 : 0);  // + the hand-over between launches
-    return 256 + (size_t)nc * chain_tpad(T) * kChainFlagStride * 4 + (size_t)planes * T * 2 * k.plane * 8;
-}
-
-__global__ void k_gfill_int(int *p, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) p[i] = 0;
 }
 
 size_t frh_grad_sweeps_adjoint_scratch_bytes(int n_sweeps, int T, int h, int w)
@@ -1389,9 +1507,10 @@ int frh_grad_sweeps_adjoint(const double *coef, const double *x0, const double *
         ch.n = n;
         ch.T = T;
         ch.Tpad = Tpad;
+        ch.pub = chain_publish_interval(k.h + k.w - 1);
         const int set = g_coef_set && li == 0 ? 1 : 0;
-        const int nflags = n * Tpad * kChainFlagStride;
-        hipLaunchKernelGGL(k_gfill_int, dim3((nflags + 255) / 256), dim3(256), 0, st, flags, nflags);
+        const int nflags = n * Tpad;
+        hipLaunchKernelGGL(k_gclear_flags, dim3((nflags + 255) / 256), dim3(256), 0, st, flags, nflags);
         FRH_GLAUNCH("chained adjoint flags");
         const dim3 g(n * Tpad);
         if (k.hs <= 256)

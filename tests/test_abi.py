@@ -73,13 +73,18 @@ def test_pipelined_sweeps_reject_overlap_without_gpu():
     L = _lib.lib()
     base = 1 << 24  # never dereferenced: the checks fail first
     state = 2 * 2 * 64 * (16 + 20 - 1)  # T = 2 frames, 2 planes of the 16 x 20 skew
-    rc = L.frh_grad_sweeps(base, base + 8 * 10**6, base + 16 * 10**6, state - 1, 3, 1, 2, 16, 20, 1.0, 1.0, None)
+    rc = L.frh_grad_sweeps(base, base + 8 * 10**6, base + 16 * 10**6, state - 1, 3, 1, 2, 16, 20, 1.0, 1.0, None, 0, None)
     assert rc == -1 and b"out_stride" in L.frh_last_error()
     out = base + 8 * 10**6
-    rc = L.frh_grad_sweeps(base, out + 8 * state, out, state, 3, 1, 2, 16, 20, 1.0, 1.0, None)
+    rc = L.frh_grad_sweeps(base, out + 8 * state, out, state, 3, 1, 2, 16, 20, 1.0, 1.0, None, 0, None)
     assert rc == -1 and b"overlap" in L.frh_last_error()
-    rc = L.frh_grad_sweeps(base, out, out, state, 0, 0, 2, 16, 20, 1.0, 1.0, None)
+    rc = L.frh_grad_sweeps(base, out, out, state, 0, 0, 2, 16, 20, 1.0, 1.0, None, 0, None)
     assert rc == -1 and b"bad arguments" in L.frh_last_error()
+    # chained: a failure word and one 256-byte flag per stage and padded frame
+    assert L.frh_grad_sweeps_scratch_bytes(5, 2) == 256 + 5 * 8 * 256
+    assert L.frh_grad_sweeps_scratch_bytes(40, 9) == 256 + 32 * 16 * 256
+    rc = L.frh_grad_sweeps(base, out, out, 0, 5, 0, 2, 16, 20, 1.0, 1.0, base, 256 + 5 * 8 * 256 - 1, None)
+    assert rc == -1 and b"scratch of" in L.frh_last_error()
 
 
 def test_chained_adjoint_scratch_without_gpu():

@@ -253,40 +253,48 @@ def _sweeps_case(L, _lib, g, T, h, w, ax, ay):
 
 
 def test_pipelined_sweeps_equal_single_sweeps(gpu):
-    """frh_grad_sweeps (up to 5 sweeps per wavefront pass, LDS exchange of
-    every sweep's edge values between waves) against the same number of
-    single-sweep launches: bit-exact, for every state when write_all, for the
-    last when not (also in place), at 1..9 sweeps and levels of 1..700 rows
-    (the 1024-lane form runs single sweeps)."""
+    """frh_grad_sweeps against the same number of single-sweep launches,
+    bit-exact, for every state when write_all, for the last when not (also
+    in place), at 1..9 sweeps and levels of 1..700 rows (the 1024-lane form
+    at 700): without scratch (up to 5 sweeps per wavefront pass, LDS exchange
+    of every sweep's edge values between waves) and with scratch (a chain of
+    work-groups per frame under progress flags; frame counts not multiples
+    of 8, and 33 sweeps: two chain launches)."""
     from pyflowreg_amd import _lib
     g = torch.Generator().manual_seed(31)
     L = _lib.lib()
     st = _lib.stream_handle()
     ax, ay = 1.2, 0.8
-    for (h, w, T) in ((17, 21, 2), (1, 9, 2), (9, 1, 1), (1, 1, 1), (65, 7, 1), (130, 5, 2), (300, 40, 1),
-                      (512, 37, 1), (700, 12, 1)):
+    for (h, w, T, ns) in ((17, 21, 2, (1, 2, 3, 4, 5, 9)), (1, 9, 2, (2, 5)), (9, 1, 1, (2, 5)), (1, 1, 1, (3,)),
+                          (65, 7, 3, (2, 5, 9)), (130, 5, 2, (4, 5)), (300, 40, 1, (5,)), (512, 37, 9, (3, 5)),
+                          (700, 12, 1, (2, 3)), (19, 6, 2, (33,))):
         cd, xd = _sweeps_case(L, _lib, g, T, h, w, ax, ay)
-        for n in (1, 2, 3, 4, 5, 9):
+        for n in ns:
             ref = torch.empty((n,) + xd.shape, dtype=F64, device="cuda")
             src = xd
             for j in range(n):
                 _lib.check(L.frh_grad_sweep(cd.data_ptr(), src.data_ptr(), ref[j].data_ptr(), T, h, w, ax, ay,
                                             st), "sweep")
                 src = ref[j]
-            allx = torch.full_like(ref, float("nan"))
-            _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), allx.data_ptr(), xd.numel(), n, 1, T, h, w,
-                                         ax, ay, st), "sweeps (all)")
-            last = torch.full_like(xd, float("nan"))
-            _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), last.data_ptr(), 0, n, 0, T, h, w,
-                                         ax, ay, st), "sweeps (last)")
-            inplace = xd.clone()
-            _lib.check(L.frh_grad_sweeps(cd.data_ptr(), inplace.data_ptr(), inplace.data_ptr(), 0, n, 0, T, h, w,
-                                         ax, ay, st), "sweeps (in place)")
-            torch.cuda.synchronize()
-            # (padding slots are never written)
-            assert torch.equal(_unsk(allx, h, w), _unsk(ref, h, w)), (h, w, n)
-            assert torch.equal(_unsk(last, h, w), _unsk(ref[-1], h, w)), (h, w, n)
-            assert torch.equal(_unsk(inplace, h, w), _unsk(ref[-1], h, w)), (h, w, n)
+            nb = L.frh_grad_sweeps_scratch_bytes(n, T)
+            scratch = torch.zeros(nb, dtype=torch.uint8, device="cuda")
+            for sp, sb in ((None, 0), (scratch.data_ptr(), nb)):
+                case = (h, w, T, n, sp is not None)
+                allx = torch.full_like(ref, float("nan"))
+                _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), allx.data_ptr(), xd.numel(), n, 1, T, h,
+                                             w, ax, ay, sp, sb, st), "sweeps (all)")
+                last = torch.full_like(xd, float("nan"))
+                _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), last.data_ptr(), 0, n, 0, T, h, w,
+                                             ax, ay, sp, sb, st), "sweeps (last)")
+                inplace = xd.clone()
+                _lib.check(L.frh_grad_sweeps(cd.data_ptr(), inplace.data_ptr(), inplace.data_ptr(), 0, n, 0, T, h,
+                                             w, ax, ay, sp, sb, st), "sweeps (in place)")
+                torch.cuda.synchronize()
+                # (padding slots are never written)
+                assert torch.equal(_unsk(allx, h, w), _unsk(ref, h, w)), case
+                assert torch.equal(_unsk(last, h, w), _unsk(ref[-1], h, w)), case
+                assert torch.equal(_unsk(inplace, h, w), _unsk(ref[-1], h, w)), case
+            assert int(scratch[:4].view(torch.int32)[0]) == 0, (h, w, T, n)
     # against the torch sweep, 5 sweeps in one pass
     coef = torch.rand((1, 6, 70, 33), dtype=F64, generator=g) * 0.5
     x = torch.randn((1, 2, 70, 33), dtype=F64, generator=g)
@@ -295,8 +303,8 @@ def test_pipelined_sweeps_equal_single_sweeps(gpu):
         y = R.sor_sweep(coef, y, ax, ay)
     cd, xd = _cuda(_sk(coef), _sk(x))
     out = torch.empty_like(xd)
-    _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), 0, 5, 0, 1, 70, 33, ax, ay, st),
-               "sweeps")
+    _lib.check(L.frh_grad_sweeps(cd.data_ptr(), xd.data_ptr(), out.data_ptr(), 0, 5, 0, 1, 70, 33, ax, ay, None, 0,
+                                 st), "sweeps")
     torch.cuda.synchronize()
     assert _rel(_unsk(out, 70, 33), y) < 1e-12
 
