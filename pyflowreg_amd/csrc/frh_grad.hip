@@ -298,18 +298,40 @@ __global__ void k_gskew(const double *src, double *dst, int N, GSkew k, int to_s
         dst[(size_t)n * hw + (size_t)r * k.w + c] = src[(size_t)n * k.plane + q];
 }
 
-// One thread per cell in row-major order (the row-major J, W and u0 reads
-// are the many coalesced ones): x, coef skewed; u0, J, W row-major.
-__global__ __launch_bounds__(64) void k_gcoef_fwd(const double *x, const double *u0, const double *J,
+// Cell of a coefficient-kernel thread: a 256-lane work-group covers a tile
+// of 16 rows x 16 columns, each wave 4 rows of it, so a wave's row-major
+// accesses (J, W, u0 and their adjoints, the most) are 128-byte runs and its
+// skewed ones (state, coefficients and their adjoints) 19 anti-diagonal
+// runs instead of one cache line per lane (a wave on 1 x 64 cells); the
+// work-group's four waves continue each other's anti-diagonal runs, whose
+// cache lines they share.
+constexpr int kCoefTR = 16, kCoefTC = 16;
+
+__host__ __device__ inline unsigned coef_tiles(int h, int w)
+{
+    return (unsigned)((w + kCoefTC - 1) / kCoefTC) * (unsigned)((h + kCoefTR - 1) / kCoefTR);
+}
+
+__device__ inline bool coef_tile_cell(int h, int w, int &r, int &c)
+{
+    const int ntx = (w + kCoefTC - 1) / kCoefTC;
+    const int tx = (int)blockIdx.x % ntx, ty = (int)blockIdx.x / ntx;
+    r = ty * kCoefTR + (int)(threadIdx.x / kCoefTC);
+    c = tx * kCoefTC + (int)(threadIdx.x % kCoefTC);
+    return r < h && c < w;
+}
+
+// x, coef skewed; u0, J, W row-major.
+__global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_fwd(const double *x, const double *u0, const double *J,
                                                    const double *W, int T, int C, GSkew k, double ax,
                                                    double ay, const double *a_data, int psi_one, double *coef)
 {
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     const int h = k.h, w = k.w;
     const size_t hw = (size_t)h * w;
-    if (p >= hw) return;
-    const int r = (int)(p / (size_t)w), c = (int)(p % (size_t)w);
+    int r, c;
+    if (!coef_tile_cell(h, w, r, c)) return;
+    const size_t p = (size_t)r * w + c;
     const size_t q = (size_t)(r + c) * k.hs + r;
     double Jr[6 * kMaxC], Wr[kMaxC], a[kMaxC], K[6];
     for (int e = 0; e < 6; ++e)
@@ -343,17 +365,17 @@ __global__ __launch_bounds__(64) void k_gcoef_fwd(const double *x, const double 
 //   g_psi = W g_wp; psi = a p, p = (e + 1e-5)^(a - 1), e = max(E, 0):
 //   dpsi/da = p (1 + a ln(e + 1e-5)), dpsi/dE = a (a - 1) p / (e + 1e-5)
 //   (zero for E < 0); E is quadratic in (x, y) and linear in J.
-__global__ __launch_bounds__(64) void k_gcoef_bwd(const double *x, const double *u0, const double *J,
+__global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_bwd(const double *x, const double *u0, const double *J,
                                                    const double *W, const double *gK, int T, int C, GSkew k,
                                                    double ax, double ay, const double *a_data, int psi_one,
                                                    double *gx, double *gu0, double *gJ, double *gpar)
 {
-    const size_t p = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
     const int t = blockIdx.y;
     const int h = k.h, w = k.w;
     const size_t hw = (size_t)h * w, P = k.plane;
-    if (p >= hw) return;
-    const int r = (int)(p / (size_t)w), c = (int)(p % (size_t)w);
+    int r, c;
+    if (!coef_tile_cell(h, w, r, c)) return;
+    const size_t p = (size_t)r * w + c;
     const size_t q = (size_t)(r + c) * k.hs + r;
     const double xv = x[(size_t)t * 2 * P + q], yv = x[((size_t)t * 2 + 1) * P + q];
     double g[6];
@@ -1333,7 +1355,7 @@ int frh_grad_coef(const double *x, const double *u0, const double *J, const doub
                "frh_grad_coef: bad arguments (at most 2 channels and 1024 rows)");
     FRH_GCHECK(!g_coef || (g_x && g_J && g_par), "frh_grad_coef: g_x, g_J and g_par are required with g_coef");
     hipStream_t st = (hipStream_t)stream;
-    dim3 g((unsigned)(((size_t)h * w + 63) / 64), T), b(64);
+    dim3 g(coef_tiles(h, w), T), b(kCoefTR * kCoefTC);
     if (coef) {
         hipLaunchKernelGGL(k_gcoef_fwd, g, b, 0, st, x, u0, J, W, T, C, k, ax, ay, a_data, psi_one, coef);
         FRH_GLAUNCH("coefficient forward");
