@@ -386,8 +386,10 @@ int frh_grad_tensor(const double *fixed, const double *moving, int T, int C, int
  * weight W (C, h, w), ax = alpha_x / hx^2, ay = alpha_y / hy^2 and a_data
  * (C, device): coef (skewed, when != NULL).  With g_coef (skewed), adds the
  * pull-backs g_x (skewed), g_u0 (row-major, when u0 and g_u0 are not NULL),
- * g_J and per-cell partials g_par (T, 2 + C, plane) of ax, ay, a_k (cell
- * (r, c) at element r w + c of its plane; the rest untouched). */
+ * g_J and partial sums g_par (T, 2 + C, plane) of ax, ay, a_k over tiles of
+ * 16 x 16 cells (tile (i, j) -- rows 16 i.., columns 16 j.. -- at element
+ * i ceil(w / 16) + j of its plane; the rest untouched: sum a plane for the
+ * frame's total). */
 int frh_grad_coef(const double *x, const double *u0, const double *J, const double *W, int T, int C,
                   int h, int w, double ax, double ay, const double *a_data, int psi_one, double *coef,
                   const double *g_coef, double *g_x, double *g_u0, double *g_J, double *g_par,

@@ -322,9 +322,10 @@ __device__ inline bool coef_tile_cell(int h, int w, int &r, int &c)
 }
 
 // x, coef skewed; u0, J, W row-major.
+template <int C>
 __global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_fwd(const double *x, const double *u0, const double *J,
-                                                   const double *W, int T, int C, GSkew k, double ax,
-                                                   double ay, const double *a_data, int psi_one, double *coef)
+                                                                  const double *W, GSkew k, double ax, double ay,
+                                                                  const double *a_data, int psi_one, double *coef)
 {
     const int t = blockIdx.y;
     const int h = k.h, w = k.w;
@@ -333,9 +334,12 @@ __global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_fwd(const double *x
     if (!coef_tile_cell(h, w, r, c)) return;
     const size_t p = (size_t)r * w + c;
     const size_t q = (size_t)(r + c) * k.hs + r;
-    double Jr[6 * kMaxC], Wr[kMaxC], a[kMaxC], K[6];
+    double Jr[6 * C], Wr[C], a[C], K[6];
+#pragma unroll
     for (int e = 0; e < 6; ++e)
+#pragma unroll
         for (int kk = 0; kk < C; ++kk) Jr[e * C + kk] = J[(((size_t)t * 6 + e) * C + kk) * hw + p];
+#pragma unroll
     for (int kk = 0; kk < C; ++kk) {
         Wr[kk] = W[kk * hw + p];
         a[kk] = a_data[kk];
@@ -351,6 +355,7 @@ __global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_fwd(const double *x
     }
     const double *X = x + (size_t)t * 2 * k.plane;
     coef_cell(X[q], X[k.plane + q], ax, ay, a, Jr, Wr, C, psi_one, nlr, nud, cu, cv, K);
+#pragma unroll
     for (int e = 0; e < 6; ++e) coef[((size_t)t * 6 + e) * k.plane + q] = K[e];
 }
 
@@ -365,95 +370,125 @@ __global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_fwd(const double *x
 //   g_psi = W g_wp; psi = a p, p = (e + 1e-5)^(a - 1), e = max(E, 0):
 //   dpsi/da = p (1 + a ln(e + 1e-5)), dpsi/dE = a (a - 1) p / (e + 1e-5)
 //   (zero for E < 0); E is quadratic in (x, y) and linear in J.
+template <int C>
 __global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_bwd(const double *x, const double *u0, const double *J,
-                                                   const double *W, const double *gK, int T, int C, GSkew k,
-                                                   double ax, double ay, const double *a_data, int psi_one,
-                                                   double *gx, double *gu0, double *gJ, double *gpar)
+                                                                  const double *W, const double *gK, GSkew k,
+                                                                  double ax, double ay, const double *a_data,
+                                                                  int psi_one, double *gx, double *gu0, double *gJ,
+                                                                  double *gpar)
 {
+    constexpr int NW = kCoefTR * kCoefTC / 64;
+    __shared__ double red[NW][2 + C];
     const int t = blockIdx.y;
     const int h = k.h, w = k.w;
     const size_t hw = (size_t)h * w, P = k.plane;
     int r, c;
-    if (!coef_tile_cell(h, w, r, c)) return;
-    const size_t p = (size_t)r * w + c;
-    const size_t q = (size_t)(r + c) * k.hs + r;
-    const double xv = x[(size_t)t * 2 * P + q], yv = x[((size_t)t * 2 + 1) * P + q];
-    double g[6];
-    for (int e = 0; e < 6; ++e) g[e] = gK[((size_t)t * 6 + e) * P + q];
-    // Du, Dv at the refresh (recomputed)
-    const double ws = (ax + ax) + (ay + ay);
-    double Du = ws, Dv = ws;
-    double psi[kMaxC], Jc[6 * kMaxC];
-    for (int kk = 0; kk < C; ++kk) {
-        for (int e = 0; e < 6; ++e) Jc[e * kMaxC + kk] = J[(((size_t)t * 6 + e) * C + kk) * hw + p];
-        const double j11 = Jc[0 * kMaxC + kk], j22 = Jc[1 * kMaxC + kk], j33 = Jc[2 * kMaxC + kk];
-        const double j12 = Jc[3 * kMaxC + kk], j13 = Jc[4 * kMaxC + kk], j23 = Jc[5 * kMaxC + kk];
-        psi[kk] = psi_one ? 1.0
-                          : psi_of(j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
-                                       2.0 * (j13 * xv) + j33,
-                                   a_data[kk]);
-        const double wp = W[kk * hw + p] * psi[kk];
-        Du += wp * j11;
-        Dv += wp * j22;
-    }
-    const double gDu = -kOmega / (Du * Du) * g[3], gDv = -kOmega / (Dv * Dv) * g[4];
-    const double gws = gDu + gDv;
-    double nlr, nud;
-    edge_counts(h, w, r, c, nlr, nud);
-    double g_ax = 2.0 * gws + nlr * g[5], g_ay = 2.0 * gws + nud * g[5];
-    double g_x = 0.0, g_y = 0.0;
-    for (int kk = 0; kk < C; ++kk) {
-        const double j11 = Jc[0 * kMaxC + kk], j22 = Jc[1 * kMaxC + kk], j33 = Jc[2 * kMaxC + kk];
-        const double j12 = Jc[3 * kMaxC + kk], j13 = Jc[4 * kMaxC + kk], j23 = Jc[5 * kMaxC + kk];
-        const double Wk = W[kk * hw + p], wp = Wk * psi[kk];
-        double gj[6];
-        gj[0] = gDu * wp;
-        gj[1] = gDv * wp;
-        gj[2] = 0.0;
-        gj[3] = g[0] * wp;
-        gj[4] = -g[1] * wp;
-        gj[5] = -g[2] * wp;
-        double g_a = 0.0;
-        if (!psi_one) {
-            const double g_psi = Wk * (gDu * j11 + gDv * j22 + g[0] * j12 - g[1] * j13 - g[2] * j23);
-            const double E = j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
-                             2.0 * (j13 * xv) + j33;
-            const double a = a_data[kk], base = (E < 0 ? 0.0 : E) + 1e-5;
-            const double pw = pow(base, a - 1.0);
-            g_a = g_psi * pw * (1.0 + a * log(base));
-            const double gE = E < 0 ? 0.0 : g_psi * a * (a - 1.0) * pw / base;
-            gj[0] += gE * xv * xv;
-            gj[1] += gE * yv * yv;
-            gj[2] += gE;
-            gj[3] += gE * 2.0 * xv * yv;
-            gj[4] += gE * 2.0 * xv;
-            gj[5] += gE * 2.0 * yv;
-            g_x += gE * 2.0 * (j11 * xv + j12 * yv + j13);
-            g_y += gE * 2.0 * (j22 * yv + j12 * xv + j23);
+    const bool in_level = coef_tile_cell(h, w, r, c);
+    double gs[2 + C];  // this cell's partials of ax, ay, a_k
+#pragma unroll
+    for (int i = 0; i < 2 + C; ++i) gs[i] = 0.0;
+    if (in_level) {
+        const size_t p = (size_t)r * w + c;
+        const size_t q = (size_t)(r + c) * k.hs + r;
+        const double xv = x[(size_t)t * 2 * P + q], yv = x[((size_t)t * 2 + 1) * P + q];
+        double g[6];
+#pragma unroll
+        for (int e = 0; e < 6; ++e) g[e] = gK[((size_t)t * 6 + e) * P + q];
+        // Du, Dv at the refresh (recomputed)
+        const double ws = (ax + ax) + (ay + ay);
+        double Du = ws, Dv = ws;
+        double psi[C], Jc[6 * C], Wc[C];
+#pragma unroll
+        for (int kk = 0; kk < C; ++kk) {
+#pragma unroll
+            for (int e = 0; e < 6; ++e) Jc[e * C + kk] = J[(((size_t)t * 6 + e) * C + kk) * hw + p];
+            const double j11 = Jc[0 * C + kk], j22 = Jc[1 * C + kk], j33 = Jc[2 * C + kk];
+            const double j12 = Jc[3 * C + kk], j13 = Jc[4 * C + kk], j23 = Jc[5 * C + kk];
+            psi[kk] = psi_one ? 1.0
+                              : psi_of(j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
+                                           2.0 * (j13 * xv) + j33,
+                                       a_data[kk]);
+            Wc[kk] = W[kk * hw + p];
+            const double wp = Wc[kk] * psi[kk];
+            Du += wp * j11;
+            Dv += wp * j22;
         }
-        gpar[((size_t)t * (2 + C) + 2 + kk) * P + p] += g_a;
-        for (int e = 0; e < 6; ++e) gJ[(((size_t)t * 6 + e) * C + kk) * hw + p] += gj[e];
-    }
-    if (u0) {
-        const double *gBu = gK + ((size_t)t * 6 + 1) * P, *gBv = gK + ((size_t)t * 6 + 2) * P;
-        double sx, sy;
-        flow_diffs(u0 + (size_t)t * 2 * hw, h, w, r, c, sx, sy);
-        g_ax += g[1] * sx;
-        g_ay += g[1] * sy;
-        flow_diffs(u0 + ((size_t)t * 2 + 1) * hw, h, w, r, c, sx, sy);
-        g_ax += g[2] * sx;
-        g_ay += g[2] * sy;
-        if (gu0) {
-            flow_diffs_skew(gBu, h, w, k.hs, r, c, q, sx, sy);
-            gu0[((size_t)t * 2) * hw + p] += ax * sx + ay * sy;
-            flow_diffs_skew(gBv, h, w, k.hs, r, c, q, sx, sy);
-            gu0[((size_t)t * 2 + 1) * hw + p] += ax * sx + ay * sy;
+        const double gDu = -kOmega / (Du * Du) * g[3], gDv = -kOmega / (Dv * Dv) * g[4];
+        const double gws = gDu + gDv;
+        double nlr, nud;
+        edge_counts(h, w, r, c, nlr, nud);
+        double g_ax = 2.0 * gws + nlr * g[5], g_ay = 2.0 * gws + nud * g[5];
+        double g_x = 0.0, g_y = 0.0;
+#pragma unroll
+        for (int kk = 0; kk < C; ++kk) {
+            const double j11 = Jc[0 * C + kk], j22 = Jc[1 * C + kk], j33 = Jc[2 * C + kk];
+            const double j12 = Jc[3 * C + kk], j13 = Jc[4 * C + kk], j23 = Jc[5 * C + kk];
+            const double Wk = Wc[kk], wp = Wk * psi[kk];
+            double gj[6];
+            gj[0] = gDu * wp;
+            gj[1] = gDv * wp;
+            gj[2] = 0.0;
+            gj[3] = g[0] * wp;
+            gj[4] = -g[1] * wp;
+            gj[5] = -g[2] * wp;
+            if (!psi_one) {
+                const double g_psi = Wk * (gDu * j11 + gDv * j22 + g[0] * j12 - g[1] * j13 - g[2] * j23);
+                const double E = j11 * xv * xv + j22 * yv * yv + 2.0 * (j23 * yv) + 2.0 * (j12 * xv * yv) +
+                                 2.0 * (j13 * xv) + j33;
+                const double a = a_data[kk], base = (E < 0 ? 0.0 : E) + 1e-5;
+                const double pw = pow(base, a - 1.0);
+                gs[2 + kk] = g_psi * pw * (1.0 + a * log(base));
+                const double gE = E < 0 ? 0.0 : g_psi * a * (a - 1.0) * pw / base;
+                gj[0] += gE * xv * xv;
+                gj[1] += gE * yv * yv;
+                gj[2] += gE;
+                gj[3] += gE * 2.0 * xv * yv;
+                gj[4] += gE * 2.0 * xv;
+                gj[5] += gE * 2.0 * yv;
+                g_x += gE * 2.0 * (j11 * xv + j12 * yv + j13);
+                g_y += gE * 2.0 * (j22 * yv + j12 * xv + j23);
+            }
+#pragma unroll
+            for (int e = 0; e < 6; ++e) gJ[(((size_t)t * 6 + e) * C + kk) * hw + p] += gj[e];
         }
+        if (u0) {
+            const double *gBu = gK + ((size_t)t * 6 + 1) * P, *gBv = gK + ((size_t)t * 6 + 2) * P;
+            double sx, sy;
+            flow_diffs(u0 + (size_t)t * 2 * hw, h, w, r, c, sx, sy);
+            g_ax += g[1] * sx;
+            g_ay += g[1] * sy;
+            flow_diffs(u0 + ((size_t)t * 2 + 1) * hw, h, w, r, c, sx, sy);
+            g_ax += g[2] * sx;
+            g_ay += g[2] * sy;
+            if (gu0) {
+                flow_diffs_skew(gBu, h, w, k.hs, r, c, q, sx, sy);
+                gu0[((size_t)t * 2) * hw + p] += ax * sx + ay * sy;
+                flow_diffs_skew(gBv, h, w, k.hs, r, c, q, sx, sy);
+                gu0[((size_t)t * 2 + 1) * hw + p] += ax * sx + ay * sy;
+            }
+        }
+        gx[(size_t)t * 2 * P + q] += g_x;
+        gx[((size_t)t * 2 + 1) * P + q] += g_y;
+        gs[0] = g_ax;
+        gs[1] = g_ay;
     }
-    gx[(size_t)t * 2 * P + q] += g_x;
-    gx[((size_t)t * 2 + 1) * P + q] += g_y;
-    gpar[((size_t)t * (2 + C) + 0) * P + p] += g_ax;
-    gpar[((size_t)t * (2 + C) + 1) * P + p] += g_ay;
+    // the tile's sums of the ax, ay, a_k partials (a butterfly in each wave,
+    // then the waves in order), one addition per tile to gpar
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+#pragma unroll
+    for (int i = 0; i < 2 + C; ++i) {
+        double v = gs[i];
+#pragma unroll
+        for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+        if (lane == 0) red[wv][i] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2 + C) {
+        double v = red[0][threadIdx.x];
+#pragma unroll
+        for (int j = 1; j < NW; ++j) v += red[j][threadIdx.x];
+        gpar[((size_t)t * (2 + C) + threadIdx.x) * P + blockIdx.x] += v;
+    }
 }
 
 // --------------------------------------------------------------- chains
@@ -1357,12 +1392,19 @@ int frh_grad_coef(const double *x, const double *u0, const double *J, const doub
     hipStream_t st = (hipStream_t)stream;
     dim3 g(coef_tiles(h, w), T), b(kCoefTR * kCoefTC);
     if (coef) {
-        hipLaunchKernelGGL(k_gcoef_fwd, g, b, 0, st, x, u0, J, W, T, C, k, ax, ay, a_data, psi_one, coef);
+        if (C == 1)
+            hipLaunchKernelGGL(k_gcoef_fwd<1>, g, b, 0, st, x, u0, J, W, k, ax, ay, a_data, psi_one, coef);
+        else
+            hipLaunchKernelGGL(k_gcoef_fwd<2>, g, b, 0, st, x, u0, J, W, k, ax, ay, a_data, psi_one, coef);
         FRH_GLAUNCH("coefficient forward");
     }
     if (g_coef) {
-        hipLaunchKernelGGL(k_gcoef_bwd, g, b, 0, st, x, u0, J, W, g_coef, T, C, k, ax, ay, a_data, psi_one,
-                           g_x, g_u0, g_J, g_par);
+        if (C == 1)
+            hipLaunchKernelGGL(k_gcoef_bwd<1>, g, b, 0, st, x, u0, J, W, g_coef, k, ax, ay, a_data, psi_one, g_x,
+                               g_u0, g_J, g_par);
+        else
+            hipLaunchKernelGGL(k_gcoef_bwd<2>, g, b, 0, st, x, u0, J, W, g_coef, k, ax, ay, a_data, psi_one, g_x,
+                               g_u0, g_J, g_par);
         FRH_GLAUNCH("coefficient backward");
     }
     return FRH_OK;
