@@ -12,6 +12,7 @@
 
 #include "../../include/flowreg_hip.h"
 #include "frh_internal.h"
+#include "frh_device.h"
 
 using namespace frh;
 
@@ -576,6 +577,14 @@ struct frh_solver {
     int g_T = 0, g_out_f64 = 0, g_pairs = 0;
     float *stage_in = nullptr, *stage_fixed = nullptr;
     void *stage_out = nullptr;
+    // small batches: a pass's sweeps as a chain of work-groups (one per
+    // sweep and frame) when they fit on the device at once; progress flags,
+    // the failure word (host memory mapped to the device: set by a stage
+    // that gave up waiting, checked at the next run) and the work-group
+    // slots of the chain kernel
+    int *chain_flags = nullptr;
+    int *chain_err = nullptr, *chain_err_dev = nullptr;
+    int chain_slots = 0;
 };
 
 static void free_solver(frh_solver *s)
@@ -594,6 +603,8 @@ static void free_solver(frh_solver *s)
     if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
     if (s->cap_stream) (void)hipStreamDestroy(s->cap_stream);
     (void)hipFree(s->stage_in); (void)hipFree(s->stage_fixed); (void)hipFree(s->stage_out);
+    (void)hipFree(s->chain_flags);
+    if (s->chain_err) (void)hipHostFree(s->chain_err);
     delete s;
 }
 
@@ -732,6 +743,16 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
         rc = dev_alloc(&s->wlow, T * 2 * (size_t)hmax * wmax, "coarse flow");
         if (!rc) rc = dev_alloc(&s->wtmp, T * 2 * (size_t)hmax * d->W, "final resize scratch");
     }
+    if (!rc && !nl) {
+        s->chain_slots = sweep_chain_slots();
+        if (s->chain_slots >= 2) {
+            rc = dev_alloc(&s->chain_flags, (size_t)5 * chain_tpad(d->max_batch) * kChainFlagStride, "chain flags");
+            hipError_t e = rc ? hipSuccess : hipHostMalloc((void **)&s->chain_err, 256, hipHostMallocMapped);
+            if (!rc && e == hipSuccess) e = hipHostGetDevicePointer((void **)&s->chain_err_dev, s->chain_err, 0);
+            if (!rc && e != hipSuccess) rc = fail(FRH_EHIP, "chain failure word: %s", hipGetErrorString(e));
+            if (!rc) *s->chain_err = 0;
+        }
+    }
     if (rc) {
         free_solver(s);
         return rc;
@@ -803,9 +824,17 @@ int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving,
 
 }  // extern "C"
 
+// Levels with fewer anti-diagonals run their passes pipelined: the lag
+// between chain stages (a few publication intervals) would be a large part
+// of the pass.
+constexpr int kChainMinDiagonals = 256;
+
 static int solver_run(frh_solver *s, const float *fixed, const float *moving, int T, void *w_out,
                       int out_f64, hipStream_t st)
 {
+    if (s->chain_err && *(volatile int *)s->chain_err)
+        return fail(FRH_EHIP, "a chained sweep stage of an earlier run gave up waiting for its predecessor "
+                              "(that run's flows are invalid)");
     const int C = s->C;
     s->marks.clear();
     s->pool_used = 0;
@@ -909,8 +938,13 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
                                         -1.0, st), "assemble");
             }
             Scope sc(s, TM_SWEEP, st);
-            FRH_HIP(launch_sweep(p.nsweeps, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw, k == 0 ? 1 : 0, st),
-                    "sor sweep");
+            if (s->chain_err && sk.hs <= 512 && sk.h + sk.w - 1 >= kChainMinDiagonals && p.nsweeps >= 2 &&
+                T * p.nsweeps <= s->chain_slots)
+                FRH_HIP(launch_sweep_chain(p.nsweeps, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw, k == 0 ? 1 : 0,
+                                           s->chain_flags, s->chain_err_dev, st), "sor sweep (chain)");
+            else
+                FRH_HIP(launch_sweep(p.nsweeps, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw, k == 0 ? 1 : 0, st),
+                        "sor sweep");
         }
         {
             Scope sc(s, TM_MEDIAN, st);

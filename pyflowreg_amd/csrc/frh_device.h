@@ -75,4 +75,54 @@ __device__ inline unsigned slot_offset(bool row_ok, int d, int r, int w, int hs)
     return ok ? ((unsigned)d * (unsigned)hs + (unsigned)r) * 8u : kOutOfRange;
 }
 
+// Chains of work-groups.  Consecutive sweeps of a pass (or their adjoints)
+// can run as one work-group per sweep and frame, each stage consuming its
+// predecessor's output while that is still being produced: the producer
+// publishes how many steps it has completed in an agent-scope flag (a
+// release store every few steps, after its waves' stores have completed),
+// the consumer waits for the steps it needs before it loads their results,
+// and acquires.  Stages are numbered stage-major, so every producer has a
+// lower work-group index than its consumers and is dispatched first; frames
+// are padded to a multiple of 8 so that a frame's stages share an XCD
+// (work-group index modulo 8) and its L2.  A wait is bounded: one that gives
+// up sets *err and proceeds (its results are then invalid), so a launch
+// always ends.
+constexpr int kChainFlagStride = 64;  // ints: one flag per 256 bytes
+constexpr unsigned kChainSpin = 1u << 18;
+
+// Steps between two publications: each costs the producer a write-back of
+// its L2, so fewer are faster, but each adds to the lag between stages;
+// measured at 32 frames (512 x 512: 32 best, 8 takes 1.3x as long).
+__host__ __device__ constexpr int chain_publish_interval(int nd)
+{
+    return nd / 32 < 4 ? 4 : nd / 32 > 32 ? 32 : nd / 32;
+}
+
+// frames padded to a multiple of 8 (the XCD count)
+__host__ __device__ constexpr int chain_tpad(int T) { return (T + 7) / 8 * 8; }
+
+// Waits until the producer's flag is at least need (pre: the flag as
+// loaded one step earlier) and acquires; returns the flag value seen.
+__device__ inline int chain_acquire(int *flag, int need, int pre, int *err)
+{
+    int v = pre;
+    if (v < need) {
+        unsigned it = 0;
+        for (;;) {
+            v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
+            if (v >= need || ++it >= kChainSpin) break;
+            __builtin_amdgcn_s_sleep(2);
+        }
+        if (v < need) {
+            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+            v = 0x7fffffff;
+        }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return v;
+}
+
+// s_waitcnt vmcnt(0): this wave's stores have been performed
+__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 }  // namespace frh

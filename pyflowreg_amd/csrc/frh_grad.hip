@@ -491,52 +491,6 @@ __global__ __launch_bounds__(kCoefTR * kCoefTC) void k_gcoef_bwd(const double *x
     }
 }
 
-// --------------------------------------------------------------- chains
-// Chains of work-groups.  A lag block's sweeps (or their adjoints) can run
-// as one work-group per sweep and frame, each stage consuming its
-// predecessor's output while that is still being produced: the producer
-// publishes how many steps it has completed in an agent-scope flag (a
-// release store every few steps, after its waves' stores have
-// completed), the consumer waits for the steps it needs before it loads
-// their results, and acquires.  Stages are numbered stage-major, so every
-// producer has a lower work-group index than its consumers and is
-// dispatched first; frames are padded to a multiple of 8 so that a frame's
-// stages share an XCD (work-group index modulo 8) and its L2.  A wait is
-// bounded: one that gives up sets *err and proceeds (its results are then
-// invalid), so a launch always ends.
-// Steps between two publications: each costs the producer a write-back of
-// its L2, so fewer are faster, but each adds to the lag between stages;
-// measured at 32 frames (512 x 512: 32 best, 8 takes 1.3x as long).
-__host__ __device__ constexpr int chain_publish_interval(int nd)
-{
-    return nd / 32 < 4 ? 4 : nd / 32 > 32 ? 32 : nd / 32;
-}
-constexpr int kChainFlagStride = 64;  // ints: one flag per 256 bytes
-constexpr unsigned kChainSpin = 1u << 18;
-
-// Waits until the producer's flag is at least need (pre: the flag as
-// loaded one step earlier) and acquires; returns the flag value seen.
-__device__ inline int chain_acquire(int *flag, int need, int pre, int *err)
-{
-    int v = pre;
-    if (v < need) {
-        unsigned it = 0;
-        for (;;) {
-            v = __builtin_amdgcn_readfirstlane(__hip_atomic_load(flag, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT));
-            if (v >= need || ++it >= kChainSpin) break;
-            __builtin_amdgcn_s_sleep(2);
-        }
-        if (v < need) {
-            __hip_atomic_store(err, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            v = 0x7fffffff;
-        }
-    }
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return v;
-}
-
-// s_waitcnt vmcnt(0): this wave's stores have been performed
-__device__ inline void wait_stores() { __builtin_amdgcn_s_waitcnt(0x0F70); }
 
 // --------------------------------------------------------------- sweeps
 // One column-major lexicographic SOR sweep (core/level_solver.py:257-364),
@@ -1248,7 +1202,6 @@ using namespace frh::grad;
 // failure word and the flags.
 constexpr int kChainMax = 32;
 
-static size_t chain_tpad(int T) { return ((size_t)T + 7) / 8 * 8; }
 
 static size_t chain_scratch_bytes(int n, int T, const GSkew &k)
 {
@@ -1257,12 +1210,6 @@ static size_t chain_scratch_bytes(int n, int T, const GSkew &k)
     return 256 + (size_t)nc * chain_tpad(T) * kChainFlagStride * 4 + (size_t)planes * T * 2 * k.plane * 8;
 }
 
-// clears the progress flags (one int per kChainFlagStride)
-__global__ void k_gclear_flags(int *flags, int n)
-{
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) flags[(size_t)i * kChainFlagStride] = 0;
-}
 
 // Sweeps per pipelined launch; a lag block of more sweeps takes several.
 // At 512 lanes (two waves per SIMD, 256 registers) 5 sweeps spill
@@ -1461,9 +1408,8 @@ int frh_grad_sweeps(const double *coef, const double *x_in, double *x_out, size_
             ch.T = T;
             ch.Tpad = Tpad;
             ch.pub = chain_publish_interval(k.h + k.w - 1);
-            const int nflags = n * Tpad;
-            hipLaunchKernelGGL(k_gclear_flags, dim3((nflags + 255) / 256), dim3(256), 0, st, ch.flags, nflags);
-            FRH_GLAUNCH("chained sweeps flags");
+            if (hipError_t e = launch_clear_flags(ch.flags, n * Tpad, st); e != hipSuccess)
+                return grad_fail("chained sweeps flags", e);
             launch_gsweep_chain(coef, ch, k, ax, ay, st);
             FRH_GLAUNCH("chained sweeps");
             src = ch.xs + (size_t)(n - 1) * ch.sstride;
@@ -1573,9 +1519,8 @@ int frh_grad_sweeps_adjoint(const double *coef, const double *x0, const double *
         ch.Tpad = Tpad;
         ch.pub = chain_publish_interval(k.h + k.w - 1);
         const int set = g_coef_set && li == 0 ? 1 : 0;
-        const int nflags = n * Tpad;
-        hipLaunchKernelGGL(k_gclear_flags, dim3((nflags + 255) / 256), dim3(256), 0, st, flags, nflags);
-        FRH_GLAUNCH("chained adjoint flags");
+        if (hipError_t e = launch_clear_flags(flags, n * Tpad, st); e != hipSuccess)
+            return grad_fail("chained adjoint flags", e);
         const dim3 g(n * Tpad);
         if (k.hs <= 256)
             hipLaunchKernelGGL((k_gsweep_bwd<4, 4, true>), g, b, 0, st, coef, nullptr, nullptr, nullptr, g_coef,

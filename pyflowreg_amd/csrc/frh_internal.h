@@ -132,6 +132,10 @@ hipError_t launch_assemble_w32(const double *J, const float *Wsk, const double *
                                double *coef, size_t frame_stride, int T, int C, const Skew &s, double ax,
                                double ay, const double *a_data_minus1, const double *a_data, int psi_one,
                                int zero_init, double gnc_beta, hipStream_t st);
+hipError_t launch_sweep_chain(int n, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
+                              const Skew &s, double ax, double ay, int zero_init, int *flags, int *err,
+                              hipStream_t st);
+int sweep_chain_slots();
 hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
                         int T, const Skew &s, double ax, double ay, int zero_init,
                         hipStream_t st);
@@ -140,6 +144,8 @@ hipError_t launch_median_add(const double *duv, size_t frame_stride, const Skew 
                              size_t uv_frame_stride, float *out_f32, double *out_f64,
                              int out_interleaved, hipStream_t st);
 hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st);
+// zeroes n progress flags of a chain (one int per kChainFlagStride)
+hipError_t launch_clear_flags(int *flags, int n, hipStream_t st);
 hipError_t launch_broadcast_f32(const float *src, size_t n, float *dst, size_t dst_stride, int T,
                                 hipStream_t st);
 hipError_t launch_to_skew_f32f(const float *src, int N, const Skew &s, float *dst, hipStream_t st);

@@ -1553,6 +1553,20 @@ struct SweepArgs {
     const double *J, *Wsk, *cuv;
     AData ad;
     int psi_one;
+    // chain of work-groups (k_sweep<1, 512, 0, false, true>): one per sweep
+    // and frame; progress flags, failure word, sweeps, frames, padded
+    // frames, publication interval
+    int *cflags, *cerr;
+    int cn, cT, cTpad, cpub;
+};
+
+// A work-group's place in a chain: its predecessor's and its own progress
+// flag (null: none), the predecessor's progress it has acquired, the flag as
+// loaded one step earlier, steps to the next publication, diagonals, and
+// whether its start values are zero (the level's first pass, stage 0).
+struct ChainState {
+    int *fl_in, *fl_out;
+    int avail, fpre, pcount, nd, zero_init;
 };
 
 // C = 0: the pass reads precomputed coefficients; C >= 1: the pass forms
@@ -1651,11 +1665,11 @@ struct SweepBufs {
     unsigned plane_bytes;
 };
 
-template <int P, int MAXW, int C, int S, bool TT, bool BAND>
+template <int P, int MAXW, int C, int S, bool TT, bool BAND, bool CH>
 __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const SweepArgs &a, SweepBufs &bf,
                                            double *rdu, double *rdv,
                                            double (*xb)[MAXW][2][P + 1][2], int t, int r,
-                                           int lane, int wv, int nw, int Tend, size_t fo)
+                                           int lane, int wv, int nw, int Tend, size_t fo, ChainState &cs)
 {
     using St = SweepState<P, C, TT>;
     constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL;
@@ -1673,6 +1687,17 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     (void)Tend;
     const int h = a.h, w = a.w, hs = a.hs;
     const bool row_ok = r < h;
+    const int zero_init = CH ? cs.zero_init : a.zero_init;
+    if constexpr (CH) {
+        // the predecessor stores diagonal x at its step x; this step loads
+        // the start values of diagonal t + DL
+        if (cs.fl_in) {
+            const int need = min(t + DL + 1, cs.nd);
+            if (cs.avail < need)
+                cs.avail = chain_acquire(cs.fl_in, need, __builtin_amdgcn_readfirstlane(cs.fpre), a.cerr);
+            cs.fpre = __hip_atomic_load(cs.fl_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
     {
         if constexpr (C <= 0) {
             const int d = t + DK;
@@ -1690,7 +1715,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
             }
         }
         const int d = t + DL;
-        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
+        const unsigned q = zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
         st.Lu[(S + DL) % NL] = buf_load_f64(bf.duv, q, 0);
         st.Lv[(S + DL) % NL] = buf_load_f64(bf.duv, q, bf.plane_bytes);
         if constexpr (BAND) {
@@ -1831,23 +1856,35 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
             xb[wb][wv][1][p][1] = cv[p];
         }
     }
-    lds_barrier();
+    if constexpr (CH) {
+        // publish every cpub steps and after the last diagonal, once this
+        // step's stores of every wave have been performed
+        const bool pub = cs.fl_out && (--cs.pcount == 0 || t + 1 == cs.nd);
+        if (cs.pcount == 0) cs.pcount = a.cpub;
+        if (pub) wait_stores();
+        lds_barrier();
+        if (pub && r == 0)
+            __hip_atomic_store(cs.fl_out, min(t + 1, cs.nd), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+        lds_barrier();
+    }
     __builtin_amdgcn_sched_barrier(0);  // keep each step's loads within the step
 }
 
-template <int P, int MAXW, int C, bool TT, bool BAND, int... S>
+template <int P, int MAXW, int C, bool TT, bool BAND, bool CH, int... S>
 __device__ __forceinline__ void sweep_block(SweepState<P, C, TT> &st, const SweepArgs &a,
                                             SweepBufs &bf, double *rdu,
                                             double *rdv, double (*xb)[MAXW][2][P + 1][2], int t0,
-                                            int r, int lane, int wv, int nw, int Tend, size_t fo,
+                                            int r, int lane, int wv, int nw, int Tend, size_t fo, ChainState &cs,
                                             std::integer_sequence<int, S...>)
 {
-    (sweep_step<P, MAXW, C, S, TT, BAND>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend, fo), ...);
+    (sweep_step<P, MAXW, C, S, TT, BAND, CH>(st, a, bf, rdu, rdv, xb, t0 + S, r, lane, wv, nw, Tend, fo, cs),
+     ...);
 }
 
 // One wavefront pass over frame f by the work-group's first nthr threads
 // (a multiple of 64; threads past them have left the kernel).
-template <int P, int MAXT, int C, bool BAND>
+template <int P, int MAXT, int C, bool BAND, bool CH = false>
 __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
                                             double (*xb)[MAXT / 64][2][P + 1][2])
 {
@@ -1855,6 +1892,21 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     using St = SweepState<P, C, TT>;
     constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL, U = St::U;
     constexpr int MAXW = MAXT / 64;
+    static_assert(!CH || (P == 1 && C == 0 && !BAND), "a chain stage runs one sweep");
+    ChainState cs{nullptr, nullptr, 0, 0, 0, 0, a.zero_init};
+    bool record = a.ring != nullptr;  // border ring: the pass's last sweep
+    if constexpr (CH) {
+        // work-groups numbered stage-major: stage = sweep of the pass
+        const int stage = f / a.cTpad;
+        f -= stage * a.cTpad;
+        if (f >= a.cT) return;
+        cs.nd = a.h + a.w - 1;
+        cs.pcount = a.cpub;
+        cs.zero_init = a.zero_init && stage == 0;
+        if (stage > 0) cs.fl_in = a.cflags + (size_t)((stage - 1) * a.cTpad + f) * kChainFlagStride;
+        if (stage < a.cn - 1) cs.fl_out = a.cflags + (size_t)(stage * a.cTpad + f) * kChainFlagStride;
+        record = record && stage == a.cn - 1;
+    }
     const int r = threadIdx.x;
     const int lane = r & 63;
     const int wv = __builtin_amdgcn_readfirstlane(r >> 6);
@@ -1862,8 +1914,8 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     const size_t fo = (size_t)f * a.fstride;
     const double *cb = a.coef + fo;
     double *du = a.duv + fo;
-    double *rdu = a.ring ? a.ring + fo : nullptr;
-    double *rdv = a.ring ? rdu + a.plane : nullptr;
+    double *rdu = record ? a.ring + fo : nullptr;
+    double *rdv = record ? rdu + a.plane : nullptr;
     for (int k = threadIdx.x; k < 2 * MAXW * 2 * (P + 1) * 2; k += nthr)
         (&xb[0][0][0][0][0])[k] = 0.0;
     const int h = a.h, w = a.w, hs = a.hs;
@@ -1887,9 +1939,12 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     bf.plane_bytes = (unsigned)(a.plane * sizeof(double));
     bf.coef = make_rsrc(cb, C <= 0 ? St::NKC * bf.plane_bytes : 0u);
     bf.duv = make_rsrc(du, 2 * bf.plane_bytes);
+    if constexpr (CH) {
+        if (cs.fl_in) cs.avail = chain_acquire(cs.fl_in, min(DL, cs.nd), 0, a.cerr);
+    }
 #pragma unroll
     for (int d = 0; d < DL; ++d) {
-        const unsigned q = a.zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
+        const unsigned q = cs.zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
         st.Lu[d] = buf_load_f64(bf.duv, q, 0);
         st.Lv[d] = buf_load_f64(bf.duv, q, bf.plane_bytes);
     }
@@ -1919,15 +1974,15 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     __syncthreads();
     const int Tend = (h + w - 2) + 2 * (P - 1);
     for (int t0 = 0; t0 <= Tend; t0 += U)
-        sweep_block<P, MAXW, C, TT, BAND>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend, fo,
-                                          std::make_integer_sequence<int, U>{});
+        sweep_block<P, MAXW, C, TT, BAND, CH>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend, fo, cs,
+                                              std::make_integer_sequence<int, U>{});
 }
 
-template <int P, int MAXT, int C, bool BAND = false>
+template <int P, int MAXT, int C, bool BAND = false, bool CH = false>
 __global__ __launch_bounds__(MAXT) void k_sweep(SweepArgs a)
 {
     __shared__ double xb[2][MAXT / 64][2][P + 1][2];
-    sweep_frame<P, MAXT, C, BAND>(a, blockIdx.x, blockDim.x, xb);
+    sweep_frame<P, MAXT, C, BAND, CH>(a, blockIdx.x, blockDim.x, xb);
 }
 
 template <int P, int C>
@@ -1985,6 +2040,49 @@ hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, si
     case 5: return launch_sweep_p<5>(a, 0, T, st);
     default: return hipErrorInvalidValue;
     }
+}
+
+// n sweeps of a pass as one chain of work-groups (levels of up to 512 rows,
+// linear smoothness); flags: n * chain_tpad(T) progress flags.
+hipError_t launch_sweep_chain(int n, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
+                              const Skew &s, double ax, double ay, int zero_init, int *flags, int *err,
+                              hipStream_t st)
+{
+    SweepArgs a{};
+    a.coef = coef; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
+    a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
+    a.r0 = 0;
+    a.hlev = a.h;
+    a.cflags = flags;
+    a.cerr = err;
+    a.cn = n;
+    a.cT = T;
+    a.cTpad = chain_tpad(T);
+    a.cpub = chain_publish_interval(s.h + s.w - 1);
+    if (n < 1 || s.hs > 512 || a.plane * sizeof(double) * kCoefPlanes >= kOutOfRange ||
+        (size_t)n * a.cTpad > 0x7fffffff)
+        return hipErrorInvalidValue;
+    hipError_t e = launch_clear_flags(flags, n * a.cTpad, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_sweep<1, 512, 0, false, true>), dim3(n * a.cTpad), dim3(a.hs), 0, st, a);
+    return hipGetLastError();
+}
+
+// Work-groups of a sweep chain that run at once, one per compute unit (0:
+// no device).  Two chain stages on one compute unit share its SIMDs and
+// measured slower than the pipelined pass (64 frames: 96.9 against 53.8 ms
+// of sweeps per step).
+int sweep_chain_slots()
+{
+    int dev = 0, ncu = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, (const void *)k_sweep<1, 512, 0, false, true>, 512, 0) !=
+            hipSuccess ||
+        per < 1)
+        return 0;
+    return ncu;
 }
 
 hipError_t launch_sweep_nl(int P, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
@@ -2255,6 +2353,19 @@ hipError_t launch_fill_f64(double *p, size_t n, double v, hipStream_t st)
     if (blocks > 65536) blocks = 65536;
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_fill_f64, dim3((unsigned)blocks), dim3(256), 0, st, p, n, v);
+    return hipGetLastError();
+}
+
+__global__ void k_clear_flags(int *flags, int n)
+{
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) flags[(size_t)i * kChainFlagStride] = 0;
+}
+
+hipError_t launch_clear_flags(int *flags, int n, hipStream_t st)
+{
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_clear_flags, dim3((n + 255) / 256), dim3(256), 0, st, flags, n);
     return hipGetLastError();
 }
 
