@@ -743,10 +743,12 @@ int frh_solver_create(frh_solver **out, const frh_solver_desc *d)
         rc = dev_alloc(&s->wlow, T * 2 * (size_t)hmax * wmax, "coarse flow");
         if (!rc) rc = dev_alloc(&s->wtmp, T * 2 * (size_t)hmax * d->W, "final resize scratch");
     }
-    if (!rc && !nl) {
+    if (!rc) {
         s->chain_slots = sweep_chain_slots();
         if (s->chain_slots >= 2) {
-            rc = dev_alloc(&s->chain_flags, (size_t)5 * chain_tpad(d->max_batch) * kChainFlagStride, "chain flags");
+            // at most chain_slots frames x stages, frames padded to a multiple of 8
+            const size_t nflags = std::max((size_t)8 * s->chain_slots, (size_t)5 * chain_tpad(d->max_batch));
+            rc = dev_alloc(&s->chain_flags, nflags * kChainFlagStride, "chain flags");
             hipError_t e = rc ? hipSuccess : hipHostMalloc((void **)&s->chain_err, 256, hipHostMallocMapped);
             if (!rc && e == hipSuccess) e = hipHostGetDevicePointer((void **)&s->chain_err_dev, s->chain_err, 0);
             if (!rc && e != hipSuccess) rc = fail(FRH_EHIP, "chain failure word: %s", hipGetErrorString(e));
@@ -823,6 +825,9 @@ int frh_solver_run_pairs(frh_solver *s, const float *fixed, const float *moving,
 }
 
 }  // extern "C"
+
+// Rows of a band in band mode (levels of more than 1024 rows).
+constexpr int kBandRowsHost = 512;
 
 // Levels with fewer anti-diagonals run their passes pipelined: the lag
 // between chain stages (a few publication intervals) would be a large part
@@ -910,8 +915,48 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         const size_t ps_stride = (size_t)(h + 2) * (w + 2);
         if (nl && !L.passes.empty() && L.passes[0].psi_one)
             FRH_HIP(launch_fill_f64(s->psis, (size_t)T * ps_stride, 1.0, st), "psi_s init");
+        // levels of more than 1024 rows: a lag block's single-sweep passes as
+        // chains of work-groups (sweeps x bands) when a sweep's bands of all
+        // frames fit one work-group per compute unit
+        const int nbands = (h + kBandRowsHost - 1) / kBandRowsHost;
+        const bool band_chain = s->chain_err && sk.hs > 1024 && T * nbands <= s->chain_slots;
         for (size_t k = 0; k < L.passes.size(); ++k) {
             const Pass &p = L.passes[k];
+            if (band_chain && p.assemble) {
+                if (nl) {
+                    Scope sc(s, TM_ASSEMBLE, st);
+                    if (!p.psi_one)
+                        FRH_HIP(launch_psi_smooth(s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr, duv, ring, fs, T, sk,
+                                                  k == 0 ? 1 : 0, s->a_smooth, -1.0, L.h_col, L.h_row, s->psis,
+                                                  ps_stride, st), "smoothness weights");
+                    FRH_HIP(launch_assemble_nl(J, s->Wlev + L.W_off, 1, s->uvl, s->uvl + hw, 2 * hw, nullptr, nullptr,
+                                               s->psis, ps_stride, duv, coef, fs, T, C, sk, L.axw, L.ayw,
+                                               s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
+                                               -1.0, st), "assemble (nonlinear smoothness)");
+                } else {
+                    Scope sc(s, TM_ASSEMBLE, st);
+                    FRH_HIP(launch_assemble_w32(J, s->Wlev + L.W_off, cuv, duv, coef, fs, T, C, sk, L.axw, L.ayw,
+                                                s->am1.data(), s->a_data.data(), p.psi_one ? 1 : 0, k == 0 ? 1 : 0,
+                                                -1.0, st), "assemble");
+                }
+                size_t m = 1;
+                while (k + m < L.passes.size() && !L.passes[k + m].assemble) ++m;
+                int left = 0;
+                for (size_t i = 0; i < m; ++i) left += L.passes[k + i].nsweeps;
+                const int per = s->chain_slots / (T * nbands);  // sweeps per launch
+                int first = k == 0 ? 1 : 0;
+                Scope sc(s, TM_SWEEP, st);
+                while (left > 0) {
+                    const int n = left < per ? left : per;
+                    FRH_HIP(launch_sweep_band_chain(n, nl ? 1 : 0, coef, duv, nl ? ring : nullptr, fs, T, sk, L.axw,
+                                                    L.ayw, first, s->chain_flags, s->chain_err_dev, st),
+                            "sor sweep (band chain)");
+                    first = 0;
+                    left -= n;
+                }
+                k += m - 1;
+                continue;
+            }
             if (nl) {
                 // a_smooth != 1: psi_s refresh, nine-coefficient assembly and
                 // sweeps that record the border ring for the next refresh

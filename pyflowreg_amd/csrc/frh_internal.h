@@ -135,6 +135,9 @@ hipError_t launch_assemble_w32(const double *J, const float *Wsk, const double *
 hipError_t launch_sweep_chain(int n, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
                               const Skew &s, double ax, double ay, int zero_init, int *flags, int *err,
                               hipStream_t st);
+hipError_t launch_sweep_band_chain(int n, int nl, const double *coef, double *duv, double *ring,
+                                   size_t frame_stride, int T, const Skew &s, double ax, double ay, int zero_init,
+                                   int *flags, int *err, hipStream_t st);
 int sweep_chain_slots();
 hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, size_t frame_stride,
                         int T, const Skew &s, double ax, double ay, int zero_init,

@@ -1558,6 +1558,7 @@ struct SweepArgs {
     // frames, publication interval
     int *cflags, *cerr;
     int cn, cT, cTpad, cpub;
+    int cnb;  // band mode: bands per sweep (stages = cn x cnb, sweep-major)
 };
 
 // A work-group's place in a chain: its predecessor's and its own progress
@@ -1567,6 +1568,10 @@ struct SweepArgs {
 struct ChainState {
     int *fl_in, *fl_out;
     int avail, fpre, pcount, nd, zero_init;
+    // band mode: the band above in the same sweep (its new bottom row) and
+    // the band below in the previous sweep (its top row's start values)
+    int *fl_top = nullptr, *fl_bot = nullptr;
+    int avail_top = 0, fpre_top = 0, nd_top = 0, avail_bot = 0, fpre_bot = 0, nd_bot = 0;
 };
 
 // C = 0: the pass reads precomputed coefficients; C >= 1: the pass forms
@@ -1632,14 +1637,14 @@ struct SweepState {
 // Band mode: byte offset of the value the band's top lane (new value of the
 // row above) or bottom lane (start value of the row below) needs at step t,
 // or kOutOfRange.
-__device__ inline unsigned band_offset(const SweepArgs &a, int r, int t)
+__device__ inline unsigned band_offset(const SweepArgs &a, int r, int t, int zero_init)
 {
     const int h = a.h, w = a.w;
     unsigned q = kOutOfRange;
     if (r == 0 && a.r0 > 0) {
         const int c = t;
         if ((unsigned)c < (unsigned)w) q = ((unsigned)(a.r0 - 1 + c) * (unsigned)a.hs + (unsigned)(a.r0 - 1)) * 8u;
-    } else if (r == h - 1 && a.r0 + h < a.hlev && !a.zero_init) {
+    } else if (r == h - 1 && a.r0 + h < a.hlev && !zero_init) {
         // (on a level's first pass every start value is 0: the memory still
         // holds the previous level's values)
         const int c = t - (h - 1);
@@ -1665,6 +1670,37 @@ struct SweepBufs {
     unsigned plane_bytes;
 };
 
+// Chain waits before step t's loads (t = -1: the prologue's): the start
+// values of diagonal t + DL from the predecessor sweep of the same rows,
+// which stores diagonal x at its step x; in band mode (bands of kBandRows
+// rows) also the new values of the row above (column t + kBandDR, the band
+// above's diagonal t + kBandDR + kBandRows - 1) and the start values of the
+// row below (column t + kBandDR - (h - 1), the lower band's diagonal of the
+// same number, from the previous sweep).
+constexpr int kBandRows = 512;
+
+template <int DL>
+__device__ __forceinline__ void chain_wait(ChainState &cs, const SweepArgs &a, int t)
+{
+    if (cs.fl_in) {
+        const int need = min(t + DL + 1, cs.nd);
+        if (cs.avail < need) cs.avail = chain_acquire(cs.fl_in, need, __builtin_amdgcn_readfirstlane(cs.fpre), a.cerr);
+        cs.fpre = __hip_atomic_load(cs.fl_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (cs.fl_top) {
+        const int need = min(t + kBandDR + kBandRows, cs.nd_top);
+        if (cs.avail_top < need)
+            cs.avail_top = chain_acquire(cs.fl_top, need, __builtin_amdgcn_readfirstlane(cs.fpre_top), a.cerr);
+        cs.fpre_top = __hip_atomic_load(cs.fl_top, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    if (cs.fl_bot) {
+        const int need = min(t + kBandDR - a.h + 2, cs.nd_bot);
+        if (cs.avail_bot < need)
+            cs.avail_bot = chain_acquire(cs.fl_bot, need, __builtin_amdgcn_readfirstlane(cs.fpre_bot), a.cerr);
+        cs.fpre_bot = __hip_atomic_load(cs.fl_bot, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
 template <int P, int MAXW, int C, int S, bool TT, bool BAND, bool CH>
 __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const SweepArgs &a, SweepBufs &bf,
                                            double *rdu, double *rdv,
@@ -1688,16 +1724,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     const int h = a.h, w = a.w, hs = a.hs;
     const bool row_ok = r < h;
     const int zero_init = CH ? cs.zero_init : a.zero_init;
-    if constexpr (CH) {
-        // the predecessor stores diagonal x at its step x; this step loads
-        // the start values of diagonal t + DL
-        if (cs.fl_in) {
-            const int need = min(t + DL + 1, cs.nd);
-            if (cs.avail < need)
-                cs.avail = chain_acquire(cs.fl_in, need, __builtin_amdgcn_readfirstlane(cs.fpre), a.cerr);
-            cs.fpre = __hip_atomic_load(cs.fl_in, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
+    if constexpr (CH) chain_wait<DL>(cs, a, t);
     {
         if constexpr (C <= 0) {
             const int d = t + DK;
@@ -1719,7 +1746,7 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
         st.Lu[(S + DL) % NL] = buf_load_f64(bf.duv, q, 0);
         st.Lv[(S + DL) % NL] = buf_load_f64(bf.duv, q, bf.plane_bytes);
         if constexpr (BAND) {
-            const unsigned qb = band_offset(a, r, t + kBandDR);
+            const unsigned qb = band_offset(a, r, t + kBandDR, zero_init);
             st.Bu[(S + kBandDR) % kBandNR] = buf_load_f64(bf.duv, qb, 0);
             st.Bv[(S + kBandDR) % kBandNR] = buf_load_f64(bf.duv, qb, bf.plane_bytes);
         }
@@ -1884,29 +1911,14 @@ __device__ __forceinline__ void sweep_block(SweepState<P, C, TT> &st, const Swee
 
 // One wavefront pass over frame f by the work-group's first nthr threads
 // (a multiple of 64; threads past them have left the kernel).
-template <int P, int MAXT, int C, bool BAND, bool CH = false>
-__device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
-                                            double (*xb)[MAXT / 64][2][P + 1][2])
+template <int P, int MAXT, int C, bool BAND, bool CH>
+__device__ __forceinline__ void sweep_frame_body(const SweepArgs &a, int f, int nthr,
+                                                 double (*xb)[MAXT / 64][2][P + 1][2], ChainState &cs, bool record)
 {
     constexpr bool TT = MAXT > 512 && C == 0 && P >= 2;  // the 1-sweep form fits without it
     using St = SweepState<P, C, TT>;
     constexpr int NK = St::NK, NL = St::NL, DK = St::DK, DL = St::DL, U = St::U;
     constexpr int MAXW = MAXT / 64;
-    static_assert(!CH || (P == 1 && C == 0 && !BAND), "a chain stage runs one sweep");
-    ChainState cs{nullptr, nullptr, 0, 0, 0, 0, a.zero_init};
-    bool record = a.ring != nullptr;  // border ring: the pass's last sweep
-    if constexpr (CH) {
-        // work-groups numbered stage-major: stage = sweep of the pass
-        const int stage = f / a.cTpad;
-        f -= stage * a.cTpad;
-        if (f >= a.cT) return;
-        cs.nd = a.h + a.w - 1;
-        cs.pcount = a.cpub;
-        cs.zero_init = a.zero_init && stage == 0;
-        if (stage > 0) cs.fl_in = a.cflags + (size_t)((stage - 1) * a.cTpad + f) * kChainFlagStride;
-        if (stage < a.cn - 1) cs.fl_out = a.cflags + (size_t)(stage * a.cTpad + f) * kChainFlagStride;
-        record = record && stage == a.cn - 1;
-    }
     const int r = threadIdx.x;
     const int lane = r & 63;
     const int wv = __builtin_amdgcn_readfirstlane(r >> 6);
@@ -1939,9 +1951,7 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     bf.plane_bytes = (unsigned)(a.plane * sizeof(double));
     bf.coef = make_rsrc(cb, C <= 0 ? St::NKC * bf.plane_bytes : 0u);
     bf.duv = make_rsrc(du, 2 * bf.plane_bytes);
-    if constexpr (CH) {
-        if (cs.fl_in) cs.avail = chain_acquire(cs.fl_in, min(DL, cs.nd), 0, a.cerr);
-    }
+    if constexpr (CH) chain_wait<DL>(cs, a, -1);
 #pragma unroll
     for (int d = 0; d < DL; ++d) {
         const unsigned q = cs.zero_init ? kOutOfRange : slot_offset(row_ok, d + R0, r + R0, w, hs);
@@ -1951,7 +1961,7 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     if constexpr (BAND) {
 #pragma unroll
         for (int k = 0; k < kBandDR; ++k) {
-            const unsigned qb = band_offset(a, r, k);
+            const unsigned qb = band_offset(a, r, k, cs.zero_init);
             st.Bu[k] = buf_load_f64(bf.duv, qb, 0);
             st.Bv[k] = buf_load_f64(bf.duv, qb, bf.plane_bytes);
         }
@@ -1976,6 +1986,50 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
     for (int t0 = 0; t0 <= Tend; t0 += U)
         sweep_block<P, MAXW, C, TT, BAND, CH>(st, a, bf, rdu, rdv, xb, t0, r, lane, wv, nw, Tend, fo, cs,
                                               std::make_integer_sequence<int, U>{});
+}
+
+template <int P, int MAXT, int C, bool BAND, bool CH = false>
+__device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
+                                            double (*xb)[MAXT / 64][2][P + 1][2])
+{
+    static_assert(!CH || (P == 1 && (BAND ? C <= 0 : C == 0)), "a chain stage runs one sweep");
+    ChainState cs{nullptr, nullptr, 0, 0, 0, 0, a.zero_init};
+    bool record = a.ring != nullptr;  // border ring: the pass's last sweep
+    if constexpr (CH) {
+        // work-groups numbered stage-major: stage = sweep j of the pass (and
+        // band b of the level in band mode: stage j nb + b)
+        const int stage = f / a.cTpad, nb = BAND ? a.cnb : 1;
+        f -= stage * a.cTpad;
+        if (f >= a.cT) return;
+        const int j = stage / nb, b = stage - j * nb;
+        auto flag = [&](int sj, int sb) {
+            return a.cflags + (size_t)((sj * nb + sb) * a.cTpad + f) * kChainFlagStride;
+        };
+        cs.pcount = a.cpub;
+        cs.zero_init = a.zero_init && j == 0;
+        if (j > 0) cs.fl_in = flag(j - 1, b);
+        if (j < a.cn - 1 || b < nb - 1) cs.fl_out = flag(j, b);
+        record = record && j == a.cn - 1;
+        if constexpr (BAND) {
+            SweepArgs ab = a;
+            ab.r0 = b * kBandRows;
+            ab.h = a.hlev - ab.r0 < kBandRows ? a.hlev - ab.r0 : kBandRows;
+            cs.nd = ab.h + a.w - 1;
+            if (b > 0) {
+                cs.fl_top = flag(j, b - 1);
+                cs.nd_top = kBandRows + a.w - 1;
+            }
+            if (j > 0 && b < nb - 1) {
+                cs.fl_bot = flag(j - 1, b + 1);
+                const int hb = a.hlev - ab.r0 - kBandRows;
+                cs.nd_bot = (hb < kBandRows ? hb : kBandRows) + a.w - 1;
+            }
+            sweep_frame_body<P, MAXT, C, BAND, CH>(ab, f, nthr, xb, cs, record);
+            return;
+        }
+        cs.nd = a.h + a.w - 1;
+    }
+    sweep_frame_body<P, MAXT, C, BAND, CH>(a, f, nthr, xb, cs, record);
 }
 
 template <int P, int MAXT, int C, bool BAND = false, bool CH = false>
@@ -2065,6 +2119,41 @@ hipError_t launch_sweep_chain(int n, const double *coef, double *duv, double *ri
     hipError_t e = launch_clear_flags(flags, n * a.cTpad, st);
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_sweep<1, 512, 0, false, true>), dim3(n * a.cTpad), dim3(a.hs), 0, st, a);
+    return hipGetLastError();
+}
+
+// n sweeps of a level of more than 1024 rows (band mode) as one chain of
+// work-groups: stages (sweep j, band b of kBandRows rows), sweep-major, each
+// waiting for the band above in its sweep and for its own and the lower
+// band's rows in the previous sweep; nl: nine coefficients per cell
+// (a_smooth != 1).  flags: n * bands * chain_tpad(T) progress flags.
+hipError_t launch_sweep_band_chain(int n, int nl, const double *coef, double *duv, double *ring,
+                                   size_t frame_stride, int T, const Skew &s, double ax, double ay, int zero_init,
+                                   int *flags, int *err, hipStream_t st)
+{
+    SweepArgs a{};
+    a.coef = coef; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
+    a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
+    a.r0 = 0;
+    a.hlev = s.h;
+    a.cflags = flags;
+    a.cerr = err;
+    a.cn = n;
+    a.cnb = (s.h + kBandRows - 1) / kBandRows;
+    a.cT = T;
+    a.cTpad = chain_tpad(T);
+    a.cpub = chain_publish_interval(kBandRows + s.w - 1);
+    const size_t stages = (size_t)n * a.cnb;
+    if (n < 1 || s.hs <= 1024 || a.plane * sizeof(double) * (nl ? 9 : kCoefPlanes) >= kOutOfRange ||
+        stages * a.cTpad > 0x7fffffff)
+        return hipErrorInvalidValue;
+    hipError_t e = launch_clear_flags(flags, (int)(stages * a.cTpad), st);
+    if (e != hipSuccess) return e;
+    const dim3 g((unsigned)(stages * a.cTpad)), b(kBandRows);
+    if (nl)
+        hipLaunchKernelGGL((k_sweep<1, 512, kSmoothNL, true, true>), g, b, 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sweep<1, 512, 0, true, true>), g, b, 0, st, a);
     return hipGetLastError();
 }
 

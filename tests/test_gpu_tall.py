@@ -1,9 +1,12 @@
-"""Levels of more than 1024 rows (band mode: one sweep per pass, swept in
-bands of 512 rows, one launch per band from the top) against the oracle,
-bit for bit: the stand-alone level solver (constant and nonlinear
-smoothness, band boundaries at rows 512 and 1024), get_displacement on a tall
-narrow frame (two levels over 1024 rows), the batched solver, and a
-2048x2048x1 pair (four levels over 1024 rows)."""
+"""Levels of more than 1024 rows (band mode: sweeps in bands of 512 rows)
+against the oracle, bit for bit: the stand-alone level solver (one launch
+per band from the top; constant and nonlinear smoothness, band boundaries at
+rows 512 and 1024), get_displacement on a tall narrow frame (two levels over
+1024 rows), the batched solver, and a 2048x2048x1 pair (four levels over 1024
+rows).  The batched solver runs a lag block's sweeps of such a level as a
+chain of work-groups (sweeps x bands) when they fit the device: the batch of
+30 splits each block into launches of 2 sweeps, and nonlinear smoothness
+runs its nine-coefficient form with the border ring."""
 
 import sys
 
@@ -97,3 +100,48 @@ def test_2048x2048x1_bit_exact(gpu, oracle):
     d = np.abs(a - b)
     assert d.max() == 0.0, f"2048x2048x1: max |dflow| {d.max():.3e}"
     assert np.abs(b).max() > 0.5
+
+
+def test_tall_batch_chains_bit_exact(gpu, oracle):
+    """30 frames of 1300 x 20 x 1: the 1300- and 1040-row levels' lag blocks
+    run as chains of 3 bands x 2 sweeps per launch (a 5-sweep block takes 3
+    launches, the first of a level with zero start values)."""
+    import torch
+    from pyflowreg_amd import synth
+    from pyflowreg_amd.optical_flow import DisplacementSolver
+    T, H, W = 30, 1300, 20
+    frames, _ = synth.video(T + 1, H, W, n_channels=1, max_disp=3.0, sigma_d=32.0, seed=5111)
+    f = frames[0].numpy()
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    movs = oracle.preprocess(frames[1:].numpy(), f.astype(np.float64)).astype(np.float32)
+    _say("1300x20 x 30: GPU")
+    s = DisplacementSolver(H, W, 1, max_batch=T)
+    s.set_reference(ref, None, None)
+    out = s.run(torch.from_numpy(movs).cuda(), out_dtype=torch.float64).cpu().numpy()
+    s.close()
+    for t in (0, 13, 29):
+        _say(f"1300x20: oracle frame {t}")
+        d = np.abs(out[t] - oracle.get_displacement(ref, movs[t]))
+        assert d.max() == 0.0, f"frame {t}: max |dflow| {d.max():.3e}"
+
+
+def test_tall_batch_nonlinear_smoothness_bit_exact(gpu, oracle):
+    """a_smooth = 0.7 on 2 frames of 1100 x 40 x 2: the 1100-row level's
+    chains carry nine coefficients per cell and record the border ring that
+    the next psi_s refresh reads."""
+    import torch
+    from pyflowreg_amd.optical_flow import DisplacementSolver
+    f, m = _pair(1100, 40, 2, 5121)
+    _, m2 = _pair(1100, 40, 2, 5122)
+    ref = oracle.preprocess(f.astype(np.float64)).astype(np.float32)
+    movs = oracle.preprocess(np.stack([m, m2]), f.astype(np.float64)).astype(np.float32)
+    _say("1100x40x2 a_smooth 0.7: GPU")
+    s = DisplacementSolver(1100, 40, 2, max_batch=2, a_smooth=0.7)
+    s.set_reference(ref, [0.5, 0.5], None)
+    out = s.run(torch.from_numpy(movs).cuda(), out_dtype=torch.float64).cpu().numpy()
+    s.close()
+    for t in range(2):
+        _say(f"1100x40x2: oracle frame {t}")
+        b = oracle.get_displacement(ref, movs[t], weight=[0.5, 0.5], a_smooth=0.7)
+        d = np.abs(out[t] - b)
+        assert d.max() == 0.0, f"frame {t}: max |dflow| {d.max():.3e}"
