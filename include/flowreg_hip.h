@@ -341,6 +341,16 @@ int frh_solver_profile(frh_solver *s, int enable);
 int frh_solver_set_graph(frh_solver *s, int enable);
 int frh_solver_timings(frh_solver *s, double *ms, int *counts, int n);
 
+/* enable != 0 (the default): when a pass's sweeps x frames (x 512-row
+ * bands on levels of more than 512 rows) fit one work-group per compute
+ * unit, the pass runs as a chain of work-groups, one per sweep (and band)
+ * and frame, each consuming its predecessor's results as they are
+ * published; bit-identical to the one-work-group-per-frame passes, and
+ * faster when the batch leaves compute units idle.  Disable it when other
+ * work shares the device (several solvers on concurrent streams).  No
+ * counterpart in the reference. */
+int frh_solver_set_chains(frh_solver *s, int enable);
+
 /* Tuning knobs of the SOR wavefront (no counterpart in the reference, whose
  * solvers have no launch geometry): each level's sweeps run as wavefront
  * passes of at most pass_limit sweeps (between coefficient refreshes); more

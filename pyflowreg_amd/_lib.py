@@ -95,6 +95,7 @@ SIGNATURES = {
                                  ctypes.POINTER(c_int), ctypes.POINTER(c_void_p), c_int, c_void_p]),
     "frh_solver_profile": (c_int, [c_void_p, c_int]),
     "frh_solver_set_graph": (c_int, [c_void_p, c_int]),
+    "frh_solver_set_chains": (c_int, [c_void_p, c_int]),
     "frh_solver_level_info": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_solver_set_pass_limit": (c_int, [c_void_p, c_int, c_int]),
     "frh_grad_skew_plane": (c_int, [c_int, c_int, ctypes.POINTER(ctypes.c_size_t)]),

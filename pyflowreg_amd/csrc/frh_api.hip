@@ -585,6 +585,7 @@ struct frh_solver {
     int *chain_flags = nullptr;
     int *chain_err = nullptr, *chain_err_dev = nullptr;
     int chain_slots = 0;
+    bool use_chains = true;
 };
 
 static void free_solver(frh_solver *s)
@@ -919,7 +920,8 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         // chains of work-groups (sweeps x bands) when a sweep's bands of all
         // frames fit one work-group per compute unit
         const int nbands = (h + kBandRowsHost - 1) / kBandRowsHost;
-        const bool band_chain = s->chain_err && sk.hs > 1024 && T * nbands <= s->chain_slots;
+        const bool chains = s->chain_err && s->use_chains;
+        const bool band_chain = chains && sk.hs > 512 && T * nbands <= s->chain_slots;
         for (size_t k = 0; k < L.passes.size(); ++k) {
             const Pass &p = L.passes[k];
             if (band_chain && p.assemble) {
@@ -983,7 +985,7 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
                                         -1.0, st), "assemble");
             }
             Scope sc(s, TM_SWEEP, st);
-            if (s->chain_err && sk.hs <= 512 && sk.h + sk.w - 1 >= kChainMinDiagonals && p.nsweeps >= 2 &&
+            if (chains && sk.hs <= 512 && sk.h + sk.w - 1 >= kChainMinDiagonals && p.nsweeps >= 2 &&
                 T * p.nsweeps <= s->chain_slots)
                 FRH_HIP(launch_sweep_chain(p.nsweeps, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw, k == 0 ? 1 : 0,
                                            s->chain_flags, s->chain_err_dev, st), "sor sweep (chain)");
@@ -1096,6 +1098,13 @@ int frh_solver_set_graph(frh_solver *s, int enable)
 {
     FRH_REQ(s, "frh_solver_set_graph: null solver");
     s->use_graph = enable != 0;
+    return FRH_OK;
+}
+
+int frh_solver_set_chains(frh_solver *s, int enable)
+{
+    FRH_REQ(s, "frh_solver_set_chains: null solver");
+    s->use_chains = enable != 0;
     return FRH_OK;
 }
 

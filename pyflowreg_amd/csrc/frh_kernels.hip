@@ -2144,7 +2144,7 @@ hipError_t launch_sweep_band_chain(int n, int nl, const double *coef, double *du
     a.cTpad = chain_tpad(T);
     a.cpub = chain_publish_interval(kBandRows + s.w - 1);
     const size_t stages = (size_t)n * a.cnb;
-    if (n < 1 || s.hs <= 1024 || a.plane * sizeof(double) * (nl ? 9 : kCoefPlanes) >= kOutOfRange ||
+    if (n < 1 || s.hs <= 512 || a.plane * sizeof(double) * (nl ? 9 : kCoefPlanes) >= kOutOfRange ||
         stages * a.cTpad > 0x7fffffff)
         return hipErrorInvalidValue;
     hipError_t e = launch_clear_flags(flags, (int)(stages * a.cTpad), st);

@@ -273,6 +273,13 @@ class DisplacementSolver:
         a given batch size and output dtype; bit-identical to eager runs)."""
         check(lib().frh_solver_set_graph(self._h, int(bool(enable))), "frh_solver_set_graph")
 
+    def set_chains(self, enable=True):
+        """Run passes of small batches as chains of work-groups (one per sweep,
+        band and frame; the default, bit-identical); disable when other work
+        shares the device."""
+        check(lib().frh_solver_set_chains(self._h, int(bool(enable))), "frh_solver_set_chains")
+        self._chains = bool(enable)
+
     def close(self):
         if getattr(self, "_h", None) is not None and self._h.value:
             lib().frh_solver_destroy(self._h)
@@ -571,12 +578,23 @@ def run_groups(groups, n_streams=4, out_dtype=torch.float64):
     # largest groups first, dealt round-robin
     order = sorted(range(len(groups)), key=lambda g: -groups[g][0].H * groups[g][0].W * groups[g][0].C)
     res = [None] * len(groups)
-    for j, g in enumerate(order):
-        s, fx, mv = groups[g]
-        st = streams[j % len(streams)]
-        st.wait_stream(cur)
-        with torch.cuda.stream(st):
-            res[g] = s.run_pairs(fx, mv, out_dtype=out_dtype)
+    # chains assume a device of their own: off while groups share it
+    shared = len(streams) > 1
+    prev = [getattr(g[0], "_chains", True) for g in groups]
+    if shared:
+        for g in groups:
+            g[0].set_chains(False)
+    try:
+        for j, g in enumerate(order):
+            s, fx, mv = groups[g]
+            st = streams[j % len(streams)]
+            st.wait_stream(cur)
+            with torch.cuda.stream(st):
+                res[g] = s.run_pairs(fx, mv, out_dtype=out_dtype)
+    finally:
+        if shared:
+            for g, p in zip(groups, prev):
+                g[0].set_chains(p)
     for st in streams:
         cur.wait_stream(st)
     return res
