@@ -916,12 +916,14 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
         const size_t ps_stride = (size_t)(h + 2) * (w + 2);
         if (nl && !L.passes.empty() && L.passes[0].psi_one)
             FRH_HIP(launch_fill_f64(s->psis, (size_t)T * ps_stride, 1.0, st), "psi_s init");
-        // levels of more than 1024 rows: a lag block's single-sweep passes as
-        // chains of work-groups (sweeps x bands) when a sweep's bands of all
-        // frames fit one work-group per compute unit
+        // levels of more than 512 rows: a lag block's passes as chains of
+        // work-groups (sweeps x 512-row bands) when two sweeps' bands of all
+        // frames fit one work-group per compute unit (1024 x 1024 x 2 frames:
+        // 128 against 93 frames/s at 32 frames, 165 against 161 at 64; at
+        // 128 frames, one sweep per chain launch, 199 against 253)
         const int nbands = (h + kBandRowsHost - 1) / kBandRowsHost;
         const bool chains = s->chain_err && s->use_chains;
-        const bool band_chain = chains && sk.hs > 512 && T * nbands <= s->chain_slots;
+        const bool band_chain = chains && sk.hs > 512 && 2 * T * nbands <= s->chain_slots;
         for (size_t k = 0; k < L.passes.size(); ++k) {
             const Pass &p = L.passes[k];
             if (band_chain && p.assemble) {
