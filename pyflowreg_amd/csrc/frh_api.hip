@@ -1106,6 +1106,10 @@ int frh_solver_set_graph(frh_solver *s, int enable)
 int frh_solver_set_chains(frh_solver *s, int enable)
 {
     FRH_REQ(s, "frh_solver_set_chains: null solver");
+    if (s->use_chains != (enable != 0) && s->gexec) {  // the captured level loop holds the old launches
+        (void)hipGraphExecDestroy(s->gexec);
+        s->gexec = nullptr;
+    }
     s->use_chains = enable != 0;
     return FRH_OK;
 }
