@@ -372,18 +372,20 @@ int frh_solver_set_pass_limit(frh_solver *s, int level, int pass_limit);
  * points have no counterparts there; their forward operations are the
  * reference's.
  *   images (T, C, h, w) and J (T, 6, C, h, w) row-major;
- *   solver states (T, 2, plane) = (du, dv), coefficients (T, 6, plane) =
+ *   solver states (T, plane, 2) = (du, dv), coefficients (T, plane, 6) =
  *   (A12, Bu, Bv, omega/Du, omega/Dv, b) and their adjoints in the skewed
- *   layout: cell (r, c) at slot (r + c) * hs + r, hs = 64 * ceil(h / 64),
- *   plane = (h + w - 1) * hs (frh_grad_skew_plane); padding slots are never
- *   read.  At most 2 channels and 1024 rows per level. */
+ *   layout with each cell's values interleaved: cell (r, c) at slot
+ *   (r + c) * hs + r, hs = 64 * ceil(h / 64), plane = (h + w - 1) * hs
+ *   slots (frh_grad_skew_plane); padding slots are never read.  At most 2
+ *   channels and 1024 rows per level. */
 
-/* Doubles per skewed plane of an h x w level. */
+/* Slots per skewed plane of an h x w level. */
 int frh_grad_skew_plane(int h, int w, size_t *plane);
 
-/* N planes (h, w) row-major -> skewed (to_skew = 1, padding slots set to 0)
- * or skewed -> row-major (to_skew = 0). */
-int frh_grad_skew(const double *src, int N, int h, int w, double *dst, int to_skew, void *stream);
+/* N groups of K planes (N, K, h, w) row-major -> (N, plane, K) skewed with
+ * the K values of a cell interleaved (to_skew = 1, padding slots set to 0),
+ * or back (to_skew = 0). */
+int frh_grad_skew(const double *src, int N, int K, int h, int w, double *dst, int to_skew, void *stream);
 
 /* get_motion_tensor_gc of fixed (C, h, w) and moving (T, C, h, w): J = f
  * (when J != NULL); g_moving += dJ/dmoving^T gJ (when gJ != NULL). */

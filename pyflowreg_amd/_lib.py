@@ -99,7 +99,7 @@ SIGNATURES = {
     "frh_solver_level_info": (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_solver_set_pass_limit": (c_int, [c_void_p, c_int, c_int]),
     "frh_grad_skew_plane": (c_int, [c_int, c_int, ctypes.POINTER(ctypes.c_size_t)]),
-    "frh_grad_skew": (c_int, [c_void_p, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
+    "frh_grad_skew": (c_int, [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_int, c_void_p]),
     "frh_grad_tensor": (c_int, [c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double, c_double,
                                 c_void_p, c_void_p, c_void_p, c_void_p]),
     "frh_grad_coef": (c_int, [c_void_p] * 4 + [c_int] * 4 + [c_double, c_double, c_void_p, c_int]

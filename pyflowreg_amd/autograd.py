@@ -138,23 +138,24 @@ def skew_plane(h: int, w: int) -> int:
 
 
 def to_skew(x: torch.Tensor) -> torch.Tensor:
-    """(..., h, w) row-major planes -> (..., plane) skewed (padding = 0)."""
-    h, w = x.shape[-2:]
-    lead = x.shape[:-2]
+    """(..., K, h, w) row-major planes -> (..., plane, K) skewed, each cell's K
+    values interleaved in its slot (padding = 0)."""
+    K, h, w = x.shape[-3:]
+    lead = x.shape[:-3]
     src = _c(x)
-    out = torch.empty(lead + (skew_plane(h, w),), dtype=_F64, device=x.device)
+    out = torch.empty(lead + (skew_plane(h, w), K), dtype=_F64, device=x.device)
     n = int(np.prod(lead)) if lead else 1
-    check(lib().frh_grad_skew(ptr(src), n, h, w, ptr(out), 1, stream_handle()), "frh_grad_skew")
+    check(lib().frh_grad_skew(ptr(src), n, K, h, w, ptr(out), 1, stream_handle()), "frh_grad_skew")
     return out
 
 
 def from_skew(x: torch.Tensor, h: int, w: int) -> torch.Tensor:
-    """(..., plane) skewed -> (..., h, w) row-major."""
-    lead = x.shape[:-1]
+    """(..., plane, K) skewed -> (..., K, h, w) row-major."""
+    lead, K = x.shape[:-2], x.shape[-1]
     src = _c(x)
-    out = torch.empty(lead + (h, w), dtype=_F64, device=x.device)
+    out = torch.empty(lead + (K, h, w), dtype=_F64, device=x.device)
     n = int(np.prod(lead)) if lead else 1
-    check(lib().frh_grad_skew(ptr(src), n, h, w, ptr(out), 0, stream_handle()), "frh_grad_skew")
+    check(lib().frh_grad_skew(ptr(src), n, K, h, w, ptr(out), 0, stream_handle()), "frh_grad_skew")
     return out
 
 
@@ -189,10 +190,10 @@ class LevelSolve(torch.autograd.Function):
         check(L.frh_grad_tensor(ptr(f1), ptr(f2w), T, C, h, w, hy, hx, ptr(J), None, None, st),
               "frh_grad_tensor")
         blocks = lag_blocks(iterations, update_lag)
-        ckpt = torch.empty((len(blocks), T, 2, P), dtype=_F64, device=dev)
-        x = torch.zeros((T, 2, P), dtype=_F64, device=dev)
+        ckpt = torch.empty((len(blocks), T, P, 2), dtype=_F64, device=dev)
+        x = torch.zeros((T, P, 2), dtype=_F64, device=dev)
         x_next = torch.empty_like(x)
-        coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
+        coef = torch.empty((T, P, 6), dtype=_F64, device=dev)
         scr = _chain_scratch(L, max(e - s for s, e, _ in blocks), T, dev)
         for b, (s, e, psi_one) in enumerate(blocks):
             ckpt[b].copy_(x)
@@ -235,7 +236,7 @@ class LevelSolve(torch.autograd.Function):
         g_J = torch.zeros_like(J)
         g_par = torch.zeros((T, 2 + C, P), dtype=_F64, device=dev)
         g_axy = torch.zeros((T, 2, P), dtype=_F64, device=dev)
-        g_coef = torch.empty((T, 6, P), dtype=_F64, device=dev)
+        g_coef = torch.empty((T, P, 6), dtype=_F64, device=dev)
         blocks = lag_blocks(iterations, update_lag)
         nmax = max(e - s for s, e, _ in blocks)
         # Lag blocks run backwards.  Recomputing a block's sweeps from its
@@ -243,13 +244,13 @@ class LevelSolve(torch.autograd.Function):
         # recompute of the next block runs on a side stream while the main
         # stream runs the current block's adjoint (each uses one work-group
         # per frame); two state / coefficient buffers alternate.
-        bufs = [torch.empty((nmax, T, 2, P), dtype=_F64, device=dev) for _ in range(2)]
+        bufs = [torch.empty((nmax, T, P, 2), dtype=_F64, device=dev) for _ in range(2)]
         chain = torch.zeros(L.frh_grad_sweeps_adjoint_scratch_bytes(nmax, T, h, w), dtype=torch.uint8, device=dev)
         # the recompute (side stream) runs beside the adjoint chain: chained
         # only when both fit (32 frames of 512^2: 93.7 frames/s with the
         # recompute pipelined against 91.0 chained)
         scr = _chain_scratch(L, nmax, T, dev, beside=2)
-        coefs = [torch.empty((T, 6, P), dtype=_F64, device=dev) for _ in range(2)]
+        coefs = [torch.empty((T, P, 6), dtype=_F64, device=dev) for _ in range(2)]
         main = torch.cuda.current_stream(dev)
         side = torch.cuda.Stream(device=dev)
         side.wait_stream(main)

@@ -185,7 +185,7 @@ extern "C" {
 
 const char *frh_last_error(void) { return g_err; }
 
-int frh_abi_version(void) { return 9; }
+int frh_abi_version(void) { return 10; }
 
 int frh_fill_resize_tables(int in_len, int out_len, int R, const float *g, int32_t *idx, float *wt)
 {

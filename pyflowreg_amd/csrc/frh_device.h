@@ -67,6 +67,13 @@ __device__ inline void buf_store_f64(double x, __amdgpu_buffer_rsrc_t r, unsigne
     __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u2, x), r, (int)voff, (int)soff, AUX);
 }
 
+// two consecutive float64 values in one 16-byte store
+__device__ inline void buf_store_f64x2(F64x2 x, __amdgpu_buffer_rsrc_t r, unsigned voff, unsigned soff)
+{
+    using u4 = __attribute__((ext_vector_type(4))) unsigned;
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u4, x), r, (int)voff, (int)soff, 0);
+}
+
 // byte offset of cell (r, d - r) of the skewed plane, or kOutOfRange when
 // the lane has no cell on diagonal d
 __device__ inline unsigned slot_offset(bool row_ok, int d, int r, int w, int hs)

@@ -35,7 +35,7 @@ def test_host_entry_points_without_gpu():
     import numpy as np
     from pyflowreg_amd import _lib
     L = _lib.lib()
-    assert L.frh_abi_version() == 9
+    assert L.frh_abi_version() == 10
     assert L.frh_warping_depth(0.8, 50, 512, 512) == 18
     g = np.array([1.0], np.float32)
     idx = np.zeros((5, 4), np.int32)

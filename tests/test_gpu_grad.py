@@ -68,17 +68,19 @@ def _skew_index(h, w):
 
 
 def _sk(x):
-    """(..., h, w) -> (..., plane) with zero padding slots."""
-    h, w = x.shape[-2:]
+    """(..., K, h, w) -> (..., plane, K): each cell's K values interleaved in
+    its slot, zero padding slots."""
+    K, h, w = x.shape[-3:]
     idx, P = _skew_index(h, w)
-    out = torch.zeros(x.shape[:-2] + (P,), dtype=x.dtype, device=x.device)
-    out[..., idx.to(x.device)] = x.reshape(x.shape[:-2] + (h * w,))
+    out = torch.zeros(x.shape[:-3] + (P, K), dtype=x.dtype, device=x.device)
+    out[..., idx.to(x.device), :] = x.reshape(x.shape[:-3] + (K, h * w)).transpose(-1, -2)
     return out
 
 
 def _unsk(x, h, w):
+    """(..., plane, K) -> (..., K, h, w)."""
     idx, _ = _skew_index(h, w)
-    return x[..., idx.to(x.device)].reshape(x.shape[:-1] + (h, w))
+    return x[..., idx.to(x.device), :].transpose(-1, -2).reshape(x.shape[:-2] + (x.shape[-1], h, w))
 
 
 # ------------------------------------------------------------------ stages
@@ -124,7 +126,7 @@ def test_coefficients_forward_and_backward(gpu, psi_one):
 
     xd, u0d, Jd, Wd, ad, gKd = _cuda(_sk(x.detach()), u0.detach(), J, W, a.detach(), _sk(gK))
     _, P = _skew_index(h, w)
-    coef = torch.empty((T, 6, P), dtype=F64, device="cuda")
+    coef = torch.empty((T, P, 6), dtype=F64, device="cuda")
     g_x = torch.zeros_like(xd)
     g_u0 = torch.zeros_like(u0d)
     g_J = torch.zeros_like(Jd)
@@ -370,7 +372,7 @@ def test_skew_conversion_round_trip(gpu):
     from pyflowreg_amd import autograd as AG
     x = torch.randn((3, 2, 37, 70), dtype=F64, device="cuda")
     s = AG.to_skew(x)
-    assert s.shape == (3, 2, AG.skew_plane(37, 70))
+    assert s.shape == (3, AG.skew_plane(37, 70), 2)
     assert torch.equal(s.cpu(), _sk(x.cpu()))
     assert torch.equal(AG.from_skew(s, 37, 70), x)
 
