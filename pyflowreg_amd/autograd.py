@@ -190,23 +190,22 @@ class LevelSolve(torch.autograd.Function):
         check(L.frh_grad_tensor(ptr(f1), ptr(f2w), T, C, h, w, hy, hx, ptr(J), None, None, st),
               "frh_grad_tensor")
         blocks = lag_blocks(iterations, update_lag)
-        ckpt = torch.empty((len(blocks), T, P, 2), dtype=_F64, device=dev)
-        x = torch.zeros((T, P, 2), dtype=_F64, device=dev)
-        x_next = torch.empty_like(x)
+        # ckpt[b]: the state entering lag block b (the checkpoints the
+        # backward recomputes from); each block's sweeps write the next one
+        ckpt = torch.empty((len(blocks) + 1, T, P, 2), dtype=_F64, device=dev)
+        ckpt[0].zero_()
         coef = torch.empty((T, P, 6), dtype=_F64, device=dev)
         scr = _chain_scratch(L, max(e - s for s, e, _ in blocks), T, dev)
         for b, (s, e, psi_one) in enumerate(blocks):
-            ckpt[b].copy_(x)
-            check(L.frh_grad_coef(ptr(x), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
+            check(L.frh_grad_coef(ptr(ckpt[b]), ptr(u0), ptr(J), ptr(weight), T, C, h, w, axf, ayf, ptr(a_data),
                                   int(psi_one), ptr(coef), None, None, None, None, None, st),
                   "frh_grad_coef")
             # the block's sweeps share the coefficients: one chain of
             # work-groups or pipelined launches
-            check(L.frh_grad_sweeps(ptr(coef), ptr(x), ptr(x_next), 0, e - s, 0, T, h, w, axf, ayf,
+            check(L.frh_grad_sweeps(ptr(coef), ptr(ckpt[b]), ptr(ckpt[b + 1]), 0, e - s, 0, T, h, w, axf, ayf,
                                     *_scratch_args(scr), st), "frh_grad_sweeps")
-            x, x_next = x_next, x
         _check_chain(scr, "LevelSolve forward")
-        du = from_skew(x, h, w)
+        du = from_skew(ckpt[len(blocks)], h, w)
         if median:
             dm = torch.empty_like(du)
             check(L.frh_grad_median(ptr(du), 2 * T, h, w, ptr(dm), None, None, st), "frh_grad_median")
