@@ -132,9 +132,9 @@ hipError_t launch_assemble_w32(const double *J, const float *Wsk, const double *
                                double *coef, size_t frame_stride, int T, int C, const Skew &s, double ax,
                                double ay, const double *a_data_minus1, const double *a_data, int psi_one,
                                int zero_init, double gnc_beta, hipStream_t st);
-hipError_t launch_sweep_chain(int n, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
-                              const Skew &s, double ax, double ay, int zero_init, int *flags, int *err,
-                              hipStream_t st);
+hipError_t launch_sweep_chain(int n, int per_stage, const double *coef, double *duv, double *ring,
+                              size_t frame_stride, int T, const Skew &s, double ax, double ay, int zero_init,
+                              int *flags, int *err, hipStream_t st);
 hipError_t launch_sweep_band_chain(int n, int nl, const double *coef, double *duv, double *ring,
                                    size_t frame_stride, int T, const Skew &s, double ax, double ay, int zero_init,
                                    int *flags, int *err, hipStream_t st);

@@ -1558,7 +1558,8 @@ struct SweepArgs {
     // frames, publication interval
     int *cflags, *cerr;
     int cn, cT, cTpad, cpub;
-    int cnb;  // band mode: bands per sweep (stages = cn x cnb, sweep-major)
+    int cnb;    // band mode: bands per sweep (stages = cn x cnb, sweep-major)
+    int cskip;  // the last stage stores its first sweep's results
 };
 
 // A work-group's place in a chain: its predecessor's and its own progress
@@ -1572,6 +1573,9 @@ struct ChainState {
     // the band below in the previous sweep (its top row's start values)
     int *fl_top = nullptr, *fl_bot = nullptr;
     int avail_top = 0, fpre_top = 0, nd_top = 0, avail_bot = 0, fpre_bot = 0, nd_bot = 0;
+    // a two-sweep stage that stores its first sweep's results (the last
+    // stage of a pass with an odd number of sweeps)
+    int skip = 0;
 };
 
 // C = 0: the pass reads precomputed coefficients; C >= 1: the pass forms
@@ -1849,10 +1853,17 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
     {
         // sweep P-1 leaves diag t - 2(P-1): store its results
         const unsigned q = slot_offset(row_ok, t - 2 * (P - 1) + R0, rg, w, hs);
+        double su = cu[P - 1], sv = cv[P - 1];
+        if constexpr (CH && P == 2) {
+            // a stage that skips its second sweep: sweep 0's results of the
+            // same diagonal, from step t - 2
+            su = cs.skip ? st.Ru[P2][0] : su;
+            sv = cs.skip ? st.Rv[P2][0] : sv;
+        }
         // (the pass's results are read again only by the next pass or the
         // assembly, after every frame's pass: non-temporal stores)
-        buf_store_f64<kNT>(cu[P - 1], bf.duv, q, 0);
-        buf_store_f64<kNT>(cv[P - 1], bf.duv, q, bf.plane_bytes);
+        buf_store_f64<kNT>(su, bf.duv, q, 0);
+        buf_store_f64<kNT>(sv, bf.duv, q, bf.plane_bytes);
     }
 #pragma unroll
     for (int p = 0; p < P; ++p) {
@@ -1884,14 +1895,16 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
         }
     }
     if constexpr (CH) {
-        // publish every cpub steps and after the last diagonal, once this
+        // publish the number of diagonals stored (those before t - 2(P - 1)
+        // + 1) every cpub steps and after the last diagonal, once this
         // step's stores of every wave have been performed
-        const bool pub = cs.fl_out && (--cs.pcount == 0 || t + 1 == cs.nd);
+        const int stored = t - 2 * (P - 1) + 1;
+        const bool pub = cs.fl_out && (--cs.pcount == 0 || stored == cs.nd);
         if (cs.pcount == 0) cs.pcount = a.cpub;
         if (pub) wait_stores();
         lds_barrier();
         if (pub && r == 0)
-            __hip_atomic_store(cs.fl_out, min(t + 1, cs.nd), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+            __hip_atomic_store(cs.fl_out, max(min(stored, cs.nd), 0), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
     } else {
         lds_barrier();
     }
@@ -1992,7 +2005,8 @@ template <int P, int MAXT, int C, bool BAND, bool CH = false>
 __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
                                             double (*xb)[MAXT / 64][2][P + 1][2])
 {
-    static_assert(!CH || (P == 1 && (BAND ? C <= 0 : C == 0)), "a chain stage runs one sweep");
+    static_assert(!CH || (BAND ? (P == 1 && C <= 0) : (P <= 2 && C == 0)),
+                  "a chain stage runs one sweep (two without bands)");
     ChainState cs{nullptr, nullptr, 0, 0, 0, 0, a.zero_init};
     bool record = a.ring != nullptr;  // border ring: the pass's last sweep
     if constexpr (CH) {
@@ -2007,6 +2021,7 @@ __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
         };
         cs.pcount = a.cpub;
         cs.zero_init = a.zero_init && j == 0;
+        cs.skip = a.cskip && j == a.cn - 1;
         if (j > 0) cs.fl_in = flag(j - 1, b);
         if (j < a.cn - 1 || b < nb - 1) cs.fl_out = flag(j, b);
         record = record && j == a.cn - 1;
@@ -2098,27 +2113,34 @@ hipError_t launch_sweep(int P, const double *coef, double *duv, double *ring, si
 
 // n sweeps of a pass as one chain of work-groups (levels of up to 512 rows,
 // linear smoothness); flags: n * chain_tpad(T) progress flags.
-hipError_t launch_sweep_chain(int n, const double *coef, double *duv, double *ring, size_t frame_stride, int T,
-                              const Skew &s, double ax, double ay, int zero_init, int *flags, int *err,
-                              hipStream_t st)
+hipError_t launch_sweep_chain(int n, int per_stage, const double *coef, double *duv, double *ring,
+                              size_t frame_stride, int T, const Skew &s, double ax, double ay, int zero_init,
+                              int *flags, int *err, hipStream_t st)
 {
     SweepArgs a{};
     a.coef = coef; a.duv = duv; a.ring = ring; a.fstride = frame_stride; a.plane = s.plane;
     a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
     a.r0 = 0;
     a.hlev = a.h;
+    // stages of per_stage (1 or 2) sweeps; with 2 and an odd n the last
+    // stage stores its first sweep's results
+    const int stages = per_stage == 2 ? (n + 1) / 2 : n;
     a.cflags = flags;
     a.cerr = err;
-    a.cn = n;
+    a.cn = stages;
+    a.cskip = per_stage == 2 && (n & 1);
     a.cT = T;
     a.cTpad = chain_tpad(T);
     a.cpub = chain_publish_interval(s.h + s.w - 1);
-    if (n < 1 || s.hs > 512 || a.plane * sizeof(double) * kCoefPlanes >= kOutOfRange ||
-        (size_t)n * a.cTpad > 0x7fffffff)
+    if (n < 1 || (per_stage != 1 && per_stage != 2) || s.hs > 512 ||
+        a.plane * sizeof(double) * kCoefPlanes >= kOutOfRange || (size_t)stages * a.cTpad > 0x7fffffff)
         return hipErrorInvalidValue;
-    hipError_t e = launch_clear_flags(flags, n * a.cTpad, st);
+    hipError_t e = launch_clear_flags(flags, stages * a.cTpad, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((k_sweep<1, 512, 0, false, true>), dim3(n * a.cTpad), dim3(a.hs), 0, st, a);
+    if (per_stage == 2)
+        hipLaunchKernelGGL((k_sweep<2, 512, 0, false, true>), dim3(stages * a.cTpad), dim3(a.hs), 0, st, a);
+    else
+        hipLaunchKernelGGL((k_sweep<1, 512, 0, false, true>), dim3(stages * a.cTpad), dim3(a.hs), 0, st, a);
     return hipGetLastError();
 }
 
