@@ -352,7 +352,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--batch", type=int, default=256, help="frames per launch (64, 128 or 256)")
+    ap.add_argument("--batch", type=int, default=256, help="frames per launch")
     ap.add_argument("--size", type=int, default=512)
     ap.add_argument("--channels", type=int, default=2)
     ap.add_argument("--plain-frames", type=int, default=3, help="0 skips the plain path")
