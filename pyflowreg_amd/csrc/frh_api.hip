@@ -987,13 +987,19 @@ static int solver_run(frh_solver *s, const float *fixed, const float *moving, in
                                         -1.0, st), "assemble");
             }
             Scope sc(s, TM_SWEEP, st);
-            // one sweep per chain stage while the stages fill at most half the
-            // compute units, else two (pipelined in the stage; 512 x 512 x 2:
-            // 16 frames 328 against 315 frames/s, 32 frames 519 against 560,
-            // 64 frames only with two, 847 against 839 without a chain)
-            const int per_stage = 2 * T * p.nsweeps <= s->chain_slots ? 1 : 2;
-            if (chains && sk.hs <= 512 && sk.h + sk.w - 1 >= kChainMinDiagonals && p.nsweeps >= 2 &&
-                T * ((p.nsweeps + per_stage - 1) / per_stage) <= s->chain_slots)
+            // the fewest sweeps per chain stage (1 to 3, the last stage at
+            // most one sweep short) whose stages fill at most half the
+            // compute units, else no chain (512 x 512 x 2, frames/s, pipelined
+            // pass / stages of 1 / 2 / 3 sweeps: 16 frames 264 / 328 / 315 /
+            // -, 32 frames 486 / 519 / 560 / 533, 64 frames 840 / 826 / 847 /
+            // 884, 128 frames 1292 / - / - / 1217)
+            int per_stage = 0;
+            for (int q = 1; q <= 3 && !per_stage; ++q) {
+                const int stages = (p.nsweeps + q - 1) / q;
+                if ((q == 1 || p.nsweeps % q == 0 || p.nsweeps % q == q - 1) && 2 * T * stages <= s->chain_slots)
+                    per_stage = q;
+            }
+            if (chains && per_stage && sk.hs <= 512 && sk.h + sk.w - 1 >= kChainMinDiagonals && p.nsweeps >= 2)
                 FRH_HIP(launch_sweep_chain(p.nsweeps, per_stage, coef, duv, nullptr, fs, T, sk, L.axw, L.ayw,
                                            k == 0 ? 1 : 0, s->chain_flags, s->chain_err_dev, st),
                         "sor sweep (chain)");

@@ -1854,11 +1854,11 @@ __device__ __forceinline__ void sweep_step(SweepState<P, C, TT> &st, const Sweep
         // sweep P-1 leaves diag t - 2(P-1): store its results
         const unsigned q = slot_offset(row_ok, t - 2 * (P - 1) + R0, rg, w, hs);
         double su = cu[P - 1], sv = cv[P - 1];
-        if constexpr (CH && P == 2) {
-            // a stage that skips its second sweep: sweep 0's results of the
+        if constexpr (CH && P >= 2) {
+            // a stage that skips its last sweep: sweep P - 2's results of the
             // same diagonal, from step t - 2
-            su = cs.skip ? st.Ru[P2][0] : su;
-            sv = cs.skip ? st.Rv[P2][0] : sv;
+            su = cs.skip ? st.Ru[P2][P - 2] : su;
+            sv = cs.skip ? st.Rv[P2][P - 2] : sv;
         }
         // (the pass's results are read again only by the next pass or the
         // assembly, after every frame's pass: non-temporal stores)
@@ -2005,8 +2005,8 @@ template <int P, int MAXT, int C, bool BAND, bool CH = false>
 __device__ __forceinline__ void sweep_frame(const SweepArgs &a, int f, int nthr,
                                             double (*xb)[MAXT / 64][2][P + 1][2])
 {
-    static_assert(!CH || (BAND ? (P == 1 && C <= 0) : (P <= 2 && C == 0)),
-                  "a chain stage runs one sweep (two without bands)");
+    static_assert(!CH || (BAND ? (P == 1 && C <= 0) : (P <= 3 && C == 0)),
+                  "a chain stage runs one sweep (up to three without bands)");
     ChainState cs{nullptr, nullptr, 0, 0, 0, 0, a.zero_init};
     bool record = a.ring != nullptr;  // border ring: the pass's last sweep
     if constexpr (CH) {
@@ -2122,22 +2122,27 @@ hipError_t launch_sweep_chain(int n, int per_stage, const double *coef, double *
     a.h = s.h; a.w = s.w; a.hs = s.hs; a.ax = ax; a.ay = ay; a.zero_init = zero_init;
     a.r0 = 0;
     a.hlev = a.h;
-    // stages of per_stage (1 or 2) sweeps; with 2 and an odd n the last
-    // stage stores its first sweep's results
-    const int stages = per_stage == 2 ? (n + 1) / 2 : n;
+    // stages of per_stage (1 to 3) sweeps; when n leaves the last stage one
+    // sweep short, that stage stores its second-to-last sweep's results
+    if (per_stage < 1 || per_stage > 3 || n < 1 || (per_stage > 1 && n % per_stage != 0 &&
+                                                    n % per_stage != per_stage - 1))
+        return hipErrorInvalidValue;
+    const int stages = (n + per_stage - 1) / per_stage;
     a.cflags = flags;
     a.cerr = err;
     a.cn = stages;
-    a.cskip = per_stage == 2 && (n & 1);
+    a.cskip = per_stage > 1 && n % per_stage != 0;
     a.cT = T;
     a.cTpad = chain_tpad(T);
     a.cpub = chain_publish_interval(s.h + s.w - 1);
-    if (n < 1 || (per_stage != 1 && per_stage != 2) || s.hs > 512 ||
+    if (s.hs > 512 ||
         a.plane * sizeof(double) * kCoefPlanes >= kOutOfRange || (size_t)stages * a.cTpad > 0x7fffffff)
         return hipErrorInvalidValue;
     hipError_t e = launch_clear_flags(flags, stages * a.cTpad, st);
     if (e != hipSuccess) return e;
-    if (per_stage == 2)
+    if (per_stage == 3)
+        hipLaunchKernelGGL((k_sweep<3, 512, 0, false, true>), dim3(stages * a.cTpad), dim3(a.hs), 0, st, a);
+    else if (per_stage == 2)
         hipLaunchKernelGGL((k_sweep<2, 512, 0, false, true>), dim3(stages * a.cTpad), dim3(a.hs), 0, st, a);
     else
         hipLaunchKernelGGL((k_sweep<1, 512, 0, false, true>), dim3(stages * a.cTpad), dim3(a.hs), 0, st, a);
